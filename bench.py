@@ -1,0 +1,192 @@
+"""Benchmark: MotionCor3 frame alignment + dose-weighted sum on one MI355X.
+
+Workload (one step = one movie): 5760 x 4092 px, 50 frames of float32
+Poisson counts at 1.0 e-/px/frame, 20k seeded blobs, global drift plus a
+3 px quadratic-in-(x,y) x cubic-in-t local deformation; 5x5 patches,
+-Bft 500 100, -PixSize 0.83, -kV 300, -FmDose 1.0 (sum, _DW, _DWS), all other
+flags at the reference defaults (-Iter 15 -Tol 0.1 -Group 1 4).  The movie is
+rendered on the GPU before timing and stays resident in HBM; a step is one
+mc_align_movie call (measurement, correction and the three output sums).
+
+Prints one JSON line.  The roofline figure is for the dominant kernel class,
+timed with HIP events on the library's stream during the timed steps.  The
+CPU baseline is the FP64 oracle (oracle/) on a bounded sample of the same
+workload: the first 8 frames of the movie, scaled by 50/8 to a movie.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "movies/s @5760x4092x50 frames, 5x5 patches; shift RMS err vs FP64 (px)"
+W, H, N = 5760, 4092, 50
+PEAK_HBM_GBS = 8000.0
+
+
+def algorithmic_bytes(kernel, g, nch):
+    """Compulsory HBM bytes of one launch of `kernel` in this workload:
+    each stage reads its inputs once and writes its outputs once."""
+    cx, xcx = W // 2 + 1, g["xcf"][0] // 2 + 1
+    S_f = 8.0 * cx * H                         # full half spectrum
+    R_f = 4.0 * W * H                          # full real frame
+    S_x = 8.0 * xcx * g["xcf"][1]              # xcf half spectrum
+    launches = 2 * ((N + nch - 1) // nch)      # forward + local-correction passes
+    if kernel == "fft_cols_fwd":               # T -> S (+ xcf crop in the first pass)
+        return (N * 2 * S_f + N * S_x + N * 2 * S_f) / launches
+    if kernel == "fft_rows_r2c":               # real -> T
+        return (2 * N * (R_f + S_f)) / launches
+    if kernel == "fft_cols_inv":               # S -> T
+        return N * 2 * S_f / (launches // 2)
+    if kernel == "fft_rows_c2r":               # T -> real
+        return N * (S_f + R_f) / (launches // 2)
+    if kernel == "local_warp":                 # all frames, read + write
+        return N * 2 * R_f
+    if kernel == "dw_sums":                    # N spectra in, 3 sums out
+        return N * S_f + 3 * S_f
+    raise ValueError(kernel)
+
+
+def cpu_baseline(recipe, params_kw, frames_needed=8):
+    """FP64 oracle on the first `frames_needed` frames of the same movie."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+    import torch
+
+    from motioncor3_amd import lib as L
+
+    sub = torch.empty((frames_needed, H, W), dtype=torch.float32, device="cuda")
+    ctx = L.Context(0)
+    # same blobs, drift and local field as the timed movie, first frames only
+    from motioncor3_amd.synth import MovieRecipe
+    r8 = MovieRecipe.__new__(MovieRecipe)
+    r8.__dict__.update(recipe.__dict__)
+    r8.N = frames_needed
+    r8.displacement = recipe.displacement[:frames_needed]
+    ctx.synth_render(sub, r8)
+    host = sub.cpu().numpy()
+    po = O.default_params(**params_kw)
+    t0 = time.time()
+    ores = O.align_movie(host, po)
+    dt = time.time() - t0
+    g = ctx.align_movie(sub, L.default_params(**params_kw))
+    ctx.close()
+    dg = g["global_shift"] - ores["global_shift"]
+    rms_g = float(np.sqrt(np.mean(np.sum(dg ** 2, axis=1))))
+    same = np.array_equal(g["patch_starts"], ores["patch_starts"])
+    rms_p = None
+    if same:
+        dp = g["patch_shift"] - ores["patch_shift"]
+        rms_p = float(np.sqrt(np.mean(np.sum(dp ** 2, axis=2))))
+    movie_s = dt * N / frames_needed
+    return dict(value=1.0 / movie_s, unit="movies/s", cores=O.lib().orc_num_threads(),
+                kind="port",
+                sample="FP64 oracle, full path on the first %d of %d frames (same flags), "
+                       "%.1f s, scaled x%.2f to one movie" % (frames_needed, N, dt,
+                                                               N / frames_needed)), \
+        dict(global_rms_vs_fp64_px=rms_g, patch_rms_vs_fp64_px=rms_p,
+             sample_frames=frames_needed, same_patch_starts=bool(same))
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--kernel", default="fft_cols_fwd", help="kernel class for the roofline")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py runs on one GPU only: the path is data-parallel over movies "
+              "(one movie per GPU); --gpus %d is not supported" % args.gpus, file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+
+    from motioncor3_amd import lib as L
+    from motioncor3_amd.synth import MovieRecipe
+
+    recipe = MovieRecipe(W, H, N, 20000, seed=20250101, dose=1.0, local_max=3.0)
+    ctx = L.Context(0)
+    frames = torch.empty((N, H, W), dtype=torch.float32, device="cuda")
+    ctx.synth_render(frames, recipe)
+    kw = dict(patch=(5, 5), bft=(500, 100), pix_size=0.83, kv=300, fm_dose=1.0)
+    prm = L.default_params(**kw)
+    ns, ow, oh = ctx.output_geometry(prm, W, H, N)
+    sums = [torch.empty((oh, ow), dtype=torch.float32, device="cuda") for _ in range(ns)]
+    for _ in range(args.warmup):
+        res = ctx.align_movie(frames, prm, sums=sums)
+
+    ctx.set_kernel_timing(args.kernel)
+    torch.cuda.synchronize()
+    ctx.sync()
+    t0 = time.perf_counter()
+    iters = []
+    for _ in range(args.steps):
+        res = ctx.align_movie(frames, prm, sums=sums)
+        iters.append((res["global_iters"], int(np.max(res["patch_iters"]))))
+    ctx.sync()
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    kms, klaunch = ctx.kernel_timing()
+    ctx.set_kernel_timing(None)
+
+    ms = (t1 - t0) * 1000.0 / args.steps
+    g = L.geometry(W, H, 5, 5)
+    per_launch = algorithmic_bytes(args.kernel, g, 2)
+    avg_ms = kms / max(klaunch, 1)
+    achieved = per_launch / (avg_ms * 1e-3) / 1e9 if klaunch else None
+
+    gs = res["global_shift"]
+    d = recipe.displacement
+    e = (gs - gs.mean(0)) + (d - d.mean(0))
+    truth_rms = float(np.sqrt(np.mean(np.sum(e ** 2, axis=1))))
+
+    out = {
+        "metric": METRIC,
+        "value": 1000.0 / ms,
+        "unit": "movies/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": "5760x4092 px x 50 frames, 5x5 patches, Bft 500/100, PixSize 0.83, "
+                        "kV 300, FmDose 1.0 (sum + DW + DWS), Iter 15, Tol 0.1, Group 1 4",
+            "movie": "20k Gaussian blobs, 1 e-/px/frame Poisson, drift + 3 px local field",
+            "iterations_global_patch": iters[-1],
+        },
+        "roofline": {
+            "kernel": args.kernel,
+            "bound": "hbm",
+            "achieved": achieved,
+            "peak": PEAK_HBM_GBS,
+            "unit": "GB/s",
+            "frac": (achieved / PEAK_HBM_GBS) if achieved else None,
+            "traffic": None,
+            "algorithmic_bytes_per_launch": per_launch,
+            "avg_launch_ms": avg_ms,
+            "launches": klaunch,
+        },
+        "accuracy": {"global_shift_rms_vs_truth_px": truth_rms},
+    }
+    if not args.no_cpu:
+        cb, acc = cpu_baseline(recipe, kw)
+        out["cpu_baseline"] = cb
+        out["accuracy"].update(acc)
+    ctx.close()
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,185 @@
+/*
+ * mc_abi.h -- C ABI of libmc3, the MI355X frame-alignment path of MotionCor3.
+ *
+ * The reference has no plugin or FFI surface: the path is entered in-process
+ * through Align::CAlignMain::DoIt(DU::CDataPackage*) (Align/CAlignInc.h:711-725,
+ * Align/CAlignMain.cpp:39-75), called from CProcessThread::mAlignStack
+ * (CProcessThread.cpp:178-182), with flags read from the CInput singleton
+ * (CMainInc.h:14-144) and frames pre-loaded into the CBufferPool frm buffer
+ * (CBufferPool.cpp:264-270).  mc_align_movie replaces that call; the stage
+ * entry points below replace the reference's per-stage classes and are what a
+ * caller (or a test) binds when it needs one stage on its own.
+ *
+ * Conventions
+ *   - Every function returns 0 on success and a negative MC_E* code on
+ *     failure; mc_last_error() gives the message.  Nothing aborts
+ *     (the reference prints and calls assert(0): Util/CSimpleFuncs.cpp:203-213).
+ *   - "dev" pointers are device (HBM) addresses, "host" pointers are host
+ *     memory.  All work is queued on the context's stream; functions that
+ *     return host results synchronise that stream.
+ *   - Half spectra are column-major: [kx][ky], kx < W/2+1, ky < H, complex
+ *     interleaved float32.  The reference stores them row-major [ky][kx]
+ *     (Util/CCufft2D.cpp:23-53); values are identical, only the order differs.
+ *   - One context per host thread; contexts are independent (the reference's
+ *     align classes keep file-static state and are not re-entrant,
+ *     e.g. Align/CAlignedSum.cpp:13-15).
+ */
+#ifndef MC_ABI_H
+#define MC_ABI_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MC_OK 0
+#define MC_E_ARG -1        /* invalid argument or unsupported size */
+#define MC_E_HIP -2        /* HIP runtime error */
+#define MC_E_NOMEM -3      /* device allocation failed */
+#define MC_E_STATE -4      /* call sequence error */
+
+typedef struct mc_ctx mc_ctx;
+
+/* Flag subset of CInput that reaches the path (CInput.cpp:82-128 defaults,
+ * :415-677 parsing).  mc_default_params fills the reference defaults. */
+typedef struct mc_params {
+    int    patch[2];      /* -Patch x y        (0 0: full-frame alignment only) */
+    int    iter;          /* -Iter             (15) */
+    float  tol;           /* -Tol              (0.1) */
+    float  bft[2];        /* -Bft global local (500 100) */
+    int    phase_only;    /* -PhaseOnly        (0) */
+    float  fm_dose;       /* -FmDose           (0: no dose weighting) */
+    float  pix_size;      /* -PixSize          (0) */
+    int    kv;            /* -kV               (300) */
+    int    group[2];      /* -Group            (1 4) */
+    int    fm_ref;        /* -FmRef            (-1: middle frame) */
+    float  ft_bin;        /* -FtBin            (1) */
+    float  sum_range[2];  /* -SumRange         (3 25) */
+    int    split_sum;     /* -SplitSum         (0) */
+    int    throw0;        /* -Throw first      (0; only moves frame centres) */
+    int    ref_quirks;    /* 1 (default): keep the reference's MakeRelative
+                             indexing and spline-refit behaviour */
+    const char* log_dir;  /* -LogDir, NULL/"" for no log files */
+    const char* movie_name; /* input file name used to name the logs */
+    int    verbose;       /* print the reference's progress lines */
+} mc_params;
+
+/* N frames already gain/dark/defect corrected, resident in HBM
+ * (the frm buffer of CBufferPool, filled by MrcUtil/CApplyRefs.cpp:186-212).
+ * Rows of W floats with a pitch of `pitch` floats (the reference's padded
+ * pitch (W/2+1)*2 or W), frames `frame_stride` floats apart.  Read only. */
+typedef struct mc_movie {
+    const float* frames;  /* dev */
+    int W, H, N;
+    long long pitch;
+    long long frame_stride;
+} mc_movie;
+
+/* Outputs (DataUtil/CAlnSums.cpp:18-43 order: sum, _DW, _DWS, _ODD, _EVN).
+ * sums[i] are device buffers of out_w*out_h floats (rows of out_w) supplied
+ * by the caller for i < mc_num_sums(); all other fields may be NULL. */
+typedef struct mc_result {
+    float* sums[5];            /* dev */
+    int    num_sums, out_w, out_h;
+    float* global_shift;       /* host N x 2, as logged (-Full.log / -Patch-Full.log) */
+    int*   global_iters;       /* host 1 */
+    float* global_errors;      /* host iter, xcf pixels x xcf bin (as printed) */
+    float* patch_shift;        /* host N x P x 2, [frame][patch][xy] */
+    unsigned char* patch_bad;  /* host N x P */
+    float* patch_center;       /* host P x 2, full-resolution pixels */
+    int*   patch_starts;       /* host P x 2, xcf pixels */
+    int*   patch_iters;        /* host P */
+    double seconds_align;      /* wall time of measurement */
+    double seconds_total;      /* wall time of the whole call */
+} mc_result;
+
+/* Replacements of measured quantities.  replay_* follow -InAln
+ * (Align/CFullAlign.cpp:85-89, Align/CPatchAlign.cpp:74-78,
+ * Align/CLoadAlign.cpp:108-249): measurement is skipped and the given
+ * full-resolution shifts are applied.  patch_starts places patches at the
+ * given xcf-pixel starts instead of the feature-driven centres. */
+typedef struct mc_override {
+    const float* replay_global;          /* host N x 2 */
+    const float* replay_patch;           /* host N x P x 2 */
+    const float* replay_centers;         /* host P x 2 */
+    const unsigned char* replay_bad;     /* host N x P, NULL: none bad */
+    const int* patch_starts;             /* host P x 2 */
+} mc_override;
+
+/* ---- context ---- */
+int mc_ctx_create(int device, void* stream, mc_ctx** out);
+int mc_ctx_destroy(mc_ctx* ctx);
+const char* mc_last_error(mc_ctx* ctx);
+int mc_sync(mc_ctx* ctx);
+void mc_default_params(mc_params* p);
+/* Number and size of output sums for a movie (CBufferPool.cpp:185-202,
+ * Align/CAlignBase.cpp:41-47). */
+int mc_output_geometry(const mc_params* p, int W, int H, int N, int* num_sums, int* out_w,
+                       int* out_h);
+/* xcf image size, patch image size and xcf bin (CBufferPool.cpp:220-262). */
+int mc_geometry(int W, int H, int px, int py, int* sizes4, float* xcf_bin2);
+
+/* ---- whole path: Align::CAlignMain::DoIt (Align/CAlignMain.cpp:39-75) ---- */
+int mc_align_movie(mc_ctx* ctx, const mc_params* p, const mc_movie* movie,
+                   const mc_override* ov, mc_result* res);
+
+/* ---- stage entry points ---- */
+/* Batched 2-D R2C (cufftExecR2C, Util/CCufft2D.cpp:65-87): `batch` real
+ * images (rows of W, pitch, image stride) -> half spectra [kx][ky], each
+ * scaled by `scale` (1/(W*H) is the reference's normalised forward). */
+int mc_rfft2(mc_ctx* ctx, const float* in, int W, int H, long long pitch, long long in_stride,
+             int batch, float scale, float* out_spec);
+/* Batched 2-D C2R (cufftExecC2R, Util/CCufft2D.cpp:104-119), unnormalised;
+ * clamp != 0 applies max(0, .) (Util/GPositivity2D.cu:7-15). */
+int mc_irfft2(mc_ctx* ctx, const float* in_spec, int W, int H, int batch, float* out,
+              long long pitch, long long out_stride, int clamp);
+/* GPhaseShift2D (Util/GPhaseShift2D.cu:12-79) fused with the aligned sum of
+ * CAlignedSum (Align/CAlignedSum.cpp:25-126): out = sum_i in_i *
+ * e^{i(kx sx_i' + ky sy_i')}, shifts in pixels (host N x 2, applied as given,
+ * no skip rule). */
+int mc_phase_shift_accumulate(mc_ctx* ctx, const float* in_spec, int W, int H, int N,
+                              const float* shifts, float* out_spec);
+/* GCorrelateSum2D (Align/GCorrelateSum2D.cu:13-140) for one group: filtered
+ * cross spectrum conj(G)(S-G), inverse transform restricted to the central
+ * sea_x x sea_y window (host result, row-major). */
+int mc_group_correlate_window(mc_ctx* ctx, const float* sum_spec, const float* grp_spec, int W,
+                              int H, float bfactor, int phase_only, int sea_x, int sea_y,
+                              float* window_host);
+/* CPeak2D (Align/CPeak2D.cpp:39-109) on `count` device windows. */
+int mc_peak_fit(mc_ctx* ctx, const float* windows, int nx, int ny, int count,
+                float* shifts_host);
+/* CRemoveSpikes1D (Util/CRemoveSpikes1D.cpp:31-148) in FP64 on the GPU for
+ * `count` trajectories of n points (host in/out, [traj][2][n]). */
+int mc_spline_despike_fp64(mc_ctx* ctx, double* traj_host, int n, int count, int quirks);
+/* CPatchShifts::MakeRelative + DetectBads (Align/CPatchShifts.cpp:263-355)
+ * in FP64 on the GPU.  shifts host N x P x 2 in/out, centers P x 2. */
+int mc_patch_stats_fp64(mc_ctx* ctx, double* shifts_host, const double* centers_host, int N,
+                        int P, int W, int H, int quirks, unsigned char* bad_host,
+                        int* ref_host);
+/* mGCorrect3D (Correct/GCorrectPatchShift.cu:17-66) on N real frames
+ * (rows of W floats, dense) with host tables. */
+int mc_local_warp(mc_ctx* ctx, const float* in, float* out, int W, int H, int N, int P,
+                  const float* centers_host, const float* shifts_host,
+                  const unsigned char* bad_host);
+/* GWeightFrame weight normaliser sqrt(sum_i w_i^2) (Correct/GWeightFrame.cu:
+ * 19-30, 79-133) as a [kx][ky] table on the device. */
+int mc_dose_weight_sum(mc_ctx* ctx, int W, int H, int N, float pix, int kv, float fm_dose,
+                       float* out_table);
+
+/* ---- instrumentation (used by the benchmark) ---- */
+/* Record HIP events around every launch of the named kernel class
+ * ("fft_cols_fwd", "fft_rows_r2c", ...); NULL disables. */
+int mc_set_kernel_timing(mc_ctx* ctx, const char* kernel);
+int mc_get_kernel_timing(mc_ctx* ctx, double* total_ms, int* launches);
+
+/* ---- synthetic input (benchmark / tests, not part of the path) ---- */
+/* Render N noiseless frames of Gaussian blobs (x, y, sigma, amp per blob,
+ * per-frame blob displacement dx, dy [frame][blob]) on a unit background,
+ * scale, then Poisson-sample with a counter-based generator (seed). */
+int mc_synth_render(mc_ctx* ctx, float* frames, int W, int H, int N, long long pitch,
+                    long long frame_stride, int nblobs, const float* blob_host,
+                    const float* disp_host, float scale, unsigned long long seed, int poisson);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

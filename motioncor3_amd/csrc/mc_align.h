@@ -1,0 +1,61 @@
+// mc_align.h -- launch wrappers for the alignment kernels (mc_align.hip).
+#pragma once
+#include "mc_fft.h"
+
+// Device-resident state of a batch of CIterativeAlign jobs (one job for the
+// global alignment, one per patch).  Doubles, [job][...].
+struct IterState {
+    int J = 0, N = 0, G = 0, max_iter = 0;
+    double* tmp = nullptr;     // [J][N][2] running frame shifts
+    double* total = nullptr;   // [J][G][2] accumulated group shifts
+    double* gshift = nullptr;  // [J][G][2] this iteration's group shifts
+    double* final_ = nullptr;  // [J][N][2] after interpolation/despike/smooth
+    double* errors = nullptr;  // [J][max_iter]
+    double* scratch = nullptr; // [J][16*N + 128]
+    int* iters = nullptr;      // [J]
+    unsigned char* done = nullptr; // [J]
+    int* gstart = nullptr;     // [G]
+    int* gsize = nullptr;      // [G]
+    int* fgroup = nullptr;     // [N] group of each frame
+    double* gcent = nullptr;   // [G]
+    double* fcent = nullptr;   // [N]
+};
+
+int mc_align_init_constants();
+
+hipError_t mc_launch_phasors(const double* shifts, double mult, int skip, const unsigned char* done,
+                             int per_job, int W, int cx, int H, int batch, cfloat* A, cfloat* B,
+                             hipStream_t s);
+hipError_t mc_launch_aligned_sum(const cfloat* F, const cfloat* A, const cfloat* B,
+                                 const unsigned char* done, int J, int N, int cx, int H,
+                                 cfloat* S, hipStream_t s);
+hipError_t mc_launch_correlate(const FftPlan1& colplan, int Cc, const cfloat* F, const cfloat* S,
+                               const cfloat* A, const cfloat* B, const IterState& st, int cx,
+                               int H, float bft, int phase_only, int seay, cfloat* T1,
+                               hipStream_t s);
+hipError_t mc_launch_row_dft(const cfloat* T1, const cfloat* twW, const unsigned char* done, int J,
+                             int G, int cx, int W, int seax, int seay, float* win, hipStream_t s);
+hipError_t mc_launch_peaks(const float* win, const unsigned char* done, int J, int G, int nx,
+                           int ny, double* gshift, hipStream_t s);
+hipError_t mc_launch_update(const IterState& st, int it, double tol, hipStream_t s);
+hipError_t mc_launch_finalize(const IterState& st, int binz, int quirks, hipStream_t s);
+hipError_t mc_launch_feature_map(const float* img128, unsigned char* feat, hipStream_t s);
+hipError_t mc_launch_patch_stats(double* shifts, const double* centers, int N, int P, int W, int H,
+                                 int quirks, unsigned char* bad, int* ref, hipStream_t s);
+hipError_t mc_launch_local_warp(const float* R, float* Rp, long long pitch, long long fstride,
+                                int W, int H, int N, int P, const float* cen, const float* sh,
+                                const unsigned char* bad, hipStream_t s);
+hipError_t mc_launch_dw_sums(const cfloat* F, long long fstride, int N, int cx, int H,
+                             const cfloat* A, const cfloat* B, int dw, double pix, double kvf,
+                             double dose, int dws, double r0, double r1, int split, cfloat* sums,
+                             long long sstride, hipStream_t s);
+hipError_t mc_launch_dose_wsum(int cx, int H, int N, double pix, double kvf, double dose,
+                               float* out, hipStream_t s);
+hipError_t mc_launch_despike(double* traj, double* scratch, int n, int count, int quirks,
+                             hipStream_t s);
+hipError_t mc_launch_peaks_f(const float* win, int nx, int ny, int count, double* out,
+                             hipStream_t s);
+hipError_t mc_launch_synth(float* frames, int W, int H, int N, long long pitch, long long fstride,
+                           const float* blobs, const float* disp, int nblobs, const int* tile_off,
+                           const int* tile_idx, int tile, int tiles_x, float scale,
+                           unsigned long long seed, int poisson, hipStream_t s);

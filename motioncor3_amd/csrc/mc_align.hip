@@ -1,0 +1,1060 @@
+// mc_align.hip -- alignment, correction and summation kernels for gfx950.
+//
+// The reference runs one CIterativeAlign at a time with ~100 launches and a
+// device->host round trip per iteration (Align/CAlignStack.cpp:103-116) and
+// the peak search on the CPU (Align/CPeak2D.cpp).  Here every job (the
+// global alignment, or all patches at once) keeps its trajectory state in
+// device memory in FP64; one iteration is six launches over all jobs and
+// groups, and a converged job's work items exit on entry.
+#include <cmath>
+
+#include "mc_align.h"
+#include "mc_fft_dev.h"
+
+#define MC_PI 3.14159265358979323846
+
+// ---------------------------------------------------------------------------
+// Phase factors of GPhaseShift2D (Util/GPhaseShift2D.cu:12-66): the factor
+// e^{i(kx*sx' + ky*sy')} separates into A[kx] * B[ky]; both are formed in
+// FP64 per transform.  skip != 0 applies the |s| < 0.1 px rule of
+// Util/GPhaseShift2D.cu:68-79 (unit factors).
+__global__ void k_phasors(const double* shifts, double mult, int skip, const unsigned char* done,
+                          int per_job, int W, int cx, int H, cfloat* A, cfloat* B) {
+    const int b = blockIdx.y;
+    if (done && done[b / per_job]) return;
+    double sx = shifts[2 * b] * mult, sy = shifts[2 * b + 1] * mult;
+    if (skip && sqrt(sx * sx + sy * sy) < 0.1) sx = sy = 0.0;
+    const double fsx = sx * (2.0 * MC_PI / W), fsy = sy * (2.0 * MC_PI / H);
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e < cx) {
+        double sn, cs;
+        sincos(e * fsx, &sn, &cs);
+        A[(long long)b * cx + e] = cmk((float)cs, (float)sn);
+    } else if (e < cx + H) {
+        const int y = e - cx;
+        const int ys = (y > H / 2) ? y - H : y;
+        double sn, cs;
+        sincos(ys * fsy, &sn, &cs);
+        B[(long long)b * H + y] = cmk((float)cs, (float)sn);
+    }
+}
+
+// CAlignedSum (Align/CAlignedSum.cpp:25-126): S = sum_i shift(F_i). Even and
+// odd frames accumulate separately and are added at the end, as the
+// reference's two streams do (:99-112).
+__global__ void k_aligned_sum(const cfloat* __restrict__ F, const cfloat* __restrict__ A,
+                              const cfloat* __restrict__ B, const unsigned char* done, int N,
+                              int cx, int H, cfloat* S) {
+    const int j = blockIdx.y;
+    if (done && done[j]) return;
+    const long long fs = (long long)cx * H;
+    const long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= fs) return;
+    const int kx = (int)(q / H);
+    const int y = (int)(q - (long long)kx * H);
+    cfloat a0 = cmk(0.f, 0.f), a1 = cmk(0.f, 0.f);
+    for (int i = 0; i < N; i++) {
+        const long long b = (long long)j * N + i;
+        const cfloat v = cmul(F[b * fs + q], cmul(A[b * cx + kx], B[b * H + y]));
+        if (i & 1) a1 = cadd(a1, v);
+        else a0 = cadd(a0, v);
+    }
+    S[(long long)j * fs + q] = cadd(a0, a1);
+}
+
+// GCorrelateSum2D for one column block of one group of one job
+// (Align/GCorrelateSum2D.cu:13-73 + the group sum of Align/CAlignStack.cpp:
+// 145-178): G = sum_{i in group} shift(F_i), X = conj(G)(S - G) * B-filter
+// (optionally / |X|), DC = 0, then the inverse transform along ky is done in
+// LDS and only the rows of the sea_y lags around zero are kept.  The
+// reference's (-1)^(kx+ky) centring and centre crop of the C2R output are
+// the same as reading lags -sea/2 .. sea/2-1 of the uncentred transform.
+struct CorrArgs {
+    FftPlan1 plan;
+    const cfloat* F;
+    const cfloat* S;
+    const cfloat* A;
+    const cfloat* B;
+    const int* gstart;
+    const int* gsize;
+    const unsigned char* done;
+    int N, G, cx, H, Cc;
+    float filt0;
+    int phase_only, seay;
+    cfloat* T1;
+};
+
+__global__ __launch_bounds__(MC_NTHR) void k_correlate(CorrArgs a) {
+    extern __shared__ cfloat lds[];
+    const int j = blockIdx.z, g = blockIdx.y;
+    if (a.done && a.done[j]) return;
+    const int c0 = blockIdx.x * a.Cc;
+    const int nc = min(a.Cc, a.cx - c0);
+    const int gs = a.gstart[g], gn = a.gsize[g];
+    const long long fs = (long long)a.cx * a.H;
+    const cfloat* S = a.S + (long long)j * fs;
+    for (int e = threadIdx.x; e < nc * a.H; e += MC_NTHR) {
+        const int c = e / a.H;
+        const int y = e - c * a.H;
+        const int kx = c0 + c;
+        const long long q = (long long)kx * a.H + y;
+        cfloat G = cmk(0.f, 0.f);
+        for (int t = 0; t < gn; t++) {
+            const long long b = (long long)j * a.N + gs + t;
+            G = cadd(G, cmul(a.F[b * fs + q], cmul(a.A[b * a.cx + kx], a.B[b * a.H + y])));
+        }
+        const cfloat sd = csub(S[q], G);
+        cfloat X = cmk(G.x * sd.x + G.y * sd.y, G.x * sd.y - G.y * sd.x);
+        const int ys = (y > a.H / 2) ? y - a.H : y;
+        float f = expf(a.filt0 * (float)(kx * kx + ys * ys));
+        if (a.phase_only) f = f / (sqrtf(X.x * X.x + X.y * X.y) + 1e-20f);
+        X = (kx == 0 && y == 0) ? cmk(0.f, 0.f) : cscale(X, f);
+        lds[e] = cconj(X);
+    }
+    __syncthreads();
+    fft_lds_fwd(lds, nc, a.plan);
+    cfloat* out = a.T1 + ((long long)j * a.G + g) * a.seay * a.cx;
+    for (int e = threadIdx.x; e < a.seay * nc; e += MC_NTHR) {
+        const int jy = e / nc;
+        const int c = e - jy * nc;
+        int lag = jy - a.seay / 2;
+        if (lag < 0) lag += a.H;
+        out[(long long)jy * a.cx + c0 + c] = cconj(lds[c * a.H + lag]);
+    }
+}
+
+// Second half of the windowed C2R: for one lag row, the Hermitian sum over
+// kx at sea_x lags, c(l) = Re T0 + (-1)^l Re T_M + 2 sum_{0<k<M} Re(T_k e^{2 pi i k l/W}).
+__global__ __launch_bounds__(256) void k_row_dft(const cfloat* T1, const cfloat* twW,
+                                                 const unsigned char* done, int G, int cx, int W,
+                                                 int seax, int seay, float* win) {
+    extern __shared__ cfloat lds[];
+    const int jy = blockIdx.x, g = blockIdx.y, j = blockIdx.z;
+    if (done && done[j]) return;
+    cfloat* row = lds;
+    cfloat* tw = lds + cx;
+    float* red = reinterpret_cast<float*>(tw + W);
+    const cfloat* src = T1 + (((long long)j * G + g) * seay + jy) * cx;
+    for (int e = threadIdx.x; e < cx; e += blockDim.x) row[e] = src[e];
+    for (int e = threadIdx.x; e < W; e += blockDim.x) tw[e] = twW[e];
+    __syncthreads();
+    const int jx = threadIdx.x & 63;
+    const int part = threadIdx.x >> 6;
+    const int M = cx - 1;
+    float acc = 0.f;
+    if (jx < seax) {
+        const int lx = jx - seax / 2;
+        int lxm = lx % W;
+        if (lxm < 0) lxm += W;
+        const int k0 = 1 + part;
+        int idx = (int)(((long long)k0 * lxm) % W);
+        const int step = (int)((4LL * lxm) % W);
+        for (int k = k0; k < M; k += 4) {
+            const cfloat t = row[k], w = tw[idx];
+            acc = fmaf(t.x, w.x, fmaf(-t.y, w.y, acc));
+            idx += step;
+            if (idx >= W) idx -= W;
+        }
+        acc *= 2.f;
+        if (part == 0) acc += row[0].x + ((lx & 1) ? -row[M].x : row[M].x);
+    }
+    red[threadIdx.x] = acc;
+    __syncthreads();
+    if (threadIdx.x < 64 && jx < seax) {
+        const float v = red[threadIdx.x] + red[threadIdx.x + 64] + red[threadIdx.x + 128] +
+                        red[threadIdx.x + 192];
+        win[(((long long)j * G + g) * seay + jy) * seax + jx] = v;
+    }
+}
+
+// CPeak2D (Align/CPeak2D.cpp:39-109): first maximum wins, integer peak moved
+// off the border, separable 3-point parabola in FP64, |delta| > 1 rejected.
+__device__ void peak_fit(const float* img, int nx, int ny, double* out) {
+    __shared__ float sv[256];
+    __shared__ int si[256];
+    float best = -INFINITY;
+    int bi = 0x7fffffff;
+    for (int i = threadIdx.x; i < nx * ny; i += blockDim.x) {
+        const float v = img[i];
+        if (v > best) {
+            best = v;
+            bi = i;
+        }
+    }
+    sv[threadIdx.x] = best;
+    si[threadIdx.x] = bi;
+    __syncthreads();
+    for (int o = blockDim.x / 2; o > 0; o >>= 1) {
+        if (threadIdx.x < o) {
+            const float v2 = sv[threadIdx.x + o];
+            const int i2 = si[threadIdx.x + o];
+            if (v2 > sv[threadIdx.x] || (v2 == sv[threadIdx.x] && i2 < si[threadIdx.x])) {
+                sv[threadIdx.x] = v2;
+                si[threadIdx.x] = i2;
+            }
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        int ip = si[0];
+        if (ip == 0x7fffffff) ip = 0;
+        int px = ip % nx, py = ip / nx;
+        if (px == 0) px = 1;
+        if (py == 0) py = 1;
+        if (px == nx - 1) px -= 1;
+        if (py == ny - 1) py -= 1;
+        const int ic = py * nx + px;
+        const double c0 = img[ic];
+        const double a = ((double)img[ic + 1] + img[ic - 1]) * 0.5 - c0;
+        const double b = ((double)img[ic + 1] - img[ic - 1]) * 0.5;
+        const double c = ((double)img[ic + nx] + img[ic - nx]) * 0.5 - c0;
+        const double d = ((double)img[ic + nx] - img[ic - nx]) * 0.5;
+        double dx = -b / (2 * a + 1e-30);
+        double dy = -d / (2 * c + 1e-30);
+        if (fabs(dx) > 1) dx = 0;
+        if (fabs(dy) > 1) dy = 0;
+        out[0] = (px + dx) - nx / 2;
+        out[1] = (py + dy) - ny / 2;
+    }
+    __syncthreads();
+}
+
+__global__ __launch_bounds__(256) void k_peaks(const float* win, const unsigned char* done, int G,
+                                               int nx, int ny, double* gshift) {
+    const int g = blockIdx.x, j = blockIdx.y;
+    if (done && done[j]) return;
+    peak_fit(win + ((long long)j * G + g) * nx * ny, nx, ny, gshift + ((long long)j * G + g) * 2);
+}
+
+__global__ __launch_bounds__(256) void k_peaks_f(const float* win, int nx, int ny, double* out) {
+    peak_fit(win + (long long)blockIdx.x * nx * ny, nx, ny, out + 2 * blockIdx.x);
+}
+
+// Iteration bookkeeping of CIterativeAlign::mAlignStack
+// (Align/CIterativeAlign.cpp:120-145) and CAlignStack::mUpdateError
+// (Align/CAlignStack.cpp:207-215): err = max |group shift|, total += group
+// shifts, running frame shifts += nearest-group interpolation
+// (Align/CInterpolateShift.cpp:60-71), stop when err < tol after the first
+// iteration.  Accumulation in FP64.
+struct UpdArgs {
+    double* tmp;
+    double* total;
+    const double* gshift;
+    double* errors;
+    int* iters;
+    unsigned char* done;
+    const int* fgroup;
+    int N, G, max_iter, it;
+    double tol;
+};
+
+__global__ void k_update(UpdArgs a) {
+    const int j = blockIdx.x;
+    if (a.done[j] || threadIdx.x != 0) return;
+    const double* gs = a.gshift + (long long)j * a.G * 2;
+    double err = 0.0;
+    for (int g = 0; g < a.G; g++) {
+        const double e = sqrt(gs[2 * g] * gs[2 * g] + gs[2 * g + 1] * gs[2 * g + 1]);
+        if (err < e) err = e;
+        a.total[((long long)j * a.G + g) * 2] += gs[2 * g];
+        a.total[((long long)j * a.G + g) * 2 + 1] += gs[2 * g + 1];
+    }
+    for (int i = 0; i < a.N; i++) {
+        const int g = a.fgroup[i];
+        a.tmp[((long long)j * a.N + i) * 2] += gs[2 * g];
+        a.tmp[((long long)j * a.N + i) * 2 + 1] += gs[2 * g + 1];
+    }
+    a.errors[(long long)j * a.max_iter + a.it] = err;
+    a.iters[j] = a.it + 1;
+    if (err < a.tol && a.it > 0) a.done[j] = 1;
+}
+
+// ---------------------------------------------------------------------------
+// FP64 trajectory numerics: Util_LinEqs (LibSrc/Util/Util_LinEqs.cpp:19-148),
+// CSplineFit1D (Util/CSplineFit1D.cpp:31-115), CRemoveSpikes1D
+// (Util/CRemoveSpikes1D.cpp:31-148), CStackShift::Smooth
+// (Align/CStackShift.cpp:180-198) and the linear group->frame interpolation
+// (Align/CInterpolateShift.cpp:75-142).
+__device__ bool dev_lin_eqs(double* a, double* v, int n) {
+    int ci[16], piv[16];
+    for (int i = 0; i < n; i++) piv[i] = 0;
+    for (int ip = 0; ip < n; ip++) {
+        int ix = 0, iy = 0;
+        bool found = false;
+        double fmax = 0.0;
+        for (int y = 0; y < n; y++) {
+            if (piv[y] == 1) continue;
+            for (int x = 0; x < n; x++) {
+                if (piv[x] == 1) continue;
+                const double c = fabs(a[y * n + x]);
+                if (c <= fmax) continue;
+                fmax = c;
+                ix = x;
+                iy = y;
+                found = true;
+            }
+        }
+        if (!found) return false;
+        piv[ix] += 1;
+        ci[ip] = ix;
+        if (a[iy * n + ix] == 0.0) return false;
+        if (ix != iy) {
+            for (int x = 0; x < n; x++) {
+                const double t = a[ix * n + x];
+                a[ix * n + x] = a[iy * n + x];
+                a[iy * n + x] = t;
+            }
+            const double t = v[ix];
+            v[ix] = v[iy];
+            v[iy] = t;
+        }
+        const int col = ci[ip];
+        const double pinv = 1.0 / (a[col * n + col] + 1e-30);
+        for (int x = 0; x < n; x++) a[col * n + x] *= pinv;
+        v[col] *= pinv;
+        for (int y = 0; y < n; y++) {
+            if (y == col) continue;
+            const double dum = a[y * n + col];
+            for (int x = 0; x < n; x++) a[y * n + x] -= a[col * n + x] * dum;
+            v[y] -= v[col] * dum;
+        }
+    }
+    return true;
+}
+
+struct DevSpline {
+    int nk;
+    const double* knots;
+    __device__ void terms(double x, double* t) const {
+        t[0] = 1.0;
+        t[1] = x;
+        t[2] = x * x;
+        for (int i = 0; i < nk; i++) t[3 + i] = 0.0;
+        for (int i = 0; i < nk; i++) {
+            if (x < knots[i]) break;
+            const double v = x - knots[i];
+            t[3 + i] = v * v;
+        }
+    }
+    __device__ void cur(double x, double* c) const {
+        for (int i = 0; i < nk + 3; i++) c[i] = 0.0;
+        c[1] = 1.0;
+        c[2] = 2.0 * x;
+        for (int i = 0; i < nk; i++) {
+            const double v = x - knots[i];
+            if (v > 0) break;
+            c[3 + i] = 2.0 * v;
+        }
+    }
+    __device__ void fit(const double* x, const double* y, int n, double* out) const {
+        const int d = nk + 3;
+        double m[81], cf[9], t[9], c[9];
+        for (int i = 0; i < d * d; i++) m[i] = 0.0;
+        for (int i = 0; i < d; i++) cf[i] = 0.0;
+        for (int i = 0; i < n; i++) {
+            terms(x[i], t);
+            cur(x[i], c);
+            for (int r = 0; r < d; r++) {
+                for (int q = 0; q < d; q++) m[r * d + q] += t[r] * t[q] + 1e-20 * c[r] * c[q];
+                cf[r] += t[r] * y[i];
+            }
+        }
+        if (!dev_lin_eqs(m, cf, d)) return;
+        for (int i = 0; i < n; i++) {
+            terms(x[i], t);
+            double f = 0.0;
+            for (int r = 0; r < d; r++) f += cf[r] * t[r];
+            out[i] = f;
+        }
+    }
+};
+
+__device__ void dev_remove_single(double* raw, double* buf, int n) {
+    for (int i = 0; i < n; i++) buf[i] = raw[i];
+    for (int i = 1; i < n - 1; i++) {
+        if (fabs(buf[i] - buf[i - 1]) < 5.0) continue;
+        if (fabs(buf[i] - buf[i + 1]) < 5.0) continue;
+        raw[i] = (buf[i - 1] + buf[i + 1]) / 2.0;
+    }
+    const int e = n - 1;
+    if (fabs(buf[0] - buf[1]) > 10.0 && fabs(buf[0]) > fabs(buf[1])) raw[0] = 2.0 * buf[1] - buf[2];
+    if (fabs(buf[e] - buf[e - 1]) > 10.0 && fabs(buf[e]) > fabs(buf[e - 1]))
+        raw[e] = 2.0 * buf[e - 1] - buf[e - 2];
+}
+
+__device__ bool dev_find_bad(double* raw, const double* fit, int n, double tol) {
+    bool has = false;
+    for (int i = 0; i < n; i++) {
+        if (fabs(fit[i] - raw[i]) > tol && fabs(raw[i]) > fabs(fit[i])) {
+            raw[i] = fit[i];
+            has = true;
+        }
+    }
+    return has;
+}
+
+// scratch: 5*n doubles
+__device__ void dev_remove_spikes(double* sx, double* sy, int n, int quirks, double* scratch) {
+    double* t = scratch;
+    double* fx = scratch + n;
+    double* fy = scratch + 2 * n;
+    double* buf = scratch + 3 * n;
+    double* buf2 = scratch + 4 * n;
+    for (int r = 0; r < 2; r++) {
+        dev_remove_single(sx, buf, n);
+        dev_remove_single(sy, buf, n);
+    }
+    const float ffact = 1.0f / n;
+    for (int i = 0; i < n; i++) t[i] = sqrt((double)(i * ffact));
+    int nk = (n - 30) / 15;
+    if (nk < 0) nk = 0;
+    else if (nk > 6) nk = 6;
+    if (nk == 0 && n >= 30) nk = 1;
+    double knots[6];
+    for (int i = 0; i < nk; i++) knots[i] = t[(i + 1) * 15];
+    DevSpline sp;
+    sp.nk = nk;
+    sp.knots = knots;
+    for (int i = 0; i < n; i++) { fx[i] = sx[i]; fy[i] = sy[i]; }
+    sp.fit(t, sx, n, fx);
+    sp.fit(t, sy, n, fy);
+    for (int i = 0; i < n; i++) buf[i] = sx[i];
+    if (dev_find_bad(buf, fx, n, 3.0)) sp.fit(t, buf, n, fx);
+    for (int i = 0; i < n; i++) buf2[i] = sy[i];
+    if (dev_find_bad(buf2, fy, n, 3.0)) sp.fit(t, quirks ? sy : buf2, n, fy);
+    for (int i = 0; i < n; i++) {
+        double tol = 5.0;
+        if (i > 20) tol = 4.0;
+        if (i < 50) tol = 3.0;
+        if (fabs(fx[i] - sx[i]) > tol) sx[i] = fx[i];
+        if (fabs(fy[i] - sy[i]) > tol) sy[i] = fy[i];
+    }
+}
+
+__device__ void dev_smooth(double* sx, double* sy, int n, double w, double* buf) {
+    if (w >= 1) return;
+    for (int i = 0; i < n; i++) { buf[i] = sx[i]; buf[n + i] = sy[i]; }
+    const double w2 = 0.5 * (1.0 - w);
+    for (int i = 1; i < n - 1; i++) {
+        sx[i] = w * buf[i] + w2 * (buf[i - 1] + buf[i + 1]);
+        sy[i] = w * buf[n + i] + w2 * (buf[n + i - 1] + buf[n + i + 1]);
+    }
+}
+
+struct FinArgs {
+    const double* total;
+    double* final_;
+    double* scratch;
+    const unsigned char* dummy;
+    const double* gcent;
+    const double* fcent;
+    int N, G, binz, quirks;
+    long long sstride;
+};
+
+// End of CIterativeAlign (Align/CIterativeAlign.cpp:63-85,146-149): linear
+// interpolation of the accumulated group shifts, RemoveSpikes, Smooth(0.5),
+// then Smooth(0.3 + 0.1 * binZ, capped at 0.9).
+__global__ void k_finalize(FinArgs a) {
+    const int j = blockIdx.x;
+    if (threadIdx.x != 0) return;
+    const int N = a.N, G = a.G;
+    double* sc = a.scratch + (long long)j * a.sstride;
+    double* sx = sc;
+    double* sy = sc + N;
+    double* work = sc + 2 * N;
+    const double* tot = a.total + (long long)j * G * 2;
+    if (N == G) {
+        for (int i = 0; i < N; i++) { sx[i] = tot[2 * i]; sy[i] = tot[2 * i + 1]; }
+    } else {
+        for (int i = 0; i < N; i++) {
+            const double fc = a.fcent[i];
+            int k;
+            if (fc <= a.gcent[0]) k = 1;
+            else {
+                k = G - 1;
+                for (int q = 1; q < G; q++)
+                    if (fc < a.gcent[q]) { k = q; break; }
+            }
+            const int f = k - 1;
+            const double dg = a.gcent[k] - a.gcent[f];
+            const double df = fc - a.gcent[f];
+            sx[i] = tot[2 * f] + (tot[2 * k] - tot[2 * f]) / dg * df;
+            sy[i] = tot[2 * f + 1] + (tot[2 * k + 1] - tot[2 * f + 1]) / dg * df;
+        }
+    }
+    dev_remove_spikes(sx, sy, N, a.quirks, work);
+    dev_smooth(sx, sy, N, 0.5, work);
+    double w = 0.3 + 0.1 * a.binz;
+    if (w > 0.9) w = 0.9;
+    dev_smooth(sx, sy, N, w, work);
+    double* out = a.final_ + (long long)j * N * 2;
+    for (int i = 0; i < N; i++) { out[2 * i] = sx[i]; out[2 * i + 1] = sy[i]; }
+}
+
+__global__ void k_despike(double* traj, double* scratch, int n, int quirks) {
+    if (threadIdx.x != 0) return;
+    double* t = traj + (long long)blockIdx.x * 2 * n;
+    dev_remove_spikes(t, t + n, n, quirks, scratch + (long long)blockIdx.x * 5 * n);
+}
+
+// ---------------------------------------------------------------------------
+// Feature map of CDetectFeatures::DoIt (Align/CDetectFeatures.cpp:58-186) on
+// the 128x128 real map: +(1 - min), divide by the local 5x5 variance
+// (Align/GNormByStd2D.cu:7-42, evaluated out of place), moments, clipped
+// statistics, feature = value in [mean - 4 sd, mean - 0.08 sd].
+__global__ __launch_bounds__(1024) void k_feature_map(const float* img, unsigned char* feat) {
+    const int B = 128, NP = B * B;
+    extern __shared__ float fm[];
+    float* a = fm;
+    float* b = fm + NP;
+    __shared__ double rd[3][1024];
+    __shared__ float rmin[1024];
+    float mn = INFINITY;
+    for (int i = threadIdx.x; i < NP; i += 1024) {
+        a[i] = img[i];
+        mn = fminf(mn, a[i]);
+    }
+    rmin[threadIdx.x] = mn;
+    __syncthreads();
+    for (int o = 512; o > 0; o >>= 1) {
+        if (threadIdx.x < o) rmin[threadIdx.x] = fminf(rmin[threadIdx.x], rmin[threadIdx.x + o]);
+        __syncthreads();
+    }
+    const float add = 1.0f - rmin[0];
+    for (int i = threadIdx.x; i < NP; i += 1024) a[i] += add;
+    __syncthreads();
+    double s1 = 0, s2 = 0;
+    for (int i = threadIdx.x; i < NP; i += 1024) {
+        const int x = i % B, y = i / B;
+        float m = 0.f, v = 0.f;
+        int c = 0;
+        for (int jj = 0; jj < 5; jj++) {
+            const int yy = y - 2 + jj;
+            if (yy < 0 || yy >= B) continue;
+            for (int ii = 0; ii < 5; ii++) {
+                const int xx = x - 2 + ii;
+                if (xx < 0 || xx >= B) continue;
+                const float t = a[yy * B + xx];
+                m += t;
+                v += t * t;
+                c++;
+            }
+        }
+        m /= c;
+        v = v / c - m * m;
+        const float o = (v < 0) ? 0.f : a[i] / (v + 1e-20f);
+        b[i] = o;
+        s1 += o;
+        s2 += (double)o * o;
+    }
+    rd[0][threadIdx.x] = s1;
+    rd[1][threadIdx.x] = s2;
+    __syncthreads();
+    for (int o = 512; o > 0; o >>= 1) {
+        if (threadIdx.x < o) {
+            rd[0][threadIdx.x] += rd[0][threadIdx.x + o];
+            rd[1][threadIdx.x] += rd[1][threadIdx.x + o];
+        }
+        __syncthreads();
+    }
+    const double mean = rd[0][0] / NP;
+    double sd = rd[1][0] / NP - mean * mean;
+    sd = (sd < 0) ? 0.0 : sqrt(sd);
+    const double lo = mean - 4.0 * sd, hi = mean + 4.0 * sd;
+    __syncthreads();
+    double c1 = 0, c2 = 0, c3 = 0;
+    for (int i = threadIdx.x; i < NP; i += 1024) {
+        const double v = b[i];
+        if (v < lo || v > hi) continue;
+        c1 += v;
+        c2 += v * v;
+        c3 += 1.0;
+    }
+    rd[0][threadIdx.x] = c1;
+    rd[1][threadIdx.x] = c2;
+    rd[2][threadIdx.x] = c3;
+    __syncthreads();
+    for (int o = 512; o > 0; o >>= 1) {
+        if (threadIdx.x < o) {
+            rd[0][threadIdx.x] += rd[0][threadIdx.x + o];
+            rd[1][threadIdx.x] += rd[1][threadIdx.x + o];
+            rd[2][threadIdx.x] += rd[2][threadIdx.x + o];
+        }
+        __syncthreads();
+    }
+    double dm = rd[0][0] / (rd[2][0] + 1e-30);
+    double ds = rd[1][0] / (rd[2][0] + 1e-30) - dm * dm;
+    ds = (ds < 0) ? 0.0 : sqrt(ds);
+    const double flo = dm - 4.0 * ds, fhi = dm - 0.08 * ds;
+    for (int i = threadIdx.x; i < NP; i += 1024) feat[i] = (b[i] < flo || b[i] > fhi) ? 0 : 1;
+}
+
+// ---------------------------------------------------------------------------
+// CPatchShifts::MakeRelative + DetectBads (Align/CPatchShifts.cpp:263-355) in
+// FP64.  With quirks the y component subtracts element k+2 of the reference
+// frame's row and that row is modified while it is used (:277-285); a read
+// past the last frame sees the (still zero) flag bytes.
+__global__ __launch_bounds__(256) void k_patch_stats(double* s, const double* cen, int N, int P,
+                                                     int W, int H, int quirks,
+                                                     unsigned char* bad, int* ref_out) {
+    __shared__ int sref;
+    if (threadIdx.x == 0) {
+        double fmin = 1e20;
+        int ref = 0;
+        for (int i = 1; i < N; i++) {
+            const double* p = s + (long long)i * P * 2;
+            double mx = 0, my = 0;
+            for (int k = 0; k < P; k++) { mx += p[2 * k]; my += p[2 * k + 1]; }
+            mx /= P;
+            my /= P;
+            double sd = 0;
+            for (int k = 0; k < P; k++) {
+                const double dx = p[2 * k] - mx, dy = p[2 * k + 1] - my;
+                sd += dx * dx + dy * dy;
+            }
+            const double v = sqrt(mx * mx + my * my) + sqrt(sd / P);
+            if (v > fmin) continue;
+            fmin = v;
+            ref = i;
+        }
+        const long long total = (long long)N * P * 2;
+        const long long rb = (long long)ref * P * 2;
+        if (quirks) {
+            for (int i = 0; i < N; i++) {
+                const long long pb = (long long)i * P * 2;
+                for (int k = 0; k < P; k++) {
+                    const long long kk = 2 * k;
+                    s[pb + kk] -= (rb + kk < total) ? s[rb + kk] : 0.0;
+                    s[pb + kk + 1] -= (rb + kk + 2 < total) ? s[rb + kk + 2] : 0.0;
+                }
+            }
+        } else {
+            for (int i = 0; i < N; i++) {
+                if (i == ref) continue;
+                const long long pb = (long long)i * P * 2;
+                for (int k = 0; k < 2 * P; k++) s[pb + k] -= s[rb + k];
+            }
+            for (int k = 0; k < 2 * P; k++) s[rb + k] = 0.0;
+        }
+        sref = ref;
+        if (ref_out) *ref_out = ref;
+    }
+    __syncthreads();
+    for (int e = threadIdx.x; e < N * P; e += blockDim.x) {
+        const int f = e / P, i = e - f * P;
+        const double* p = s + (long long)f * P * 2;
+        const double ds = sqrt(p[2 * i] * p[2 * i] + p[2 * i + 1] * p[2 * i + 1]);
+        double rms = 0, sw = 0;
+        for (int k = 0; k < P; k++) {
+            if (k == i) continue;
+            const double fx = (cen[2 * k] - cen[2 * i]) / W;
+            const double fy = (cen[2 * k + 1] - cen[2 * i + 1]) / H;
+            const double w = exp(-100.0 * (fx * fx + fy * fy));
+            rms += w * (p[2 * k] * p[2 * k] + p[2 * k + 1] * p[2 * k + 1]);
+            sw += w;
+        }
+        double lr = sqrt(rms / sw);
+        if (lr < 2) lr = 2.0;
+        bad[e] = (ds > lr) ? 1 : 0;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// mGCorrect3D (Correct/GCorrectPatchShift.cu:17-66, upsample 1) with the
+// frames inside: the patch weights exp(-100 r) (r <= 0.6) of a pixel are
+// formed once and reused for every frame; per frame the bad-patch mask is
+// applied, the weighted mean shift formed in the reference's FP32 order,
+// and the pixel gathered with (int) truncation and 811*|x| mod N reflection.
+#define MC_WARP_MAXP 64
+#define MC_WARP_TX 64
+__global__ __launch_bounds__(MC_WARP_TX) void k_local_warp(const float* __restrict__ R,
+                                                           float* __restrict__ Rp, long long pitch,
+                                                           long long fstride, int W, int H, int N,
+                                                           int P, const float* cen, const float* sh,
+                                                           const unsigned char* bad) {
+    __shared__ float wts[MC_WARP_MAXP * MC_WARP_TX];
+    const int x = blockIdx.x * MC_WARP_TX + threadIdx.x;
+    const int y = blockIdx.y;
+    const bool live = x < W;
+    for (int p = 0; p < P; p++) {
+        float w = 0.f;
+        if (live) {
+            const float ax = ((float)x - cen[2 * p]) / (float)W;
+            const float ay = ((float)y - cen[2 * p + 1]) / (float)H;
+            const float r = sqrtf(ax * ax + ay * ay);
+            w = (r > 0.6f) ? 0.f : expf(-100.0f * r);
+        }
+        wts[p * MC_WARP_TX + threadIdx.x] = w;
+    }
+    if (!live) return;
+    for (int i = 0; i < N; i++) {
+        const float* s = sh + (long long)i * P * 2;
+        const unsigned char* bd = bad + (long long)i * P;
+        float fw = 0.f, fsx = 0.f, fsy = 0.f;
+        for (int p = 0; p < P; p++) {
+            if (bd[p]) continue;
+            const float w = wts[p * MC_WARP_TX + threadIdx.x];
+            fw += w;
+            fsx += s[2 * p] * w;
+            fsy += s[2 * p + 1] * w;
+        }
+        if (fw > 0) {
+            fsx = fsx / fw;
+            fsy = fsy / fw;
+        }
+        // keep the gather index finite for non-finite or absurd shifts
+        if (!(fabsf(fsx) < 1e6f)) fsx = 0.f;
+        if (!(fabsf(fsy) < 1e6f)) fsy = 0.f;
+        int xs = (int)((float)x - fsx);
+        int ys = (int)((float)y - fsy);
+        if (xs < 0 || ys < 0 || xs >= W || ys >= H) {
+            xs = xs < 0 ? -xs : xs;
+            ys = ys < 0 ? -ys : ys;
+            xs = (int)((811LL * xs) % W);
+            ys = (int)((811LL * ys) % H);
+        }
+        Rp[(long long)i * fstride + (long long)y * pitch + x] =
+            R[(long long)i * fstride + (long long)ys * pitch + xs];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// CCorrectFullShift::mGenSums (Correct/CCorrectFullShift.cpp:271-300) over
+// all frames in one pass per frequency: sum0 += F_i; with dose weighting
+// F_i *= w_i / sqrt(sum_j w_j^2) (Correct/GWeightFrame.cu:19-71, DC left
+// alone), sum1 += F_i, sum2 += F_i if the accumulated dose is in
+// -SumRange; -SplitSum adds F_i to the odd/even sum.  Optional phasors apply
+// the global shift of Correct/CCorrectFullShift.cpp:302-309 on load.  The
+// weights w_i = exp(-0.5 D_i / Dc(k)) with D_i = dose*(i+1)
+// (DataUtil/CFmIntParam.cpp:128-135) are generated as powers of
+// q = exp(-0.5 dose / Dc) in FP64.
+struct DwArgs {
+    const cfloat* F;
+    long long fstride;
+    int N, cx, H;
+    const cfloat* A;
+    const cfloat* B;
+    int dw;
+    double pix, kvf, dose;
+    int dws;
+    double r0, r1;
+    int split;
+    cfloat* sums;
+    long long sstride;
+};
+
+__device__ __forceinline__ double crit_q(int kx, int y, int cx, int H, double pix, double kvf,
+                                         double dose) {
+    double fx = kx * 0.5 / (cx - 1);
+    double fy = y / (double)H;
+    if (fy >= 0.5) fy -= 1.0;
+    const double f = sqrt(fx * fx + fy * fy) / pix;
+    const double crit = (0.24499 * pow(f, -1.6649) + 2.8141) * kvf;
+    return exp(-0.5 * dose / crit);
+}
+
+__global__ __launch_bounds__(256) void k_dw_sums(DwArgs a) {
+    const long long fs = (long long)a.cx * a.H;
+    const long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= fs) return;
+    const int kx = (int)(q / a.H);
+    const int y = (int)(q - (long long)kx * a.H);
+    const bool dc = (kx == 0 && y == 0);
+    double qd = 1.0, inv_ws = 1.0;
+    if (a.dw && !dc) {
+        qd = crit_q(kx, y, a.cx, a.H, a.pix, a.kvf, a.dose);
+        double w = 1.0, s = 0.0;
+        for (int i = 0; i < a.N; i++) {
+            w *= qd;
+            s += w * w;
+        }
+        inv_ws = 1.0 / sqrt(s);
+    }
+    cfloat s0 = cmk(0.f, 0.f), s1 = s0, s2 = s0, so = s0, se = s0;
+    double w = 1.0;
+    for (int i = 0; i < a.N; i++) {
+        cfloat v = a.F[(long long)i * a.fstride + q];
+        if (a.A) v = cmul(v, cmul(a.A[(long long)i * a.cx + kx], a.B[(long long)i * a.H + y]));
+        s0 = cadd(s0, v);
+        if (a.dw) {
+            w *= qd;
+            if (!dc) v = cscale(v, (float)(w * inv_ws));
+            s1 = cadd(s1, v);
+            if (a.dws) {
+                const double d = a.dose * (i + 1);
+                if (!(d < a.r0) && !(d > a.r1)) s2 = cadd(s2, v);
+            }
+        }
+        if (a.split) {
+            if (i & 1) so = cadd(so, v);
+            else se = cadd(se, v);
+        }
+    }
+    int ns = 0;
+    a.sums[(long long)(ns++) * a.sstride + q] = s0;
+    if (a.dw) {
+        a.sums[(long long)(ns++) * a.sstride + q] = s1;
+        if (a.dws) a.sums[(long long)(ns++) * a.sstride + q] = s2;
+    }
+    if (a.split) {
+        a.sums[(long long)ns * a.sstride + q] = se;
+        a.sums[(long long)(ns + 1) * a.sstride + q] = so;
+    }
+}
+
+__global__ void k_dose_wsum(int cx, int H, int N, double pix, double kvf, double dose, float* out) {
+    const long long fs = (long long)cx * H;
+    const long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= fs) return;
+    const int kx = (int)(q / H);
+    const int y = (int)(q - (long long)kx * H);
+    if (kx == 0 && y == 0) {
+        out[q] = 0.f;
+        return;
+    }
+    const double qd = crit_q(kx, y, cx, H, pix, kvf, dose);
+    double w = 1.0, s = 0.0;
+    for (int i = 0; i < N; i++) {
+        w *= qd;
+        s += w * w;
+    }
+    out[q] = (float)sqrt(s);
+}
+
+// ---------------------------------------------------------------------------
+// Synthetic movie renderer (benchmark input only): Gaussian blobs on a unit
+// background from per-tile blob lists, then Poisson counts by CDF inversion
+// of a counter-based uniform.
+__device__ __forceinline__ unsigned long long mix64(unsigned long long z) {
+    z += 0x9E3779B97F4A7C15ULL;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+    return z ^ (z >> 31);
+}
+
+__global__ void k_synth(float* frames, int W, int H, long long pitch, long long fstride,
+                        const float* blobs, const float* disp, int nblobs, const int* toff,
+                        const int* tidx, int tile, int tiles_x, float scale,
+                        unsigned long long seed, int poisson) {
+    const int x = blockIdx.x * blockDim.x + threadIdx.x;
+    const int y = blockIdx.y;
+    const int f = blockIdx.z;
+    if (x >= W) return;
+    const int t = (y / tile) * tiles_x + (x / tile);
+    float v = 1.0f;
+    const float* d = disp + (long long)f * nblobs * 2;
+    for (int e = toff[t]; e < toff[t + 1]; e++) {
+        const int k = tidx[e];
+        const float bx = blobs[4 * k] + d[2 * k], by = blobs[4 * k + 1] + d[2 * k + 1];
+        float dx = x - bx, dy = y - by;
+        dx -= W * rintf(dx / W);
+        dy -= H * rintf(dy / H);
+        const float s = blobs[4 * k + 2];
+        const float r2 = dx * dx + dy * dy;
+        if (r2 > 16.f * s * s) continue;
+        v += blobs[4 * k + 3] * expf(-r2 / (2.f * s * s));
+    }
+    v *= scale;
+    if (poisson) {
+        const unsigned long long key = mix64(seed ^ mix64(((unsigned long long)f << 40) ^
+                                                          ((unsigned long long)y << 20) ^
+                                                          (unsigned long long)x));
+        const double u = (key >> 11) * (1.0 / 9007199254740992.0);
+        double p = exp(-(double)v), c = p;
+        int k = 0;
+        while (u > c && k < 1000) {
+            k++;
+            p *= v / k;
+            c += p;
+        }
+        v = (float)k;
+    }
+    frames[(long long)f * fstride + (long long)y * pitch + x] = v;
+}
+
+// ===========================================================================
+// launch wrappers
+// ===========================================================================
+int mc_align_init_constants() {
+    if (mc_upload_prime_table() != 0) return -1;
+    const int bytes = (MC_LDS_ELEMS + 64) * (int)sizeof(cfloat);
+    hipFuncSetAttribute((const void*)k_correlate, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
+    hipFuncSetAttribute((const void*)k_row_dft, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
+    hipFuncSetAttribute((const void*)k_feature_map, hipFuncAttributeMaxDynamicSharedMemorySize,
+                        2 * 128 * 128 * (int)sizeof(float));
+    return 0;
+}
+
+hipError_t mc_launch_phasors(const double* shifts, double mult, int skip, const unsigned char* done,
+                             int per_job, int W, int cx, int H, int batch, cfloat* A, cfloat* B,
+                             hipStream_t s) {
+    dim3 grid(mc_div_up(cx + H, 256), batch);
+    hipLaunchKernelGGL(k_phasors, grid, dim3(256), 0, s, shifts, mult, skip, done,
+                       per_job > 0 ? per_job : 1, W, cx, H, A, B);
+    return hipGetLastError();
+}
+
+hipError_t mc_launch_aligned_sum(const cfloat* F, const cfloat* A, const cfloat* B,
+                                 const unsigned char* done, int J, int N, int cx, int H,
+                                 cfloat* S, hipStream_t s) {
+    dim3 grid(mc_div_up((long long)cx * H, 256), J);
+    hipLaunchKernelGGL(k_aligned_sum, grid, dim3(256), 0, s, F, A, B, done, N, cx, H, S);
+    return hipGetLastError();
+}
+
+hipError_t mc_launch_correlate(const FftPlan1& colplan, int Cc, const cfloat* F, const cfloat* S,
+                               const cfloat* A, const cfloat* B, const IterState& st, int cx,
+                               int H, float bft, int phase_only, int seay, cfloat* T1,
+                               hipStream_t s) {
+    CorrArgs a;
+    a.plan = colplan;
+    a.F = F;
+    a.S = S;
+    a.A = A;
+    a.B = B;
+    a.gstart = st.gstart;
+    a.gsize = st.gsize;
+    a.done = st.done;
+    a.N = st.N;
+    a.G = st.G;
+    a.cx = cx;
+    a.H = H;
+    a.Cc = Cc;
+    a.filt0 = -0.25f * bft / ((cx - 1) * H);
+    a.phase_only = phase_only;
+    a.seay = seay;
+    a.T1 = T1;
+    dim3 grid(mc_div_up(cx, Cc), st.G, st.J);
+    const size_t lds = (size_t)Cc * H * sizeof(cfloat);
+    hipLaunchKernelGGL(k_correlate, grid, dim3(MC_NTHR), lds, s, a);
+    return hipGetLastError();
+}
+
+hipError_t mc_launch_row_dft(const cfloat* T1, const cfloat* twW, const unsigned char* done, int J,
+                             int G, int cx, int W, int seax, int seay, float* win, hipStream_t s) {
+    dim3 grid(seay, G, J);
+    const size_t lds = (size_t)(cx + W) * sizeof(cfloat) + 256 * sizeof(float);
+    hipLaunchKernelGGL(k_row_dft, grid, dim3(256), lds, s, T1, twW, done, G, cx, W, seax, seay, win);
+    return hipGetLastError();
+}
+
+hipError_t mc_launch_peaks(const float* win, const unsigned char* done, int J, int G, int nx,
+                           int ny, double* gshift, hipStream_t s) {
+    hipLaunchKernelGGL(k_peaks, dim3(G, J), dim3(256), 0, s, win, done, G, nx, ny, gshift);
+    return hipGetLastError();
+}
+
+hipError_t mc_launch_peaks_f(const float* win, int nx, int ny, int count, double* out,
+                             hipStream_t s) {
+    hipLaunchKernelGGL(k_peaks_f, dim3(count), dim3(256), 0, s, win, nx, ny, out);
+    return hipGetLastError();
+}
+
+hipError_t mc_launch_update(const IterState& st, int it, double tol, hipStream_t s) {
+    UpdArgs a;
+    a.tmp = st.tmp;
+    a.total = st.total;
+    a.gshift = st.gshift;
+    a.errors = st.errors;
+    a.iters = st.iters;
+    a.done = st.done;
+    a.fgroup = st.fgroup;
+    a.N = st.N;
+    a.G = st.G;
+    a.max_iter = st.max_iter;
+    a.it = it;
+    a.tol = tol;
+    hipLaunchKernelGGL(k_update, dim3(st.J), dim3(64), 0, s, a);
+    return hipGetLastError();
+}
+
+hipError_t mc_launch_finalize(const IterState& st, int binz, int quirks, hipStream_t s) {
+    FinArgs a;
+    a.total = st.total;
+    a.final_ = st.final_;
+    a.scratch = st.scratch;
+    a.dummy = nullptr;
+    a.gcent = st.gcent;
+    a.fcent = st.fcent;
+    a.N = st.N;
+    a.G = st.G;
+    a.binz = binz;
+    a.quirks = quirks;
+    a.sstride = 16LL * st.N + 128;
+    hipLaunchKernelGGL(k_finalize, dim3(st.J), dim3(64), 0, s, a);
+    return hipGetLastError();
+}
+
+hipError_t mc_launch_despike(double* traj, double* scratch, int n, int count, int quirks,
+                             hipStream_t s) {
+    hipLaunchKernelGGL(k_despike, dim3(count), dim3(64), 0, s, traj, scratch, n, quirks);
+    return hipGetLastError();
+}
+
+hipError_t mc_launch_feature_map(const float* img128, unsigned char* feat, hipStream_t s) {
+    hipLaunchKernelGGL(k_feature_map, dim3(1), dim3(1024), 2 * 128 * 128 * sizeof(float), s,
+                       img128, feat);
+    return hipGetLastError();
+}
+
+hipError_t mc_launch_patch_stats(double* shifts, const double* centers, int N, int P, int W, int H,
+                                 int quirks, unsigned char* bad, int* ref, hipStream_t s) {
+    hipLaunchKernelGGL(k_patch_stats, dim3(1), dim3(256), 0, s, shifts, centers, N, P, W, H,
+                       quirks, bad, ref);
+    return hipGetLastError();
+}
+
+hipError_t mc_launch_local_warp(const float* R, float* Rp, long long pitch, long long fstride,
+                                int W, int H, int N, int P, const float* cen, const float* sh,
+                                const unsigned char* bad, hipStream_t s) {
+    if (P > MC_WARP_MAXP) return hipErrorInvalidValue;
+    dim3 grid(mc_div_up(W, MC_WARP_TX), H);
+    hipLaunchKernelGGL(k_local_warp, grid, dim3(MC_WARP_TX), 0, s, R, Rp, pitch, fstride, W, H, N,
+                       P, cen, sh, bad);
+    return hipGetLastError();
+}
+
+hipError_t mc_launch_dw_sums(const cfloat* F, long long fstride, int N, int cx, int H,
+                             const cfloat* A, const cfloat* B, int dw, double pix, double kvf,
+                             double dose, int dws, double r0, double r1, int split, cfloat* sums,
+                             long long sstride, hipStream_t s) {
+    DwArgs a;
+    a.F = F;
+    a.fstride = fstride;
+    a.N = N;
+    a.cx = cx;
+    a.H = H;
+    a.A = A;
+    a.B = B;
+    a.dw = dw;
+    a.pix = pix;
+    a.kvf = kvf;
+    a.dose = dose;
+    a.dws = dws;
+    a.r0 = r0;
+    a.r1 = r1;
+    a.split = split;
+    a.sums = sums;
+    a.sstride = sstride;
+    dim3 grid(mc_div_up((long long)cx * H, 256));
+    hipLaunchKernelGGL(k_dw_sums, grid, dim3(256), 0, s, a);
+    return hipGetLastError();
+}
+
+hipError_t mc_launch_dose_wsum(int cx, int H, int N, double pix, double kvf, double dose,
+                               float* out, hipStream_t s) {
+    dim3 grid(mc_div_up((long long)cx * H, 256));
+    hipLaunchKernelGGL(k_dose_wsum, grid, dim3(256), 0, s, cx, H, N, pix, kvf, dose, out);
+    return hipGetLastError();
+}
+
+hipError_t mc_launch_synth(float* frames, int W, int H, int N, long long pitch, long long fstride,
+                           const float* blobs, const float* disp, int nblobs, const int* tile_off,
+                           const int* tile_idx, int tile, int tiles_x, float scale,
+                           unsigned long long seed, int poisson, hipStream_t s) {
+    dim3 grid(mc_div_up(W, 256), H, N);
+    hipLaunchKernelGGL(k_synth, grid, dim3(256), 0, s, frames, W, H, pitch, fstride, blobs, disp,
+                       nblobs, tile_off, tile_idx, tile, tiles_x, scale, seed, poisson);
+    return hipGetLastError();
+}

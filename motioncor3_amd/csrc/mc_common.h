@@ -1,0 +1,43 @@
+// mc_common.h -- shared device helpers and plan structures for libmc3.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#define MC_NTHR 512          // threads per FFT workgroup (8 waves of 64)
+#define MC_LDS_ELEMS 16384   // complex fp32 elements of LDS per FFT workgroup (128 KiB)
+#define MC_MAX_STAGES 16
+
+struct __align__(8) cfloat {
+    float x, y;
+};
+
+__device__ __forceinline__ cfloat cmk(float x, float y) { cfloat r; r.x = x; r.y = y; return r; }
+__device__ __forceinline__ cfloat cadd(cfloat a, cfloat b) { return cmk(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ cfloat csub(cfloat a, cfloat b) { return cmk(a.x - b.x, a.y - b.y); }
+__device__ __forceinline__ cfloat cmul(cfloat a, cfloat b) {
+    return cmk(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+__device__ __forceinline__ cfloat cconj(cfloat a) { return cmk(a.x, -a.y); }
+__device__ __forceinline__ cfloat cscale(cfloat a, float s) { return cmk(a.x * s, a.y * s); }
+// multiply by -i
+__device__ __forceinline__ cfloat cmuli_neg(cfloat a) { return cmk(a.y, -a.x); }
+
+// One radix stage of a Stockham autosort FFT: radix r, sub-transform size ns
+// before the stage, twiddles at tw + tw_off laid out [k][t-1] (k < ns,
+// 1 <= t < r), roots of unity of a large prime at tw + root_off.
+struct FftStage {
+    int r;
+    int ns;
+    int tw_off;
+    int root_off;
+};
+
+// Plan of one complex FFT length, passed to kernels by value.
+struct FftPlan1 {
+    int n;
+    int nst;
+    FftStage st[MC_MAX_STAGES];
+    const cfloat* tw;  // device table (forward sign)
+};
+
+static inline int mc_div_up(long long a, long long b) { return (int)((a + b - 1) / b); }

@@ -1,0 +1,1056 @@
+// mc_host.cpp -- host side of libmc3: Align::CAlignMain for one MI355X.
+//
+// Mirrors the reference's control flow (Align/CAlignMain.cpp:39-75,
+// Align/CFullAlign.cpp:70-171, Align/CPatchAlign.cpp:25-99,
+// Correct/CCorrectFullShift.cpp, Correct/GCorrectPatchShift.cu) and its flag
+// semantics (CInput, CAlignParam, CFmIntParam, CFmGroupParam), but every
+// per-frame and per-patch loop of the reference is a batched launch here and
+// all intermediate data stays in HBM.  The host only sees the shift tables.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "../../include/mc_abi.h"
+#include "mc_align.h"
+#include "mc_fft.h"
+
+namespace {
+
+struct DevBuf {
+    void* p = nullptr;
+    size_t bytes = 0;
+};
+
+struct TimedEv {
+    hipEvent_t a, b;
+};
+
+}  // namespace
+
+struct mc_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    bool own_stream = false;
+    std::string err;
+    std::map<std::pair<int, int>, Fft2D*> plans;
+    std::map<int, cfloat*> rowtw;
+    std::map<std::string, DevBuf> bufs;
+    std::string timed;
+    std::vector<TimedEv> evs;
+    size_t nev = 0;
+};
+
+namespace {
+
+#define MC_TRY(x)                                                                  \
+    do {                                                                           \
+        hipError_t e_ = (x);                                                       \
+        if (e_ != hipSuccess) {                                                    \
+            c->err = std::string(#x) + ": " + hipGetErrorString(e_);               \
+            return MC_E_HIP;                                                       \
+        }                                                                          \
+    } while (0)
+
+#define MC_CHECK(x)              \
+    do {                         \
+        int r_ = (x);            \
+        if (r_ != MC_OK) return r_; \
+    } while (0)
+
+int fail(mc_ctx* c, int code, const std::string& msg) {
+    c->err = msg;
+    return code;
+}
+
+void* buf(mc_ctx* c, const char* name, size_t bytes) {
+    DevBuf& b = c->bufs[name];
+    if (b.bytes >= bytes && b.p) return b.p;
+    if (b.p) hipFree(b.p);
+    b.p = nullptr;
+    b.bytes = 0;
+    if (hipMalloc(&b.p, bytes < 256 ? 256 : bytes) != hipSuccess) {
+        b.p = nullptr;
+        return nullptr;
+    }
+    b.bytes = bytes;
+    return b.p;
+}
+
+#define MC_BUF(var, type, name, count)                                                    \
+    type* var = (type*)buf(c, name, sizeof(type) * (size_t)(count));                       \
+    if (!var) return fail(c, MC_E_NOMEM, std::string("hipMalloc failed for ") + name)
+
+Fft2D* plan(mc_ctx* c, int W, int H) {
+    auto key = std::make_pair(W, H);
+    auto it = c->plans.find(key);
+    if (it != c->plans.end()) return it->second;
+    Fft2D* p = new Fft2D;
+    if (mc_fft2d_create(p, W, H) != 0) {
+        delete p;
+        return nullptr;
+    }
+    c->plans[key] = p;
+    return p;
+}
+
+cfloat* row_twiddles(mc_ctx* c, int W) {
+    auto it = c->rowtw.find(W);
+    if (it != c->rowtw.end()) return it->second;
+    std::vector<cfloat> h(W);
+    for (int q = 0; q < W; q++) {
+        const double a = 2.0 * M_PI * q / W;
+        h[q].x = (float)std::cos(a);
+        h[q].y = (float)std::sin(a);
+    }
+    cfloat* d = nullptr;
+    if (hipMalloc(&d, W * sizeof(cfloat)) != hipSuccess) return nullptr;
+    hipMemcpy(d, h.data(), W * sizeof(cfloat), hipMemcpyHostToDevice);
+    c->rowtw[W] = d;
+    return d;
+}
+
+// kernel timing hooks for the benchmark
+struct KTimer {
+    mc_ctx* c;
+    bool on;
+    size_t idx;
+    KTimer(mc_ctx* ctx, const char* name) : c(ctx), on(false), idx(0) {
+        if (c->timed.empty() || c->timed != name) return;
+        if (c->nev >= c->evs.size()) {
+            TimedEv e;
+            hipEventCreate(&e.a);
+            hipEventCreate(&e.b);
+            c->evs.push_back(e);
+        }
+        idx = c->nev++;
+        hipEventRecord(c->evs[idx].a, c->stream);
+        on = true;
+    }
+    ~KTimer() {
+        if (on) hipEventRecord(c->evs[idx].b, c->stream);
+    }
+};
+
+// ---------------------------------------------------------------------------
+// geometry: CBufferPool.cpp:185-262, Align/CAlignBase.cpp:41-47
+struct Geo {
+    int W, H, N, cx;
+    int xcx, xh, xw;
+    int pw, ph, pcx;
+    float xbin[2];
+    int ocx, oh, ow;
+};
+
+Geo make_geo(int W, int H, int N, int px, int py, float ftbin) {
+    Geo g;
+    g.W = W;
+    g.H = H;
+    g.N = N;
+    g.cx = W / 2 + 1;
+    float fb = std::fmin(W / 2048.0f, H / 2048.0f);
+    if (fb < 1) fb = 1.0f;
+    const int ixx = (int)(W / fb + 0.5f), ixy = (int)(H / fb + 0.5f);
+    g.xcx = ixx / 2 + 1;
+    g.xh = ixy / 2 * 2;
+    g.xw = (g.xcx - 1) * 2;
+    g.xbin[0] = (g.cx - 1.0f) / (g.xcx - 1.0f);
+    g.xbin[1] = g.H * 1.0f / g.xh;
+    g.pw = g.ph = g.pcx = 0;
+    if (px > 1 && py > 1) {
+        const int ipx = g.xw / px, ipy = g.xh / py;
+        g.pcx = ipx / 2 + 1;
+        g.pw = (g.pcx - 1) * 2;
+        g.ph = ipy / 2 * 2;
+    }
+    g.ocx = g.cx;
+    g.oh = g.H;
+    if (ftbin != 1.0f) {
+        int nx = (int)((g.cx - 1) * 2 / ftbin), ny = (int)(g.H / ftbin);
+        nx = nx / 2 * 2;
+        ny = ny / 2 * 2;
+        g.ocx = nx / 2 + 1;
+        g.oh = ny;
+    }
+    g.ow = (g.ocx - 1) * 2;
+    return g;
+}
+
+// DataUtil/CFmGroupParam.cpp:26-78 with one raw frame per integrated frame;
+// frame centres DataUtil/CFmIntParam.cpp:280-287.
+struct Groups {
+    int G = 0, binz = 1;
+    std::vector<int> start, size, fgroup;
+    std::vector<double> gcent, fcent;
+};
+
+Groups make_groups(int binz, int N, int throw0) {
+    Groups g;
+    g.binz = binz;
+    g.fgroup.assign(N, 0);
+    for (int i = 0; i < N; i++) g.fcent.push_back((double)(i + throw0));
+    int f = 0;
+    for (int i = 0; i < N; i++) {
+        g.start.push_back(f);
+        int cnt = 0, raw = 0;
+        while (true) {
+            raw += 1;
+            cnt += 1;
+            g.fgroup[f] = g.G;
+            f += 1;
+            if (f >= N) break;
+            if (raw >= binz) break;
+        }
+        g.size.push_back(cnt);
+        g.G += 1;
+        if (f >= N) break;
+    }
+    for (int k = 0; k < g.G; k++) {
+        double s = 0;
+        for (int j = 0; j < g.size[k]; j++) s += g.fcent[g.start[k] + j];
+        g.gcent.push_back(s / g.size[k]);
+    }
+    return g;
+}
+
+// CDetectFeatures::mFindCenters/mFindCenter/mSetUsed/mCheckFeature
+// (Align/CDetectFeatures.cpp:289-396) and CPatchCenters::Calculate
+// (Align/CPatchCenters.cpp:42-105): patch starts from the feature map.
+void patch_starts(const unsigned char* feat, int px, int py, int xw, int xh, int* starts) {
+    const int B = 128;
+    std::vector<unsigned char> used(B * B, 0);
+    const float pat[2] = {B * 1.0f / px, B * 1.0f / py};
+    int sea[4];
+    sea[0] = std::max(4, (int)(pat[0] * 0.5f - 0.5f));
+    sea[1] = B - sea[0];
+    sea[2] = std::max(4, (int)(pat[1] * 0.5f - 0.5f));
+    sea[3] = B - sea[2];
+    auto range = [](int start, int size, const int* r) {
+        if (start < r[0]) return r[0];
+        if (start + size > r[1]) return r[1] - size;
+        return start;
+    };
+    auto set_used = [&](float cxv, float cyv) {
+        const int wx = (int)(pat[0] * 0.8f), wy = (int)(pat[1] * 0.8f);
+        int sx = (int)(cxv - wx / 2), sy = (int)(cyv - wy / 2);
+        sx = range(sx, wx, &sea[0]);
+        sy = range(sy, wy, &sea[2]);
+        for (int y = 0; y < wy; y++)
+            for (int x = 0; x < wx; x++) used[(sy + y) * B + sx + x] = 1;
+    };
+    const int np = px * py;
+    std::vector<float> cen(3 * np);
+    for (int i = 0; i < np; i++) {
+        const float fx = (i % px + 0.5f) * pat[0], fy = (i / px + 0.5f) * pat[1];
+        const int ix = (int)fx, iy = (int)fy;
+        bool has = !(ix < sea[0] || ix > sea[1] || iy < sea[2] || iy > sea[3]) && feat[iy * B + ix];
+        cen[3 * i] = fx;
+        cen[3 * i + 1] = fy;
+        cen[3 * i + 2] = has ? 1.0f : -1.0f;
+        if (has) set_used(fx, fy);
+    }
+    for (int i = 0; i < np; i++) {
+        float* c = &cen[3 * i];
+        if (c[2] > 0) continue;
+        const int wx = B / 3, wy = B / 3;
+        int sx = (int)(c[0] - wx * 0.5f + 0.5f), sy = (int)(c[1] - wy * 0.5f + 0.5f);
+        sx = range(sx, wx, &sea[0]);
+        sy = range(sy, wy, &sea[2]);
+        float minr = 1e20f, ac[2] = {-1.0f, 0.0f};
+        for (int y = 0; y < wy; y++)
+            for (int x = 0; x < wx; x++) {
+                const int j = (sy + y) * B + sx + x;
+                if (used[j] || !feat[j]) continue;
+                const float fx = x + sx + 0.5f, fy = y + sy + 0.5f;
+                const float dx = fx - c[0], dy = fy - c[1];
+                const float r = sqrtf(dx * dx + dy * dy);
+                if (r >= minr) continue;
+                minr = r;
+                ac[0] = fx;
+                ac[1] = fy;
+            }
+        if (ac[0] > 0) {
+            c[0] = ac[0];
+            c[1] = ac[1];
+        }
+        set_used(c[0], c[1]);
+    }
+    const int psx = xw / px / 2 * 2, psy = xh / py / 2 * 2;
+    auto calc = [](float cv, int ps, int xs) {
+        int s = (int)(cv - ps / 2);
+        if (s < 0) s = 0;
+        else if (s + ps > xs) s = xs - ps;
+        return s;
+    };
+    for (int i = 0; i < np; i++) {
+        starts[2 * i] = calc(cen[3 * i] * (xw * 1.0f / B), psx, xw);
+        starts[2 * i + 1] = calc(cen[3 * i + 1] * (xh * 1.0f / B), psy, xh);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// CIterativeAlign for J jobs at once (Align/CIterativeAlign.cpp:32-175).
+struct IterOut {
+    std::vector<double> shifts;  // [J][N][2]
+    std::vector<double> errors;  // [J][max_iter]
+    std::vector<int> iters;      // [J]
+};
+
+int run_iterative(mc_ctx* c, const char* tag, const cfloat* F, int J, int N, int W, int H,
+                  Fft2D* pl, const Groups& grp, float bft, int phase_only, int max_iter,
+                  double tol, int quirks, IterOut* out) {
+    const int cx = W / 2 + 1, G = grp.G;
+    // linear group->frame interpolation needs two groups
+    // (Align/CInterpolateShift.cpp:129-142 reads group 1)
+    if (G < 2 && G != N) return fail(c, MC_E_ARG, "-Group leaves fewer than two frame groups");
+    const int seax = std::min(64, W), seay = std::min(64, H);
+    std::string t(tag);
+    auto nm = [&](const char* s) { return (t + s); };
+    IterState st;
+    st.J = J;
+    st.N = N;
+    st.G = G;
+    st.max_iter = max_iter;
+    const size_t sstride = 16 * (size_t)N + 128;
+    st.tmp = (double*)buf(c, nm("tmp").c_str(), sizeof(double) * J * N * 2);
+    st.total = (double*)buf(c, nm("total").c_str(), sizeof(double) * J * G * 2);
+    st.gshift = (double*)buf(c, nm("gshift").c_str(), sizeof(double) * J * G * 2);
+    st.final_ = (double*)buf(c, nm("final").c_str(), sizeof(double) * J * N * 2);
+    st.errors = (double*)buf(c, nm("errors").c_str(), sizeof(double) * J * max_iter);
+    st.scratch = (double*)buf(c, nm("scratch").c_str(), sizeof(double) * J * sstride);
+    st.iters = (int*)buf(c, nm("iters").c_str(), sizeof(int) * J);
+    st.done = (unsigned char*)buf(c, nm("done").c_str(), J);
+    st.gstart = (int*)buf(c, nm("gstart").c_str(), sizeof(int) * G);
+    st.gsize = (int*)buf(c, nm("gsize").c_str(), sizeof(int) * G);
+    st.fgroup = (int*)buf(c, nm("fgroup").c_str(), sizeof(int) * N);
+    st.gcent = (double*)buf(c, nm("gcent").c_str(), sizeof(double) * G);
+    st.fcent = (double*)buf(c, nm("fcent").c_str(), sizeof(double) * N);
+    if (!st.tmp || !st.total || !st.gshift || !st.final_ || !st.errors || !st.scratch ||
+        !st.iters || !st.done || !st.gstart || !st.gsize || !st.fgroup || !st.gcent || !st.fcent)
+        return fail(c, MC_E_NOMEM, "iteration state allocation failed");
+    MC_BUF(A, cfloat, nm("phA").c_str(), (size_t)J * N * cx);
+    MC_BUF(B, cfloat, nm("phB").c_str(), (size_t)J * N * H);
+    MC_BUF(S, cfloat, nm("sum").c_str(), (size_t)J * cx * H);
+    MC_BUF(T1, cfloat, nm("t1").c_str(), (size_t)J * G * seay * cx);
+    MC_BUF(win, float, nm("win").c_str(), (size_t)J * G * seay * seax);
+    cfloat* twW = row_twiddles(c, W);
+    if (!twW) return fail(c, MC_E_NOMEM, "twiddle table");
+    hipStream_t s = c->stream;
+    MC_TRY(hipMemsetAsync(st.tmp, 0, sizeof(double) * J * N * 2, s));
+    MC_TRY(hipMemsetAsync(st.total, 0, sizeof(double) * J * G * 2, s));
+    MC_TRY(hipMemsetAsync(st.errors, 0, sizeof(double) * J * max_iter, s));
+    MC_TRY(hipMemsetAsync(st.iters, 0, sizeof(int) * J, s));
+    MC_TRY(hipMemsetAsync(st.done, 0, J, s));
+    MC_TRY(hipMemcpyAsync(st.gstart, grp.start.data(), sizeof(int) * G, hipMemcpyHostToDevice, s));
+    MC_TRY(hipMemcpyAsync(st.gsize, grp.size.data(), sizeof(int) * G, hipMemcpyHostToDevice, s));
+    MC_TRY(hipMemcpyAsync(st.fgroup, grp.fgroup.data(), sizeof(int) * N, hipMemcpyHostToDevice, s));
+    MC_TRY(hipMemcpyAsync(st.gcent, grp.gcent.data(), sizeof(double) * G, hipMemcpyHostToDevice, s));
+    MC_TRY(hipMemcpyAsync(st.fcent, grp.fcent.data(), sizeof(double) * N, hipMemcpyHostToDevice, s));
+    float b = bft;
+    for (int it = 0; it < max_iter; it++) {
+        {
+            KTimer kt(c, "phasors");
+            MC_TRY(mc_launch_phasors(st.tmp, -1.0, 0, st.done, N, W, cx, H, J * N, A, B, s));
+        }
+        {
+            KTimer kt(c, "aligned_sum");
+            MC_TRY(mc_launch_aligned_sum(F, A, B, st.done, J, N, cx, H, S, s));
+        }
+        {
+            KTimer kt(c, "correlate");
+            MC_TRY(mc_launch_correlate(pl->col, pl->C, F, S, A, B, st, cx, H, b, phase_only, seay,
+                                       T1, s));
+        }
+        {
+            KTimer kt(c, "row_dft");
+            MC_TRY(mc_launch_row_dft(T1, twW, st.done, J, G, cx, W, seax, seay, win, s));
+        }
+        MC_TRY(mc_launch_peaks(win, st.done, J, G, seax, seay, st.gshift, s));
+        MC_TRY(mc_launch_update(st, it, tol, s));
+        b -= 2;
+        if (b < 20) b = 20.0f;
+    }
+    MC_TRY(mc_launch_finalize(st, grp.binz, quirks, s));
+    out->shifts.resize((size_t)J * N * 2);
+    out->errors.resize((size_t)J * max_iter);
+    out->iters.resize(J);
+    MC_TRY(hipMemcpyAsync(out->shifts.data(), st.final_, sizeof(double) * J * N * 2,
+                          hipMemcpyDeviceToHost, s));
+    MC_TRY(hipMemcpyAsync(out->errors.data(), st.errors, sizeof(double) * J * max_iter,
+                          hipMemcpyDeviceToHost, s));
+    MC_TRY(hipMemcpyAsync(out->iters.data(), st.iters, sizeof(int) * J, hipMemcpyDeviceToHost, s));
+    MC_TRY(hipStreamSynchronize(s));
+    return MC_OK;
+}
+
+// ---------------------------------------------------------------------------
+// logs: Align/CAlignMain.cpp:77-107 (name), Align/CFullAlign.cpp:173-194,
+// Align/CPatchShifts.cpp:155-249 (formats)
+std::string log_stem(const mc_params* p) {
+    if (!p->log_dir || !p->log_dir[0] || !p->movie_name) return "";
+    const char* base = strrchr(p->movie_name, '/');
+    base = base ? base + 1 : p->movie_name;
+    std::string f = std::string(p->log_dir) + base;
+    std::string low = f;
+    for (auto& ch : low) ch = (char)tolower(ch);
+    for (const char* ext : {".mrc", ".tif", ".eer"}) {
+        const size_t pos = low.find(ext);
+        if (pos != std::string::npos) return f.substr(0, pos);
+    }
+    return "";
+}
+
+void write_logs(const mc_params* p, int W, int H, int N, int P, const std::vector<float>& gs,
+                const std::vector<float>& ps, const std::vector<unsigned char>& bad,
+                const std::vector<float>& pc) {
+    const std::string stem = log_stem(p);
+    if (stem.empty()) return;
+    if (P == 0) {
+        FILE* f = fopen((stem + "-Full.log").c_str(), "wt");
+        if (!f) return;
+        fprintf(f, "# full frame alignment\n");
+        for (int i = 0; i < N; i++) fprintf(f, "%4d  %8.2f  %8.2f\n", i + 1, gs[2 * i], gs[2 * i + 1]);
+        fclose(f);
+        return;
+    }
+    FILE* f = fopen((stem + "-Patch-Full.log").c_str(), "wt");
+    if (f) {
+        fprintf(f, "# Patch based alignment\n");
+        fprintf(f, "# Initial alignment based upon full frames\n\n");
+        for (int i = 0; i < N; i++) fprintf(f, "%4d  %8.2f  %8.2f\n", i + 1, gs[2 * i], gs[2 * i + 1]);
+        fclose(f);
+    }
+    for (int mode = 0; mode < 2; mode++) {
+        f = fopen((stem + (mode == 0 ? "-Patch-Patch.log" : "-Patch-Frame.log")).c_str(), "wt");
+        if (!f) continue;
+        fprintf(f, "# Patch based alignment\n");
+        fprintf(f, "# Number of patches: %d\n", P);
+        fprintf(f, mode == 0 ? "# Shifts are listed per patch.\n" : "# Shifts are listed per frame.\n");
+        fprintf(f, "# Movie size: %d  %d  %d\n\n", W, H, N);
+        if (mode == 0) {
+            for (int i = 0; i < P; i++) {
+                fprintf(f, "#Patch %03d raw and fit shifts\n", i + 1);
+                for (int j = 0; j < N; j++) {
+                    const int k = j * P + i;
+                    fprintf(f, "%4d %8.2f %8.2f ", j + 1, pc[2 * i], pc[2 * i + 1]);
+                    fprintf(f, "%8.2f %8.2f %4d\n", ps[2 * k], ps[2 * k + 1], (int)bad[k]);
+                }
+                fprintf(f, "\n");
+            }
+        } else {
+            for (int j = 0; j < N; j++) {
+                for (int i = 0; i < P; i++) {
+                    const int k = j * P + i;
+                    fprintf(f, "%4d %8.2f %8.2f ", j + 1, pc[2 * i], pc[2 * i + 1]);
+                    fprintf(f, "%8.2f %8.2f %4d\n", ps[2 * k], ps[2 * k + 1], (int)bad[k]);
+                }
+                fprintf(f, "\n");
+            }
+        }
+        fclose(f);
+    }
+}
+
+double kv_factor(int kv) {
+    if (kv >= 300) return 1.0;
+    if (kv >= 200) return 0.002 * (kv - 200) + 0.8;
+    if (kv >= 120) return 0.004375 * (kv - 120) + 0.45;
+    return 0.45;
+}
+
+// ---------------------------------------------------------------------------
+int align_movie(mc_ctx* c, const mc_params* p, const mc_movie* mv, const mc_override* ov,
+                mc_result* res) {
+    using clk = std::chrono::steady_clock;
+    const auto t0 = clk::now();
+    const int W = mv->W, H = mv->H, N = mv->N;
+    if (!mv->frames || W < 8 || (W % 2) || H < 8 || N < 2 || mv->pitch < W ||
+        mv->frame_stride < mv->pitch * H)
+        return fail(c, MC_E_ARG, "invalid movie geometry");
+    if (p->iter < 1) return fail(c, MC_E_ARG, "-Iter must be >= 1");
+    const bool patch = p->patch[0] > 1 && p->patch[1] > 1;
+    const Geo g = make_geo(W, H, N, p->patch[0], p->patch[1], p->ft_bin);
+    const int P = patch ? p->patch[0] * p->patch[1] : 0;
+    if (patch && (g.pw > g.xw / 2 || g.ph > g.xh / 2 || g.pw < 8 || g.ph < 8))
+        return fail(c, MC_E_ARG, "patch grid too fine for the xcf size");
+    if (P > 64) return fail(c, MC_E_ARG, "at most 64 patches");
+    const bool replay = ov && ov->replay_global;
+    if (patch && replay && (!ov->replay_patch || !ov->replay_centers))
+        return fail(c, MC_E_ARG, "patch replay needs local shifts and centres");
+    const int quirks = p->ref_quirks;
+    hipStream_t s = c->stream;
+
+    // DataUtil/CFmIntParam.cpp:80-123 and CBufferPool.cpp:185-202
+    const bool dw = !(p->kv < 80 || p->kv > 300) && p->pix_size > 0 && p->fm_dose > 0;
+    const bool dws = dw && !(p->sum_range[0] < 0) && !(p->sum_range[1] <= 0) &&
+                     (p->sum_range[1] - p->sum_range[0] > 1.0f);
+    const int nsum = 1 + (p->split_sum ? 2 : 0) + (dw ? 1 + (dws ? 1 : 0) : 0);
+    res->num_sums = nsum;
+    res->out_w = g.ow;
+    res->out_h = g.oh;
+    for (int i = 0; i < nsum; i++)
+        if (!res->sums[i]) return fail(c, MC_E_ARG, "missing output sum buffer");
+
+    Fft2D* pf = plan(c, W, H);
+    Fft2D* px = plan(c, g.xw, g.xh);
+    Fft2D* pp = patch ? plan(c, g.pw, g.ph) : nullptr;
+    Fft2D* po = (g.ow == W && g.oh == H) ? pf : plan(c, g.ow, g.oh);
+    if (!pf || !px || (patch && !pp) || !po) return fail(c, MC_E_ARG, "unsupported FFT size");
+
+    const long long fs = (long long)g.cx * H;
+    const long long xfs = (long long)g.xcx * g.xh;
+    const bool xcf_alias = (g.xcx == g.cx && g.xh == H);
+    const int CH = 2;  // frames per forward/inverse chunk (staging stays in the 256 MB MALL)
+    MC_BUF(Sf, cfloat, "Sf", (size_t)N * fs);
+    MC_BUF(T, cfloat, "T", (size_t)CH * pf->t_elems);
+    cfloat* Sx = Sf;
+    if (!xcf_alias && !replay) {
+        Sx = (cfloat*)buf(c, "Sx", sizeof(cfloat) * (size_t)N * xfs);
+        if (!Sx) return fail(c, MC_E_NOMEM, "xcf stack");
+    }
+
+    // Align/CTransformStack.cpp + Align/CGenXcfStack.cpp: normalised R2C of
+    // every frame and the Fourier crop to the xcf size in the same pass.
+    const float scale = (float)(1.0 / ((double)W * H));
+    for (int f0 = 0; f0 < N; f0 += CH) {
+        const int nb = std::min(CH, N - f0);
+        {
+            KTimer kt(c, "fft_rows_r2c");
+            MC_TRY(mc_rows_r2c(*pf, mv->frames + (long long)f0 * mv->frame_stride, mv->pitch,
+                               mv->frame_stride, nullptr, T, pf->t_elems, nb, scale, s));
+        }
+        {
+            KTimer kt(c, "fft_cols_fwd");
+            MC_TRY(mc_cols_fwd(*pf, T, pf->t_elems, Sf + (long long)f0 * fs, fs,
+                               (xcf_alias || replay) ? nullptr : Sx + (long long)f0 * xfs, g.xcx,
+                               g.xh, xfs, nb, s));
+        }
+    }
+
+    std::vector<double> gshift(2 * N, 0.0);
+    std::vector<float> gsf(2 * N);
+    std::vector<double> ps, pc;
+    std::vector<unsigned char> bad;
+    std::vector<int> starts(2 * P, 0);
+    int giters = 0;
+    std::vector<double> gerr;
+    std::vector<int> piters(P, 0);
+    const double tol = p->tol / std::max(g.xbin[0], g.xbin[1]);
+    const auto ta = clk::now();
+
+    if (!replay) {
+        Groups g0 = make_groups(p->group[0], N, p->throw0);
+        IterOut go;
+        MC_CHECK(run_iterative(c, "g_", Sx, 1, N, g.xw, g.xh, px, g0, p->bft[0], p->phase_only,
+                               p->iter, tol, quirks, &go));
+        giters = go.iters[0];
+        gerr.assign(go.errors.begin(), go.errors.begin() + giters);
+        std::vector<double> xs = go.shifts;
+        if (patch) {
+            // Align/CFullAlign.cpp:147-159
+            MC_BUF(fin, double, "xfinal", 2 * N);
+            MC_BUF(A, cfloat, "xphA", (size_t)N * g.xcx);
+            MC_BUF(B, cfloat, "xphB", (size_t)N * g.xh);
+            MC_BUF(xsum, cfloat, "xsum", (size_t)xfs);
+            MC_TRY(hipMemcpyAsync(fin, xs.data(), sizeof(double) * 2 * N, hipMemcpyHostToDevice, s));
+            MC_TRY(mc_launch_phasors(fin, -1.0, 0, nullptr, N, g.xw, g.xcx, g.xh, N, A, B, s));
+            MC_TRY(mc_launch_aligned_sum(Sx, A, B, nullptr, 1, N, g.xcx, g.xh, xsum, s));
+            Fft2D* p128 = plan(c, 128, 128);
+            if (!p128) return fail(c, MC_E_ARG, "128x128 plan");
+            MC_BUF(T128, cfloat, "T128", (size_t)p128->t_elems);
+            MC_BUF(img128, float, "img128", 128 * 128);
+            MC_BUF(feat, unsigned char, "feat", 128 * 128);
+            MC_TRY(mc_cols_inv(*p128, xsum, xfs, g.xcx, g.xh, nullptr, T128, p128->t_elems, 1, s));
+            MC_TRY(mc_rows_c2r(*p128, T128, p128->t_elems, img128, 128, 128 * 128, 1, 0, s));
+            MC_TRY(mc_launch_feature_map(img128, feat, s));
+            std::vector<unsigned char> hf(128 * 128);
+            MC_TRY(hipMemcpyAsync(hf.data(), feat, hf.size(), hipMemcpyDeviceToHost, s));
+            MC_TRY(hipStreamSynchronize(s));
+            if (ov && ov->patch_starts) std::copy(ov->patch_starts, ov->patch_starts + 2 * P, starts.begin());
+            else patch_starts(hf.data(), p->patch[0], p->patch[1], g.xw, g.xh, starts.data());
+            // xcf frames: shift (|s| >= 0.1) then C2R (Correct/CGenRealStack.cpp:147-166)
+            MC_BUF(Tx, cfloat, "Tx", (size_t)N * px->t_elems);
+            MC_BUF(Rx, float, "Rx", (size_t)N * g.xw * g.xh);
+            MC_TRY(mc_launch_phasors(fin, -1.0, 1, nullptr, N, g.xw, g.xcx, g.xh, N, A, B, s));
+            PhasorTabs ph{A, B};
+            MC_TRY(mc_cols_inv(*px, Sx, xfs, g.xcx, g.xh, &ph, Tx, px->t_elems, N, s));
+            MC_TRY(mc_rows_c2r(*px, Tx, px->t_elems, Rx, g.xw, (long long)g.xw * g.xh, N, 0, s));
+            // patches: CExtractPatch + unnormalised R2C, all patches and frames at once
+            MC_BUF(dstart, int, "starts", 2 * P);
+            MC_TRY(hipMemcpyAsync(dstart, starts.data(), sizeof(int) * 2 * P, hipMemcpyHostToDevice, s));
+            const long long pfs = (long long)g.pcx * g.ph;
+            MC_BUF(Tp, cfloat, "Tp", (size_t)P * N * pp->t_elems);
+            MC_BUF(Sp, cfloat, "Sp", (size_t)P * N * pfs);
+            MC_TRY(mc_rows_r2c_mod(*pp, Rx, g.xw, (long long)g.xw * g.xh, N, dstart, Tp, pp->t_elems,
+                                   P * N, 1.0f, s));
+            MC_TRY(mc_cols_fwd(*pp, Tp, pp->t_elems, Sp, pfs, nullptr, 0, 0, 0, P * N, s));
+            Groups g1 = make_groups(p->group[1], N, p->throw0);
+            IterOut po_;
+            MC_CHECK(run_iterative(c, "p_", Sp, P, N, g.pw, g.ph, pp, g1, p->bft[1], p->phase_only,
+                                   p->iter, tol, quirks, &po_));
+            // Align/CMeasurePatches.cpp:66-82: centre and shifts in full-res px
+            ps.assign((size_t)N * P * 2, 0.0);
+            pc.assign(2 * P, 0.0);
+            for (int q = 0; q < P; q++) {
+                piters[q] = po_.iters[q];
+                const int cxp = starts[2 * q] + g.pw / 2, cyp = starts[2 * q + 1] + g.ph / 2;
+                pc[2 * q] = (double)(cxp * g.xbin[0]);
+                pc[2 * q + 1] = (double)(cyp * g.xbin[1]);
+                for (int i = 0; i < N; i++) {
+                    ps[((size_t)i * P + q) * 2] = po_.shifts[((size_t)q * N + i) * 2] * g.xbin[0];
+                    ps[((size_t)i * P + q) * 2 + 1] = po_.shifts[((size_t)q * N + i) * 2 + 1] * g.xbin[1];
+                }
+            }
+            MC_BUF(dps, double, "ps", (size_t)N * P * 2);
+            MC_BUF(dpc, double, "pc", 2 * P);
+            MC_BUF(dbad, unsigned char, "bad", (size_t)N * P);
+            MC_TRY(hipMemcpyAsync(dps, ps.data(), sizeof(double) * ps.size(), hipMemcpyHostToDevice, s));
+            MC_TRY(hipMemcpyAsync(dpc, pc.data(), sizeof(double) * pc.size(), hipMemcpyHostToDevice, s));
+            MC_TRY(mc_launch_patch_stats(dps, dpc, N, P, W, H, quirks, dbad, nullptr, s));
+            bad.assign((size_t)N * P, 0);
+            MC_TRY(hipMemcpyAsync(ps.data(), dps, sizeof(double) * ps.size(), hipMemcpyDeviceToHost, s));
+            MC_TRY(hipMemcpyAsync(bad.data(), dbad, bad.size(), hipMemcpyDeviceToHost, s));
+            MC_TRY(hipStreamSynchronize(s));
+        }
+        // Align/CFullAlign.cpp:160-163,96: xcf bin multiply, truncation to 0.01 px
+        for (int i = 0; i < N; i++) {
+            const float sx = (float)xs[2 * i] * g.xbin[0];
+            const float sy = (float)xs[2 * i + 1] * g.xbin[1];
+            gshift[2 * i] = 0.01 * (double)(long long)((double)sx * 100.0);
+            gshift[2 * i + 1] = 0.01 * (double)(long long)((double)sy * 100.0);
+        }
+    } else {
+        for (int i = 0; i < 2 * N; i++) gshift[i] = ov->replay_global[i];
+        if (patch) {
+            ps.assign(ov->replay_patch, ov->replay_patch + (size_t)N * P * 2);
+            pc.assign(ov->replay_centers, ov->replay_centers + 2 * P);
+            bad.assign((size_t)N * P, 0);
+            if (ov->replay_bad) bad.assign(ov->replay_bad, ov->replay_bad + (size_t)N * P);
+        }
+    }
+    const auto tb = clk::now();
+
+    // ---- correction and sums ----
+    MC_BUF(sums, cfloat, "sums", (size_t)nsum * fs);
+    MC_BUF(gsh, double, "gsh", 2 * N);
+    MC_BUF(fA, cfloat, "fphA", (size_t)N * g.cx);
+    MC_BUF(fB, cfloat, "fphB", (size_t)N * H);
+    const double kvf = kv_factor(p->kv);
+    if (patch) {
+        // Align/CPatchAlign.cpp:47-65: shifts relative to the reference frame
+        const int ref = (p->fm_ref < 0) ? N / 2 : std::min(std::max(p->fm_ref - 1, 0), N - 1);
+        const double rx = gshift[2 * ref], ry = gshift[2 * ref + 1];
+        for (int i = 0; i < N; i++) {
+            gshift[2 * i] -= rx;
+            gshift[2 * i + 1] -= ry;
+        }
+        MC_TRY(hipMemcpyAsync(gsh, gshift.data(), sizeof(double) * 2 * N, hipMemcpyHostToDevice, s));
+        MC_TRY(mc_launch_phasors(gsh, -1.0, 1, nullptr, N, W, g.cx, H, N, fA, fB, s));
+        // Correct/GCorrectPatchShift.cu:214-285: C2R, warp, normalised R2C
+        MC_BUF(Rr, float, "Rr", (size_t)N * W * H);
+        MC_BUF(Rw, float, "Rw", (size_t)N * W * H);
+        const long long rfs = (long long)W * H;
+        for (int f0 = 0; f0 < N; f0 += CH) {
+            const int nb = std::min(CH, N - f0);
+            PhasorTabs ph{fA + (long long)f0 * g.cx, fB + (long long)f0 * H};
+            {
+                KTimer kt(c, "fft_cols_inv");
+                MC_TRY(mc_cols_inv(*pf, Sf + (long long)f0 * fs, fs, g.cx, H, &ph, T, pf->t_elems, nb, s));
+            }
+            {
+                KTimer kt(c, "fft_rows_c2r");
+                MC_TRY(mc_rows_c2r(*pf, T, pf->t_elems, Rr + f0 * rfs, W, rfs, nb, 0, s));
+            }
+        }
+        std::vector<float> psf(ps.size()), pcf(pc.size());
+        for (size_t i = 0; i < ps.size(); i++) psf[i] = (float)ps[i];
+        for (size_t i = 0; i < pc.size(); i++) pcf[i] = (float)pc[i];
+        MC_BUF(dpsf, float, "psf", psf.size());
+        MC_BUF(dpcf, float, "pcf", pcf.size());
+        MC_BUF(dbad2, unsigned char, "bad2", bad.size());
+        MC_TRY(hipMemcpyAsync(dpsf, psf.data(), sizeof(float) * psf.size(), hipMemcpyHostToDevice, s));
+        MC_TRY(hipMemcpyAsync(dpcf, pcf.data(), sizeof(float) * pcf.size(), hipMemcpyHostToDevice, s));
+        MC_TRY(hipMemcpyAsync(dbad2, bad.data(), bad.size(), hipMemcpyHostToDevice, s));
+        {
+            KTimer kt(c, "local_warp");
+            MC_TRY(mc_launch_local_warp(Rr, Rw, W, rfs, W, H, N, P, dpcf, dpsf, dbad2, s));
+        }
+        for (int f0 = 0; f0 < N; f0 += CH) {
+            const int nb = std::min(CH, N - f0);
+            {
+                KTimer kt(c, "fft_rows_r2c");
+                MC_TRY(mc_rows_r2c(*pf, Rw + f0 * rfs, W, rfs, nullptr, T, pf->t_elems, nb, scale, s));
+            }
+            {
+                KTimer kt(c, "fft_cols_fwd");
+                MC_TRY(mc_cols_fwd(*pf, T, pf->t_elems, Sf + (long long)f0 * fs, fs, nullptr, 0, 0, 0,
+                                   nb, s));
+            }
+        }
+        KTimer kt(c, "dw_sums");
+        MC_TRY(mc_launch_dw_sums(Sf, fs, N, g.cx, H, nullptr, nullptr, dw, p->pix_size, kvf,
+                                 p->fm_dose, dws, p->sum_range[0], p->sum_range[1], p->split_sum,
+                                 sums, fs, s));
+    } else {
+        // Correct/CCorrectFullShift.cpp:229-309 with the absolute shifts
+        MC_TRY(hipMemcpyAsync(gsh, gshift.data(), sizeof(double) * 2 * N, hipMemcpyHostToDevice, s));
+        MC_TRY(mc_launch_phasors(gsh, -1.0, 1, nullptr, N, W, g.cx, H, N, fA, fB, s));
+        KTimer kt(c, "dw_sums");
+        MC_TRY(mc_launch_dw_sums(Sf, fs, N, g.cx, H, fA, fB, dw, p->pix_size, kvf, p->fm_dose, dws,
+                                 p->sum_range[0], p->sum_range[1], p->split_sum, sums, fs, s));
+    }
+    // Correct/CCorrectFullShift.cpp:146-185: crop to -FtBin size, C2R, >= 0
+    MC_BUF(To, cfloat, "To", (size_t)po->t_elems);
+    for (int i = 0; i < nsum; i++) {
+        MC_TRY(mc_cols_inv(*po, sums + (long long)i * fs, fs, g.cx, H, nullptr, To, po->t_elems, 1, s));
+        MC_TRY(mc_rows_c2r(*po, To, po->t_elems, res->sums[i], g.ow, (long long)g.ow * g.oh, 1, 1, s));
+    }
+    MC_TRY(hipStreamSynchronize(s));
+    const auto tc = clk::now();
+
+    for (int i = 0; i < 2 * N; i++) gsf[i] = (float)gshift[i];
+    if (res->global_shift) std::copy(gsf.begin(), gsf.end(), res->global_shift);
+    if (res->global_iters) res->global_iters[0] = giters;
+    if (res->global_errors) {
+        const float xb = std::max(g.xbin[0], g.xbin[1]);
+        for (int i = 0; i < giters; i++) res->global_errors[i] = (float)(gerr[i] * xb);
+    }
+    std::vector<float> psf(ps.size()), pcf(pc.size());
+    for (size_t i = 0; i < ps.size(); i++) psf[i] = (float)ps[i];
+    for (size_t i = 0; i < pc.size(); i++) pcf[i] = (float)pc[i];
+    if (patch) {
+        if (res->patch_shift) std::copy(psf.begin(), psf.end(), res->patch_shift);
+        if (res->patch_bad) std::copy(bad.begin(), bad.end(), res->patch_bad);
+        if (res->patch_center) std::copy(pcf.begin(), pcf.end(), res->patch_center);
+        if (res->patch_starts) std::copy(starts.begin(), starts.end(), res->patch_starts);
+        if (res->patch_iters) std::copy(piters.begin(), piters.end(), res->patch_iters);
+    }
+    write_logs(p, W, H, N, P, gsf, psf, bad, pcf);
+    if (p->verbose) {
+        for (int i = 0; i < giters; i++)
+            printf("Iteration %2d  Error: %9.3f\n", i + 1, gerr[i] * std::max(g.xbin[0], g.xbin[1]));
+        printf("Full-frame alignment shift\n");
+        for (int i = 0; i < N; i++)
+            printf("...... Frame (%3d) shift: %9.3f  %9.3f\n", i + 1, gsf[2 * i] - gsf[0],
+                   gsf[2 * i + 1] - gsf[1]);
+        printf("\n");
+    }
+    res->seconds_align = std::chrono::duration<double>(tb - ta).count();
+    res->seconds_total = std::chrono::duration<double>(tc - t0).count();
+    return MC_OK;
+}
+
+}  // namespace
+
+// ===========================================================================
+extern "C" {
+
+void mc_default_params(mc_params* p) {
+    std::memset(p, 0, sizeof(*p));
+    p->iter = 15;
+    p->tol = 0.1f;
+    p->bft[0] = 500.0f;
+    p->bft[1] = 100.0f;
+    p->kv = 300;
+    p->group[0] = 1;
+    p->group[1] = 4;
+    p->fm_ref = -1;
+    p->ft_bin = 1.0f;
+    p->sum_range[0] = 3.0f;
+    p->sum_range[1] = 25.0f;
+    p->ref_quirks = 1;
+}
+
+int mc_ctx_create(int device, void* stream, mc_ctx** out) {
+    if (!out) return MC_E_ARG;
+    *out = nullptr;
+    if (hipSetDevice(device) != hipSuccess) return MC_E_HIP;
+    mc_ctx* c = new mc_ctx;
+    c->device = device;
+    if (stream) {
+        c->stream = (hipStream_t)stream;
+    } else {
+        if (hipStreamCreate(&c->stream) != hipSuccess) {
+            delete c;
+            return MC_E_HIP;
+        }
+        c->own_stream = true;
+    }
+    if (mc_fft_init_constants() != 0 || mc_align_init_constants() != 0) {
+        if (c->own_stream) hipStreamDestroy(c->stream);
+        delete c;
+        return MC_E_HIP;
+    }
+    *out = c;
+    return MC_OK;
+}
+
+int mc_ctx_destroy(mc_ctx* c) {
+    if (!c) return MC_E_ARG;
+    hipSetDevice(c->device);
+    hipStreamSynchronize(c->stream);
+    for (auto& kv : c->plans) {
+        mc_fft2d_destroy(kv.second);
+        delete kv.second;
+    }
+    for (auto& kv : c->rowtw) hipFree(kv.second);
+    for (auto& kv : c->bufs)
+        if (kv.second.p) hipFree(kv.second.p);
+    for (auto& e : c->evs) {
+        hipEventDestroy(e.a);
+        hipEventDestroy(e.b);
+    }
+    if (c->own_stream) hipStreamDestroy(c->stream);
+    delete c;
+    return MC_OK;
+}
+
+const char* mc_last_error(mc_ctx* c) { return c ? c->err.c_str() : "null context"; }
+
+int mc_sync(mc_ctx* c) {
+    if (!c) return MC_E_ARG;
+    MC_TRY(hipStreamSynchronize(c->stream));
+    return MC_OK;
+}
+
+int mc_geometry(int W, int H, int px, int py, int* sizes, float* xbin) {
+    if (W < 8 || H < 8 || (W % 2)) return MC_E_ARG;
+    const Geo g = make_geo(W, H, 2, px, py, 1.0f);
+    sizes[0] = g.xw;
+    sizes[1] = g.xh;
+    sizes[2] = g.pw;
+    sizes[3] = g.ph;
+    xbin[0] = g.xbin[0];
+    xbin[1] = g.xbin[1];
+    return MC_OK;
+}
+
+int mc_output_geometry(const mc_params* p, int W, int H, int N, int* ns, int* ow, int* oh) {
+    if (!p || W < 8 || H < 8 || (W % 2)) return MC_E_ARG;
+    const Geo g = make_geo(W, H, N, p->patch[0], p->patch[1], p->ft_bin);
+    const bool dw = !(p->kv < 80 || p->kv > 300) && p->pix_size > 0 && p->fm_dose > 0;
+    const bool dws = dw && !(p->sum_range[0] < 0) && !(p->sum_range[1] <= 0) &&
+                     (p->sum_range[1] - p->sum_range[0] > 1.0f);
+    *ns = 1 + (p->split_sum ? 2 : 0) + (dw ? 1 + (dws ? 1 : 0) : 0);
+    *ow = g.ow;
+    *oh = g.oh;
+    return MC_OK;
+}
+
+int mc_align_movie(mc_ctx* c, const mc_params* p, const mc_movie* mv, const mc_override* ov,
+                   mc_result* res) {
+    if (!c || !p || !mv || !res) return MC_E_ARG;
+    hipSetDevice(c->device);
+    return align_movie(c, p, mv, ov, res);
+}
+
+int mc_rfft2(mc_ctx* c, const float* in, int W, int H, long long pitch, long long in_stride,
+             int batch, float scale, float* out_spec) {
+    if (!c || !in || !out_spec || batch < 1 || pitch < W) return MC_E_ARG;
+    Fft2D* pl = plan(c, W, H);
+    if (!pl) return fail(c, MC_E_ARG, "unsupported FFT size");
+    MC_BUF(T, cfloat, "api_T", (size_t)batch * pl->t_elems);
+    MC_TRY(mc_rows_r2c(*pl, in, pitch, in_stride, nullptr, T, pl->t_elems, batch, scale, c->stream));
+    MC_TRY(mc_cols_fwd(*pl, T, pl->t_elems, (cfloat*)out_spec, pl->s_elems, nullptr, 0, 0, 0, batch,
+                       c->stream));
+    return MC_OK;
+}
+
+int mc_irfft2(mc_ctx* c, const float* in_spec, int W, int H, int batch, float* out,
+              long long pitch, long long out_stride, int clamp) {
+    if (!c || !in_spec || !out || batch < 1 || pitch < W) return MC_E_ARG;
+    Fft2D* pl = plan(c, W, H);
+    if (!pl) return fail(c, MC_E_ARG, "unsupported FFT size");
+    MC_BUF(T, cfloat, "api_T", (size_t)batch * pl->t_elems);
+    MC_TRY(mc_cols_inv(*pl, (const cfloat*)in_spec, pl->s_elems, pl->cx, H, nullptr, T, pl->t_elems,
+                       batch, c->stream));
+    MC_TRY(mc_rows_c2r(*pl, T, pl->t_elems, out, pitch, out_stride, batch, clamp, c->stream));
+    return MC_OK;
+}
+
+int mc_phase_shift_accumulate(mc_ctx* c, const float* in_spec, int W, int H, int N,
+                              const float* shifts, float* out_spec) {
+    if (!c || !in_spec || !shifts || !out_spec || N < 1 || W < 2 || (W % 2)) return MC_E_ARG;
+    const int cx = W / 2 + 1;
+    std::vector<double> sh(shifts, shifts + 2 * N);
+    MC_BUF(ds, double, "api_sh", 2 * N);
+    MC_BUF(A, cfloat, "api_A", (size_t)N * cx);
+    MC_BUF(B, cfloat, "api_B", (size_t)N * H);
+    MC_TRY(hipMemcpyAsync(ds, sh.data(), sizeof(double) * 2 * N, hipMemcpyHostToDevice, c->stream));
+    MC_TRY(mc_launch_phasors(ds, 1.0, 0, nullptr, N, W, cx, H, N, A, B, c->stream));
+    MC_TRY(mc_launch_aligned_sum((const cfloat*)in_spec, A, B, nullptr, 1, N, cx, H,
+                                 (cfloat*)out_spec, c->stream));
+    return MC_OK;
+}
+
+int mc_group_correlate_window(mc_ctx* c, const float* sum_spec, const float* grp_spec, int W,
+                              int H, float bfactor, int phase_only, int sea_x, int sea_y,
+                              float* window_host) {
+    if (!c || !sum_spec || !grp_spec || !window_host || W < 4 || (W % 2) || sea_x < 3 ||
+        sea_y < 3 || sea_x > 64 || sea_y > std::min(64, H) || sea_x > W)
+        return MC_E_ARG;
+    Fft2D* pl = plan(c, W, H);
+    if (!pl) return fail(c, MC_E_ARG, "unsupported FFT size");
+    const int cx = W / 2 + 1;
+    IterState st;
+    st.J = 1;
+    st.N = 1;
+    st.G = 1;
+    MC_BUF(gs, int, "api_g", 2);
+    const int hg[2] = {0, 1};
+    MC_TRY(hipMemcpyAsync(gs, hg, sizeof(hg), hipMemcpyHostToDevice, c->stream));
+    st.gstart = gs;
+    st.gsize = gs + 1;
+    st.done = nullptr;
+    MC_BUF(zero, double, "api_z", 2);
+    MC_TRY(hipMemsetAsync(zero, 0, 2 * sizeof(double), c->stream));
+    MC_BUF(A, cfloat, "api_A", cx);
+    MC_BUF(B, cfloat, "api_B", H);
+    MC_TRY(mc_launch_phasors(zero, 1.0, 0, nullptr, 1, W, cx, H, 1, A, B, c->stream));
+    MC_BUF(T1, cfloat, "api_T1", (size_t)sea_y * cx);
+    MC_BUF(win, float, "api_win", (size_t)sea_x * sea_y);
+    cfloat* twW = row_twiddles(c, W);
+    MC_TRY(mc_launch_correlate(pl->col, pl->C, (const cfloat*)grp_spec, (const cfloat*)sum_spec, A,
+                               B, st, cx, H, bfactor, phase_only, sea_y, T1, c->stream));
+    MC_TRY(mc_launch_row_dft(T1, twW, nullptr, 1, 1, cx, W, sea_x, sea_y, win, c->stream));
+    MC_TRY(hipMemcpyAsync(window_host, win, sizeof(float) * sea_x * sea_y, hipMemcpyDeviceToHost,
+                          c->stream));
+    MC_TRY(hipStreamSynchronize(c->stream));
+    return MC_OK;
+}
+
+int mc_peak_fit(mc_ctx* c, const float* windows, int nx, int ny, int count, float* shifts_host) {
+    if (!c || !windows || !shifts_host || nx < 3 || ny < 3 || count < 1) return MC_E_ARG;
+    MC_BUF(d, double, "api_pk", 2 * count);
+    MC_TRY(mc_launch_peaks_f(windows, nx, ny, count, d, c->stream));
+    std::vector<double> h(2 * count);
+    MC_TRY(hipMemcpyAsync(h.data(), d, sizeof(double) * 2 * count, hipMemcpyDeviceToHost, c->stream));
+    MC_TRY(hipStreamSynchronize(c->stream));
+    for (int i = 0; i < 2 * count; i++) shifts_host[i] = (float)h[i];
+    return MC_OK;
+}
+
+int mc_spline_despike_fp64(mc_ctx* c, double* traj, int n, int count, int quirks) {
+    if (!c || !traj || n < 3 || count < 1) return MC_E_ARG;
+    MC_BUF(d, double, "api_traj", (size_t)2 * n * count);
+    MC_BUF(sc, double, "api_trs", (size_t)5 * n * count);
+    MC_TRY(hipMemcpyAsync(d, traj, sizeof(double) * 2 * n * count, hipMemcpyHostToDevice, c->stream));
+    MC_TRY(mc_launch_despike(d, sc, n, count, quirks, c->stream));
+    MC_TRY(hipMemcpyAsync(traj, d, sizeof(double) * 2 * n * count, hipMemcpyDeviceToHost, c->stream));
+    MC_TRY(hipStreamSynchronize(c->stream));
+    return MC_OK;
+}
+
+int mc_patch_stats_fp64(mc_ctx* c, double* shifts, const double* centers, int N, int P, int W,
+                        int H, int quirks, unsigned char* bad_host, int* ref_host) {
+    if (!c || !shifts || !centers || !bad_host || N < 2 || P < 2) return MC_E_ARG;
+    MC_BUF(ds, double, "api_ps", (size_t)N * P * 2);
+    MC_BUF(dc, double, "api_pc", 2 * P);
+    MC_BUF(db, unsigned char, "api_bad", (size_t)N * P);
+    MC_BUF(dr, int, "api_ref", 1);
+    MC_TRY(hipMemcpyAsync(ds, shifts, sizeof(double) * N * P * 2, hipMemcpyHostToDevice, c->stream));
+    MC_TRY(hipMemcpyAsync(dc, centers, sizeof(double) * 2 * P, hipMemcpyHostToDevice, c->stream));
+    MC_TRY(mc_launch_patch_stats(ds, dc, N, P, W, H, quirks, db, dr, c->stream));
+    MC_TRY(hipMemcpyAsync(shifts, ds, sizeof(double) * N * P * 2, hipMemcpyDeviceToHost, c->stream));
+    MC_TRY(hipMemcpyAsync(bad_host, db, (size_t)N * P, hipMemcpyDeviceToHost, c->stream));
+    int r = 0;
+    MC_TRY(hipMemcpyAsync(&r, dr, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    MC_TRY(hipStreamSynchronize(c->stream));
+    if (ref_host) *ref_host = r;
+    return MC_OK;
+}
+
+int mc_local_warp(mc_ctx* c, const float* in, float* out, int W, int H, int N, int P,
+                  const float* centers, const float* shifts, const unsigned char* bad) {
+    if (!c || !in || !out || !centers || !shifts || W < 2 || H < 2 || N < 1 || P < 1 || P > 64)
+        return MC_E_ARG;
+    MC_BUF(dc, float, "api_wc", 2 * P);
+    MC_BUF(ds, float, "api_ws", (size_t)2 * N * P);
+    MC_BUF(db, unsigned char, "api_wb", (size_t)N * P);
+    MC_TRY(hipMemcpyAsync(dc, centers, sizeof(float) * 2 * P, hipMemcpyHostToDevice, c->stream));
+    MC_TRY(hipMemcpyAsync(ds, shifts, sizeof(float) * 2 * N * P, hipMemcpyHostToDevice, c->stream));
+    if (bad) MC_TRY(hipMemcpyAsync(db, bad, (size_t)N * P, hipMemcpyHostToDevice, c->stream));
+    else MC_TRY(hipMemsetAsync(db, 0, (size_t)N * P, c->stream));
+    MC_TRY(mc_launch_local_warp(in, out, W, (long long)W * H, W, H, N, P, dc, ds, db, c->stream));
+    return MC_OK;
+}
+
+int mc_dose_weight_sum(mc_ctx* c, int W, int H, int N, float pix, int kv, float fm_dose,
+                       float* out) {
+    if (!c || !out || W < 2 || (W % 2) || H < 2 || N < 1 || pix <= 0) return MC_E_ARG;
+    MC_TRY(mc_launch_dose_wsum(W / 2 + 1, H, N, pix, kv_factor(kv), fm_dose, out, c->stream));
+    return MC_OK;
+}
+
+int mc_set_kernel_timing(mc_ctx* c, const char* kernel) {
+    if (!c) return MC_E_ARG;
+    c->timed = kernel ? kernel : "";
+    c->nev = 0;
+    return MC_OK;
+}
+
+int mc_get_kernel_timing(mc_ctx* c, double* total_ms, int* launches) {
+    if (!c || !total_ms || !launches) return MC_E_ARG;
+    MC_TRY(hipStreamSynchronize(c->stream));
+    double t = 0;
+    for (size_t i = 0; i < c->nev; i++) {
+        float ms = 0;
+        MC_TRY(hipEventElapsedTime(&ms, c->evs[i].a, c->evs[i].b));
+        t += ms;
+    }
+    *total_ms = t;
+    *launches = (int)c->nev;
+    c->nev = 0;
+    return MC_OK;
+}
+
+int mc_synth_render(mc_ctx* c, float* frames, int W, int H, int N, long long pitch,
+                    long long fstride, int nblobs, const float* blobs, const float* disp,
+                    float scale, unsigned long long seed, int poisson) {
+    if (!c || !frames || !blobs || !disp || W < 2 || H < 2 || N < 1 || nblobs < 1 || pitch < W)
+        return MC_E_ARG;
+    const int tile = 32, tx = (W + tile - 1) / tile, ty = (H + tile - 1) / tile;
+    std::vector<std::vector<int>> lists((size_t)tx * ty);
+    for (int k = 0; k < nblobs; k++) {
+        float dmax = 0;
+        for (int f = 0; f < N; f++) {
+            const float dx = disp[((size_t)f * nblobs + k) * 2], dy = disp[((size_t)f * nblobs + k) * 2 + 1];
+            dmax = std::max(dmax, std::sqrt(dx * dx + dy * dy));
+        }
+        const float r = 4.0f * blobs[4 * k + 2] + dmax + 1.0f;
+        const int x0 = (int)std::floor((blobs[4 * k] - r) / tile), x1 = (int)std::floor((blobs[4 * k] + r) / tile);
+        const int y0 = (int)std::floor((blobs[4 * k + 1] - r) / tile), y1 = (int)std::floor((blobs[4 * k + 1] + r) / tile);
+        for (int yy = y0; yy <= y1; yy++)
+            for (int xx = x0; xx <= x1; xx++) {
+                const int wx = ((xx % tx) + tx) % tx, wy = ((yy % ty) + ty) % ty;
+                std::vector<int>& L = lists[(size_t)wy * tx + wx];
+                if (L.empty() || L.back() != k) L.push_back(k);
+            }
+    }
+    std::vector<int> off(1, 0), idx;
+    for (auto& L : lists) {
+        idx.insert(idx.end(), L.begin(), L.end());
+        off.push_back((int)idx.size());
+    }
+    if (idx.empty()) idx.push_back(0);
+    MC_BUF(db, float, "syn_b", 4 * (size_t)nblobs);
+    MC_BUF(dd, float, "syn_d", 2 * (size_t)nblobs * N);
+    MC_BUF(doff, int, "syn_o", off.size());
+    MC_BUF(didx, int, "syn_i", idx.size());
+    MC_TRY(hipMemcpyAsync(db, blobs, sizeof(float) * 4 * nblobs, hipMemcpyHostToDevice, c->stream));
+    MC_TRY(hipMemcpyAsync(dd, disp, sizeof(float) * 2 * (size_t)nblobs * N, hipMemcpyHostToDevice, c->stream));
+    MC_TRY(hipMemcpyAsync(doff, off.data(), sizeof(int) * off.size(), hipMemcpyHostToDevice, c->stream));
+    MC_TRY(hipMemcpyAsync(didx, idx.data(), sizeof(int) * idx.size(), hipMemcpyHostToDevice, c->stream));
+    MC_TRY(mc_launch_synth(frames, W, H, N, pitch, fstride, db, dd, nblobs, doff, didx, tile, tx,
+                           scale, seed, poisson, c->stream));
+    MC_TRY(hipStreamSynchronize(c->stream));
+    return MC_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,329 @@
+"""ctypes binding of libmc3.so (the C ABI in include/mc_abi.h).
+
+Device buffers are torch CUDA tensors (plumbing only); every call goes to the
+HIP library.  A missing library raises: there is no CPU fallback.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libmc3.so")
+_LIB = None
+
+MC_OK = 0
+
+
+class McParams(C.Structure):
+    """mc_params: the CInput flag subset (CInput.cpp:82-128)."""
+    _fields_ = [
+        ("patch", C.c_int * 2), ("iter", C.c_int), ("tol", C.c_float),
+        ("bft", C.c_float * 2), ("phase_only", C.c_int), ("fm_dose", C.c_float),
+        ("pix_size", C.c_float), ("kv", C.c_int), ("group", C.c_int * 2),
+        ("fm_ref", C.c_int), ("ft_bin", C.c_float), ("sum_range", C.c_float * 2),
+        ("split_sum", C.c_int), ("throw0", C.c_int), ("ref_quirks", C.c_int),
+        ("log_dir", C.c_char_p), ("movie_name", C.c_char_p), ("verbose", C.c_int),
+    ]
+
+
+class McMovie(C.Structure):
+    _fields_ = [("frames", C.c_void_p), ("W", C.c_int), ("H", C.c_int), ("N", C.c_int),
+                ("pitch", C.c_longlong), ("frame_stride", C.c_longlong)]
+
+
+class McResult(C.Structure):
+    _fields_ = [
+        ("sums", C.c_void_p * 5), ("num_sums", C.c_int), ("out_w", C.c_int),
+        ("out_h", C.c_int), ("global_shift", C.POINTER(C.c_float)),
+        ("global_iters", C.POINTER(C.c_int)), ("global_errors", C.POINTER(C.c_float)),
+        ("patch_shift", C.POINTER(C.c_float)), ("patch_bad", C.POINTER(C.c_ubyte)),
+        ("patch_center", C.POINTER(C.c_float)), ("patch_starts", C.POINTER(C.c_int)),
+        ("patch_iters", C.POINTER(C.c_int)), ("seconds_align", C.c_double),
+        ("seconds_total", C.c_double),
+    ]
+
+
+class McOverride(C.Structure):
+    _fields_ = [
+        ("replay_global", C.POINTER(C.c_float)), ("replay_patch", C.POINTER(C.c_float)),
+        ("replay_centers", C.POINTER(C.c_float)), ("replay_bad", C.POINTER(C.c_ubyte)),
+        ("patch_starts", C.POINTER(C.c_int)),
+    ]
+
+
+# Every symbol include/mc_abi.h declares.
+ABI_SYMBOLS = (
+    "mc_ctx_create", "mc_ctx_destroy", "mc_last_error", "mc_sync", "mc_default_params",
+    "mc_output_geometry", "mc_geometry", "mc_align_movie", "mc_rfft2", "mc_irfft2",
+    "mc_phase_shift_accumulate", "mc_group_correlate_window", "mc_peak_fit",
+    "mc_spline_despike_fp64", "mc_patch_stats_fp64", "mc_local_warp", "mc_dose_weight_sum",
+    "mc_set_kernel_timing", "mc_get_kernel_timing", "mc_synth_render",
+)
+
+
+def load():
+    """Load libmc3.so; raises if it has not been built."""
+    global _LIB
+    if _LIB is not None:
+        return _LIB
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError("libmc3.so is not built (run __graft_entry__.build())")
+    lib = C.CDLL(LIB_PATH)
+    lib.mc_last_error.restype = C.c_char_p
+    for name in ABI_SYMBOLS:
+        if name != "mc_last_error" and name != "mc_default_params":
+            getattr(lib, name).restype = C.c_int
+    _LIB = lib
+    return lib
+
+
+def default_params(**kw):
+    p = McParams()
+    load().mc_default_params(C.byref(p))
+    for k, v in kw.items():
+        if k in ("patch", "bft", "group", "sum_range"):
+            arr = getattr(p, k)
+            arr[0], arr[1] = v[0], v[1]
+        elif k in ("log_dir", "movie_name"):
+            setattr(p, k, v.encode() if isinstance(v, str) else v)
+        else:
+            setattr(p, k, v)
+    return p
+
+
+def geometry(W, H, px=0, py=0):
+    s = (C.c_int * 4)()
+    b = (C.c_float * 2)()
+    rc = load().mc_geometry(W, H, px, py, s, b)
+    if rc != MC_OK:
+        raise ValueError("unsupported geometry")
+    return dict(xcf=(s[0], s[1]), patch=(s[2], s[3]), xcf_bin=(b[0], b[1]))
+
+
+def _fptr(a):
+    return a.ctypes.data_as(C.POINTER(C.c_float))
+
+
+class Context:
+    """One mc_ctx (stream, FFT plans, device workspace) on one GPU."""
+
+    def __init__(self, device=0, stream=None):
+        self.lib = load()
+        self.ptr = C.c_void_p()
+        rc = self.lib.mc_ctx_create(device, C.c_void_p(stream) if stream else None,
+                                    C.byref(self.ptr))
+        if rc != MC_OK:
+            raise RuntimeError("mc_ctx_create failed: %d" % rc)
+
+    def close(self):
+        if self.ptr:
+            self.lib.mc_ctx_destroy(self.ptr)
+            self.ptr = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, rc, what):
+        if rc != MC_OK:
+            msg = self.lib.mc_last_error(self.ptr)
+            raise RuntimeError("%s failed (%d): %s" % (what, rc, msg.decode() if msg else ""))
+
+    def sync(self):
+        self._check(self.lib.mc_sync(self.ptr), "mc_sync")
+
+    # ---- stage entry points (torch tensors in/out) ----
+    def rfft2(self, frames, scale=1.0):
+        """frames: cuda float32 [B, H, W] -> complex64 [B, W/2+1, H] ([kx][ky])."""
+        import torch
+        B, H, W = frames.shape
+        out = torch.empty((B, W // 2 + 1, H), dtype=torch.complex64, device=frames.device)
+        torch.cuda.synchronize()
+        self._check(self.lib.mc_rfft2(self.ptr, C.c_void_p(frames.data_ptr()), W, H,
+                                      C.c_longlong(frames.stride(1)),
+                                      C.c_longlong(frames.stride(0)), B, C.c_float(scale),
+                                      C.c_void_p(out.data_ptr())), "mc_rfft2")
+        self.sync()
+        return out
+
+    def irfft2(self, spec, W, clamp=False):
+        """spec: complex64 [B, W/2+1, H] -> float32 [B, H, W]."""
+        import torch
+        B, cx, H = spec.shape
+        out = torch.empty((B, H, W), dtype=torch.float32, device=spec.device)
+        torch.cuda.synchronize()
+        self._check(self.lib.mc_irfft2(self.ptr, C.c_void_p(spec.data_ptr()), W, H, B,
+                                       C.c_void_p(out.data_ptr()), C.c_longlong(W),
+                                       C.c_longlong(W * H), int(clamp)), "mc_irfft2")
+        self.sync()
+        return out
+
+    def phase_shift_accumulate(self, spec, shifts, W):
+        import torch
+        N, cx, H = spec.shape
+        out = torch.empty((cx, H), dtype=torch.complex64, device=spec.device)
+        sh = np.ascontiguousarray(shifts, dtype=np.float32)
+        torch.cuda.synchronize()
+        self._check(self.lib.mc_phase_shift_accumulate(
+            self.ptr, C.c_void_p(spec.data_ptr()), W, H, N, _fptr(sh),
+            C.c_void_p(out.data_ptr())), "mc_phase_shift_accumulate")
+        self.sync()
+        return out
+
+    def correlate_window(self, sum_spec, grp_spec, W, bft, phase_only=False, sea=(64, 64)):
+        import torch
+        H = sum_spec.shape[-1]
+        win = np.zeros((sea[1], sea[0]), np.float32)
+        torch.cuda.synchronize()
+        self._check(self.lib.mc_group_correlate_window(
+            self.ptr, C.c_void_p(sum_spec.data_ptr()), C.c_void_p(grp_spec.data_ptr()), W, H,
+            C.c_float(bft), int(phase_only), sea[0], sea[1], _fptr(win)),
+            "mc_group_correlate_window")
+        return win
+
+    def peak_fit(self, windows):
+        import torch
+        cnt, ny, nx = windows.shape
+        out = np.zeros((cnt, 2), np.float32)
+        torch.cuda.synchronize()
+        self._check(self.lib.mc_peak_fit(self.ptr, C.c_void_p(windows.data_ptr()), nx, ny, cnt,
+                                         _fptr(out)), "mc_peak_fit")
+        return out
+
+    def despike(self, sx, sy, quirks=True):
+        t = np.ascontiguousarray(np.stack([sx, sy])[None], dtype=np.float64).copy()
+        self._check(self.lib.mc_spline_despike_fp64(
+            self.ptr, t.ctypes.data_as(C.POINTER(C.c_double)), t.shape[-1], 1, int(quirks)),
+            "mc_spline_despike_fp64")
+        return t[0, 0], t[0, 1]
+
+    def patch_stats(self, shifts, centers, W, H, quirks=True):
+        s = np.ascontiguousarray(shifts, dtype=np.float64).copy()
+        cen = np.ascontiguousarray(centers, dtype=np.float64)
+        N, P = s.shape[0], s.shape[1]
+        bad = np.zeros((N, P), np.uint8)
+        ref = C.c_int(0)
+        self._check(self.lib.mc_patch_stats_fp64(
+            self.ptr, s.ctypes.data_as(C.POINTER(C.c_double)),
+            cen.ctypes.data_as(C.POINTER(C.c_double)), N, P, W, H, int(quirks),
+            bad.ctypes.data_as(C.POINTER(C.c_ubyte)), C.byref(ref)), "mc_patch_stats_fp64")
+        return s, bad, ref.value
+
+    def local_warp(self, frames, centers, shifts, bad=None):
+        import torch
+        N, H, W = frames.shape
+        P = len(centers)
+        out = torch.empty_like(frames)
+        cen = np.ascontiguousarray(centers, dtype=np.float32)
+        sh = np.ascontiguousarray(shifts, dtype=np.float32)
+        bp = None
+        if bad is not None:
+            bad = np.ascontiguousarray(bad, dtype=np.uint8)
+            bp = bad.ctypes.data_as(C.POINTER(C.c_ubyte))
+        torch.cuda.synchronize()
+        self._check(self.lib.mc_local_warp(self.ptr, C.c_void_p(frames.data_ptr()),
+                                           C.c_void_p(out.data_ptr()), W, H, N, P, _fptr(cen),
+                                           _fptr(sh), bp), "mc_local_warp")
+        self.sync()
+        return out
+
+    def dose_weight_sum(self, W, H, N, pix, kv, fm_dose):
+        import torch
+        out = torch.empty((W // 2 + 1, H), dtype=torch.float32, device="cuda")
+        self._check(self.lib.mc_dose_weight_sum(self.ptr, W, H, N, C.c_float(pix), kv,
+                                                C.c_float(fm_dose), C.c_void_p(out.data_ptr())),
+                    "mc_dose_weight_sum")
+        self.sync()
+        return out
+
+    def synth_render(self, frames, recipe, poisson=True, seed=None):
+        """Render a MovieRecipe into a cuda float32 [N, H, W] tensor."""
+        N, H, W = frames.shape
+        blobs = np.stack([recipe.bx, recipe.by, recipe.bs, recipe.ba], 1).astype(np.float32)
+        disp = np.zeros((N, len(recipe.bx), 2), np.float32)
+        for t in range(N):
+            dx, dy = recipe.blob_displacement(t)
+            disp[t, :, 0] = dx
+            disp[t, :, 1] = dy
+        self._check(self.lib.mc_synth_render(
+            self.ptr, C.c_void_p(frames.data_ptr()), W, H, N, C.c_longlong(frames.stride(1)),
+            C.c_longlong(frames.stride(0)), len(blobs), _fptr(blobs), _fptr(disp),
+            C.c_float(recipe.scale), C.c_ulonglong(recipe.seed if seed is None else seed),
+            int(poisson)), "mc_synth_render")
+
+    def set_kernel_timing(self, name):
+        self._check(self.lib.mc_set_kernel_timing(self.ptr, name.encode() if name else None),
+                    "mc_set_kernel_timing")
+
+    def kernel_timing(self):
+        ms = C.c_double(0)
+        n = C.c_int(0)
+        self._check(self.lib.mc_get_kernel_timing(self.ptr, C.byref(ms), C.byref(n)),
+                    "mc_get_kernel_timing")
+        return ms.value, n.value
+
+    # ---- whole path ----
+    def output_geometry(self, params, W, H, N):
+        ns, ow, oh = C.c_int(), C.c_int(), C.c_int()
+        self._check(self.lib.mc_output_geometry(C.byref(params), W, H, N, C.byref(ns),
+                                                C.byref(ow), C.byref(oh)), "mc_output_geometry")
+        return ns.value, ow.value, oh.value
+
+    def align_movie(self, frames, params, replay_global=None, replay_patch=None,
+                    replay_centers=None, replay_bad=None, patch_starts=None, sums=None,
+                    width=None):
+        """Align an [N, H, pitch] cuda float32 movie whose rows hold `width`
+        (default pitch) pixels.  Returns numpy shift tables and cuda sums."""
+        import torch
+        N, H, pitch = frames.shape
+        W = pitch if width is None else width
+        ns, ow, oh = self.output_geometry(params, W, H, N)
+        P = params.patch[0] * params.patch[1] if (params.patch[0] > 1 and params.patch[1] > 1) else 0
+        if sums is None:
+            sums = [torch.empty((oh, ow), dtype=torch.float32, device=frames.device)
+                    for _ in range(ns)]
+        mv = McMovie(C.c_void_p(frames.data_ptr()), W, H, N, frames.stride(1), frames.stride(0))
+        res = McResult()
+        for i in range(ns):
+            res.sums[i] = C.c_void_p(sums[i].data_ptr())
+        out = dict(global_shift=np.zeros((N, 2), np.float32),
+                   global_iters=np.zeros(1, np.int32),
+                   global_errors=np.zeros(max(params.iter, 1), np.float32))
+        res.global_shift = _fptr(out["global_shift"])
+        res.global_iters = out["global_iters"].ctypes.data_as(C.POINTER(C.c_int))
+        res.global_errors = _fptr(out["global_errors"])
+        if P:
+            out.update(patch_shift=np.zeros((N, P, 2), np.float32),
+                       patch_bad=np.zeros((N, P), np.uint8),
+                       patch_center=np.zeros((P, 2), np.float32),
+                       patch_starts=np.zeros((P, 2), np.int32),
+                       patch_iters=np.zeros(P, np.int32))
+            res.patch_shift = _fptr(out["patch_shift"])
+            res.patch_bad = out["patch_bad"].ctypes.data_as(C.POINTER(C.c_ubyte))
+            res.patch_center = _fptr(out["patch_center"])
+            res.patch_starts = out["patch_starts"].ctypes.data_as(C.POINTER(C.c_int))
+            res.patch_iters = out["patch_iters"].ctypes.data_as(C.POINTER(C.c_int))
+        keep = []
+
+        def fp(a, dt=np.float32, ct=C.c_float):
+            if a is None:
+                return None
+            a = np.ascontiguousarray(a, dtype=dt)
+            keep.append(a)
+            return a.ctypes.data_as(C.POINTER(ct))
+
+        ov = McOverride(fp(replay_global), fp(replay_patch), fp(replay_centers),
+                        fp(replay_bad, np.uint8, C.c_ubyte), fp(patch_starts, np.int32, C.c_int))
+        torch.cuda.synchronize()
+        self._check(self.lib.mc_align_movie(self.ptr, C.byref(params), C.byref(mv), C.byref(ov),
+                                            C.byref(res)), "mc_align_movie")
+        out["global_iters"] = int(out["global_iters"][0])
+        out["global_errors"] = out["global_errors"][: out["global_iters"]]
+        out["sums"] = sums
+        out["num_sums"] = ns
+        out["seconds_align"] = res.seconds_align
+        out["seconds_total"] = res.seconds_total
+        return out
