@@ -56,12 +56,11 @@ def cpu_baseline(recipe, params_kw, frames_needed=8):
     """FP64 oracle on the first `frames_needed` frames of the same movie."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
-    import torch
 
     from motioncor3_amd import lib as L
 
-    sub = torch.empty((frames_needed, H, W), dtype=torch.float32, device="cuda")
     ctx = L.Context(0)
+    sub = ctx.empty((frames_needed, H, W), np.float32)
     # same blobs, drift and local field as the timed movie, first frames only
     from motioncor3_amd.synth import MovieRecipe
     r8 = MovieRecipe.__new__(MovieRecipe)
@@ -69,7 +68,7 @@ def cpu_baseline(recipe, params_kw, frames_needed=8):
     r8.N = frames_needed
     r8.displacement = recipe.displacement[:frames_needed]
     ctx.synth_render(sub, r8)
-    host = sub.cpu().numpy()
+    host = sub.numpy()
     po = O.default_params(**params_kw)
     t0 = time.time()
     ores = O.align_movie(host, po)
@@ -106,24 +105,21 @@ def main():
               "(one movie per GPU); --gpus %d is not supported" % args.gpus, file=sys.stderr)
         sys.exit(2)
 
-    import torch
-
     from motioncor3_amd import lib as L
     from motioncor3_amd.synth import MovieRecipe
 
     recipe = MovieRecipe(W, H, N, 20000, seed=20250101, dose=1.0, local_max=3.0)
     ctx = L.Context(0)
-    frames = torch.empty((N, H, W), dtype=torch.float32, device="cuda")
+    frames = ctx.empty((N, H, W), np.float32)
     ctx.synth_render(frames, recipe)
     kw = dict(patch=(5, 5), bft=(500, 100), pix_size=0.83, kv=300, fm_dose=1.0)
     prm = L.default_params(**kw)
     ns, ow, oh = ctx.output_geometry(prm, W, H, N)
-    sums = [torch.empty((oh, ow), dtype=torch.float32, device="cuda") for _ in range(ns)]
+    sums = [ctx.empty((oh, ow), np.float32) for _ in range(ns)]
     for _ in range(args.warmup):
         res = ctx.align_movie(frames, prm, sums=sums)
 
     ctx.set_kernel_timing(args.kernel)
-    torch.cuda.synchronize()
     ctx.sync()
     t0 = time.perf_counter()
     iters = []
@@ -131,7 +127,6 @@ def main():
         res = ctx.align_movie(frames, prm, sums=sums)
         iters.append((res["global_iters"], int(np.max(res["patch_iters"]))))
     ctx.sync()
-    torch.cuda.synchronize()
     t1 = time.perf_counter()
     kms, klaunch = ctx.kernel_timing()
     ctx.set_kernel_timing(None)

@@ -118,6 +118,14 @@ int mc_output_geometry(const mc_params* p, int W, int H, int N, int* num_sums, i
 /* xcf image size, patch image size and xcf bin (CBufferPool.cpp:220-262). */
 int mc_geometry(int W, int H, int px, int py, int* sizes4, float* xcf_bin2);
 
+/* ---- device memory for callers without their own HIP allocator ----
+ * (the reference's CGpuBuffer / cudaMalloc role, CGpuBuffer.cpp:56-176).
+ * Copies are ordered on the context stream and complete on return. */
+int mc_malloc(mc_ctx* ctx, unsigned long long bytes, void** out);
+int mc_free(mc_ctx* ctx, void* ptr);
+/* dir: 0 host->device, 1 device->host, 2 device->device */
+int mc_memcpy(mc_ctx* ctx, void* dst, const void* src, unsigned long long bytes, int dir);
+
 /* ---- whole path: Align::CAlignMain::DoIt (Align/CAlignMain.cpp:39-75) ---- */
 int mc_align_movie(mc_ctx* ctx, const mc_params* p, const mc_movie* movie,
                    const mc_override* ov, mc_result* res);

@@ -45,13 +45,13 @@ __global__ void k_phasors(const double* shifts, double mult, int skip, const uns
 __global__ void k_aligned_sum(const cfloat* __restrict__ F, const cfloat* __restrict__ A,
                               const cfloat* __restrict__ B, const unsigned char* done, int N,
                               int cx, int H, cfloat* S) {
-    const int j = blockIdx.y;
+    const int j = blockIdx.z;
     if (done && done[j]) return;
     const long long fs = (long long)cx * H;
-    const long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (q >= fs) return;
-    const int kx = (int)(q / H);
-    const int y = (int)(q - (long long)kx * H);
+    const int kx = blockIdx.y;
+    const int y = blockIdx.x * blockDim.x + threadIdx.x;
+    if (y >= H) return;
+    const long long q = (long long)kx * H + y;
     cfloat a0 = cmk(0.f, 0.f), a1 = cmk(0.f, 0.f);
     for (int i = 0; i < N; i++) {
         const long long b = (long long)j * N + i;
@@ -93,8 +93,9 @@ __global__ __launch_bounds__(MC_NTHR) void k_correlate(CorrArgs a) {
     const int gs = a.gstart[g], gn = a.gsize[g];
     const long long fs = (long long)a.cx * a.H;
     const cfloat* S = a.S + (long long)j * fs;
+    const float rH = 1.0f / a.H;
     for (int e = threadIdx.x; e < nc * a.H; e += MC_NTHR) {
-        const int c = e / a.H;
+        const int c = fdiv(e, a.H, rH);
         const int y = e - c * a.H;
         const int kx = c0 + c;
         const long long q = (long long)kx * a.H + y;
@@ -114,8 +115,9 @@ __global__ __launch_bounds__(MC_NTHR) void k_correlate(CorrArgs a) {
     __syncthreads();
     fft_lds_fwd(lds, nc, a.plan);
     cfloat* out = a.T1 + ((long long)j * a.G + g) * a.seay * a.cx;
+    const float rnc = 1.0f / nc;
     for (int e = threadIdx.x; e < a.seay * nc; e += MC_NTHR) {
-        const int jy = e / nc;
+        const int jy = fdiv(e, nc, rnc);
         const int c = e - jy * nc;
         int lag = jy - a.seay / 2;
         if (lag < 0) lag += a.H;
@@ -662,60 +664,121 @@ __global__ __launch_bounds__(256) void k_patch_stats(double* s, const double* ce
 
 // ---------------------------------------------------------------------------
 // mGCorrect3D (Correct/GCorrectPatchShift.cu:17-66, upsample 1) with the
-// frames inside: the patch weights exp(-100 r) (r <= 0.6) of a pixel are
-// formed once and reused for every frame; per frame the bad-patch mask is
-// applied, the weighted mean shift formed in the reference's FP32 order,
-// and the pixel gathered with (int) truncation and 811*|x| mod N reflection.
+// frames inside.  Per pixel the reference forms w_p = exp(-100 r_p) for every
+// patch with r_p <= 0.6 (r in units of the frame size), skips bad patches,
+// sums w, w*sx, w*sy in patch order in FP32, divides, and gathers the pixel at
+// (int)(x - sx), (int)(y - sy) with 811*|.| mod N reflection.  Here the
+// weights of a 256-pixel tile are formed once for all frames.  A patch whose
+// weight is below 1e-14 of the pixel's largest weight (r > r_min + 0.3224)
+// for every pixel of the tile is left out: adding it cannot change any of the
+// FP32 sums (it is far below half an ulp), so the kept patches, taken in the
+// reference's order, give the same sums.
 #define MC_WARP_MAXP 64
-#define MC_WARP_TX 64
+#define MC_WARP_TX 256
+#define MC_WARP_FCH 16
 __global__ __launch_bounds__(MC_WARP_TX) void k_local_warp(const float* __restrict__ R,
                                                            float* __restrict__ Rp, long long pitch,
                                                            long long fstride, int W, int H, int N,
                                                            int P, const float* cen, const float* sh,
                                                            const unsigned char* bad) {
-    __shared__ float wts[MC_WARP_MAXP * MC_WARP_TX];
+    extern __shared__ float wk[];  // [kept patch][thread]
+    __shared__ unsigned int kmask[2];
+    __shared__ int klist[MC_WARP_MAXP];
+    __shared__ int nk_s;
+    __shared__ float2 shk[MC_WARP_FCH][MC_WARP_MAXP];
+    __shared__ unsigned long long bm[MC_WARP_FCH];
     const int x = blockIdx.x * MC_WARP_TX + threadIdx.x;
     const int y = blockIdx.y;
     const bool live = x < W;
-    for (int p = 0; p < P; p++) {
+    if (threadIdx.x == 0) kmask[0] = kmask[1] = 0u;
+    __syncthreads();
+    float rmin = 1e30f;
+    for (int p = 0; p < P && live; p++) {
+        const float ax = ((float)x - cen[2 * p]) / (float)W;
+        const float ay = ((float)y - cen[2 * p + 1]) / (float)H;
+        rmin = fminf(rmin, sqrtf(ax * ax + ay * ay));
+    }
+    unsigned int lo = 0u, hi = 0u;
+    for (int p = 0; p < P && live; p++) {
+        const float ax = ((float)x - cen[2 * p]) / (float)W;
+        const float ay = ((float)y - cen[2 * p + 1]) / (float)H;
+        const float r = sqrtf(ax * ax + ay * ay);
+        if (r <= 0.6f && r <= rmin + 0.3224f) {
+            if (p < 32) lo |= 1u << p;
+            else hi |= 1u << (p - 32);
+        }
+    }
+    if (lo) atomicOr(&kmask[0], lo);
+    if (hi) atomicOr(&kmask[1], hi);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int n = 0;
+        for (int p = 0; p < P; p++) {
+            const unsigned int wd = kmask[p >> 5];
+            if ((wd >> (p & 31)) & 1u) klist[n++] = p;
+        }
+        nk_s = n;
+    }
+    __syncthreads();
+    const int nk = nk_s;
+    for (int q = 0; q < nk; q++) {
         float w = 0.f;
         if (live) {
+            const int p = klist[q];
             const float ax = ((float)x - cen[2 * p]) / (float)W;
             const float ay = ((float)y - cen[2 * p + 1]) / (float)H;
             const float r = sqrtf(ax * ax + ay * ay);
             w = (r > 0.6f) ? 0.f : expf(-100.0f * r);
         }
-        wts[p * MC_WARP_TX + threadIdx.x] = w;
+        wk[q * MC_WARP_TX + threadIdx.x] = w;
     }
-    if (!live) return;
-    for (int i = 0; i < N; i++) {
-        const float* s = sh + (long long)i * P * 2;
-        const unsigned char* bd = bad + (long long)i * P;
-        float fw = 0.f, fsx = 0.f, fsy = 0.f;
-        for (int p = 0; p < P; p++) {
-            if (bd[p]) continue;
-            const float w = wts[p * MC_WARP_TX + threadIdx.x];
-            fw += w;
-            fsx += s[2 * p] * w;
-            fsy += s[2 * p + 1] * w;
+    for (int f0 = 0; f0 < N; f0 += MC_WARP_FCH) {
+        const int nf = min(MC_WARP_FCH, N - f0);
+        __syncthreads();
+        for (int e = threadIdx.x; e < nf * nk; e += MC_WARP_TX) {
+            const int fi = e / nk;
+            const int q = e - fi * nk;
+            const float* s = sh + ((long long)(f0 + fi) * P + klist[q]) * 2;
+            shk[fi][q] = make_float2(s[0], s[1]);
         }
-        if (fw > 0) {
-            fsx = fsx / fw;
-            fsy = fsy / fw;
+        for (int fi = threadIdx.x; fi < nf; fi += MC_WARP_TX) {
+            unsigned long long m = 0ull;
+            const unsigned char* bd = bad + (long long)(f0 + fi) * P;
+            for (int p = 0; p < P; p++)
+                if (bd[p]) m |= 1ull << p;
+            bm[fi] = m;
         }
-        // keep the gather index finite for non-finite or absurd shifts
-        if (!(fabsf(fsx) < 1e6f)) fsx = 0.f;
-        if (!(fabsf(fsy) < 1e6f)) fsy = 0.f;
-        int xs = (int)((float)x - fsx);
-        int ys = (int)((float)y - fsy);
-        if (xs < 0 || ys < 0 || xs >= W || ys >= H) {
-            xs = xs < 0 ? -xs : xs;
-            ys = ys < 0 ? -ys : ys;
-            xs = (int)((811LL * xs) % W);
-            ys = (int)((811LL * ys) % H);
+        __syncthreads();
+        if (!live) continue;
+        for (int fi = 0; fi < nf; fi++) {
+            const unsigned long long m = bm[fi];
+            float fw = 0.f, fsx = 0.f, fsy = 0.f;
+            for (int q = 0; q < nk; q++) {
+                if ((m >> klist[q]) & 1ull) continue;
+                const float w = wk[q * MC_WARP_TX + threadIdx.x];
+                const float2 s = shk[fi][q];
+                fw += w;
+                fsx += s.x * w;
+                fsy += s.y * w;
+            }
+            if (fw > 0) {
+                fsx = fsx / fw;
+                fsy = fsy / fw;
+            }
+            // keep the gather index finite for non-finite or absurd shifts
+            if (!(fabsf(fsx) < 1e6f)) fsx = 0.f;
+            if (!(fabsf(fsy) < 1e6f)) fsy = 0.f;
+            int xs = (int)((float)x - fsx);
+            int ys = (int)((float)y - fsy);
+            if (xs < 0 || ys < 0 || xs >= W || ys >= H) {
+                xs = xs < 0 ? -xs : xs;
+                ys = ys < 0 ? -ys : ys;
+                xs = (int)((811LL * xs) % W);
+                ys = (int)((811LL * ys) % H);
+            }
+            const long long fo = (long long)(f0 + fi) * fstride;
+            Rp[fo + (long long)y * pitch + x] = R[fo + (long long)ys * pitch + xs];
         }
-        Rp[(long long)i * fstride + (long long)y * pitch + x] =
-            R[(long long)i * fstride + (long long)ys * pitch + xs];
     }
 }
 
@@ -755,11 +818,10 @@ __device__ __forceinline__ double crit_q(int kx, int y, int cx, int H, double pi
 }
 
 __global__ __launch_bounds__(256) void k_dw_sums(DwArgs a) {
-    const long long fs = (long long)a.cx * a.H;
-    const long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (q >= fs) return;
-    const int kx = (int)(q / a.H);
-    const int y = (int)(q - (long long)kx * a.H);
+    const int kx = blockIdx.y;
+    const int y = blockIdx.x * blockDim.x + threadIdx.x;
+    if (y >= a.H) return;
+    const long long q = (long long)kx * a.H + y;
     const bool dc = (kx == 0 && y == 0);
     double qd = 1.0, inv_ws = 1.0;
     if (a.dw && !dc) {
@@ -883,6 +945,8 @@ int mc_align_init_constants() {
     hipFuncSetAttribute((const void*)k_row_dft, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
     hipFuncSetAttribute((const void*)k_feature_map, hipFuncAttributeMaxDynamicSharedMemorySize,
                         2 * 128 * 128 * (int)sizeof(float));
+    hipFuncSetAttribute((const void*)k_local_warp, hipFuncAttributeMaxDynamicSharedMemorySize,
+                        MC_WARP_MAXP * MC_WARP_TX * (int)sizeof(float));
     return 0;
 }
 
@@ -898,7 +962,7 @@ hipError_t mc_launch_phasors(const double* shifts, double mult, int skip, const 
 hipError_t mc_launch_aligned_sum(const cfloat* F, const cfloat* A, const cfloat* B,
                                  const unsigned char* done, int J, int N, int cx, int H,
                                  cfloat* S, hipStream_t s) {
-    dim3 grid(mc_div_up((long long)cx * H, 256), J);
+    dim3 grid(mc_div_up(H, 256), cx, J);
     hipLaunchKernelGGL(k_aligned_sum, grid, dim3(256), 0, s, F, A, B, done, N, cx, H, S);
     return hipGetLastError();
 }
@@ -1010,7 +1074,8 @@ hipError_t mc_launch_local_warp(const float* R, float* Rp, long long pitch, long
                                 const unsigned char* bad, hipStream_t s) {
     if (P > MC_WARP_MAXP) return hipErrorInvalidValue;
     dim3 grid(mc_div_up(W, MC_WARP_TX), H);
-    hipLaunchKernelGGL(k_local_warp, grid, dim3(MC_WARP_TX), 0, s, R, Rp, pitch, fstride, W, H, N,
+    const size_t lds = (size_t)P * MC_WARP_TX * sizeof(float);
+    hipLaunchKernelGGL(k_local_warp, grid, dim3(MC_WARP_TX), lds, s, R, Rp, pitch, fstride, W, H, N,
                        P, cen, sh, bad);
     return hipGetLastError();
 }
@@ -1037,7 +1102,7 @@ hipError_t mc_launch_dw_sums(const cfloat* F, long long fstride, int N, int cx, 
     a.split = split;
     a.sums = sums;
     a.sstride = sstride;
-    dim3 grid(mc_div_up((long long)cx * H, 256));
+    dim3 grid(mc_div_up(H, 256), cx);
     hipLaunchKernelGGL(k_dw_sums, grid, dim3(256), 0, s, a);
     return hipGetLastError();
 }

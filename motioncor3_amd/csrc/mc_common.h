@@ -30,7 +30,22 @@ struct FftStage {
     int ns;
     int tw_off;
     int root_off;
+    int m;         // butterflies per transform (n / r)
+    int nch;       // output chunks per butterfly (large primes)
+    float rcp_m;   // 1/m, 1/ns, 1/nch for reciprocal division
+    float rcp_ns;
+    float rcp_nch;
 };
+
+// q = a / d, r = a % d for 0 <= a < 2^22 via a float reciprocal and one
+// correction step (integer division is a long instruction sequence).
+__device__ __forceinline__ int fdiv(int a, int d, float rcp) {
+    int q = (int)((float)a * rcp);
+    const int r = a - q * d;
+    q += (r >= d) ? 1 : 0;
+    q -= (r < 0) ? 1 : 0;
+    return q;
+}
 
 // Plan of one complex FFT length, passed to kernels by value.
 struct FftPlan1 {

@@ -86,11 +86,23 @@ __global__ __launch_bounds__(MC_NTHR) void k_rows_r2c(RowR2CArgs a) {
         y0 = a.win[2 * wi + 1];
     }
     const float* base = a.in + (long long)img * a.in_bstride;
-    for (int e = threadIdx.x; e < nrows * a.m; e += MC_NTHR) {
-        const int r = e / a.m;
-        const int j = e - r * a.m;
-        const float* src = base + (long long)(y0 + row0 + r) * a.pitch + x0 + 2 * j;
-        lds[e] = cmk(src[0], src[1]);
+    const float rm = 1.0f / a.m;
+    if (((x0 | (int)(a.pitch & 1) | (int)(a.in_bstride & 1)) & 1) == 0 &&
+        (((uintptr_t)a.in) & 7) == 0) {
+        for (int e = threadIdx.x; e < nrows * a.m; e += MC_NTHR) {
+            const int r = fdiv(e, a.m, rm);
+            const int j = e - r * a.m;
+            const float2 v = *reinterpret_cast<const float2*>(
+                base + (long long)(y0 + row0 + r) * a.pitch + x0 + 2 * j);
+            lds[e] = cmk(v.x, v.y);
+        }
+    } else {
+        for (int e = threadIdx.x; e < nrows * a.m; e += MC_NTHR) {
+            const int r = fdiv(e, a.m, rm);
+            const int j = e - r * a.m;
+            const float* src = base + (long long)(y0 + row0 + r) * a.pitch + x0 + 2 * j;
+            lds[e] = cmk(src[0], src[1]);
+        }
     }
     __syncthreads();
     fft_lds_fwd(lds, nrows, a.plan);
@@ -98,10 +110,13 @@ __global__ __launch_bounds__(MC_NTHR) void k_rows_r2c(RowR2CArgs a) {
     const int nb = (kc + a.C - 1) / a.C;
     cfloat* tb = a.T + (long long)b * a.t_bstride;
     const int tot = nb * nrows * a.C;
+    const int lc = __ffs(a.C) - 1;
+    const float rnr = 1.0f / nrows;
     for (int e = threadIdx.x; e < tot; e += MC_NTHR) {
-        const int c = e % a.C;
-        const int r = (e / a.C) % nrows;
-        const int kb = e / (a.C * nrows);
+        const int c = e & (a.C - 1);
+        const int rc = e >> lc;
+        const int kb = fdiv(rc, nrows, rnr);
+        const int r = rc - kb * nrows;
         const int k = kb * a.C + c;
         if (k > a.m) continue;
         const cfloat zk = lds[r * a.m + (k == a.m ? 0 : k)];
@@ -124,10 +139,13 @@ __global__ __launch_bounds__(MC_NTHR) void k_rows_c2r(RowC2RArgs a) {
     const int nb = (kc + a.C - 1) / a.C;
     const cfloat* tb = a.T + (long long)b * a.t_bstride;
     const int tot = nb * nrows * a.C;
+    const int lc = __ffs(a.C) - 1;
+    const float rnr = 1.0f / nrows;
     for (int e = threadIdx.x; e < tot; e += MC_NTHR) {
-        const int c = e % a.C;
-        const int r = (e / a.C) % nrows;
-        const int kb = e / (a.C * nrows);
+        const int c = e & (a.C - 1);
+        const int rc = e >> lc;
+        const int kb = fdiv(rc, nrows, rnr);
+        const int r = rc - kb * nrows;
         const int k = kb * a.C + c;
         if (k > a.m) continue;
         const cfloat v = tb[(long long)kb * a.H * a.C + (long long)(row0 + r) * a.C + c];
@@ -138,8 +156,9 @@ __global__ __launch_bounds__(MC_NTHR) void k_rows_c2r(RowC2RArgs a) {
     // Hermitian pre-processing into Z'[k] = (X_k + X*_{m-k}) +
     // i e^{+2 pi i k/W} (X_k - X*_{m-k}); imag parts of X_0, X_m dropped.
     const int half = a.m / 2 + 1;
+    const float rhalf = 1.0f / half;
     for (int e = threadIdx.x; e < nrows * half; e += MC_NTHR) {
-        const int r = e / half;
+        const int r = fdiv(e, half, rhalf);
         const int k = e - r * half;
         cfloat* row = lds + r * a.m;
         if (k == 0) {
@@ -167,8 +186,10 @@ __global__ __launch_bounds__(MC_NTHR) void k_rows_c2r(RowC2RArgs a) {
     __syncthreads();
     fft_lds_fwd(lds, nrows, a.plan);
     float* ob = a.out + (long long)b * a.out_bstride;
+    const float rm = 1.0f / a.m;
+    const bool vec = ((a.pitch | a.out_bstride) & 1) == 0 && (((uintptr_t)a.out) & 7) == 0;
     for (int e = threadIdx.x; e < nrows * a.m; e += MC_NTHR) {
-        const int r = e / a.m;
+        const int r = fdiv(e, a.m, rm);
         const int j = e - r * a.m;
         const cfloat z = lds[e];
         float v0 = z.x, v1 = -z.y;
@@ -177,8 +198,12 @@ __global__ __launch_bounds__(MC_NTHR) void k_rows_c2r(RowC2RArgs a) {
             v1 = fmaxf(v1, 0.f);
         }
         float* dst = ob + (long long)(row0 + r) * a.pitch + 2 * j;
-        dst[0] = v0;
-        dst[1] = v1;
+        if (vec) {
+            *reinterpret_cast<float2*>(dst) = make_float2(v0, v1);
+        } else {
+            dst[0] = v0;
+            dst[1] = v1;
+        }
     }
 }
 
@@ -189,9 +214,10 @@ __global__ __launch_bounds__(MC_NTHR) void k_cols_fwd(ColFwdArgs a) {
     const int c0 = kb * a.C;
     const int nc = min(a.C, a.ncols - c0);
     const cfloat* src = a.T + (long long)b * a.t_bstride + (long long)kb * a.H * a.C;
+    const int lc = __ffs(a.C) - 1;
     for (int e = threadIdx.x; e < a.H * a.C; e += MC_NTHR) {
-        const int y = e / a.C;
-        const int c = e - y * a.C;
+        const int y = e >> lc;
+        const int c = e & (a.C - 1);
         if (c < nc) lds[c * a.H + y] = src[e];
     }
     __syncthreads();
@@ -202,8 +228,9 @@ __global__ __launch_bounds__(MC_NTHR) void k_cols_fwd(ColFwdArgs a) {
     }
     if (a.X) {
         cfloat* xd = a.X + (long long)b * a.x_bstride;
+        const float rx = 1.0f / a.xcy;
         for (int e = threadIdx.x; e < nc * a.xcy; e += MC_NTHR) {
-            const int c = e / a.xcy;
+            const int c = fdiv(e, a.xcy, rx);
             const int yo = e - c * a.xcy;
             const int kx = c0 + c;
             if (kx >= a.xcx) continue;
@@ -222,8 +249,9 @@ __global__ __launch_bounds__(MC_NTHR) void k_cols_inv(ColInvArgs a) {
     const int nc = min(a.C, a.ncols - c0);
     const bool same = (a.src_cx == a.ncols && a.src_cy == a.H);
     const cfloat* sb = a.S + (long long)b * a.s_bstride;
+    const float rH = 1.0f / a.H;
     for (int e = threadIdx.x; e < nc * a.H; e += MC_NTHR) {
-        const int c = e / a.H;
+        const int c = fdiv(e, a.H, rH);
         const int y = e - c * a.H;
         const int kx = c0 + c;
         cfloat v;
@@ -239,9 +267,10 @@ __global__ __launch_bounds__(MC_NTHR) void k_cols_inv(ColInvArgs a) {
     __syncthreads();
     fft_lds_fwd(lds, nc, a.plan);
     cfloat* dst = a.T + (long long)b * a.t_bstride + (long long)kb * a.H * a.C;
+    const int lc = __ffs(a.C) - 1;
     for (int e = threadIdx.x; e < a.H * a.C; e += MC_NTHR) {
-        const int y = e / a.C;
-        const int c = e - y * a.C;
+        const int y = e >> lc;
+        const int c = e & (a.C - 1);
         if (c < nc) dst[e] = cconj(lds[c * a.H + y]);
     }
 }
@@ -273,6 +302,11 @@ int build_plan1(FftPlan1* P, cfloat** dev_tw, int n) {
         FftStage& st = P->st[s];
         st.r = r;
         st.ns = ns;
+        st.m = n / r;
+        st.nch = (r + MC_BIG_KC - 1) / MC_BIG_KC;
+        st.rcp_m = 1.0f / st.m;
+        st.rcp_ns = 1.0f / ns;
+        st.rcp_nch = 1.0f / st.nch;
         st.tw_off = (int)h.size();
         for (int k = 0; k < ns; k++)
             for (int t = 1; t < r; t++) {

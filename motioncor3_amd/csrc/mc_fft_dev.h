@@ -128,7 +128,7 @@ template <int R>
 static __device__ __noinline__ void fft_stage(cfloat* lds, int n, int batch, const FftStage st,
                                  const cfloat* __restrict__ tw) {
     constexpr int MAXIT = (MC_LDS_ELEMS + R * MC_NTHR - 1) / (R * MC_NTHR);
-    const int m = n / R;
+    const int m = st.m;
     const int total = batch * m;
     const int ns = st.ns;
     const cfloat* twp = tw + st.tw_off;
@@ -139,9 +139,10 @@ static __device__ __noinline__ void fft_stage(cfloat* lds, int n, int batch, con
         const int j = threadIdx.x + it * MC_NTHR;
         dst[it] = -1;
         if (j < total) {
-            const int b = j / m;
+            const int b = fdiv(j, m, st.rcp_m);
             const int jj = j - b * m;
-            const int k = jj % ns;
+            const int q = fdiv(jj, ns, st.rcp_ns);
+            const int k = jj - q * ns;
             const cfloat* src = lds + b * n + jj;
 #pragma unroll
             for (int t = 0; t < R; t++) v[it][t] = src[t * m];
@@ -151,7 +152,7 @@ static __device__ __noinline__ void fft_stage(cfloat* lds, int n, int batch, con
                 for (int t = 1; t < R; t++) v[it][t] = cmul(v[it][t], w[t - 1]);
             }
             bfly<R>(v[it]);
-            dst[it] = b * n + (jj / ns) * ns * R + k;
+            dst[it] = b * n + q * ns * R + k;
         }
     }
     __syncthreads();
@@ -172,8 +173,8 @@ static __device__ __noinline__ void fft_stage(cfloat* lds, int n, int batch, con
 static __device__ __noinline__ void fft_stage_big(cfloat* lds, int n, int batch, const FftStage st,
                                      const cfloat* __restrict__ tw) {
     const int p = st.r;
-    const int m = n / p;
-    const int nch = (p + MC_BIG_KC - 1) / MC_BIG_KC;
+    const int m = st.m;
+    const int nch = st.nch;
     const int total = batch * m * nch;
     const int ns = st.ns;
     const cfloat* twp = tw + st.tw_off;
@@ -188,11 +189,12 @@ static __device__ __noinline__ void fft_stage_big(cfloat* lds, int n, int batch,
 #pragma unroll
         for (int q = 0; q < MC_BIG_KC; q++) acc[it][q] = cmk(0.f, 0.f);
         if (item < total) {
-            const int bf = item / nch;
+            const int bf = fdiv(item, nch, st.rcp_nch);
             const int ch = item - bf * nch;
-            const int b = bf / m;
+            const int b = fdiv(bf, m, st.rcp_m);
             const int jj = bf - b * m;
-            const int k = jj % ns;
+            const int qd = fdiv(jj, ns, st.rcp_ns);
+            const int k = jj - qd * ns;
             const int q0 = ch * MC_BIG_KC;
             const cfloat* src = lds + b * n + jj;
             int ridx[MC_BIG_KC], step[MC_BIG_KC];
@@ -213,7 +215,7 @@ static __device__ __noinline__ void fft_stage_big(cfloat* lds, int n, int batch,
                     ridx[q] = nx >= p ? nx - p : nx;
                 }
             }
-            dst[it] = b * n + (jj / ns) * ns * p + k;
+            dst[it] = b * n + qd * ns * p + k;
             q0s[it] = q0;
         }
     }

@@ -819,6 +819,31 @@ int mc_sync(mc_ctx* c) {
     return MC_OK;
 }
 
+int mc_malloc(mc_ctx* c, unsigned long long bytes, void** out) {
+    if (!c || !out) return MC_E_ARG;
+    *out = nullptr;
+    hipSetDevice(c->device);
+    if (hipMalloc(out, bytes ? bytes : 1) != hipSuccess) return fail(c, MC_E_NOMEM, "hipMalloc");
+    return MC_OK;
+}
+
+int mc_free(mc_ctx* c, void* ptr) {
+    if (!c) return MC_E_ARG;
+    if (!ptr) return MC_OK;
+    hipStreamSynchronize(c->stream);
+    MC_TRY(hipFree(ptr));
+    return MC_OK;
+}
+
+int mc_memcpy(mc_ctx* c, void* dst, const void* src, unsigned long long bytes, int dir) {
+    if (!c || (!dst && bytes) || (!src && bytes) || dir < 0 || dir > 2) return MC_E_ARG;
+    const hipMemcpyKind k = dir == 0 ? hipMemcpyHostToDevice
+                                     : (dir == 1 ? hipMemcpyDeviceToHost : hipMemcpyDeviceToDevice);
+    MC_TRY(hipMemcpyAsync(dst, src, bytes, k, c->stream));
+    MC_TRY(hipStreamSynchronize(c->stream));
+    return MC_OK;
+}
+
 int mc_geometry(int W, int H, int px, int py, int* sizes, float* xbin) {
     if (W < 8 || H < 8 || (W % 2)) return MC_E_ARG;
     const Geo g = make_geo(W, H, 2, px, py, 1.0f);

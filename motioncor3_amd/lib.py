@@ -1,7 +1,11 @@
 """ctypes binding of libmc3.so (the C ABI in include/mc_abi.h).
 
-Device buffers are torch CUDA tensors (plumbing only); every call goes to the
-HIP library.  A missing library raises: there is no CPU fallback.
+Device buffers are DevArray objects allocated through the library's own
+mc_malloc/mc_memcpy, with numpy arrays on the host side.  (torch's wheel
+bundles a different HIP runtime; loading both in one process leaves the
+second without a device, so torch is not used for device memory here.)
+Every call goes to the HIP library; a missing library raises and there is
+no CPU fallback.
 """
 import ctypes as C
 import os
@@ -55,7 +59,7 @@ class McOverride(C.Structure):
 # Every symbol include/mc_abi.h declares.
 ABI_SYMBOLS = (
     "mc_ctx_create", "mc_ctx_destroy", "mc_last_error", "mc_sync", "mc_default_params",
-    "mc_output_geometry", "mc_geometry", "mc_align_movie", "mc_rfft2", "mc_irfft2",
+    "mc_output_geometry", "mc_geometry", "mc_malloc", "mc_free", "mc_memcpy", "mc_align_movie", "mc_rfft2", "mc_irfft2",
     "mc_phase_shift_accumulate", "mc_group_correlate_window", "mc_peak_fit",
     "mc_spline_despike_fp64", "mc_patch_stats_fp64", "mc_local_warp", "mc_dose_weight_sum",
     "mc_set_kernel_timing", "mc_get_kernel_timing", "mc_synth_render",
@@ -105,6 +109,53 @@ def _fptr(a):
     return a.ctypes.data_as(C.POINTER(C.c_float))
 
 
+class DevArray:
+    """A C-contiguous device buffer owned by a Context."""
+
+    def __init__(self, ctx, shape, dtype):
+        self.ctx = ctx
+        self.shape = tuple(int(v) for v in shape)
+        self.dtype = np.dtype(dtype)
+        self.size = int(np.prod(self.shape)) if self.shape else 1
+        self.nbytes = self.size * self.dtype.itemsize
+        p = C.c_void_p()
+        ctx._check(ctx.lib.mc_malloc(ctx.ptr, C.c_ulonglong(self.nbytes), C.byref(p)), "mc_malloc")
+        self.ptr = p.value
+
+    def stride(self, axis):
+        """Stride of `axis` in elements."""
+        return int(np.prod(self.shape[axis + 1:])) if axis + 1 < len(self.shape) else 1
+
+    def vp(self, offset_elems=0):
+        return C.c_void_p(self.ptr + offset_elems * self.dtype.itemsize)
+
+    def numpy(self):
+        out = np.empty(self.shape, self.dtype)
+        self.ctx._check(self.ctx.lib.mc_memcpy(self.ctx.ptr, out.ctypes.data_as(C.c_void_p),
+                                               C.c_void_p(self.ptr), C.c_ulonglong(self.nbytes), 1),
+                        "mc_memcpy")
+        return out
+
+    def upload(self, arr):
+        a = np.ascontiguousarray(arr, dtype=self.dtype)
+        assert a.nbytes == self.nbytes
+        self.ctx._check(self.ctx.lib.mc_memcpy(self.ctx.ptr, C.c_void_p(self.ptr),
+                                               a.ctypes.data_as(C.c_void_p),
+                                               C.c_ulonglong(self.nbytes), 0), "mc_memcpy")
+        return self
+
+    def free(self):
+        if self.ptr and self.ctx.ptr:
+            self.ctx.lib.mc_free(self.ctx.ptr, C.c_void_p(self.ptr))
+        self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
 class Context:
     """One mc_ctx (stream, FFT plans, device workspace) on one GPU."""
 
@@ -135,62 +186,55 @@ class Context:
     def sync(self):
         self._check(self.lib.mc_sync(self.ptr), "mc_sync")
 
-    # ---- stage entry points (torch tensors in/out) ----
+    def empty(self, shape, dtype=np.float32):
+        return DevArray(self, shape, dtype)
+
+    def upload(self, arr, dtype=None):
+        arr = np.asarray(arr)
+        return DevArray(self, arr.shape, dtype or arr.dtype).upload(arr)
+
+    # ---- stage entry points ----
     def rfft2(self, frames, scale=1.0):
-        """frames: cuda float32 [B, H, W] -> complex64 [B, W/2+1, H] ([kx][ky])."""
-        import torch
+        """frames: DevArray float32 [B, H, W] -> complex64 [B, W/2+1, H] ([kx][ky])."""
         B, H, W = frames.shape
-        out = torch.empty((B, W // 2 + 1, H), dtype=torch.complex64, device=frames.device)
-        torch.cuda.synchronize()
-        self._check(self.lib.mc_rfft2(self.ptr, C.c_void_p(frames.data_ptr()), W, H,
-                                      C.c_longlong(frames.stride(1)),
+        out = self.empty((B, W // 2 + 1, H), np.complex64)
+        self._check(self.lib.mc_rfft2(self.ptr, frames.vp(), W, H, C.c_longlong(frames.stride(1)),
                                       C.c_longlong(frames.stride(0)), B, C.c_float(scale),
-                                      C.c_void_p(out.data_ptr())), "mc_rfft2")
+                                      out.vp()), "mc_rfft2")
         self.sync()
         return out
 
     def irfft2(self, spec, W, clamp=False):
         """spec: complex64 [B, W/2+1, H] -> float32 [B, H, W]."""
-        import torch
         B, cx, H = spec.shape
-        out = torch.empty((B, H, W), dtype=torch.float32, device=spec.device)
-        torch.cuda.synchronize()
-        self._check(self.lib.mc_irfft2(self.ptr, C.c_void_p(spec.data_ptr()), W, H, B,
-                                       C.c_void_p(out.data_ptr()), C.c_longlong(W),
+        out = self.empty((B, H, W), np.float32)
+        self._check(self.lib.mc_irfft2(self.ptr, spec.vp(), W, H, B, out.vp(), C.c_longlong(W),
                                        C.c_longlong(W * H), int(clamp)), "mc_irfft2")
         self.sync()
         return out
 
     def phase_shift_accumulate(self, spec, shifts, W):
-        import torch
         N, cx, H = spec.shape
-        out = torch.empty((cx, H), dtype=torch.complex64, device=spec.device)
+        out = self.empty((cx, H), np.complex64)
         sh = np.ascontiguousarray(shifts, dtype=np.float32)
-        torch.cuda.synchronize()
-        self._check(self.lib.mc_phase_shift_accumulate(
-            self.ptr, C.c_void_p(spec.data_ptr()), W, H, N, _fptr(sh),
-            C.c_void_p(out.data_ptr())), "mc_phase_shift_accumulate")
+        self._check(self.lib.mc_phase_shift_accumulate(self.ptr, spec.vp(), W, H, N, _fptr(sh),
+                                                       out.vp()), "mc_phase_shift_accumulate")
         self.sync()
         return out
 
     def correlate_window(self, sum_spec, grp_spec, W, bft, phase_only=False, sea=(64, 64)):
-        import torch
         H = sum_spec.shape[-1]
         win = np.zeros((sea[1], sea[0]), np.float32)
-        torch.cuda.synchronize()
         self._check(self.lib.mc_group_correlate_window(
-            self.ptr, C.c_void_p(sum_spec.data_ptr()), C.c_void_p(grp_spec.data_ptr()), W, H,
-            C.c_float(bft), int(phase_only), sea[0], sea[1], _fptr(win)),
-            "mc_group_correlate_window")
+            self.ptr, sum_spec.vp(), grp_spec.vp(), W, H, C.c_float(bft), int(phase_only),
+            sea[0], sea[1], _fptr(win)), "mc_group_correlate_window")
         return win
 
     def peak_fit(self, windows):
-        import torch
         cnt, ny, nx = windows.shape
         out = np.zeros((cnt, 2), np.float32)
-        torch.cuda.synchronize()
-        self._check(self.lib.mc_peak_fit(self.ptr, C.c_void_p(windows.data_ptr()), nx, ny, cnt,
-                                         _fptr(out)), "mc_peak_fit")
+        self._check(self.lib.mc_peak_fit(self.ptr, windows.vp(), nx, ny, cnt, _fptr(out)),
+                    "mc_peak_fit")
         return out
 
     def despike(self, sx, sy, quirks=True):
@@ -213,34 +257,30 @@ class Context:
         return s, bad, ref.value
 
     def local_warp(self, frames, centers, shifts, bad=None):
-        import torch
         N, H, W = frames.shape
         P = len(centers)
-        out = torch.empty_like(frames)
+        out = self.empty(frames.shape, np.float32)
         cen = np.ascontiguousarray(centers, dtype=np.float32)
         sh = np.ascontiguousarray(shifts, dtype=np.float32)
         bp = None
         if bad is not None:
             bad = np.ascontiguousarray(bad, dtype=np.uint8)
             bp = bad.ctypes.data_as(C.POINTER(C.c_ubyte))
-        torch.cuda.synchronize()
-        self._check(self.lib.mc_local_warp(self.ptr, C.c_void_p(frames.data_ptr()),
-                                           C.c_void_p(out.data_ptr()), W, H, N, P, _fptr(cen),
-                                           _fptr(sh), bp), "mc_local_warp")
+        self._check(self.lib.mc_local_warp(self.ptr, frames.vp(), out.vp(), W, H, N, P,
+                                           _fptr(cen), _fptr(sh), bp), "mc_local_warp")
         self.sync()
         return out
 
     def dose_weight_sum(self, W, H, N, pix, kv, fm_dose):
-        import torch
-        out = torch.empty((W // 2 + 1, H), dtype=torch.float32, device="cuda")
+        out = self.empty((W // 2 + 1, H), np.float32)
         self._check(self.lib.mc_dose_weight_sum(self.ptr, W, H, N, C.c_float(pix), kv,
-                                                C.c_float(fm_dose), C.c_void_p(out.data_ptr())),
+                                                C.c_float(fm_dose), out.vp()),
                     "mc_dose_weight_sum")
         self.sync()
         return out
 
     def synth_render(self, frames, recipe, poisson=True, seed=None):
-        """Render a MovieRecipe into a cuda float32 [N, H, W] tensor."""
+        """Render a MovieRecipe into a DevArray float32 [N, H, W]."""
         N, H, W = frames.shape
         blobs = np.stack([recipe.bx, recipe.by, recipe.bs, recipe.ba], 1).astype(np.float32)
         disp = np.zeros((N, len(recipe.bx), 2), np.float32)
@@ -249,7 +289,7 @@ class Context:
             disp[t, :, 0] = dx
             disp[t, :, 1] = dy
         self._check(self.lib.mc_synth_render(
-            self.ptr, C.c_void_p(frames.data_ptr()), W, H, N, C.c_longlong(frames.stride(1)),
+            self.ptr, frames.vp(), W, H, N, C.c_longlong(frames.stride(1)),
             C.c_longlong(frames.stride(0)), len(blobs), _fptr(blobs), _fptr(disp),
             C.c_float(recipe.scale), C.c_ulonglong(recipe.seed if seed is None else seed),
             int(poisson)), "mc_synth_render")
@@ -275,20 +315,18 @@ class Context:
     def align_movie(self, frames, params, replay_global=None, replay_patch=None,
                     replay_centers=None, replay_bad=None, patch_starts=None, sums=None,
                     width=None):
-        """Align an [N, H, pitch] cuda float32 movie whose rows hold `width`
-        (default pitch) pixels.  Returns numpy shift tables and cuda sums."""
-        import torch
+        """Align an [N, H, pitch] float32 DevArray movie whose rows hold `width`
+        (default pitch) pixels.  Returns numpy shift tables and DevArray sums."""
         N, H, pitch = frames.shape
         W = pitch if width is None else width
         ns, ow, oh = self.output_geometry(params, W, H, N)
         P = params.patch[0] * params.patch[1] if (params.patch[0] > 1 and params.patch[1] > 1) else 0
         if sums is None:
-            sums = [torch.empty((oh, ow), dtype=torch.float32, device=frames.device)
-                    for _ in range(ns)]
-        mv = McMovie(C.c_void_p(frames.data_ptr()), W, H, N, frames.stride(1), frames.stride(0))
+            sums = [self.empty((oh, ow), np.float32) for _ in range(ns)]
+        mv = McMovie(frames.vp(), W, H, N, frames.stride(1), frames.stride(0))
         res = McResult()
         for i in range(ns):
-            res.sums[i] = C.c_void_p(sums[i].data_ptr())
+            res.sums[i] = sums[i].vp()
         out = dict(global_shift=np.zeros((N, 2), np.float32),
                    global_iters=np.zeros(1, np.int32),
                    global_errors=np.zeros(max(params.iter, 1), np.float32))
@@ -317,7 +355,6 @@ class Context:
 
         ov = McOverride(fp(replay_global), fp(replay_patch), fp(replay_centers),
                         fp(replay_bad, np.uint8, C.c_ubyte), fp(patch_starts, np.int32, C.c_int))
-        torch.cuda.synchronize()
         self._check(self.lib.mc_align_movie(self.ptr, C.byref(params), C.byref(mv), C.byref(ov),
                                             C.byref(res)), "mc_align_movie")
         out["global_iters"] = int(out["global_iters"][0])

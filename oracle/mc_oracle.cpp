@@ -951,7 +951,10 @@ void local_warp(const double* img, int W, int H, const double* cen, const double
                 sy += sh[2 * p + 1] * e;
             }
             if (w > 0) { sx /= w; sy /= w; }
-            int xs = (int)(x - sx), ys = (int)(y - sy);
+            /* The gather index is the FP32 difference of the reference kernel
+             * ((int)((blockIdx.x - fSx) * iUpSample), :52-53): its rounding
+             * (6e-5 px near x = 1000) decides which pixel is read. */
+            int xs = (int)((float)x - (float)sx), ys = (int)((float)y - (float)sy);
             if (xs < 0 || ys < 0 || xs >= W || ys >= H) {
                 xs = xs < 0 ? -xs : xs;
                 ys = ys < 0 ? -ys : ys;

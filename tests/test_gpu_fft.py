@@ -15,12 +15,11 @@ def _rel(a, b):
 
 @pytest.mark.parametrize("W,H", SIZES)
 def test_rfft2_matches_numpy(ctx, W, H):
-    import torch
     print("rfft2", W, H, flush=True)
     B = 2 if W * H < 5_000_000 else 1
     rng = np.random.default_rng(W + H)
     x = rng.standard_normal((B, H, W)).astype(np.float32)
-    spec = ctx.rfft2(torch.from_numpy(x).cuda()).cpu().numpy()
+    spec = ctx.rfft2(ctx.upload(x)).numpy()
     for b in range(B):
         ref = np.fft.rfft2(x[b].astype(np.float64))
         assert _rel(spec[b].T, ref) < 2e-6, (W, H)
@@ -28,27 +27,25 @@ def test_rfft2_matches_numpy(ctx, W, H):
 
 @pytest.mark.parametrize("W,H", SIZES)
 def test_irfft2_roundtrip(ctx, W, H):
-    import torch
     print("irfft2", W, H, flush=True)
     rng = np.random.default_rng(7 * W + H)
     x = rng.standard_normal((1, H, W)).astype(np.float32)
-    d = torch.from_numpy(x).cuda()
+    d = ctx.upload(x)
     spec = ctx.rfft2(d, scale=1.0 / (W * H))
-    back = ctx.irfft2(spec, W).cpu().numpy()
+    back = ctx.irfft2(spec, W).numpy()
     assert _rel(back, x) < 2e-6
     # C2R of a non-Hermitian half spectrum follows numpy's irfft2 convention
-    s = spec.cpu().numpy()[0].T.astype(np.complex128)
+    s = spec.numpy()[0].T.astype(np.complex128)
     s[0, 0] += 0.3j
-    spec2 = torch.from_numpy(np.ascontiguousarray(s.T[None]).astype(np.complex64)).cuda()
-    out = ctx.irfft2(spec2, W).cpu().numpy()[0]
+    spec2 = ctx.upload(np.ascontiguousarray(s.T[None]).astype(np.complex64))
+    out = ctx.irfft2(spec2, W).numpy()[0]
     ref = np.fft.irfft2(s, s=(H, W)) * (W * H)
     assert _rel(out, ref) < 2e-6
 
 
 def test_clamped_c2r(ctx):
-    import torch
     rng = np.random.default_rng(1)
     x = rng.standard_normal((1, 64, 96)).astype(np.float32)
-    spec = ctx.rfft2(torch.from_numpy(x).cuda(), scale=1.0 / (64 * 96))
-    out = ctx.irfft2(spec, 96, clamp=True).cpu().numpy()
+    spec = ctx.rfft2(ctx.upload(x), scale=1.0 / (64 * 96))
+    out = ctx.irfft2(spec, 96, clamp=True).numpy()
     assert np.allclose(out, np.maximum(x, 0), atol=1e-5)

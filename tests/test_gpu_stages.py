@@ -17,21 +17,19 @@ def _blobs(W, H, n, seed):
     return img
 
 
-def _to_dev(cmp_ref_layout):
-    """[H][cx] complex -> cuda complex64 [cx][H]."""
-    import torch
-    return torch.from_numpy(np.ascontiguousarray(cmp_ref_layout.T).astype(np.complex64)).cuda()
+def _col_major(cmp_ref_layout):
+    """[H][cx] complex -> complex64 [cx][H] (the library's spectrum layout)."""
+    return np.ascontiguousarray(cmp_ref_layout.T).astype(np.complex64)
 
 
 def test_phase_shift_accumulate(ctx):
-    import torch
     W, H, N = 96, 80, 5
     rng = np.random.default_rng(2)
     specs = [O.rfft2(rng.standard_normal((H, W))) / (W * H) for _ in range(N)]
     shifts = rng.uniform(-6, 6, (N, 2))
     ref = sum(O.phase_shift(specs[i], *shifts[i]) for i in range(N))
-    d = torch.stack([_to_dev(s) for s in specs])
-    out = ctx.phase_shift_accumulate(d, shifts, W).cpu().numpy().T
+    d = ctx.upload(np.stack([_col_major(s) for s in specs]))
+    out = ctx.phase_shift_accumulate(d, shifts, W).numpy().T
     assert np.linalg.norm(out - ref) / np.linalg.norm(ref) < 2e-6
 
 
@@ -42,19 +40,19 @@ def test_correlate_window(ctx, W, H, bft, po):
     g = np.roll(img, (3, -5), axis=(0, 1)) + 0.05 * np.random.default_rng(1).standard_normal((H, W))
     F, G = O.rfft2(img) / (W * H), O.rfft2(g) / (W * H)
     ref = O.correlate_window(F + G, G, bft, bool(po))
-    out = ctx.correlate_window(_to_dev(F + G), _to_dev(G), W, bft, bool(po))
+    out = ctx.correlate_window(ctx.upload(_col_major(F + G)), ctx.upload(_col_major(G)), W, bft,
+                              bool(po))
     scale = np.max(np.abs(ref))
     assert np.max(np.abs(out - ref)) < 2e-5 * scale
     assert np.allclose(O.peak2d(out), O.peak2d(ref), atol=1e-3)
 
 
 def test_peak_fit(ctx):
-    import torch
     rng = np.random.default_rng(4)
     wins = rng.standard_normal((40, 64, 64)).astype(np.float32)
     wins[3, 10, 20] = wins[3, 40, 50] = 9.0   # tie -> first
     wins[5, 0, 0] = 9.0                       # border clamp
-    out = ctx.peak_fit(torch.from_numpy(wins).cuda())
+    out = ctx.peak_fit(ctx.upload(wins))
     ref = np.array([O.peak2d(w.astype(np.float64)) for w in wins])
     assert np.allclose(out, ref, atol=1e-5)
 
@@ -90,15 +88,14 @@ def test_patch_stats_fp64(ctx, quirks):
 
 
 def test_local_warp(ctx):
-    import torch
     rng = np.random.default_rng(8)
     N, H, W, P = 3, 300, 420, 9
     frames = rng.standard_normal((N, H, W)).astype(np.float32)
     cen = np.array([[(i % 3 + 0.5) * W / 3, (i // 3 + 0.5) * H / 3] for i in range(P)])
     sh = rng.uniform(-4, 4, (N, P, 2))
     bad = (rng.uniform(0, 1, (N, P)) < 0.15).astype(np.uint8)
-    out = ctx.local_warp(torch.from_numpy(frames).cuda(), cen.astype(np.float32),
-                         sh.astype(np.float32), bad).cpu().numpy()
+    out = ctx.local_warp(ctx.upload(frames), cen.astype(np.float32),
+                         sh.astype(np.float32), bad).numpy()
     for i in range(N):
         ref = O.local_warp(frames[i].astype(np.float64), cen.astype(np.float32).astype(np.float64),
                            sh[i].astype(np.float32).astype(np.float64), bad[i])
@@ -108,6 +105,6 @@ def test_local_warp(ctx):
 
 def test_dose_weight_sum(ctx):
     W, H, N = 512, 384, 30
-    out = ctx.dose_weight_sum(W, H, N, 0.83, 300, 1.2).cpu().numpy().T
+    out = ctx.dose_weight_sum(W, H, N, 0.83, 300, 1.2).numpy().T
     ref = O.dose_weight_sum(W // 2 + 1, H, 0.83, 300, 1.2 * (np.arange(N) + 1))
     assert np.allclose(out, ref, rtol=2e-6, atol=1e-7)
