@@ -164,7 +164,9 @@ int mc_patch_stats_fp64(mc_ctx* ctx, double* shifts_host, const double* centers_
                         int P, int W, int H, int quirks, unsigned char* bad_host,
                         int* ref_host);
 /* mGCorrect3D (Correct/GCorrectPatchShift.cu:17-66) on N real frames
- * (rows of W floats, dense) with host tables. */
+ * (rows of W floats, dense) with host tables; in and out must not overlap
+ * (the reference gathers from the upsampled copy into a separate tmp frame,
+ * Correct/GCorrectPatchShift.cu:268-281). */
 int mc_local_warp(mc_ctx* ctx, const float* in, float* out, int W, int H, int N, int P,
                   const float* centers_host, const float* shifts_host,
                   const unsigned char* bad_host);

@@ -110,7 +110,7 @@ __global__ __launch_bounds__(MC_NTHR) void k_correlate(CorrArgs a) {
         float f = expf(a.filt0 * (float)(kx * kx + ys * ys));
         if (a.phase_only) f = f / (sqrtf(X.x * X.x + X.y * X.y) + 1e-20f);
         X = (kx == 0 && y == 0) ? cmk(0.f, 0.f) : cscale(X, f);
-        lds[e] = cconj(X);
+        lds[lp(e)] = cconj(X);
     }
     __syncthreads();
     fft_lds_fwd(lds, nc, a.plan);
@@ -121,7 +121,7 @@ __global__ __launch_bounds__(MC_NTHR) void k_correlate(CorrArgs a) {
         const int c = e - jy * nc;
         int lag = jy - a.seay / 2;
         if (lag < 0) lag += a.H;
-        out[(long long)jy * a.cx + c0 + c] = cconj(lds[c * a.H + lag]);
+        out[(long long)jy * a.cx + c0 + c] = cconj(lds[lp(c * a.H + lag)]);
     }
 }
 
@@ -668,40 +668,183 @@ __global__ __launch_bounds__(256) void k_patch_stats(double* s, const double* ce
 // patch with r_p <= 0.6 (r in units of the frame size), skips bad patches,
 // sums w, w*sx, w*sy in patch order in FP32, divides, and gathers the pixel at
 // (int)(x - sx), (int)(y - sy) with 811*|.| mod N reflection.  Here the
-// weights of a 256-pixel tile are formed once for all frames.  A patch whose
+// weights of a 64x4-pixel tile are formed once for all frames.  A patch whose
 // weight is below 1e-14 of the pixel's largest weight (r > r_min + 0.3224)
 // for every pixel of the tile is left out: adding it cannot change any of the
 // FP32 sums (it is far below half an ulp), so the kept patches, taken in the
-// reference's order, give the same sums.
+// reference's order, give the same sums.  The frame table holds
+// {ok*sx, ok*sy, ok} per kept patch (ok = 0 for a bad patch), so
+// fw = fma(w, ok, fw) and fsx = fma(w, ok*sx, fsx) reproduce the reference's
+// skip exactly; up to 16 kept patches the weights live in registers.
 #define MC_WARP_MAXP 64
-#define MC_WARP_TX 256
+#define MC_WARP_TX 64
+#define MC_WARP_TY 4
+#define MC_WARP_NT (MC_WARP_TX * MC_WARP_TY)
 #define MC_WARP_FCH 16
-__global__ __launch_bounds__(MC_WARP_TX) void k_local_warp(const float* __restrict__ R,
-                                                           float* __restrict__ Rp, long long pitch,
-                                                           long long fstride, int W, int H, int N,
-                                                           int P, const float* cen, const float* sh,
-                                                           const unsigned char* bad) {
-    extern __shared__ float wk[];  // [kept patch][thread]
+#define MC_WARP_KREG 16
+#define MC_WARP_U 8
+
+struct WarpArgs {
+    const float* R;
+    float* Rp;
+    long long pitch, fstride;
+    int W, H, N, P;
+    const float* cen;
+    const float* sh;
+    const unsigned char* bad;
+};
+
+// Source offset (within a frame) of output pixel (x, y) for the weighted
+// shift sums of one frame.
+__device__ __forceinline__ long long warp_src(int W, int H, long long pitch, int x, int y,
+                                              float fw, float fsx, float fsy) {
+    if (fw > 0) {
+        fsx = fsx / fw;
+        fsy = fsy / fw;
+    }
+    // keep the gather index finite for non-finite or absurd shifts
+    if (!(fabsf(fsx) < 1e6f)) fsx = 0.f;
+    if (!(fabsf(fsy) < 1e6f)) fsy = 0.f;
+    int xs = (int)((float)x - fsx);
+    int ys = (int)((float)y - fsy);
+    if (xs < 0 || ys < 0 || xs >= W || ys >= H) {
+        xs = xs < 0 ? -xs : xs;
+        ys = ys < 0 ? -ys : ys;
+        xs = (int)((811LL * xs) % W);
+        ys = (int)((811LL * ys) % H);
+    }
+    return (long long)ys * pitch + xs;
+}
+
+// Fill the table of frames f0..f0+nf-1 for the K (>= nk) kept patches.
+__device__ __forceinline__ void warp_table(const WarpArgs& a, const int* klist, int nk, int K,
+                                           int f0, int nf, float4* tab) {
+    for (int e = threadIdx.x; e < nf * K; e += MC_WARP_NT) {
+        const int fi = e / K, q = e - fi * K;
+        float4 t = make_float4(0.f, 0.f, 0.f, 0.f);
+        if (q < nk) {
+            const int p = klist[q];
+            const long long o = (long long)(f0 + fi) * a.P + p;
+            if (!a.bad[o]) {
+                t.x = a.sh[2 * o];
+                t.y = a.sh[2 * o + 1];
+                t.z = 1.f;
+            }
+        }
+        tab[e] = t;
+    }
+}
+
+template <int K>
+__device__ __forceinline__ void warp_tile_reg(const WarpArgs& a, const int* klist, int nk,
+                                              float4* tab, int x, int y, bool live) {
+    float w[K];
+#pragma unroll
+    for (int q = 0; q < K; q++) {
+        w[q] = 0.f;
+        if (live && q < nk) {
+            const int p = klist[q];
+            const float ax = ((float)x - a.cen[2 * p]) / (float)a.W;
+            const float ay = ((float)y - a.cen[2 * p + 1]) / (float)a.H;
+            const float r = sqrtf(ax * ax + ay * ay);
+            w[q] = (r > 0.6f) ? 0.f : expf(-100.0f * r);
+        }
+    }
+    for (int f0 = 0; f0 < a.N; f0 += MC_WARP_FCH) {
+        const int nf = min(MC_WARP_FCH, a.N - f0);
+        __syncthreads();
+        warp_table(a, klist, nk, K, f0, nf, tab);
+        __syncthreads();
+        if (!live) continue;
+        // gathers of MC_WARP_U frames are issued before their stores so that
+        // each wave keeps several loads in flight
+        const float* __restrict__ R = a.R + (long long)f0 * a.fstride;
+        float* __restrict__ Rp = a.Rp + (long long)f0 * a.fstride + (long long)y * a.pitch + x;
+        for (int fb = 0; fb < nf; fb += MC_WARP_U) {
+            long long so[MC_WARP_U];
+#pragma unroll
+            for (int u = 0; u < MC_WARP_U; u++) {
+                const float4* t = tab + min(fb + u, nf - 1) * K;
+                float fw = 0.f, fsx = 0.f, fsy = 0.f;
+#pragma unroll
+                for (int q = 0; q < K; q++) {
+                    const float4 v = t[q];
+                    fw = fmaf(w[q], v.z, fw);
+                    fsx = fmaf(v.x, w[q], fsx);
+                    fsy = fmaf(v.y, w[q], fsy);
+                }
+                so[u] = warp_src(a.W, a.H, a.pitch, x, y, fw, fsx, fsy);
+            }
+            float v[MC_WARP_U];
+#pragma unroll
+            for (int u = 0; u < MC_WARP_U; u++)
+                if (fb + u < nf) v[u] = R[(long long)(fb + u) * a.fstride + so[u]];
+#pragma unroll
+            for (int u = 0; u < MC_WARP_U; u++)
+                if (fb + u < nf) Rp[(long long)(fb + u) * a.fstride] = v[u];
+        }
+    }
+}
+
+// More than MC_WARP_KREG kept patches: weights in LDS, run-time loop.
+__device__ __forceinline__ void warp_tile_lds(const WarpArgs& a, const int* klist, int nk,
+                                           float4* tab, float* wk, int x, int y, bool live) {
+    for (int q = 0; q < nk; q++) {
+        float w = 0.f;
+        if (live) {
+            const int p = klist[q];
+            const float ax = ((float)x - a.cen[2 * p]) / (float)a.W;
+            const float ay = ((float)y - a.cen[2 * p + 1]) / (float)a.H;
+            const float r = sqrtf(ax * ax + ay * ay);
+            w = (r > 0.6f) ? 0.f : expf(-100.0f * r);
+        }
+        wk[q * MC_WARP_NT + threadIdx.x] = w;
+    }
+    for (int f0 = 0; f0 < a.N; f0 += MC_WARP_FCH) {
+        const int nf = min(MC_WARP_FCH, a.N - f0);
+        __syncthreads();
+        warp_table(a, klist, nk, nk, f0, nf, tab);
+        __syncthreads();
+        if (!live) continue;
+        for (int fi = 0; fi < nf; fi++) {
+            const float4* t = tab + fi * nk;
+            float fw = 0.f, fsx = 0.f, fsy = 0.f;
+            for (int q = 0; q < nk; q++) {
+                const float4 v = t[q];
+                const float wq = wk[q * MC_WARP_NT + threadIdx.x];
+                fw = fmaf(wq, v.z, fw);
+                fsx = fmaf(v.x, wq, fsx);
+                fsy = fmaf(v.y, wq, fsy);
+            }
+            const long long fo = (long long)(f0 + fi) * a.fstride;
+            a.Rp[fo + (long long)y * a.pitch + x] =
+                a.R[fo + warp_src(a.W, a.H, a.pitch, x, y, fw, fsx, fsy)];
+        }
+    }
+}
+
+// Dynamic LDS: frame table MC_WARP_FCH x max(K) float4, then (only when the
+// patch count exceeds MC_WARP_KREG) weights [kept patch][thread].
+__global__ __launch_bounds__(MC_WARP_NT) void k_local_warp(WarpArgs a) {
+    extern __shared__ float4 tab[];
     __shared__ unsigned int kmask[2];
     __shared__ int klist[MC_WARP_MAXP];
     __shared__ int nk_s;
-    __shared__ float2 shk[MC_WARP_FCH][MC_WARP_MAXP];
-    __shared__ unsigned long long bm[MC_WARP_FCH];
-    const int x = blockIdx.x * MC_WARP_TX + threadIdx.x;
-    const int y = blockIdx.y;
-    const bool live = x < W;
+    const int x = blockIdx.x * MC_WARP_TX + (threadIdx.x % MC_WARP_TX);
+    const int y = blockIdx.y * MC_WARP_TY + (threadIdx.x / MC_WARP_TX);
+    const bool live = x < a.W && y < a.H;
     if (threadIdx.x == 0) kmask[0] = kmask[1] = 0u;
     __syncthreads();
     float rmin = 1e30f;
-    for (int p = 0; p < P && live; p++) {
-        const float ax = ((float)x - cen[2 * p]) / (float)W;
-        const float ay = ((float)y - cen[2 * p + 1]) / (float)H;
+    for (int p = 0; p < a.P && live; p++) {
+        const float ax = ((float)x - a.cen[2 * p]) / (float)a.W;
+        const float ay = ((float)y - a.cen[2 * p + 1]) / (float)a.H;
         rmin = fminf(rmin, sqrtf(ax * ax + ay * ay));
     }
     unsigned int lo = 0u, hi = 0u;
-    for (int p = 0; p < P && live; p++) {
-        const float ax = ((float)x - cen[2 * p]) / (float)W;
-        const float ay = ((float)y - cen[2 * p + 1]) / (float)H;
+    for (int p = 0; p < a.P && live; p++) {
+        const float ax = ((float)x - a.cen[2 * p]) / (float)a.W;
+        const float ay = ((float)y - a.cen[2 * p + 1]) / (float)a.H;
         const float r = sqrtf(ax * ax + ay * ay);
         if (r <= 0.6f && r <= rmin + 0.3224f) {
             if (p < 32) lo |= 1u << p;
@@ -713,7 +856,7 @@ __global__ __launch_bounds__(MC_WARP_TX) void k_local_warp(const float* __restri
     __syncthreads();
     if (threadIdx.x == 0) {
         int n = 0;
-        for (int p = 0; p < P; p++) {
+        for (int p = 0; p < a.P; p++) {
             const unsigned int wd = kmask[p >> 5];
             if ((wd >> (p & 31)) & 1u) klist[n++] = p;
         }
@@ -721,65 +864,20 @@ __global__ __launch_bounds__(MC_WARP_TX) void k_local_warp(const float* __restri
     }
     __syncthreads();
     const int nk = nk_s;
-    for (int q = 0; q < nk; q++) {
-        float w = 0.f;
-        if (live) {
-            const int p = klist[q];
-            const float ax = ((float)x - cen[2 * p]) / (float)W;
-            const float ay = ((float)y - cen[2 * p + 1]) / (float)H;
-            const float r = sqrtf(ax * ax + ay * ay);
-            w = (r > 0.6f) ? 0.f : expf(-100.0f * r);
-        }
-        wk[q * MC_WARP_TX + threadIdx.x] = w;
-    }
-    for (int f0 = 0; f0 < N; f0 += MC_WARP_FCH) {
-        const int nf = min(MC_WARP_FCH, N - f0);
-        __syncthreads();
-        for (int e = threadIdx.x; e < nf * nk; e += MC_WARP_TX) {
-            const int fi = e / nk;
-            const int q = e - fi * nk;
-            const float* s = sh + ((long long)(f0 + fi) * P + klist[q]) * 2;
-            shk[fi][q] = make_float2(s[0], s[1]);
-        }
-        for (int fi = threadIdx.x; fi < nf; fi += MC_WARP_TX) {
-            unsigned long long m = 0ull;
-            const unsigned char* bd = bad + (long long)(f0 + fi) * P;
-            for (int p = 0; p < P; p++)
-                if (bd[p]) m |= 1ull << p;
-            bm[fi] = m;
-        }
-        __syncthreads();
-        if (!live) continue;
-        for (int fi = 0; fi < nf; fi++) {
-            const unsigned long long m = bm[fi];
-            float fw = 0.f, fsx = 0.f, fsy = 0.f;
-            for (int q = 0; q < nk; q++) {
-                if ((m >> klist[q]) & 1ull) continue;
-                const float w = wk[q * MC_WARP_TX + threadIdx.x];
-                const float2 s = shk[fi][q];
-                fw += w;
-                fsx += s.x * w;
-                fsy += s.y * w;
-            }
-            if (fw > 0) {
-                fsx = fsx / fw;
-                fsy = fsy / fw;
-            }
-            // keep the gather index finite for non-finite or absurd shifts
-            if (!(fabsf(fsx) < 1e6f)) fsx = 0.f;
-            if (!(fabsf(fsy) < 1e6f)) fsy = 0.f;
-            int xs = (int)((float)x - fsx);
-            int ys = (int)((float)y - fsy);
-            if (xs < 0 || ys < 0 || xs >= W || ys >= H) {
-                xs = xs < 0 ? -xs : xs;
-                ys = ys < 0 ? -ys : ys;
-                xs = (int)((811LL * xs) % W);
-                ys = (int)((811LL * ys) % H);
-            }
-            const long long fo = (long long)(f0 + fi) * fstride;
-            Rp[fo + (long long)y * pitch + x] = R[fo + (long long)ys * pitch + xs];
-        }
-    }
+    if (nk <= 4) warp_tile_reg<4>(a, klist, nk, tab, x, y, live);
+    else if (nk <= 8) warp_tile_reg<8>(a, klist, nk, tab, x, y, live);
+    else if (nk <= 12) warp_tile_reg<12>(a, klist, nk, tab, x, y, live);
+    else if (nk <= MC_WARP_KREG) warp_tile_reg<MC_WARP_KREG>(a, klist, nk, tab, x, y, live);
+    else
+        warp_tile_lds(a, klist, nk, tab, reinterpret_cast<float*>(tab + MC_WARP_FCH * a.P), x, y,
+                      live);
+}
+
+static size_t warp_lds_bytes(int P) {
+    const int kt = P > MC_WARP_KREG ? P : MC_WARP_KREG;
+    size_t b = (size_t)MC_WARP_FCH * kt * sizeof(float4);
+    if (P > MC_WARP_KREG) b += (size_t)P * MC_WARP_NT * sizeof(float);
+    return b;
 }
 
 // ---------------------------------------------------------------------------
@@ -940,13 +1038,13 @@ __global__ void k_synth(float* frames, int W, int H, long long pitch, long long 
 // ===========================================================================
 int mc_align_init_constants() {
     if (mc_upload_prime_table() != 0) return -1;
-    const int bytes = (MC_LDS_ELEMS + 64) * (int)sizeof(cfloat);
+    const int bytes = (mc_lds_padded(MC_LDS_ELEMS) + 64) * (int)sizeof(cfloat);
     hipFuncSetAttribute((const void*)k_correlate, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
     hipFuncSetAttribute((const void*)k_row_dft, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
     hipFuncSetAttribute((const void*)k_feature_map, hipFuncAttributeMaxDynamicSharedMemorySize,
                         2 * 128 * 128 * (int)sizeof(float));
     hipFuncSetAttribute((const void*)k_local_warp, hipFuncAttributeMaxDynamicSharedMemorySize,
-                        MC_WARP_MAXP * MC_WARP_TX * (int)sizeof(float));
+                        (int)warp_lds_bytes(MC_WARP_MAXP));
     return 0;
 }
 
@@ -990,7 +1088,7 @@ hipError_t mc_launch_correlate(const FftPlan1& colplan, int Cc, const cfloat* F,
     a.seay = seay;
     a.T1 = T1;
     dim3 grid(mc_div_up(cx, Cc), st.G, st.J);
-    const size_t lds = (size_t)Cc * H * sizeof(cfloat);
+    const size_t lds = (size_t)mc_lds_padded(Cc * H) * sizeof(cfloat);
     hipLaunchKernelGGL(k_correlate, grid, dim3(MC_NTHR), lds, s, a);
     return hipGetLastError();
 }
@@ -1072,11 +1170,10 @@ hipError_t mc_launch_patch_stats(double* shifts, const double* centers, int N, i
 hipError_t mc_launch_local_warp(const float* R, float* Rp, long long pitch, long long fstride,
                                 int W, int H, int N, int P, const float* cen, const float* sh,
                                 const unsigned char* bad, hipStream_t s) {
-    if (P > MC_WARP_MAXP) return hipErrorInvalidValue;
-    dim3 grid(mc_div_up(W, MC_WARP_TX), H);
-    const size_t lds = (size_t)P * MC_WARP_TX * sizeof(float);
-    hipLaunchKernelGGL(k_local_warp, grid, dim3(MC_WARP_TX), lds, s, R, Rp, pitch, fstride, W, H, N,
-                       P, cen, sh, bad);
+    if (P < 1 || P > MC_WARP_MAXP) return hipErrorInvalidValue;
+    WarpArgs a{R, Rp, pitch, fstride, W, H, N, P, cen, sh, bad};
+    dim3 grid(mc_div_up(W, MC_WARP_TX), mc_div_up(H, MC_WARP_TY));
+    hipLaunchKernelGGL(k_local_warp, grid, dim3(MC_WARP_NT), warp_lds_bytes(P), s, a);
     return hipGetLastError();
 }
 

@@ -47,6 +47,21 @@ __device__ __forceinline__ int fdiv(int a, int d, float rcp) {
     return q;
 }
 
+// Workgroups are dispatched round-robin over the 8 XCDs of an MI355X, each
+// with its own L2.  Map the linear block id so that consecutive tiles (which
+// share cache lines of the staging layout) run on the same XCD.
+__device__ __forceinline__ int xcd_tile(int bid, int nb) {
+    const int q = nb >> 3, r = nb & 7;
+    const int x = bid & 7, i = bid >> 3;
+    return x * q + (x < r ? x : r) + i;
+}
+
+// LDS index with one pad element per 16: strided butterfly writes then
+// spread over all banks instead of hitting two of them.
+__device__ __forceinline__ int lp(int i) { return i + (i >> 4); }
+// LDS elements to allocate for `count` padded complex values.
+__host__ __device__ inline int mc_lds_padded(int count) { return count + (count >> 4) + 1; }
+
 // Plan of one complex FFT length, passed to kernels by value.
 struct FftPlan1 {
     int n;

@@ -9,7 +9,7 @@
 #include <vector>
 
 #include "mc_fft.h"
-#include "mc_fft_dev.h"
+#include "mc_fft_ct.h"
 
 namespace {
 
@@ -73,10 +73,14 @@ __device__ __forceinline__ int resize_row(int yo, int outy, int iny) {
     return (y > iny / 2) ? -1 : y;
 }
 
+#define MC_IO_UNROLL 8
+
 __global__ __launch_bounds__(MC_NTHR) void k_rows_r2c(RowR2CArgs a) {
     extern __shared__ cfloat lds[];
-    const int b = blockIdx.y;
-    const int row0 = blockIdx.x * a.R;
+    const int nrb = (a.H + a.R - 1) / a.R;
+    const int t = xcd_tile(blockIdx.x, gridDim.x);
+    const int b = t / nrb;
+    const int row0 = (t - b * nrb) * a.R;
     const int nrows = min(a.R, a.H - row0);
     const int img = a.in_mod > 0 ? b % a.in_mod : b;
     const int wi = a.in_mod > 0 ? b / a.in_mod : b;
@@ -85,33 +89,38 @@ __global__ __launch_bounds__(MC_NTHR) void k_rows_r2c(RowR2CArgs a) {
         x0 = a.win[2 * wi];
         y0 = a.win[2 * wi + 1];
     }
-    const float* base = a.in + (long long)img * a.in_bstride;
+    const float* base = a.in + (long long)img * a.in_bstride + (long long)(y0 + row0) * a.pitch + x0;
     const float rm = 1.0f / a.m;
-    if (((x0 | (int)(a.pitch & 1) | (int)(a.in_bstride & 1)) & 1) == 0 &&
-        (((uintptr_t)a.in) & 7) == 0) {
-        for (int e = threadIdx.x; e < nrows * a.m; e += MC_NTHR) {
-            const int r = fdiv(e, a.m, rm);
-            const int j = e - r * a.m;
-            const float2 v = *reinterpret_cast<const float2*>(
-                base + (long long)(y0 + row0 + r) * a.pitch + x0 + 2 * j);
-            lds[e] = cmk(v.x, v.y);
+    const int total = nrows * a.m;
+    const bool vec = ((x0 | (int)(a.pitch & 1) | (int)(a.in_bstride & 1)) & 1) == 0 &&
+                     (((uintptr_t)a.in) & 7) == 0;
+    for (int e0 = threadIdx.x; e0 < total; e0 += MC_NTHR * MC_IO_UNROLL) {
+        float2 v[MC_IO_UNROLL];
+#pragma unroll
+        for (int u = 0; u < MC_IO_UNROLL; u++) {
+            const int e = e0 + u * MC_NTHR;
+            if (e < total) {
+                const int r = fdiv(e, a.m, rm);
+                const float* src = base + (long long)r * a.pitch + 2 * (e - r * a.m);
+                v[u] = vec ? *reinterpret_cast<const float2*>(src) : make_float2(src[0], src[1]);
+            }
         }
-    } else {
-        for (int e = threadIdx.x; e < nrows * a.m; e += MC_NTHR) {
-            const int r = fdiv(e, a.m, rm);
-            const int j = e - r * a.m;
-            const float* src = base + (long long)(y0 + row0 + r) * a.pitch + x0 + 2 * j;
-            lds[e] = cmk(src[0], src[1]);
+#pragma unroll
+        for (int u = 0; u < MC_IO_UNROLL; u++) {
+            const int e = e0 + u * MC_NTHR;
+            if (e < total) lds[lp(e)] = cmk(v[u].x, v[u].y);
         }
     }
     __syncthreads();
     fft_lds_fwd(lds, nrows, a.plan);
     const int kc = a.m + 1;
     const int nb = (kc + a.C - 1) / a.C;
-    cfloat* tb = a.T + (long long)b * a.t_bstride;
+    cfloat* tb = a.T + (long long)b * a.t_bstride + (long long)row0 * a.C;
     const int tot = nb * nrows * a.C;
     const int lc = __ffs(a.C) - 1;
     const float rnr = 1.0f / nrows;
+    const long long kbs = (long long)a.H * a.C;
+#pragma unroll 4
     for (int e = threadIdx.x; e < tot; e += MC_NTHR) {
         const int c = e & (a.C - 1);
         const int rc = e >> lc;
@@ -119,38 +128,53 @@ __global__ __launch_bounds__(MC_NTHR) void k_rows_r2c(RowR2CArgs a) {
         const int r = rc - kb * nrows;
         const int k = kb * a.C + c;
         if (k > a.m) continue;
-        const cfloat zk = lds[r * a.m + (k == a.m ? 0 : k)];
-        const cfloat zm = lds[r * a.m + (k == 0 ? 0 : a.m - k)];
+        const cfloat zk = lds[lp(r * a.m + (k == a.m ? 0 : k))];
+        const cfloat zm = lds[lp(r * a.m + (k == 0 ? 0 : a.m - k))];
         const cfloat E = cmk(0.5f * (zk.x + zm.x), 0.5f * (zk.y - zm.y));
         const cfloat D = cmk(zk.x - zm.x, zk.y + zm.y);
         const cfloat O = cmk(0.5f * D.y, -0.5f * D.x);
         const cfloat X = cadd(E, cmul(a.post_tw[k], O));
-        tb[(long long)kb * a.H * a.C + (long long)(row0 + r) * a.C + c] = cscale(X, a.scale);
+        tb[kb * kbs + r * a.C + c] = cscale(X, a.scale);
     }
 }
 
 __global__ __launch_bounds__(MC_NTHR) void k_rows_c2r(RowC2RArgs a) {
     extern __shared__ cfloat lds[];
-    const int b = blockIdx.y;
-    const int row0 = blockIdx.x * a.R;
+    const int nrb = (a.H + a.R - 1) / a.R;
+    const int t = xcd_tile(blockIdx.x, gridDim.x);
+    const int b = t / nrb;
+    const int row0 = (t - b * nrb) * a.R;
     const int nrows = min(a.R, a.H - row0);
-    cfloat* nyq = lds + nrows * a.m;
     const int kc = a.m + 1;
     const int nb = (kc + a.C - 1) / a.C;
-    const cfloat* tb = a.T + (long long)b * a.t_bstride;
+    const cfloat* tb = a.T + (long long)b * a.t_bstride + (long long)row0 * a.C;
     const int tot = nb * nrows * a.C;
     const int lc = __ffs(a.C) - 1;
     const float rnr = 1.0f / nrows;
-    for (int e = threadIdx.x; e < tot; e += MC_NTHR) {
-        const int c = e & (a.C - 1);
-        const int rc = e >> lc;
-        const int kb = fdiv(rc, nrows, rnr);
-        const int r = rc - kb * nrows;
-        const int k = kb * a.C + c;
-        if (k > a.m) continue;
-        const cfloat v = tb[(long long)kb * a.H * a.C + (long long)(row0 + r) * a.C + c];
-        if (k < a.m) lds[r * a.m + k] = v;
-        else nyq[r] = v;
+    const long long kbs = (long long)a.H * a.C;
+    for (int e0 = threadIdx.x; e0 < tot; e0 += MC_NTHR * MC_IO_UNROLL) {
+        cfloat v[MC_IO_UNROLL];
+        int dst[MC_IO_UNROLL];
+#pragma unroll
+        for (int u = 0; u < MC_IO_UNROLL; u++) {
+            const int e = e0 + u * MC_NTHR;
+            dst[u] = -1;
+            if (e < tot) {
+                const int c = e & (a.C - 1);
+                const int rc = e >> lc;
+                const int kb = fdiv(rc, nrows, rnr);
+                const int r = rc - kb * nrows;
+                const int k = kb * a.C + c;
+                if (k <= a.m) {
+                    v[u] = tb[kb * kbs + r * a.C + c];
+                    // X_m (Nyquist) goes to the slots after the rows (nyq[r])
+                    dst[u] = (k < a.m) ? r * a.m + k : nrows * a.m + r;
+                }
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < MC_IO_UNROLL; u++)
+            if (dst[u] >= 0) lds[lp(dst[u])] = v[u];
     }
     __syncthreads();
     // Hermitian pre-processing into Z'[k] = (X_k + X*_{m-k}) +
@@ -160,44 +184,46 @@ __global__ __launch_bounds__(MC_NTHR) void k_rows_c2r(RowC2RArgs a) {
     for (int e = threadIdx.x; e < nrows * half; e += MC_NTHR) {
         const int r = fdiv(e, half, rhalf);
         const int k = e - r * half;
-        cfloat* row = lds + r * a.m;
+        const int row = r * a.m;
         if (k == 0) {
-            const float x0 = row[0].x, xm = nyq[r].x;
-            row[0] = cconj(cmk(x0 + xm, x0 - xm));
+            const float x0 = lds[lp(row)].x, xm = lds[lp(nrows * a.m + r)].x;
+            lds[lp(row)] = cconj(cmk(x0 + xm, x0 - xm));
             continue;
         }
         const int k2 = a.m - k;
         if (k2 < k) continue;
-        const cfloat xa = row[k], xb = row[k2];
+        const cfloat xa = lds[lp(row + k)], xb = lds[lp(row + k2)];
         const cfloat twk = cconj(a.post_tw[k]);
         const cfloat s1 = cmk(xa.x + xb.x, xa.y - xb.y);
         const cfloat d1 = cmk(xa.x - xb.x, xa.y + xb.y);
         const cfloat t1 = cmul(twk, d1);
         const cfloat zk = cmk(s1.x - t1.y, s1.y + t1.x);
-        row[k] = cconj(zk);
+        lds[lp(row + k)] = cconj(zk);
         if (k2 != k) {
             const cfloat twk2 = cconj(a.post_tw[k2]);
             const cfloat s2 = cmk(xb.x + xa.x, xb.y - xa.y);
             const cfloat d2 = cmk(xb.x - xa.x, xb.y + xa.y);
             const cfloat t2 = cmul(twk2, d2);
-            row[k2] = cconj(cmk(s2.x - t2.y, s2.y + t2.x));
+            lds[lp(row + k2)] = cconj(cmk(s2.x - t2.y, s2.y + t2.x));
         }
     }
     __syncthreads();
     fft_lds_fwd(lds, nrows, a.plan);
-    float* ob = a.out + (long long)b * a.out_bstride;
+    float* ob = a.out + (long long)b * a.out_bstride + (long long)row0 * a.pitch;
     const float rm = 1.0f / a.m;
     const bool vec = ((a.pitch | a.out_bstride) & 1) == 0 && (((uintptr_t)a.out) & 7) == 0;
-    for (int e = threadIdx.x; e < nrows * a.m; e += MC_NTHR) {
+    const int total = nrows * a.m;
+#pragma unroll 4
+    for (int e = threadIdx.x; e < total; e += MC_NTHR) {
         const int r = fdiv(e, a.m, rm);
         const int j = e - r * a.m;
-        const cfloat z = lds[e];
+        const cfloat z = lds[lp(e)];
         float v0 = z.x, v1 = -z.y;
         if (a.clamp) {
             v0 = fmaxf(v0, 0.f);
             v1 = fmaxf(v1, 0.f);
         }
-        float* dst = ob + (long long)(row0 + r) * a.pitch + 2 * j;
+        float* dst = ob + (long long)r * a.pitch + 2 * j;
         if (vec) {
             *reinterpret_cast<float2*>(dst) = make_float2(v0, v1);
         } else {
@@ -209,26 +235,40 @@ __global__ __launch_bounds__(MC_NTHR) void k_rows_c2r(RowC2RArgs a) {
 
 __global__ __launch_bounds__(MC_NTHR) void k_cols_fwd(ColFwdArgs a) {
     extern __shared__ cfloat lds[];
-    const int b = blockIdx.y;
-    const int kb = blockIdx.x;
+    const int ncb = (a.ncols + a.C - 1) / a.C;
+    const int t = xcd_tile(blockIdx.x, gridDim.x);
+    const int b = t / ncb;
+    const int kb = t - b * ncb;
     const int c0 = kb * a.C;
     const int nc = min(a.C, a.ncols - c0);
     const cfloat* src = a.T + (long long)b * a.t_bstride + (long long)kb * a.H * a.C;
     const int lc = __ffs(a.C) - 1;
-    for (int e = threadIdx.x; e < a.H * a.C; e += MC_NTHR) {
-        const int y = e >> lc;
-        const int c = e & (a.C - 1);
-        if (c < nc) lds[c * a.H + y] = src[e];
+    const int total = a.H * a.C;
+    for (int e0 = threadIdx.x; e0 < total; e0 += MC_NTHR * MC_IO_UNROLL) {
+        cfloat v[MC_IO_UNROLL];
+#pragma unroll
+        for (int u = 0; u < MC_IO_UNROLL; u++) {
+            const int e = e0 + u * MC_NTHR;
+            if (e < total && (e & (a.C - 1)) < nc) v[u] = src[e];
+        }
+#pragma unroll
+        for (int u = 0; u < MC_IO_UNROLL; u++) {
+            const int e = e0 + u * MC_NTHR;
+            const int c = e & (a.C - 1);
+            if (e < total && c < nc) lds[lp(c * a.H + (e >> lc))] = v[u];
+        }
     }
     __syncthreads();
     fft_lds_fwd(lds, nc, a.plan);
     if (a.S) {
         cfloat* dst = a.S + (long long)b * a.s_bstride + (long long)c0 * a.H;
-        for (int e = threadIdx.x; e < nc * a.H; e += MC_NTHR) dst[e] = lds[e];
+#pragma unroll 4
+        for (int e = threadIdx.x; e < nc * a.H; e += MC_NTHR) dst[e] = lds[lp(e)];
     }
     if (a.X) {
         cfloat* xd = a.X + (long long)b * a.x_bstride;
         const float rx = 1.0f / a.xcy;
+#pragma unroll 4
         for (int e = threadIdx.x; e < nc * a.xcy; e += MC_NTHR) {
             const int c = fdiv(e, a.xcy, rx);
             const int yo = e - c * a.xcy;
@@ -236,57 +276,329 @@ __global__ __launch_bounds__(MC_NTHR) void k_cols_fwd(ColFwdArgs a) {
             if (kx >= a.xcx) continue;
             const int yi = resize_row(yo, a.xcy, a.H);
             if (yi < 0) continue;
-            xd[(long long)kx * a.xcy + yo] = lds[c * a.H + yi];
+            xd[(long long)kx * a.xcy + yo] = lds[lp(c * a.H + yi)];
         }
     }
 }
 
 __global__ __launch_bounds__(MC_NTHR) void k_cols_inv(ColInvArgs a) {
     extern __shared__ cfloat lds[];
-    const int b = blockIdx.y;
-    const int kb = blockIdx.x;
+    const int ncb = (a.ncols + a.C - 1) / a.C;
+    const int t = xcd_tile(blockIdx.x, gridDim.x);
+    const int b = t / ncb;
+    const int kb = t - b * ncb;
     const int c0 = kb * a.C;
     const int nc = min(a.C, a.ncols - c0);
     const bool same = (a.src_cx == a.ncols && a.src_cy == a.H);
     const cfloat* sb = a.S + (long long)b * a.s_bstride;
     const float rH = 1.0f / a.H;
-    for (int e = threadIdx.x; e < nc * a.H; e += MC_NTHR) {
-        const int c = fdiv(e, a.H, rH);
-        const int y = e - c * a.H;
-        const int kx = c0 + c;
-        cfloat v;
-        if (same) {
-            v = sb[(long long)kx * a.H + y];
-        } else {
-            const int yi = resize_row(y, a.H, a.src_cy);
-            v = (kx < a.src_cx && yi >= 0) ? sb[(long long)kx * a.src_cy + yi] : cmk(0.f, 0.f);
+    const int total = nc * a.H;
+    for (int e0 = threadIdx.x; e0 < total; e0 += MC_NTHR * MC_IO_UNROLL) {
+        cfloat v[MC_IO_UNROLL];
+#pragma unroll
+        for (int u = 0; u < MC_IO_UNROLL; u++) {
+            const int e = e0 + u * MC_NTHR;
+            if (e < total) {
+                const int c = fdiv(e, a.H, rH);
+                const int y = e - c * a.H;
+                const int kx = c0 + c;
+                if (same) {
+                    v[u] = sb[(long long)kx * a.H + y];
+                } else {
+                    const int yi = resize_row(y, a.H, a.src_cy);
+                    v[u] = (kx < a.src_cx && yi >= 0) ? sb[(long long)kx * a.src_cy + yi]
+                                                      : cmk(0.f, 0.f);
+                }
+                if (a.phA)
+                    v[u] = cmul(v[u], cmul(a.phA[(long long)b * a.ncols + kx],
+                                           a.phB[(long long)b * a.H + y]));
+            }
         }
-        if (a.phA) v = cmul(v, cmul(a.phA[(long long)b * a.ncols + kx], a.phB[(long long)b * a.H + y]));
-        lds[e] = cconj(v);
+#pragma unroll
+        for (int u = 0; u < MC_IO_UNROLL; u++) {
+            const int e = e0 + u * MC_NTHR;
+            if (e < total) lds[lp(e)] = cconj(v[u]);
+        }
     }
     __syncthreads();
     fft_lds_fwd(lds, nc, a.plan);
     cfloat* dst = a.T + (long long)b * a.t_bstride + (long long)kb * a.H * a.C;
     const int lc = __ffs(a.C) - 1;
+#pragma unroll 4
     for (int e = threadIdx.x; e < a.H * a.C; e += MC_NTHR) {
-        const int y = e >> lc;
         const int c = e & (a.C - 1);
-        if (c < nc) dst[e] = cconj(lds[c * a.H + y]);
+        if (c < nc) dst[e] = cconj(lds[lp(c * a.H + (e >> lc))]);
     }
 }
 
+// ---------------------------------------------------------------------------
+// Specialised passes: the row length M (or column length H), rows per
+// workgroup RR and column block C are template constants; each transform is
+// owned by MC_NTHR/RR (or MC_NTHR/C) threads.
+template <int M, int RR, int C>
+__global__ __launch_bounds__(MC_NTHR) void k_rows_r2c_ct(RowR2CArgs a) {
+    constexpr int TPT = MC_NTHR / RR;
+    constexpr int LIT = (M + TPT - 1) / TPT;
+    constexpr int NB = (M + 1 + C - 1) / C;
+    constexpr int TOT = NB * RR * C;
+    extern __shared__ cfloat lds[];
+    const int nrb = (a.H + RR - 1) / RR;
+    const int t = xcd_tile(blockIdx.x, gridDim.x);
+    const int b = t / nrb;
+    const int row0 = (t - b * nrb) * RR;
+    const int nrows = min(RR, a.H - row0);
+    const int g = threadIdx.x / TPT, tid = threadIdx.x % TPT;
+    const int img = a.in_mod > 0 ? b % a.in_mod : b;
+    const int wi = a.in_mod > 0 ? b / a.in_mod : b;
+    int x0 = 0, y0 = 0;
+    if (a.win) {
+        x0 = a.win[2 * wi];
+        y0 = a.win[2 * wi + 1];
+    }
+    const bool vec = ((x0 | (int)(a.pitch & 1) | (int)(a.in_bstride & 1)) & 1) == 0 &&
+                     (((uintptr_t)a.in) & 7) == 0;
+    const bool live = g < nrows;
+    const float* src = a.in + (long long)img * a.in_bstride +
+                       (long long)(y0 + row0 + (live ? g : 0)) * a.pitch + x0;
+    float2 v[LIT];
+#pragma unroll
+    for (int it = 0; it < LIT; it++) {
+        const int j = tid + it * TPT;
+        v[it] = make_float2(0.f, 0.f);
+        if (live && (M % TPT == 0 || j < M))
+            v[it] = vec ? *reinterpret_cast<const float2*>(src + 2 * j)
+                        : make_float2(src[2 * j], src[2 * j + 1]);
+    }
+#pragma unroll
+    for (int it = 0; it < LIT; it++) {
+        const int j = tid + it * TPT;
+        if (M % TPT == 0 || j < M) lds[lp(g * M + j)] = cmk(v[it].x, v[it].y);
+    }
+    __syncthreads();
+    ct_fft<M, TPT>(lds, g * M, tid, a.plan.tw,
+                   lds + mc_lds_padded(RR * (M + 1)) + g * ct_scratch(M));
+    cfloat* tb = a.T + (long long)b * a.t_bstride + (long long)row0 * C;
+    const long long kbs = (long long)a.H * C;
+#pragma unroll 4
+    for (int e = threadIdx.x; e < TOT; e += MC_NTHR) {
+        const int c = e % C, r = (e / C) % RR, kb = e / (C * RR);
+        const int k = kb * C + c;
+        if (k > M || r >= nrows) continue;
+        const cfloat zk = lds[lp(r * M + (k == M ? 0 : k))];
+        const cfloat zm = lds[lp(r * M + (k == 0 ? 0 : M - k))];
+        const cfloat E = cmk(0.5f * (zk.x + zm.x), 0.5f * (zk.y - zm.y));
+        const cfloat D = cmk(zk.x - zm.x, zk.y + zm.y);
+        const cfloat O = cmk(0.5f * D.y, -0.5f * D.x);
+        const cfloat X = cadd(E, cmul(a.post_tw[k], O));
+        tb[kb * kbs + r * C + c] = cscale(X, a.scale);
+    }
+}
+
+template <int M, int RR, int C>
+__global__ __launch_bounds__(MC_NTHR) void k_rows_c2r_ct(RowC2RArgs a) {
+    constexpr int TPT = MC_NTHR / RR;
+    constexpr int NB = (M + 1 + C - 1) / C;
+    constexpr int TOT = NB * RR * C;
+    constexpr int HALF = M / 2 + 1;
+    constexpr int SIT = (M + TPT - 1) / TPT;
+    extern __shared__ cfloat lds[];
+    const int nrb = (a.H + RR - 1) / RR;
+    const int t = xcd_tile(blockIdx.x, gridDim.x);
+    const int b = t / nrb;
+    const int row0 = (t - b * nrb) * RR;
+    const int nrows = min(RR, a.H - row0);
+    const int g = threadIdx.x / TPT, tid = threadIdx.x % TPT;
+    const cfloat* tb = a.T + (long long)b * a.t_bstride + (long long)row0 * C;
+    const long long kbs = (long long)a.H * C;
+    for (int e0 = threadIdx.x; e0 < TOT; e0 += MC_NTHR * MC_IO_UNROLL) {
+        cfloat v[MC_IO_UNROLL];
+        int dst[MC_IO_UNROLL];
+#pragma unroll
+        for (int u = 0; u < MC_IO_UNROLL; u++) {
+            const int e = e0 + u * MC_NTHR;
+            dst[u] = -1;
+            if (e < TOT) {
+                const int c = e % C, r = (e / C) % RR, kb = e / (C * RR);
+                const int k = kb * C + c;
+                if (k <= M && r < nrows) {
+                    v[u] = tb[kb * kbs + r * C + c];
+                    dst[u] = (k < M) ? r * M + k : RR * M + r;
+                }
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < MC_IO_UNROLL; u++)
+            if (dst[u] >= 0) lds[lp(dst[u])] = v[u];
+    }
+    __syncthreads();
+    const int row = g * M;
+    for (int k = tid; k < HALF; k += TPT) {
+        if (k == 0) {
+            const float x0 = lds[lp(row)].x, xm = lds[lp(RR * M + g)].x;
+            lds[lp(row)] = cconj(cmk(x0 + xm, x0 - xm));
+            continue;
+        }
+        const int k2 = M - k;
+        if (k2 < k) continue;
+        const cfloat xa = lds[lp(row + k)], xb = lds[lp(row + k2)];
+        const cfloat t1 = cmul(cconj(a.post_tw[k]), cmk(xa.x - xb.x, xa.y + xb.y));
+        lds[lp(row + k)] = cconj(cmk(xa.x + xb.x - t1.y, xa.y - xb.y + t1.x));
+        if (k2 != k) {
+            const cfloat t2 = cmul(cconj(a.post_tw[k2]), cmk(xb.x - xa.x, xb.y + xa.y));
+            lds[lp(row + k2)] = cconj(cmk(xb.x + xa.x - t2.y, xb.y - xa.y + t2.x));
+        }
+    }
+    __syncthreads();
+    ct_fft<M, TPT>(lds, g * M, tid, a.plan.tw,
+                   lds + mc_lds_padded(RR * (M + 1)) + g * ct_scratch(M));
+    if (g >= nrows) return;
+    float* ob = a.out + (long long)b * a.out_bstride + (long long)(row0 + g) * a.pitch;
+    const bool vec = ((a.pitch | a.out_bstride) & 1) == 0 && (((uintptr_t)a.out) & 7) == 0;
+#pragma unroll
+    for (int it = 0; it < SIT; it++) {
+        const int j = tid + it * TPT;
+        if (M % TPT != 0 && j >= M) continue;
+        const cfloat z = lds[lp(row + j)];
+        float v0 = z.x, v1 = -z.y;
+        if (a.clamp) {
+            v0 = fmaxf(v0, 0.f);
+            v1 = fmaxf(v1, 0.f);
+        }
+        if (vec) *reinterpret_cast<float2*>(ob + 2 * j) = make_float2(v0, v1);
+        else {
+            ob[2 * j] = v0;
+            ob[2 * j + 1] = v1;
+        }
+    }
+}
+
+template <int H, int C, int NT>
+__global__ __launch_bounds__(NT) void k_cols_fwd_ct(ColFwdArgs a) {
+    constexpr int TPT = NT / C;
+    extern __shared__ cfloat lds[];
+    const int ncb = (a.ncols + C - 1) / C;
+    const int t = xcd_tile(blockIdx.x, gridDim.x);
+    const int b = t / ncb;
+    const int kb = t - b * ncb;
+    const int c0 = kb * C;
+    const int nc = min(C, a.ncols - c0);
+    const int g = threadIdx.x / TPT, tid = threadIdx.x % TPT;
+    const cfloat* src = a.T + (long long)b * a.t_bstride + (long long)kb * H * C;
+    for (int e0 = threadIdx.x; e0 < H * C; e0 += NT * MC_IO_UNROLL) {
+        cfloat v[MC_IO_UNROLL];
+#pragma unroll
+        for (int u = 0; u < MC_IO_UNROLL; u++) {
+            const int e = e0 + u * NT;
+            if (e < H * C && (e % C) < nc) v[u] = src[e];
+        }
+#pragma unroll
+        for (int u = 0; u < MC_IO_UNROLL; u++) {
+            const int e = e0 + u * NT;
+            if (e < H * C && (e % C) < nc) lds[lp((e % C) * H + e / C)] = v[u];
+        }
+    }
+    __syncthreads();
+    static_assert(ct_big_prime(H) == 0, "column lengths with a prime factor above 31 use the generic kernels");
+    ct_fft<H, TPT>(lds, g * H, tid, a.plan.tw);
+    if (a.S) {
+        cfloat* dst = a.S + (long long)b * a.s_bstride + (long long)c0 * H;
+#pragma unroll 4
+        for (int e = threadIdx.x; e < nc * H; e += NT) dst[e] = lds[lp(e)];
+    }
+    if (a.X) {
+        cfloat* xd = a.X + (long long)b * a.x_bstride;
+        const float rx = 1.0f / a.xcy;
+#pragma unroll 4
+        for (int e = threadIdx.x; e < nc * a.xcy; e += NT) {
+            const int c = fdiv(e, a.xcy, rx);
+            const int yo = e - c * a.xcy;
+            const int kx = c0 + c;
+            if (kx >= a.xcx) continue;
+            const int yi = resize_row(yo, a.xcy, H);
+            if (yi < 0) continue;
+            xd[(long long)kx * a.xcy + yo] = lds[lp(c * H + yi)];
+        }
+    }
+}
+
+template <int H, int C, int NT>
+__global__ __launch_bounds__(NT) void k_cols_inv_ct(ColInvArgs a) {
+    constexpr int TPT = NT / C;
+    extern __shared__ cfloat lds[];
+    const int ncb = (a.ncols + C - 1) / C;
+    const int t = xcd_tile(blockIdx.x, gridDim.x);
+    const int b = t / ncb;
+    const int kb = t - b * ncb;
+    const int c0 = kb * C;
+    const int nc = min(C, a.ncols - c0);
+    const int g = threadIdx.x / TPT, tid = threadIdx.x % TPT;
+    const bool same = (a.src_cx == a.ncols && a.src_cy == H);
+    const cfloat* sb = a.S + (long long)b * a.s_bstride;
+    const int total = nc * H;
+    for (int e0 = threadIdx.x; e0 < total; e0 += NT * MC_IO_UNROLL) {
+        cfloat v[MC_IO_UNROLL];
+#pragma unroll
+        for (int u = 0; u < MC_IO_UNROLL; u++) {
+            const int e = e0 + u * NT;
+            if (e < total) {
+                const int c = e / H, y = e % H;
+                const int kx = c0 + c;
+                if (same) {
+                    v[u] = sb[(long long)kx * H + y];
+                } else {
+                    const int yi = resize_row(y, H, a.src_cy);
+                    v[u] = (kx < a.src_cx && yi >= 0) ? sb[(long long)kx * a.src_cy + yi]
+                                                      : cmk(0.f, 0.f);
+                }
+                if (a.phA)
+                    v[u] = cmul(v[u], cmul(a.phA[(long long)b * a.ncols + kx],
+                                           a.phB[(long long)b * H + y]));
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < MC_IO_UNROLL; u++) {
+            const int e = e0 + u * NT;
+            if (e < total) lds[lp(e)] = cconj(v[u]);
+        }
+    }
+    __syncthreads();
+    static_assert(ct_big_prime(H) == 0, "column lengths with a prime factor above 31 use the generic kernels");
+    ct_fft<H, TPT>(lds, g * H, tid, a.plan.tw);
+    cfloat* dst = a.T + (long long)b * a.t_bstride + (long long)kb * H * C;
+#pragma unroll 4
+    for (int e = threadIdx.x; e < H * C; e += NT) {
+        const int c = e % C;
+        if (c < nc) dst[e] = cconj(lds[lp(c * H + e / C)]);
+    }
+}
+
+// (row length M, rows per workgroup, column block) of the reference
+// geometries: frames 5760x4092, 4096^2, 1024^2; xcf 2882x2048, 2048^2;
+// patches 576x408; the 128^2 feature map.
+#define MC_ROW_SPECS(X) \
+    X(2880, 4, 4) X(2048, 4, 4) X(512, 2, 16) X(1441, 2, 8) X(1024, 2, 8) X(288, 2, 16) X(64, 2, 16)
+#define MC_COL_SPECS(X) \
+    X(4092, 4, 576) X(4096, 4, 512) X(1024, 16, 512) X(2048, 8, 512) X(408, 16, 512) X(128, 16, 512)
+
+template <class F>
+static void set_lds_attr(F* f) {
+    const int bytes = (mc_lds_padded(MC_LDS_ELEMS) + 64) * (int)sizeof(cfloat);
+    hipFuncSetAttribute((const void*)f, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
+}
+
+static void set_ct_attrs() {
+#define X(M, RR, C) set_lds_attr(k_rows_r2c_ct<M, RR, C>); set_lds_attr(k_rows_c2r_ct<M, RR, C>);
+    MC_ROW_SPECS(X)
+#undef X
+#define X(H, C, NT) set_lds_attr(k_cols_fwd_ct<H, C, NT>); set_lds_attr(k_cols_inv_ct<H, C, NT>);
+    MC_COL_SPECS(X)
+#undef X
+}
+
+// Same factorisation as the compile-time stages (mc_fft_ct.h).
 void factor_device(int n, std::vector<int>& r) {
     r.clear();
-    while (n % 8 == 0) { r.push_back(8); n /= 8; }
-    if (n % 4 == 0) { r.push_back(4); n /= 4; }
-    if (n % 2 == 0) { r.push_back(2); n /= 2; }
-    static const int small[] = {3, 5, 7, 11, 13, 17, 19, 23, 29, 31};
-    for (int p : small)
-        while (n % p == 0) { r.push_back(p); n /= p; }
-    for (int p = 37; n > 1; p += 2) {
-        if ((long long)p * p > n) { r.push_back(n); break; }
-        while (n % p == 0) { r.push_back(p); n /= p; }
-    }
+    for (int s = 0; s < ct_nst(n); s++) r.push_back(ct_radix(n, s));
 }
 
 int build_plan1(FftPlan1* P, cfloat** dev_tw, int n) {
@@ -343,11 +655,12 @@ bool g_attr_done = false;
 int mc_fft_init_constants() {
     if (mc_upload_prime_table() != 0) return -1;
     if (!g_attr_done) {
-        const int bytes = (MC_LDS_ELEMS + 64) * (int)sizeof(cfloat);
+        const int bytes = (mc_lds_padded(MC_LDS_ELEMS) + 64) * (int)sizeof(cfloat);
         hipFuncSetAttribute((const void*)k_rows_r2c, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
         hipFuncSetAttribute((const void*)k_rows_c2r, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
         hipFuncSetAttribute((const void*)k_cols_fwd, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
         hipFuncSetAttribute((const void*)k_cols_inv, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
+        set_ct_attrs();
         g_attr_done = true;
     }
     return 0;
@@ -419,8 +732,17 @@ hipError_t mc_rows_r2c_mod(const Fft2D& p, const float* in, long long pitch, lon
     a.t_bstride = t_bstride;
     a.post_tw = p.post_tw;
     a.scale = scale;
-    dim3 grid(mc_div_up(p.H, p.R), batch);
-    const size_t lds = (size_t)p.R * p.m * sizeof(cfloat);
+    dim3 grid(mc_div_up(p.H, p.R) * batch);
+    const size_t lds = (size_t)mc_lds_padded(p.R * p.m) * sizeof(cfloat);
+#define X(MM, RRR, CCC)                                                             \
+    if (p.m == MM && p.R == RRR && p.C == CCC) {                                    \
+        const size_t ldsct =                                                        \
+            (size_t)(mc_lds_padded(RRR * (MM + 1)) + RRR * ct_scratch(MM)) * sizeof(cfloat); \
+        hipLaunchKernelGGL((k_rows_r2c_ct<MM, RRR, CCC>), grid, dim3(MC_NTHR), ldsct, s, a); \
+        return hipGetLastError();                                                   \
+    }
+    MC_ROW_SPECS(X)
+#undef X
     hipLaunchKernelGGL(k_rows_r2c, grid, dim3(MC_NTHR), lds, s, a);
     return hipGetLastError();
 }
@@ -441,8 +763,17 @@ hipError_t mc_rows_c2r(const Fft2D& p, const cfloat* T, long long t_bstride, flo
     a.out_bstride = out_bstride;
     a.post_tw = p.post_tw;
     a.clamp = clamp;
-    dim3 grid(mc_div_up(p.H, p.R), batch);
-    const size_t lds = (size_t)p.R * (p.m + 1) * sizeof(cfloat);
+    dim3 grid(mc_div_up(p.H, p.R) * batch);
+    const size_t lds = (size_t)mc_lds_padded(p.R * (p.m + 1)) * sizeof(cfloat);
+#define X(MM, RRR, CCC)                                                             \
+    if (p.m == MM && p.R == RRR && p.C == CCC) {                                    \
+        const size_t ldsct =                                                        \
+            (size_t)(mc_lds_padded(RRR * (MM + 1)) + RRR * ct_scratch(MM)) * sizeof(cfloat); \
+        hipLaunchKernelGGL((k_rows_c2r_ct<MM, RRR, CCC>), grid, dim3(MC_NTHR), ldsct, s, a); \
+        return hipGetLastError();                                                   \
+    }
+    MC_ROW_SPECS(X)
+#undef X
     hipLaunchKernelGGL(k_rows_c2r, grid, dim3(MC_NTHR), lds, s, a);
     return hipGetLastError();
 }
@@ -463,8 +794,15 @@ hipError_t mc_cols_fwd(const Fft2D& p, const cfloat* T, long long t_bstride, cfl
     a.xcx = xcx;
     a.xcy = xcy;
     a.x_bstride = x_bstride;
-    dim3 grid(mc_div_up(p.cx, p.C), batch);
-    const size_t lds = (size_t)p.C * p.H * sizeof(cfloat);
+    dim3 grid(mc_div_up(p.cx, p.C) * batch);
+    const size_t lds = (size_t)mc_lds_padded(p.C * p.H) * sizeof(cfloat);
+#define X(HH, CC, NT)                                                             \
+    if (p.H == HH && p.C == CC) {                                                 \
+        hipLaunchKernelGGL((k_cols_fwd_ct<HH, CC, NT>), grid, dim3(NT), lds, s, a);  \
+        return hipGetLastError();                                                 \
+    }
+    MC_COL_SPECS(X)
+#undef X
     hipLaunchKernelGGL(k_cols_fwd, grid, dim3(MC_NTHR), lds, s, a);
     return hipGetLastError();
 }
@@ -485,8 +823,15 @@ hipError_t mc_cols_inv(const Fft2D& p, const cfloat* S, long long s_bstride, int
     a.phB = ph ? ph->B : nullptr;
     a.T = T;
     a.t_bstride = t_bstride;
-    dim3 grid(mc_div_up(p.cx, p.C), batch);
-    const size_t lds = (size_t)p.C * p.H * sizeof(cfloat);
+    dim3 grid(mc_div_up(p.cx, p.C) * batch);
+    const size_t lds = (size_t)mc_lds_padded(p.C * p.H) * sizeof(cfloat);
+#define X(HH, CC, NT)                                                             \
+    if (p.H == HH && p.C == CC) {                                                 \
+        hipLaunchKernelGGL((k_cols_inv_ct<HH, CC, NT>), grid, dim3(NT), lds, s, a);  \
+        return hipGetLastError();                                                 \
+    }
+    MC_COL_SPECS(X)
+#undef X
     hipLaunchKernelGGL(k_cols_inv, grid, dim3(MC_NTHR), lds, s, a);
     return hipGetLastError();
 }

@@ -143,9 +143,9 @@ static __device__ __noinline__ void fft_stage(cfloat* lds, int n, int batch, con
             const int jj = j - b * m;
             const int q = fdiv(jj, ns, st.rcp_ns);
             const int k = jj - q * ns;
-            const cfloat* src = lds + b * n + jj;
+            const int src = b * n + jj;
 #pragma unroll
-            for (int t = 0; t < R; t++) v[it][t] = src[t * m];
+            for (int t = 0; t < R; t++) v[it][t] = lds[lp(src + t * m)];
             if (ns > 1) {
                 const cfloat* w = twp + k * (R - 1);
 #pragma unroll
@@ -160,7 +160,7 @@ static __device__ __noinline__ void fft_stage(cfloat* lds, int n, int batch, con
     for (int it = 0; it < MAXIT; it++) {
         if (dst[it] >= 0) {
 #pragma unroll
-            for (int t = 0; t < R; t++) lds[dst[it] + t * ns] = v[it][t];
+            for (int t = 0; t < R; t++) lds[lp(dst[it] + t * ns)] = v[it][t];
         }
     }
     __syncthreads();
@@ -196,7 +196,7 @@ static __device__ __noinline__ void fft_stage_big(cfloat* lds, int n, int batch,
             const int qd = fdiv(jj, ns, st.rcp_ns);
             const int k = jj - qd * ns;
             const int q0 = ch * MC_BIG_KC;
-            const cfloat* src = lds + b * n + jj;
+            const int src = b * n + jj;
             int ridx[MC_BIG_KC], step[MC_BIG_KC];
 #pragma unroll
             for (int q = 0; q < MC_BIG_KC; q++) {
@@ -204,7 +204,7 @@ static __device__ __noinline__ void fft_stage_big(cfloat* lds, int n, int batch,
                 step[q] = (q0 + q < p) ? q0 + q : 0;
             }
             for (int t = 0; t < p; t++) {
-                cfloat x = src[t * m];
+                cfloat x = lds[lp(src + t * m)];
                 if (ns > 1 && t > 0) x = cmul(x, twp[k * (p - 1) + t - 1]);
 #pragma unroll
                 for (int q = 0; q < MC_BIG_KC; q++) {
@@ -226,7 +226,7 @@ static __device__ __noinline__ void fft_stage_big(cfloat* lds, int n, int batch,
 #pragma unroll
             for (int q = 0; q < MC_BIG_KC; q++) {
                 const int qq = q0s[it] + q;
-                if (qq < p) lds[dst[it] + qq * ns] = acc[it][q];
+                if (qq < p) lds[lp(dst[it] + qq * ns)] = acc[it][q];
             }
         }
     }

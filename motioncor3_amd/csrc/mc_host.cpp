@@ -995,6 +995,12 @@ int mc_local_warp(mc_ctx* c, const float* in, float* out, int W, int H, int N, i
                   const float* centers, const float* shifts, const unsigned char* bad) {
     if (!c || !in || !out || !centers || !shifts || W < 2 || H < 2 || N < 1 || P < 1 || P > 64)
         return MC_E_ARG;
+    {
+        const char* i0 = (const char*)in;
+        const char* o0 = (const char*)out;
+        const size_t bytes = sizeof(float) * (size_t)W * H * N;
+        if (i0 < o0 + bytes && o0 < i0 + bytes) return fail(c, MC_E_ARG, "mc_local_warp: in and out overlap");
+    }
     MC_BUF(dc, float, "api_wc", 2 * P);
     MC_BUF(ds, float, "api_ws", (size_t)2 * N * P);
     MC_BUF(db, unsigned char, "api_wb", (size_t)N * P);
