@@ -11,7 +11,8 @@ mc_align_movie call (measurement, correction and the three output sums).
 Prints one JSON line.  The roofline figure is for the dominant kernel class,
 timed with HIP events on the library's stream during the timed steps.  The
 CPU baseline is the FP64 oracle (oracle/) on a bounded sample of the same
-workload: the first 8 frames of the movie, scaled by 50/8 to a movie.
+workload: the first 5 frames of the movie (two patch groups of -Group 4),
+scaled by 50/5 to a movie.
 """
 import argparse
 import json
@@ -52,7 +53,19 @@ def algorithmic_bytes(kernel, g, nch):
     raise ValueError(kernel)
 
 
-def cpu_baseline(recipe, params_kw, frames_needed=8):
+def measured_traffic(kernel):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 FETCH_SIZE /
+    WRITE_SIZE summary (copy-calibrated), or None when none was recorded."""
+    import glob
+    for path in sorted(glob.glob(os.path.join(ROOT, "profiles", "*_hbm_bytes.md"))):
+        for line in open(path):
+            f = line.split()
+            if len(f) == 4 and f[0] == "traffic_bytes_per_launch" and f[1] == kernel:
+                return int(f[3])
+    return None
+
+
+def cpu_baseline(recipe, params_kw, frames_needed=5):
     """FP64 oracle on the first `frames_needed` frames of the same movie."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
@@ -99,6 +112,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--kernel", default="fft_cols_fwd", help="kernel class for the roofline")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--copy-calib", type=int, default=0, metavar="MIB",
+                    help="one plain device-to-device copy of MIB MiB before the steps, as the "
+                         "known-size reference for hardware byte counters")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on one GPU only: the path is data-parallel over movies "
@@ -116,6 +132,13 @@ def main():
     prm = L.default_params(**kw)
     ns, ow, oh = ctx.output_geometry(prm, W, H, N)
     sums = [ctx.empty((oh, ow), np.float32) for _ in range(ns)]
+    if args.copy_calib > 0:
+        nb = args.copy_calib << 20
+        a = ctx.empty((nb // 4,), np.float32)
+        b = ctx.empty((nb // 4,), np.float32)
+        ctx.copy(b, a)
+        ctx.sync()
+        del a, b
     for _ in range(args.warmup):
         res = ctx.align_movie(frames, prm, sums=sums)
 
@@ -168,7 +191,7 @@ def main():
             "peak": PEAK_HBM_GBS,
             "unit": "GB/s",
             "frac": (achieved / PEAK_HBM_GBS) if achieved else None,
-            "traffic": None,
+            "traffic": measured_traffic(args.kernel),
             "algorithmic_bytes_per_launch": per_launch,
             "avg_launch_ms": avg_ms,
             "launches": klaunch,
@@ -176,6 +199,8 @@ def main():
         "accuracy": {"global_shift_rms_vs_truth_px": truth_rms},
     }
     if not args.no_cpu:
+        print("bench: GPU leg done (%.2f ms/movie); timing the FP64 oracle on a 5-frame sample"
+              % ms, file=sys.stderr, flush=True)
         cb, acc = cpu_baseline(recipe, kw)
         out["cpu_baseline"] = cb
         out["accuracy"].update(acc)

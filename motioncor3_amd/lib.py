@@ -193,6 +193,12 @@ class Context:
         arr = np.asarray(arr)
         return DevArray(self, arr.shape, dtype or arr.dtype).upload(arr)
 
+    def copy(self, dst, src):
+        """Device-to-device copy of src into dst (same byte size)."""
+        assert dst.nbytes == src.nbytes
+        self._check(self.lib.mc_memcpy(self.ptr, C.c_void_p(dst.ptr), C.c_void_p(src.ptr),
+                                       C.c_ulonglong(src.nbytes), 2), "mc_memcpy")
+
     # ---- stage entry points ----
     def rfft2(self, frames, scale=1.0):
         """frames: DevArray float32 [B, H, W] -> complex64 [B, W/2+1, H] ([kx][ky])."""

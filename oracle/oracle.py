@@ -178,6 +178,14 @@ def remove_spikes(sx, sy, single=False):
     return x, y
 
 
+def smooth(sx, sy, weight):
+    """CStackShift::Smooth (Align/CStackShift.cpp:180-198)."""
+    x = np.ascontiguousarray(sx, dtype=np.float64).copy()
+    y = np.ascontiguousarray(sy, dtype=np.float64).copy()
+    lib().orc_smooth(_d(x), _d(y), len(x), C.c_double(weight))
+    return x, y
+
+
 def dose_weight_sum(cmpx, cmpy, pix, kv, acc_dose):
     acc = np.ascontiguousarray(acc_dose, dtype=np.float64)
     w = np.zeros((cmpy, cmpx))

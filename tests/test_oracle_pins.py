@@ -1,0 +1,185 @@
+"""Closed-form and invariant pins of the oracle stages that the reference holds
+no vectors for: trajectory smoothing, patch placement, the feature map and the
+whole global/patch path on movies whose answer is known exactly."""
+import numpy as np
+import pytest
+
+import oracle as O
+
+
+# ---- CStackShift::Smooth (Align/CStackShift.cpp:180-198) ----
+def test_smooth_keeps_linear_and_ends():
+    t = np.arange(9.0)
+    x, y = O.smooth(3 * t - 2, -0.5 * t + 7, 0.5)
+    assert np.allclose(x, 3 * t - 2, atol=1e-12)
+    assert np.allclose(y, -0.5 * t + 7, atol=1e-12)
+
+
+@pytest.mark.parametrize("w", [0.5, 0.7])
+def test_smooth_impulse_response(w):
+    n, k = 9, 4
+    sx = np.zeros(n)
+    sx[k] = 1.0
+    sy = np.zeros(n)
+    sy[0] = 5.0  # an end point is never changed ...
+    x, y = O.smooth(sx, sy, w)
+    want = np.zeros(n)
+    want[k] = w
+    want[k - 1] = want[k + 1] = 0.5 * (1 - w)
+    assert np.allclose(x, want, atol=1e-12)
+    # ... but its neighbour sees it
+    assert y[0] == 5.0 and abs(y[1] - 0.5 * (1 - w) * 5.0) < 1e-12
+
+
+def test_smooth_weight_one_is_identity():
+    rng = np.random.default_rng(0)
+    a, b = rng.normal(size=7), rng.normal(size=7)
+    x, y = O.smooth(a, b, 1.0)
+    assert np.array_equal(x, a) and np.array_equal(y, b)
+
+
+# ---- CDetectFeatures::mFindCenters + CPatchCenters (Align/CDetectFeatures.cpp:289-396,
+#      Align/CPatchCenters.cpp:42-105) ----
+def _grid_starts(px, py, xw, xh):
+    """Every grid centre is a feature: centres stay at (i + 0.5) * 128 / p in
+    map pixels, scaled by x/128, start = (int)(centre - size/2), clamped."""
+    psx, psy = xw // px // 2 * 2, xh // py // 2 * 2
+    out = []
+    for j in range(py):
+        for i in range(px):
+            cx = (i + 0.5) * np.float32(128.0 / px) * np.float32(xw / 128.0)
+            cy = (j + 0.5) * np.float32(128.0 / py) * np.float32(xh / 128.0)
+            sx = min(max(int(cx - psx // 2), 0), xw - psx)
+            sy = min(max(int(cy - psy // 2), 0), xh - psy)
+            out.append((sx, sy))
+    return np.array(out)
+
+
+@pytest.mark.parametrize("px,py,xw,xh", [(5, 5, 2882, 2048), (7, 5, 2882, 2048),
+                                         (5, 3, 1024, 1536)])
+def test_patch_starts_all_features(px, py, xw, xh):
+    st = O.patch_starts(np.ones((128, 128), np.uint8), px, py, xw, xh)
+    assert np.array_equal(st, _grid_starts(px, py, xw, xh))
+
+
+def test_patch_starts_single_feature_taken_by_first_patch():
+    """One feature pixel at map (x=70, y=60).  Patch (2,2) (grid centre (64,64)
+    is no feature) searches the 42x42 window from (43,43) and takes the pixel
+    centre (70.5, 60.5); that marks a 20x20 area used, so patch (3,2), whose
+    window also holds the pixel, keeps its grid centre.  Every other patch
+    keeps its grid centre."""
+    px = py = 5
+    xw, xh = 2560, 1280
+    f = np.zeros((128, 128), np.uint8)
+    f[60, 70] = 1
+    st = O.patch_starts(f, px, py, xw, xh)
+    want = _grid_starts(px, py, xw, xh)
+    psx, psy = xw // px // 2 * 2, xh // py // 2 * 2
+    want[12] = (int(70.5 * (xw / 128.0) - psx // 2), int(60.5 * (xh / 128.0) - psy // 2))
+    assert np.array_equal(st, want)
+
+
+# ---- CDetectFeatures feature map (Align/CDetectFeatures.cpp:58-186) ----
+def _xcf_sum(seed, W=512, H=384):
+    rng = np.random.default_rng(seed)
+    img = rng.normal(size=(H, W))
+    k = np.fft.rfft2(img)
+    ky = np.fft.fftfreq(H)[:, None]
+    kx = np.fft.rfftfreq(W)[None, :]
+    k *= np.exp(-(kx ** 2 + ky ** 2) * 400.0)  # structure on the 128-map scale
+    return k / (W * H), W, H
+
+
+def test_feature_map_ignores_dc():
+    s, W, H = _xcf_sum(1)
+    f0 = O.feature_map(s, W, H)
+    s2 = s.copy()
+    s2[0, 0] += 37.0  # adds a constant to the image: the +(1-min) offset removes it
+    assert 0 < f0.sum() < 128 * 128
+    assert np.array_equal(O.feature_map(s2, W, H), f0)
+
+
+def test_feature_map_follows_image_translation():
+    """A circular shift of the image by (+16 map px in x, -8 in y) moves the
+    map by the same amount away from the (non-periodic) 5x5 border and the
+    clipped-statistics region; transposed axes or a sign slip would not."""
+    s, W, H = _xcf_sum(2)
+    dx, dy = 16 * W // 128, -8 * H // 128
+    ky = np.fft.fftfreq(H)[:, None]
+    kx = np.fft.rfftfreq(W)[None, :]
+    s2 = s * np.exp(-2j * np.pi * (kx * dx + ky * dy))
+    f0 = O.feature_map(s, W, H).astype(int)
+    f1 = O.feature_map(s2, W, H).astype(int)
+    moved = np.roll(np.roll(f0, 16, axis=1), -8, axis=0)
+    inner = (slice(12, 116), slice(20, 108))
+    agree = np.mean(moved[inner] == f1[inner])
+    assert agree > 0.97, agree
+
+
+# ---- whole path on movies with a known answer ----
+def _drift_movie(W, H, N, dxy, seed=5):
+    rng = np.random.default_rng(seed)
+    img = rng.normal(size=(H, W))
+    k = np.fft.rfft2(img)
+    ky = np.fft.fftfreq(H)[:, None]
+    kx = np.fft.rfftfreq(W)[None, :]
+    img = np.fft.irfft2(k * np.exp(-(kx ** 2 + ky ** 2) * 60.0), s=(H, W))
+    img = 50.0 + 10.0 * img / img.std()
+    return np.stack([np.roll(img, (int(dxy[i, 1]), int(dxy[i, 0])), axis=(0, 1))
+                     for i in range(N)]).astype(np.float32)
+
+
+def test_global_path_recovers_integer_linear_drift():
+    """Frames are exact circular integer shifts along a line: correlation
+    peaks are exact, the spline in sqrt(t) (no knots at N=8) holds c*t
+    exactly, the smoothing keeps linear data, so the logged shifts equal
+    minus the drift up to a constant (full-frame shifts are not made
+    relative to a frame) and the 0.01 px truncation."""
+    N = 8
+    t = np.arange(N)
+    d = np.stack([2 * t - 5, -3 * t + 4], axis=1).astype(float)
+    mv = _drift_movie(256, 256, N, d)
+    r = O.align_movie(mv, O.default_params(bft=(500, 100), iter=7))
+    g = r["global_shift"]
+    e = (g - g.mean(0)) + (d - d.mean(0))
+    assert np.max(np.abs(e)) < 0.02, e
+    # the sum is every frame moved by its logged (truncated) shift: content at
+    # x goes to x + s, F(k) * exp(-2 pi i k.s / n)
+    H, W = mv.shape[1:]
+    ky = np.fft.fftfreq(H)[:, None]
+    kx = np.fft.rfftfreq(W)[None, :]
+    want = sum(np.fft.irfft2(np.fft.rfft2(mv[i].astype(np.float64)) *
+                             np.exp(-2j * np.pi * (kx * g[i, 0] + ky * g[i, 1])), s=(H, W))
+               for i in range(N))
+    assert np.max(np.abs(r["sums"][0] - want)) < 1e-4 * np.max(np.abs(want))
+
+
+def test_patch_path_rigid_drift_has_no_local_motion():
+    N = 8
+    t = np.arange(N)
+    d = np.stack([t - 3, 2 * t - 7], axis=1).astype(float)
+    mv = _drift_movie(512, 512, N, d, seed=7)
+    p = O.default_params(patch=(2, 2), bft=(500, 100), iter=7)
+    r = O.align_movie(mv, p)
+    g = r["global_shift"]
+    e = (g - g.mean(0)) + (d - d.mean(0))
+    assert np.max(np.abs(e)) < 0.02
+    assert np.max(np.abs(r["patch_shift"])) < 0.05
+    assert not r["patch_bad"].any()
+
+
+def test_sums_are_linear_and_clamped():
+    """With replayed zero shifts the plain sum is the clamped sum of the
+    frames; doubling the movie doubles every sum (dose weights normalised)."""
+    rng = np.random.default_rng(11)
+    N, H, W = 6, 96, 128
+    mv = rng.normal(3.0, 1.0, size=(N, H, W)).astype(np.float32)
+    p = O.default_params(fm_dose=1.5, pix_size=1.0, kv=300)
+    z = np.zeros((N, 2))
+    r1 = O.align_movie(mv, p, replay_global=z)
+    r2 = O.align_movie(2 * mv, p, replay_global=z)
+    assert r1["sums"].shape[0] == 3
+    assert np.allclose(r1["sums"][0], np.maximum(mv.astype(np.float64).sum(0), 0), atol=1e-6)
+    assert np.allclose(r2["sums"], 2 * r1["sums"], rtol=1e-9, atol=1e-9)
+    # dose weighting leaves the DC (the frame mean) alone
+    assert abs(r1["sums"][1].mean() - r1["sums"][0].mean()) < 1e-9 * abs(r1["sums"][0].mean())
