@@ -53,6 +53,23 @@ def algorithmic_bytes(kernel, g, nch):
     raise ValueError(kernel)
 
 
+def path_bytes(g, iters_global, iters_patch, nsum, P=25):
+    """Compulsory HBM bytes of one movie through the whole path: every stage
+    reads its inputs once and writes its outputs once (FFT = one read + one
+    write), adjacent elementwise stages fused."""
+    cx, H_ = W // 2 + 1, H
+    S_f, R_f = 8.0 * cx * H_, 4.0 * W * H_
+    xw, xh = g["xcf"]
+    S_x, R_x = 8.0 * (xw // 2 + 1) * xh, 4.0 * xw * xh
+    pw, ph = g["patch"]
+    S_p, R_p = 8.0 * (pw // 2 + 1) * ph, 4.0 * pw * ph
+    b = N * (R_f + S_f) + 2 * N * S_x + iters_global * (2 * N + 2) * S_x
+    b += ((N + 1) * S_x + N * (S_x + R_x) + 2 * N * S_f + P * N * (R_p + S_p)
+          + P * iters_patch * (2 * N + 2) * S_p + N * (2 * S_f + 2 * R_f))
+    b += N * S_f + nsum * S_f
+    return b
+
+
 def measured_traffic(kernel):
     """HBM bytes per launch of `kernel` from the committed rocprofv3 FETCH_SIZE /
     WRITE_SIZE summary (copy-calibrated), or None when none was recorded."""
@@ -112,6 +129,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--kernel", default="fft_cols_fwd", help="kernel class for the roofline")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-h2d", action="store_true",
+                    help="skip timing the host-to-device upload of one movie")
     ap.add_argument("--copy-calib", type=int, default=0, metavar="MIB",
                     help="one plain device-to-device copy of MIB MiB before the steps, as the "
                          "known-size reference for hardware byte counters")
@@ -160,6 +179,18 @@ def main():
     avg_ms = kms / max(klaunch, 1)
     achieved = per_launch / (avg_ms * 1e-3) / 1e9 if klaunch else None
 
+    gi, pi = iters[-1]
+    pb = path_bytes(g, gi, pi, ns)
+    h2d = None
+    if not args.no_h2d:
+        host = frames.numpy()
+        ctx.sync()
+        t2 = time.perf_counter()
+        frames.upload(host)
+        ctx.sync()
+        h2d = (time.perf_counter() - t2) * 1000.0
+        del host
+
     gs = res["global_shift"]
     d = recipe.displacement
     e = (gs - gs.mean(0)) + (d - d.mean(0))
@@ -195,9 +226,15 @@ def main():
             "algorithmic_bytes_per_launch": per_launch,
             "avg_launch_ms": avg_ms,
             "launches": klaunch,
+            "path_bytes_per_movie": pb,
+            "path_achieved": pb / (ms * 1e-3) / 1e9,
+            "path_frac": pb / (ms * 1e-3) / 1e9 / PEAK_HBM_GBS,
         },
         "accuracy": {"global_shift_rms_vs_truth_px": truth_rms},
     }
+    if h2d is not None:
+        out["h2d"] = {"ms_per_movie": h2d, "pageable_host_memory": True,
+                      "pcie_inclusive_movies_per_s": 1000.0 / (ms + h2d)}
     if not args.no_cpu:
         print("bench: GPU leg done (%.2f ms/movie); timing the FP64 oracle on a 5-frame sample"
               % ms, file=sys.stderr, flush=True)
