@@ -29,10 +29,10 @@ hipError_t mc_launch_phasors(const double* shifts, double mult, int skip, const 
 hipError_t mc_launch_aligned_sum(const cfloat* F, const cfloat* A, const cfloat* B,
                                  const unsigned char* done, int J, int N, int cx, int H,
                                  cfloat* S, hipStream_t s);
-hipError_t mc_launch_correlate(const FftPlan1& colplan, int Cc, const cfloat* F, const cfloat* S,
-                               const cfloat* A, const cfloat* B, const IterState& st, int cx,
-                               int H, float bft, int phase_only, int seay, cfloat* T1,
-                               hipStream_t s);
+hipError_t mc_launch_correlate(const FftPlan1& colplan, const cfloat* ptw, int Cc, const cfloat* F,
+                               const cfloat* S, const cfloat* A, const cfloat* B,
+                               const IterState& st, int cx, int H, float bft, int phase_only,
+                               int seay, cfloat* T1, hipStream_t s);
 hipError_t mc_launch_row_dft(const cfloat* T1, const cfloat* twW, const unsigned char* done, int J,
                              int G, int cx, int W, int seax, int seay, float* win, hipStream_t s);
 hipError_t mc_launch_peaks(const float* win, const unsigned char* done, int J, int G, int nx,

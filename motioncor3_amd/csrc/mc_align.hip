@@ -9,7 +9,7 @@
 #include <cmath>
 
 #include "mc_align.h"
-#include "mc_fft_dev.h"
+#include "mc_fft_rp.h"
 
 #define MC_PI 3.14159265358979323846
 
@@ -124,6 +124,100 @@ __global__ __launch_bounds__(MC_NTHR) void k_correlate(CorrArgs a) {
         out[(long long)jy * a.cx + c0 + c] = cconj(lds[lp(c * a.H + lag)]);
     }
 }
+
+// GCorrelateSum2D with a register-pass column schedule: the filtered cross
+// spectrum of C columns is formed straight into the registers of pass 0, and
+// the last pass only evaluates the butterflies that own one of the seay
+// kept lags (y in [0, seay/2) and [H - seay/2, H)).
+template <int H, int C, int NT>
+__global__ __launch_bounds__(NT) void k_correlate_rp(CorrArgs a, const cfloat* __restrict__ ptw) {
+    constexpr int NP = rp_npass(H), R0 = rp_radix(H, 0), M0 = H / R0;
+    constexpr int RL = rp_radix(H, NP - 1), ML = H / RL;
+    constexpr int IT0 = (C * M0 + NT - 1) / NT, ITL = (C * ML + NT - 1) / NT;
+    extern __shared__ cfloat lds[];
+    const int j = blockIdx.z, g = blockIdx.y;
+    if (a.done && a.done[j]) return;
+    const int c0 = blockIdx.x * C;
+    const int nc = min(C, a.cx - c0);
+    const int gs = a.gstart[g], gn = a.gsize[g];
+    const long long fs = (long long)a.cx * H;
+    const cfloat* S = a.S + (long long)j * fs;
+#pragma unroll
+    for (int it = 0; it < IT0; it++) {
+        const int i = threadIdx.x + it * NT;
+        if ((C * M0) % NT == 0 || i < C * M0) {
+            const int c = i / M0, jj = i - c * M0;
+            const int kx = c0 + c;
+            cfloat v[R0];
+#pragma unroll
+            for (int t = 0; t < R0; t++) v[t] = cmk(0.f, 0.f);
+            if (c < nc) {
+                for (int f = 0; f < gn; f++) {
+                    const long long b = (long long)j * a.N + gs + f;
+                    const cfloat* Fb = a.F + b * fs + (long long)kx * H + jj;
+                    const cfloat* Bb = a.B + b * H + jj;
+                    const cfloat pa = a.A[b * a.cx + kx];
+#pragma unroll
+                    for (int t = 0; t < R0; t++)
+                        v[t] = cadd(v[t], cmul(Fb[t * M0], cmul(pa, Bb[t * M0])));
+                }
+                const cfloat* Sc = S + (long long)kx * H + jj;
+#pragma unroll
+                for (int t = 0; t < R0; t++) {
+                    const int y = jj + t * M0;
+                    const cfloat G = v[t];
+                    const cfloat sd = csub(Sc[t * M0], G);
+                    cfloat X = cmk(G.x * sd.x + G.y * sd.y, G.x * sd.y - G.y * sd.x);
+                    const int ys = (y > H / 2) ? y - H : y;
+                    float fl = expf(a.filt0 * (float)(kx * kx + ys * ys));
+                    if (a.phase_only) fl = fl / (sqrtf(X.x * X.x + X.y * X.y) + 1e-20f);
+                    X = (kx == 0 && y == 0) ? cmk(0.f, 0.f) : cscale(X, fl);
+                    v[t] = cconj(X);
+                }
+            }
+            RegDft<R0>::run(v);
+            cfloat* d = lds + c * H + jj * R0;
+#pragma unroll
+            for (int t = 0; t < R0; t++) d[t] = v[t];
+        }
+    }
+    __syncthreads();
+    RpMid<H, 1, C, NT>::run(lds, ptw);
+    cfloat* out = a.T1 + ((long long)j * a.G + g) * a.seay * a.cx;
+    const int half = a.seay / 2;
+#pragma unroll
+    for (int it = 0; it < ITL; it++) {
+        const int i = threadIdx.x + it * NT;
+        if ((C * ML) % NT == 0 || i < C * ML) {
+            const int c = i / ML, k = i - c * ML;
+            bool need = false;
+#pragma unroll
+            for (int t = 0; t < RL; t++) {
+                const int y = k + t * ML;
+                const int jy = (y < H / 2 ? y : y - H) + half;
+                need |= (jy >= 0 && jy < a.seay);
+            }
+            if (c < nc && need) {
+                cfloat v[RL];
+                const cfloat* sl = lds + c * H + k;
+#pragma unroll
+                for (int t = 0; t < RL; t++) v[t] = sl[t * ML];
+                rp_twiddle<H, NP - 1>(v, k, ptw);
+                RegDft<RL>::run(v);
+#pragma unroll
+                for (int t = 0; t < RL; t++) {
+                    const int y = k + t * ML;
+                    const int jy = (y < H / 2 ? y : y - H) + half;
+                    if (jy >= 0 && jy < a.seay) out[(long long)jy * a.cx + c0 + c] = cconj(v[t]);
+                }
+            }
+        }
+    }
+}
+
+#ifndef MC_RP_CORR_SPECS
+#define MC_RP_CORR_SPECS(X) X(2048, 4, 512) X(408, 16, 384)
+#endif
 
 // Second half of the windowed C2R: for one lag row, the Hermitian sum over
 // kx at sea_x lags, c(l) = Re T0 + (-1)^l Re T_M + 2 sum_{0<k<M} Re(T_k e^{2 pi i k l/W}).
@@ -1040,6 +1134,11 @@ int mc_align_init_constants() {
     if (mc_upload_prime_table() != 0) return -1;
     const int bytes = (mc_lds_padded(MC_LDS_ELEMS) + 64) * (int)sizeof(cfloat);
     hipFuncSetAttribute((const void*)k_correlate, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
+#define X(HH, CC, NT)                                                                     \
+    hipFuncSetAttribute((const void*)k_correlate_rp<HH, CC, NT>,                         \
+                        hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
+    MC_RP_CORR_SPECS(X)
+#undef X
     hipFuncSetAttribute((const void*)k_row_dft, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
     hipFuncSetAttribute((const void*)k_feature_map, hipFuncAttributeMaxDynamicSharedMemorySize,
                         2 * 128 * 128 * (int)sizeof(float));
@@ -1065,10 +1164,10 @@ hipError_t mc_launch_aligned_sum(const cfloat* F, const cfloat* A, const cfloat*
     return hipGetLastError();
 }
 
-hipError_t mc_launch_correlate(const FftPlan1& colplan, int Cc, const cfloat* F, const cfloat* S,
-                               const cfloat* A, const cfloat* B, const IterState& st, int cx,
-                               int H, float bft, int phase_only, int seay, cfloat* T1,
-                               hipStream_t s) {
+hipError_t mc_launch_correlate(const FftPlan1& colplan, const cfloat* ptw, int Cc, const cfloat* F,
+                               const cfloat* S, const cfloat* A, const cfloat* B,
+                               const IterState& st, int cx, int H, float bft, int phase_only,
+                               int seay, cfloat* T1, hipStream_t s) {
     CorrArgs a;
     a.plan = colplan;
     a.F = F;
@@ -1088,6 +1187,14 @@ hipError_t mc_launch_correlate(const FftPlan1& colplan, int Cc, const cfloat* F,
     a.seay = seay;
     a.T1 = T1;
     dim3 grid(mc_div_up(cx, Cc), st.G, st.J);
+#define X(HH, CC, NT)                                                                     \
+    if (H == HH && Cc == CC && ptw && seay <= HH) {                                      \
+        hipLaunchKernelGGL((k_correlate_rp<HH, CC, NT>), grid, dim3(NT),                 \
+                           (size_t)CC * HH * sizeof(cfloat), s, a, ptw);                 \
+        return hipGetLastError();                                                        \
+    }
+    MC_RP_CORR_SPECS(X)
+#undef X
     const size_t lds = (size_t)mc_lds_padded(Cc * H) * sizeof(cfloat);
     hipLaunchKernelGGL(k_correlate, grid, dim3(MC_NTHR), lds, s, a);
     return hipGetLastError();

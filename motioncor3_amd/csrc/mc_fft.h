@@ -11,6 +11,16 @@
 #pragma once
 #include "mc_common.h"
 
+// Tile shape of the two passes: C = the largest power of two <= MC_TILE_CMAX
+// whose C columns fit in LDS, R = ceil(MC_TILE_RC / C) rows per row-pass
+// workgroup (so a row pass writes R*C contiguous staging values).
+#ifndef MC_TILE_CMAX
+#define MC_TILE_CMAX 16
+#endif
+#ifndef MC_TILE_RC
+#define MC_TILE_RC 16
+#endif
+
 struct Fft2D {
     int W = 0, H = 0, m = 0;   // m = W/2 (complex row length)
     int cx = 0;                // W/2 + 1
@@ -20,6 +30,8 @@ struct Fft2D {
     cfloat* post_tw = nullptr; // m+1 entries e^{-2 pi i k / W}
     cfloat* row_tw = nullptr;  // owned device tables
     cfloat* col_tw = nullptr;
+    cfloat* col_ptw = nullptr;  // register-pass twiddles of the column length (or null)
+    cfloat* row_ptw = nullptr;  // register-pass twiddles of the row length m (or null)
     long long t_elems = 0;     // T elements per transform
     long long s_elems = 0;     // S elements per transform (cx * H)
 };

@@ -10,6 +10,7 @@
 
 #include "mc_fft.h"
 #include "mc_fft_ct.h"
+#include "mc_fft_rp.h"
 
 namespace {
 
@@ -575,10 +576,324 @@ __global__ __launch_bounds__(NT) void k_cols_inv_ct(ColInvArgs a) {
 // (row length M, rows per workgroup, column block) of the reference
 // geometries: frames 5760x4092, 4096^2, 1024^2; xcf 2882x2048, 2048^2;
 // patches 576x408; the 128^2 feature map.
+#ifndef MC_ROW_SPECS
 #define MC_ROW_SPECS(X) \
-    X(2880, 4, 4) X(2048, 4, 4) X(512, 2, 16) X(1441, 2, 8) X(1024, 2, 8) X(288, 2, 16) X(64, 2, 16)
+    X(2880, 2, 2) X(2048, 2, 2) X(512, 2, 8) X(1441, 2, 4) X(1024, 2, 8) X(288, 2, 16) X(64, 2, 16)
+#endif
+#ifndef MC_COL_SPECS
 #define MC_COL_SPECS(X) \
-    X(4092, 4, 576) X(4096, 4, 512) X(1024, 16, 512) X(2048, 8, 512) X(408, 16, 512) X(128, 16, 512)
+    X(4092, 2, 384) X(4096, 2, 512) X(1024, 8, 512) X(2048, 4, 512) X(408, 16, 512) X(128, 16, 512)
+#endif
+
+// ---------------------------------------------------------------------------
+// Column passes with a register-pass schedule (mc_fft_rp.h), C columns per
+// workgroup held unpadded in LDS (C*H*8 bytes).  Forward: staging T -> S
+// (+ the xcf crop X); inverse: S (phase factors, conjugated) -> T.  Same
+// results as the stage-by-stage kernels up to FP32 rounding order.
+
+// Output row of a cropped (xcy-row) spectrum that input row y maps to, or -1
+// (the inverse of resize_row for xcy < H).
+__device__ __forceinline__ int crop_row(int y, int xcy, int H) {
+    if (y <= xcy / 2) return y;
+    const int d = H - xcy;
+    return (y > xcy / 2 + d) ? y - d : -1;
+}
+
+template <int H, int C, int NT>
+__global__ __launch_bounds__(NT) void k_cols_fwd_rp(ColFwdArgs a, const cfloat* __restrict__ ptw) {
+    constexpr int NP = rp_npass(H);
+    constexpr int R0 = rp_radix(H, 0), M0 = H / R0;
+    constexpr int RL = rp_radix(H, NP - 1), ML = H / RL;
+    constexpr int IT0 = (C * M0 + NT - 1) / NT, ITL = (C * ML + NT - 1) / NT;
+    extern __shared__ cfloat lds[];
+    const int ncb = (a.ncols + C - 1) / C;
+    const int tt = xcd_tile(blockIdx.x, gridDim.x);
+    const int b = tt / ncb, kb = tt - b * ncb, c0 = kb * C;
+    const int nc = min(C, a.ncols - c0);
+    const cfloat* src = a.T + (long long)b * a.t_bstride + (long long)kb * H * C;
+    // pass 0: staging [y][c] -> registers -> LDS (items: column fastest)
+#pragma unroll
+    for (int it = 0; it < IT0; it++) {
+        const int i = threadIdx.x + it * NT;
+        if ((C * M0) % NT == 0 || i < C * M0) {
+            const int c = i % C, j = i / C;
+            cfloat v[R0];
+            const bool ok = c < nc;
+#pragma unroll
+            for (int t = 0; t < R0; t++) v[t] = ok ? src[(j + t * M0) * C + c] : cmk(0.f, 0.f);
+            RegDft<R0>::run(v);
+            cfloat* d = lds + c * H + j * R0;
+#pragma unroll
+            for (int t = 0; t < R0; t++) d[t] = v[t];
+        }
+    }
+    __syncthreads();
+    RpMid<H, 1, C, NT>::run(lds, ptw);
+    // last pass: LDS -> registers -> S (column-major) and the crop X
+#pragma unroll
+    for (int it = 0; it < ITL; it++) {
+        const int i = threadIdx.x + it * NT;
+        if ((C * ML) % NT == 0 || i < C * ML) {
+            const int c = i / ML, k = i - c * ML;
+            if (c < nc) {
+                cfloat v[RL];
+                const cfloat* sl = lds + c * H + k;
+#pragma unroll
+                for (int t = 0; t < RL; t++) v[t] = sl[t * ML];
+                rp_twiddle<H, NP - 1>(v, k, ptw);
+                RegDft<RL>::run(v);
+                const int kx = c0 + c;
+                if (a.S) {
+                    cfloat* d = a.S + (long long)b * a.s_bstride + (long long)kx * H + k;
+#pragma unroll
+                    for (int t = 0; t < RL; t++) d[t * ML] = v[t];
+                }
+                if (a.X && kx < a.xcx) {
+                    cfloat* xd = a.X + (long long)b * a.x_bstride + (long long)kx * a.xcy;
+#pragma unroll
+                    for (int t = 0; t < RL; t++) {
+                        const int yo = crop_row(k + t * ML, a.xcy, H);
+                        if (yo >= 0) xd[yo] = v[t];
+                    }
+                }
+            }
+        }
+    }
+}
+
+template <int H, int C, int NT>
+__global__ __launch_bounds__(NT) void k_cols_inv_rp(ColInvArgs a, const cfloat* __restrict__ ptw) {
+    constexpr int NP = rp_npass(H);
+    constexpr int R0 = rp_radix(H, 0), M0 = H / R0;
+    constexpr int RL = rp_radix(H, NP - 1), ML = H / RL;
+    constexpr int IT0 = (C * M0 + NT - 1) / NT, ITL = (C * ML + NT - 1) / NT;
+    extern __shared__ cfloat lds[];
+    const int ncb = (a.ncols + C - 1) / C;
+    const int tt = xcd_tile(blockIdx.x, gridDim.x);
+    const int b = tt / ncb, kb = tt - b * ncb, c0 = kb * C;
+    const int nc = min(C, a.ncols - c0);
+    const cfloat* sb = a.S + (long long)b * a.s_bstride;
+    // pass 0: spectrum columns (phase factors, conjugated) -> LDS (items: row fastest)
+#pragma unroll
+    for (int it = 0; it < IT0; it++) {
+        const int i = threadIdx.x + it * NT;
+        if ((C * M0) % NT == 0 || i < C * M0) {
+            const int c = i / M0, j = i - c * M0;
+            const int kx = c0 + c;
+            cfloat v[R0];
+            if (c < nc) {
+                const cfloat* sc = sb + (long long)kx * H + j;
+#pragma unroll
+                for (int t = 0; t < R0; t++) v[t] = sc[t * M0];
+                if (a.phA) {
+                    const cfloat pa = a.phA[(long long)b * a.ncols + kx];
+                    const cfloat* pb = a.phB + (long long)b * H + j;
+#pragma unroll
+                    for (int t = 0; t < R0; t++) v[t] = cmul(v[t], cmul(pa, pb[t * M0]));
+                }
+#pragma unroll
+                for (int t = 0; t < R0; t++) v[t] = cconj(v[t]);
+            } else {
+#pragma unroll
+                for (int t = 0; t < R0; t++) v[t] = cmk(0.f, 0.f);
+            }
+            RegDft<R0>::run(v);
+            cfloat* d = lds + c * H + j * R0;
+#pragma unroll
+            for (int t = 0; t < R0; t++) d[t] = v[t];
+        }
+    }
+    __syncthreads();
+    RpMid<H, 1, C, NT>::run(lds, ptw);
+    // last pass: LDS -> registers -> staging [y][c] (items: column fastest)
+    cfloat* dst = a.T + (long long)b * a.t_bstride + (long long)kb * H * C;
+#pragma unroll
+    for (int it = 0; it < ITL; it++) {
+        const int i = threadIdx.x + it * NT;
+        if ((C * ML) % NT == 0 || i < C * ML) {
+            const int c = i % C, k = i / C;
+            if (c < nc) {
+                cfloat v[RL];
+                const cfloat* sl = lds + c * H + k;
+#pragma unroll
+                for (int t = 0; t < RL; t++) v[t] = sl[t * ML];
+                rp_twiddle<H, NP - 1>(v, k, ptw);
+                RegDft<RL>::run(v);
+#pragma unroll
+                for (int t = 0; t < RL; t++) dst[(k + t * ML) * C + c] = cconj(v[t]);
+            }
+        }
+    }
+}
+
+#ifndef MC_RP_COL_SPECS
+#define MC_RP_COL_SPECS(X) X(4092, 2, 264) X(4096, 2, 256) X(2048, 4, 512) X(408, 16, 384)
+#endif
+
+// ---------------------------------------------------------------------------
+// Row passes with a register-pass schedule for the complex row length M,
+// RR rows per workgroup held unpadded in LDS.  R2C: real rows -> registers
+// (pass 0) -> LDS passes -> post-processing -> staging T.  C2R: staging T ->
+// LDS -> pre-processing -> LDS passes -> last pass -> real rows.
+
+template <int M, int RR, int C, int NT>
+__global__ __launch_bounds__(NT) void k_rows_r2c_rp(RowR2CArgs a, const cfloat* __restrict__ ptw) {
+    constexpr int NP = rp_npass(M), R0 = rp_radix(M, 0), M0 = M / R0;
+    constexpr int IT0 = (RR * M0 + NT - 1) / NT;
+    constexpr int NB = (M + 1 + C - 1) / C;
+    constexpr int TOT = NB * RR * C;
+    extern __shared__ cfloat lds[];
+    const int nrb = (a.H + RR - 1) / RR;
+    const int tt = xcd_tile(blockIdx.x, gridDim.x);
+    const int b = tt / nrb;
+    const int row0 = (tt - b * nrb) * RR;
+    const int nrows = min(RR, a.H - row0);
+    const int img = a.in_mod > 0 ? b % a.in_mod : b;
+    const int wi = a.in_mod > 0 ? b / a.in_mod : b;
+    int x0 = 0, y0 = 0;
+    if (a.win) {
+        x0 = a.win[2 * wi];
+        y0 = a.win[2 * wi + 1];
+    }
+    const bool vec = ((x0 | (int)(a.pitch & 1) | (int)(a.in_bstride & 1)) & 1) == 0 &&
+                     (((uintptr_t)a.in) & 7) == 0;
+    const float* base = a.in + (long long)img * a.in_bstride + (long long)(y0 + row0) * a.pitch + x0;
+#pragma unroll
+    for (int it = 0; it < IT0; it++) {
+        const int i = threadIdx.x + it * NT;
+        if ((RR * M0) % NT == 0 || i < RR * M0) {
+            const int r = i / M0, j = i - r * M0;
+            cfloat v[R0];
+            if (r < nrows) {
+                const float* src = base + (long long)r * a.pitch + 2 * j;
+#pragma unroll
+                for (int t = 0; t < R0; t++) {
+                    const float2 f = vec ? *reinterpret_cast<const float2*>(src + 2 * t * M0)
+                                         : make_float2(src[2 * t * M0], src[2 * t * M0 + 1]);
+                    v[t] = cmk(f.x, f.y);
+                }
+            } else {
+#pragma unroll
+                for (int t = 0; t < R0; t++) v[t] = cmk(0.f, 0.f);
+            }
+            RegDft<R0>::run(v);
+            cfloat* d = lds + r * M + j * R0;
+#pragma unroll
+            for (int t = 0; t < R0; t++) d[t] = v[t];
+        }
+    }
+    __syncthreads();
+    RpRange<M, 1, NP, RR, NT>::run(lds, ptw);
+    cfloat* tb = a.T + (long long)b * a.t_bstride + (long long)row0 * C;
+    const long long kbs = (long long)a.H * C;
+#pragma unroll 4
+    for (int e = threadIdx.x; e < TOT; e += NT) {
+        const int c = e % C, r = (e / C) % RR, kb = e / (C * RR);
+        const int k = kb * C + c;
+        if (k > M || r >= nrows) continue;
+        const cfloat zk = lds[r * M + (k == M ? 0 : k)];
+        const cfloat zm = lds[r * M + (k == 0 ? 0 : M - k)];
+        const cfloat E = cmk(0.5f * (zk.x + zm.x), 0.5f * (zk.y - zm.y));
+        const cfloat D = cmk(zk.x - zm.x, zk.y + zm.y);
+        const cfloat O = cmk(0.5f * D.y, -0.5f * D.x);
+        const cfloat X = cadd(E, cmul(a.post_tw[k], O));
+        tb[kb * kbs + r * C + c] = cscale(X, a.scale);
+    }
+}
+
+template <int M, int RR, int C, int NT>
+__global__ __launch_bounds__(NT) void k_rows_c2r_rp(RowC2RArgs a, const cfloat* __restrict__ ptw) {
+    constexpr int NP = rp_npass(M);
+    constexpr int RL = rp_radix(M, NP - 1), ML = M / RL;
+    constexpr int ITL = (RR * ML + NT - 1) / NT;
+    constexpr int NB = (M + 1 + C - 1) / C;
+    constexpr int TOT = NB * RR * C;
+    constexpr int HALF = M / 2 + 1;
+    extern __shared__ cfloat lds[];
+    const int nrb = (a.H + RR - 1) / RR;
+    const int tt = xcd_tile(blockIdx.x, gridDim.x);
+    const int b = tt / nrb;
+    const int row0 = (tt - b * nrb) * RR;
+    const int nrows = min(RR, a.H - row0);
+    const cfloat* tb = a.T + (long long)b * a.t_bstride + (long long)row0 * C;
+    const long long kbs = (long long)a.H * C;
+    for (int e0 = threadIdx.x; e0 < TOT; e0 += NT * MC_IO_UNROLL) {
+        cfloat v[MC_IO_UNROLL];
+        int dst[MC_IO_UNROLL];
+#pragma unroll
+        for (int u = 0; u < MC_IO_UNROLL; u++) {
+            const int e = e0 + u * NT;
+            dst[u] = -1;
+            if (e < TOT) {
+                const int c = e % C, r = (e / C) % RR, kb = e / (C * RR);
+                const int k = kb * C + c;
+                if (k <= M && r < nrows) {
+                    v[u] = tb[kb * kbs + r * C + c];
+                    dst[u] = (k < M) ? r * M + k : RR * M + r;
+                }
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < MC_IO_UNROLL; u++)
+            if (dst[u] >= 0) lds[dst[u]] = v[u];
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < RR * HALF; i += NT) {
+        const int g = i / HALF, k = i - g * HALF;
+        const int row = g * M;
+        if (k == 0) {
+            const float x0 = lds[row].x, xm = lds[RR * M + g].x;
+            lds[row] = cconj(cmk(x0 + xm, x0 - xm));
+            continue;
+        }
+        const int k2 = M - k;
+        if (k2 < k) continue;
+        const cfloat xa = lds[row + k], xb = lds[row + k2];
+        const cfloat t1 = cmul(cconj(a.post_tw[k]), cmk(xa.x - xb.x, xa.y + xb.y));
+        lds[row + k] = cconj(cmk(xa.x + xb.x - t1.y, xa.y - xb.y + t1.x));
+        if (k2 != k) {
+            const cfloat t2 = cmul(cconj(a.post_tw[k2]), cmk(xb.x - xa.x, xb.y + xa.y));
+            lds[row + k2] = cconj(cmk(xb.x + xa.x - t2.y, xb.y - xa.y + t2.x));
+        }
+    }
+    __syncthreads();
+    RpRange<M, 0, NP - 1, RR, NT>::run(lds, ptw);
+    const bool vec = ((a.pitch | a.out_bstride) & 1) == 0 && (((uintptr_t)a.out) & 7) == 0;
+#pragma unroll
+    for (int it = 0; it < ITL; it++) {
+        const int i = threadIdx.x + it * NT;
+        if ((RR * ML) % NT == 0 || i < RR * ML) {
+            const int r = i / ML, k = i - r * ML;
+            if (r < nrows) {
+                cfloat v[RL];
+                const cfloat* sl = lds + r * M + k;
+#pragma unroll
+                for (int t = 0; t < RL; t++) v[t] = sl[t * ML];
+                rp_twiddle<M, NP - 1>(v, k, ptw);
+                RegDft<RL>::run(v);
+                float* ob = a.out + (long long)b * a.out_bstride + (long long)(row0 + r) * a.pitch;
+#pragma unroll
+                for (int t = 0; t < RL; t++) {
+                    const int y = k + t * ML;
+                    float v0 = v[t].x, v1 = -v[t].y;
+                    if (a.clamp) {
+                        v0 = fmaxf(v0, 0.f);
+                        v1 = fmaxf(v1, 0.f);
+                    }
+                    if (vec) *reinterpret_cast<float2*>(ob + 2 * y) = make_float2(v0, v1);
+                    else {
+                        ob[2 * y] = v0;
+                        ob[2 * y + 1] = v1;
+                    }
+                }
+            }
+        }
+    }
+}
+
+#ifndef MC_RP_ROW_SPECS
+#define MC_RP_ROW_SPECS(X) X(2880, 2, 2, 512) X(2048, 2, 2, 512)
+#endif
 
 template <class F>
 static void set_lds_attr(F* f) {
@@ -592,6 +907,12 @@ static void set_ct_attrs() {
 #undef X
 #define X(H, C, NT) set_lds_attr(k_cols_fwd_ct<H, C, NT>); set_lds_attr(k_cols_inv_ct<H, C, NT>);
     MC_COL_SPECS(X)
+#undef X
+#define X(H, C, NT) set_lds_attr(k_cols_fwd_rp<H, C, NT>); set_lds_attr(k_cols_inv_rp<H, C, NT>);
+    MC_RP_COL_SPECS(X)
+#undef X
+#define X(M, RR, C, NT) set_lds_attr(k_rows_r2c_rp<M, RR, C, NT>); set_lds_attr(k_rows_c2r_rp<M, RR, C, NT>);
+    MC_RP_ROW_SPECS(X)
 #undef X
 }
 
@@ -676,14 +997,16 @@ int mc_fft2d_create(Fft2D* p, int W, int H) {
     p->H = H;
     p->m = m;
     p->cx = m + 1;
-    int C = 16;
-    while (C > 1 && (long long)C * H > MC_LDS_ELEMS) C /= 2;
+    // half the LDS per workgroup, so that two workgroups share a CU
+    const int budget = MC_LDS_ELEMS / 2;
+    int C = MC_TILE_CMAX;
+    while (C > 1 && (long long)C * H > budget) C /= 2;
     p->C = C;
-    int R = (16 + C - 1) / C;
-    if (R < 2) R = 2;
-    const int rmax = MC_LDS_ELEMS / (m + 1);
-    if (R > rmax) R = rmax;
-    if (R < 1) return -1;
+    int R = 1;
+    const int want = std::max((MC_TILE_RC + C - 1) / C, 2);
+    const int per_row = m + 1 + ct_scratch_host(m);
+    while (R * 2 <= want && (R * 2) * per_row <= budget) R *= 2;
+    if (R * per_row > MC_LDS_ELEMS) return -1;
     p->R = R;
     if (build_plan1(&p->row, &p->row_tw, m) != 0) return -2;
     if (build_plan1(&p->col, &p->col_tw, H) != 0) return -2;
@@ -695,6 +1018,13 @@ int mc_fft2d_create(Fft2D* p, int W, int H) {
     }
     if (hipMalloc(&p->post_tw, pt.size() * sizeof(cfloat)) != hipSuccess) return -2;
     hipMemcpy(p->post_tw, pt.data(), pt.size() * sizeof(cfloat), hipMemcpyHostToDevice);
+    for (int dim = 0; dim < 2; dim++) {
+        const std::vector<cfloat> tw = rp_twiddle_table(dim ? H : m);
+        if (tw.empty()) continue;
+        cfloat** dstp = dim ? &p->col_ptw : &p->row_ptw;
+        if (hipMalloc(dstp, tw.size() * sizeof(cfloat)) != hipSuccess) return -2;
+        hipMemcpy(*dstp, tw.data(), tw.size() * sizeof(cfloat), hipMemcpyHostToDevice);
+    }
     const int nb = (m + 1 + C - 1) / C;
     p->t_elems = (long long)nb * H * C;
     p->s_elems = (long long)(m + 1) * H;
@@ -705,7 +1035,9 @@ void mc_fft2d_destroy(Fft2D* p) {
     if (p->row_tw) hipFree(p->row_tw);
     if (p->col_tw) hipFree(p->col_tw);
     if (p->post_tw) hipFree(p->post_tw);
-    p->row_tw = p->col_tw = p->post_tw = nullptr;
+    if (p->col_ptw) hipFree(p->col_ptw);
+    if (p->row_ptw) hipFree(p->row_ptw);
+    p->row_tw = p->col_tw = p->post_tw = p->col_ptw = p->row_ptw = nullptr;
 }
 
 hipError_t mc_rows_r2c(const Fft2D& p, const float* in, long long pitch, long long in_bstride,
@@ -734,6 +1066,14 @@ hipError_t mc_rows_r2c_mod(const Fft2D& p, const float* in, long long pitch, lon
     a.scale = scale;
     dim3 grid(mc_div_up(p.H, p.R) * batch);
     const size_t lds = (size_t)mc_lds_padded(p.R * p.m) * sizeof(cfloat);
+#define X(MM, RRR, CCC, NT)                                                                 \
+    if (p.m == MM && p.R == RRR && p.C == CCC && p.row_ptw) {                              \
+        hipLaunchKernelGGL((k_rows_r2c_rp<MM, RRR, CCC, NT>), grid, dim3(NT),              \
+                           (size_t)RRR * MM * sizeof(cfloat), s, a, (const cfloat*)p.row_ptw); \
+        return hipGetLastError();                                                          \
+    }
+    MC_RP_ROW_SPECS(X)
+#undef X
 #define X(MM, RRR, CCC)                                                             \
     if (p.m == MM && p.R == RRR && p.C == CCC) {                                    \
         const size_t ldsct =                                                        \
@@ -765,6 +1105,15 @@ hipError_t mc_rows_c2r(const Fft2D& p, const cfloat* T, long long t_bstride, flo
     a.clamp = clamp;
     dim3 grid(mc_div_up(p.H, p.R) * batch);
     const size_t lds = (size_t)mc_lds_padded(p.R * (p.m + 1)) * sizeof(cfloat);
+#define X(MM, RRR, CCC, NT)                                                                 \
+    if (p.m == MM && p.R == RRR && p.C == CCC && p.row_ptw) {                              \
+        hipLaunchKernelGGL((k_rows_c2r_rp<MM, RRR, CCC, NT>), grid, dim3(NT),              \
+                           (size_t)RRR * (MM + 1) * sizeof(cfloat), s, a,                  \
+                           (const cfloat*)p.row_ptw);                                      \
+        return hipGetLastError();                                                          \
+    }
+    MC_RP_ROW_SPECS(X)
+#undef X
 #define X(MM, RRR, CCC)                                                             \
     if (p.m == MM && p.R == RRR && p.C == CCC) {                                    \
         const size_t ldsct =                                                        \
@@ -796,6 +1145,14 @@ hipError_t mc_cols_fwd(const Fft2D& p, const cfloat* T, long long t_bstride, cfl
     a.x_bstride = x_bstride;
     dim3 grid(mc_div_up(p.cx, p.C) * batch);
     const size_t lds = (size_t)mc_lds_padded(p.C * p.H) * sizeof(cfloat);
+#define X(HH, CC, NT)                                                                       \
+    if (p.H == HH && p.C == CC && p.col_ptw) {                                             \
+        hipLaunchKernelGGL((k_cols_fwd_rp<HH, CC, NT>), grid, dim3(NT),                    \
+                           (size_t)CC * HH * sizeof(cfloat), s, a, (const cfloat*)p.col_ptw); \
+        return hipGetLastError();                                                          \
+    }
+    MC_RP_COL_SPECS(X)
+#undef X
 #define X(HH, CC, NT)                                                             \
     if (p.H == HH && p.C == CC) {                                                 \
         hipLaunchKernelGGL((k_cols_fwd_ct<HH, CC, NT>), grid, dim3(NT), lds, s, a);  \
@@ -825,6 +1182,15 @@ hipError_t mc_cols_inv(const Fft2D& p, const cfloat* S, long long s_bstride, int
     a.t_bstride = t_bstride;
     dim3 grid(mc_div_up(p.cx, p.C) * batch);
     const size_t lds = (size_t)mc_lds_padded(p.C * p.H) * sizeof(cfloat);
+    const bool same = (src_cx == p.cx && src_cy == p.H);
+#define X(HH, CC, NT)                                                                       \
+    if (p.H == HH && p.C == CC && p.col_ptw && same) {                                     \
+        hipLaunchKernelGGL((k_cols_inv_rp<HH, CC, NT>), grid, dim3(NT),                    \
+                           (size_t)CC * HH * sizeof(cfloat), s, a, (const cfloat*)p.col_ptw); \
+        return hipGetLastError();                                                          \
+    }
+    MC_RP_COL_SPECS(X)
+#undef X
 #define X(HH, CC, NT)                                                             \
     if (p.H == HH && p.C == CC) {                                                 \
         hipLaunchKernelGGL((k_cols_inv_ct<HH, CC, NT>), grid, dim3(NT), lds, s, a);  \

@@ -157,6 +157,7 @@ __host__ __device__ constexpr int ct_scratch(int n) {
     const int p = ct_big_prime(n);
     return p ? (n / p) * (p - 1) + p : 0;
 }
+inline int ct_scratch_host(int n) { return ct_scratch(n); }
 
 // Prime radix above 31, symmetric-pair form: phase 1 forms
 // u_t = x_t + x_{P-t}, v_t = x_t - x_{P-t} (twiddled) of every butterfly in

@@ -1,0 +1,244 @@
+// mc_fft_rp.h -- register-pass FFT schedules (Stockham with composite radices).
+//
+// A length-N transform is done in a few passes of radix R_p (N = R_0 R_1 ...).
+// Each work item of a pass holds one whole radix-R_p butterfly in registers,
+// computed by a composite register DFT (RegDft), so a length like
+// 4092 = 31 * 11 * 12 needs three passes and two LDS exchanges instead of one
+// LDS round trip per prime factor.  The first pass reads its inputs straight
+// from global memory and the last pass writes straight to global memory.
+// LDS holds unpadded columns: an element's address is the work item's base
+// plus a compile-time offset, so there is no per-element index arithmetic.
+//
+// Pass p (sub-transform size NS_p = R_0 ... R_{p-1}, M_p = N / R_p items):
+//   item j = q NS_p + k reads x[j + t M_p] (t < R_p), multiplies input t by
+//   W_{NS_p R_p}^{k t} (forward sign), transforms, and writes output t to
+//   q NS_p R_p + k + t NS_p.  For the last pass NS_p R_p = N, so q = 0.
+// Twiddles of pass p >= 1 are stored [t-1][k] (lanes with consecutive k read
+// consecutive entries) at rp_twoff(N, p).
+#pragma once
+#include <cmath>
+#include <vector>
+#include "mc_fft_ct.h"
+
+// ---- compile-time trigonometry (Taylor series in double, |x| <= pi) ----
+__host__ __device__ constexpr double rp_pi() { return 3.14159265358979323846; }
+__host__ __device__ constexpr double rp_sin_red(double x) {
+    double term = x, sum = x;
+    for (int n = 1; n < 30; n++) {
+        term *= -x * x / ((2.0 * n) * (2.0 * n + 1.0));
+        sum += term;
+    }
+    return sum;
+}
+__host__ __device__ constexpr double rp_cos_red(double x) {
+    double term = 1.0, sum = 1.0;
+    for (int n = 1; n < 30; n++) {
+        term *= -x * x / ((2.0 * n - 1.0) * (2.0 * n));
+        sum += term;
+    }
+    return sum;
+}
+// cos and sin of 2 pi a / b for integers, argument reduced to [-pi, pi]
+__host__ __device__ constexpr double rp_cos2pi(int a, int b) {
+    int r = a % b;
+    if (r < 0) r += b;
+    if (2 * r > b) r -= b;
+    return rp_cos_red(2.0 * rp_pi() * r / b);
+}
+__host__ __device__ constexpr double rp_sin2pi(int a, int b) {
+    int r = a % b;
+    if (r < 0) r += b;
+    if (2 * r > b) r -= b;
+    return rp_sin_red(2.0 * rp_pi() * r / b);
+}
+
+// ---- composite register DFT: v[k] = sum_t v[t] e^{-2 pi i t k / R} ----
+__host__ __device__ constexpr int rp_first(int r) {
+    // leading factor of a composite register DFT: 4 or 2 for even sizes
+    // (small butterflies), else the smallest odd prime
+    return r % 4 == 0 && r != 4 && r != 8 ? 4 : (r % 2 == 0 && r != 2 && r != 4 && r != 8 ? 2 : ct_spf(r));
+}
+__host__ __device__ constexpr bool rp_leaf(int r) {
+    return r == 1 || r == 2 || r == 4 || r == 8 || (ct_spf(r) == r && r <= 31);
+}
+
+template <int R, bool LEAF = rp_leaf(R)>
+struct RegDft;
+
+template <int R>
+struct RegDft<R, true> {
+    static __device__ __forceinline__ void run(cfloat* v) {
+        if constexpr (R > 1) bfly<R>(v);
+    }
+};
+
+// cos/sin(2 pi i / R) for a composite register DFT, built at compile time.
+struct RpTab {
+    float c[64];
+    float s[64];
+};
+template <int R>
+__host__ __device__ constexpr RpTab rp_tab() {
+    RpTab t{};
+    for (int i = 0; i < R && i < 64; i++) {
+        t.c[i] = (float)rp_cos2pi(i, R);
+        t.s[i] = (float)rp_sin2pi(i, R);
+    }
+    return t;
+}
+
+// R = A * B: A-point DFTs over n1 (stride B), twiddle W_R^{n2 k1},
+// B-point DFTs over n2; output k1 + A k2.
+template <int R>
+struct RegDft<R, false> {
+    static_assert(R <= 64, "composite register DFTs up to 64 points");
+    static __device__ __forceinline__ void run(cfloat* v) {
+        constexpr int A = rp_first(R), B = R / A;
+        constexpr RpTab TW = rp_tab<R>();
+        cfloat w[R];
+#pragma unroll
+        for (int n2 = 0; n2 < B; n2++) {
+            cfloat s[A];
+#pragma unroll
+            for (int n1 = 0; n1 < A; n1++) s[n1] = v[n2 + B * n1];
+            RegDft<A>::run(s);
+#pragma unroll
+            for (int k1 = 0; k1 < A; k1++) {
+                if (n2 == 0 || k1 == 0) {
+                    w[n2 * A + k1] = s[k1];
+                } else {
+                    const float c = TW.c[(n2 * k1) % R];
+                    const float sn = TW.s[(n2 * k1) % R];
+                    // multiply by e^{-i theta} = (c, -sn)
+                    w[n2 * A + k1] = cmk(s[k1].x * c + s[k1].y * sn, s[k1].y * c - s[k1].x * sn);
+                }
+            }
+        }
+#pragma unroll
+        for (int k1 = 0; k1 < A; k1++) {
+            cfloat s[B];
+#pragma unroll
+            for (int n2 = 0; n2 < B; n2++) s[n2] = w[n2 * A + k1];
+            RegDft<B>::run(s);
+#pragma unroll
+            for (int k2 = 0; k2 < B; k2++) v[k1 + A * k2] = s[k2];
+        }
+    }
+};
+
+// ---- pass schedules ----
+// Radix of pass p for the lengths with a register-pass schedule (0 = none).
+__host__ __device__ constexpr int rp_radix(int n, int p) {
+    int r0 = 0, r1 = 0, r2 = 0, r3 = 0;
+    if (n == 4092) { r0 = 31; r1 = 11; r2 = 12; }
+    else if (n == 4096) { r0 = 16; r1 = 16; r2 = 16; }
+    else if (n == 2048) { r0 = 16; r1 = 16; r2 = 8; }
+    else if (n == 2880) { r0 = 15; r1 = 12; r2 = 16; }
+    else if (n == 408) { r0 = 17; r1 = 24; }
+    return p == 0 ? r0 : p == 1 ? r1 : p == 2 ? r2 : p == 3 ? r3 : 0;
+}
+__host__ __device__ constexpr int rp_npass(int n) {
+    int c = 0;
+    while (c < 4 && rp_radix(n, c) != 0) c++;
+    return c;
+}
+__host__ __device__ constexpr bool rp_has(int n) {
+    if (rp_npass(n) < 2) return false;
+    long long prod = 1;
+    for (int p = 0; p < rp_npass(n); p++) prod *= rp_radix(n, p);
+    return prod == n;
+}
+__host__ __device__ constexpr int rp_ns(int n, int p) {
+    int ns = 1;
+    for (int i = 0; i < p; i++) ns *= rp_radix(n, i);
+    return ns;
+}
+__host__ __device__ constexpr int rp_twoff(int n, int p) {
+    int off = 0;
+    for (int i = 1; i < p; i++) off += rp_ns(n, i) * (rp_radix(n, i) - 1);
+    return off;
+}
+__host__ __device__ constexpr int rp_twcount(int n) { return rp_twoff(n, rp_npass(n)); }
+
+// Multiply input t (1 <= t < R) of an item with twiddle index k.
+template <int N, int P>
+static __device__ __forceinline__ void rp_twiddle(cfloat* v, int k, const cfloat* __restrict__ tw) {
+    constexpr int R = rp_radix(N, P), NS = rp_ns(N, P), TWO = rp_twoff(N, P);
+    if constexpr (NS > 1) {
+#pragma unroll
+        for (int t = 1; t < R; t++) v[t] = cmul(v[t], tw[TWO + (t - 1) * NS + k]);
+    }
+}
+
+// A middle pass over the C columns held in LDS at col0 + c*N, NT threads;
+// item i -> column i / M, butterfly i % M.  Ends with a barrier.
+template <int N, int P, int C, int NT>
+static __device__ __forceinline__ void rp_pass_lds(cfloat* lds, const cfloat* __restrict__ tw) {
+    constexpr int R = rp_radix(N, P), NS = rp_ns(N, P), M = N / R;
+    constexpr int IT = (C * M + NT - 1) / NT;
+    cfloat v[IT][R];
+    int dst[IT];
+#pragma unroll
+    for (int it = 0; it < IT; it++) {
+        const int i = threadIdx.x + it * NT;
+        dst[it] = -1;
+        if ((C * M) % NT == 0 || i < C * M) {
+            const int c = i / M, j = i - c * M;
+            const int k = j % NS, q = j / NS;
+            const cfloat* src = lds + c * N + j;
+#pragma unroll
+            for (int t = 0; t < R; t++) v[it][t] = src[t * M];
+            rp_twiddle<N, P>(v[it], k, tw);
+            RegDft<R>::run(v[it]);
+            dst[it] = c * N + q * NS * R + k;
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int it = 0; it < IT; it++) {
+        if (dst[it] >= 0) {
+            cfloat* d = lds + dst[it];
+#pragma unroll
+            for (int t = 0; t < R; t++) d[t * NS] = v[it][t];
+        }
+    }
+    __syncthreads();
+}
+
+template <int N, int P, int C, int NT>
+struct RpMid {
+    static __device__ __forceinline__ void run(cfloat* lds, const cfloat* tw) {
+        if constexpr (P < rp_npass(N) - 1) {
+            rp_pass_lds<N, P, C, NT>(lds, tw);
+            RpMid<N, P + 1, C, NT>::run(lds, tw);
+        }
+    }
+};
+
+// Passes P .. PEND-1 in LDS.
+template <int N, int P, int PEND, int C, int NT>
+struct RpRange {
+    static __device__ __forceinline__ void run(cfloat* lds, const cfloat* tw) {
+        if constexpr (P < PEND) {
+            rp_pass_lds<N, P, C, NT>(lds, tw);
+            RpRange<N, P + 1, PEND, C, NT>::run(lds, tw);
+        }
+    }
+};
+
+// Host: the twiddle table of a register-pass schedule (forward sign), or an
+// empty vector when n has none.
+inline std::vector<cfloat> rp_twiddle_table(int n) {
+    std::vector<cfloat> tw;
+    if (!rp_has(n)) return tw;
+    tw.resize(rp_twcount(n) > 0 ? rp_twcount(n) : 1);
+    for (int pp = 1; pp < rp_npass(n); pp++) {
+        const int ns = rp_ns(n, pp), r = rp_radix(n, pp), off = rp_twoff(n, pp);
+        for (int t = 1; t < r; t++)
+            for (int k = 0; k < ns; k++) {
+                const double ang = -2.0 * 3.14159265358979323846 * (double)k * t / ((double)ns * r);
+                tw[off + (t - 1) * ns + k] = cfloat{(float)std::cos(ang), (float)std::sin(ang)};
+            }
+    }
+    return tw;
+}

@@ -364,7 +364,7 @@ int run_iterative(mc_ctx* c, const char* tag, const cfloat* F, int J, int N, int
         }
         {
             KTimer kt(c, "correlate");
-            MC_TRY(mc_launch_correlate(pl->col, pl->C, F, S, A, B, st, cx, H, b, phase_only, seay,
+            MC_TRY(mc_launch_correlate(pl->col, pl->col_ptw, pl->C, F, S, A, B, st, cx, H, b, phase_only, seay,
                                        T1, s));
         }
         {
@@ -941,7 +941,7 @@ int mc_group_correlate_window(mc_ctx* c, const float* sum_spec, const float* grp
     MC_BUF(T1, cfloat, "api_T1", (size_t)sea_y * cx);
     MC_BUF(win, float, "api_win", (size_t)sea_x * sea_y);
     cfloat* twW = row_twiddles(c, W);
-    MC_TRY(mc_launch_correlate(pl->col, pl->C, (const cfloat*)grp_spec, (const cfloat*)sum_spec, A,
+    MC_TRY(mc_launch_correlate(pl->col, pl->col_ptw, pl->C, (const cfloat*)grp_spec, (const cfloat*)sum_spec, A,
                                B, st, cx, H, bfactor, phase_only, sea_y, T1, c->stream));
     MC_TRY(mc_launch_row_dft(T1, twW, nullptr, 1, 1, cx, W, sea_x, sea_y, win, c->stream));
     MC_TRY(hipMemcpyAsync(window_host, win, sizeof(float) * sea_x * sea_y, hipMemcpyDeviceToHost,
