@@ -11,6 +11,9 @@
 #include "mc_align.h"
 #include "mc_fft_rp.h"
 
+#define MC_WARP_MAXP 64   // patches of the local-motion kernels
+#define MC_PS_MAXN 4096   // frames of the patch-statistics kernel
+
 #define MC_PI 3.14159265358979323846
 
 // ---------------------------------------------------------------------------
@@ -371,8 +374,11 @@ __global__ void k_update(UpdArgs a) {
 // (Util/CRemoveSpikes1D.cpp:31-148), CStackShift::Smooth
 // (Align/CStackShift.cpp:180-198) and the linear group->frame interpolation
 // (Align/CInterpolateShift.cpp:75-142).
+// The FP64 trajectory helpers below run on one thread per workgroup
+// (k_finalize, k_despike); their small fixed-size arrays are static LDS so
+// that indexing them does not spill to per-lane scratch memory.
 __device__ bool dev_lin_eqs(double* a, double* v, int n) {
-    int ci[16], piv[16];
+    __shared__ int ci[16], piv[16];
     for (int i = 0; i < n; i++) piv[i] = 0;
     for (int ip = 0; ip < n; ip++) {
         int ix = 0, iy = 0;
@@ -444,7 +450,7 @@ struct DevSpline {
     }
     __device__ void fit(const double* x, const double* y, int n, double* out) const {
         const int d = nk + 3;
-        double m[81], cf[9], t[9], c[9];
+        __shared__ double m[81], cf[9], t[9], c[9];
         for (int i = 0; i < d * d; i++) m[i] = 0.0;
         for (int i = 0; i < d; i++) cf[i] = 0.0;
         for (int i = 0; i < n; i++) {
@@ -506,7 +512,7 @@ __device__ void dev_remove_spikes(double* sx, double* sy, int n, int quirks, dou
     if (nk < 0) nk = 0;
     else if (nk > 6) nk = 6;
     if (nk == 0 && n >= 30) nk = 1;
-    double knots[6];
+    __shared__ double knots[6];
     for (int i = 0; i < nk; i++) knots[i] = t[(i + 1) * 15];
     DevSpline sp;
     sp.nk = nk;
@@ -546,16 +552,20 @@ struct FinArgs {
     const double* fcent;
     int N, G, binz, quirks;
     long long sstride;
+    int use_lds;   // work in LDS (sstride doubles) instead of the global scratch
 };
 
 // End of CIterativeAlign (Align/CIterativeAlign.cpp:63-85,146-149): linear
 // interpolation of the accumulated group shifts, RemoveSpikes, Smooth(0.5),
-// then Smooth(0.3 + 0.1 * binZ, capped at 0.9).
+// then Smooth(0.3 + 0.1 * binZ, capped at 0.9).  The chain is serial per
+// job; its work arrays live in LDS when they fit, so every access is an LDS
+// round trip rather than an HBM one.
 __global__ void k_finalize(FinArgs a) {
+    extern __shared__ double fsh[];
     const int j = blockIdx.x;
     if (threadIdx.x != 0) return;
     const int N = a.N, G = a.G;
-    double* sc = a.scratch + (long long)j * a.sstride;
+    double* sc = a.use_lds ? fsh : a.scratch + (long long)j * a.sstride;
     double* sx = sc;
     double* sy = sc + N;
     double* work = sc + 2 * N;
@@ -694,47 +704,73 @@ __global__ __launch_bounds__(1024) void k_feature_map(const float* img, unsigned
 __global__ __launch_bounds__(256) void k_patch_stats(double* s, const double* cen, int N, int P,
                                                      int W, int H, int quirks,
                                                      unsigned char* bad, int* ref_out) {
+    __shared__ double vfr[MC_PS_MAXN];
+    __shared__ double rrow[2 * MC_WARP_MAXP + 1];
     __shared__ int sref;
+    // mean + std of every frame's patch shifts (frames >= 1), in parallel
+    for (int i = 1 + threadIdx.x; i < N; i += blockDim.x) {
+        const double* p = s + (long long)i * P * 2;
+        double mx = 0, my = 0;
+        for (int k = 0; k < P; k++) { mx += p[2 * k]; my += p[2 * k + 1]; }
+        mx /= P;
+        my /= P;
+        double sd = 0;
+        for (int k = 0; k < P; k++) {
+            const double dx = p[2 * k] - mx, dy = p[2 * k + 1] - my;
+            sd += dx * dx + dy * dy;
+        }
+        vfr[i] = sqrt(mx * mx + my * my) + sqrt(sd / P);
+    }
+    __syncthreads();
     if (threadIdx.x == 0) {
+        // the reference's scan: the last frame whose value does not exceed
+        // the running minimum
         double fmin = 1e20;
         int ref = 0;
         for (int i = 1; i < N; i++) {
-            const double* p = s + (long long)i * P * 2;
-            double mx = 0, my = 0;
-            for (int k = 0; k < P; k++) { mx += p[2 * k]; my += p[2 * k + 1]; }
-            mx /= P;
-            my /= P;
-            double sd = 0;
-            for (int k = 0; k < P; k++) {
-                const double dx = p[2 * k] - mx, dy = p[2 * k + 1] - my;
-                sd += dx * dx + dy * dy;
-            }
-            const double v = sqrt(mx * mx + my * my) + sqrt(sd / P);
-            if (v > fmin) continue;
-            fmin = v;
+            if (vfr[i] > fmin) continue;
+            fmin = vfr[i];
             ref = i;
-        }
-        const long long total = (long long)N * P * 2;
-        const long long rb = (long long)ref * P * 2;
-        if (quirks) {
-            for (int i = 0; i < N; i++) {
-                const long long pb = (long long)i * P * 2;
-                for (int k = 0; k < P; k++) {
-                    const long long kk = 2 * k;
-                    s[pb + kk] -= (rb + kk < total) ? s[rb + kk] : 0.0;
-                    s[pb + kk + 1] -= (rb + kk + 2 < total) ? s[rb + kk + 2] : 0.0;
-                }
-            }
-        } else {
-            for (int i = 0; i < N; i++) {
-                if (i == ref) continue;
-                const long long pb = (long long)i * P * 2;
-                for (int k = 0; k < 2 * P; k++) s[pb + k] -= s[rb + k];
-            }
-            for (int k = 0; k < 2 * P; k++) s[rb + k] = 0.0;
         }
         sref = ref;
         if (ref_out) *ref_out = ref;
+    }
+    __syncthreads();
+    const int ref = sref;
+    const long long rb = (long long)ref * P * 2;
+    // original values the in-place subtraction reads: the reference row and
+    // (quirk) the x of the frame after it
+    for (int k = threadIdx.x; k < 2 * P; k += blockDim.x) rrow[k] = s[rb + k];
+    if (threadIdx.x == 0) rrow[2 * P] = (ref + 1 < N) ? s[rb + 2 * P] : 0.0;
+    __syncthreads();
+    for (int e = threadIdx.x; e < N * P; e += blockDim.x) {
+        const int i = e / P, k = e - i * P;
+        double* q = s + (long long)i * P * 2 + 2 * k;
+        if (quirks) {
+            // sequential in-place semantics of the reference loop (x uses
+            // element k, y element k+2 of the reference row as modified so
+            // far): before the reference frame both read original values;
+            // the reference frame's own x become 0 first; after it every x
+            // read is 0 and only the last y reads the next frame's x_0
+            const double nx = (k + 1 < P) ? rrow[2 * k + 2] : rrow[2 * P];
+            if (i < ref) {
+                q[0] -= rrow[2 * k];
+                q[1] -= nx;
+            } else if (i == ref) {
+                q[0] = 0.0;
+                q[1] -= nx;
+            } else if (k == P - 1) {
+                q[1] -= rrow[2 * P];
+            }
+        } else {
+            if (i == ref) {
+                q[0] = 0.0;
+                q[1] = 0.0;
+            } else {
+                q[0] -= rrow[2 * k];
+                q[1] -= rrow[2 * k + 1];
+            }
+        }
     }
     __syncthreads();
     for (int e = threadIdx.x; e < N * P; e += blockDim.x) {
@@ -790,8 +826,8 @@ struct WarpArgs {
 
 // Source offset (within a frame) of output pixel (x, y) for the weighted
 // shift sums of one frame.
-__device__ __forceinline__ long long warp_src(int W, int H, long long pitch, int x, int y,
-                                              float fw, float fsx, float fsy) {
+__device__ __forceinline__ int warp_src(int W, int H, int pitch, int x, int y, float fw, float fsx,
+                                        float fsy) {
     if (fw > 0) {
         fsx = fsx / fw;
         fsy = fsy / fw;
@@ -807,7 +843,7 @@ __device__ __forceinline__ long long warp_src(int W, int H, long long pitch, int
         xs = (int)((811LL * xs) % W);
         ys = (int)((811LL * ys) % H);
     }
-    return (long long)ys * pitch + xs;
+    return ys * pitch + xs;
 }
 
 // Fill the table of frames f0..f0+nf-1 for the K (>= nk) kept patches.
@@ -851,11 +887,12 @@ __device__ __forceinline__ void warp_tile_reg(const WarpArgs& a, const int* klis
         __syncthreads();
         if (!live) continue;
         // gathers of MC_WARP_U frames are issued before their stores so that
-        // each wave keeps several loads in flight
-        const float* __restrict__ R = a.R + (long long)f0 * a.fstride;
-        float* __restrict__ Rp = a.Rp + (long long)f0 * a.fstride + (long long)y * a.pitch + x;
+        // each wave keeps several loads in flight; offsets within a frame
+        // are 32-bit (the host checks pitch * H < 2^31)
+        const int pitch = (int)a.pitch;
+        const int oo = y * pitch + x;
         for (int fb = 0; fb < nf; fb += MC_WARP_U) {
-            long long so[MC_WARP_U];
+            int so[MC_WARP_U];
 #pragma unroll
             for (int u = 0; u < MC_WARP_U; u++) {
                 const float4* t = tab + min(fb + u, nf - 1) * K;
@@ -867,15 +904,15 @@ __device__ __forceinline__ void warp_tile_reg(const WarpArgs& a, const int* klis
                     fsx = fmaf(v.x, w[q], fsx);
                     fsy = fmaf(v.y, w[q], fsy);
                 }
-                so[u] = warp_src(a.W, a.H, a.pitch, x, y, fw, fsx, fsy);
+                so[u] = warp_src(a.W, a.H, pitch, x, y, fw, fsx, fsy);
             }
             float v[MC_WARP_U];
 #pragma unroll
             for (int u = 0; u < MC_WARP_U; u++)
-                if (fb + u < nf) v[u] = R[(long long)(fb + u) * a.fstride + so[u]];
+                if (fb + u < nf) v[u] = (a.R + (long long)(f0 + fb + u) * a.fstride)[so[u]];
 #pragma unroll
             for (int u = 0; u < MC_WARP_U; u++)
-                if (fb + u < nf) Rp[(long long)(fb + u) * a.fstride] = v[u];
+                if (fb + u < nf) (a.Rp + (long long)(f0 + fb + u) * a.fstride)[oo] = v[u];
         }
     }
 }
@@ -912,7 +949,7 @@ __device__ __forceinline__ void warp_tile_lds(const WarpArgs& a, const int* klis
             }
             const long long fo = (long long)(f0 + fi) * a.fstride;
             a.Rp[fo + (long long)y * a.pitch + x] =
-                a.R[fo + warp_src(a.W, a.H, a.pitch, x, y, fw, fsx, fsy)];
+                a.R[fo + warp_src(a.W, a.H, (int)a.pitch, x, y, fw, fsx, fsy)];
         }
     }
 }
@@ -929,18 +966,23 @@ __global__ __launch_bounds__(MC_WARP_NT) void k_local_warp(WarpArgs a) {
     const bool live = x < a.W && y < a.H;
     if (threadIdx.x == 0) kmask[0] = kmask[1] = 0u;
     __syncthreads();
-    float rmin = 1e30f;
+    // Selection only has to be a superset of the patches that can matter, so
+    // it uses squared distances with reciprocal scales and a small margin;
+    // the weights themselves are computed exactly as the reference does.
+    const float rw = 1.0f / (float)a.W, rh = 1.0f / (float)a.H;
+    float r2min = 1e30f;
     for (int p = 0; p < a.P && live; p++) {
-        const float ax = ((float)x - a.cen[2 * p]) / (float)a.W;
-        const float ay = ((float)y - a.cen[2 * p + 1]) / (float)a.H;
-        rmin = fminf(rmin, sqrtf(ax * ax + ay * ay));
+        const float ax = ((float)x - a.cen[2 * p]) * rw;
+        const float ay = ((float)y - a.cen[2 * p + 1]) * rh;
+        r2min = fminf(r2min, ax * ax + ay * ay);
     }
+    const float rlim = fminf(sqrtf(r2min) + 0.3225f, 0.6001f);
+    const float r2lim = rlim * rlim;
     unsigned int lo = 0u, hi = 0u;
     for (int p = 0; p < a.P && live; p++) {
-        const float ax = ((float)x - a.cen[2 * p]) / (float)a.W;
-        const float ay = ((float)y - a.cen[2 * p + 1]) / (float)a.H;
-        const float r = sqrtf(ax * ax + ay * ay);
-        if (r <= 0.6f && r <= rmin + 0.3224f) {
+        const float ax = ((float)x - a.cen[2 * p]) * rw;
+        const float ay = ((float)y - a.cen[2 * p + 1]) * rh;
+        if (ax * ax + ay * ay <= r2lim) {
             if (p < 32) lo |= 1u << p;
             else hi |= 1u << (p - 32);
         }
@@ -958,11 +1000,18 @@ __global__ __launch_bounds__(MC_WARP_NT) void k_local_warp(WarpArgs a) {
     }
     __syncthreads();
     const int nk = nk_s;
-    if (nk <= 4) warp_tile_reg<4>(a, klist, nk, tab, x, y, live);
-    else if (nk <= 8) warp_tile_reg<8>(a, klist, nk, tab, x, y, live);
-    else if (nk <= 12) warp_tile_reg<12>(a, klist, nk, tab, x, y, live);
-    else if (nk <= MC_WARP_KREG) warp_tile_reg<MC_WARP_KREG>(a, klist, nk, tab, x, y, live);
-    else
+    switch (nk) {
+#define MC_WK(K) \
+    case K: warp_tile_reg<K>(a, klist, nk, tab, x, y, live); return;
+        MC_WK(1) MC_WK(2) MC_WK(3) MC_WK(4) MC_WK(5) MC_WK(6) MC_WK(7) MC_WK(8)
+        MC_WK(9) MC_WK(10) MC_WK(11) MC_WK(12) MC_WK(13) MC_WK(14) MC_WK(15) MC_WK(16)
+#undef MC_WK
+        default: break;
+    }
+    if (nk == 0) {
+        warp_tile_reg<1>(a, klist, 0, tab, x, y, live);
+        return;
+    }
         warp_tile_lds(a, klist, nk, tab, reinterpret_cast<float*>(tab + MC_WARP_FCH * a.P), x, y,
                       live);
 }
@@ -1134,6 +1183,7 @@ int mc_align_init_constants() {
     if (mc_upload_prime_table() != 0) return -1;
     const int bytes = (mc_lds_padded(MC_LDS_ELEMS) + 64) * (int)sizeof(cfloat);
     hipFuncSetAttribute((const void*)k_correlate, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
+    hipFuncSetAttribute((const void*)k_finalize, hipFuncAttributeMaxDynamicSharedMemorySize, 65536);
 #define X(HH, CC, NT)                                                                     \
     hipFuncSetAttribute((const void*)k_correlate_rp<HH, CC, NT>,                         \
                         hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
@@ -1251,7 +1301,9 @@ hipError_t mc_launch_finalize(const IterState& st, int binz, int quirks, hipStre
     a.binz = binz;
     a.quirks = quirks;
     a.sstride = 16LL * st.N + 128;
-    hipLaunchKernelGGL(k_finalize, dim3(st.J), dim3(64), 0, s, a);
+    const size_t bytes = (size_t)a.sstride * sizeof(double);
+    a.use_lds = bytes <= 65536;
+    hipLaunchKernelGGL(k_finalize, dim3(st.J), dim3(64), a.use_lds ? bytes : 0, s, a);
     return hipGetLastError();
 }
 
@@ -1269,6 +1321,7 @@ hipError_t mc_launch_feature_map(const float* img128, unsigned char* feat, hipSt
 
 hipError_t mc_launch_patch_stats(double* shifts, const double* centers, int N, int P, int W, int H,
                                  int quirks, unsigned char* bad, int* ref, hipStream_t s) {
+    if (N < 1 || N > MC_PS_MAXN || P < 1 || P > MC_WARP_MAXP) return hipErrorInvalidValue;
     hipLaunchKernelGGL(k_patch_stats, dim3(1), dim3(256), 0, s, shifts, centers, N, P, W, H,
                        quirks, bad, ref);
     return hipGetLastError();
@@ -1278,6 +1331,7 @@ hipError_t mc_launch_local_warp(const float* R, float* Rp, long long pitch, long
                                 int W, int H, int N, int P, const float* cen, const float* sh,
                                 const unsigned char* bad, hipStream_t s) {
     if (P < 1 || P > MC_WARP_MAXP) return hipErrorInvalidValue;
+    if (pitch * H >= (1LL << 31) || pitch < W) return hipErrorInvalidValue;
     WarpArgs a{R, Rp, pitch, fstride, W, H, N, P, cen, sh, bad};
     dim3 grid(mc_div_up(W, MC_WARP_TX), mc_div_up(H, MC_WARP_TY));
     hipLaunchKernelGGL(k_local_warp, grid, dim3(MC_WARP_NT), warp_lds_bytes(P), s, a);

@@ -892,7 +892,7 @@ __global__ __launch_bounds__(NT) void k_rows_c2r_rp(RowC2RArgs a, const cfloat* 
 }
 
 #ifndef MC_RP_ROW_SPECS
-#define MC_RP_ROW_SPECS(X) X(2880, 2, 2, 512) X(2048, 2, 2, 512)
+#define MC_RP_ROW_SPECS(X) X(2880, 2, 2, 512) X(2048, 2, 2, 512) X(288, 16, 16, 512)
 #endif
 
 template <class F>
@@ -1067,8 +1067,9 @@ hipError_t mc_rows_r2c_mod(const Fft2D& p, const float* in, long long pitch, lon
     dim3 grid(mc_div_up(p.H, p.R) * batch);
     const size_t lds = (size_t)mc_lds_padded(p.R * p.m) * sizeof(cfloat);
 #define X(MM, RRR, CCC, NT)                                                                 \
-    if (p.m == MM && p.R == RRR && p.C == CCC && p.row_ptw) {                              \
-        hipLaunchKernelGGL((k_rows_r2c_rp<MM, RRR, CCC, NT>), grid, dim3(NT),              \
+    if (p.m == MM && p.C == CCC && p.row_ptw) {                                            \
+        hipLaunchKernelGGL((k_rows_r2c_rp<MM, RRR, CCC, NT>),                              \
+                           dim3(mc_div_up(p.H, RRR) * batch), dim3(NT),                    \
                            (size_t)RRR * MM * sizeof(cfloat), s, a, (const cfloat*)p.row_ptw); \
         return hipGetLastError();                                                          \
     }
@@ -1106,8 +1107,9 @@ hipError_t mc_rows_c2r(const Fft2D& p, const cfloat* T, long long t_bstride, flo
     dim3 grid(mc_div_up(p.H, p.R) * batch);
     const size_t lds = (size_t)mc_lds_padded(p.R * (p.m + 1)) * sizeof(cfloat);
 #define X(MM, RRR, CCC, NT)                                                                 \
-    if (p.m == MM && p.R == RRR && p.C == CCC && p.row_ptw) {                              \
-        hipLaunchKernelGGL((k_rows_c2r_rp<MM, RRR, CCC, NT>), grid, dim3(NT),              \
+    if (p.m == MM && p.C == CCC && p.row_ptw) {                                            \
+        hipLaunchKernelGGL((k_rows_c2r_rp<MM, RRR, CCC, NT>),                              \
+                           dim3(mc_div_up(p.H, RRR) * batch), dim3(NT),                    \
                            (size_t)RRR * (MM + 1) * sizeof(cfloat), s, a,                  \
                            (const cfloat*)p.row_ptw);                                      \
         return hipGetLastError();                                                          \
