@@ -99,7 +99,7 @@ def cpu_baseline(recipe, params_kw, frames_needed=5):
     r8.displacement = recipe.displacement[:frames_needed]
     ctx.synth_render(sub, r8)
     host = sub.numpy()
-    po = O.default_params(**params_kw)
+    po = O.default_params(**{k: v for k, v in params_kw.items() if k != "spectra_fmt"})
     t0 = time.time()
     ores = O.align_movie(host, po)
     dt = time.time() - t0
@@ -129,6 +129,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--kernel", default="fft_cols_fwd", help="kernel class for the roofline")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--spectra", default="f32", choices=["f32", "f16", "bf16"],
+                    help="storage of the xcf and patch spectrum stacks (arithmetic stays FP32)")
     ap.add_argument("--no-h2d", action="store_true",
                     help="skip timing the host-to-device upload of one movie")
     ap.add_argument("--copy-calib", type=int, default=0, metavar="MIB",
@@ -147,7 +149,8 @@ def main():
     ctx = L.Context(0)
     frames = ctx.empty((N, H, W), np.float32)
     ctx.synth_render(frames, recipe)
-    kw = dict(patch=(5, 5), bft=(500, 100), pix_size=0.83, kv=300, fm_dose=1.0)
+    kw = dict(patch=(5, 5), bft=(500, 100), pix_size=0.83, kv=300, fm_dose=1.0,
+              spectra_fmt={"f32": 0, "f16": 1, "bf16": 2}[args.spectra])
     prm = L.default_params(**kw)
     ns, ow, oh = ctx.output_geometry(prm, W, H, N)
     sums = [ctx.empty((oh, ow), np.float32) for _ in range(ns)]
@@ -214,6 +217,8 @@ def main():
                         "kV 300, FmDose 1.0 (sum + DW + DWS), Iter 15, Tol 0.1, Group 1 4",
             "movie": "20k Gaussian blobs, 1 e-/px/frame Poisson, drift + 3 px local field",
             "iterations_global_patch": iters[-1],
+            "spectra_storage": args.spectra,
+            "spectra_storage_used": [["f32", "f16", "bf16"][v] for v in res["spectra_fmt_used"]],
         },
         "roofline": {
             "kernel": args.kernel,

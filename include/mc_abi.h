@@ -61,6 +61,14 @@ typedef struct mc_params {
     const char* log_dir;  /* -LogDir, NULL/"" for no log files */
     const char* movie_name; /* input file name used to name the logs */
     int    verbose;       /* print the reference's progress lines */
+    int    spectra_fmt;   /* storage of the xcf and patch spectrum stacks that
+                             the alignment iterations read: 0 FP32 (default),
+                             1 FP16, 2 BF16 (per-transform scale, FP32
+                             arithmetic).  No reference counterpart: the
+                             reference keeps cufftComplex stacks. */
+    int    launch_graphs; /* 1 (default): run the alignment iterations as
+                             captured HIP graphs, two iterations per graph
+                             (same results as direct launches) */
 } mc_params;
 
 /* N frames already gain/dark/defect corrected, resident in HBM
@@ -90,6 +98,9 @@ typedef struct mc_result {
     int*   patch_iters;        /* host P */
     double seconds_align;      /* wall time of measurement */
     double seconds_total;      /* wall time of the whole call */
+    int    spectra_fmt_used[2];/* storage actually used for the xcf and patch
+                                  stacks (a geometry without the register-pass
+                                  kernels falls back to FP32) */
 } mc_result;
 
 /* Replacements of measured quantities.  replay_* follow -InAln

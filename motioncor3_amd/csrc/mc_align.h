@@ -28,11 +28,16 @@ hipError_t mc_launch_phasors(const double* shifts, double mult, int skip, const 
                              hipStream_t s);
 hipError_t mc_launch_aligned_sum(const cfloat* F, const cfloat* A, const cfloat* B,
                                  const unsigned char* done, int J, int N, int cx, int H,
-                                 cfloat* S, hipStream_t s);
+                                 cfloat* S, hipStream_t s, int f_fmt = MC_SPEC_F32,
+                                 const float* f_inv = nullptr);
+// Whether the correlation of column length H runs the register-pass kernel
+// (the one that reads reduced spectrum formats).
+bool mc_correlate_has_rp(int H);
 hipError_t mc_launch_correlate(const FftPlan1& colplan, const cfloat* ptw, int Cc, const cfloat* F,
                                const cfloat* S, const cfloat* A, const cfloat* B,
                                const IterState& st, int cx, int H, float bft, int phase_only,
-                               int seay, cfloat* T1, hipStream_t s);
+                               int seay, cfloat* T1, hipStream_t s, int f_fmt = MC_SPEC_F32,
+                               const float* f_inv = nullptr);
 hipError_t mc_launch_row_dft(const cfloat* T1, const cfloat* twW, const unsigned char* done, int J,
                              int G, int cx, int W, int seax, int seay, float* win, hipStream_t s);
 hipError_t mc_launch_peaks(const float* win, const unsigned char* done, int J, int G, int nx,

@@ -10,6 +10,7 @@
 
 #include "mc_align.h"
 #include "mc_fft_rp.h"
+#include "mc_spec.h"
 
 #define MC_WARP_MAXP 64   // patches of the local-motion kernels
 #define MC_PS_MAXN 4096   // frames of the patch-statistics kernel
@@ -45,9 +46,10 @@ __global__ void k_phasors(const double* shifts, double mult, int skip, const uns
 // CAlignedSum (Align/CAlignedSum.cpp:25-126): S = sum_i shift(F_i). Even and
 // odd frames accumulate separately and are added at the end, as the
 // reference's two streams do (:99-112).
-__global__ void k_aligned_sum(const cfloat* __restrict__ F, const cfloat* __restrict__ A,
-                              const cfloat* __restrict__ B, const unsigned char* done, int N,
-                              int cx, int H, cfloat* S) {
+template <int FF>
+__global__ void k_aligned_sum(const cfloat* __restrict__ F, const float* __restrict__ finv,
+                              const cfloat* __restrict__ A, const cfloat* __restrict__ B,
+                              const unsigned char* done, int N, int cx, int H, cfloat* S) {
     const int j = blockIdx.z;
     if (done && done[j]) return;
     const long long fs = (long long)cx * H;
@@ -58,7 +60,8 @@ __global__ void k_aligned_sum(const cfloat* __restrict__ F, const cfloat* __rest
     cfloat a0 = cmk(0.f, 0.f), a1 = cmk(0.f, 0.f);
     for (int i = 0; i < N; i++) {
         const long long b = (long long)j * N + i;
-        const cfloat v = cmul(F[b * fs + q], cmul(A[b * cx + kx], B[b * H + y]));
+        const cfloat f = Spec<FF>::ld(F, b * fs + q, FF == MC_SPEC_F32 ? 1.f : finv[b]);
+        const cfloat v = cmul(f, cmul(A[b * cx + kx], B[b * H + y]));
         if (i & 1) a1 = cadd(a1, v);
         else a0 = cadd(a0, v);
     }
@@ -85,6 +88,7 @@ struct CorrArgs {
     float filt0;
     int phase_only, seay;
     cfloat* T1;
+    const float* finv;   // load factors of a reduced F format (per transform)
 };
 
 __global__ __launch_bounds__(MC_NTHR) void k_correlate(CorrArgs a) {
@@ -132,7 +136,7 @@ __global__ __launch_bounds__(MC_NTHR) void k_correlate(CorrArgs a) {
 // spectrum of C columns is formed straight into the registers of pass 0, and
 // the last pass only evaluates the butterflies that own one of the seay
 // kept lags (y in [0, seay/2) and [H - seay/2, H)).
-template <int H, int C, int NT>
+template <int H, int C, int NT, int FF>
 __global__ __launch_bounds__(NT) void k_correlate_rp(CorrArgs a, const cfloat* __restrict__ ptw) {
     constexpr int NP = rp_npass(H), R0 = rp_radix(H, 0), M0 = H / R0;
     constexpr int RL = rp_radix(H, NP - 1), ML = H / RL;
@@ -157,12 +161,14 @@ __global__ __launch_bounds__(NT) void k_correlate_rp(CorrArgs a, const cfloat* _
             if (c < nc) {
                 for (int f = 0; f < gn; f++) {
                     const long long b = (long long)j * a.N + gs + f;
-                    const cfloat* Fb = a.F + b * fs + (long long)kx * H + jj;
+                    const long long fo = b * fs + (long long)kx * H + jj;
+                    const float fi = FF == MC_SPEC_F32 ? 1.f : a.finv[b];
                     const cfloat* Bb = a.B + b * H + jj;
                     const cfloat pa = a.A[b * a.cx + kx];
 #pragma unroll
                     for (int t = 0; t < R0; t++)
-                        v[t] = cadd(v[t], cmul(Fb[t * M0], cmul(pa, Bb[t * M0])));
+                        v[t] = cadd(v[t], cmul(Spec<FF>::ld(a.F, fo + t * M0, fi),
+                                               cmul(pa, Bb[t * M0])));
                 }
                 const cfloat* Sc = S + (long long)kx * H + jj;
 #pragma unroll
@@ -1185,7 +1191,11 @@ int mc_align_init_constants() {
     hipFuncSetAttribute((const void*)k_correlate, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
     hipFuncSetAttribute((const void*)k_finalize, hipFuncAttributeMaxDynamicSharedMemorySize, 65536);
 #define X(HH, CC, NT)                                                                     \
-    hipFuncSetAttribute((const void*)k_correlate_rp<HH, CC, NT>,                         \
+    hipFuncSetAttribute((const void*)k_correlate_rp<HH, CC, NT, 0>,                      \
+                        hipFuncAttributeMaxDynamicSharedMemorySize, bytes);              \
+    hipFuncSetAttribute((const void*)k_correlate_rp<HH, CC, NT, 1>,                      \
+                        hipFuncAttributeMaxDynamicSharedMemorySize, bytes);              \
+    hipFuncSetAttribute((const void*)k_correlate_rp<HH, CC, NT, 2>,                      \
                         hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
     MC_RP_CORR_SPECS(X)
 #undef X
@@ -1208,16 +1218,31 @@ hipError_t mc_launch_phasors(const double* shifts, double mult, int skip, const 
 
 hipError_t mc_launch_aligned_sum(const cfloat* F, const cfloat* A, const cfloat* B,
                                  const unsigned char* done, int J, int N, int cx, int H,
-                                 cfloat* S, hipStream_t s) {
+                                 cfloat* S, hipStream_t s, int f_fmt, const float* f_inv) {
     dim3 grid(mc_div_up(H, 256), cx, J);
-    hipLaunchKernelGGL(k_aligned_sum, grid, dim3(256), 0, s, F, A, B, done, N, cx, H, S);
+    switch (f_fmt) {
+        case MC_SPEC_F32:
+            hipLaunchKernelGGL(k_aligned_sum<MC_SPEC_F32>, grid, dim3(256), 0, s, F, f_inv, A, B,
+                               done, N, cx, H, S);
+            break;
+        case MC_SPEC_F16:
+            hipLaunchKernelGGL(k_aligned_sum<MC_SPEC_F16>, grid, dim3(256), 0, s, F, f_inv, A, B,
+                               done, N, cx, H, S);
+            break;
+        case MC_SPEC_BF16:
+            hipLaunchKernelGGL(k_aligned_sum<MC_SPEC_BF16>, grid, dim3(256), 0, s, F, f_inv, A, B,
+                               done, N, cx, H, S);
+            break;
+        default: return hipErrorInvalidValue;
+    }
     return hipGetLastError();
 }
 
 hipError_t mc_launch_correlate(const FftPlan1& colplan, const cfloat* ptw, int Cc, const cfloat* F,
                                const cfloat* S, const cfloat* A, const cfloat* B,
                                const IterState& st, int cx, int H, float bft, int phase_only,
-                               int seay, cfloat* T1, hipStream_t s) {
+                               int seay, cfloat* T1, hipStream_t s, int f_fmt,
+                               const float* f_inv) {
     CorrArgs a;
     a.plan = colplan;
     a.F = F;
@@ -1236,18 +1261,39 @@ hipError_t mc_launch_correlate(const FftPlan1& colplan, const cfloat* ptw, int C
     a.phase_only = phase_only;
     a.seay = seay;
     a.T1 = T1;
-    dim3 grid(mc_div_up(cx, Cc), st.G, st.J);
+    a.finv = f_inv;
 #define X(HH, CC, NT)                                                                     \
-    if (H == HH && Cc == CC && ptw && seay <= HH) {                                      \
-        hipLaunchKernelGGL((k_correlate_rp<HH, CC, NT>), grid, dim3(NT),                 \
-                           (size_t)CC * HH * sizeof(cfloat), s, a, ptw);                 \
+    if (H == HH && ptw && seay <= HH) {                                                  \
+        const dim3 grid(mc_div_up(cx, CC), st.G, st.J);                                  \
+        const size_t lds = (size_t)CC * HH * sizeof(cfloat);                             \
+        if (f_fmt == MC_SPEC_F32)                                                        \
+            hipLaunchKernelGGL((k_correlate_rp<HH, CC, NT, 0>), grid, dim3(NT), lds, s, a, ptw); \
+        else if (f_fmt == MC_SPEC_F16)                                                   \
+            hipLaunchKernelGGL((k_correlate_rp<HH, CC, NT, 1>), grid, dim3(NT), lds, s, a, ptw); \
+        else if (f_fmt == MC_SPEC_BF16)                                                  \
+            hipLaunchKernelGGL((k_correlate_rp<HH, CC, NT, 2>), grid, dim3(NT), lds, s, a, ptw); \
+        else                                                                             \
+            return hipErrorInvalidValue;                                                 \
         return hipGetLastError();                                                        \
     }
     MC_RP_CORR_SPECS(X)
 #undef X
+    if (f_fmt != MC_SPEC_F32) return hipErrorInvalidValue;
+    while (Cc > 1 && (long long)Cc * H > MC_LDS_ELEMS) Cc /= 2;
+    if ((long long)Cc * H > MC_LDS_ELEMS) return hipErrorInvalidValue;
+    a.Cc = Cc;
+    dim3 grid(mc_div_up(cx, Cc), st.G, st.J);
     const size_t lds = (size_t)mc_lds_padded(Cc * H) * sizeof(cfloat);
     hipLaunchKernelGGL(k_correlate, grid, dim3(MC_NTHR), lds, s, a);
     return hipGetLastError();
+}
+
+bool mc_correlate_has_rp(int H) {
+#define X(HH, CC, NT) \
+    if (H == HH) return true;
+    MC_RP_CORR_SPECS(X)
+#undef X
+    return false;
 }
 
 hipError_t mc_launch_row_dft(const cfloat* T1, const cfloat* twW, const unsigned char* done, int J,

@@ -10,6 +10,7 @@
 // unless a scale is given (same conventions as cufftExecR2C/C2R).
 #pragma once
 #include "mc_common.h"
+#include "mc_spec.h"
 
 // Tile shape of the two passes: C = the largest power of two <= MC_TILE_CMAX
 // whose C columns fit in LDS, R = ceil(MC_TILE_RC / C) rows per row-pass
@@ -19,6 +20,10 @@
 #endif
 #ifndef MC_TILE_RC
 #define MC_TILE_RC 16
+#endif
+// staging chunk width for heights with a register-pass column schedule
+#ifndef MC_TILE_CT
+#define MC_TILE_CT 8
 #endif
 
 struct Fft2D {
@@ -57,15 +62,19 @@ hipError_t mc_rows_r2c(const Fft2D& p, const float* in, long long pitch, long lo
                        hipStream_t s);
 // cols: T -> S forward; optional xcf crop emission (Util/GFourierResize2D.cu
 // mapping) into X with half-spectrum size (xcx, xcy), S may be null.
+// S and X may be stored in a reduced format (mc_spec.h) with per-transform
+// store factors s_g / x_g; strides are in elements of the format.
 hipError_t mc_cols_fwd(const Fft2D& p, const cfloat* T, long long t_bstride, cfloat* S,
                        long long s_bstride, cfloat* X, int xcx, int xcy, long long x_bstride,
-                       int batch, hipStream_t s);
+                       int batch, hipStream_t s, int s_fmt = MC_SPEC_F32,
+                       const float* s_g = nullptr, int x_fmt = MC_SPEC_F32,
+                       const float* x_g = nullptr);
 // cols: S -> T inverse.  Optional phasors multiply the spectrum on load.  If
 // src_cx/src_cy differ from the plan's (cx, H) the spectrum is read through
 // the Fourier crop/pad mapping of Util/GFourierResize2D.cu:7-39.
 hipError_t mc_cols_inv(const Fft2D& p, const cfloat* S, long long s_bstride, int src_cx,
                        int src_cy, const PhasorTabs* ph, cfloat* T, long long t_bstride, int batch,
-                       hipStream_t s);
+                       hipStream_t s, int s_fmt = MC_SPEC_F32, const float* s_inv = nullptr);
 // rows: T -> real (C2R along x), optional clamp at zero.
 hipError_t mc_rows_c2r(const Fft2D& p, const cfloat* T, long long t_bstride, float* out,
                        long long pitch, long long out_bstride, int batch, int clamp,
@@ -75,4 +84,12 @@ hipError_t mc_rows_c2r(const Fft2D& p, const cfloat* T, long long t_bstride, flo
 // (b / in_mod) when in_mod > 0 (patch stacks: b = patch * N + frame).
 hipError_t mc_rows_r2c_mod(const Fft2D& p, const float* in, long long pitch, long long in_bstride,
                            int in_mod, const int* win_xy, cfloat* T, long long t_bstride,
-                           int batch, float scale, hipStream_t s);
+                           int batch, float scale, hipStream_t s, float* dc = nullptr);
+// Per-transform factors of a reduced spectrum format from |row DC| values
+// dc[batch][rows] (mc_spec.h).
+hipError_t mc_launch_spec_scale(const float* dc, int rows, int batch, float* g, float* inv,
+                                hipStream_t s);
+// Whether the register-pass kernels (the ones that support reduced spectrum
+// formats and row DC output) run the plan's rows / columns.
+bool mc_fft_rp_rows(const Fft2D& p);
+bool mc_fft_rp_cols(const Fft2D& p);

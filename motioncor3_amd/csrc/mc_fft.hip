@@ -25,6 +25,7 @@ struct RowR2CArgs {
     long long t_bstride;
     const cfloat* post_tw;
     float scale;
+    float* dc;             // |row DC| per (transform, row) of the output, or null
 };
 
 struct RowC2RArgs {
@@ -48,6 +49,8 @@ struct ColFwdArgs {
     cfloat* X;
     int xcx, xcy;
     long long x_bstride;
+    const float* s_g;      // store factors of a reduced S format (per transform)
+    const float* x_g;      // store factors of a reduced X format
 };
 
 struct ColInvArgs {
@@ -60,6 +63,7 @@ struct ColInvArgs {
     const cfloat* phB;
     cfloat* T;
     long long t_bstride;
+    const float* s_inv;    // load factors of a reduced S format (per transform)
 };
 
 // Util/GFourierResize2D.cu:15-35 row map: output row yo of a spectrum with
@@ -578,7 +582,7 @@ __global__ __launch_bounds__(NT) void k_cols_inv_ct(ColInvArgs a) {
 // patches 576x408; the 128^2 feature map.
 #ifndef MC_ROW_SPECS
 #define MC_ROW_SPECS(X) \
-    X(2880, 2, 2) X(2048, 2, 2) X(512, 2, 8) X(1441, 2, 4) X(1024, 2, 8) X(288, 2, 16) X(64, 2, 16)
+    X(2880, 2, 8) X(2048, 2, 8) X(512, 2, 8) X(1441, 2, 8) X(1024, 2, 8) X(288, 2, 16) X(64, 2, 16)
 #endif
 #ifndef MC_COL_SPECS
 #define MC_COL_SPECS(X) \
@@ -599,18 +603,21 @@ __device__ __forceinline__ int crop_row(int y, int xcy, int H) {
     return (y > xcy / 2 + d) ? y - d : -1;
 }
 
-template <int H, int C, int NT>
+template <int H, int CW, int NT, int FS, int FX>
 __global__ __launch_bounds__(NT) void k_cols_fwd_rp(ColFwdArgs a, const cfloat* __restrict__ ptw) {
+    constexpr int C = CW;
     constexpr int NP = rp_npass(H);
     constexpr int R0 = rp_radix(H, 0), M0 = H / R0;
     constexpr int RL = rp_radix(H, NP - 1), ML = H / RL;
     constexpr int IT0 = (C * M0 + NT - 1) / NT, ITL = (C * ML + NT - 1) / NT;
     extern __shared__ cfloat lds[];
+    const int CT = a.C;  // chunk width of the staging layout (a multiple of CW)
     const int ncb = (a.ncols + C - 1) / C;
     const int tt = xcd_tile(blockIdx.x, gridDim.x);
-    const int b = tt / ncb, kb = tt - b * ncb, c0 = kb * C;
+    const int b = tt / ncb, cb = tt - b * ncb, c0 = cb * C;
+    const int kb = c0 / CT, coff = c0 - kb * CT;
     const int nc = min(C, a.ncols - c0);
-    const cfloat* src = a.T + (long long)b * a.t_bstride + (long long)kb * H * C;
+    const cfloat* src = a.T + (long long)b * a.t_bstride + (long long)kb * H * CT + coff;
     // pass 0: staging [y][c] -> registers -> LDS (items: column fastest)
 #pragma unroll
     for (int it = 0; it < IT0; it++) {
@@ -620,7 +627,7 @@ __global__ __launch_bounds__(NT) void k_cols_fwd_rp(ColFwdArgs a, const cfloat* 
             cfloat v[R0];
             const bool ok = c < nc;
 #pragma unroll
-            for (int t = 0; t < R0; t++) v[t] = ok ? src[(j + t * M0) * C + c] : cmk(0.f, 0.f);
+            for (int t = 0; t < R0; t++) v[t] = ok ? src[(j + t * M0) * CT + c] : cmk(0.f, 0.f);
             RegDft<R0>::run(v);
             cfloat* d = lds + c * H + j * R0;
 #pragma unroll
@@ -644,16 +651,18 @@ __global__ __launch_bounds__(NT) void k_cols_fwd_rp(ColFwdArgs a, const cfloat* 
                 RegDft<RL>::run(v);
                 const int kx = c0 + c;
                 if (a.S) {
-                    cfloat* d = a.S + (long long)b * a.s_bstride + (long long)kx * H + k;
+                    const float g = FS == MC_SPEC_F32 ? 1.f : a.s_g[b];
+                    const long long d = (long long)b * a.s_bstride + (long long)kx * H + k;
 #pragma unroll
-                    for (int t = 0; t < RL; t++) d[t * ML] = v[t];
+                    for (int t = 0; t < RL; t++) Spec<FS>::st(a.S, d + t * ML, v[t], g);
                 }
                 if (a.X && kx < a.xcx) {
-                    cfloat* xd = a.X + (long long)b * a.x_bstride + (long long)kx * a.xcy;
+                    const float g = FX == MC_SPEC_F32 ? 1.f : a.x_g[b];
+                    const long long xd = (long long)b * a.x_bstride + (long long)kx * a.xcy;
 #pragma unroll
                     for (int t = 0; t < RL; t++) {
                         const int yo = crop_row(k + t * ML, a.xcy, H);
-                        if (yo >= 0) xd[yo] = v[t];
+                        if (yo >= 0) Spec<FX>::st(a.X, xd + yo, v[t], g);
                     }
                 }
             }
@@ -661,18 +670,20 @@ __global__ __launch_bounds__(NT) void k_cols_fwd_rp(ColFwdArgs a, const cfloat* 
     }
 }
 
-template <int H, int C, int NT>
+template <int H, int CW, int NT, int FS>
 __global__ __launch_bounds__(NT) void k_cols_inv_rp(ColInvArgs a, const cfloat* __restrict__ ptw) {
+    constexpr int C = CW;
     constexpr int NP = rp_npass(H);
     constexpr int R0 = rp_radix(H, 0), M0 = H / R0;
     constexpr int RL = rp_radix(H, NP - 1), ML = H / RL;
     constexpr int IT0 = (C * M0 + NT - 1) / NT, ITL = (C * ML + NT - 1) / NT;
     extern __shared__ cfloat lds[];
+    const int CT = a.C;  // chunk width of the staging layout (a multiple of CW)
     const int ncb = (a.ncols + C - 1) / C;
     const int tt = xcd_tile(blockIdx.x, gridDim.x);
-    const int b = tt / ncb, kb = tt - b * ncb, c0 = kb * C;
+    const int b = tt / ncb, cb = tt - b * ncb, c0 = cb * C;
+    const int kb = c0 / CT, coff = c0 - kb * CT;
     const int nc = min(C, a.ncols - c0);
-    const cfloat* sb = a.S + (long long)b * a.s_bstride;
     // pass 0: spectrum columns (phase factors, conjugated) -> LDS (items: row fastest)
 #pragma unroll
     for (int it = 0; it < IT0; it++) {
@@ -682,9 +693,22 @@ __global__ __launch_bounds__(NT) void k_cols_inv_rp(ColInvArgs a, const cfloat* 
             const int kx = c0 + c;
             cfloat v[R0];
             if (c < nc) {
-                const cfloat* sc = sb + (long long)kx * H + j;
+                const float si = FS == MC_SPEC_F32 ? 1.f : a.s_inv[b];
+                const long long sbo = (long long)b * a.s_bstride;
+                if (a.src_cx == a.ncols && a.src_cy == H) {
+                    const long long sc = sbo + (long long)kx * H + j;
 #pragma unroll
-                for (int t = 0; t < R0; t++) v[t] = sc[t * M0];
+                    for (int t = 0; t < R0; t++) v[t] = Spec<FS>::ld(a.S, sc + t * M0, si);
+                } else {
+                    // Fourier crop or pad from a src_cx x src_cy spectrum
+#pragma unroll
+                    for (int t = 0; t < R0; t++) {
+                        const int yi = resize_row(j + t * M0, H, a.src_cy);
+                        v[t] = (kx < a.src_cx && yi >= 0)
+                                   ? Spec<FS>::ld(a.S, sbo + (long long)kx * a.src_cy + yi, si)
+                                   : cmk(0.f, 0.f);
+                    }
+                }
                 if (a.phA) {
                     const cfloat pa = a.phA[(long long)b * a.ncols + kx];
                     const cfloat* pb = a.phB + (long long)b * H + j;
@@ -706,7 +730,7 @@ __global__ __launch_bounds__(NT) void k_cols_inv_rp(ColInvArgs a, const cfloat* 
     __syncthreads();
     RpMid<H, 1, C, NT>::run(lds, ptw);
     // last pass: LDS -> registers -> staging [y][c] (items: column fastest)
-    cfloat* dst = a.T + (long long)b * a.t_bstride + (long long)kb * H * C;
+    cfloat* dst = a.T + (long long)b * a.t_bstride + (long long)kb * H * CT + coff;
 #pragma unroll
     for (int it = 0; it < ITL; it++) {
         const int i = threadIdx.x + it * NT;
@@ -720,7 +744,7 @@ __global__ __launch_bounds__(NT) void k_cols_inv_rp(ColInvArgs a, const cfloat* 
                 rp_twiddle<H, NP - 1>(v, k, ptw);
                 RegDft<RL>::run(v);
 #pragma unroll
-                for (int t = 0; t < RL; t++) dst[(k + t * ML) * C + c] = cconj(v[t]);
+                for (int t = 0; t < RL; t++) dst[(k + t * ML) * CT + c] = cconj(v[t]);
             }
         }
     }
@@ -798,6 +822,7 @@ __global__ __launch_bounds__(NT) void k_rows_r2c_rp(RowR2CArgs a, const cfloat* 
         const cfloat O = cmk(0.5f * D.y, -0.5f * D.x);
         const cfloat X = cadd(E, cmul(a.post_tw[k], O));
         tb[kb * kbs + r * C + c] = cscale(X, a.scale);
+        if (a.dc && k == 0) a.dc[(long long)b * a.H + row0 + r] = fabsf(X.x * a.scale);
     }
 }
 
@@ -892,8 +917,63 @@ __global__ __launch_bounds__(NT) void k_rows_c2r_rp(RowC2RArgs a, const cfloat* 
 }
 
 #ifndef MC_RP_ROW_SPECS
-#define MC_RP_ROW_SPECS(X) X(2880, 2, 2, 512) X(2048, 2, 2, 512) X(288, 16, 16, 512)
+#define MC_RP_ROW_SPECS(X) X(2880, 2, 8, 512) X(2048, 2, 8, 512) X(288, 16, 16, 512)
 #endif
+
+
+// Launch a register-pass column kernel for the given storage formats.
+template <int H, int CW, int NT>
+static hipError_t launch_cols_fwd_rp(const ColFwdArgs& a, const cfloat* ptw, int batch, int s_fmt,
+                                     int x_fmt, hipStream_t s) {
+    const dim3 grid(mc_div_up(a.ncols, CW) * batch), blk(NT);
+    const size_t lds = (size_t)CW * H * sizeof(cfloat);
+    const int key = s_fmt * 3 + x_fmt;
+    switch (key) {
+        case 0: hipLaunchKernelGGL((k_cols_fwd_rp<H, CW, NT, 0, 0>), grid, blk, lds, s, a, ptw); break;
+        case 1: hipLaunchKernelGGL((k_cols_fwd_rp<H, CW, NT, 0, 1>), grid, blk, lds, s, a, ptw); break;
+        case 2: hipLaunchKernelGGL((k_cols_fwd_rp<H, CW, NT, 0, 2>), grid, blk, lds, s, a, ptw); break;
+        case 3: hipLaunchKernelGGL((k_cols_fwd_rp<H, CW, NT, 1, 0>), grid, blk, lds, s, a, ptw); break;
+        case 6: hipLaunchKernelGGL((k_cols_fwd_rp<H, CW, NT, 2, 0>), grid, blk, lds, s, a, ptw); break;
+        default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+template <int H, int CW, int NT>
+static hipError_t launch_cols_inv_rp(const ColInvArgs& a, const cfloat* ptw, int batch, int s_fmt,
+                                     hipStream_t s) {
+    const dim3 grid(mc_div_up(a.ncols, CW) * batch), blk(NT);
+    const size_t lds = (size_t)CW * H * sizeof(cfloat);
+    switch (s_fmt) {
+        case 0: hipLaunchKernelGGL((k_cols_inv_rp<H, CW, NT, 0>), grid, blk, lds, s, a, ptw); break;
+        case 1: hipLaunchKernelGGL((k_cols_inv_rp<H, CW, NT, 1>), grid, blk, lds, s, a, ptw); break;
+        case 2: hipLaunchKernelGGL((k_cols_inv_rp<H, CW, NT, 2>), grid, blk, lds, s, a, ptw); break;
+        default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+// g[b] = TARGET / sum_r |dc[b][r]| and inv[b] = 1 / g[b], summed in a fixed
+// order (deterministic).  One workgroup per transform.
+__global__ __launch_bounds__(256) void k_spec_scale(const float* __restrict__ dc, int rows,
+                                                    float* __restrict__ g, float* __restrict__ inv) {
+    __shared__ float part[256];
+    const float* d = dc + (long long)blockIdx.x * rows;
+    float acc = 0.f;
+    for (int r = threadIdx.x; r < rows; r += 256) acc += d[r];
+    part[threadIdx.x] = acc;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (threadIdx.x < w) part[threadIdx.x] += part[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        const float tot = part[0];
+        const float gg = tot > 0.f ? MC_SPEC_TARGET / tot : 1.f;
+        g[blockIdx.x] = gg;
+        inv[blockIdx.x] = 1.f / gg;
+    }
+}
+
 
 template <class F>
 static void set_lds_attr(F* f) {
@@ -908,7 +988,11 @@ static void set_ct_attrs() {
 #define X(H, C, NT) set_lds_attr(k_cols_fwd_ct<H, C, NT>); set_lds_attr(k_cols_inv_ct<H, C, NT>);
     MC_COL_SPECS(X)
 #undef X
-#define X(H, C, NT) set_lds_attr(k_cols_fwd_rp<H, C, NT>); set_lds_attr(k_cols_inv_rp<H, C, NT>);
+#define X(H, C, NT)                                                                        \
+    set_lds_attr(k_cols_fwd_rp<H, C, NT, 0, 0>); set_lds_attr(k_cols_fwd_rp<H, C, NT, 0, 1>); \
+    set_lds_attr(k_cols_fwd_rp<H, C, NT, 0, 2>); set_lds_attr(k_cols_fwd_rp<H, C, NT, 1, 0>); \
+    set_lds_attr(k_cols_fwd_rp<H, C, NT, 2, 0>); set_lds_attr(k_cols_inv_rp<H, C, NT, 0>);    \
+    set_lds_attr(k_cols_inv_rp<H, C, NT, 1>); set_lds_attr(k_cols_inv_rp<H, C, NT, 2>);
     MC_RP_COL_SPECS(X)
 #undef X
 #define X(M, RR, C, NT) set_lds_attr(k_rows_r2c_rp<M, RR, C, NT>); set_lds_attr(k_rows_c2r_rp<M, RR, C, NT>);
@@ -973,6 +1057,12 @@ bool g_attr_done = false;
 
 }  // namespace
 
+hipError_t mc_launch_spec_scale(const float* dc, int rows, int batch, float* g, float* inv,
+                                hipStream_t s) {
+    hipLaunchKernelGGL(k_spec_scale, dim3(batch), dim3(256), 0, s, dc, rows, g, inv);
+    return hipGetLastError();
+}
+
 int mc_fft_init_constants() {
     if (mc_upload_prime_table() != 0) return -1;
     if (!g_attr_done) {
@@ -1001,6 +1091,9 @@ int mc_fft2d_create(Fft2D* p, int W, int H) {
     const int budget = MC_LDS_ELEMS / 2;
     int C = MC_TILE_CMAX;
     while (C > 1 && (long long)C * H > budget) C /= 2;
+    // Heights with a register-pass column schedule read a few columns of a
+    // wider staging chunk, so the row passes move whole 128-byte lines.
+    if (rp_has(H) && H >= 1024) C = MC_TILE_CT;
     p->C = C;
     int R = 1;
     const int want = std::max((MC_TILE_RC + C - 1) / C, 2);
@@ -1040,6 +1133,22 @@ void mc_fft2d_destroy(Fft2D* p) {
     p->row_tw = p->col_tw = p->post_tw = p->col_ptw = p->row_ptw = nullptr;
 }
 
+bool mc_fft_rp_rows(const Fft2D& p) {
+#define X(MM, RRR, CCC, NT) \
+    if (p.m == MM && p.C == CCC && p.row_ptw) return true;
+    MC_RP_ROW_SPECS(X)
+#undef X
+    return false;
+}
+
+bool mc_fft_rp_cols(const Fft2D& p) {
+#define X(HH, CC, NT) \
+    if (p.H == HH && p.C % CC == 0 && p.col_ptw) return true;
+    MC_RP_COL_SPECS(X)
+#undef X
+    return false;
+}
+
 hipError_t mc_rows_r2c(const Fft2D& p, const float* in, long long pitch, long long in_bstride,
                        const int* win_xy, cfloat* T, long long t_bstride, int batch, float scale,
                        hipStream_t s) {
@@ -1048,7 +1157,7 @@ hipError_t mc_rows_r2c(const Fft2D& p, const float* in, long long pitch, long lo
 
 hipError_t mc_rows_r2c_mod(const Fft2D& p, const float* in, long long pitch, long long in_bstride,
                            int in_mod, const int* win_xy, cfloat* T, long long t_bstride,
-                           int batch, float scale, hipStream_t s) {
+                           int batch, float scale, hipStream_t s, float* dc) {
     RowR2CArgs a;
     a.plan = p.row;
     a.H = p.H;
@@ -1064,6 +1173,7 @@ hipError_t mc_rows_r2c_mod(const Fft2D& p, const float* in, long long pitch, lon
     a.t_bstride = t_bstride;
     a.post_tw = p.post_tw;
     a.scale = scale;
+    a.dc = dc;
     dim3 grid(mc_div_up(p.H, p.R) * batch);
     const size_t lds = (size_t)mc_lds_padded(p.R * p.m) * sizeof(cfloat);
 #define X(MM, RRR, CCC, NT)                                                                 \
@@ -1075,6 +1185,7 @@ hipError_t mc_rows_r2c_mod(const Fft2D& p, const float* in, long long pitch, lon
     }
     MC_RP_ROW_SPECS(X)
 #undef X
+    if (dc) return hipErrorInvalidValue;  // row DCs only from the register-pass rows
 #define X(MM, RRR, CCC)                                                             \
     if (p.m == MM && p.R == RRR && p.C == CCC) {                                    \
         const size_t ldsct =                                                        \
@@ -1131,7 +1242,8 @@ hipError_t mc_rows_c2r(const Fft2D& p, const cfloat* T, long long t_bstride, flo
 
 hipError_t mc_cols_fwd(const Fft2D& p, const cfloat* T, long long t_bstride, cfloat* S,
                        long long s_bstride, cfloat* X, int xcx, int xcy, long long x_bstride,
-                       int batch, hipStream_t s) {
+                       int batch, hipStream_t s, int s_fmt, const float* s_g, int x_fmt,
+                       const float* x_g) {
     ColFwdArgs a;
     a.plan = p.col;
     a.H = p.H;
@@ -1145,16 +1257,16 @@ hipError_t mc_cols_fwd(const Fft2D& p, const cfloat* T, long long t_bstride, cfl
     a.xcx = xcx;
     a.xcy = xcy;
     a.x_bstride = x_bstride;
+    a.s_g = s_g;
+    a.x_g = x_g;
     dim3 grid(mc_div_up(p.cx, p.C) * batch);
     const size_t lds = (size_t)mc_lds_padded(p.C * p.H) * sizeof(cfloat);
-#define X(HH, CC, NT)                                                                       \
-    if (p.H == HH && p.C == CC && p.col_ptw) {                                             \
-        hipLaunchKernelGGL((k_cols_fwd_rp<HH, CC, NT>), grid, dim3(NT),                    \
-                           (size_t)CC * HH * sizeof(cfloat), s, a, (const cfloat*)p.col_ptw); \
-        return hipGetLastError();                                                          \
-    }
+#define X(HH, CC, NT)                                                              \
+    if (p.H == HH && p.C % CC == 0 && p.col_ptw)                                   \
+        return launch_cols_fwd_rp<HH, CC, NT>(a, p.col_ptw, batch, s_fmt, x_fmt, s);
     MC_RP_COL_SPECS(X)
 #undef X
+    if (s_fmt != MC_SPEC_F32 || x_fmt != MC_SPEC_F32) return hipErrorInvalidValue;
 #define X(HH, CC, NT)                                                             \
     if (p.H == HH && p.C == CC) {                                                 \
         hipLaunchKernelGGL((k_cols_fwd_ct<HH, CC, NT>), grid, dim3(NT), lds, s, a);  \
@@ -1162,13 +1274,14 @@ hipError_t mc_cols_fwd(const Fft2D& p, const cfloat* T, long long t_bstride, cfl
     }
     MC_COL_SPECS(X)
 #undef X
+    if ((long long)p.C * p.H > MC_LDS_ELEMS) return hipErrorInvalidConfiguration;
     hipLaunchKernelGGL(k_cols_fwd, grid, dim3(MC_NTHR), lds, s, a);
     return hipGetLastError();
 }
 
 hipError_t mc_cols_inv(const Fft2D& p, const cfloat* S, long long s_bstride, int src_cx,
                        int src_cy, const PhasorTabs* ph, cfloat* T, long long t_bstride, int batch,
-                       hipStream_t s) {
+                       hipStream_t s, int s_fmt, const float* s_inv) {
     ColInvArgs a;
     a.plan = p.col;
     a.H = p.H;
@@ -1182,17 +1295,15 @@ hipError_t mc_cols_inv(const Fft2D& p, const cfloat* S, long long s_bstride, int
     a.phB = ph ? ph->B : nullptr;
     a.T = T;
     a.t_bstride = t_bstride;
+    a.s_inv = s_inv;
     dim3 grid(mc_div_up(p.cx, p.C) * batch);
     const size_t lds = (size_t)mc_lds_padded(p.C * p.H) * sizeof(cfloat);
-    const bool same = (src_cx == p.cx && src_cy == p.H);
-#define X(HH, CC, NT)                                                                       \
-    if (p.H == HH && p.C == CC && p.col_ptw && same) {                                     \
-        hipLaunchKernelGGL((k_cols_inv_rp<HH, CC, NT>), grid, dim3(NT),                    \
-                           (size_t)CC * HH * sizeof(cfloat), s, a, (const cfloat*)p.col_ptw); \
-        return hipGetLastError();                                                          \
-    }
+#define X(HH, CC, NT)                                                              \
+    if (p.H == HH && p.C % CC == 0 && p.col_ptw)                                   \
+        return launch_cols_inv_rp<HH, CC, NT>(a, p.col_ptw, batch, s_fmt, s);
     MC_RP_COL_SPECS(X)
 #undef X
+    if (s_fmt != MC_SPEC_F32) return hipErrorInvalidValue;
 #define X(HH, CC, NT)                                                             \
     if (p.H == HH && p.C == CC) {                                                 \
         hipLaunchKernelGGL((k_cols_inv_ct<HH, CC, NT>), grid, dim3(NT), lds, s, a);  \
@@ -1200,6 +1311,7 @@ hipError_t mc_cols_inv(const Fft2D& p, const cfloat* S, long long s_bstride, int
     }
     MC_COL_SPECS(X)
 #undef X
+    if ((long long)p.C * p.H > MC_LDS_ELEMS) return hipErrorInvalidConfiguration;
     hipLaunchKernelGGL(k_cols_inv, grid, dim3(MC_NTHR), lds, s, a);
     return hipGetLastError();
 }

@@ -45,6 +45,8 @@ struct mc_ctx {
     std::string timed;
     std::vector<TimedEv> evs;
     size_t nev = 0;
+    // captured alignment-iteration blocks, keyed by everything they bake in
+    std::map<std::string, hipGraphExec_t> graphs;
 };
 
 namespace {
@@ -302,9 +304,15 @@ struct IterOut {
     std::vector<int> iters;      // [J]
 };
 
+// Element `elems` of a spectrum stack stored in format fmt (mc_spec.h).
+static cfloat* spec_at(void* base, long long elems, int fmt) {
+    return (cfloat*)((char*)base + elems * (fmt == MC_SPEC_F32 ? 8 : 4));
+}
+
 int run_iterative(mc_ctx* c, const char* tag, const cfloat* F, int J, int N, int W, int H,
                   Fft2D* pl, const Groups& grp, float bft, int phase_only, int max_iter,
-                  double tol, int quirks, IterOut* out) {
+                  double tol, int quirks, IterOut* out, int f_fmt = MC_SPEC_F32,
+                  const float* f_inv = nullptr, bool graphs = false) {
     const int cx = W / 2 + 1, G = grp.G;
     // linear group->frame interpolation needs two groups
     // (Align/CInterpolateShift.cpp:129-142 reads group 1)
@@ -352,29 +360,94 @@ int run_iterative(mc_ctx* c, const char* tag, const cfloat* F, int J, int N, int
     MC_TRY(hipMemcpyAsync(st.fgroup, grp.fgroup.data(), sizeof(int) * N, hipMemcpyHostToDevice, s));
     MC_TRY(hipMemcpyAsync(st.gcent, grp.gcent.data(), sizeof(double) * G, hipMemcpyHostToDevice, s));
     MC_TRY(hipMemcpyAsync(st.fcent, grp.fcent.data(), sizeof(double) * N, hipMemcpyHostToDevice, s));
-    float b = bft;
-    for (int it = 0; it < max_iter; it++) {
+    // One iteration (Align/CIterativeAlign.cpp:117-140): B-factor -2 per
+    // iteration with a floor of 20.
+    auto bfac = [&](int it) {
+        float b = bft;
+        for (int i = 0; i < it; i++) {
+            b -= 2;
+            if (b < 20) b = 20.0f;
+        }
+        return b;
+    };
+    auto enqueue = [&](int it, bool timed) -> hipError_t {
+        const float b = bfac(it);
+        hipError_t e;
         {
-            KTimer kt(c, "phasors");
-            MC_TRY(mc_launch_phasors(st.tmp, -1.0, 0, st.done, N, W, cx, H, J * N, A, B, s));
+            KTimer kt(c, timed ? "phasors" : "");
+            if ((e = mc_launch_phasors(st.tmp, -1.0, 0, st.done, N, W, cx, H, J * N, A, B, s))) return e;
         }
         {
-            KTimer kt(c, "aligned_sum");
-            MC_TRY(mc_launch_aligned_sum(F, A, B, st.done, J, N, cx, H, S, s));
+            KTimer kt(c, timed ? "aligned_sum" : "");
+            if ((e = mc_launch_aligned_sum(F, A, B, st.done, J, N, cx, H, S, s, f_fmt, f_inv))) return e;
         }
         {
-            KTimer kt(c, "correlate");
-            MC_TRY(mc_launch_correlate(pl->col, pl->col_ptw, pl->C, F, S, A, B, st, cx, H, b, phase_only, seay,
-                                       T1, s));
+            KTimer kt(c, timed ? "correlate" : "");
+            if ((e = mc_launch_correlate(pl->col, pl->col_ptw, pl->C, F, S, A, B, st, cx, H, b,
+                                         phase_only, seay, T1, s, f_fmt, f_inv)))
+                return e;
         }
         {
-            KTimer kt(c, "row_dft");
-            MC_TRY(mc_launch_row_dft(T1, twW, st.done, J, G, cx, W, seax, seay, win, s));
+            KTimer kt(c, timed ? "row_dft" : "");
+            if ((e = mc_launch_row_dft(T1, twW, st.done, J, G, cx, W, seax, seay, win, s))) return e;
         }
-        MC_TRY(mc_launch_peaks(win, st.done, J, G, seax, seay, st.gshift, s));
-        MC_TRY(mc_launch_update(st, it, tol, s));
-        b -= 2;
-        if (b < 20) b = 20.0f;
+        if ((e = mc_launch_peaks(win, st.done, J, G, seax, seay, st.gshift, s))) return e;
+        return mc_launch_update(st, it, tol, s);
+    };
+    // Blocks of two iterations; with `graphs` each block is captured once
+    // into a HIP graph (no memset nodes) and replayed.  Kernel timing needs
+    // the direct launches.
+    const bool use_graph = graphs && c->timed.empty();
+    std::vector<unsigned char> hdone(J);
+    for (int it0 = 0; it0 < max_iter; it0 += 2) {
+        const int it1 = std::min(it0 + 2, max_iter);
+        if (use_graph) {
+            char key[1024];
+            snprintf(key, sizeof(key),
+                     "%s|%d|%d|%d %d %d %d %d %d|%.9g %.17g %d %d %d %d|%p %p %p %p %p %p %p %p %p %p "
+                     "%p %p %p %p %p %p %p %p %p %p %p %p %d %p",
+                     tag, it0, it1, J, N, W, H, G, max_iter, (double)bft, tol, phase_only, seax,
+                     seay, pl->C, (const void*)F, (void*)A, (void*)B, (void*)S, (void*)T1,
+                     (void*)win, (void*)twW, (void*)st.tmp, (void*)st.total, (void*)st.gshift,
+                     (void*)st.final_, (void*)st.errors, (void*)st.iters, (void*)st.done,
+                     (void*)st.gstart, (void*)st.gsize, (void*)st.fgroup, (void*)st.gcent,
+                     (void*)st.fcent, (const void*)pl->col_ptw, (const void*)pl->col.tw,
+                     (const void*)f_inv, f_fmt, (const void*)pl);
+            auto itg = c->graphs.find(key);
+            hipGraphExec_t ex = nullptr;
+            if (itg == c->graphs.end()) {
+                MC_TRY(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+                hipError_t e = hipSuccess;
+                for (int it = it0; it < it1 && e == hipSuccess; it++) e = enqueue(it, false);
+                hipGraph_t gr = nullptr;
+                const hipError_t e2 = hipStreamEndCapture(s, &gr);
+                if (e != hipSuccess || e2 != hipSuccess || !gr) {
+                    if (gr) hipGraphDestroy(gr);
+                    return fail(c, MC_E_HIP, std::string("iteration graph capture: ") +
+                                                 hipGetErrorString(e != hipSuccess ? e : e2));
+                }
+                const hipError_t e3 = hipGraphInstantiate(&ex, gr, nullptr, nullptr, 0);
+                hipGraphDestroy(gr);
+                if (e3 != hipSuccess) return fail(c, MC_E_HIP, "iteration graph instantiation");
+                c->graphs[key] = ex;
+            } else {
+                ex = itg->second;
+            }
+            MC_TRY(hipGraphLaunch(ex, s));
+        } else {
+            for (int it = it0; it < it1; it++) MC_TRY(enqueue(it, true));
+        }
+        // A converged job skips every kernel of the later iterations, so once
+        // all jobs are done the remaining launches would change nothing: look
+        // at the flags after every block of two iterations (2 is the
+        // reference's minimum, Align/CIterativeAlign.cpp:144) and stop.
+        if (it1 < max_iter) {
+            MC_TRY(hipMemcpyAsync(hdone.data(), st.done, J, hipMemcpyDeviceToHost, s));
+            MC_TRY(hipStreamSynchronize(s));
+            bool all = true;
+            for (int j = 0; j < J; j++) all = all && hdone[j];
+            if (all) break;
+        }
     }
     MC_TRY(mc_launch_finalize(st, grp.binz, quirks, s));
     out->shifts.resize((size_t)J * N * 2);
@@ -502,17 +575,39 @@ int align_movie(mc_ctx* c, const mc_params* p, const mc_movie* mv, const mc_over
     Fft2D* pp = patch ? plan(c, g.pw, g.ph) : nullptr;
     Fft2D* po = (g.ow == W && g.oh == H) ? pf : plan(c, g.ow, g.oh);
     if (!pf || !px || (patch && !pp) || !po) return fail(c, MC_E_ARG, "unsupported FFT size");
+    if (p->spectra_fmt < MC_SPEC_F32 || p->spectra_fmt > MC_SPEC_BF16)
+        return fail(c, MC_E_ARG, "spectra_fmt must be 0 (FP32), 1 (FP16) or 2 (BF16)");
 
     const long long fs = (long long)g.cx * H;
     const long long xfs = (long long)g.xcx * g.xh;
     const bool xcf_alias = (g.xcx == g.cx && g.xh == H);
+    // A reduced stack format needs the register-pass kernels on every stage
+    // that writes or reads the stack; otherwise the stack stays FP32.
+    int fx = p->spectra_fmt, fpch = p->spectra_fmt;
+    if (xcf_alias || replay || !mc_fft_rp_rows(*pf) || !mc_fft_rp_cols(*pf) ||
+        !mc_correlate_has_rp(g.xh) || (patch && !mc_fft_rp_cols(*px)))
+        fx = MC_SPEC_F32;
+    if (!patch || replay || !mc_fft_rp_rows(*pp) || !mc_fft_rp_cols(*pp) ||
+        !mc_correlate_has_rp(g.ph))
+        fpch = MC_SPEC_F32;
+    res->spectra_fmt_used[0] = fx;
+    res->spectra_fmt_used[1] = fpch;
     const int CH = 2;  // frames per forward/inverse chunk (staging stays in the 256 MB MALL)
     MC_BUF(Sf, cfloat, "Sf", (size_t)N * fs);
     MC_BUF(T, cfloat, "T", (size_t)CH * pf->t_elems);
     cfloat* Sx = Sf;
     if (!xcf_alias && !replay) {
-        Sx = (cfloat*)buf(c, "Sx", sizeof(cfloat) * (size_t)N * xfs);
+        Sx = (cfloat*)buf(c, "Sx", (fx == MC_SPEC_F32 ? 8 : 4) * (size_t)N * xfs);
         if (!Sx) return fail(c, MC_E_NOMEM, "xcf stack");
+    }
+    float* dcf = nullptr;   // |row DC| of the chunk's frames, and per-frame factors
+    float* gx = nullptr;
+    float* ix = nullptr;
+    if (fx != MC_SPEC_F32) {
+        dcf = (float*)buf(c, "dcf", sizeof(float) * (size_t)CH * H);
+        gx = (float*)buf(c, "gx", sizeof(float) * (size_t)N);
+        ix = (float*)buf(c, "ix", sizeof(float) * (size_t)N);
+        if (!dcf || !gx || !ix) return fail(c, MC_E_NOMEM, "stack scale factors");
     }
 
     // Align/CTransformStack.cpp + Align/CGenXcfStack.cpp: normalised R2C of
@@ -522,14 +617,17 @@ int align_movie(mc_ctx* c, const mc_params* p, const mc_movie* mv, const mc_over
         const int nb = std::min(CH, N - f0);
         {
             KTimer kt(c, "fft_rows_r2c");
-            MC_TRY(mc_rows_r2c(*pf, mv->frames + (long long)f0 * mv->frame_stride, mv->pitch,
-                               mv->frame_stride, nullptr, T, pf->t_elems, nb, scale, s));
+            MC_TRY(mc_rows_r2c_mod(*pf, mv->frames + (long long)f0 * mv->frame_stride, mv->pitch,
+                                   mv->frame_stride, 0, nullptr, T, pf->t_elems, nb, scale, s,
+                                   dcf));
         }
+        if (fx != MC_SPEC_F32) MC_TRY(mc_launch_spec_scale(dcf, H, nb, gx + f0, ix + f0, s));
         {
             KTimer kt(c, "fft_cols_fwd");
             MC_TRY(mc_cols_fwd(*pf, T, pf->t_elems, Sf + (long long)f0 * fs, fs,
-                               (xcf_alias || replay) ? nullptr : Sx + (long long)f0 * xfs, g.xcx,
-                               g.xh, xfs, nb, s));
+                               (xcf_alias || replay) ? nullptr : spec_at(Sx, (long long)f0 * xfs, fx),
+                               g.xcx, g.xh, xfs, nb, s, MC_SPEC_F32, nullptr, fx,
+                               fx != MC_SPEC_F32 ? gx + f0 : nullptr));
         }
     }
 
@@ -548,7 +646,7 @@ int align_movie(mc_ctx* c, const mc_params* p, const mc_movie* mv, const mc_over
         Groups g0 = make_groups(p->group[0], N, p->throw0);
         IterOut go;
         MC_CHECK(run_iterative(c, "g_", Sx, 1, N, g.xw, g.xh, px, g0, p->bft[0], p->phase_only,
-                               p->iter, tol, quirks, &go));
+                               p->iter, tol, quirks, &go, fx, ix, p->launch_graphs != 0));
         giters = go.iters[0];
         gerr.assign(go.errors.begin(), go.errors.begin() + giters);
         std::vector<double> xs = go.shifts;
@@ -560,7 +658,7 @@ int align_movie(mc_ctx* c, const mc_params* p, const mc_movie* mv, const mc_over
             MC_BUF(xsum, cfloat, "xsum", (size_t)xfs);
             MC_TRY(hipMemcpyAsync(fin, xs.data(), sizeof(double) * 2 * N, hipMemcpyHostToDevice, s));
             MC_TRY(mc_launch_phasors(fin, -1.0, 0, nullptr, N, g.xw, g.xcx, g.xh, N, A, B, s));
-            MC_TRY(mc_launch_aligned_sum(Sx, A, B, nullptr, 1, N, g.xcx, g.xh, xsum, s));
+            MC_TRY(mc_launch_aligned_sum(Sx, A, B, nullptr, 1, N, g.xcx, g.xh, xsum, s, fx, ix));
             Fft2D* p128 = plan(c, 128, 128);
             if (!p128) return fail(c, MC_E_ARG, "128x128 plan");
             MC_BUF(T128, cfloat, "T128", (size_t)p128->t_elems);
@@ -579,21 +677,32 @@ int align_movie(mc_ctx* c, const mc_params* p, const mc_movie* mv, const mc_over
             MC_BUF(Rx, float, "Rx", (size_t)N * g.xw * g.xh);
             MC_TRY(mc_launch_phasors(fin, -1.0, 1, nullptr, N, g.xw, g.xcx, g.xh, N, A, B, s));
             PhasorTabs ph{A, B};
-            MC_TRY(mc_cols_inv(*px, Sx, xfs, g.xcx, g.xh, &ph, Tx, px->t_elems, N, s));
+            MC_TRY(mc_cols_inv(*px, Sx, xfs, g.xcx, g.xh, &ph, Tx, px->t_elems, N, s, fx, ix));
             MC_TRY(mc_rows_c2r(*px, Tx, px->t_elems, Rx, g.xw, (long long)g.xw * g.xh, N, 0, s));
             // patches: CExtractPatch + unnormalised R2C, all patches and frames at once
             MC_BUF(dstart, int, "starts", 2 * P);
             MC_TRY(hipMemcpyAsync(dstart, starts.data(), sizeof(int) * 2 * P, hipMemcpyHostToDevice, s));
             const long long pfs = (long long)g.pcx * g.ph;
             MC_BUF(Tp, cfloat, "Tp", (size_t)P * N * pp->t_elems);
-            MC_BUF(Sp, cfloat, "Sp", (size_t)P * N * pfs);
+            cfloat* Sp = (cfloat*)buf(c, "Sp", (fpch == MC_SPEC_F32 ? 8 : 4) * (size_t)P * N * pfs);
+            if (!Sp) return fail(c, MC_E_NOMEM, "patch stack");
+            float* dcp = nullptr;
+            float* gp = nullptr;
+            float* ip = nullptr;
+            if (fpch != MC_SPEC_F32) {
+                dcp = (float*)buf(c, "dcp", sizeof(float) * (size_t)P * N * g.ph);
+                gp = (float*)buf(c, "gp", sizeof(float) * (size_t)P * N);
+                ip = (float*)buf(c, "ip", sizeof(float) * (size_t)P * N);
+                if (!dcp || !gp || !ip) return fail(c, MC_E_NOMEM, "patch scale factors");
+            }
             MC_TRY(mc_rows_r2c_mod(*pp, Rx, g.xw, (long long)g.xw * g.xh, N, dstart, Tp, pp->t_elems,
-                                   P * N, 1.0f, s));
-            MC_TRY(mc_cols_fwd(*pp, Tp, pp->t_elems, Sp, pfs, nullptr, 0, 0, 0, P * N, s));
+                                   P * N, 1.0f, s, dcp));
+            if (fpch != MC_SPEC_F32) MC_TRY(mc_launch_spec_scale(dcp, g.ph, P * N, gp, ip, s));
+            MC_TRY(mc_cols_fwd(*pp, Tp, pp->t_elems, Sp, pfs, nullptr, 0, 0, 0, P * N, s, fpch, gp));
             Groups g1 = make_groups(p->group[1], N, p->throw0);
             IterOut po_;
             MC_CHECK(run_iterative(c, "p_", Sp, P, N, g.pw, g.ph, pp, g1, p->bft[1], p->phase_only,
-                                   p->iter, tol, quirks, &po_));
+                                   p->iter, tol, quirks, &po_, fpch, ip, p->launch_graphs != 0));
             // Align/CMeasurePatches.cpp:66-82: centre and shifts in full-res px
             ps.assign((size_t)N * P * 2, 0.0);
             pc.assign(2 * P, 0.0);
@@ -765,6 +874,7 @@ void mc_default_params(mc_params* p) {
     p->sum_range[0] = 3.0f;
     p->sum_range[1] = 25.0f;
     p->ref_quirks = 1;
+    p->launch_graphs = 1;
 }
 
 int mc_ctx_create(int device, void* stream, mc_ctx** out) {
@@ -806,6 +916,7 @@ int mc_ctx_destroy(mc_ctx* c) {
         hipEventDestroy(e.a);
         hipEventDestroy(e.b);
     }
+    for (auto& kv : c->graphs) hipGraphExecDestroy(kv.second);
     if (c->own_stream) hipStreamDestroy(c->stream);
     delete c;
     return MC_OK;

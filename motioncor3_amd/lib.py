@@ -28,6 +28,7 @@ class McParams(C.Structure):
         ("fm_ref", C.c_int), ("ft_bin", C.c_float), ("sum_range", C.c_float * 2),
         ("split_sum", C.c_int), ("throw0", C.c_int), ("ref_quirks", C.c_int),
         ("log_dir", C.c_char_p), ("movie_name", C.c_char_p), ("verbose", C.c_int),
+        ("spectra_fmt", C.c_int), ("launch_graphs", C.c_int),
     ]
 
 
@@ -44,7 +45,7 @@ class McResult(C.Structure):
         ("patch_shift", C.POINTER(C.c_float)), ("patch_bad", C.POINTER(C.c_ubyte)),
         ("patch_center", C.POINTER(C.c_float)), ("patch_starts", C.POINTER(C.c_int)),
         ("patch_iters", C.POINTER(C.c_int)), ("seconds_align", C.c_double),
-        ("seconds_total", C.c_double),
+        ("seconds_total", C.c_double), ("spectra_fmt_used", C.c_int * 2),
     ]
 
 
@@ -369,4 +370,5 @@ class Context:
         out["num_sums"] = ns
         out["seconds_align"] = res.seconds_align
         out["seconds_total"] = res.seconds_total
+        out["spectra_fmt_used"] = (res.spectra_fmt_used[0], res.spectra_fmt_used[1])
         return out

@@ -98,3 +98,71 @@ def test_production_geometry_ground_truth(ctx):
     assert abs(raw.mean() / (N * 1.0) - 1.0) < 0.05
     assert res["patch_shift"].shape == (N, 25, 2)
     assert np.isfinite(res["patch_shift"]).all()
+
+
+def test_fourier_binned_output(ctx):
+    """-FtBin 2: every sum is Fourier-cropped to half size before the C2R
+    (Correct/CCorrectFullShift.cpp:146-185); 1152 x 816 frames give a
+    576 x 408 output (cropping inverse column pass and 288-point rows)."""
+    W, H, N = 1152, 816, 6
+    r = MovieRecipe(W, H, N, 150, seed=33, dose=2.0)
+    mv = r.movie()
+    pg, po = _params(iter=5, pix_size=1.0, kv=300, fm_dose=1.0, ft_bin=2.0)
+    gres = ctx.align_movie(ctx.upload(mv), pg)
+    assert (gres["sums"][0].shape[0], gres["sums"][0].shape[1]) == (408, 576)
+    orep = O.align_movie(mv, po, replay_global=gres["global_shift"])
+    assert orep["sums"].shape[1:] == (408, 576)
+    for i in range(gres["num_sums"]):
+        assert _rel(gres["sums"][i].numpy(), orep["sums"][i]) < 1e-4, i
+
+
+@pytest.mark.parametrize("fmt", [1, 2])
+def test_reduced_precision_stacks(ctx, fmt):
+    """FP16 (1) and BF16 (2) storage of the xcf and patch spectrum stacks
+    (FP32 arithmetic, per-transform scale) on the 5760 x 4092 geometry with
+    5 x 5 patches: the shifts stay within the north-star bars of the FP32
+    path, which the other tests pin to the FP64 oracle."""
+    W, H, N = 5760, 4092, 8
+    r = MovieRecipe(W, H, N, 20000, seed=77, dose=1.0, local_max=3.0)
+    frames = ctx.empty((N, H, W), np.float32)
+    ctx.synth_render(frames, r)
+    kw = dict(patch=(5, 5), bft=(500, 100), pix_size=0.83, kv=300, fm_dose=1.0)
+    ref = ctx.align_movie(frames, L.default_params(**kw))
+    red = ctx.align_movie(frames, L.default_params(spectra_fmt=fmt, **kw))
+    assert ref["spectra_fmt_used"] == (0, 0)
+    assert red["spectra_fmt_used"] == (fmt, fmt)
+    assert np.max(np.abs(red["global_shift"] - ref["global_shift"])) <= 0.05
+    assert np.array_equal(red["patch_starts"], ref["patch_starts"])
+    d = red["patch_shift"] - ref["patch_shift"]
+    assert float(np.sqrt(np.mean(np.sum(d ** 2, axis=2)))) <= 0.1
+    # the sums use FP32 full spectra: replaying the reduced run's shift
+    # tables through the FP32 path gives the same sums (the nearest-pixel
+    # warp turns small shift differences into discrete changes, so the sums
+    # of the two runs differ by more than their shifts suggest)
+    rep = ctx.align_movie(frames, L.default_params(**kw), replay_global=red["global_shift"],
+                          replay_patch=red["patch_shift"], replay_centers=red["patch_center"],
+                          replay_bad=red["patch_bad"])
+    for a, b in zip(red["sums"], rep["sums"]):
+        assert _rel(a.numpy(), b.numpy()) < 1e-5
+    print("fmt %d: sum rel-L2 vs the FP32 run %.2e" % (fmt, _rel(red["sums"][0].numpy(),
+                                                                  ref["sums"][0].numpy())))
+
+
+def test_iteration_graphs_match_direct_launches(ctx):
+    """The alignment iterations replayed from captured HIP graphs give the
+    same bits as direct launches, on the capturing call and on later
+    replays with direct work in between."""
+    W, H, N = 1024, 1024, 12
+    r = MovieRecipe(W, H, N, 400, seed=5, dose=4.0, local_max=2.0)
+    mv = ctx.upload(r.movie())
+    kw = dict(patch=(3, 3), iter=10, pix_size=1.0, kv=300, fm_dose=1.0)
+    direct = ctx.align_movie(mv, L.default_params(launch_graphs=0, **kw))
+    for rep in range(3):
+        g = ctx.align_movie(mv, L.default_params(launch_graphs=1, **kw))
+        assert np.array_equal(g["global_shift"], direct["global_shift"]), rep
+        assert np.array_equal(g["patch_shift"], direct["patch_shift"]), rep
+        assert g["global_iters"] == direct["global_iters"]
+        for a, b in zip(g["sums"], direct["sums"]):
+            assert np.array_equal(a.numpy(), b.numpy()), rep
+        # direct work between replays
+        ctx.align_movie(mv, L.default_params(launch_graphs=0, **kw))
