@@ -141,6 +141,18 @@ int mc_memcpy(mc_ctx* ctx, void* dst, const void* src, unsigned long long bytes,
 int mc_align_movie(mc_ctx* ctx, const mc_params* p, const mc_movie* movie,
                    const mc_override* ov, mc_result* res);
 
+/* ---- several movies: no reference counterpart (the reference aligns one
+ * movie per CProcessThread pass, CProcessThread.cpp:178-182) ----
+ * Movies of equal size and frame count run their alignment iterations as one
+ * batch of jobs (the global alignment of each movie and every movie's patches
+ * share the iteration launches); results are bit-identical to separate
+ * mc_align_movie calls.  Movies of other sizes go in later batches, and a
+ * batch holds the per-movie stacks of at most half the device memory.
+ * params[m] names movie m's logs; every other field must equal params[0]'s.
+ * results[m] as for mc_align_movie.  No overrides. */
+int mc_align_movies(mc_ctx* ctx, const mc_params* params, const mc_movie* movies, int count,
+                    mc_result* results);
+
 /* ---- stage entry points ---- */
 /* Batched 2-D R2C (cufftExecR2C, Util/CCufft2D.cpp:65-87): `batch` real
  * images (rows of W, pitch, image stride) -> half spectra [kx][ky], each

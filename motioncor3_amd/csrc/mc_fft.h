@@ -23,7 +23,7 @@
 #endif
 // staging chunk width for heights with a register-pass column schedule
 #ifndef MC_TILE_CT
-#define MC_TILE_CT 8
+#define MC_TILE_CT 4
 #endif
 
 struct Fft2D {

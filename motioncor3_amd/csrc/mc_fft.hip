@@ -582,7 +582,7 @@ __global__ __launch_bounds__(NT) void k_cols_inv_ct(ColInvArgs a) {
 // patches 576x408; the 128^2 feature map.
 #ifndef MC_ROW_SPECS
 #define MC_ROW_SPECS(X) \
-    X(2880, 2, 8) X(2048, 2, 8) X(512, 2, 8) X(1441, 2, 8) X(1024, 2, 8) X(288, 2, 16) X(64, 2, 16)
+    X(2880, 2, MC_TILE_CT) X(2048, 2, MC_TILE_CT) X(512, 2, 8) X(1441, 2, MC_TILE_CT) X(1024, 2, 8) X(288, 2, 16) X(64, 2, 16)
 #endif
 #ifndef MC_COL_SPECS
 #define MC_COL_SPECS(X) \
@@ -917,7 +917,7 @@ __global__ __launch_bounds__(NT) void k_rows_c2r_rp(RowC2RArgs a, const cfloat* 
 }
 
 #ifndef MC_RP_ROW_SPECS
-#define MC_RP_ROW_SPECS(X) X(2880, 2, 8, 512) X(2048, 2, 8, 512) X(288, 16, 16, 512)
+#define MC_RP_ROW_SPECS(X) X(2880, 2, MC_TILE_CT, 512) X(2048, 2, MC_TILE_CT, 512) X(288, 16, 16, 512)
 #endif
 
 

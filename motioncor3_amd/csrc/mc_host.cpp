@@ -538,14 +538,25 @@ double kv_factor(int kv) {
 }
 
 // ---------------------------------------------------------------------------
-int align_movie(mc_ctx* c, const mc_params* p, const mc_movie* mv, const mc_override* ov,
-                mc_result* res) {
+// `count` movies of one geometry.  The alignment iterations of all movies
+// run as one batch of jobs (the global alignment of movie m is job m, patch
+// q of movie m is job m * P + q); every job's arithmetic is the same as in a
+// batch of one, so results do not depend on the batching.  Overrides need
+// count == 1.  pp[m] supplies movie m's log name; everything else comes
+// from pp[0].
+int align_batch(mc_ctx* c, const mc_params* const* pp_, const mc_movie* const* mvs,
+                const mc_override* ov, mc_result* const* rs, int count) {
     using clk = std::chrono::steady_clock;
     const auto t0 = clk::now();
-    const int W = mv->W, H = mv->H, N = mv->N;
-    if (!mv->frames || W < 8 || (W % 2) || H < 8 || N < 2 || mv->pitch < W ||
-        mv->frame_stride < mv->pitch * H)
-        return fail(c, MC_E_ARG, "invalid movie geometry");
+    if (count < 1 || (count > 1 && ov)) return fail(c, MC_E_ARG, "overrides need a single movie");
+    const mc_params* p = pp_[0];
+    const int W = mvs[0]->W, H = mvs[0]->H, N = mvs[0]->N;
+    for (int m = 0; m < count; m++) {
+        const mc_movie* mv = mvs[m];
+        if (!mv->frames || mv->W != W || mv->H != H || mv->N != N || W < 8 || (W % 2) || H < 8 ||
+            N < 2 || mv->pitch < W || mv->frame_stride < mv->pitch * H)
+            return fail(c, MC_E_ARG, "invalid movie geometry");
+    }
     if (p->iter < 1) return fail(c, MC_E_ARG, "-Iter must be >= 1");
     const bool patch = p->patch[0] > 1 && p->patch[1] > 1;
     const Geo g = make_geo(W, H, N, p->patch[0], p->patch[1], p->ft_bin);
@@ -564,11 +575,13 @@ int align_movie(mc_ctx* c, const mc_params* p, const mc_movie* mv, const mc_over
     const bool dws = dw && !(p->sum_range[0] < 0) && !(p->sum_range[1] <= 0) &&
                      (p->sum_range[1] - p->sum_range[0] > 1.0f);
     const int nsum = 1 + (p->split_sum ? 2 : 0) + (dw ? 1 + (dws ? 1 : 0) : 0);
-    res->num_sums = nsum;
-    res->out_w = g.ow;
-    res->out_h = g.oh;
-    for (int i = 0; i < nsum; i++)
-        if (!res->sums[i]) return fail(c, MC_E_ARG, "missing output sum buffer");
+    for (int m = 0; m < count; m++) {
+        rs[m]->num_sums = nsum;
+        rs[m]->out_w = g.ow;
+        rs[m]->out_h = g.oh;
+        for (int i = 0; i < nsum; i++)
+            if (!rs[m]->sums[i]) return fail(c, MC_E_ARG, "missing output sum buffer");
+    }
 
     Fft2D* pf = plan(c, W, H);
     Fft2D* px = plan(c, g.xw, g.xh);
@@ -590,67 +603,104 @@ int align_movie(mc_ctx* c, const mc_params* p, const mc_movie* mv, const mc_over
     if (!patch || replay || !mc_fft_rp_rows(*pp) || !mc_fft_rp_cols(*pp) ||
         !mc_correlate_has_rp(g.ph))
         fpch = MC_SPEC_F32;
-    res->spectra_fmt_used[0] = fx;
-    res->spectra_fmt_used[1] = fpch;
+    for (int m = 0; m < count; m++) {
+        rs[m]->spectra_fmt_used[0] = fx;
+        rs[m]->spectra_fmt_used[1] = fpch;
+    }
     const int CH = 2;  // frames per forward/inverse chunk (staging stays in the 256 MB MALL)
-    MC_BUF(Sf, cfloat, "Sf", (size_t)N * fs);
+    // movie m's full spectra at Sf_all + m * N * fs, its xcf stack at element
+    // m * N * xfs of Sx_all, its patch stack at element m * P * N * pfs of Sp_all
+    MC_BUF(Sf_all, cfloat, "Sf", (size_t)count * N * fs);
     MC_BUF(T, cfloat, "T", (size_t)CH * pf->t_elems);
-    cfloat* Sx = Sf;
+    cfloat* Sx_all = Sf_all;
     if (!xcf_alias && !replay) {
-        Sx = (cfloat*)buf(c, "Sx", (fx == MC_SPEC_F32 ? 8 : 4) * (size_t)N * xfs);
-        if (!Sx) return fail(c, MC_E_NOMEM, "xcf stack");
+        Sx_all = (cfloat*)buf(c, "Sx", (fx == MC_SPEC_F32 ? 8 : 4) * (size_t)count * N * xfs);
+        if (!Sx_all) return fail(c, MC_E_NOMEM, "xcf stack");
     }
     float* dcf = nullptr;   // |row DC| of the chunk's frames, and per-frame factors
     float* gx = nullptr;
     float* ix = nullptr;
     if (fx != MC_SPEC_F32) {
         dcf = (float*)buf(c, "dcf", sizeof(float) * (size_t)CH * H);
-        gx = (float*)buf(c, "gx", sizeof(float) * (size_t)N);
-        ix = (float*)buf(c, "ix", sizeof(float) * (size_t)N);
+        gx = (float*)buf(c, "gx", sizeof(float) * (size_t)count * N);
+        ix = (float*)buf(c, "ix", sizeof(float) * (size_t)count * N);
         if (!dcf || !gx || !ix) return fail(c, MC_E_NOMEM, "stack scale factors");
     }
 
     // Align/CTransformStack.cpp + Align/CGenXcfStack.cpp: normalised R2C of
     // every frame and the Fourier crop to the xcf size in the same pass.
     const float scale = (float)(1.0 / ((double)W * H));
-    for (int f0 = 0; f0 < N; f0 += CH) {
-        const int nb = std::min(CH, N - f0);
-        {
-            KTimer kt(c, "fft_rows_r2c");
-            MC_TRY(mc_rows_r2c_mod(*pf, mv->frames + (long long)f0 * mv->frame_stride, mv->pitch,
-                                   mv->frame_stride, 0, nullptr, T, pf->t_elems, nb, scale, s,
-                                   dcf));
-        }
-        if (fx != MC_SPEC_F32) MC_TRY(mc_launch_spec_scale(dcf, H, nb, gx + f0, ix + f0, s));
-        {
-            KTimer kt(c, "fft_cols_fwd");
-            MC_TRY(mc_cols_fwd(*pf, T, pf->t_elems, Sf + (long long)f0 * fs, fs,
-                               (xcf_alias || replay) ? nullptr : spec_at(Sx, (long long)f0 * xfs, fx),
-                               g.xcx, g.xh, xfs, nb, s, MC_SPEC_F32, nullptr, fx,
-                               fx != MC_SPEC_F32 ? gx + f0 : nullptr));
+    for (int m = 0; m < count; m++) {
+        const mc_movie* mv = mvs[m];
+        for (int f0 = 0; f0 < N; f0 += CH) {
+            const int nb = std::min(CH, N - f0);
+            const long long fi = (long long)m * N + f0;
+            {
+                KTimer kt(c, "fft_rows_r2c");
+                MC_TRY(mc_rows_r2c_mod(*pf, mv->frames + (long long)f0 * mv->frame_stride, mv->pitch,
+                                       mv->frame_stride, 0, nullptr, T, pf->t_elems, nb, scale, s,
+                                       dcf));
+            }
+            if (fx != MC_SPEC_F32) MC_TRY(mc_launch_spec_scale(dcf, H, nb, gx + fi, ix + fi, s));
+            {
+                KTimer kt(c, "fft_cols_fwd");
+                MC_TRY(mc_cols_fwd(*pf, T, pf->t_elems, Sf_all + fi * fs, fs,
+                                   (xcf_alias || replay) ? nullptr : spec_at(Sx_all, fi * xfs, fx),
+                                   g.xcx, g.xh, xfs, nb, s, MC_SPEC_F32, nullptr, fx,
+                                   fx != MC_SPEC_F32 ? gx + fi : nullptr));
+            }
         }
     }
 
-    std::vector<double> gshift(2 * N, 0.0);
-    std::vector<float> gsf(2 * N);
-    std::vector<double> ps, pc;
-    std::vector<unsigned char> bad;
-    std::vector<int> starts(2 * P, 0);
-    int giters = 0;
-    std::vector<double> gerr;
-    std::vector<int> piters(P, 0);
+    struct MovieState {
+        std::vector<double> gshift, ps, pc, gerr, xs;
+        std::vector<float> gsf;
+        std::vector<unsigned char> bad;
+        std::vector<int> starts, piters;
+        int giters = 0;
+    };
+    std::vector<MovieState> ms(count);
+    for (auto& q : ms) {
+        q.gshift.assign(2 * N, 0.0);
+        q.gsf.assign(2 * N, 0.f);
+        q.starts.assign(2 * P, 0);
+        q.piters.assign(P, 0);
+    }
     const double tol = p->tol / std::max(g.xbin[0], g.xbin[1]);
     const auto ta = clk::now();
 
     if (!replay) {
         Groups g0 = make_groups(p->group[0], N, p->throw0);
         IterOut go;
-        MC_CHECK(run_iterative(c, "g_", Sx, 1, N, g.xw, g.xh, px, g0, p->bft[0], p->phase_only,
-                               p->iter, tol, quirks, &go, fx, ix, p->launch_graphs != 0));
-        giters = go.iters[0];
-        gerr.assign(go.errors.begin(), go.errors.begin() + giters);
-        std::vector<double> xs = go.shifts;
+        MC_CHECK(run_iterative(c, "g_", Sx_all, count, N, g.xw, g.xh, px, g0, p->bft[0],
+                               p->phase_only, p->iter, tol, quirks, &go, fx, ix,
+                               p->launch_graphs != 0));
+        for (int m = 0; m < count; m++) {
+            ms[m].giters = go.iters[m];
+            ms[m].gerr.assign(go.errors.begin() + (size_t)m * p->iter,
+                              go.errors.begin() + (size_t)m * p->iter + ms[m].giters);
+            ms[m].xs.assign(go.shifts.begin() + (size_t)m * N * 2,
+                            go.shifts.begin() + (size_t)(m + 1) * N * 2);
+        }
         if (patch) {
+            const long long pfs = (long long)g.pcx * g.ph;
+            MC_BUF(Tp, cfloat, "Tp", (size_t)P * N * pp->t_elems);
+            cfloat* Sp_all = (cfloat*)buf(c, "Sp", (fpch == MC_SPEC_F32 ? 8 : 4) * (size_t)count * P * N * pfs);
+            if (!Sp_all) return fail(c, MC_E_NOMEM, "patch stack");
+            float* dcp = nullptr;
+            float* gp = nullptr;
+            float* ip = nullptr;
+            if (fpch != MC_SPEC_F32) {
+                dcp = (float*)buf(c, "dcp", sizeof(float) * (size_t)P * N * g.ph);
+                gp = (float*)buf(c, "gp", sizeof(float) * (size_t)count * P * N);
+                ip = (float*)buf(c, "ip", sizeof(float) * (size_t)count * P * N);
+                if (!dcp || !gp || !ip) return fail(c, MC_E_NOMEM, "patch scale factors");
+            }
+          for (int m = 0; m < count; m++) {
+            std::vector<double>& xs = ms[m].xs;
+            std::vector<int>& starts = ms[m].starts;
+            cfloat* Sx = spec_at(Sx_all, (long long)m * N * xfs, fx);
+            const float* ixm = ix ? ix + (long long)m * N : nullptr;
             // Align/CFullAlign.cpp:147-159
             MC_BUF(fin, double, "xfinal", 2 * N);
             MC_BUF(A, cfloat, "xphA", (size_t)N * g.xcx);
@@ -658,7 +708,7 @@ int align_movie(mc_ctx* c, const mc_params* p, const mc_movie* mv, const mc_over
             MC_BUF(xsum, cfloat, "xsum", (size_t)xfs);
             MC_TRY(hipMemcpyAsync(fin, xs.data(), sizeof(double) * 2 * N, hipMemcpyHostToDevice, s));
             MC_TRY(mc_launch_phasors(fin, -1.0, 0, nullptr, N, g.xw, g.xcx, g.xh, N, A, B, s));
-            MC_TRY(mc_launch_aligned_sum(Sx, A, B, nullptr, 1, N, g.xcx, g.xh, xsum, s, fx, ix));
+            MC_TRY(mc_launch_aligned_sum(Sx, A, B, nullptr, 1, N, g.xcx, g.xh, xsum, s, fx, ixm));
             Fft2D* p128 = plan(c, 128, 128);
             if (!p128) return fail(c, MC_E_ARG, "128x128 plan");
             MC_BUF(T128, cfloat, "T128", (size_t)p128->t_elems);
@@ -677,181 +727,203 @@ int align_movie(mc_ctx* c, const mc_params* p, const mc_movie* mv, const mc_over
             MC_BUF(Rx, float, "Rx", (size_t)N * g.xw * g.xh);
             MC_TRY(mc_launch_phasors(fin, -1.0, 1, nullptr, N, g.xw, g.xcx, g.xh, N, A, B, s));
             PhasorTabs ph{A, B};
-            MC_TRY(mc_cols_inv(*px, Sx, xfs, g.xcx, g.xh, &ph, Tx, px->t_elems, N, s, fx, ix));
+            MC_TRY(mc_cols_inv(*px, Sx, xfs, g.xcx, g.xh, &ph, Tx, px->t_elems, N, s, fx, ixm));
             MC_TRY(mc_rows_c2r(*px, Tx, px->t_elems, Rx, g.xw, (long long)g.xw * g.xh, N, 0, s));
             // patches: CExtractPatch + unnormalised R2C, all patches and frames at once
             MC_BUF(dstart, int, "starts", 2 * P);
             MC_TRY(hipMemcpyAsync(dstart, starts.data(), sizeof(int) * 2 * P, hipMemcpyHostToDevice, s));
-            const long long pfs = (long long)g.pcx * g.ph;
-            MC_BUF(Tp, cfloat, "Tp", (size_t)P * N * pp->t_elems);
-            cfloat* Sp = (cfloat*)buf(c, "Sp", (fpch == MC_SPEC_F32 ? 8 : 4) * (size_t)P * N * pfs);
-            if (!Sp) return fail(c, MC_E_NOMEM, "patch stack");
-            float* dcp = nullptr;
-            float* gp = nullptr;
-            float* ip = nullptr;
-            if (fpch != MC_SPEC_F32) {
-                dcp = (float*)buf(c, "dcp", sizeof(float) * (size_t)P * N * g.ph);
-                gp = (float*)buf(c, "gp", sizeof(float) * (size_t)P * N);
-                ip = (float*)buf(c, "ip", sizeof(float) * (size_t)P * N);
-                if (!dcp || !gp || !ip) return fail(c, MC_E_NOMEM, "patch scale factors");
-            }
+            const long long pm = (long long)m * P * N;
             MC_TRY(mc_rows_r2c_mod(*pp, Rx, g.xw, (long long)g.xw * g.xh, N, dstart, Tp, pp->t_elems,
                                    P * N, 1.0f, s, dcp));
-            if (fpch != MC_SPEC_F32) MC_TRY(mc_launch_spec_scale(dcp, g.ph, P * N, gp, ip, s));
-            MC_TRY(mc_cols_fwd(*pp, Tp, pp->t_elems, Sp, pfs, nullptr, 0, 0, 0, P * N, s, fpch, gp));
+            if (fpch != MC_SPEC_F32) MC_TRY(mc_launch_spec_scale(dcp, g.ph, P * N, gp + pm, ip + pm, s));
+            MC_TRY(mc_cols_fwd(*pp, Tp, pp->t_elems, spec_at(Sp_all, pm * pfs, fpch), pfs, nullptr, 0,
+                               0, 0, P * N, s, fpch, gp ? gp + pm : nullptr));
+          }
             Groups g1 = make_groups(p->group[1], N, p->throw0);
             IterOut po_;
-            MC_CHECK(run_iterative(c, "p_", Sp, P, N, g.pw, g.ph, pp, g1, p->bft[1], p->phase_only,
-                                   p->iter, tol, quirks, &po_, fpch, ip, p->launch_graphs != 0));
+            MC_CHECK(run_iterative(c, "p_", Sp_all, count * P, N, g.pw, g.ph, pp, g1, p->bft[1],
+                                   p->phase_only, p->iter, tol, quirks, &po_, fpch, ip,
+                                   p->launch_graphs != 0));
+          for (int m = 0; m < count; m++) {
+            MovieState& q0 = ms[m];
             // Align/CMeasurePatches.cpp:66-82: centre and shifts in full-res px
-            ps.assign((size_t)N * P * 2, 0.0);
-            pc.assign(2 * P, 0.0);
+            q0.ps.assign((size_t)N * P * 2, 0.0);
+            q0.pc.assign(2 * P, 0.0);
             for (int q = 0; q < P; q++) {
-                piters[q] = po_.iters[q];
-                const int cxp = starts[2 * q] + g.pw / 2, cyp = starts[2 * q + 1] + g.ph / 2;
-                pc[2 * q] = (double)(cxp * g.xbin[0]);
-                pc[2 * q + 1] = (double)(cyp * g.xbin[1]);
+                const size_t jq = (size_t)m * P + q;
+                q0.piters[q] = po_.iters[jq];
+                const int cxp = q0.starts[2 * q] + g.pw / 2, cyp = q0.starts[2 * q + 1] + g.ph / 2;
+                q0.pc[2 * q] = (double)(cxp * g.xbin[0]);
+                q0.pc[2 * q + 1] = (double)(cyp * g.xbin[1]);
                 for (int i = 0; i < N; i++) {
-                    ps[((size_t)i * P + q) * 2] = po_.shifts[((size_t)q * N + i) * 2] * g.xbin[0];
-                    ps[((size_t)i * P + q) * 2 + 1] = po_.shifts[((size_t)q * N + i) * 2 + 1] * g.xbin[1];
+                    q0.ps[((size_t)i * P + q) * 2] = po_.shifts[(jq * N + i) * 2] * g.xbin[0];
+                    q0.ps[((size_t)i * P + q) * 2 + 1] = po_.shifts[(jq * N + i) * 2 + 1] * g.xbin[1];
                 }
             }
             MC_BUF(dps, double, "ps", (size_t)N * P * 2);
             MC_BUF(dpc, double, "pc", 2 * P);
             MC_BUF(dbad, unsigned char, "bad", (size_t)N * P);
-            MC_TRY(hipMemcpyAsync(dps, ps.data(), sizeof(double) * ps.size(), hipMemcpyHostToDevice, s));
-            MC_TRY(hipMemcpyAsync(dpc, pc.data(), sizeof(double) * pc.size(), hipMemcpyHostToDevice, s));
+            MC_TRY(hipMemcpyAsync(dps, q0.ps.data(), sizeof(double) * q0.ps.size(), hipMemcpyHostToDevice, s));
+            MC_TRY(hipMemcpyAsync(dpc, q0.pc.data(), sizeof(double) * q0.pc.size(), hipMemcpyHostToDevice, s));
             MC_TRY(mc_launch_patch_stats(dps, dpc, N, P, W, H, quirks, dbad, nullptr, s));
-            bad.assign((size_t)N * P, 0);
-            MC_TRY(hipMemcpyAsync(ps.data(), dps, sizeof(double) * ps.size(), hipMemcpyDeviceToHost, s));
-            MC_TRY(hipMemcpyAsync(bad.data(), dbad, bad.size(), hipMemcpyDeviceToHost, s));
+            q0.bad.assign((size_t)N * P, 0);
+            MC_TRY(hipMemcpyAsync(q0.ps.data(), dps, sizeof(double) * q0.ps.size(), hipMemcpyDeviceToHost, s));
+            MC_TRY(hipMemcpyAsync(q0.bad.data(), dbad, q0.bad.size(), hipMemcpyDeviceToHost, s));
             MC_TRY(hipStreamSynchronize(s));
+          }
         }
         // Align/CFullAlign.cpp:160-163,96: xcf bin multiply, truncation to 0.01 px
-        for (int i = 0; i < N; i++) {
-            const float sx = (float)xs[2 * i] * g.xbin[0];
-            const float sy = (float)xs[2 * i + 1] * g.xbin[1];
-            gshift[2 * i] = 0.01 * (double)(long long)((double)sx * 100.0);
-            gshift[2 * i + 1] = 0.01 * (double)(long long)((double)sy * 100.0);
+        for (int m = 0; m < count; m++) {
+            const std::vector<double>& xs = ms[m].xs;
+            for (int i = 0; i < N; i++) {
+                const float sx = (float)xs[2 * i] * g.xbin[0];
+                const float sy = (float)xs[2 * i + 1] * g.xbin[1];
+                ms[m].gshift[2 * i] = 0.01 * (double)(long long)((double)sx * 100.0);
+                ms[m].gshift[2 * i + 1] = 0.01 * (double)(long long)((double)sy * 100.0);
+            }
         }
     } else {
-        for (int i = 0; i < 2 * N; i++) gshift[i] = ov->replay_global[i];
+        MovieState& q0 = ms[0];
+        for (int i = 0; i < 2 * N; i++) q0.gshift[i] = ov->replay_global[i];
         if (patch) {
-            ps.assign(ov->replay_patch, ov->replay_patch + (size_t)N * P * 2);
-            pc.assign(ov->replay_centers, ov->replay_centers + 2 * P);
-            bad.assign((size_t)N * P, 0);
-            if (ov->replay_bad) bad.assign(ov->replay_bad, ov->replay_bad + (size_t)N * P);
+            q0.ps.assign(ov->replay_patch, ov->replay_patch + (size_t)N * P * 2);
+            q0.pc.assign(ov->replay_centers, ov->replay_centers + 2 * P);
+            q0.bad.assign((size_t)N * P, 0);
+            if (ov->replay_bad) q0.bad.assign(ov->replay_bad, ov->replay_bad + (size_t)N * P);
         }
     }
     const auto tb = clk::now();
 
-    // ---- correction and sums ----
-    MC_BUF(sums, cfloat, "sums", (size_t)nsum * fs);
-    MC_BUF(gsh, double, "gsh", 2 * N);
-    MC_BUF(fA, cfloat, "fphA", (size_t)N * g.cx);
-    MC_BUF(fB, cfloat, "fphB", (size_t)N * H);
-    const double kvf = kv_factor(p->kv);
-    if (patch) {
-        // Align/CPatchAlign.cpp:47-65: shifts relative to the reference frame
-        const int ref = (p->fm_ref < 0) ? N / 2 : std::min(std::max(p->fm_ref - 1, 0), N - 1);
-        const double rx = gshift[2 * ref], ry = gshift[2 * ref + 1];
-        for (int i = 0; i < N; i++) {
-            gshift[2 * i] -= rx;
-            gshift[2 * i + 1] -= ry;
-        }
-        MC_TRY(hipMemcpyAsync(gsh, gshift.data(), sizeof(double) * 2 * N, hipMemcpyHostToDevice, s));
-        MC_TRY(mc_launch_phasors(gsh, -1.0, 1, nullptr, N, W, g.cx, H, N, fA, fB, s));
-        // Correct/GCorrectPatchShift.cu:214-285: C2R, warp, normalised R2C
-        MC_BUF(Rr, float, "Rr", (size_t)N * W * H);
-        MC_BUF(Rw, float, "Rw", (size_t)N * W * H);
-        const long long rfs = (long long)W * H;
-        for (int f0 = 0; f0 < N; f0 += CH) {
-            const int nb = std::min(CH, N - f0);
-            PhasorTabs ph{fA + (long long)f0 * g.cx, fB + (long long)f0 * H};
-            {
-                KTimer kt(c, "fft_cols_inv");
-                MC_TRY(mc_cols_inv(*pf, Sf + (long long)f0 * fs, fs, g.cx, H, &ph, T, pf->t_elems, nb, s));
+    // ---- correction and sums, one movie at a time ----
+    for (int m = 0; m < count; m++) {
+        const mc_movie* mv = mvs[m];
+        mc_result* res = rs[m];
+        std::vector<double>& gshift = ms[m].gshift;
+        const std::vector<double>& ps = ms[m].ps;
+        const std::vector<double>& pc = ms[m].pc;
+        const std::vector<unsigned char>& bad = ms[m].bad;
+        cfloat* Sf = Sf_all + (long long)m * N * fs;
+        MC_BUF(sums, cfloat, "sums", (size_t)nsum * fs);
+        MC_BUF(gsh, double, "gsh", 2 * N);
+        MC_BUF(fA, cfloat, "fphA", (size_t)N * g.cx);
+        MC_BUF(fB, cfloat, "fphB", (size_t)N * H);
+        const double kvf = kv_factor(p->kv);
+        if (patch) {
+            // Align/CPatchAlign.cpp:47-65: shifts relative to the reference frame
+            const int ref = (p->fm_ref < 0) ? N / 2 : std::min(std::max(p->fm_ref - 1, 0), N - 1);
+            const double rx = gshift[2 * ref], ry = gshift[2 * ref + 1];
+            for (int i = 0; i < N; i++) {
+                gshift[2 * i] -= rx;
+                gshift[2 * i + 1] -= ry;
             }
-            {
-                KTimer kt(c, "fft_rows_c2r");
-                MC_TRY(mc_rows_c2r(*pf, T, pf->t_elems, Rr + f0 * rfs, W, rfs, nb, 0, s));
+            MC_TRY(hipMemcpyAsync(gsh, gshift.data(), sizeof(double) * 2 * N, hipMemcpyHostToDevice, s));
+            MC_TRY(mc_launch_phasors(gsh, -1.0, 1, nullptr, N, W, g.cx, H, N, fA, fB, s));
+            // Correct/GCorrectPatchShift.cu:214-285: C2R, warp, normalised R2C
+            MC_BUF(Rr, float, "Rr", (size_t)N * W * H);
+            MC_BUF(Rw, float, "Rw", (size_t)N * W * H);
+            const long long rfs = (long long)W * H;
+            for (int f0 = 0; f0 < N; f0 += CH) {
+                const int nb = std::min(CH, N - f0);
+                PhasorTabs ph{fA + (long long)f0 * g.cx, fB + (long long)f0 * H};
+                {
+                    KTimer kt(c, "fft_cols_inv");
+                    MC_TRY(mc_cols_inv(*pf, Sf + (long long)f0 * fs, fs, g.cx, H, &ph, T, pf->t_elems, nb, s));
+                }
+                {
+                    KTimer kt(c, "fft_rows_c2r");
+                    MC_TRY(mc_rows_c2r(*pf, T, pf->t_elems, Rr + f0 * rfs, W, rfs, nb, 0, s));
+                }
             }
-        }
-        std::vector<float> psf(ps.size()), pcf(pc.size());
-        for (size_t i = 0; i < ps.size(); i++) psf[i] = (float)ps[i];
-        for (size_t i = 0; i < pc.size(); i++) pcf[i] = (float)pc[i];
-        MC_BUF(dpsf, float, "psf", psf.size());
-        MC_BUF(dpcf, float, "pcf", pcf.size());
-        MC_BUF(dbad2, unsigned char, "bad2", bad.size());
-        MC_TRY(hipMemcpyAsync(dpsf, psf.data(), sizeof(float) * psf.size(), hipMemcpyHostToDevice, s));
-        MC_TRY(hipMemcpyAsync(dpcf, pcf.data(), sizeof(float) * pcf.size(), hipMemcpyHostToDevice, s));
-        MC_TRY(hipMemcpyAsync(dbad2, bad.data(), bad.size(), hipMemcpyHostToDevice, s));
-        {
-            KTimer kt(c, "local_warp");
-            MC_TRY(mc_launch_local_warp(Rr, Rw, W, rfs, W, H, N, P, dpcf, dpsf, dbad2, s));
-        }
-        for (int f0 = 0; f0 < N; f0 += CH) {
-            const int nb = std::min(CH, N - f0);
+            std::vector<float> psf(ps.size()), pcf(pc.size());
+            for (size_t i = 0; i < ps.size(); i++) psf[i] = (float)ps[i];
+            for (size_t i = 0; i < pc.size(); i++) pcf[i] = (float)pc[i];
+            MC_BUF(dpsf, float, "psf", psf.size());
+            MC_BUF(dpcf, float, "pcf", pcf.size());
+            MC_BUF(dbad2, unsigned char, "bad2", bad.size());
+            MC_TRY(hipMemcpyAsync(dpsf, psf.data(), sizeof(float) * psf.size(), hipMemcpyHostToDevice, s));
+            MC_TRY(hipMemcpyAsync(dpcf, pcf.data(), sizeof(float) * pcf.size(), hipMemcpyHostToDevice, s));
+            MC_TRY(hipMemcpyAsync(dbad2, bad.data(), bad.size(), hipMemcpyHostToDevice, s));
             {
-                KTimer kt(c, "fft_rows_r2c");
-                MC_TRY(mc_rows_r2c(*pf, Rw + f0 * rfs, W, rfs, nullptr, T, pf->t_elems, nb, scale, s));
+                KTimer kt(c, "local_warp");
+                MC_TRY(mc_launch_local_warp(Rr, Rw, W, rfs, W, H, N, P, dpcf, dpsf, dbad2, s));
             }
-            {
-                KTimer kt(c, "fft_cols_fwd");
-                MC_TRY(mc_cols_fwd(*pf, T, pf->t_elems, Sf + (long long)f0 * fs, fs, nullptr, 0, 0, 0,
-                                   nb, s));
+            for (int f0 = 0; f0 < N; f0 += CH) {
+                const int nb = std::min(CH, N - f0);
+                {
+                    KTimer kt(c, "fft_rows_r2c");
+                    MC_TRY(mc_rows_r2c(*pf, Rw + f0 * rfs, W, rfs, nullptr, T, pf->t_elems, nb, scale, s));
+                }
+                {
+                    KTimer kt(c, "fft_cols_fwd");
+                    MC_TRY(mc_cols_fwd(*pf, T, pf->t_elems, Sf + (long long)f0 * fs, fs, nullptr, 0, 0, 0,
+                                       nb, s));
+                }
             }
+            KTimer kt(c, "dw_sums");
+            MC_TRY(mc_launch_dw_sums(Sf, fs, N, g.cx, H, nullptr, nullptr, dw, p->pix_size, kvf,
+                                     p->fm_dose, dws, p->sum_range[0], p->sum_range[1], p->split_sum,
+                                     sums, fs, s));
+        } else {
+            // Correct/CCorrectFullShift.cpp:229-309 with the absolute shifts
+            MC_TRY(hipMemcpyAsync(gsh, gshift.data(), sizeof(double) * 2 * N, hipMemcpyHostToDevice, s));
+            MC_TRY(mc_launch_phasors(gsh, -1.0, 1, nullptr, N, W, g.cx, H, N, fA, fB, s));
+            KTimer kt(c, "dw_sums");
+            MC_TRY(mc_launch_dw_sums(Sf, fs, N, g.cx, H, fA, fB, dw, p->pix_size, kvf, p->fm_dose, dws,
+                                     p->sum_range[0], p->sum_range[1], p->split_sum, sums, fs, s));
         }
-        KTimer kt(c, "dw_sums");
-        MC_TRY(mc_launch_dw_sums(Sf, fs, N, g.cx, H, nullptr, nullptr, dw, p->pix_size, kvf,
-                                 p->fm_dose, dws, p->sum_range[0], p->sum_range[1], p->split_sum,
-                                 sums, fs, s));
-    } else {
-        // Correct/CCorrectFullShift.cpp:229-309 with the absolute shifts
-        MC_TRY(hipMemcpyAsync(gsh, gshift.data(), sizeof(double) * 2 * N, hipMemcpyHostToDevice, s));
-        MC_TRY(mc_launch_phasors(gsh, -1.0, 1, nullptr, N, W, g.cx, H, N, fA, fB, s));
-        KTimer kt(c, "dw_sums");
-        MC_TRY(mc_launch_dw_sums(Sf, fs, N, g.cx, H, fA, fB, dw, p->pix_size, kvf, p->fm_dose, dws,
-                                 p->sum_range[0], p->sum_range[1], p->split_sum, sums, fs, s));
-    }
-    // Correct/CCorrectFullShift.cpp:146-185: crop to -FtBin size, C2R, >= 0
-    MC_BUF(To, cfloat, "To", (size_t)po->t_elems);
-    for (int i = 0; i < nsum; i++) {
-        MC_TRY(mc_cols_inv(*po, sums + (long long)i * fs, fs, g.cx, H, nullptr, To, po->t_elems, 1, s));
-        MC_TRY(mc_rows_c2r(*po, To, po->t_elems, res->sums[i], g.ow, (long long)g.ow * g.oh, 1, 1, s));
+        // Correct/CCorrectFullShift.cpp:146-185: crop to -FtBin size, C2R, >= 0
+        MC_BUF(To, cfloat, "To", (size_t)po->t_elems);
+        for (int i = 0; i < nsum; i++) {
+            MC_TRY(mc_cols_inv(*po, sums + (long long)i * fs, fs, g.cx, H, nullptr, To, po->t_elems, 1, s));
+            MC_TRY(mc_rows_c2r(*po, To, po->t_elems, res->sums[i], g.ow, (long long)g.ow * g.oh, 1, 1, s));
+        }
     }
     MC_TRY(hipStreamSynchronize(s));
     const auto tc = clk::now();
 
-    for (int i = 0; i < 2 * N; i++) gsf[i] = (float)gshift[i];
-    if (res->global_shift) std::copy(gsf.begin(), gsf.end(), res->global_shift);
-    if (res->global_iters) res->global_iters[0] = giters;
-    if (res->global_errors) {
-        const float xb = std::max(g.xbin[0], g.xbin[1]);
-        for (int i = 0; i < giters; i++) res->global_errors[i] = (float)(gerr[i] * xb);
+    for (int m = 0; m < count; m++) {
+        mc_result* res = rs[m];
+        MovieState& q0 = ms[m];
+        std::vector<float>& gsf = q0.gsf;
+        const std::vector<double>& gshift = q0.gshift;
+        const std::vector<double>& gerr = q0.gerr;
+        const std::vector<double>& ps = q0.ps;
+        const std::vector<double>& pc = q0.pc;
+        const std::vector<unsigned char>& bad = q0.bad;
+        const std::vector<int>& starts = q0.starts;
+        const std::vector<int>& piters = q0.piters;
+        const int giters = q0.giters;
+        const mc_params* pm = pp_[m];
+        for (int i = 0; i < 2 * N; i++) gsf[i] = (float)gshift[i];
+        if (res->global_shift) std::copy(gsf.begin(), gsf.end(), res->global_shift);
+        if (res->global_iters) res->global_iters[0] = giters;
+        if (res->global_errors) {
+            const float xb = std::max(g.xbin[0], g.xbin[1]);
+            for (int i = 0; i < giters; i++) res->global_errors[i] = (float)(gerr[i] * xb);
+        }
+        std::vector<float> psf(ps.size()), pcf(pc.size());
+        for (size_t i = 0; i < ps.size(); i++) psf[i] = (float)ps[i];
+        for (size_t i = 0; i < pc.size(); i++) pcf[i] = (float)pc[i];
+        if (patch) {
+            if (res->patch_shift) std::copy(psf.begin(), psf.end(), res->patch_shift);
+            if (res->patch_bad) std::copy(bad.begin(), bad.end(), res->patch_bad);
+            if (res->patch_center) std::copy(pcf.begin(), pcf.end(), res->patch_center);
+            if (res->patch_starts) std::copy(starts.begin(), starts.end(), res->patch_starts);
+            if (res->patch_iters) std::copy(piters.begin(), piters.end(), res->patch_iters);
+        }
+        write_logs(pm, W, H, N, P, gsf, psf, bad, pcf);
+        if (p->verbose) {
+            for (int i = 0; i < giters; i++)
+                printf("Iteration %2d  Error: %9.3f\n", i + 1, gerr[i] * std::max(g.xbin[0], g.xbin[1]));
+            printf("Full-frame alignment shift\n");
+            for (int i = 0; i < N; i++)
+                printf("...... Frame (%3d) shift: %9.3f  %9.3f\n", i + 1, gsf[2 * i] - gsf[0],
+                       gsf[2 * i + 1] - gsf[1]);
+            printf("\n");
+        }
+        res->seconds_align = std::chrono::duration<double>(tb - ta).count();
+        res->seconds_total = std::chrono::duration<double>(tc - t0).count();
     }
-    std::vector<float> psf(ps.size()), pcf(pc.size());
-    for (size_t i = 0; i < ps.size(); i++) psf[i] = (float)ps[i];
-    for (size_t i = 0; i < pc.size(); i++) pcf[i] = (float)pc[i];
-    if (patch) {
-        if (res->patch_shift) std::copy(psf.begin(), psf.end(), res->patch_shift);
-        if (res->patch_bad) std::copy(bad.begin(), bad.end(), res->patch_bad);
-        if (res->patch_center) std::copy(pcf.begin(), pcf.end(), res->patch_center);
-        if (res->patch_starts) std::copy(starts.begin(), starts.end(), res->patch_starts);
-        if (res->patch_iters) std::copy(piters.begin(), piters.end(), res->patch_iters);
-    }
-    write_logs(p, W, H, N, P, gsf, psf, bad, pcf);
-    if (p->verbose) {
-        for (int i = 0; i < giters; i++)
-            printf("Iteration %2d  Error: %9.3f\n", i + 1, gerr[i] * std::max(g.xbin[0], g.xbin[1]));
-        printf("Full-frame alignment shift\n");
-        for (int i = 0; i < N; i++)
-            printf("...... Frame (%3d) shift: %9.3f  %9.3f\n", i + 1, gsf[2 * i] - gsf[0],
-                   gsf[2 * i + 1] - gsf[1]);
-        printf("\n");
-    }
-    res->seconds_align = std::chrono::duration<double>(tb - ta).count();
-    res->seconds_total = std::chrono::duration<double>(tc - t0).count();
     return MC_OK;
 }
 
@@ -976,6 +1048,72 @@ int mc_output_geometry(const mc_params* p, int W, int H, int N, int* ns, int* ow
     *ns = 1 + (p->split_sum ? 2 : 0) + (dw ? 1 + (dws ? 1 : 0) : 0);
     *ow = g.ow;
     *oh = g.oh;
+    return MC_OK;
+}
+
+static int align_movie(mc_ctx* c, const mc_params* p, const mc_movie* mv, const mc_override* ov,
+                       mc_result* res) {
+    const mc_params* pa[1] = {p};
+    const mc_movie* ma[1] = {mv};
+    mc_result* ra[1] = {res};
+    return align_batch(c, pa, ma, ov, ra, 1);
+}
+
+static bool same_alignment_flags(const mc_params& a, const mc_params& b) {
+    return a.patch[0] == b.patch[0] && a.patch[1] == b.patch[1] && a.iter == b.iter &&
+           a.tol == b.tol && a.bft[0] == b.bft[0] && a.bft[1] == b.bft[1] &&
+           a.phase_only == b.phase_only && a.fm_dose == b.fm_dose && a.pix_size == b.pix_size &&
+           a.kv == b.kv && a.group[0] == b.group[0] && a.group[1] == b.group[1] &&
+           a.fm_ref == b.fm_ref && a.ft_bin == b.ft_bin && a.sum_range[0] == b.sum_range[0] &&
+           a.sum_range[1] == b.sum_range[1] && a.split_sum == b.split_sum &&
+           a.throw0 == b.throw0 && a.ref_quirks == b.ref_quirks &&
+           a.spectra_fmt == b.spectra_fmt && a.launch_graphs == b.launch_graphs &&
+           a.verbose == b.verbose;
+}
+
+int mc_align_movies(mc_ctx* c, const mc_params* params, const mc_movie* movies, int count,
+                    mc_result* results) {
+    if (!c || !params || !movies || !results || count < 1) return MC_E_ARG;
+    hipSetDevice(c->device);
+    for (int m = 1; m < count; m++)
+        if (!same_alignment_flags(params[0], params[m]))
+            return fail(c, MC_E_ARG, "mc_align_movies: movies differ in alignment flags");
+    size_t fr = 0, tot = 0;
+    if (hipMemGetInfo(&fr, &tot) != hipSuccess) return fail(c, MC_E_HIP, "hipMemGetInfo");
+    std::vector<char> taken(count, 0);
+    for (int m0 = 0; m0 < count; m0++) {
+        if (taken[m0]) continue;
+        // movies of movie m0's size, in order
+        std::vector<int> grp;
+        for (int m = m0; m < count; m++)
+            if (!taken[m] && movies[m].W == movies[m0].W && movies[m].H == movies[m0].H &&
+                movies[m].N == movies[m0].N) {
+                grp.push_back(m);
+                taken[m] = 1;
+            }
+        const mc_params& p = params[m0];
+        const mc_movie& mv = movies[m0];
+        if (mv.W < 8 || mv.H < 8 || mv.N < 2) return fail(c, MC_E_ARG, "invalid movie geometry");
+        const bool patch = p.patch[0] > 1 && p.patch[1] > 1;
+        const Geo g = make_geo(mv.W, mv.H, mv.N, p.patch[0], p.patch[1], p.ft_bin);
+        const double per_movie = (double)mv.N * ((double)g.cx * mv.H * 8 + (double)g.xcx * g.xh * 8 +
+                                                 (patch ? (double)p.patch[0] * p.patch[1] *
+                                                              g.pcx * g.ph * 8
+                                                        : 0.0));
+        const int cap = std::max(1, (int)std::min<double>(grp.size(), 0.5 * (double)tot / per_movie));
+        for (size_t b0 = 0; b0 < grp.size(); b0 += cap) {
+            const int nb = (int)std::min<size_t>(cap, grp.size() - b0);
+            std::vector<const mc_params*> pa(nb);
+            std::vector<const mc_movie*> ma(nb);
+            std::vector<mc_result*> ra(nb);
+            for (int k = 0; k < nb; k++) {
+                pa[k] = &params[grp[b0 + k]];
+                ma[k] = &movies[grp[b0 + k]];
+                ra[k] = &results[grp[b0 + k]];
+            }
+            MC_CHECK(align_batch(c, pa.data(), ma.data(), nullptr, ra.data(), nb));
+        }
+    }
     return MC_OK;
 }
 

@@ -60,7 +60,8 @@ class McOverride(C.Structure):
 # Every symbol include/mc_abi.h declares.
 ABI_SYMBOLS = (
     "mc_ctx_create", "mc_ctx_destroy", "mc_last_error", "mc_sync", "mc_default_params",
-    "mc_output_geometry", "mc_geometry", "mc_malloc", "mc_free", "mc_memcpy", "mc_align_movie", "mc_rfft2", "mc_irfft2",
+    "mc_output_geometry", "mc_geometry", "mc_malloc", "mc_free", "mc_memcpy", "mc_align_movie",
+    "mc_align_movies", "mc_rfft2", "mc_irfft2",
     "mc_phase_shift_accumulate", "mc_group_correlate_window", "mc_peak_fit",
     "mc_spline_despike_fp64", "mc_patch_stats_fp64", "mc_local_warp", "mc_dose_weight_sum",
     "mc_set_kernel_timing", "mc_get_kernel_timing", "mc_synth_render",
@@ -319,11 +320,7 @@ class Context:
                                                 C.byref(ow), C.byref(oh)), "mc_output_geometry")
         return ns.value, ow.value, oh.value
 
-    def align_movie(self, frames, params, replay_global=None, replay_patch=None,
-                    replay_centers=None, replay_bad=None, patch_starts=None, sums=None,
-                    width=None):
-        """Align an [N, H, pitch] float32 DevArray movie whose rows hold `width`
-        (default pitch) pixels.  Returns numpy shift tables and DevArray sums."""
+    def _prepare(self, frames, params, sums, width):
         N, H, pitch = frames.shape
         W = pitch if width is None else width
         ns, ow, oh = self.output_geometry(params, W, H, N)
@@ -351,6 +348,25 @@ class Context:
             res.patch_center = _fptr(out["patch_center"])
             res.patch_starts = out["patch_starts"].ctypes.data_as(C.POINTER(C.c_int))
             res.patch_iters = out["patch_iters"].ctypes.data_as(C.POINTER(C.c_int))
+        out["sums"] = sums
+        out["num_sums"] = ns
+        return mv, res, out
+
+    @staticmethod
+    def _finish(res, out):
+        out["global_iters"] = int(out["global_iters"][0])
+        out["global_errors"] = out["global_errors"][: out["global_iters"]]
+        out["seconds_align"] = res.seconds_align
+        out["seconds_total"] = res.seconds_total
+        out["spectra_fmt_used"] = (res.spectra_fmt_used[0], res.spectra_fmt_used[1])
+        return out
+
+    def align_movie(self, frames, params, replay_global=None, replay_patch=None,
+                    replay_centers=None, replay_bad=None, patch_starts=None, sums=None,
+                    width=None):
+        """Align an [N, H, pitch] float32 DevArray movie whose rows hold `width`
+        (default pitch) pixels.  Returns numpy shift tables and DevArray sums."""
+        mv, res, out = self._prepare(frames, params, sums, width)
         keep = []
 
         def fp(a, dt=np.float32, ct=C.c_float):
@@ -364,11 +380,16 @@ class Context:
                         fp(replay_bad, np.uint8, C.c_ubyte), fp(patch_starts, np.int32, C.c_int))
         self._check(self.lib.mc_align_movie(self.ptr, C.byref(params), C.byref(mv), C.byref(ov),
                                             C.byref(res)), "mc_align_movie")
-        out["global_iters"] = int(out["global_iters"][0])
-        out["global_errors"] = out["global_errors"][: out["global_iters"]]
-        out["sums"] = sums
-        out["num_sums"] = ns
-        out["seconds_align"] = res.seconds_align
-        out["seconds_total"] = res.seconds_total
-        out["spectra_fmt_used"] = (res.spectra_fmt_used[0], res.spectra_fmt_used[1])
-        return out
+        return self._finish(res, out)
+
+    def align_movies(self, movies, params):
+        """mc_align_movies over a list of [N, H, W] float32 DevArrays with one
+        McParams (or a list of them).  Returns one result dict per movie."""
+        n = len(movies)
+        plist = params if isinstance(params, (list, tuple)) else [params] * n
+        prep = [self._prepare(f, p, None, None) for f, p in zip(movies, plist)]
+        mva = (McMovie * n)(*[q[0] for q in prep])
+        ra = (McResult * n)(*[q[1] for q in prep])
+        pa = (McParams * n)(*plist)
+        self._check(self.lib.mc_align_movies(self.ptr, pa, mva, n, ra), "mc_align_movies")
+        return [self._finish(ra[k], prep[k][2]) for k in range(n)]

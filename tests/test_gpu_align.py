@@ -166,3 +166,22 @@ def test_iteration_graphs_match_direct_launches(ctx):
             assert np.array_equal(a.numpy(), b.numpy()), rep
         # direct work between replays
         ctx.align_movie(mv, L.default_params(launch_graphs=0, **kw))
+
+
+def test_batched_movies_match_single_calls(ctx):
+    """mc_align_movies: two movies of one size share the alignment batches, a
+    third of another size goes in its own batch; every result is the one a
+    separate mc_align_movie call gives, bit for bit."""
+    kw = dict(patch=(3, 3), iter=10, pix_size=1.0, kv=300, fm_dose=1.0)
+    shapes = [(1024, 1024, 12, 11), (1152, 816, 6, 12), (1024, 1024, 12, 13)]
+    movies = [ctx.upload(MovieRecipe(W, H, N, 300, seed=sd, dose=4.0, local_max=2.0).movie())
+              for W, H, N, sd in shapes]
+    prm = L.default_params(**kw)
+    single = [ctx.align_movie(m, prm) for m in movies]
+    batched = ctx.align_movies(movies, prm)
+    for a, b in zip(batched, single):
+        for k in ("global_shift", "patch_shift", "patch_bad", "patch_center", "patch_starts"):
+            assert np.array_equal(a[k], b[k]), k
+        assert a["global_iters"] == b["global_iters"]
+        for x, y in zip(a["sums"], b["sums"]):
+            assert np.array_equal(x.numpy(), y.numpy())
