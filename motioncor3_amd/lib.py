@@ -382,12 +382,14 @@ class Context:
                                             C.byref(res)), "mc_align_movie")
         return self._finish(res, out)
 
-    def align_movies(self, movies, params):
+    def align_movies(self, movies, params, sums=None):
         """mc_align_movies over a list of [N, H, W] float32 DevArrays with one
-        McParams (or a list of them).  Returns one result dict per movie."""
+        McParams (or a list of them); `sums` optionally gives each movie's
+        output DevArrays.  Returns one result dict per movie."""
         n = len(movies)
         plist = params if isinstance(params, (list, tuple)) else [params] * n
-        prep = [self._prepare(f, p, None, None) for f, p in zip(movies, plist)]
+        slist = sums if sums is not None else [None] * n
+        prep = [self._prepare(f, p, sm, None) for f, p, sm in zip(movies, plist, slist)]
         mva = (McMovie * n)(*[q[0] for q in prep])
         ra = (McResult * n)(*[q[1] for q in prep])
         pa = (McParams * n)(*plist)
