@@ -192,6 +192,15 @@ def main():
     t1 = time.perf_counter()
     kms, klaunch = ctx.kernel_timing()
     ctx.set_kernel_timing(None)
+    # the same kernel class with the frame chunks on one stream (not the
+    # timed configuration): its launches then run without the second
+    # stream's kernels beside them
+    iso = L.default_params(chunk_streams=1, **kw)
+    ctx.set_kernel_timing(args.kernel)
+    ctx.align_movie(frames, iso, sums=sums)
+    ctx.sync()
+    iso_ms, iso_n = ctx.kernel_timing()
+    ctx.set_kernel_timing(None)
 
     ms = (t1 - t0) * 1000.0 / (args.steps * K)
     g = L.geometry(W, H, 5, 5)
@@ -249,6 +258,12 @@ def main():
             "algorithmic_bytes_per_launch": per_launch,
             "avg_launch_ms": avg_ms,
             "launches": klaunch,
+            "concurrency": "frame chunks alternate over 2 streams; a launch's time includes "
+                           "sharing the GPU with the other stream's kernels",
+            "isolated": {"avg_launch_ms": iso_ms / max(iso_n, 1),
+                         "achieved": per_launch / (iso_ms / max(iso_n, 1) * 1e-3) / 1e9,
+                         "frac": per_launch / (iso_ms / max(iso_n, 1) * 1e-3) / 1e9 / PEAK_HBM_GBS,
+                         "note": "one extra call with chunk_streams=1, outside the timed steps"},
             "path_bytes_per_movie": pb,
             "path_achieved": pb / (ms * 1e-3) / 1e9,
             "path_frac": pb / (ms * 1e-3) / 1e9 / PEAK_HBM_GBS,

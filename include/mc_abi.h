@@ -69,6 +69,8 @@ typedef struct mc_params {
     int    launch_graphs; /* 1 (default): run the alignment iterations as
                              captured HIP graphs, two iterations per graph
                              (same results as direct launches) */
+    int    chunk_streams; /* 2 (default): frame-chunk FFT loops alternate over
+                             two streams; 1: one stream (same results) */
 } mc_params;
 
 /* N frames already gain/dark/defect corrected, resident in HBM

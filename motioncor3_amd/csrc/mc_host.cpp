@@ -47,6 +47,9 @@ struct mc_ctx {
     size_t nev = 0;
     // captured alignment-iteration blocks, keyed by everything they bake in
     std::map<std::string, hipGraphExec_t> graphs;
+    // second stream for the frame-chunk loops, and its fork/join events
+    hipStream_t s2 = nullptr;
+    hipEvent_t evf = nullptr, evj = nullptr;
 };
 
 namespace {
@@ -123,7 +126,9 @@ struct KTimer {
     mc_ctx* c;
     bool on;
     size_t idx;
-    KTimer(mc_ctx* ctx, const char* name) : c(ctx), on(false), idx(0) {
+    hipStream_t st;
+    KTimer(mc_ctx* ctx, const char* name, hipStream_t stream = nullptr)
+        : c(ctx), on(false), idx(0), st(stream ? stream : ctx->stream) {
         if (c->timed.empty() || c->timed != name) return;
         if (c->nev >= c->evs.size()) {
             TimedEv e;
@@ -132,11 +137,11 @@ struct KTimer {
             c->evs.push_back(e);
         }
         idx = c->nev++;
-        hipEventRecord(c->evs[idx].a, c->stream);
+        hipEventRecord(c->evs[idx].a, st);
         on = true;
     }
     ~KTimer() {
-        if (on) hipEventRecord(c->evs[idx].b, c->stream);
+        if (on) hipEventRecord(c->evs[idx].b, st);
     }
 };
 
@@ -627,30 +632,57 @@ int align_batch(mc_ctx* c, const mc_params* const* pp_, const mc_movie* const* m
         if (!dcf || !gx || !ix) return fail(c, MC_E_NOMEM, "stack scale factors");
     }
 
+    // Frame-chunk loops alternate between the context stream and a second
+    // stream, each with its own staging buffer, so that one chunk's column
+    // pass can run beside the next chunk's row pass (their workgroups are
+    // LDS-limited to two per CU and latency-bound).  Fork and join by events.
+    const bool two = p->chunk_streams >= 2;
+    MC_BUF(T2, cfloat, "T2", (size_t)CH * pf->t_elems);
+    float* dcf2 = nullptr;
+    if (fx != MC_SPEC_F32) {
+        dcf2 = (float*)buf(c, "dcf2", sizeof(float) * (size_t)CH * H);
+        if (!dcf2) return fail(c, MC_E_NOMEM, "stack scale factors");
+    }
+    auto fork = [&]() -> hipError_t {
+        hipError_t e = hipEventRecord(c->evf, s);
+        return e ? e : hipStreamWaitEvent(c->s2, c->evf, 0);
+    };
+    auto join = [&]() -> hipError_t {
+        hipError_t e = hipEventRecord(c->evj, c->s2);
+        return e ? e : hipStreamWaitEvent(s, c->evj, 0);
+    };
+
     // Align/CTransformStack.cpp + Align/CGenXcfStack.cpp: normalised R2C of
     // every frame and the Fourier crop to the xcf size in the same pass.
     const float scale = (float)(1.0 / ((double)W * H));
+    MC_TRY(fork());
+    int chunk = 0;
     for (int m = 0; m < count; m++) {
         const mc_movie* mv = mvs[m];
-        for (int f0 = 0; f0 < N; f0 += CH) {
+        for (int f0 = 0; f0 < N; f0 += CH, chunk++) {
             const int nb = std::min(CH, N - f0);
             const long long fi = (long long)m * N + f0;
+            const bool odd = two && (chunk & 1);
+            hipStream_t sk = odd ? c->s2 : s;
+            cfloat* Tk = odd ? T2 : T;
+            float* dck = odd ? dcf2 : dcf;
             {
-                KTimer kt(c, "fft_rows_r2c");
+                KTimer kt(c, "fft_rows_r2c", sk);
                 MC_TRY(mc_rows_r2c_mod(*pf, mv->frames + (long long)f0 * mv->frame_stride, mv->pitch,
-                                       mv->frame_stride, 0, nullptr, T, pf->t_elems, nb, scale, s,
-                                       dcf));
+                                       mv->frame_stride, 0, nullptr, Tk, pf->t_elems, nb, scale, sk,
+                                       dck));
             }
-            if (fx != MC_SPEC_F32) MC_TRY(mc_launch_spec_scale(dcf, H, nb, gx + fi, ix + fi, s));
+            if (fx != MC_SPEC_F32) MC_TRY(mc_launch_spec_scale(dck, H, nb, gx + fi, ix + fi, sk));
             {
-                KTimer kt(c, "fft_cols_fwd");
-                MC_TRY(mc_cols_fwd(*pf, T, pf->t_elems, Sf_all + fi * fs, fs,
+                KTimer kt(c, "fft_cols_fwd", sk);
+                MC_TRY(mc_cols_fwd(*pf, Tk, pf->t_elems, Sf_all + fi * fs, fs,
                                    (xcf_alias || replay) ? nullptr : spec_at(Sx_all, fi * xfs, fx),
-                                   g.xcx, g.xh, xfs, nb, s, MC_SPEC_F32, nullptr, fx,
+                                   g.xcx, g.xh, xfs, nb, sk, MC_SPEC_F32, nullptr, fx,
                                    fx != MC_SPEC_F32 ? gx + fi : nullptr));
             }
         }
     }
+    MC_TRY(join());
 
     struct MovieState {
         std::vector<double> gshift, ps, pc, gerr, xs;
@@ -822,18 +854,23 @@ int align_batch(mc_ctx* c, const mc_params* const* pp_, const mc_movie* const* m
             MC_BUF(Rr, float, "Rr", (size_t)N * W * H);
             MC_BUF(Rw, float, "Rw", (size_t)N * W * H);
             const long long rfs = (long long)W * H;
+            MC_TRY(fork());
             for (int f0 = 0; f0 < N; f0 += CH) {
                 const int nb = std::min(CH, N - f0);
+                const bool odd = two && ((f0 / CH) & 1);
+                hipStream_t sk = odd ? c->s2 : s;
+                cfloat* Tk = odd ? T2 : T;
                 PhasorTabs ph{fA + (long long)f0 * g.cx, fB + (long long)f0 * H};
                 {
-                    KTimer kt(c, "fft_cols_inv");
-                    MC_TRY(mc_cols_inv(*pf, Sf + (long long)f0 * fs, fs, g.cx, H, &ph, T, pf->t_elems, nb, s));
+                    KTimer kt(c, "fft_cols_inv", sk);
+                    MC_TRY(mc_cols_inv(*pf, Sf + (long long)f0 * fs, fs, g.cx, H, &ph, Tk, pf->t_elems, nb, sk));
                 }
                 {
-                    KTimer kt(c, "fft_rows_c2r");
-                    MC_TRY(mc_rows_c2r(*pf, T, pf->t_elems, Rr + f0 * rfs, W, rfs, nb, 0, s));
+                    KTimer kt(c, "fft_rows_c2r", sk);
+                    MC_TRY(mc_rows_c2r(*pf, Tk, pf->t_elems, Rr + f0 * rfs, W, rfs, nb, 0, sk));
                 }
             }
+            MC_TRY(join());
             std::vector<float> psf(ps.size()), pcf(pc.size());
             for (size_t i = 0; i < ps.size(); i++) psf[i] = (float)ps[i];
             for (size_t i = 0; i < pc.size(); i++) pcf[i] = (float)pc[i];
@@ -847,18 +884,23 @@ int align_batch(mc_ctx* c, const mc_params* const* pp_, const mc_movie* const* m
                 KTimer kt(c, "local_warp");
                 MC_TRY(mc_launch_local_warp(Rr, Rw, W, rfs, W, H, N, P, dpcf, dpsf, dbad2, s));
             }
+            MC_TRY(fork());
             for (int f0 = 0; f0 < N; f0 += CH) {
                 const int nb = std::min(CH, N - f0);
+                const bool odd = two && ((f0 / CH) & 1);
+                hipStream_t sk = odd ? c->s2 : s;
+                cfloat* Tk = odd ? T2 : T;
                 {
-                    KTimer kt(c, "fft_rows_r2c");
-                    MC_TRY(mc_rows_r2c(*pf, Rw + f0 * rfs, W, rfs, nullptr, T, pf->t_elems, nb, scale, s));
+                    KTimer kt(c, "fft_rows_r2c", sk);
+                    MC_TRY(mc_rows_r2c(*pf, Rw + f0 * rfs, W, rfs, nullptr, Tk, pf->t_elems, nb, scale, sk));
                 }
                 {
-                    KTimer kt(c, "fft_cols_fwd");
-                    MC_TRY(mc_cols_fwd(*pf, T, pf->t_elems, Sf + (long long)f0 * fs, fs, nullptr, 0, 0, 0,
-                                       nb, s));
+                    KTimer kt(c, "fft_cols_fwd", sk);
+                    MC_TRY(mc_cols_fwd(*pf, Tk, pf->t_elems, Sf + (long long)f0 * fs, fs, nullptr, 0, 0, 0,
+                                       nb, sk));
                 }
             }
+            MC_TRY(join());
             KTimer kt(c, "dw_sums");
             MC_TRY(mc_launch_dw_sums(Sf, fs, N, g.cx, H, nullptr, nullptr, dw, p->pix_size, kvf,
                                      p->fm_dose, dws, p->sum_range[0], p->sum_range[1], p->split_sum,
@@ -947,6 +989,7 @@ void mc_default_params(mc_params* p) {
     p->sum_range[1] = 25.0f;
     p->ref_quirks = 1;
     p->launch_graphs = 1;
+    p->chunk_streams = 2;
 }
 
 int mc_ctx_create(int device, void* stream, mc_ctx** out) {
@@ -964,7 +1007,11 @@ int mc_ctx_create(int device, void* stream, mc_ctx** out) {
         }
         c->own_stream = true;
     }
-    if (mc_fft_init_constants() != 0 || mc_align_init_constants() != 0) {
+    if (mc_fft_init_constants() != 0 || mc_align_init_constants() != 0 ||
+        hipStreamCreate(&c->s2) != hipSuccess ||
+        hipEventCreateWithFlags(&c->evf, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&c->evj, hipEventDisableTiming) != hipSuccess) {
+        if (c->s2) hipStreamDestroy(c->s2);
         if (c->own_stream) hipStreamDestroy(c->stream);
         delete c;
         return MC_E_HIP;
@@ -989,6 +1036,12 @@ int mc_ctx_destroy(mc_ctx* c) {
         hipEventDestroy(e.b);
     }
     for (auto& kv : c->graphs) hipGraphExecDestroy(kv.second);
+    if (c->s2) {
+        hipStreamSynchronize(c->s2);
+        hipStreamDestroy(c->s2);
+    }
+    if (c->evf) hipEventDestroy(c->evf);
+    if (c->evj) hipEventDestroy(c->evj);
     if (c->own_stream) hipStreamDestroy(c->stream);
     delete c;
     return MC_OK;
@@ -1068,6 +1121,7 @@ static bool same_alignment_flags(const mc_params& a, const mc_params& b) {
            a.sum_range[1] == b.sum_range[1] && a.split_sum == b.split_sum &&
            a.throw0 == b.throw0 && a.ref_quirks == b.ref_quirks &&
            a.spectra_fmt == b.spectra_fmt && a.launch_graphs == b.launch_graphs &&
+           a.chunk_streams == b.chunk_streams &&
            a.verbose == b.verbose;
 }
 

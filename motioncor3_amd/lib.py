@@ -28,7 +28,7 @@ class McParams(C.Structure):
         ("fm_ref", C.c_int), ("ft_bin", C.c_float), ("sum_range", C.c_float * 2),
         ("split_sum", C.c_int), ("throw0", C.c_int), ("ref_quirks", C.c_int),
         ("log_dir", C.c_char_p), ("movie_name", C.c_char_p), ("verbose", C.c_int),
-        ("spectra_fmt", C.c_int), ("launch_graphs", C.c_int),
+        ("spectra_fmt", C.c_int), ("launch_graphs", C.c_int), ("chunk_streams", C.c_int),
     ]
 
 

@@ -185,3 +185,18 @@ def test_batched_movies_match_single_calls(ctx):
         assert a["global_iters"] == b["global_iters"]
         for x, y in zip(a["sums"], b["sums"]):
             assert np.array_equal(x.numpy(), y.numpy())
+
+
+def test_chunk_streams_match_single_stream(ctx):
+    """Frame chunks on two streams give the same bits as on one stream."""
+    W, H, N = 5760, 4092, 6
+    r = MovieRecipe(W, H, N, 20000, seed=9, dose=1.0, local_max=3.0)
+    frames = ctx.empty((N, H, W), np.float32)
+    ctx.synth_render(frames, r)
+    kw = dict(patch=(5, 5), bft=(500, 100), pix_size=0.83, kv=300, fm_dose=1.0)
+    one = ctx.align_movie(frames, L.default_params(chunk_streams=1, **kw))
+    two = ctx.align_movie(frames, L.default_params(chunk_streams=2, **kw))
+    assert np.array_equal(one["global_shift"], two["global_shift"])
+    assert np.array_equal(one["patch_shift"], two["patch_shift"])
+    for a, b in zip(one["sums"], two["sums"]):
+        assert np.array_equal(a.numpy(), b.numpy())
