@@ -1186,25 +1186,26 @@ __global__ void k_synth(float* frames, int W, int H, long long pitch, long long 
 // launch wrappers
 // ===========================================================================
 int mc_align_init_constants() {
+    bool attr_ok = true;
     if (mc_upload_prime_table() != 0) return -1;
     const int bytes = (mc_lds_padded(MC_LDS_ELEMS) + 64) * (int)sizeof(cfloat);
-    hipFuncSetAttribute((const void*)k_correlate, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
-    hipFuncSetAttribute((const void*)k_finalize, hipFuncAttributeMaxDynamicSharedMemorySize, 65536);
+    attr_ok &= hipSuccess == hipFuncSetAttribute((const void*)k_correlate, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
+    attr_ok &= hipSuccess == hipFuncSetAttribute((const void*)k_finalize, hipFuncAttributeMaxDynamicSharedMemorySize, 65536);
 #define X(HH, CC, NT)                                                                     \
-    hipFuncSetAttribute((const void*)k_correlate_rp<HH, CC, NT, 0>,                      \
+    attr_ok &= hipSuccess == hipFuncSetAttribute((const void*)k_correlate_rp<HH, CC, NT, 0>,                      \
                         hipFuncAttributeMaxDynamicSharedMemorySize, bytes);              \
-    hipFuncSetAttribute((const void*)k_correlate_rp<HH, CC, NT, 1>,                      \
+    attr_ok &= hipSuccess == hipFuncSetAttribute((const void*)k_correlate_rp<HH, CC, NT, 1>,                      \
                         hipFuncAttributeMaxDynamicSharedMemorySize, bytes);              \
-    hipFuncSetAttribute((const void*)k_correlate_rp<HH, CC, NT, 2>,                      \
+    attr_ok &= hipSuccess == hipFuncSetAttribute((const void*)k_correlate_rp<HH, CC, NT, 2>,                      \
                         hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
     MC_RP_CORR_SPECS(X)
 #undef X
-    hipFuncSetAttribute((const void*)k_row_dft, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
-    hipFuncSetAttribute((const void*)k_feature_map, hipFuncAttributeMaxDynamicSharedMemorySize,
+    attr_ok &= hipSuccess == hipFuncSetAttribute((const void*)k_row_dft, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
+    attr_ok &= hipSuccess == hipFuncSetAttribute((const void*)k_feature_map, hipFuncAttributeMaxDynamicSharedMemorySize,
                         2 * 128 * 128 * (int)sizeof(float));
-    hipFuncSetAttribute((const void*)k_local_warp, hipFuncAttributeMaxDynamicSharedMemorySize,
+    attr_ok &= hipSuccess == hipFuncSetAttribute((const void*)k_local_warp, hipFuncAttributeMaxDynamicSharedMemorySize,
                         (int)warp_lds_bytes(MC_WARP_MAXP));
-    return 0;
+    return attr_ok ? 0 : -1;
 }
 
 hipError_t mc_launch_phasors(const double* shifts, double mult, int skip, const unsigned char* done,

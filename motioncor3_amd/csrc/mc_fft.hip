@@ -751,7 +751,7 @@ __global__ __launch_bounds__(NT) void k_cols_inv_rp(ColInvArgs a, const cfloat* 
 }
 
 #ifndef MC_RP_COL_SPECS
-#define MC_RP_COL_SPECS(X) X(4092, 2, 264) X(4096, 2, 256) X(2048, 4, 512) X(408, 16, 384)
+#define MC_RP_COL_SPECS(X) X(4092, 2, 264) X(4096, 2, 256) X(2048, 4, 512) X(408, 16, 384) X(8184, 1, 264)
 #endif
 
 // ---------------------------------------------------------------------------
@@ -917,7 +917,8 @@ __global__ __launch_bounds__(NT) void k_rows_c2r_rp(RowC2RArgs a, const cfloat* 
 }
 
 #ifndef MC_RP_ROW_SPECS
-#define MC_RP_ROW_SPECS(X) X(2880, 2, MC_TILE_CT, 512) X(2048, 2, MC_TILE_CT, 512) X(288, 16, 16, 512)
+#define MC_RP_ROW_SPECS(X) X(2880, 2, MC_TILE_CT, 512) X(2048, 2, MC_TILE_CT, 512) X(288, 16, 16, 512) \
+    X(5760, 1, MC_TILE_CT, 512)
 #endif
 
 
@@ -975,10 +976,13 @@ __global__ __launch_bounds__(256) void k_spec_scale(const float* __restrict__ dc
 }
 
 
+static bool g_attr_fail = false;
 template <class F>
 static void set_lds_attr(F* f) {
     const int bytes = (mc_lds_padded(MC_LDS_ELEMS) + 64) * (int)sizeof(cfloat);
-    hipFuncSetAttribute((const void*)f, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
+    if (hipFuncSetAttribute((const void*)f, hipFuncAttributeMaxDynamicSharedMemorySize, bytes) !=
+        hipSuccess)
+        g_attr_fail = true;
 }
 
 static void set_ct_attrs() {
@@ -1066,12 +1070,12 @@ hipError_t mc_launch_spec_scale(const float* dc, int rows, int batch, float* g, 
 int mc_fft_init_constants() {
     if (mc_upload_prime_table() != 0) return -1;
     if (!g_attr_done) {
-        const int bytes = (mc_lds_padded(MC_LDS_ELEMS) + 64) * (int)sizeof(cfloat);
-        hipFuncSetAttribute((const void*)k_rows_r2c, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
-        hipFuncSetAttribute((const void*)k_rows_c2r, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
-        hipFuncSetAttribute((const void*)k_cols_fwd, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
-        hipFuncSetAttribute((const void*)k_cols_inv, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
+        set_lds_attr(k_rows_r2c);
+        set_lds_attr(k_rows_c2r);
+        set_lds_attr(k_cols_fwd);
+        set_lds_attr(k_cols_inv);
         set_ct_attrs();
+        if (g_attr_fail) return -1;
         g_attr_done = true;
     }
     return 0;
@@ -1110,13 +1114,17 @@ int mc_fft2d_create(Fft2D* p, int W, int H) {
         pt[k].y = (float)std::sin(a);
     }
     if (hipMalloc(&p->post_tw, pt.size() * sizeof(cfloat)) != hipSuccess) return -2;
-    hipMemcpy(p->post_tw, pt.data(), pt.size() * sizeof(cfloat), hipMemcpyHostToDevice);
+    if (hipMemcpy(p->post_tw, pt.data(), pt.size() * sizeof(cfloat), hipMemcpyHostToDevice) !=
+        hipSuccess)
+        return -2;
     for (int dim = 0; dim < 2; dim++) {
         const std::vector<cfloat> tw = rp_twiddle_table(dim ? H : m);
         if (tw.empty()) continue;
         cfloat** dstp = dim ? &p->col_ptw : &p->row_ptw;
         if (hipMalloc(dstp, tw.size() * sizeof(cfloat)) != hipSuccess) return -2;
-        hipMemcpy(*dstp, tw.data(), tw.size() * sizeof(cfloat), hipMemcpyHostToDevice);
+        if (hipMemcpy(*dstp, tw.data(), tw.size() * sizeof(cfloat), hipMemcpyHostToDevice) !=
+            hipSuccess)
+            return -2;
     }
     const int nb = (m + 1 + C - 1) / C;
     p->t_elems = (long long)nb * H * C;
@@ -1125,11 +1133,11 @@ int mc_fft2d_create(Fft2D* p, int W, int H) {
 }
 
 void mc_fft2d_destroy(Fft2D* p) {
-    if (p->row_tw) hipFree(p->row_tw);
-    if (p->col_tw) hipFree(p->col_tw);
-    if (p->post_tw) hipFree(p->post_tw);
-    if (p->col_ptw) hipFree(p->col_ptw);
-    if (p->row_ptw) hipFree(p->row_ptw);
+    if (p->row_tw) (void)hipFree(p->row_tw);
+    if (p->col_tw) (void)hipFree(p->col_tw);
+    if (p->post_tw) (void)hipFree(p->post_tw);
+    if (p->col_ptw) (void)hipFree(p->col_ptw);
+    if (p->row_ptw) (void)hipFree(p->row_ptw);
     p->row_tw = p->col_tw = p->post_tw = p->col_ptw = p->row_ptw = nullptr;
 }
 

@@ -136,6 +136,8 @@ __host__ __device__ constexpr int rp_radix(int n, int p) {
     else if (n == 2880) { r0 = 15; r1 = 12; r2 = 16; }
     else if (n == 408) { r0 = 17; r1 = 24; }
     else if (n == 288) { r0 = 9; r1 = 32; }
+    else if (n == 8184) { r0 = 31; r1 = 11; r2 = 24; }
+    else if (n == 5760) { r0 = 15; r1 = 24; r2 = 16; }
     return p == 0 ? r0 : p == 1 ? r1 : p == 2 ? r2 : p == 3 ? r3 : 0;
 }
 __host__ __device__ constexpr int rp_npass(int n) {

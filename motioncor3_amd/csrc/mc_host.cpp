@@ -77,7 +77,7 @@ int fail(mc_ctx* c, int code, const std::string& msg) {
 void* buf(mc_ctx* c, const char* name, size_t bytes) {
     DevBuf& b = c->bufs[name];
     if (b.bytes >= bytes && b.p) return b.p;
-    if (b.p) hipFree(b.p);
+    if (b.p) (void)hipFree(b.p);
     b.p = nullptr;
     b.bytes = 0;
     if (hipMalloc(&b.p, bytes < 256 ? 256 : bytes) != hipSuccess) {
@@ -116,7 +116,10 @@ cfloat* row_twiddles(mc_ctx* c, int W) {
     }
     cfloat* d = nullptr;
     if (hipMalloc(&d, W * sizeof(cfloat)) != hipSuccess) return nullptr;
-    hipMemcpy(d, h.data(), W * sizeof(cfloat), hipMemcpyHostToDevice);
+    if (hipMemcpy(d, h.data(), W * sizeof(cfloat), hipMemcpyHostToDevice) != hipSuccess) {
+        (void)hipFree(d);
+        return nullptr;
+    }
     c->rowtw[W] = d;
     return d;
 }
@@ -132,16 +135,16 @@ struct KTimer {
         if (c->timed.empty() || c->timed != name) return;
         if (c->nev >= c->evs.size()) {
             TimedEv e;
-            hipEventCreate(&e.a);
-            hipEventCreate(&e.b);
+            (void)hipEventCreate(&e.a);
+            (void)hipEventCreate(&e.b);
             c->evs.push_back(e);
         }
         idx = c->nev++;
-        hipEventRecord(c->evs[idx].a, st);
+        (void)hipEventRecord(c->evs[idx].a, st);
         on = true;
     }
     ~KTimer() {
-        if (on) hipEventRecord(c->evs[idx].b, st);
+        if (on) (void)hipEventRecord(c->evs[idx].b, st);
     }
 };
 
@@ -427,12 +430,12 @@ int run_iterative(mc_ctx* c, const char* tag, const cfloat* F, int J, int N, int
                 hipGraph_t gr = nullptr;
                 const hipError_t e2 = hipStreamEndCapture(s, &gr);
                 if (e != hipSuccess || e2 != hipSuccess || !gr) {
-                    if (gr) hipGraphDestroy(gr);
+                    if (gr) (void)hipGraphDestroy(gr);
                     return fail(c, MC_E_HIP, std::string("iteration graph capture: ") +
                                                  hipGetErrorString(e != hipSuccess ? e : e2));
                 }
                 const hipError_t e3 = hipGraphInstantiate(&ex, gr, nullptr, nullptr, 0);
-                hipGraphDestroy(gr);
+                (void)hipGraphDestroy(gr);
                 if (e3 != hipSuccess) return fail(c, MC_E_HIP, "iteration graph instantiation");
                 c->graphs[key] = ex;
             } else {
@@ -828,7 +831,6 @@ int align_batch(mc_ctx* c, const mc_params* const* pp_, const mc_movie* const* m
 
     // ---- correction and sums, one movie at a time ----
     for (int m = 0; m < count; m++) {
-        const mc_movie* mv = mvs[m];
         mc_result* res = rs[m];
         std::vector<double>& gshift = ms[m].gshift;
         const std::vector<double>& ps = ms[m].ps;
@@ -1011,8 +1013,8 @@ int mc_ctx_create(int device, void* stream, mc_ctx** out) {
         hipStreamCreate(&c->s2) != hipSuccess ||
         hipEventCreateWithFlags(&c->evf, hipEventDisableTiming) != hipSuccess ||
         hipEventCreateWithFlags(&c->evj, hipEventDisableTiming) != hipSuccess) {
-        if (c->s2) hipStreamDestroy(c->s2);
-        if (c->own_stream) hipStreamDestroy(c->stream);
+        if (c->s2) (void)hipStreamDestroy(c->s2);
+        if (c->own_stream) (void)hipStreamDestroy(c->stream);
         delete c;
         return MC_E_HIP;
     }
@@ -1022,27 +1024,27 @@ int mc_ctx_create(int device, void* stream, mc_ctx** out) {
 
 int mc_ctx_destroy(mc_ctx* c) {
     if (!c) return MC_E_ARG;
-    hipSetDevice(c->device);
-    hipStreamSynchronize(c->stream);
+    (void)hipSetDevice(c->device);
+    (void)hipStreamSynchronize(c->stream);
     for (auto& kv : c->plans) {
         mc_fft2d_destroy(kv.second);
         delete kv.second;
     }
-    for (auto& kv : c->rowtw) hipFree(kv.second);
+    for (auto& kv : c->rowtw) (void)hipFree(kv.second);
     for (auto& kv : c->bufs)
-        if (kv.second.p) hipFree(kv.second.p);
+        if (kv.second.p) (void)hipFree(kv.second.p);
     for (auto& e : c->evs) {
-        hipEventDestroy(e.a);
-        hipEventDestroy(e.b);
+        (void)hipEventDestroy(e.a);
+        (void)hipEventDestroy(e.b);
     }
-    for (auto& kv : c->graphs) hipGraphExecDestroy(kv.second);
+    for (auto& kv : c->graphs) (void)hipGraphExecDestroy(kv.second);
     if (c->s2) {
-        hipStreamSynchronize(c->s2);
-        hipStreamDestroy(c->s2);
+        (void)hipStreamSynchronize(c->s2);
+        (void)hipStreamDestroy(c->s2);
     }
-    if (c->evf) hipEventDestroy(c->evf);
-    if (c->evj) hipEventDestroy(c->evj);
-    if (c->own_stream) hipStreamDestroy(c->stream);
+    if (c->evf) (void)hipEventDestroy(c->evf);
+    if (c->evj) (void)hipEventDestroy(c->evj);
+    if (c->own_stream) (void)hipStreamDestroy(c->stream);
     delete c;
     return MC_OK;
 }
@@ -1058,7 +1060,7 @@ int mc_sync(mc_ctx* c) {
 int mc_malloc(mc_ctx* c, unsigned long long bytes, void** out) {
     if (!c || !out) return MC_E_ARG;
     *out = nullptr;
-    hipSetDevice(c->device);
+    MC_TRY(hipSetDevice(c->device));
     if (hipMalloc(out, bytes ? bytes : 1) != hipSuccess) return fail(c, MC_E_NOMEM, "hipMalloc");
     return MC_OK;
 }
@@ -1066,7 +1068,7 @@ int mc_malloc(mc_ctx* c, unsigned long long bytes, void** out) {
 int mc_free(mc_ctx* c, void* ptr) {
     if (!c) return MC_E_ARG;
     if (!ptr) return MC_OK;
-    hipStreamSynchronize(c->stream);
+    MC_TRY(hipStreamSynchronize(c->stream));
     MC_TRY(hipFree(ptr));
     return MC_OK;
 }
@@ -1128,7 +1130,7 @@ static bool same_alignment_flags(const mc_params& a, const mc_params& b) {
 int mc_align_movies(mc_ctx* c, const mc_params* params, const mc_movie* movies, int count,
                     mc_result* results) {
     if (!c || !params || !movies || !results || count < 1) return MC_E_ARG;
-    hipSetDevice(c->device);
+    if (hipSetDevice(c->device) != hipSuccess) return fail(c, MC_E_HIP, "hipSetDevice");
     for (int m = 1; m < count; m++)
         if (!same_alignment_flags(params[0], params[m]))
             return fail(c, MC_E_ARG, "mc_align_movies: movies differ in alignment flags");
@@ -1174,7 +1176,7 @@ int mc_align_movies(mc_ctx* c, const mc_params* params, const mc_movie* movies, 
 int mc_align_movie(mc_ctx* c, const mc_params* p, const mc_movie* mv, const mc_override* ov,
                    mc_result* res) {
     if (!c || !p || !mv || !res) return MC_E_ARG;
-    hipSetDevice(c->device);
+    if (hipSetDevice(c->device) != hipSuccess) return fail(c, MC_E_HIP, "hipSetDevice");
     return align_movie(c, p, mv, ov, res);
 }
 

@@ -6,7 +6,7 @@ pytestmark = pytest.mark.gpu
 
 # frame, xcf, patch and feature-map sizes of the reference geometries
 SIZES = [(64, 48), (128, 128), (340, 340), (576, 408), (292, 408), (410, 408), (1024, 1024),
-         (2882, 2048), (4096, 4096), (5760, 4092)]
+         (2882, 2048), (4096, 4096), (5760, 4092), (11520, 8184)]
 
 
 def _rel(a, b):
