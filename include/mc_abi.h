@@ -71,6 +71,14 @@ typedef struct mc_params {
                              (same results as direct launches) */
     int    chunk_streams; /* 2 (default): frame-chunk FFT loops alternate over
                              two streams; 1: one stream (same results) */
+    int    prior;         /* 1: start every patch's iterations from the learned
+                             prior attached by mc_set_prior (0, default: from
+                             zero shifts as the reference does).  No reference
+                             counterpart (the reference resets the patch shifts,
+                             Align/CIterativeAlign.cpp:68). */
+    float  prior_fallback;/* full-resolution px (0.2): a patch whose largest
+                             group shift in the guided second iteration exceeds
+                             this restarts from zero shifts (plain solve) */
 } mc_params;
 
 /* N frames already gain/dark/defect corrected, resident in HBM
@@ -103,6 +111,11 @@ typedef struct mc_result {
     int    spectra_fmt_used[2];/* storage actually used for the xcf and patch
                                   stacks (a geometry without the register-pass
                                   kernels falls back to FP32) */
+    float* prior_features;     /* host P x (68 N + 2): each patch's prior input
+                                  after its first iteration (training data) */
+    float* prior_targets;      /* host P x N x 2: converged frame shifts minus
+                                  the first iteration's, in prior units */
+    int    prior_fallbacks;    /* patches that fell back to the plain solve */
 } mc_result;
 
 /* Replacements of measured quantities.  replay_* follow -InAln
@@ -213,6 +226,41 @@ int mc_get_kernel_timing(mc_ctx* ctx, double* total_ms, int* launches);
 int mc_synth_render(mc_ctx* ctx, float* frames, int W, int H, int N, long long pitch,
                     long long frame_stride, int nblobs, const float* blob_host,
                     const float* disp_host, float scale, unsigned long long seed, int poisson);
+
+/* ---- learned shift prior (no reference counterpart) ----
+ * An MLP in -> H -> H -> H -> out (ReLU) over rows of
+ *   in  = 68 N + 2 features of one patch after its first alignment iteration:
+ *         global trajectory (2N, xcf px / unit), patch centre (2, in [-1, 1]),
+ *         first-iteration frame shifts (2N, / unit), and per frame the 8x8
+ *         2x2-pooled centre of its group's correlation window (64N),
+ *   out = 2N: converged minus first-iteration frame shifts, / unit
+ *         (unit = 2 xcf px).
+ * Layer products are BF16 MFMA with FP32 accumulation, FP32 master weights.
+ * Parameters in one FP32 vector: W0 (in x H row-major), b0, W1, b1, W2, b2,
+ * W3 (H x out), b3.  x, t, y below are device row-major FP32 arrays. */
+typedef struct mc_prior mc_prior;
+int mc_prior_create(mc_ctx* ctx, int frames, int hidden, unsigned long long seed,
+                    mc_prior** out);
+int mc_prior_destroy(mc_ctx* ctx, mc_prior* prior);
+int mc_prior_dims(const mc_prior* prior, int* in_dim, int* out_dim, long long* n_params);
+/* copy the FP32 parameters out (get_host) and/or in (set_host), either NULL */
+int mc_prior_params(mc_ctx* ctx, mc_prior* prior, float* get_host, const float* set_host);
+int mc_prior_forward(mc_ctx* ctx, mc_prior* prior, const float* x, int n, float* y);
+/* loss = mean((y - t)^2) over n rows; grad_host (n_params floats) or NULL */
+int mc_prior_loss_grad(mc_ctx* ctx, mc_prior* prior, const float* x, const float* t, int n,
+                       double* loss, float* grad_host);
+/* minibatch Adam; epoch_loss_host (epochs floats) or NULL */
+int mc_prior_train(mc_ctx* ctx, mc_prior* prior, const float* x, const float* t, int n,
+                   int epochs, int batch, float lr, unsigned long long seed,
+                   float* epoch_loss_host);
+/* the prior used by mc_align_movie(s) when params.prior = 1 (NULL detaches);
+ * the context does not own it */
+int mc_set_prior(mc_ctx* ctx, mc_prior* prior);
+/* c[i][j] = sum_k a[i*sai + k*sak] b[k*sbk + j*sbj] (BF16 bit patterns in,
+ * FP32 out with row pitch ldc): the MFMA product the prior's layers use */
+int mc_gemm_bf16(mc_ctx* ctx, const unsigned short* a, long long sai, long long sak,
+                 const unsigned short* b, long long sbk, long long sbj, float* c, int ldc, int m,
+                 int n, int k);
 
 #ifdef __cplusplus
 }

@@ -20,6 +20,15 @@
 #include "../../include/mc_abi.h"
 #include "mc_align.h"
 #include "mc_fft.h"
+#include "mc_prior.h"
+
+// A learned shift prior on the context's device (mc_prior_create).
+constexpr float kPriorUnit = 2.0f;  // xcf px per unit of the prior's shift inputs and outputs
+struct mc_prior {
+    Prior net;
+    int frames = 0, hidden = 0;
+    int device = 0;
+};
 
 namespace {
 
@@ -50,6 +59,8 @@ struct mc_ctx {
     // second stream for the frame-chunk loops, and its fork/join events
     hipStream_t s2 = nullptr;
     hipEvent_t evf = nullptr, evj = nullptr;
+    // learned prior for guided patch alignment (not owned)
+    mc_prior* prior = nullptr;
 };
 
 namespace {
@@ -310,6 +321,25 @@ struct IterOut {
     std::vector<double> shifts;  // [J][N][2]
     std::vector<double> errors;  // [J][max_iter]
     std::vector<int> iters;      // [J]
+    std::vector<double> state;   // [J][N][2] running shifts at the end (with a PriorRun)
+    std::vector<unsigned char> fell;  // [J] guided jobs that restarted plainly
+};
+
+// Learned-prior hooks of run_iterative.  With X set, the first iteration
+// also writes every job's prior features to X.  With net set as well, the
+// iterations are guided: after the first iteration the network's correction
+// is added to the shifts, the second iteration measures the residual, and a
+// job whose residual exceeds `fallback` restarts from zero shifts (the plain
+// solve); all others stop after these two iterations.
+struct PriorRun {
+    mc_prior* net = nullptr;
+    const float* side = nullptr;  // [J][ns] device
+    int ns = 0;
+    float* X = nullptr;           // [J][ns + 66 N] device
+    float* Y = nullptr;           // [J][2N] device (guided)
+    unsigned char* fell = nullptr;  // [J] device (guided)
+    double fallback = 0.2;        // xcf px
+    float unit = 2.0f;
 };
 
 // Element `elems` of a spectrum stack stored in format fmt (mc_spec.h).
@@ -320,7 +350,7 @@ static cfloat* spec_at(void* base, long long elems, int fmt) {
 int run_iterative(mc_ctx* c, const char* tag, const cfloat* F, int J, int N, int W, int H,
                   Fft2D* pl, const Groups& grp, float bft, int phase_only, int max_iter,
                   double tol, int quirks, IterOut* out, int f_fmt = MC_SPEC_F32,
-                  const float* f_inv = nullptr, bool graphs = false) {
+                  const float* f_inv = nullptr, bool graphs = false, PriorRun* pr = nullptr) {
     const int cx = W / 2 + 1, G = grp.G;
     // linear group->frame interpolation needs two groups
     // (Align/CInterpolateShift.cpp:129-142 reads group 1)
@@ -368,6 +398,7 @@ int run_iterative(mc_ctx* c, const char* tag, const cfloat* F, int J, int N, int
     MC_TRY(hipMemcpyAsync(st.fgroup, grp.fgroup.data(), sizeof(int) * N, hipMemcpyHostToDevice, s));
     MC_TRY(hipMemcpyAsync(st.gcent, grp.gcent.data(), sizeof(double) * G, hipMemcpyHostToDevice, s));
     MC_TRY(hipMemcpyAsync(st.fcent, grp.fcent.data(), sizeof(double) * N, hipMemcpyHostToDevice, s));
+    bool feat = pr && pr->X;  // write the prior features after iteration 0
     // One iteration (Align/CIterativeAlign.cpp:117-140): B-factor -2 per
     // iteration with a floor of 20.
     auto bfac = [&](int it) {
@@ -400,27 +431,53 @@ int run_iterative(mc_ctx* c, const char* tag, const cfloat* F, int J, int N, int
             if ((e = mc_launch_row_dft(T1, twW, st.done, J, G, cx, W, seax, seay, win, s))) return e;
         }
         if ((e = mc_launch_peaks(win, st.done, J, G, seax, seay, st.gshift, s))) return e;
-        return mc_launch_update(st, it, tol, s);
+        if ((e = mc_launch_update(st, it, tol, s))) return e;
+        if (it == 0 && feat)
+            return prior_features(pr->side, pr->ns, st.tmp, pr->unit, win, J, N, G, st.fgroup, seax,
+                                  seay, pr->X, pr->ns + 66 * N, s);
+        return hipSuccess;
     };
+    // Guided start: iterations 0 and 1 with the prior's correction between
+    // them, then the residual gate; the block loop below then runs only for
+    // the jobs that fell back (all others are done).
+    std::vector<unsigned char> hdone(J);
+    bool skip_loop = false;
+    if (pr && pr->net) {
+        if (max_iter < 2) return fail(c, MC_E_ARG, "the learned prior needs -Iter >= 2");
+        Prior& net = pr->net->net;
+        if (!net.reserve(J)) return fail(c, MC_E_NOMEM, "prior work buffers");
+        MC_TRY(enqueue(0, true));
+        MC_TRY(net.forward(pr->X, J, pr->Y, s));
+        MC_TRY(prior_apply(pr->Y, J, N, G, st.gstart, st.gsize, st.fgroup, pr->unit, st.total,
+                           st.tmp, s));
+        MC_TRY(enqueue(1, true));
+        MC_TRY(prior_gate(st.errors, max_iter, st.done, st.iters, st.tmp, st.total, J, N, G,
+                          pr->fallback, pr->fell, s));
+        MC_TRY(hipMemcpyAsync(hdone.data(), st.done, J, hipMemcpyDeviceToHost, s));
+        MC_TRY(hipStreamSynchronize(s));
+        skip_loop = std::all_of(hdone.begin(), hdone.end(), [](unsigned char d) { return d != 0; });
+        feat = false;  // the restarted jobs keep the features of the guided pass
+    }
     // Blocks of two iterations; with `graphs` each block is captured once
     // into a HIP graph (no memset nodes) and replayed.  Kernel timing needs
     // the direct launches.
     const bool use_graph = graphs && c->timed.empty();
-    std::vector<unsigned char> hdone(J);
-    for (int it0 = 0; it0 < max_iter; it0 += 2) {
+    for (int it0 = 0; it0 < max_iter && !skip_loop; it0 += 2) {
         const int it1 = std::min(it0 + 2, max_iter);
         if (use_graph) {
             char key[1024];
             snprintf(key, sizeof(key),
                      "%s|%d|%d|%d %d %d %d %d %d|%.9g %.17g %d %d %d %d|%p %p %p %p %p %p %p %p %p %p "
-                     "%p %p %p %p %p %p %p %p %p %p %p %p %d %p",
+                     "%p %p %p %p %p %p %p %p %p %p %p %p %d %p|%d %p %p %d",
                      tag, it0, it1, J, N, W, H, G, max_iter, (double)bft, tol, phase_only, seax,
                      seay, pl->C, (const void*)F, (void*)A, (void*)B, (void*)S, (void*)T1,
                      (void*)win, (void*)twW, (void*)st.tmp, (void*)st.total, (void*)st.gshift,
                      (void*)st.final_, (void*)st.errors, (void*)st.iters, (void*)st.done,
                      (void*)st.gstart, (void*)st.gsize, (void*)st.fgroup, (void*)st.gcent,
                      (void*)st.fcent, (const void*)pl->col_ptw, (const void*)pl->col.tw,
-                     (const void*)f_inv, f_fmt, (const void*)pl);
+                     (const void*)f_inv, f_fmt, (const void*)pl, feat ? 1 : 0,
+                     feat ? (const void*)pr->side : nullptr, feat ? (void*)pr->X : nullptr,
+                     feat ? pr->ns : 0);
             auto itg = c->graphs.find(key);
             hipGraphExec_t ex = nullptr;
             if (itg == c->graphs.end()) {
@@ -466,6 +523,14 @@ int run_iterative(mc_ctx* c, const char* tag, const cfloat* F, int J, int N, int
     MC_TRY(hipMemcpyAsync(out->errors.data(), st.errors, sizeof(double) * J * max_iter,
                           hipMemcpyDeviceToHost, s));
     MC_TRY(hipMemcpyAsync(out->iters.data(), st.iters, sizeof(int) * J, hipMemcpyDeviceToHost, s));
+    if (pr) {
+        out->state.resize((size_t)J * N * 2);
+        MC_TRY(hipMemcpyAsync(out->state.data(), st.tmp, sizeof(double) * J * N * 2,
+                              hipMemcpyDeviceToHost, s));
+        out->fell.assign(J, 0);
+        if (pr->net)
+            MC_TRY(hipMemcpyAsync(out->fell.data(), pr->fell, J, hipMemcpyDeviceToHost, s));
+    }
     MC_TRY(hipStreamSynchronize(s));
     return MC_OK;
 }
@@ -614,6 +679,7 @@ int align_batch(mc_ctx* c, const mc_params* const* pp_, const mc_movie* const* m
     for (int m = 0; m < count; m++) {
         rs[m]->spectra_fmt_used[0] = fx;
         rs[m]->spectra_fmt_used[1] = fpch;
+        rs[m]->prior_fallbacks = 0;
     }
     const int CH = 2;  // frames per forward/inverse chunk (staging stays in the 256 MB MALL)
     // movie m's full spectra at Sf_all + m * N * fs, its xcf stack at element
@@ -775,10 +841,76 @@ int align_batch(mc_ctx* c, const mc_params* const* pp_, const mc_movie* const* m
                                0, 0, P * N, s, fpch, gp ? gp + pm : nullptr));
           }
             Groups g1 = make_groups(p->group[1], N, p->throw0);
+            // Learned prior: features of every patch after its first iteration
+            // (training data, when a result asks for them) and/or iterations
+            // guided by the attached network.
+            const bool want_feat = std::any_of(rs, rs + count, [](mc_result* r) {
+                return r->prior_features != nullptr || r->prior_targets != nullptr;
+            });
+            const bool guided = p->prior != 0;
+            const int J = count * P, ns = 2 * N + 2, D = ns + 66 * N;
+            PriorRun prr;
+            if (guided || want_feat) {
+                if (guided && (!c->prior || c->prior->frames != N))
+                    return fail(c, MC_E_ARG, "no learned prior for this frame count attached");
+                if (g.pw < 16 || g.ph < 16)
+                    return fail(c, MC_E_ARG, "the learned prior needs patches of >= 16 xcf px");
+                std::vector<float> side((size_t)J * ns);
+                for (int m = 0; m < count; m++)
+                    for (int q = 0; q < P; q++) {
+                        float* sd = side.data() + ((size_t)m * P + q) * ns;
+                        for (int i = 0; i < 2 * N; i++) sd[i] = (float)(ms[m].xs[i] / kPriorUnit);
+                        sd[2 * N] = (ms[m].starts[2 * q] + g.pw / 2) / (float)g.xw * 2.0f - 1.0f;
+                        sd[2 * N + 1] = (ms[m].starts[2 * q + 1] + g.ph / 2) / (float)g.xh * 2.0f - 1.0f;
+                    }
+                MC_BUF(dside, float, "prior_side", side.size());
+                MC_BUF(dX, float, "prior_X", (size_t)J * D);
+                MC_BUF(dY, float, "prior_Y", (size_t)J * 2 * N);
+                MC_BUF(dfell, unsigned char, "prior_fell", (size_t)J);
+                MC_TRY(hipMemcpyAsync(dside, side.data(), sizeof(float) * side.size(),
+                                      hipMemcpyHostToDevice, s));
+                MC_TRY(hipStreamSynchronize(s));
+                prr.net = guided ? c->prior : nullptr;
+                prr.side = dside;
+                prr.ns = ns;
+                prr.X = dX;
+                prr.Y = dY;
+                prr.fell = dfell;
+                prr.fallback = p->prior_fallback / std::max(g.xbin[0], g.xbin[1]);
+                prr.unit = kPriorUnit;
+            }
             IterOut po_;
-            MC_CHECK(run_iterative(c, "p_", Sp_all, count * P, N, g.pw, g.ph, pp, g1, p->bft[1],
+            MC_CHECK(run_iterative(c, "p_", Sp_all, J, N, g.pw, g.ph, pp, g1, p->bft[1],
                                    p->phase_only, p->iter, tol, quirks, &po_, fpch, ip,
-                                   p->launch_graphs != 0));
+                                   p->launch_graphs != 0, (guided || want_feat) ? &prr : nullptr));
+            std::vector<float> hx;
+            if (want_feat) {
+                hx.resize((size_t)J * D);
+                MC_TRY(hipMemcpyAsync(hx.data(), prr.X, sizeof(float) * hx.size(),
+                                      hipMemcpyDeviceToHost, s));
+                MC_TRY(hipStreamSynchronize(s));
+            }
+            for (int m = 0; m < count; m++) {
+                mc_result* r = rs[m];
+                if (r->prior_features)
+                    std::copy(hx.begin() + (size_t)m * P * D, hx.begin() + (size_t)(m + 1) * P * D,
+                              r->prior_features);
+                if (r->prior_targets)
+                    for (int q = 0; q < P; q++) {
+                        const size_t jq = (size_t)m * P + q;
+                        for (int i = 0; i < 2 * N; i++)
+                            r->prior_targets[(size_t)q * 2 * N + i] =
+                                (float)(po_.state[jq * 2 * N + i] / kPriorUnit) - hx[jq * D + ns + i];
+                    }
+                if (guided)
+                    for (int q = 0; q < P; q++) {
+                        const size_t jq = (size_t)m * P + q;
+                        if (po_.fell[jq]) {
+                            r->prior_fallbacks++;
+                            po_.iters[jq] += 2;  // the guided pass before the restart
+                        }
+                    }
+            }
           for (int m = 0; m < count; m++) {
             MovieState& q0 = ms[m];
             // Align/CMeasurePatches.cpp:66-82: centre and shifts in full-res px
@@ -992,6 +1124,8 @@ void mc_default_params(mc_params* p) {
     p->ref_quirks = 1;
     p->launch_graphs = 1;
     p->chunk_streams = 2;
+    p->prior = 0;
+    p->prior_fallback = 0.2f;
 }
 
 int mc_ctx_create(int device, void* stream, mc_ctx** out) {
@@ -1123,8 +1257,8 @@ static bool same_alignment_flags(const mc_params& a, const mc_params& b) {
            a.sum_range[1] == b.sum_range[1] && a.split_sum == b.split_sum &&
            a.throw0 == b.throw0 && a.ref_quirks == b.ref_quirks &&
            a.spectra_fmt == b.spectra_fmt && a.launch_graphs == b.launch_graphs &&
-           a.chunk_streams == b.chunk_streams &&
-           a.verbose == b.verbose;
+           a.chunk_streams == b.chunk_streams && a.prior == b.prior &&
+           a.prior_fallback == b.prior_fallback && a.verbose == b.verbose;
 }
 
 int mc_align_movies(mc_ctx* c, const mc_params* params, const mc_movie* movies, int count,
@@ -1390,3 +1524,105 @@ int mc_synth_render(mc_ctx* c, float* frames, int W, int H, int N, long long pit
 }
 
 }  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Learned shift prior (no reference counterpart; mc_prior.hip).
+int mc_prior_create(mc_ctx* c, int frames, int hidden, unsigned long long seed, mc_prior** out) {
+    if (!c || !out) return MC_E_ARG;
+    *out = nullptr;
+    if (frames < 2 || hidden < 1 || hidden > 4096) return fail(c, MC_E_ARG, "prior: frames >= 2, 1 <= hidden <= 4096");
+    MC_TRY(hipSetDevice(c->device));
+    mc_prior* pr = new mc_prior;
+    pr->frames = frames;
+    pr->hidden = hidden;
+    pr->device = c->device;
+    if (pr->net.init(68 * frames + 2, hidden, 2 * frames, seed) != 0) {
+        delete pr;
+        return fail(c, MC_E_NOMEM, "prior parameter allocation failed");
+    }
+    *out = pr;
+    return MC_OK;
+}
+
+int mc_prior_destroy(mc_ctx* c, mc_prior* pr) {
+    if (!c || !pr) return MC_E_ARG;
+    (void)hipStreamSynchronize(c->stream);
+    if (c->prior == pr) c->prior = nullptr;
+    delete pr;
+    return MC_OK;
+}
+
+int mc_prior_dims(const mc_prior* pr, int* in_dim, int* out_dim, long long* n_params) {
+    if (!pr) return MC_E_ARG;
+    if (in_dim) *in_dim = pr->net.dims[0];
+    if (out_dim) *out_dim = pr->net.dims[Prior::kLayers];
+    if (n_params) *n_params = pr->net.nparam;
+    return MC_OK;
+}
+
+int mc_prior_params(mc_ctx* c, mc_prior* pr, float* get_host, const float* set_host) {
+    if (!c || !pr) return MC_E_ARG;
+    const size_t b = sizeof(float) * pr->net.nparam;
+    if (get_host) {
+        MC_TRY(hipMemcpyAsync(get_host, pr->net.w, b, hipMemcpyDeviceToHost, c->stream));
+        MC_TRY(hipStreamSynchronize(c->stream));
+    }
+    if (set_host) {
+        MC_TRY(hipMemcpyAsync(pr->net.w, set_host, b, hipMemcpyHostToDevice, c->stream));
+        MC_TRY(pr->net.refresh16(c->stream));
+        MC_TRY(pr->net.reset_moments(c->stream));
+        MC_TRY(hipStreamSynchronize(c->stream));
+    }
+    return MC_OK;
+}
+
+int mc_prior_forward(mc_ctx* c, mc_prior* pr, const float* x, int n, float* y) {
+    if (!c || !pr || !x || !y || n < 1) return MC_E_ARG;
+    if (!pr->net.reserve(n)) return fail(c, MC_E_NOMEM, "prior work buffers");
+    MC_TRY(pr->net.forward(x, n, y, c->stream));
+    return MC_OK;
+}
+
+int mc_prior_loss_grad(mc_ctx* c, mc_prior* pr, const float* x, const float* t, int n, double* loss,
+                       float* grad_host) {
+    if (!c || !pr || !x || !t || n < 1) return MC_E_ARG;
+    Prior& net = pr->net;
+    if (!net.reserve(n)) return fail(c, MC_E_NOMEM, "prior work buffers");
+    float* y = (float*)net.scratch("yl", sizeof(float) * (size_t)n * net.dims[Prior::kLayers]);
+    double* acc = (double*)net.work["acc"];
+    if (!y || !acc) return fail(c, MC_E_NOMEM, "prior work buffers");
+    hipStream_t s = c->stream;
+    MC_TRY(net.forward(x, n, y, s));
+    MC_TRY(net.backward(y, t, n, s));
+    MC_TRY(hipMemsetAsync(acc, 0, sizeof(double), s));
+    MC_TRY(net.loss_sum(n, acc, s));
+    double h = 0.0;
+    MC_TRY(hipMemcpyAsync(&h, acc, sizeof(double), hipMemcpyDeviceToHost, s));
+    if (grad_host)
+        MC_TRY(hipMemcpyAsync(grad_host, net.g, sizeof(float) * net.nparam, hipMemcpyDeviceToHost, s));
+    MC_TRY(hipStreamSynchronize(s));
+    if (loss) *loss = h / ((double)n * net.dims[Prior::kLayers]);
+    return MC_OK;
+}
+
+int mc_prior_train(mc_ctx* c, mc_prior* pr, const float* x, const float* t, int n, int epochs,
+                   int batch, float lr, unsigned long long seed, float* epoch_loss) {
+    if (!c || !pr || !x || !t || n < 1 || epochs < 1 || batch < 1 || !(lr > 0)) return MC_E_ARG;
+    MC_TRY(pr->net.train(x, t, n, epochs, batch, lr, seed, epoch_loss, c->stream));
+    return MC_OK;
+}
+
+int mc_set_prior(mc_ctx* c, mc_prior* pr) {
+    if (!c) return MC_E_ARG;
+    if (pr && pr->device != c->device) return fail(c, MC_E_ARG, "prior lives on another device");
+    c->prior = pr;
+    return MC_OK;
+}
+
+int mc_gemm_bf16(mc_ctx* c, const unsigned short* a, long long sai, long long sak,
+                 const unsigned short* b, long long sbk, long long sbj, float* out, int ldc, int m,
+                 int n, int k) {
+    if (!c || !a || !b || !out || m < 1 || n < 1 || k < 1 || ldc < n) return MC_E_ARG;
+    MC_TRY(prior_gemm(a, sai, sak, b, sbk, sbj, out, ldc, m, n, k, c->stream));
+    return MC_OK;
+}

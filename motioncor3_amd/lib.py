@@ -29,6 +29,7 @@ class McParams(C.Structure):
         ("split_sum", C.c_int), ("throw0", C.c_int), ("ref_quirks", C.c_int),
         ("log_dir", C.c_char_p), ("movie_name", C.c_char_p), ("verbose", C.c_int),
         ("spectra_fmt", C.c_int), ("launch_graphs", C.c_int), ("chunk_streams", C.c_int),
+        ("prior", C.c_int), ("prior_fallback", C.c_float),
     ]
 
 
@@ -46,6 +47,8 @@ class McResult(C.Structure):
         ("patch_center", C.POINTER(C.c_float)), ("patch_starts", C.POINTER(C.c_int)),
         ("patch_iters", C.POINTER(C.c_int)), ("seconds_align", C.c_double),
         ("seconds_total", C.c_double), ("spectra_fmt_used", C.c_int * 2),
+        ("prior_features", C.POINTER(C.c_float)), ("prior_targets", C.POINTER(C.c_float)),
+        ("prior_fallbacks", C.c_int),
     ]
 
 
@@ -65,6 +68,8 @@ ABI_SYMBOLS = (
     "mc_phase_shift_accumulate", "mc_group_correlate_window", "mc_peak_fit",
     "mc_spline_despike_fp64", "mc_patch_stats_fp64", "mc_local_warp", "mc_dose_weight_sum",
     "mc_set_kernel_timing", "mc_get_kernel_timing", "mc_synth_render",
+    "mc_prior_create", "mc_prior_destroy", "mc_prior_dims", "mc_prior_params",
+    "mc_prior_forward", "mc_prior_loss_grad", "mc_prior_train", "mc_set_prior", "mc_gemm_bf16",
 )
 
 
@@ -320,7 +325,7 @@ class Context:
                                                 C.byref(ow), C.byref(oh)), "mc_output_geometry")
         return ns.value, ow.value, oh.value
 
-    def _prepare(self, frames, params, sums, width):
+    def _prepare(self, frames, params, sums, width, prior_data=False):
         N, H, pitch = frames.shape
         W = pitch if width is None else width
         ns, ow, oh = self.output_geometry(params, W, H, N)
@@ -348,6 +353,11 @@ class Context:
             res.patch_center = _fptr(out["patch_center"])
             res.patch_starts = out["patch_starts"].ctypes.data_as(C.POINTER(C.c_int))
             res.patch_iters = out["patch_iters"].ctypes.data_as(C.POINTER(C.c_int))
+            if prior_data:
+                out.update(prior_features=np.zeros((P, 68 * N + 2), np.float32),
+                           prior_targets=np.zeros((P, N, 2), np.float32))
+                res.prior_features = _fptr(out["prior_features"])
+                res.prior_targets = _fptr(out["prior_targets"])
         out["sums"] = sums
         out["num_sums"] = ns
         return mv, res, out
@@ -359,14 +369,16 @@ class Context:
         out["seconds_align"] = res.seconds_align
         out["seconds_total"] = res.seconds_total
         out["spectra_fmt_used"] = (res.spectra_fmt_used[0], res.spectra_fmt_used[1])
+        out["prior_fallbacks"] = res.prior_fallbacks
         return out
 
     def align_movie(self, frames, params, replay_global=None, replay_patch=None,
                     replay_centers=None, replay_bad=None, patch_starts=None, sums=None,
-                    width=None):
+                    width=None, prior_data=False):
         """Align an [N, H, pitch] float32 DevArray movie whose rows hold `width`
-        (default pitch) pixels.  Returns numpy shift tables and DevArray sums."""
-        mv, res, out = self._prepare(frames, params, sums, width)
+        (default pitch) pixels.  Returns numpy shift tables and DevArray sums;
+        with prior_data also each patch's prior features and targets."""
+        mv, res, out = self._prepare(frames, params, sums, width, prior_data)
         keep = []
 
         def fp(a, dt=np.float32, ct=C.c_float):
@@ -382,16 +394,97 @@ class Context:
                                             C.byref(res)), "mc_align_movie")
         return self._finish(res, out)
 
-    def align_movies(self, movies, params, sums=None):
+    def align_movies(self, movies, params, sums=None, prior_data=False):
         """mc_align_movies over a list of [N, H, W] float32 DevArrays with one
         McParams (or a list of them); `sums` optionally gives each movie's
         output DevArrays.  Returns one result dict per movie."""
         n = len(movies)
         plist = params if isinstance(params, (list, tuple)) else [params] * n
         slist = sums if sums is not None else [None] * n
-        prep = [self._prepare(f, p, sm, None) for f, p, sm in zip(movies, plist, slist)]
+        prep = [self._prepare(f, p, sm, None, prior_data) for f, p, sm in zip(movies, plist, slist)]
         mva = (McMovie * n)(*[q[0] for q in prep])
         ra = (McResult * n)(*[q[1] for q in prep])
         pa = (McParams * n)(*plist)
         self._check(self.lib.mc_align_movies(self.ptr, pa, mva, n, ra), "mc_align_movies")
         return [self._finish(ra[k], prep[k][2]) for k in range(n)]
+
+
+class Prior:
+    """A learned shift prior (mc_prior_*) on a context's device."""
+
+    def __init__(self, ctx, frames, hidden=128, seed=1):
+        self.ctx = ctx
+        self.ptr = C.c_void_p()
+        ctx._check(ctx.lib.mc_prior_create(ctx.ptr, frames, hidden, C.c_ulonglong(seed),
+                                           C.byref(self.ptr)), "mc_prior_create")
+        i, o, n = C.c_int(), C.c_int(), C.c_longlong()
+        ctx.lib.mc_prior_dims(self.ptr, C.byref(i), C.byref(o), C.byref(n))
+        self.in_dim, self.out_dim, self.n_params = i.value, o.value, n.value
+        self.frames, self.hidden = frames, hidden
+
+    def close(self):
+        if self.ptr:
+            self.ctx.lib.mc_prior_destroy(self.ctx.ptr, self.ptr)
+            self.ptr = C.c_void_p()
+
+    def params(self):
+        out = np.zeros(self.n_params, np.float32)
+        self.ctx._check(self.ctx.lib.mc_prior_params(self.ctx.ptr, self.ptr, _fptr(out), None),
+                        "mc_prior_params")
+        return out
+
+    def set_params(self, w):
+        w = np.ascontiguousarray(w, np.float32)
+        assert w.size == self.n_params
+        self.ctx._check(self.ctx.lib.mc_prior_params(self.ctx.ptr, self.ptr, None, _fptr(w)),
+                        "mc_prior_params")
+
+    def layers(self, w=None):
+        """Split a parameter vector into [(W0, b0), ... (W3, b3)]."""
+        w = self.params() if w is None else w
+        dims = [self.in_dim, self.hidden, self.hidden, self.hidden, self.out_dim]
+        out, off = [], 0
+        for l in range(4):
+            W = w[off: off + dims[l] * dims[l + 1]].reshape(dims[l], dims[l + 1])
+            off += dims[l] * dims[l + 1]
+            b = w[off: off + dims[l + 1]]
+            off += dims[l + 1]
+            out.append((W, b))
+        return out
+
+    def forward(self, x):
+        """x: [n, in_dim] DevArray -> [n, out_dim] numpy."""
+        n = x.shape[0]
+        y = self.ctx.empty((n, self.out_dim), np.float32)
+        self.ctx._check(self.ctx.lib.mc_prior_forward(self.ctx.ptr, self.ptr, x.vp(), n, y.vp()),
+                        "mc_prior_forward")
+        return y.numpy()
+
+    def loss_grad(self, x, t):
+        loss = C.c_double()
+        g = np.zeros(self.n_params, np.float32)
+        self.ctx._check(self.ctx.lib.mc_prior_loss_grad(self.ctx.ptr, self.ptr, x.vp(), t.vp(),
+                                                        x.shape[0], C.byref(loss), _fptr(g)),
+                        "mc_prior_loss_grad")
+        return loss.value, g
+
+    def train(self, x, t, epochs, batch=64, lr=1e-3, seed=7):
+        hist = np.zeros(epochs, np.float32)
+        self.ctx._check(self.ctx.lib.mc_prior_train(self.ctx.ptr, self.ptr, x.vp(), t.vp(),
+                                                    x.shape[0], epochs, batch, C.c_float(lr),
+                                                    C.c_ulonglong(seed), _fptr(hist)),
+                        "mc_prior_train")
+        return hist
+
+    def attach(self, on=True):
+        self.ctx._check(self.ctx.lib.mc_set_prior(self.ctx.ptr, self.ptr if on else None),
+                        "mc_set_prior")
+
+
+def gemm_bf16(ctx, a16, sa, b16, sb, c, m, n, k):
+    """c[i][j] = sum_k a16[i*sa[0] + k*sa[1]] * b16[k*sb[0] + j*sb[1]] (uint16 BF16
+    DevArrays in, [m, n] float32 DevArray out)."""
+    ctx._check(ctx.lib.mc_gemm_bf16(ctx.ptr, a16.vp(), C.c_longlong(sa[0]), C.c_longlong(sa[1]),
+                                    b16.vp(), C.c_longlong(sb[0]), C.c_longlong(sb[1]), c.vp(),
+                                    c.shape[1], m, n, k),
+               "mc_gemm_bf16")
