@@ -139,6 +139,12 @@ def main():
     ap.add_argument("--copy-calib", type=int, default=0, metavar="MIB",
                     help="one plain device-to-device copy of MIB MiB before the steps, as the "
                          "known-size reference for hardware byte counters")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="keep the default FFT tiles instead of timing the compiled variants "
+                         "(mc_autotune) before the steps")
+    ap.add_argument("--prior", type=int, default=0, metavar="K",
+                    help="train the learned shift prior on K synthetic movies of the workload "
+                         "(labels from the plain path), then time the guided path")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on one GPU only: the path is data-parallel over movies "
@@ -157,6 +163,10 @@ def main():
     prm = L.default_params(**kw)
     ns, ow, oh = ctx.output_geometry(prm, W, H, N)
     sums = [ctx.empty((oh, ow), np.float32) for _ in range(ns)]
+    tiles = None
+    if not args.no_autotune:
+        rep = ctx.autotune(W, H, N, patch=(5, 5))
+        tiles = [ln.strip() for ln in rep.splitlines() if "->" in ln]
     if args.copy_calib > 0:
         nb = args.copy_calib << 20
         a = ctx.empty((nb // 4,), np.float32)
@@ -172,6 +182,38 @@ def main():
         ctx.synth_render(fk, rk)
         movies.append(fk)
         msums.append([ctx.empty((oh, ow), np.float32) for _ in range(ns)])
+
+    prior_info = None
+    if args.prior > 0:
+        # training data: the plain path's own first-iteration features and
+        # converged patch shifts on movies of the same workload (other seeds)
+        t_p = time.perf_counter()
+        fx = ctx.empty((N, H, W), np.float32)
+        X, T, pit = [], [], []
+        for k in range(args.prior):
+            ctx.synth_render(fx, MovieRecipe(W, H, N, 20000, seed=7000 + k, dose=1.0, local_max=3.0))
+            rr = ctx.align_movie(fx, prm, sums=sums, prior_data=True)
+            X.append(rr["prior_features"])
+            T.append(rr["prior_targets"].reshape(rr["prior_targets"].shape[0], -1))
+            pit.append(rr["patch_iters"])
+        del fx
+        X, T = np.concatenate(X), np.concatenate(T)
+        t_d = time.perf_counter()
+        prior = L.Prior(ctx, N, 128, seed=1)
+        nv = max(1, len(X) // 8)
+        xtr, ttr = ctx.upload(X[nv:]), ctx.upload(T[nv:])
+        epochs = 150
+        hist = prior.train(xtr, ttr, epochs=epochs, batch=64, lr=5e-4, seed=0)
+        yv = prior.forward(ctx.upload(X[:nv]))
+        t_t = time.perf_counter()
+        prior.attach()
+        prm = L.default_params(prior=1, **kw)
+        prior_info = {"train_movies": args.prior, "samples": int(len(X)), "epochs": epochs,
+                      "params": int(prior.n_params), "data_s": t_d - t_p, "train_s": t_t - t_d,
+                      "train_mse": float(hist[-1]),
+                      "heldout_mse": float(((yv - T[:nv]) ** 2).mean()),
+                      "heldout_mse_zero_correction": float((T[:nv] ** 2).mean()),
+                      "plain_patch_iters_mean_on_training_movies": float(np.mean(pit))}
 
     def step():
         if K == 1:
@@ -246,6 +288,8 @@ def main():
             "spectra_storage": args.spectra,
             "movies_per_call": K,
             "spectra_storage_used": [["f32", "f16", "bf16"][v] for v in res["spectra_fmt_used"]],
+            "learned_prior": args.prior > 0,
+            "fft_tiles": tiles if tiles is not None else "defaults",
         },
         "roofline": {
             "kernel": args.kernel,
@@ -270,6 +314,10 @@ def main():
         },
         "accuracy": {"global_shift_rms_vs_truth_px": truth_rms},
     }
+    if prior_info is not None:
+        prior_info["fallbacks_last_step"] = res["prior_fallbacks"]
+        prior_info["patch_iters_mean_last_step"] = float(np.mean(res["patch_iters"]))
+        out["prior"] = prior_info
     if h2d is not None:
         out["h2d"] = {"ms_per_movie": h2d, "pageable_host_memory": True,
                       "pcie_inclusive_movies_per_s": 1000.0 / (ms + h2d)}

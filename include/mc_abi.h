@@ -227,6 +227,19 @@ int mc_synth_render(mc_ctx* ctx, float* frames, int W, int H, int N, long long p
                     long long frame_stride, int nblobs, const float* blob_host,
                     const float* disp_host, float scale, unsigned long long seed, int poisson);
 
+/* ---- FFT tile autotuner (no reference counterpart: cuFFT plans its own
+ * kernels, Util/CCufft2D.cpp:23-53) ----
+ * The register-pass FFT kernels have several compiled tile variants per
+ * length (rows or columns per workgroup, threads), all bit-identical in
+ * results.  mc_autotune times them on the batch shapes an alignment of a
+ * W x H x N movie with px x py patches uses and keeps the fastest in the
+ * context's plans; report (may be NULL) receives the timings as text.
+ * mc_fft_variant queries (set < 0) or forces (set >= 0) the variant of the
+ * W x H plan's rows (cols = 0) or columns (cols = 1). */
+int mc_autotune(mc_ctx* ctx, int W, int H, int N, int px, int py, float ft_bin, char* report,
+                int report_len);
+int mc_fft_variant(mc_ctx* ctx, int W, int H, int cols, int set, int* count, int* current);
+
 /* ---- learned shift prior (no reference counterpart) ----
  * An MLP in -> H -> H -> H -> out (ReLU) over rows of
  *   in  = 68 N + 2 features of one patch after its first alignment iteration:

@@ -9,6 +9,8 @@
 // Forward transforms use e^{-i}, inverse e^{+i}; both are unnormalised
 // unless a scale is given (same conventions as cufftExecR2C/C2R).
 #pragma once
+#include <string>
+#include <vector>
 #include "mc_common.h"
 #include "mc_spec.h"
 
@@ -39,6 +41,9 @@ struct Fft2D {
     cfloat* row_ptw = nullptr;  // register-pass twiddles of the row length m (or null)
     long long t_elems = 0;     // T elements per transform
     long long s_elems = 0;     // S elements per transform (cx * H)
+    // tile variant of the register-pass row / column kernels (index among the
+    // compiled variants for this length; 0 = the default, mc_autotune picks)
+    int row_var = 0, col_var = 0;
 };
 
 // Per-transform phase factors e^{i(kx*ax + ky*ay)} applied on load (skip
@@ -93,3 +98,7 @@ hipError_t mc_launch_spec_scale(const float* dc, int rows, int batch, float* g, 
 // formats and row DC output) run the plan's rows / columns.
 bool mc_fft_rp_rows(const Fft2D& p);
 bool mc_fft_rp_cols(const Fft2D& p);
+// Compiled tile variants of the plan's register-pass row (cols = 0) or
+// column (cols = 1) kernels, with a printable name each (0 when the plan has
+// no register-pass kernels on that side).
+int mc_fft_variants(const Fft2D& p, int cols, std::vector<std::string>* names);

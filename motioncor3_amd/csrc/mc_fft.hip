@@ -6,6 +6,7 @@
 // (complex along y, C adjacent columns per workgroup).  The staging layout
 // between the passes is column-blocked so both passes move contiguous runs.
 #include <cmath>
+#include <cstdio>
 #include <vector>
 
 #include "mc_fft.h"
@@ -751,7 +752,10 @@ __global__ __launch_bounds__(NT) void k_cols_inv_rp(ColInvArgs a, const cfloat* 
 }
 
 #ifndef MC_RP_COL_SPECS
-#define MC_RP_COL_SPECS(X) X(4092, 2, 264) X(4096, 2, 256) X(2048, 4, 512) X(408, 16, 384) X(8184, 1, 264)
+// (height, columns per workgroup, threads).  The first entry of a height is
+// its default tile; later entries are variants that mc_autotune may choose.
+#define MC_RP_COL_SPECS(X) X(4092, 2, 264) X(4096, 2, 256) X(2048, 4, 512) X(408, 16, 384) X(8184, 1, 264) \
+    X(4092, 4, 528) X(4092, 1, 384) X(4096, 4, 512) X(2048, 2, 256) X(408, 8, 384)
 #endif
 
 // ---------------------------------------------------------------------------
@@ -917,8 +921,10 @@ __global__ __launch_bounds__(NT) void k_rows_c2r_rp(RowC2RArgs a, const cfloat* 
 }
 
 #ifndef MC_RP_ROW_SPECS
+// (row length, rows per workgroup, staging chunk width, threads); the first
+// entry of a length is its default, later ones are autotuner variants.
 #define MC_RP_ROW_SPECS(X) X(2880, 2, MC_TILE_CT, 512) X(2048, 2, MC_TILE_CT, 512) X(288, 16, 16, 512) \
-    X(5760, 1, MC_TILE_CT, 512)
+    X(5760, 1, MC_TILE_CT, 512) X(2880, 1, MC_TILE_CT, 256) X(2880, 4, MC_TILE_CT, 512) X(288, 8, 16, 256)
 #endif
 
 
@@ -1157,6 +1163,31 @@ bool mc_fft_rp_cols(const Fft2D& p) {
     return false;
 }
 
+int mc_fft_variants(const Fft2D& p, int cols, std::vector<std::string>* names) {
+    int n = 0;
+    char nm[96];
+    if (cols) {
+#define X(HH, CC, NT)                                                                          \
+    if (p.H == HH && p.C % CC == 0 && p.col_ptw) {                                             \
+        snprintf(nm, sizeof(nm), "cols<%d> %d col/wg %d thr", HH, CC, NT);                    \
+        if (names) names->push_back(nm);                                                       \
+        n++;                                                                                   \
+    }
+        MC_RP_COL_SPECS(X)
+#undef X
+    } else {
+#define X(MM, RRR, CCC, NT)                                                                    \
+    if (p.m == MM && p.C == CCC && p.row_ptw) {                                                \
+        snprintf(nm, sizeof(nm), "rows<%d> %d row/wg %d thr", MM, RRR, NT);                   \
+        if (names) names->push_back(nm);                                                       \
+        n++;                                                                                   \
+    }
+        MC_RP_ROW_SPECS(X)
+#undef X
+    }
+    return n;
+}
+
 hipError_t mc_rows_r2c(const Fft2D& p, const float* in, long long pitch, long long in_bstride,
                        const int* win_xy, cfloat* T, long long t_bstride, int batch, float scale,
                        hipStream_t s) {
@@ -1184,8 +1215,9 @@ hipError_t mc_rows_r2c_mod(const Fft2D& p, const float* in, long long pitch, lon
     a.dc = dc;
     dim3 grid(mc_div_up(p.H, p.R) * batch);
     const size_t lds = (size_t)mc_lds_padded(p.R * p.m) * sizeof(cfloat);
+    int v_ = 0;
 #define X(MM, RRR, CCC, NT)                                                                 \
-    if (p.m == MM && p.C == CCC && p.row_ptw) {                                            \
+    if (p.m == MM && p.C == CCC && p.row_ptw && v_++ == p.row_var) {                       \
         hipLaunchKernelGGL((k_rows_r2c_rp<MM, RRR, CCC, NT>),                              \
                            dim3(mc_div_up(p.H, RRR) * batch), dim3(NT),                    \
                            (size_t)RRR * MM * sizeof(cfloat), s, a, (const cfloat*)p.row_ptw); \
@@ -1225,8 +1257,9 @@ hipError_t mc_rows_c2r(const Fft2D& p, const cfloat* T, long long t_bstride, flo
     a.clamp = clamp;
     dim3 grid(mc_div_up(p.H, p.R) * batch);
     const size_t lds = (size_t)mc_lds_padded(p.R * (p.m + 1)) * sizeof(cfloat);
+    int v_ = 0;
 #define X(MM, RRR, CCC, NT)                                                                 \
-    if (p.m == MM && p.C == CCC && p.row_ptw) {                                            \
+    if (p.m == MM && p.C == CCC && p.row_ptw && v_++ == p.row_var) {                       \
         hipLaunchKernelGGL((k_rows_c2r_rp<MM, RRR, CCC, NT>),                              \
                            dim3(mc_div_up(p.H, RRR) * batch), dim3(NT),                    \
                            (size_t)RRR * (MM + 1) * sizeof(cfloat), s, a,                  \
@@ -1269,8 +1302,9 @@ hipError_t mc_cols_fwd(const Fft2D& p, const cfloat* T, long long t_bstride, cfl
     a.x_g = x_g;
     dim3 grid(mc_div_up(p.cx, p.C) * batch);
     const size_t lds = (size_t)mc_lds_padded(p.C * p.H) * sizeof(cfloat);
+    int v_ = 0;
 #define X(HH, CC, NT)                                                              \
-    if (p.H == HH && p.C % CC == 0 && p.col_ptw)                                   \
+    if (p.H == HH && p.C % CC == 0 && p.col_ptw && v_++ == p.col_var)              \
         return launch_cols_fwd_rp<HH, CC, NT>(a, p.col_ptw, batch, s_fmt, x_fmt, s);
     MC_RP_COL_SPECS(X)
 #undef X
@@ -1306,8 +1340,9 @@ hipError_t mc_cols_inv(const Fft2D& p, const cfloat* S, long long s_bstride, int
     a.s_inv = s_inv;
     dim3 grid(mc_div_up(p.cx, p.C) * batch);
     const size_t lds = (size_t)mc_lds_padded(p.C * p.H) * sizeof(cfloat);
+    int v_ = 0;
 #define X(HH, CC, NT)                                                              \
-    if (p.H == HH && p.C % CC == 0 && p.col_ptw)                                   \
+    if (p.H == HH && p.C % CC == 0 && p.col_ptw && v_++ == p.col_var)              \
         return launch_cols_inv_rp<HH, CC, NT>(a, p.col_ptw, batch, s_fmt, s);
     MC_RP_COL_SPECS(X)
 #undef X

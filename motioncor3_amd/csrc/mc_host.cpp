@@ -1626,3 +1626,146 @@ int mc_gemm_bf16(mc_ctx* c, const unsigned short* a, long long sai, long long sa
     MC_TRY(prior_gemm(a, sai, sak, b, sbk, sbj, out, ldc, m, n, k, c->stream));
     return MC_OK;
 }
+
+// ---------------------------------------------------------------------------
+// FFT tile autotuner (no reference counterpart: cuFFT plans its own kernels,
+// Util/CCufft2D.cpp:23-53).  Every register-pass row and column kernel has a
+// few compiled tile variants (rows or columns per workgroup, threads); all
+// of them compute each row or column with the same arithmetic, so results
+// are bit-identical whichever runs.  mc_autotune times them on the batch
+// shapes the alignment uses and keeps the fastest in the plan.
+namespace {
+
+// Median of `reps` timed runs of fn() on the context stream, in ms.
+template <class F>
+int time_median(mc_ctx* c, F fn, int reps, double* ms) {
+    hipEvent_t a, b;
+    MC_TRY(hipEventCreate(&a));
+    MC_TRY(hipEventCreate(&b));
+    std::vector<double> t;
+    hipError_t e = fn();  // warm-up
+    for (int r = 0; r < reps && e == hipSuccess; r++) {
+        e = hipEventRecord(a, c->stream);
+        if (e == hipSuccess) e = fn();
+        if (e == hipSuccess) e = hipEventRecord(b, c->stream);
+        if (e == hipSuccess) e = hipEventSynchronize(b);
+        float f = 0.f;
+        if (e == hipSuccess) e = hipEventElapsedTime(&f, a, b);
+        t.push_back(f);
+    }
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+    if (e != hipSuccess) return fail(c, MC_E_HIP, std::string("autotune timing: ") + hipGetErrorString(e));
+    std::sort(t.begin(), t.end());
+    *ms = t[t.size() / 2];
+    return MC_OK;
+}
+
+// Tune one plan's rows and columns for `batch` transforms.
+int tune_plan(mc_ctx* c, Fft2D* p, int batch, std::string* rep) {
+    const long long te = p->t_elems, se = p->s_elems;
+    float* real = nullptr;
+    cfloat* T = nullptr;
+    cfloat* S = nullptr;
+    const size_t rb = sizeof(float) * (size_t)batch * p->W * p->H;
+    if (hipMalloc(&real, rb) != hipSuccess || hipMalloc(&T, sizeof(cfloat) * (size_t)batch * te) != hipSuccess ||
+        hipMalloc(&S, sizeof(cfloat) * (size_t)batch * se) != hipSuccess) {
+        (void)hipFree(real);
+        (void)hipFree(T);
+        (void)hipFree(S);
+        return fail(c, MC_E_NOMEM, "autotune buffers");
+    }
+    int rc = MC_OK;
+    hipStream_t s = c->stream;
+    if (hipMemsetAsync(real, 0, rb, s) != hipSuccess ||
+        hipMemsetAsync(T, 0, sizeof(cfloat) * (size_t)batch * te, s) != hipSuccess)
+        rc = fail(c, MC_E_HIP, "autotune buffer clear");
+    char line[256];
+    for (int cols = 0; cols < 2 && rc == MC_OK; cols++) {
+        std::vector<std::string> names;
+        const int nv = mc_fft_variants(*p, cols, &names);
+        if (nv < 2) continue;
+        int best = 0;
+        double best_ms = 1e30;
+        for (int v = 0; v < nv && rc == MC_OK; v++) {
+            (cols ? p->col_var : p->row_var) = v;
+            double ms = 0;
+            if (cols)
+                rc = time_median(c, [&]() -> hipError_t {
+                    hipError_t e = mc_cols_fwd(*p, T, te, S, se, nullptr, 0, 0, 0, batch, s);
+                    return e ? e : mc_cols_inv(*p, S, se, p->cx, p->H, nullptr, T, te, batch, s);
+                }, 5, &ms);
+            else
+                rc = time_median(c, [&]() -> hipError_t {
+                    hipError_t e = mc_rows_r2c(*p, real, p->W, (long long)p->W * p->H, nullptr, T, te,
+                                               batch, 1.0f, s);
+                    return e ? e : mc_rows_c2r(*p, T, te, real, p->W, (long long)p->W * p->H, batch, 0, s);
+                }, 5, &ms);
+            snprintf(line, sizeof(line), "%dx%d x%d %s: %.4f ms (forward + inverse)\n", p->W, p->H,
+                     batch, names[v].c_str(), ms);
+            *rep += line;
+            if (ms < best_ms) {
+                best_ms = ms;
+                best = v;
+            }
+        }
+        (cols ? p->col_var : p->row_var) = best;
+        snprintf(line, sizeof(line), "%dx%d %s -> %s\n", p->W, p->H, cols ? "columns" : "rows",
+                 names[best].c_str());
+        *rep += line;
+    }
+    (void)hipStreamSynchronize(s);
+    (void)hipFree(real);
+    (void)hipFree(T);
+    (void)hipFree(S);
+    return rc;
+}
+
+}  // namespace
+
+int mc_autotune(mc_ctx* c, int W, int H, int N, int px, int py, float ft_bin, char* report,
+                int report_len) {
+    if (!c || W < 8 || H < 8 || (W % 2) || N < 1) return c ? fail(c, MC_E_ARG, "autotune: bad geometry") : MC_E_ARG;
+    MC_TRY(hipSetDevice(c->device));
+    const bool patch = px > 1 && py > 1;
+    const Geo g = make_geo(W, H, N, px, py, ft_bin > 0 ? ft_bin : 1.0f);
+    // the batches the alignment launches: 2-frame chunks of full frames and
+    // output sums, N xcf frames, P * N patch frames
+    struct Item { int w, h, batch; };
+    std::vector<Item> items = {{W, H, 2}, {g.xw, g.xh, N}};
+    if (patch) items.push_back({g.pw, g.ph, px * py * N});
+    if (g.ow != W || g.oh != H) items.push_back({g.ow, g.oh, 2});
+    std::string rep;
+    std::vector<Fft2D*> done;
+    for (const Item& it : items) {
+        Fft2D* p = plan(c, it.w, it.h);
+        if (!p) return fail(c, MC_E_ARG, "autotune: unsupported FFT size");
+        if (std::find(done.begin(), done.end(), p) != done.end()) continue;
+        done.push_back(p);
+        MC_CHECK(tune_plan(c, p, it.batch, &rep));
+    }
+    // captured iteration graphs bake in launch shapes: drop them
+    for (auto& kv : c->graphs) (void)hipGraphExecDestroy(kv.second);
+    c->graphs.clear();
+    if (report && report_len > 0) {
+        const size_t n = std::min((size_t)report_len - 1, rep.size());
+        std::memcpy(report, rep.data(), n);
+        report[n] = 0;
+    }
+    return MC_OK;
+}
+
+int mc_fft_variant(mc_ctx* c, int W, int H, int cols, int set, int* count, int* current) {
+    if (!c) return MC_E_ARG;
+    Fft2D* p = plan(c, W, H);
+    if (!p) return fail(c, MC_E_ARG, "unsupported FFT size");
+    const int n = mc_fft_variants(*p, cols ? 1 : 0, nullptr);
+    if (set >= 0) {
+        if (set >= std::max(n, 1)) return fail(c, MC_E_ARG, "no such FFT tile variant");
+        MC_TRY(hipStreamSynchronize(c->stream));
+        (cols ? p->col_var : p->row_var) = set;
+    }
+    if (count) *count = n;
+    if (current) *current = cols ? p->col_var : p->row_var;
+    return MC_OK;
+}

@@ -70,6 +70,7 @@ ABI_SYMBOLS = (
     "mc_set_kernel_timing", "mc_get_kernel_timing", "mc_synth_render",
     "mc_prior_create", "mc_prior_destroy", "mc_prior_dims", "mc_prior_params",
     "mc_prior_forward", "mc_prior_loss_grad", "mc_prior_train", "mc_set_prior", "mc_gemm_bf16",
+    "mc_autotune", "mc_fft_variant",
 )
 
 
@@ -306,6 +307,22 @@ class Context:
             C.c_longlong(frames.stride(0)), len(blobs), _fptr(blobs), _fptr(disp),
             C.c_float(recipe.scale), C.c_ulonglong(recipe.seed if seed is None else seed),
             int(poisson)), "mc_synth_render")
+
+    def autotune(self, W, H, N, patch=(0, 0), ft_bin=1.0):
+        """mc_autotune: time the FFT tile variants of this geometry and keep
+        the fastest.  Returns the report text."""
+        buf = C.create_string_buffer(16384)
+        self._check(self.lib.mc_autotune(self.ptr, W, H, N, patch[0], patch[1], C.c_float(ft_bin),
+                                         buf, len(buf)), "mc_autotune")
+        return buf.value.decode()
+
+    def fft_variant(self, W, H, cols, set=-1):
+        """(count, current) tile variants of the W x H plan's rows or columns;
+        set >= 0 forces one."""
+        n, cur = C.c_int(), C.c_int()
+        self._check(self.lib.mc_fft_variant(self.ptr, W, H, int(cols), set, C.byref(n),
+                                            C.byref(cur)), "mc_fft_variant")
+        return n.value, cur.value
 
     def set_kernel_timing(self, name):
         self._check(self.lib.mc_set_kernel_timing(self.ptr, name.encode() if name else None),

@@ -49,3 +49,53 @@ def test_clamped_c2r(ctx):
     spec = ctx.rfft2(ctx.upload(x), scale=1.0 / (64 * 96))
     out = ctx.irfft2(spec, 96, clamp=True).numpy()
     assert np.allclose(out, np.maximum(x, 0), atol=1e-5)
+
+
+@pytest.mark.parametrize("W,H", [(5760, 4092), (2882, 2048), (576, 408), (8192, 4096)])
+def test_tile_variants_are_bit_identical(W, H):
+    """Every compiled tile variant of a register-pass row or column kernel
+    computes each row or column with the same arithmetic, so forward and
+    inverse transforms must agree bit for bit with the default tile."""
+    from motioncor3_amd.lib import Context
+    c = Context(0)
+    try:
+        rng = np.random.default_rng(W + H)
+        fr = c.upload(rng.normal(1.0, 0.5, (2, H, W)).astype(np.float32))
+        counts = [c.fft_variant(W, H, cols)[0] for cols in (0, 1)]
+        assert max(counts) >= 2, counts  # each size here has variants on some side
+        ref_s = c.rfft2(fr).numpy()
+        ref_r = c.irfft2(c.upload(ref_s), W).numpy()
+        for cols in (0, 1):
+            for v in range(1, counts[cols]):
+                c.fft_variant(W, H, cols, set=v)
+                s = c.rfft2(fr).numpy()
+                r = c.irfft2(c.upload(ref_s), W).numpy()
+                assert np.array_equal(s, ref_s), (cols, v)
+                assert np.array_equal(r, ref_r), (cols, v)
+            c.fft_variant(W, H, cols, set=0)
+    finally:
+        c.close()
+
+
+def test_autotune_keeps_alignment_results():
+    """mc_autotune picks tile variants by timing; the alignment results must
+    not change."""
+    from motioncor3_amd import lib as L
+    from motioncor3_amd.synth import MovieRecipe
+    W, H, N = 5760, 4092, 6
+    c = L.Context(0)
+    try:
+        r = MovieRecipe(W, H, N, 20000, seed=606, dose=1.0, local_max=3.0)
+        fr = c.empty((N, H, W), np.float32)
+        c.synth_render(fr, r)
+        pg = L.default_params(patch=(5, 5), pix_size=0.83, kv=300, fm_dose=1.0)
+        a = c.align_movie(fr, pg)
+        rep = c.autotune(W, H, N, patch=(5, 5))
+        assert "5760x4092 columns ->" in rep and "5760x4092 rows ->" in rep, rep
+        b = c.align_movie(fr, pg)
+        assert np.array_equal(a["global_shift"], b["global_shift"])
+        assert np.array_equal(a["patch_shift"], b["patch_shift"])
+        for i in range(a["num_sums"]):
+            assert np.array_equal(a["sums"][i].numpy(), b["sums"][i].numpy())
+    finally:
+        c.close()
