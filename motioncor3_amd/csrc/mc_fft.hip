@@ -754,8 +754,9 @@ __global__ __launch_bounds__(NT) void k_cols_inv_rp(ColInvArgs a, const cfloat* 
 #ifndef MC_RP_COL_SPECS
 // (height, columns per workgroup, threads).  The first entry of a height is
 // its default tile; later entries are variants that mc_autotune may choose.
-#define MC_RP_COL_SPECS(X) X(4092, 2, 264) X(4096, 2, 256) X(2048, 4, 512) X(408, 16, 384) X(8184, 1, 264) \
-    X(4092, 4, 528) X(4092, 1, 384) X(4096, 4, 512) X(2048, 2, 256) X(408, 8, 384)
+#define MC_RP_COL_SPECS(X) X(4092, 2, 512) X(4096, 2, 256) X(2048, 4, 512) X(408, 16, 384) X(8184, 1, 264) \
+    X(4092, 2, 264) X(4092, 2, 384) X(4092, 2, 448) X(4092, 1, 192) X(4092, 1, 256) X(4092, 4, 528) \
+    X(4092, 4, 1024) X(4096, 2, 384) X(4096, 2, 512) X(4096, 4, 512) X(2048, 2, 256) X(408, 8, 384)
 #endif
 
 // ---------------------------------------------------------------------------
@@ -924,7 +925,8 @@ __global__ __launch_bounds__(NT) void k_rows_c2r_rp(RowC2RArgs a, const cfloat* 
 // (row length, rows per workgroup, staging chunk width, threads); the first
 // entry of a length is its default, later ones are autotuner variants.
 #define MC_RP_ROW_SPECS(X) X(2880, 2, MC_TILE_CT, 512) X(2048, 2, MC_TILE_CT, 512) X(288, 16, 16, 512) \
-    X(5760, 1, MC_TILE_CT, 512) X(2880, 1, MC_TILE_CT, 256) X(2880, 4, MC_TILE_CT, 512) X(288, 8, 16, 256)
+    X(5760, 1, MC_TILE_CT, 512) X(2880, 1, MC_TILE_CT, 256) X(2880, 2, MC_TILE_CT, 384) \
+    X(2880, 1, MC_TILE_CT, 192) X(288, 8, 16, 256)
 #endif
 
 
