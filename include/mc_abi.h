@@ -227,6 +227,48 @@ int mc_synth_render(mc_ctx* ctx, float* frames, int W, int H, int N, long long p
                     long long frame_stride, int nblobs, const float* blob_host,
                     const float* disp_host, float scale, unsigned long long seed, int poisson);
 
+/* ---- ingest: raw camera frames to the corrected frame buffer ----
+ * The step before the boundary: CApplyRefs::DoIt (MrcUtil/CApplyRefs.cpp:34-
+ * 212) with GApplyRefsToFrame (MrcUtil/GApplyRefsToFrame.cu:15-225) turns each
+ * raw frame into padded FP32, cropped to the frame size from the centre of
+ * the stack frame, v = (raw - dark) * gain.  Raw frames arrive in MRC modes
+ * 0 and 5 (unsigned byte), 1 (int16), 6 (uint16), 2 (float) and 101 (4-bit
+ * packed, rows of (sx + 1) / 2 bytes, even x in the low nibble). */
+typedef struct mc_raw_stack {
+    const void* data;     /* n raw frames, host or dev; host memory from
+                             mc_host_alloc makes the copy asynchronous */
+    int mode;             /* MRC mode */
+    int sx, sy, n;        /* stack frame size and frame count */
+} mc_raw_stack;
+/* Pinned host memory (the reference's cudaMallocHost staging,
+ * CBufferPool.cpp:159-165,204-218), e.g. to read frames from disk into. */
+int mc_host_alloc(mc_ctx* ctx, unsigned long long bytes, void** out);
+int mc_host_free(mc_ctx* ctx, void* ptr);
+/* Gain and dark references (host, ref_w x ref_h floats, either may be NULL)
+ * with -RotGain (0..3 x 90 degrees), -FlipGain (1 upside down, 2 left-right)
+ * and -InvGain as CLoadRefs::PostProcess (CLoadRefs.cpp:204-228); quirks = 1
+ * keeps its rule for the dark (rotated and flipped only when rot != 0 or
+ * flip == 0). */
+int mc_set_refs(mc_ctx* ctx, const float* gain, const float* dark, int ref_w, int ref_h, int rot,
+                int flip, int inv, int quirks);
+/* The references as post-processed (host outputs may be NULL). */
+int mc_get_refs(mc_ctx* ctx, float* gain, float* dark, int* ref_w, int* ref_h);
+/* Correct raw frames into `frames` (dev, W x H, rows of `pitch` floats,
+ * `frame_stride` floats apart); W <= sx, H <= sy.  async = 0 queues the work
+ * on the context stream; async = 1 queues it on a separate ingest stream and
+ * returns at once, so that the copy of the next movie overlaps the current
+ * alignment.  An alignment whose frames an asynchronous load writes waits
+ * for that load; mc_load_wait makes the context stream wait for all loads. */
+int mc_load_frames(mc_ctx* ctx, const mc_raw_stack* raw, float* frames, int W, int H,
+                   long long pitch, long long frame_stride, int async);
+int mc_load_wait(mc_ctx* ctx);
+/* MRC stack files (LibSrc/Mrcfile/Header/CLoadMainHeader.cpp:18-40,
+ * LibSrc/Mrcfile/Image/CLoadImage.cpp:45-90): header size, frame count,
+ * mode and data offset (1024 + extended header); frames [first, first+count)
+ * read into dst (host).  Byte-swapped files are swapped on reading. */
+int mc_mrc_info(const char* path, int* sx, int* sy, int* n, int* mode, long long* data_offset);
+int mc_mrc_read(const char* path, int first, int count, void* dst);
+
 /* ---- FFT tile autotuner (no reference counterpart: cuFFT plans its own
  * kernels, Util/CCufft2D.cpp:23-53) ----
  * The register-pass FFT kernels have several compiled tile variants per

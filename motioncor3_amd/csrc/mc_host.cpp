@@ -20,6 +20,7 @@
 #include "../../include/mc_abi.h"
 #include "mc_align.h"
 #include "mc_fft.h"
+#include "mc_ingest.h"
 #include "mc_prior.h"
 
 // A learned shift prior on the context's device (mc_prior_create).
@@ -61,6 +62,19 @@ struct mc_ctx {
     hipEvent_t evf = nullptr, evj = nullptr;
     // learned prior for guided patch alignment (not owned)
     mc_prior* prior = nullptr;
+    // ingest: stream and completion event of asynchronous frame loads, and
+    // the gain / dark references (device, ref_w x ref_h)
+    hipStream_t s_in = nullptr;
+    struct PendingLoad {
+        const char* lo;
+        const char* hi;  // output byte range of the load
+        hipEvent_t ev;   // recorded on s_in after the load
+    };
+    std::vector<PendingLoad> loads;
+    std::vector<hipEvent_t> free_ev;
+    float* gain = nullptr;
+    float* dark = nullptr;
+    int ref_w = 0, ref_h = 0;
 };
 
 namespace {
@@ -611,6 +625,31 @@ double kv_factor(int kv) {
 }
 
 // ---------------------------------------------------------------------------
+// Make the context stream wait for the asynchronous loads (mc_load_frames)
+// whose output overlaps [lo, hi), or for all of them when lo is null.
+int wait_loads(mc_ctx* c, const char* lo, const char* hi) {
+    std::vector<mc_ctx::PendingLoad> keep;
+    for (auto& l : c->loads) {
+        if (!lo || (l.lo < hi && lo < l.hi)) {
+            MC_TRY(hipStreamWaitEvent(c->stream, l.ev, 0));
+            c->free_ev.push_back(l.ev);
+        } else {
+            keep.push_back(l);
+        }
+    }
+    c->loads.swap(keep);
+    // events of finished work can go
+    std::vector<hipEvent_t> still;
+    for (hipEvent_t e : c->free_ev) {
+        if (hipEventQuery(e) == hipSuccess) (void)hipEventDestroy(e);
+        else still.push_back(e);
+    }
+    (void)hipGetLastError();
+    c->free_ev.swap(still);
+    return MC_OK;
+}
+
+// ---------------------------------------------------------------------------
 // `count` movies of one geometry.  The alignment iterations of all movies
 // run as one batch of jobs (the global alignment of movie m is job m, patch
 // q of movie m is job m * P + q); every job's arithmetic is the same as in a
@@ -642,6 +681,12 @@ int align_batch(mc_ctx* c, const mc_params* const* pp_, const mc_movie* const* m
         return fail(c, MC_E_ARG, "patch replay needs local shifts and centres");
     const int quirks = p->ref_quirks;
     hipStream_t s = c->stream;
+    // frames written by asynchronous loads: wait for exactly those loads
+    for (int m = 0; m < count; m++) {
+        const mc_movie* mv = mvs[m];
+        const char* lo = (const char*)mv->frames;
+        MC_CHECK(wait_loads(c, lo, lo + sizeof(float) * (size_t)(mv->frame_stride * (N - 1) + mv->pitch * (H - 1) + W)));
+    }
 
     // DataUtil/CFmIntParam.cpp:80-123 and CBufferPool.cpp:185-202
     const bool dw = !(p->kv < 80 || p->kv > 300) && p->pix_size > 0 && p->fm_dose > 0;
@@ -1178,6 +1223,14 @@ int mc_ctx_destroy(mc_ctx* c) {
     }
     if (c->evf) (void)hipEventDestroy(c->evf);
     if (c->evj) (void)hipEventDestroy(c->evj);
+    if (c->s_in) {
+        (void)hipStreamSynchronize(c->s_in);
+        (void)hipStreamDestroy(c->s_in);
+    }
+    for (auto& l : c->loads) (void)hipEventDestroy(l.ev);
+    for (hipEvent_t e : c->free_ev) (void)hipEventDestroy(e);
+    if (c->gain) (void)hipFree(c->gain);
+    if (c->dark) (void)hipFree(c->dark);
     if (c->own_stream) (void)hipStreamDestroy(c->stream);
     delete c;
     return MC_OK;
@@ -1768,4 +1821,261 @@ int mc_fft_variant(mc_ctx* c, int W, int H, int cols, int set, int* count, int* 
     if (count) *count = n;
     if (current) *current = cols ? p->col_var : p->row_var;
     return MC_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Ingest: raw frames to the corrected FP32 frame buffer (MrcUtil/CApplyRefs.cpp,
+// MrcUtil/GApplyRefsToFrame.cu), gain/dark post-processing (CLoadRefs.cpp:204-
+// 318) and a minimal MRC reader (LibSrc/Mrcfile header layout).
+namespace {
+
+// In-place 90-degree rotation by k (G90Rotate2D, MrcUtil/G90Rotate2D.cu:17-
+// 33,80-91): nearest source pixel of the rotated grid about the image centre.
+void ref_rotate(std::vector<float>& img, int& w, int& h, int k) {
+    if (k == 0) return;
+    const double ang = std::atan(1.0) * 4 / 180.0 * 90.0 * k;
+    const float fc = (float)std::cos(ang), fs = (float)std::sin(ang);
+    const int ic = fc > 0 ? (int)(fc + 0.1f) : (int)(fc - 0.1f);
+    const int is = fs > 0 ? (int)(fs + 0.1f) : (int)(fs - 0.1f);
+    const int rw = (k % 2 == 0) ? w : h, rh = (k % 2 == 0) ? h : w;
+    std::vector<float> out((size_t)rw * rh);
+    for (int y = 0; y < rh; y++)
+        for (int x = 0; x < rw; x++) {
+            const float nx = x - (rw - 1) * 0.5f, ny = y - (rh - 1) * 0.5f;
+            const float fx = nx * ic + ny * is, fy = -nx * is + ny * ic;
+            const int sx = (int)(fx + (w - 1) * 0.5f + 0.5f), sy = (int)(fy + (h - 1) * 0.5f + 0.5f);
+            out[(size_t)y * rw + x] = img[(size_t)sy * w + sx];
+        }
+    img.swap(out);
+    w = rw;
+    h = rh;
+}
+
+// 1: upside down (rows reversed), 2: left-right (GFlip2D, MrcUtil/GFlip2D.cu).
+void ref_flip(std::vector<float>& img, int w, int h, int f) {
+    if (f == 1)
+        for (int y = 0; y < h / 2; y++)
+            std::swap_ranges(img.begin() + (size_t)y * w, img.begin() + (size_t)(y + 1) * w,
+                             img.begin() + (size_t)(h - 1 - y) * w);
+    else if (f == 2)
+        for (int y = 0; y < h; y++) std::reverse(img.begin() + (size_t)y * w, img.begin() + (size_t)(y + 1) * w);
+}
+
+}  // namespace
+
+int mc_host_alloc(mc_ctx* c, unsigned long long bytes, void** out) {
+    if (!c || !out || bytes == 0) return MC_E_ARG;
+    *out = nullptr;
+    MC_TRY(hipSetDevice(c->device));
+    if (hipHostMalloc(out, bytes, hipHostMallocDefault) != hipSuccess) {
+        *out = nullptr;
+        return fail(c, MC_E_NOMEM, "pinned host allocation failed");
+    }
+    return MC_OK;
+}
+
+int mc_host_free(mc_ctx* c, void* p) {
+    if (!c) return MC_E_ARG;
+    if (p) MC_TRY(hipHostFree(p));
+    return MC_OK;
+}
+
+int mc_set_refs(mc_ctx* c, const float* gain, const float* dark, int rw, int rh, int rot, int flip,
+                int inv, int quirks) {
+    if (!c) return MC_E_ARG;
+    if ((gain || dark) && (rw < 1 || rh < 1)) return fail(c, MC_E_ARG, "reference size");
+    if (rot < 0 || rot > 3 || flip < 0 || flip > 2) return fail(c, MC_E_ARG, "-RotGain 0..3, -FlipGain 0..2");
+    MC_TRY(hipSetDevice(c->device));
+    if (c->s_in) MC_TRY(hipStreamSynchronize(c->s_in));
+    MC_TRY(hipStreamSynchronize(c->stream));
+    for (float** p : {&c->gain, &c->dark})
+        if (*p) {
+            MC_TRY(hipFree(*p));
+            *p = nullptr;
+        }
+    c->ref_w = c->ref_h = 0;
+    if (!gain && !dark) return MC_OK;
+    const size_t n = (size_t)rw * rh;
+    std::vector<float> g, d;
+    int gw = rw, gh = rh, dw = rw, dh = rh;
+    if (gain) g.assign(gain, gain + n);
+    if (dark) d.assign(dark, dark + n);
+    // CLoadRefs::PostProcess (CLoadRefs.cpp:204-228): nothing without a gain
+    // or without any operation; the gain is rotated, flipped, inverted; the
+    // dark is rotated and flipped when (rot != 0 || flip == 0) -- the
+    // reference's condition, kept with quirks -- or when any of the two is
+    // set otherwise
+    if (gain && (rot || flip || inv)) {
+        ref_rotate(g, gw, gh, rot);
+        ref_flip(g, gw, gh, flip);
+        if (inv)
+            for (float& v : g) v = (v == 0.f) ? 0.f : 1.0f / v;
+        if (dark && (quirks ? (rot != 0 || flip == 0) : (rot != 0 || flip != 0))) {
+            ref_rotate(d, dw, dh, rot);
+            ref_flip(d, dw, dh, flip);
+        }
+    }
+    if (gain && dark && (gw != dw || gh != dh)) return fail(c, MC_E_ARG, "gain and dark sizes differ");
+    const int fw = gain ? gw : dw, fh = gain ? gh : dh;
+    if (gain) {
+        MC_TRY(hipMalloc(&c->gain, sizeof(float) * n));
+        MC_TRY(hipMemcpy(c->gain, g.data(), sizeof(float) * n, hipMemcpyHostToDevice));
+    }
+    if (dark) {
+        MC_TRY(hipMalloc(&c->dark, sizeof(float) * n));
+        MC_TRY(hipMemcpy(c->dark, d.data(), sizeof(float) * n, hipMemcpyHostToDevice));
+    }
+    c->ref_w = fw;
+    c->ref_h = fh;
+    return MC_OK;
+}
+
+int mc_get_refs(mc_ctx* c, float* gain, float* dark, int* rw, int* rh) {
+    if (!c) return MC_E_ARG;
+    if (rw) *rw = c->ref_w;
+    if (rh) *rh = c->ref_h;
+    const size_t n = (size_t)c->ref_w * c->ref_h;
+    if (gain && c->gain) MC_TRY(hipMemcpy(gain, c->gain, sizeof(float) * n, hipMemcpyDeviceToHost));
+    if (dark && c->dark) MC_TRY(hipMemcpy(dark, c->dark, sizeof(float) * n, hipMemcpyDeviceToHost));
+    return MC_OK;
+}
+
+int mc_load_frames(mc_ctx* c, const mc_raw_stack* r, float* frames, int W, int H, long long pitch,
+                   long long frame_stride, int async) {
+    if (!c || !r || !r->data || !frames) return MC_E_ARG;
+    const long long rb = mc_raw_row_bytes(r->mode, r->sx);
+    if (rb < 0) return fail(c, MC_E_ARG, "MRC mode must be 0, 1, 2, 5, 6 or 101");
+    if (r->sx < 1 || r->sy < 1 || r->n < 1 || W < 1 || H < 1 || W > r->sx || H > r->sy ||
+        pitch < W || frame_stride < pitch * H)
+        return fail(c, MC_E_ARG, "invalid raw stack or frame geometry");
+    if ((c->gain || c->dark) && (c->ref_w != r->sx || c->ref_h != r->sy))
+        return fail(c, MC_E_ARG, "gain/dark reference size differs from the stack frame size");
+    MC_TRY(hipSetDevice(c->device));
+    if (async && !c->s_in) MC_TRY(hipStreamCreate(&c->s_in));
+    hipStream_t s = async ? c->s_in : c->stream;
+    if (async) {
+        // the load runs after everything already queued on the context stream
+        // (which may still read the destination frames)
+        hipEvent_t e = nullptr;
+        MC_TRY(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+        MC_TRY(hipEventRecord(e, c->stream));
+        MC_TRY(hipStreamWaitEvent(c->s_in, e, 0));
+        c->free_ev.push_back(e);
+    }
+    const long long fb = rb * r->sy, bytes = fb * r->n;
+    // device input is read in place; host input is copied by DMA (pinned
+    // host memory from mc_host_alloc makes the copy asynchronous)
+    const unsigned char* src = nullptr;
+    hipPointerAttribute_t at;
+    if (hipPointerGetAttributes(&at, r->data) == hipSuccess && at.type == hipMemoryTypeDevice) {
+        src = (const unsigned char*)r->data;
+    } else {
+        (void)hipGetLastError();  // a plain host pointer is not an error here
+        unsigned char* staging = (unsigned char*)buf(c, "raw_in", (size_t)bytes);
+        if (!staging) return fail(c, MC_E_NOMEM, "raw frame staging");
+        MC_TRY(hipMemcpyAsync(staging, r->data, (size_t)bytes, hipMemcpyHostToDevice, s));
+        src = staging;
+    }
+    IngestArgs a;
+    a.raw = src;
+    a.frame_bytes = fb;
+    a.row_bytes = rb;
+    a.sx = r->sx;
+    a.sy = r->sy;
+    a.n = r->n;
+    a.x0 = (r->sx - W) / 2;
+    a.y0 = (r->sy - H) / 2;
+    a.W = W;
+    a.H = H;
+    a.gain = c->gain;
+    a.dark = c->dark;
+    a.out = frames;
+    a.pitch = pitch;
+    a.frame_stride = frame_stride;
+    {
+        KTimer kt(c, "apply_refs", s);
+        MC_TRY(mc_launch_apply_refs(a, r->mode, s));
+    }
+    if (async) {
+        hipEvent_t e = nullptr;
+        MC_TRY(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+        MC_TRY(hipEventRecord(e, c->s_in));
+        const char* lo = (const char*)frames;
+        c->loads.push_back({lo, lo + sizeof(float) * (size_t)(frame_stride * (r->n - 1) + pitch * (H - 1) + W), e});
+    }
+    return MC_OK;
+}
+
+int mc_load_wait(mc_ctx* c) {
+    if (!c) return MC_E_ARG;
+    return wait_loads(c, nullptr, nullptr);
+}
+
+namespace {
+
+unsigned int bswap32(unsigned int v) {
+    return (v >> 24) | ((v >> 8) & 0xff00u) | ((v << 8) & 0xff0000u) | (v << 24);
+}
+
+// Header words 1-4 (nx ny nz mode) and 24 (nsymbt); a file written with the
+// other byte order shows a negative or oversized value there
+// (LibSrc/Mrcfile/Header/CLoadMainHeader.cpp:27-35).
+int mrc_header(const char* path, int* w, bool* swapped) {
+    FILE* f = fopen(path, "rb");
+    if (!f) return MC_E_ARG;
+    unsigned int hdr[256];
+    const size_t got = fread(hdr, 4, 256, f);
+    fclose(f);
+    if (got != 256) return MC_E_ARG;
+    int v[5] = {(int)hdr[0], (int)hdr[1], (int)hdr[2], (int)hdr[3], (int)hdr[23]};
+    const int lim = 32 * 1024;
+    bool sw = v[3] < 0 || v[0] < 0 || v[1] < 0 || v[2] < 0 || v[0] > lim || v[1] > lim || v[2] > lim;
+    if (sw)
+        for (int i = 0; i < 5; i++) v[i] = (int)bswap32((unsigned int)v[i]);
+    for (int i = 0; i < 5; i++) w[i] = v[i];
+    *swapped = sw;
+    if (w[0] < 1 || w[1] < 1 || w[2] < 1 || mc_raw_row_bytes(w[3], w[0]) < 0 || w[4] < 0) return MC_E_ARG;
+    return MC_OK;
+}
+
+}  // namespace
+
+int mc_mrc_info(const char* path, int* sx, int* sy, int* n, int* mode, long long* data_offset) {
+    if (!path) return MC_E_ARG;
+    int w[5];
+    bool sw;
+    if (mrc_header(path, w, &sw) != MC_OK) return MC_E_ARG;
+    if (sx) *sx = w[0];
+    if (sy) *sy = w[1];
+    if (n) *n = w[2];
+    if (mode) *mode = w[3];
+    if (data_offset) *data_offset = 1024LL + w[4];
+    return MC_OK;
+}
+
+int mc_mrc_read(const char* path, int first, int count, void* dst) {
+    int w[5];
+    bool sw;
+    if (!path || !dst || mrc_header(path, w, &sw) != MC_OK) return MC_E_ARG;
+    if (first < 0 || count < 1 || first + count > w[2]) return MC_E_ARG;
+    const long long fb = mc_raw_row_bytes(w[3], w[0]) * w[1];
+    const long long off = 1024LL + w[4];
+    FILE* f = fopen(path, "rb");
+    if (!f) return MC_E_ARG;
+    int rc = MC_OK;
+    const size_t want = (size_t)(fb * count);
+    if (fseeko(f, (off_t)(off + fb * first), SEEK_SET) != 0 || fread(dst, 1, want, f) != want) rc = MC_E_ARG;
+    fclose(f);
+    if (rc == MC_OK && sw) {
+        const int es = (w[3] == 1 || w[3] == 6) ? 2 : (w[3] == 2 ? 4 : 1);
+        unsigned char* b = (unsigned char*)dst;
+        if (es == 2)
+            for (size_t i = 0; i + 1 < want; i += 2) std::swap(b[i], b[i + 1]);
+        else if (es == 4)
+            for (size_t i = 0; i + 3 < want; i += 4) {
+                std::swap(b[i], b[i + 3]);
+                std::swap(b[i + 1], b[i + 2]);
+            }
+    }
+    return rc;
 }

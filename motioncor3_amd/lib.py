@@ -52,6 +52,11 @@ class McResult(C.Structure):
     ]
 
 
+class McRawStack(C.Structure):
+    _fields_ = [("data", C.c_void_p), ("mode", C.c_int), ("sx", C.c_int), ("sy", C.c_int),
+                ("n", C.c_int)]
+
+
 class McOverride(C.Structure):
     _fields_ = [
         ("replay_global", C.POINTER(C.c_float)), ("replay_patch", C.POINTER(C.c_float)),
@@ -71,6 +76,8 @@ ABI_SYMBOLS = (
     "mc_prior_create", "mc_prior_destroy", "mc_prior_dims", "mc_prior_params",
     "mc_prior_forward", "mc_prior_loss_grad", "mc_prior_train", "mc_set_prior", "mc_gemm_bf16",
     "mc_autotune", "mc_fft_variant",
+    "mc_host_alloc", "mc_host_free", "mc_set_refs", "mc_get_refs", "mc_load_frames", "mc_load_wait",
+    "mc_mrc_info", "mc_mrc_read",
 )
 
 
@@ -308,6 +315,52 @@ class Context:
             C.c_float(recipe.scale), C.c_ulonglong(recipe.seed if seed is None else seed),
             int(poisson)), "mc_synth_render")
 
+    def host_array(self, shape, dtype):
+        """Pinned host memory (mc_host_alloc) viewed as a numpy array."""
+        return HostArray(self, shape, dtype)
+
+    def set_refs(self, gain=None, dark=None, rot=0, flip=0, inv=0, quirks=True):
+        """mc_set_refs: gain / dark references (numpy [ref_h, ref_w]) with
+        -RotGain / -FlipGain / -InvGain post-processing."""
+        ref = gain if gain is not None else dark
+        g = None if gain is None else np.ascontiguousarray(gain, np.float32)
+        d = None if dark is None else np.ascontiguousarray(dark, np.float32)
+        rh, rw = (ref.shape if ref is not None else (0, 0))
+        self._check(self.lib.mc_set_refs(self.ptr, None if g is None else _fptr(g),
+                                         None if d is None else _fptr(d), rw, rh, rot, flip, inv,
+                                         int(quirks)), "mc_set_refs")
+
+    def get_refs(self):
+        w, h = C.c_int(), C.c_int()
+        self._check(self.lib.mc_get_refs(self.ptr, None, None, C.byref(w), C.byref(h)), "mc_get_refs")
+        g = np.zeros((h.value, w.value), np.float32)
+        d = np.zeros((h.value, w.value), np.float32)
+        self._check(self.lib.mc_get_refs(self.ptr, _fptr(g), _fptr(d), None, None), "mc_get_refs")
+        return g, d
+
+    def load_frames(self, raw, mode, sx, sy, n, frames, width=None, height=None, async_=False):
+        """mc_load_frames: `raw` (numpy, HostArray or DevArray) holds n frames
+        of an sx x sy stack in MRC `mode`; `frames` is an [N, H, pitch]
+        float32 DevArray receiving the centre-cropped corrected frames."""
+        N, H, pitch = frames.shape
+        W = pitch if width is None else width
+        Hh = H if height is None else height
+        if isinstance(raw, (DevArray, HostArray)):
+            ptr = raw.vp()
+        else:
+            raw = np.ascontiguousarray(raw)
+            ptr = raw.ctypes.data_as(C.c_void_p)
+        rs = McRawStack(ptr, mode, sx, sy, n)
+        self._check(self.lib.mc_load_frames(self.ptr, C.byref(rs), frames.vp(), W, Hh,
+                                            C.c_longlong(frames.stride(1)),
+                                            C.c_longlong(frames.stride(0)), int(async_)),
+                    "mc_load_frames")
+        if not async_:
+            self.sync()
+
+    def load_wait(self):
+        self._check(self.lib.mc_load_wait(self.ptr), "mc_load_wait")
+
     def autotune(self, W, H, N, patch=(0, 0), ft_bin=1.0):
         """mc_autotune: time the FFT tile variants of this geometry and keep
         the fastest.  Returns the report text."""
@@ -505,3 +558,69 @@ def gemm_bf16(ctx, a16, sa, b16, sb, c, m, n, k):
                                     b16.vp(), C.c_longlong(sb[0]), C.c_longlong(sb[1]), c.vp(),
                                     c.shape[1], m, n, k),
                "mc_gemm_bf16")
+
+
+class HostArray:
+    """Pinned host memory from mc_host_alloc with a numpy view (.a)."""
+
+    def __init__(self, ctx, shape, dtype):
+        self.ctx = ctx
+        self.shape = tuple(int(v) for v in shape)
+        self.dtype = np.dtype(dtype)
+        self.nbytes = int(np.prod(self.shape)) * self.dtype.itemsize
+        p = C.c_void_p()
+        ctx._check(ctx.lib.mc_host_alloc(ctx.ptr, C.c_ulonglong(self.nbytes), C.byref(p)),
+                   "mc_host_alloc")
+        self.ptr = p.value
+        buf = (C.c_char * self.nbytes).from_address(self.ptr)
+        self.a = np.frombuffer(buf, dtype=self.dtype).reshape(self.shape)
+
+    def vp(self):
+        return C.c_void_p(self.ptr)
+
+    def free(self):
+        if self.ptr and self.ctx.ptr:
+            self.a = None
+            self.ctx.lib.mc_host_free(self.ctx.ptr, C.c_void_p(self.ptr))
+        self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+def raw_frame_bytes(mode, sx, sy):
+    """Bytes of one raw frame in an MRC mode (4-bit: rows of (sx + 1) // 2)."""
+    per_row = {0: sx, 5: sx, 1: 2 * sx, 6: 2 * sx, 2: 4 * sx, 101: (sx + 1) // 2}[mode]
+    return per_row * sy
+
+
+def mrc_info(path):
+    """mc_mrc_info: dict(sx, sy, n, mode, data_offset)."""
+    v = [C.c_int() for _ in range(4)]
+    off = C.c_longlong()
+    rc = load().mc_mrc_info(path.encode(), *[C.byref(x) for x in v], C.byref(off))
+    if rc != MC_OK:
+        raise RuntimeError("mc_mrc_info failed for %s" % path)
+    return dict(sx=v[0].value, sy=v[1].value, n=v[2].value, mode=v[3].value,
+                data_offset=off.value)
+
+
+def mrc_read(path, first=0, count=None, out=None):
+    """mc_mrc_read: frames [first, first+count) as raw bytes into `out`
+    (numpy array or HostArray, allocated as uint8 when None)."""
+    info = mrc_info(path)
+    count = info["n"] - first if count is None else count
+    nb = raw_frame_bytes(info["mode"], info["sx"], info["sy"]) * count
+    if out is None:
+        out = np.empty(nb, np.uint8)
+    ptr = out.vp() if isinstance(out, HostArray) else out.ctypes.data_as(C.c_void_p)
+    size = out.nbytes
+    if size < nb:
+        raise ValueError("output buffer too small")
+    rc = load().mc_mrc_read(path.encode(), first, count, ptr)
+    if rc != MC_OK:
+        raise RuntimeError("mc_mrc_read failed for %s" % path)
+    return out

@@ -1,0 +1,135 @@
+"""Ingest on the GPU: raw frames to corrected FP32 frames against the oracle
+(oracle/ingest_oracle.py), bit for bit (one FP32 subtraction and one
+multiplication per pixel, as in the reference); reference post-processing;
+asynchronous loads overlapping alignments; an MRC file through the whole
+path."""
+import numpy as np
+import pytest
+
+import ingest_oracle as IO
+from motioncor3_amd import lib as L
+from motioncor3_amd.mrc import pack4, write_mrc
+from motioncor3_amd.synth import MovieRecipe
+
+pytestmark = pytest.mark.gpu
+
+_DT = {0: np.uint8, 5: np.uint8, 1: np.int16, 6: np.uint16, 2: np.float32}
+
+
+def _raw(rng, mode, n, sy, sx):
+    if mode == 101:
+        vals = rng.integers(0, 16, (n, sy, sx)).astype(np.uint8)
+        return pack4(vals), vals
+    if mode == 2:
+        v = rng.normal(3, 2, (n, sy, sx)).astype(np.float32)
+        return v, v
+    lo, hi = {0: (0, 255), 5: (0, 255), 1: (-3000, 3000), 6: (0, 65535)}[mode]
+    v = rng.integers(lo, hi + 1, (n, sy, sx)).astype(_DT[mode])
+    return v, v
+
+
+@pytest.mark.parametrize("mode", [0, 5, 1, 6, 2, 101])
+@pytest.mark.parametrize("refs", ["none", "gain", "dark", "both"])
+def test_apply_refs_matches_oracle(ctx, mode, refs):
+    rng = np.random.default_rng(mode * 7 + len(refs))
+    n, sy, sx = 3, 37, 301
+    W, H = 290, 30   # centre crop (odd offsets in x)
+    raw, vals = _raw(rng, mode, n, sy, sx)
+    gain = rng.uniform(0.5, 1.5, (sy, sx)).astype(np.float32) if refs in ("gain", "both") else None
+    dark = rng.uniform(0.0, 2.0, (sy, sx)).astype(np.float32) if refs in ("dark", "both") else None
+    ctx.set_refs(gain, dark)
+    try:
+        pitch = (W // 2 + 1) * 2
+        fr = ctx.empty((n, H, pitch), np.float32)
+        ctx.load_frames(raw, mode, sx, sy, n, fr, width=W)
+        got = fr.numpy()[:, :, :W]
+        want = IO.apply_refs(vals, gain, dark, W, H)
+        assert np.array_equal(got, want)
+    finally:
+        ctx.set_refs(None, None)
+
+
+@pytest.mark.parametrize("rot,flip,inv", [(1, 0, 0), (2, 1, 0), (3, 2, 1), (0, 1, 0), (0, 0, 1)])
+@pytest.mark.parametrize("quirks", [True, False])
+def test_reference_post_processing(ctx, rot, flip, inv, quirks):
+    rng = np.random.default_rng(rot * 9 + flip * 3 + inv)
+    g = rng.uniform(0.5, 1.5, (11, 14)).astype(np.float32)
+    g[2, 3] = 0.0
+    d = rng.uniform(0, 1, (11, 14)).astype(np.float32)
+    ctx.set_refs(g, d, rot=rot, flip=flip, inv=inv, quirks=quirks)
+    try:
+        gg, dd = ctx.get_refs()
+        wg, wd = IO.post_process(g, d, rot, flip, inv, quirks)
+        assert np.array_equal(gg, wg) and np.array_equal(dd, wd)
+    finally:
+        ctx.set_refs(None, None)
+
+
+def _same_alignment(a, b):
+    assert np.array_equal(a["global_shift"], b["global_shift"])
+    assert np.array_equal(a["patch_shift"], b["patch_shift"])
+    for i in range(a["num_sums"]):
+        assert np.array_equal(a["sums"][i].numpy(), b["sums"][i].numpy())
+
+
+def test_async_loads_overlap_alignments(ctx):
+    """Movie m+1 is loaded asynchronously (pinned host memory, ingest stream)
+    while movie m is aligned; every alignment must equal the one of frames
+    loaded synchronously."""
+    W = H = 1024
+    N = 8
+    pg = L.default_params(patch=(3, 3), pix_size=1.0, kv=300, fm_dose=1.0)
+    movies = [MovieRecipe(W, H, N, 400, seed=70 + k, dose=4.0, local_max=2.0).movie().astype(np.uint8)
+              for k in range(3)]
+    host = [ctx.host_array(m.shape, np.uint8) for m in movies]
+    for h, m in zip(host, movies):
+        h.a[...] = m
+    gain = np.random.default_rng(3).uniform(0.9, 1.1, (H, W)).astype(np.float32)
+    ctx.set_refs(gain, None)
+    try:
+        bufs = [ctx.empty((N, H, W), np.float32) for _ in range(2)]
+        ref = []
+        for m in range(3):
+            ctx.load_frames(host[m], 0, W, H, N, bufs[0])
+            ref.append(ctx.align_movie(bufs[0], pg))
+        ctx.load_frames(host[0], 0, W, H, N, bufs[0], async_=True)
+        got = []
+        for m in range(3):
+            if m + 1 < 3:
+                ctx.load_frames(host[m + 1], 0, W, H, N, bufs[(m + 1) % 2], async_=True)
+            got.append(ctx.align_movie(bufs[m % 2], pg))
+        for a, b in zip(got, ref):
+            _same_alignment(a, b)
+    finally:
+        ctx.set_refs(None, None)
+        for h in host:
+            h.free()
+
+
+def test_mrc_file_through_the_path(ctx, tmp_path):
+    """A 4-bit MRC stack read with mc_mrc_read into pinned memory, corrected
+    and aligned, equals aligning the oracle-corrected frames."""
+    W = H = 1024
+    N = 8
+    counts = MovieRecipe(W, H, N, 400, seed=81, dose=2.0, local_max=2.0).movie()
+    counts = np.minimum(counts, 15).astype(np.uint8)
+    path = str(tmp_path / "movie.mrc")
+    write_mrc(path, counts, 101, ext_bytes=1024)
+    info = L.mrc_info(path)
+    host = ctx.host_array((L.raw_frame_bytes(101, W, H) * N,), np.uint8)
+    try:
+        L.mrc_read(path, 0, N, out=host)
+        gain = np.random.default_rng(4).uniform(0.8, 1.2, (H, W)).astype(np.float32)
+        dark = np.random.default_rng(5).uniform(0.0, 0.1, (H, W)).astype(np.float32)
+        ctx.set_refs(gain, dark)
+        fr = ctx.empty((N, H, W), np.float32)
+        ctx.load_frames(host, info["mode"], info["sx"], info["sy"], info["n"], fr)
+        pg = L.default_params(patch=(3, 3), pix_size=1.0, kv=300, fm_dose=1.0)
+        a = ctx.align_movie(fr, pg)
+        want = IO.apply_refs(counts, gain, dark, W, H)
+        assert np.array_equal(fr.numpy(), want)
+        b = ctx.align_movie(ctx.upload(want), pg)
+        _same_alignment(a, b)
+    finally:
+        ctx.set_refs(None, None)
+        host.free()
