@@ -269,6 +269,30 @@ int mc_load_wait(mc_ctx* ctx);
 int mc_mrc_info(const char* path, int* sx, int* sy, int* n, int* mode, long long* data_offset);
 int mc_mrc_read(const char* path, int first, int count, void* dst);
 
+/* ---- output files after the alignment ----
+ * mc_save_sums writes every output sum of `result` as a float MRC image
+ * named like sGenOutputName (CSaveSerialCryoEM.cpp:13-32): out_mrc without
+ * ".mrc" / ".st", + "" / "_DW" / "_DWS" / "_ODD" / "_EVN"
+ * (DataUtil/CAlnSums.cpp:18-44) + ".mrc", with the main header of
+ * CSaveSerialCryoEM::mSaveImage (:208-240) and pixel size PixSize x FtBin.
+ * The header min / max / mean are those of the sum (a GPU reduction); the
+ * reference takes them from a device buffer that the sum is never copied
+ * into.  mc_save_aln / mc_load_aln write and read the -OutAln / -InAln text
+ * format (Align/CSaveAlign.cpp:84-151, Align/CLoadAlign.cpp:108-250): the
+ * setting block, N global shifts and, in patch mode, every patch's centre
+ * and local shifts ([frame][patch][xy], as mc_result), the input of
+ * mc_override's replay; mc_load_aln with NULL arrays returns only the sizes
+ * (n = stack frames minus throws).  mc_save_star writes the Relion star file
+ * of CGenStarFile (CGenStarFile.cpp:39-120), global shifts relative to the
+ * first frame. */
+int mc_save_sums(mc_ctx* ctx, const mc_params* p, const mc_result* result, const char* out_mrc);
+int mc_save_aln(const char* path, const mc_params* p, int sx, int sy, int n_stack, int N,
+                const float* global_shift, const float* patch_shift, const float* patch_center);
+int mc_load_aln(const char* path, int* sx, int* sy, int* n, int* px, int* py, float* global_shift,
+                float* patch_shift, float* patch_center);
+int mc_save_star(const char* path, const mc_params* p, int sx, int sy, int n_stack,
+                 const char* gain_name, const float* global_shift, int N);
+
 /* ---- FFT tile autotuner (no reference counterpart: cuFFT plans its own
  * kernels, Util/CCufft2D.cpp:23-53) ----
  * The register-pass FFT kernels have several compiled tile variants per

@@ -21,6 +21,7 @@
 #include "mc_align.h"
 #include "mc_fft.h"
 #include "mc_ingest.h"
+#include "mc_output.h"
 #include "mc_prior.h"
 
 // A learned shift prior on the context's device (mc_prior_create).
@@ -2078,4 +2079,81 @@ int mc_mrc_read(const char* path, int first, int count, void* dst) {
             }
     }
     return rc;
+}
+
+// ---------------------------------------------------------------------------
+// Output files (CSaveSerialCryoEM.cpp:126-240, Align/CSaveAlign.cpp:84-151,
+// Align/CLoadAlign.cpp:108-250, CGenStarFile.cpp:39-120).
+int mc_save_sums(mc_ctx* c, const mc_params* p, const mc_result* r, const char* out_mrc) {
+    if (!c || !p || !r || !out_mrc || !out_mrc[0]) return c ? fail(c, MC_E_ARG, "save sums: arguments") : MC_E_ARG;
+    // CAlnSums::Setup (DataUtil/CAlnSums.cpp:18-44) file suffixes, in the
+    // order of the sums
+    const bool dw = !(p->kv < 80 || p->kv > 300) && p->pix_size > 0 && p->fm_dose > 0;
+    const bool dws = dw && !(p->sum_range[0] < 0) && !(p->sum_range[1] <= 0) &&
+                     (p->sum_range[1] - p->sum_range[0] > 1.0f);
+    std::vector<const char*> sfx = {""};
+    if (dw) sfx.push_back("_DW");
+    if (dws) sfx.push_back("_DWS");
+    if (p->split_sum) {
+        sfx.push_back("_ODD");
+        sfx.push_back("_EVN");
+    }
+    if ((int)sfx.size() != r->num_sums) return fail(c, MC_E_ARG, "save sums: result does not match the flags");
+    const int w = r->out_w, h = r->out_h;
+    MC_BUF(pmin, float, "st_min", MC_STATS_BLOCKS);
+    MC_BUF(pmax, float, "st_max", MC_STATS_BLOCKS);
+    MC_BUF(psum, double, "st_sum", MC_STATS_BLOCKS);
+    std::vector<float> hmin(MC_STATS_BLOCKS), hmax(MC_STATS_BLOCKS), img((size_t)w * h);
+    std::vector<double> hsum(MC_STATS_BLOCKS);
+    // GetFinalPixelSize (CInput.cpp:709-715) with unit magnification
+    const float pix = p->pix_size * p->ft_bin;
+    for (int i = 0; i < r->num_sums; i++) {
+        if (!r->sums[i]) return fail(c, MC_E_ARG, "save sums: missing sum buffer");
+        MC_TRY(mc_launch_image_stats(r->sums[i], w, h, w, pmin, pmax, psum, c->stream));
+        MC_TRY(hipMemcpyAsync(hmin.data(), pmin, sizeof(float) * MC_STATS_BLOCKS, hipMemcpyDeviceToHost, c->stream));
+        MC_TRY(hipMemcpyAsync(hmax.data(), pmax, sizeof(float) * MC_STATS_BLOCKS, hipMemcpyDeviceToHost, c->stream));
+        MC_TRY(hipMemcpyAsync(hsum.data(), psum, sizeof(double) * MC_STATS_BLOCKS, hipMemcpyDeviceToHost, c->stream));
+        MC_TRY(hipMemcpyAsync(img.data(), r->sums[i], sizeof(float) * img.size(), hipMemcpyDeviceToHost, c->stream));
+        MC_TRY(hipStreamSynchronize(c->stream));
+        float lo = hmin[0], hi = hmax[0];
+        double tot = 0.0;
+        for (int b = 0; b < MC_STATS_BLOCKS; b++) {
+            lo = std::min(lo, hmin[b]);
+            hi = std::max(hi, hmax[b]);
+            tot += hsum[b];
+        }
+        const std::string path = mc_output_name(out_mrc, sfx[i], ".mrc");
+        if (mc_write_mrc_float(path.c_str(), img.data(), w, h, pix, lo, hi, (float)(tot / ((double)w * h))) != 0)
+            return fail(c, MC_E_ARG, "cannot write " + path);
+    }
+    return MC_OK;
+}
+
+int mc_save_aln(const char* path, const mc_params* p, int sx, int sy, int n_stack, int N,
+                const float* global_shift, const float* patch_shift, const float* patch_center) {
+    if (!path || !p || !global_shift || N < 1) return MC_E_ARG;
+    const bool patch = p->patch[0] > 1 && p->patch[1] > 1;
+    const int P = patch ? p->patch[0] * p->patch[1] : 0;
+    if (P && (!patch_shift || !patch_center)) return MC_E_ARG;
+    return mc_write_aln(path, sx, sy, n_stack, p->patch[0], p->patch[1], p->throw0,
+                        n_stack - N - p->throw0 > 0 ? n_stack - N - p->throw0 : 0, N, P, global_shift,
+                        patch_shift, patch_center) == 0
+               ? MC_OK
+               : MC_E_ARG;
+}
+
+int mc_load_aln(const char* path, int* sx, int* sy, int* n, int* px, int* py, float* global_shift,
+                float* patch_shift, float* patch_center) {
+    if (!path) return MC_E_ARG;
+    return mc_read_aln(path, sx, sy, n, px, py, global_shift, patch_shift, patch_center) == 0 ? MC_OK
+                                                                                             : MC_E_ARG;
+}
+
+int mc_save_star(const char* path, const mc_params* p, int sx, int sy, int n_stack,
+                 const char* gain_name, const float* global_shift, int N) {
+    if (!path || !p || !global_shift || N < 1) return MC_E_ARG;
+    return mc_write_star(path, p->movie_name, gain_name, sx, sy, n_stack, p->pix_size, p->ft_bin,
+                         p->fm_dose, p->kv, p->throw0, global_shift, N) == 0
+               ? MC_OK
+               : MC_E_ARG;
 }

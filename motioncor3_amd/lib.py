@@ -78,6 +78,7 @@ ABI_SYMBOLS = (
     "mc_autotune", "mc_fft_variant",
     "mc_host_alloc", "mc_host_free", "mc_set_refs", "mc_get_refs", "mc_load_frames", "mc_load_wait",
     "mc_mrc_info", "mc_mrc_read",
+    "mc_save_sums", "mc_save_aln", "mc_load_aln", "mc_save_star",
 )
 
 
@@ -361,6 +362,16 @@ class Context:
     def load_wait(self):
         self._check(self.lib.mc_load_wait(self.ptr), "mc_load_wait")
 
+    def save_sums(self, params, result, out_mrc):
+        """mc_save_sums: the result's output sums as float MRC files."""
+        r = McResult()
+        for i, sm in enumerate(result["sums"]):
+            r.sums[i] = sm.vp()
+        r.num_sums = result["num_sums"]
+        r.out_h, r.out_w = result["sums"][0].shape
+        self._check(self.lib.mc_save_sums(self.ptr, C.byref(params), C.byref(r), out_mrc.encode()),
+                    "mc_save_sums")
+
     def autotune(self, W, H, N, patch=(0, 0), ft_bin=1.0):
         """mc_autotune: time the FFT tile variants of this geometry and keep
         the fastest.  Returns the report text."""
@@ -624,3 +635,43 @@ def mrc_read(path, first=0, count=None, out=None):
     if rc != MC_OK:
         raise RuntimeError("mc_mrc_read failed for %s" % path)
     return out
+
+
+def save_aln(path, params, sx, sy, n_stack, global_shift, patch_shift=None, patch_center=None):
+    """mc_save_aln (-OutAln text format)."""
+    g = np.ascontiguousarray(global_shift, np.float32)
+    ps = None if patch_shift is None else np.ascontiguousarray(patch_shift, np.float32)
+    pc = None if patch_center is None else np.ascontiguousarray(patch_center, np.float32)
+    rc = load().mc_save_aln(path.encode(), C.byref(params), sx, sy, n_stack, g.shape[0], _fptr(g),
+                            None if ps is None else _fptr(ps), None if pc is None else _fptr(pc))
+    if rc != MC_OK:
+        raise RuntimeError("mc_save_aln failed for %s" % path)
+
+
+def load_aln(path):
+    """mc_load_aln: dict(sx, sy, n, patch, global_shift, patch_shift, patch_center)."""
+    v = [C.c_int() for _ in range(5)]
+    lib = load()
+    if lib.mc_load_aln(path.encode(), *[C.byref(x) for x in v], None, None, None) != MC_OK:
+        raise RuntimeError("mc_load_aln failed for %s" % path)
+    sx, sy, n, px, py = [x.value for x in v]
+    P = px * py if (px > 1 and py > 1) else 0
+    g = np.zeros((n, 2), np.float32)
+    ps = np.zeros((n, max(P, 1), 2), np.float32)
+    pc = np.zeros((max(P, 1), 2), np.float32)
+    if lib.mc_load_aln(path.encode(), None, None, None, None, None, _fptr(g), _fptr(ps),
+                       _fptr(pc)) != MC_OK:
+        raise RuntimeError("mc_load_aln failed for %s" % path)
+    out = dict(sx=sx, sy=sy, n=n, patch=(px, py), global_shift=g)
+    if P:
+        out.update(patch_shift=ps, patch_center=pc)
+    return out
+
+
+def save_star(path, params, sx, sy, n_stack, global_shift, gain_name=None):
+    """mc_save_star (Relion global-shift star file)."""
+    g = np.ascontiguousarray(global_shift, np.float32)
+    rc = load().mc_save_star(path.encode(), C.byref(params), sx, sy, n_stack,
+                             gain_name.encode() if gain_name else None, _fptr(g), g.shape[0])
+    if rc != MC_OK:
+        raise RuntimeError("mc_save_star failed for %s" % path)

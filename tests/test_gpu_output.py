@@ -1,0 +1,68 @@
+"""Output files on the GPU path: MRC sums with header statistics, and an
+.aln file written after an alignment and replayed (-InAln) against the
+oracle's replay of the same file."""
+import os
+
+import numpy as np
+import pytest
+
+import oracle as O
+from motioncor3_amd import lib as L
+from motioncor3_amd.synth import MovieRecipe
+
+pytestmark = pytest.mark.gpu
+
+
+def _read_mrc(path):
+    raw = open(path, "rb").read()
+    hi = np.frombuffer(raw[:1024], np.int32)
+    hf = np.frombuffer(raw[:1024], np.float32)
+    nx, ny, nz, mode = hi[:4]
+    data = np.frombuffer(raw[1024 + hi[23]:], np.float32).reshape(nz, ny, nx)
+    return hi, hf, data
+
+
+def test_save_sums_mrc(ctx, tmp_path):
+    W, H, N = 512, 512, 10
+    mv = ctx.upload(MovieRecipe(W, H, N, 200, seed=91, dose=4.0).movie())
+    pg = L.default_params(pix_size=1.1, kv=300, fm_dose=1.0, split_sum=1, ft_bin=1.0)
+    res = ctx.align_movie(mv, pg)
+    assert res["num_sums"] == 5
+    out = str(tmp_path / "corr.MRC")
+    ctx.save_sums(pg, res, out)
+    names = ["corr", "corr_DW", "corr_DWS", "corr_ODD", "corr_EVN"]
+    for i, nm in enumerate(names):
+        path = str(tmp_path / (nm + ".mrc"))
+        assert os.path.exists(path), nm
+        hi, hf, data = _read_mrc(path)
+        want = res["sums"][i].numpy()
+        assert tuple(hi[:4]) == (W, H, 1, 2) and tuple(hi[7:10]) == (W, H, 1)
+        assert np.allclose(hf[10:13], [W * np.float32(1.1), H * np.float32(1.1), np.float32(1.1)])
+        assert tuple(hi[16:19]) == (1, 2, 3)
+        assert np.array_equal(data[0], want)
+        assert hf[19] == want.min() and hf[20] == want.max()
+        assert abs(hf[21] - want.astype(np.float64).mean()) <= 1e-6 * abs(want.mean())
+
+
+def test_aln_replay_matches_oracle(ctx, tmp_path):
+    """-OutAln then -InAln: the GPU path replays the loaded shifts, and the
+    sums agree with the oracle replaying the same loaded values."""
+    W, H, N = 1024, 1024, 8
+    mvh = MovieRecipe(W, H, N, 400, seed=93, dose=4.0, local_max=2.0).movie()
+    mv = ctx.upload(mvh)
+    kw = dict(patch=(3, 3), pix_size=1.0, kv=300, fm_dose=1.0)
+    pg = L.default_params(**kw)
+    res = ctx.align_movie(mv, pg)
+    path = str(tmp_path / "movie.aln")
+    L.save_aln(path, pg, W, H, N, res["global_shift"], res["patch_shift"], res["patch_center"])
+    a = L.load_aln(path)
+    assert a["n"] == N and a["patch"] == (3, 3)
+    g = ctx.align_movie(mv, pg, replay_global=a["global_shift"], replay_patch=a["patch_shift"],
+                        replay_centers=a["patch_center"])
+    assert np.array_equal(g["global_shift"], a["global_shift"])
+    po = O.default_params(**kw)
+    o = O.align_movie(mvh, po, replay_global=a["global_shift"], replay_patch=a["patch_shift"],
+                      replay_centers=a["patch_center"])
+    for i in range(g["num_sums"]):
+        s, r = g["sums"][i].numpy(), o["sums"][i]
+        assert np.linalg.norm(s - r) / np.linalg.norm(r) < 1e-4, i
