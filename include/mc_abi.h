@@ -79,6 +79,13 @@ typedef struct mc_params {
     float  prior_fallback;/* full-resolution px (0.2): a patch whose largest
                              group shift in the guided second iteration exceeds
                              this restarts from zero shifts (plain solve) */
+    const int* fm_int_start; /* -FmIntFile: raw start of each of the N frames
+                                (NULL: i + throw0), from mc_fm_int_table */
+    const int* fm_int_size;  /* raw frames integrated into each frame (NULL: 1);
+                                frame centres, grouping by raw-frame count and
+                                the accumulated dose FmDose x (start + size)
+                                follow DataUtil/CFmIntParam.cpp:138-287 and
+                                DataUtil/CFmGroupParam.cpp:26-97 */
 } mc_params;
 
 /* N frames already gain/dark/defect corrected, resident in HBM
@@ -239,6 +246,14 @@ typedef struct mc_raw_stack {
                              mc_host_alloc makes the copy asynchronous */
     int mode;             /* MRC mode */
     int sx, sy, n;        /* stack frame size and frame count */
+    int n_int;            /* frame integration (-FmIntFile, modes 0, 5, 101):
+                             0 = none, else output frame k is the sum of raw
+                             frames int_start[k] .. + int_size[k] - 1 (host
+                             arrays), added as unsigned bytes as
+                             MrcUtil/CLoadCryoEMStack.cpp:184-259 does
+                             (Util/GAddFrames.cu:32-41, wrapping at 256) */
+    const int* int_start;
+    const int* int_size;
 } mc_raw_stack;
 /* Pinned host memory (the reference's cudaMallocHost staging,
  * CBufferPool.cpp:159-165,204-218), e.g. to read frames from disk into. */
@@ -267,6 +282,15 @@ int mc_load_wait(mc_ctx* ctx);
  * mode and data offset (1024 + extended header); frames [first, first+count)
  * read into dst (host).  Byte-swapped files are swapped on reading. */
 int mc_mrc_info(const char* path, int* sx, int* sy, int* n, int* mode, long long* data_offset);
+/* Frame-integration table of a movie of n_raw raw frames:
+ * CReadFmIntFile::DoIt (DataUtil/CReadFmIntFile.cpp:110-173) reads path
+ * (NULL or "": no file), CFmIntParam::mSetup / mCalcIntFms / mThrowIntFrames
+ * (DataUtil/CFmIntParam.cpp:118-262) give the n_out frames' raw start, size
+ * and accumulated dose (arrays of cap entries, or NULL to query n_out) and
+ * the raw-frame dose; integrated = 1 when the file applies. */
+int mc_fm_int_table(const char* path, int n_raw, float fm_dose, int throw0, int throw1, int cap,
+                    int* n_out, int* start, int* size, float* acc_dose, float* raw_dose,
+                    int* integrated);
 int mc_mrc_read(const char* path, int first, int count, void* dst);
 
 /* ---- output files after the alignment ----

@@ -1052,6 +1052,7 @@ struct DwArgs {
     int split;
     cfloat* sums;
     long long sstride;
+    const int* cum;  // raw frames counted into frame i's dose (null: i + 1)
 };
 
 __device__ __forceinline__ double crit_q(int kx, int y, int cx, int H, double pix, double kvf,
@@ -1074,24 +1075,31 @@ __global__ __launch_bounds__(256) void k_dw_sums(DwArgs a) {
     if (a.dw && !dc) {
         qd = crit_q(kx, y, a.cx, a.H, a.pix, a.kvf, a.dose);
         double w = 1.0, s = 0.0;
+        int prev = 0;
         for (int i = 0; i < a.N; i++) {
-            w *= qd;
+            const int c = a.cum ? a.cum[i] : i + 1;
+            for (int k = prev; k < c; k++) w *= qd;
+            prev = c;
             s += w * w;
         }
         inv_ws = 1.0 / sqrt(s);
     }
     cfloat s0 = cmk(0.f, 0.f), s1 = s0, s2 = s0, so = s0, se = s0;
     double w = 1.0;
+    int prev = 0;
     for (int i = 0; i < a.N; i++) {
         cfloat v = a.F[(long long)i * a.fstride + q];
         if (a.A) v = cmul(v, cmul(a.A[(long long)i * a.cx + kx], a.B[(long long)i * a.H + y]));
         s0 = cadd(s0, v);
         if (a.dw) {
-            w *= qd;
+            // w_i = exp(-0.5 D_i / Dc) with D_i = dose x c_i, c_i raw frames
+            const int c = a.cum ? a.cum[i] : i + 1;
+            for (int k = prev; k < c; k++) w *= qd;
+            prev = c;
             if (!dc) v = cscale(v, (float)(w * inv_ws));
             s1 = cadd(s1, v);
             if (a.dws) {
-                const double d = a.dose * (i + 1);
+                const double d = a.dose * c;
                 if (!(d < a.r0) && !(d > a.r1)) s2 = cadd(s2, v);
             }
         }
@@ -1388,7 +1396,7 @@ hipError_t mc_launch_local_warp(const float* R, float* Rp, long long pitch, long
 hipError_t mc_launch_dw_sums(const cfloat* F, long long fstride, int N, int cx, int H,
                              const cfloat* A, const cfloat* B, int dw, double pix, double kvf,
                              double dose, int dws, double r0, double r1, int split, cfloat* sums,
-                             long long sstride, hipStream_t s) {
+                             long long sstride, hipStream_t s, const int* cum) {
     DwArgs a;
     a.F = F;
     a.fstride = fstride;
@@ -1407,6 +1415,7 @@ hipError_t mc_launch_dw_sums(const cfloat* F, long long fstride, int N, int cx, 
     a.split = split;
     a.sums = sums;
     a.sstride = sstride;
+    a.cum = cum;
     dim3 grid(mc_div_up(H, 256), cx);
     hipLaunchKernelGGL(k_dw_sums, grid, dim3(256), 0, s, a);
     return hipGetLastError();

@@ -73,6 +73,8 @@ struct mc_ctx {
     };
     std::vector<PendingLoad> loads;
     std::vector<hipEvent_t> free_ev;
+    // host integration tables of queued loads, kept until their copy ran
+    std::vector<std::pair<hipEvent_t, std::vector<int>>> tab_keep;
     float* gain = nullptr;
     float* dark = nullptr;
     int ref_w = 0, ref_h = 0;
@@ -226,22 +228,33 @@ struct Groups {
     std::vector<double> gcent, fcent;
 };
 
-Groups make_groups(int binz, int N, int throw0) {
+// Frames are integrated frames: frame i starts at raw frame fst[i] and holds
+// fsz[i] raw frames (null: fst[i] = i + throw0, fsz[i] = 1).  Frame centres
+// (DataUtil/CFmIntParam.cpp:281-287), groups of at most binz x (smallest
+// frame) raw frames (DataUtil/CFmGroupParam.cpp:26-97).
+Groups make_groups(int binz, int N, int throw0, const int* fst = nullptr, const int* fsz = nullptr) {
     Groups g;
     g.binz = binz;
     g.fgroup.assign(N, 0);
-    for (int i = 0; i < N; i++) g.fcent.push_back((double)(i + throw0));
+    int minraw = 1 << 30;
+    for (int i = 0; i < N; i++) {
+        const int sz = fsz ? fsz[i] : 1;
+        const int st = fst ? fst[i] : i + throw0;
+        g.fcent.push_back((double)((float)st + 0.5f * ((float)sz - 1.0f)));
+        minraw = std::min(minraw, sz);
+    }
+    const int maxraw = binz * minraw;
     int f = 0;
     for (int i = 0; i < N; i++) {
         g.start.push_back(f);
         int cnt = 0, raw = 0;
         while (true) {
-            raw += 1;
+            raw += fsz ? fsz[f] : 1;
             cnt += 1;
             g.fgroup[f] = g.G;
             f += 1;
             if (f >= N) break;
-            if (raw >= binz) break;
+            if (raw >= maxraw) break;
         }
         g.size.push_back(cnt);
         g.G += 1;
@@ -817,7 +830,7 @@ int align_batch(mc_ctx* c, const mc_params* const* pp_, const mc_movie* const* m
     const auto ta = clk::now();
 
     if (!replay) {
-        Groups g0 = make_groups(p->group[0], N, p->throw0);
+        Groups g0 = make_groups(p->group[0], N, p->throw0, p->fm_int_start, p->fm_int_size);
         IterOut go;
         MC_CHECK(run_iterative(c, "g_", Sx_all, count, N, g.xw, g.xh, px, g0, p->bft[0],
                                p->phase_only, p->iter, tol, quirks, &go, fx, ix,
@@ -886,7 +899,7 @@ int align_batch(mc_ctx* c, const mc_params* const* pp_, const mc_movie* const* m
             MC_TRY(mc_cols_fwd(*pp, Tp, pp->t_elems, spec_at(Sp_all, pm * pfs, fpch), pfs, nullptr, 0,
                                0, 0, P * N, s, fpch, gp ? gp + pm : nullptr));
           }
-            Groups g1 = make_groups(p->group[1], N, p->throw0);
+            Groups g1 = make_groups(p->group[1], N, p->throw0, p->fm_int_start, p->fm_int_size);
             // Learned prior: features of every patch after its first iteration
             // (training data, when a result asks for them) and/or iterations
             // guided by the attached network.
@@ -1020,6 +1033,17 @@ int align_batch(mc_ctx* c, const mc_params* const* pp_, const mc_movie* const* m
         MC_BUF(fA, cfloat, "fphA", (size_t)N * g.cx);
         MC_BUF(fB, cfloat, "fphB", (size_t)N * H);
         const double kvf = kv_factor(p->kv);
+        // integrated frames: raw frames counted into each frame's dose
+        int* dcum = nullptr;
+        if (p->fm_int_size) {
+            std::vector<int> cum(N);
+            for (int i = 0; i < N; i++)
+                cum[i] = (p->fm_int_start ? p->fm_int_start[i] : i + p->throw0) + p->fm_int_size[i];
+            dcum = (int*)buf(c, "dcum", sizeof(int) * N);
+            if (!dcum) return fail(c, MC_E_NOMEM, "dose table");
+            MC_TRY(hipMemcpyAsync(dcum, cum.data(), sizeof(int) * N, hipMemcpyHostToDevice, s));
+            MC_TRY(hipStreamSynchronize(s));
+        }
         if (patch) {
             // Align/CPatchAlign.cpp:47-65: shifts relative to the reference frame
             const int ref = (p->fm_ref < 0) ? N / 2 : std::min(std::max(p->fm_ref - 1, 0), N - 1);
@@ -1084,14 +1108,14 @@ int align_batch(mc_ctx* c, const mc_params* const* pp_, const mc_movie* const* m
             KTimer kt(c, "dw_sums");
             MC_TRY(mc_launch_dw_sums(Sf, fs, N, g.cx, H, nullptr, nullptr, dw, p->pix_size, kvf,
                                      p->fm_dose, dws, p->sum_range[0], p->sum_range[1], p->split_sum,
-                                     sums, fs, s));
+                                     sums, fs, s, dcum));
         } else {
             // Correct/CCorrectFullShift.cpp:229-309 with the absolute shifts
             MC_TRY(hipMemcpyAsync(gsh, gshift.data(), sizeof(double) * 2 * N, hipMemcpyHostToDevice, s));
             MC_TRY(mc_launch_phasors(gsh, -1.0, 1, nullptr, N, W, g.cx, H, N, fA, fB, s));
             KTimer kt(c, "dw_sums");
             MC_TRY(mc_launch_dw_sums(Sf, fs, N, g.cx, H, fA, fB, dw, p->pix_size, kvf, p->fm_dose, dws,
-                                     p->sum_range[0], p->sum_range[1], p->split_sum, sums, fs, s));
+                                     p->sum_range[0], p->sum_range[1], p->split_sum, sums, fs, s, dcum));
         }
         // Correct/CCorrectFullShift.cpp:146-185: crop to -FtBin size, C2R, >= 0
         MC_BUF(To, cfloat, "To", (size_t)po->t_elems);
@@ -1230,6 +1254,7 @@ int mc_ctx_destroy(mc_ctx* c) {
     }
     for (auto& l : c->loads) (void)hipEventDestroy(l.ev);
     for (hipEvent_t e : c->free_ev) (void)hipEventDestroy(e);
+    for (auto& t : c->tab_keep) (void)hipEventDestroy(t.first);
     if (c->gain) (void)hipFree(c->gain);
     if (c->dark) (void)hipFree(c->dark);
     if (c->own_stream) (void)hipStreamDestroy(c->stream);
@@ -1312,6 +1337,7 @@ static bool same_alignment_flags(const mc_params& a, const mc_params& b) {
            a.throw0 == b.throw0 && a.ref_quirks == b.ref_quirks &&
            a.spectra_fmt == b.spectra_fmt && a.launch_graphs == b.launch_graphs &&
            a.chunk_streams == b.chunk_streams && a.prior == b.prior &&
+           a.fm_int_start == b.fm_int_start && a.fm_int_size == b.fm_int_size &&
            a.prior_fallback == b.prior_fallback && a.verbose == b.verbose;
 }
 
@@ -1964,6 +1990,36 @@ int mc_load_frames(mc_ctx* c, const mc_raw_stack* r, float* frames, int W, int H
         c->free_ev.push_back(e);
     }
     const long long fb = rb * r->sy, bytes = fb * r->n;
+    const bool integ = r->n_int > 0;
+    if (integ && (!r->int_start || !r->int_size || !(r->mode == 0 || r->mode == 5 || r->mode == 101)))
+        return fail(c, MC_E_ARG, "frame integration needs modes 0, 5 or 101 and start/size tables");
+    int* dtab = nullptr;
+    if (integ) {
+        for (int k = 0; k < r->n_int; k++)
+            if (r->int_size[k] < 1 || r->int_start[k] < 0 || r->int_start[k] + r->int_size[k] > r->n)
+                return fail(c, MC_E_ARG, "frame integration table exceeds the raw frames");
+        dtab = (int*)buf(c, async ? "int_tab_a" : "int_tab", sizeof(int) * 2 * (size_t)r->n_int);
+        if (!dtab) return fail(c, MC_E_NOMEM, "integration table");
+        // retire host tables whose copies have run
+        std::vector<std::pair<hipEvent_t, std::vector<int>>> keep;
+        for (auto& t : c->tab_keep) {
+            if (hipEventQuery(t.first) == hipSuccess) (void)hipEventDestroy(t.first);
+            else keep.push_back(std::move(t));
+        }
+        (void)hipGetLastError();
+        c->tab_keep.swap(keep);
+        std::vector<int> tab(2 * (size_t)r->n_int);
+        for (int k = 0; k < r->n_int; k++) {
+            tab[2 * k] = r->int_start[k];
+            tab[2 * k + 1] = r->int_size[k];
+        }
+        hipEvent_t te = nullptr;
+        MC_TRY(hipEventCreateWithFlags(&te, hipEventDisableTiming));
+        c->tab_keep.emplace_back(te, std::move(tab));
+        const std::vector<int>& kt = c->tab_keep.back().second;
+        MC_TRY(hipMemcpyAsync(dtab, kt.data(), sizeof(int) * kt.size(), hipMemcpyHostToDevice, s));
+        MC_TRY(hipEventRecord(te, s));
+    }
     // device input is read in place; host input is copied by DMA (pinned
     // host memory from mc_host_alloc makes the copy asynchronous)
     const unsigned char* src = nullptr;
@@ -1979,6 +2035,8 @@ int mc_load_frames(mc_ctx* c, const mc_raw_stack* r, float* frames, int W, int H
     }
     IngestArgs a;
     a.raw = src;
+    a.tab = dtab;
+    a.n_out = integ ? r->n_int : r->n;
     a.frame_bytes = fb;
     a.row_bytes = rb;
     a.sx = r->sx;
@@ -2002,7 +2060,8 @@ int mc_load_frames(mc_ctx* c, const mc_raw_stack* r, float* frames, int W, int H
         MC_TRY(hipEventCreateWithFlags(&e, hipEventDisableTiming));
         MC_TRY(hipEventRecord(e, c->s_in));
         const char* lo = (const char*)frames;
-        c->loads.push_back({lo, lo + sizeof(float) * (size_t)(frame_stride * (r->n - 1) + pitch * (H - 1) + W), e});
+        const int nout = integ ? r->n_int : r->n;
+        c->loads.push_back({lo, lo + sizeof(float) * (size_t)(frame_stride * (nout - 1) + pitch * (H - 1) + W), e});
     }
     return MC_OK;
 }
@@ -2156,4 +2215,127 @@ int mc_save_star(const char* path, const mc_params* p, int sx, int sy, int n_sta
                          p->fm_dose, p->kv, p->throw0, global_shift, N) == 0
                ? MC_OK
                : MC_E_ARG;
+}
+
+// ---------------------------------------------------------------------------
+// Frame integration (-FmIntFile): CReadFmIntFile::DoIt
+// (DataUtil/CReadFmIntFile.cpp:110-173) and CFmIntParam::Setup / mSetup /
+// mCalcIntFms / mThrowIntFrames (DataUtil/CFmIntParam.cpp:29-46,118-262).
+int mc_fm_int_table(const char* path, int n_raw, float fm_dose, int throw0, int throw1, int cap,
+                    int* n_out, int* start, int* size, float* acc_dose, float* raw_dose,
+                    int* integrated) {
+    if (n_raw < 1 || !n_out) return MC_E_ARG;
+    struct Entry { int group, isz; float dose; };
+    std::vector<Entry> ent;
+    FILE* f = path && path[0] ? fopen(path, "rt") : nullptr;
+    if (f) {
+        char buf[256] = {0};
+        int g = 0, isz = 0;
+        float d = 0.f;
+        if (fgets(buf, sizeof(buf), f)) {
+            int items = sscanf(buf, "%d  %d  %f", &g, &isz, &d);
+            bool ok = items >= 2;
+            const bool two = items == 2;
+            if (two) d = fm_dose;
+            while (ok) {
+                ent.push_back({g, isz, d});
+                if (feof(f)) break;
+                if (!fgets(buf, sizeof(buf), f)) break;
+                if (two) {
+                    if (sscanf(buf, "%d  %d", &g, &isz) != 2) continue;
+                } else {
+                    // the reference compares instead of assigning here
+                    // (`iItems == sscanf(...)`), so a line that does not
+                    // parse repeats the previous entry
+                    (void)sscanf(buf, "%d  %d  %f", &g, &isz, &d);
+                    if (d < 0) ok = false;
+                }
+            }
+        }
+        fclose(f);
+        for (const Entry& e : ent)
+            if (e.dose != ent[0].dose) {
+                ent.clear();  // variable frame dose: no integration
+                break;
+            }
+    }
+    const bool integrate =
+        !ent.empty() && std::fabs(ent.front().dose - ent.back().dose) < 0.0001f;
+    std::vector<int> st, sz;
+    std::vector<float> acc;
+    float rd = fm_dose;
+    if (!integrate) {
+        const int n = n_raw - throw0 - throw1;
+        if (n < 1) return MC_E_ARG;
+        for (int i = 0; i < n; i++) {
+            st.push_back(i + throw0);
+            sz.push_back(1);
+            acc.push_back(fm_dose * (i + 1));
+        }
+    } else {
+        const int ne = (int)ent.size();
+        std::vector<int> gs(ne), ni(ne);
+        int count = 0;
+        for (int i = 0; i < ne; i++) {
+            gs[i] = ent[i].group;
+            count += gs[i];
+        }
+        int extra = count - n_raw;
+        if (extra > 0) {
+            for (int i = ne - 1; i >= 0; i--) {
+                const int g = gs[i];
+                gs[i] -= extra;
+                if (gs[i] <= 0) gs[i] = 0;
+                else extra = extra - g;
+                if (extra <= 0) break;
+            }
+        } else if (extra < 0) {
+            gs[ne - 1] -= extra;
+        }
+        for (int i = 0; i < ne - 1; i++) {
+            const int isz = std::max(ent[i].isz, 1);
+            ni[i] = gs[i] / isz;
+            gs[i + 1] += gs[i] % isz;
+        }
+        const int lsz = std::max(ent[ne - 1].isz, 1);
+        extra = gs[ne - 1] % lsz;
+        ni[ne - 1] = gs[ne - 1] / lsz;
+        for (int i = 0; i < ne; i++)
+            for (int k = 0; k < ni[i]; k++) sz.push_back(std::max(ent[i].isz, 1));
+        const int nint = (int)sz.size();
+        if (nint < 1) return MC_E_ARG;
+        for (int i = 0; i < extra && i < nint; i++) sz[nint - 1 - i] += 1;
+        st.assign(nint, 0);
+        for (int i = 1; i < nint; i++) st[i] = st[i - 1] + sz[i - 1];
+        rd = ent[0].dose;
+        std::vector<float> idose(nint);
+        for (int i = 0; i < nint; i++) idose[i] = sz[i] * rd;
+        acc.assign(nint, 0.f);
+        acc[0] = idose[0];
+        for (int i = 1; i < nint; i++) acc[i] = acc[i - 1] + idose[i];
+        // mThrowIntFrames: ignored when five or fewer frames would remain
+        const int nt = throw0 + throw1;
+        if (nt > 0 && nint - nt > 5) {
+            const int keep = nint - nt;
+            std::vector<int> st2(st.begin() + throw0, st.begin() + throw0 + keep);
+            std::vector<int> sz2(sz.begin() + throw0, sz.begin() + throw0 + keep);
+            std::vector<float> acc2(acc.begin() + throw0, acc.begin() + throw0 + keep);
+            st.swap(st2);
+            sz.swap(sz2);
+            acc.swap(acc2);
+        }
+    }
+    const int n = (int)st.size();
+    *n_out = n;
+    if (raw_dose) *raw_dose = rd;
+    if (integrated) *integrated = integrate ? 1 : 0;
+    if (start || size || acc_dose) {
+        if (cap < n) return MC_E_ARG;
+        for (int i = 0; i < n; i++) {
+            if (start) start[i] = st[i];
+            if (size) size[i] = sz[i];
+            if (acc_dose) acc_dose[i] = acc[i];
+        }
+    }
+    return MC_OK;
 }

@@ -4,6 +4,8 @@
 
 struct IngestArgs {
     const unsigned char* raw;  // n frames of frame_bytes, stack size sx x sy
+    const int* tab;            // integration: (start, size) per output frame, or null
+    int n_out;                 // output frames (n without integration)
     long long frame_bytes;     // bytes per raw frame
     long long row_bytes;       // bytes per raw row (4-bit packed rows)
     int sx, sy, n;

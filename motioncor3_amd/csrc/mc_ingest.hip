@@ -45,9 +45,20 @@ __global__ __launch_bounds__(256) void k_apply_refs(IngestArgs a) {
     const float gn = a.gain ? a.gain[ri] : 1.f;
     const bool has_d = a.dark != nullptr, has_g = a.gain != nullptr;
     float* out = a.out + (long long)y * a.pitch + x;
-    for (int n = 0; n < a.n; n++) {
-        const unsigned char* f = a.raw + (long long)n * a.frame_bytes;
-        float v = raw_at<MODE>(f, ri, xs, a.row_bytes, ys);
+    for (int n = 0; n < a.n_out; n++) {
+        float v;
+        if (a.tab) {
+            // frame integration: unsigned-byte sum of the raw frames
+            // (Util/GAddFrames.cu:32-41 wraps at 256)
+            const int s0 = a.tab[2 * n], ns = a.tab[2 * n + 1];
+            unsigned int acc = 0;
+            for (int k = 0; k < ns; k++)
+                acc += (unsigned int)raw_at<MODE>(a.raw + (long long)(s0 + k) * a.frame_bytes, ri, xs,
+                                                  a.row_bytes, ys);
+            v = (float)(acc & 0xffu);
+        } else {
+            v = raw_at<MODE>(a.raw + (long long)n * a.frame_bytes, ri, xs, a.row_bytes, ys);
+        }
         if (has_d) v -= dk;
         if (has_g) v *= gn;
         out[(long long)n * a.frame_stride] = v;
@@ -69,7 +80,7 @@ long long mc_raw_row_bytes(int mode, int sx) {
 }
 
 hipError_t mc_launch_apply_refs(const IngestArgs& a, int mode, hipStream_t s) {
-    if (a.W < 1 || a.H < 1 || a.n < 1 || a.x0 < 0 || a.y0 < 0 || a.x0 + a.W > a.sx ||
+    if (a.W < 1 || a.H < 1 || a.n < 1 || a.n_out < 1 || a.x0 < 0 || a.y0 < 0 || a.x0 + a.W > a.sx ||
         a.y0 + a.H > a.sy || a.pitch < a.W || a.frame_stride < a.pitch * a.H)
         return hipErrorInvalidValue;
     const dim3 grid((a.W + 255) / 256, a.H), blk(256);

@@ -30,6 +30,7 @@ class McParams(C.Structure):
         ("log_dir", C.c_char_p), ("movie_name", C.c_char_p), ("verbose", C.c_int),
         ("spectra_fmt", C.c_int), ("launch_graphs", C.c_int), ("chunk_streams", C.c_int),
         ("prior", C.c_int), ("prior_fallback", C.c_float),
+        ("fm_int_start", C.POINTER(C.c_int)), ("fm_int_size", C.POINTER(C.c_int)),
     ]
 
 
@@ -54,7 +55,8 @@ class McResult(C.Structure):
 
 class McRawStack(C.Structure):
     _fields_ = [("data", C.c_void_p), ("mode", C.c_int), ("sx", C.c_int), ("sy", C.c_int),
-                ("n", C.c_int)]
+                ("n", C.c_int), ("n_int", C.c_int), ("int_start", C.POINTER(C.c_int)),
+                ("int_size", C.POINTER(C.c_int))]
 
 
 class McOverride(C.Structure):
@@ -78,7 +80,7 @@ ABI_SYMBOLS = (
     "mc_autotune", "mc_fft_variant",
     "mc_host_alloc", "mc_host_free", "mc_set_refs", "mc_get_refs", "mc_load_frames", "mc_load_wait",
     "mc_mrc_info", "mc_mrc_read",
-    "mc_save_sums", "mc_save_aln", "mc_load_aln", "mc_save_star",
+    "mc_save_sums", "mc_save_aln", "mc_load_aln", "mc_save_star", "mc_fm_int_table",
 )
 
 
@@ -107,6 +109,10 @@ def default_params(**kw):
             arr[0], arr[1] = v[0], v[1]
         elif k in ("log_dir", "movie_name"):
             setattr(p, k, v.encode() if isinstance(v, str) else v)
+        elif k in ("fm_int_start", "fm_int_size"):
+            a = np.ascontiguousarray(v, np.int32)
+            p.__dict__.setdefault("_keep", []).append(a)
+            setattr(p, k, a.ctypes.data_as(C.POINTER(C.c_int)))
         else:
             setattr(p, k, v)
     return p
@@ -339,7 +345,8 @@ class Context:
         self._check(self.lib.mc_get_refs(self.ptr, _fptr(g), _fptr(d), None, None), "mc_get_refs")
         return g, d
 
-    def load_frames(self, raw, mode, sx, sy, n, frames, width=None, height=None, async_=False):
+    def load_frames(self, raw, mode, sx, sy, n, frames, width=None, height=None, async_=False,
+                    int_start=None, int_size=None):
         """mc_load_frames: `raw` (numpy, HostArray or DevArray) holds n frames
         of an sx x sy stack in MRC `mode`; `frames` is an [N, H, pitch]
         float32 DevArray receiving the centre-cropped corrected frames."""
@@ -352,6 +359,12 @@ class Context:
             raw = np.ascontiguousarray(raw)
             ptr = raw.ctypes.data_as(C.c_void_p)
         rs = McRawStack(ptr, mode, sx, sy, n)
+        if int_size is not None:
+            ist = np.ascontiguousarray(int_start, np.int32)
+            isz = np.ascontiguousarray(int_size, np.int32)
+            rs.n_int = len(isz)
+            rs.int_start = ist.ctypes.data_as(C.POINTER(C.c_int))
+            rs.int_size = isz.ctypes.data_as(C.POINTER(C.c_int))
         self._check(self.lib.mc_load_frames(self.ptr, C.byref(rs), frames.vp(), W, Hh,
                                             C.c_longlong(frames.stride(1)),
                                             C.c_longlong(frames.stride(0)), int(async_)),
@@ -675,3 +688,24 @@ def save_star(path, params, sx, sy, n_stack, global_shift, gain_name=None):
                              gain_name.encode() if gain_name else None, _fptr(g), g.shape[0])
     if rc != MC_OK:
         raise RuntimeError("mc_save_star failed for %s" % path)
+
+
+def fm_int_table(path, n_raw, fm_dose, throw0=0, throw1=0):
+    """mc_fm_int_table: dict(start, size, acc_dose, raw_dose, integrated)."""
+    lib = load()
+    n = C.c_int()
+    pb = path.encode() if path else None
+    if lib.mc_fm_int_table(pb, n_raw, C.c_float(fm_dose), throw0, throw1, 0, C.byref(n), None,
+                           None, None, None, None) != MC_OK:
+        raise RuntimeError("mc_fm_int_table failed")
+    st = np.zeros(n.value, np.int32)
+    sz = np.zeros(n.value, np.int32)
+    acc = np.zeros(n.value, np.float32)
+    rd = C.c_float()
+    integ = C.c_int()
+    if lib.mc_fm_int_table(pb, n_raw, C.c_float(fm_dose), throw0, throw1, n.value, C.byref(n),
+                           st.ctypes.data_as(C.POINTER(C.c_int)),
+                           sz.ctypes.data_as(C.POINTER(C.c_int)), _fptr(acc), C.byref(rd),
+                           C.byref(integ)) != MC_OK:
+        raise RuntimeError("mc_fm_int_table failed")
+    return dict(start=st, size=sz, acc_dose=acc, raw_dose=rd.value, integrated=bool(integ.value))

@@ -488,21 +488,25 @@ struct Groups {
     std::vector<double> center;
 };
 
-/* DataUtil/CFmGroupParam.cpp:26-78 with every integrated frame of size 1;
- * frame centres from DataUtil/CFmIntParam.cpp:138-146,280-287. */
-Groups make_groups(int binz, int n, const std::vector<double>& fmcent) {
+/* DataUtil/CFmGroupParam.cpp:26-97: groups of at most binz x (smallest
+ * integrated frame) raw frames; frame centres from
+ * DataUtil/CFmIntParam.cpp:138-146,280-287. */
+Groups make_groups(int binz, int n, const std::vector<double>& fmcent, const int* fmsize) {
     Groups g;
     g.binz = binz;
+    int minraw = 1 << 30;
+    for (int i = 0; i < n; i++) minraw = std::min(minraw, fmsize ? fmsize[i] : 1);
+    const int maxraw = binz * minraw;
     int f = 0;
     for (int i = 0; i < n; i++) {
         g.start.push_back(f);
         int cnt = 0, raw = 0;
         while (true) {
-            raw += 1;
+            raw += fmsize ? fmsize[f] : 1;
             cnt += 1;
             f += 1;
             if (f >= n) break;
-            if (raw >= binz) break;
+            if (raw >= maxraw) break;
         }
         g.size.push_back(cnt);
         g.ng += 1;
@@ -1152,8 +1156,13 @@ int orc_align_movie(const float* frames, int W, int H, int N, const orc_params* 
     /* DataUtil/CFmIntParam.cpp:80-135,280-287 */
     std::vector<double> fmcent(N), accdose(N);
     for (int i = 0; i < N; i++) {
-        fmcent[i] = (double)(i + prm->throw0);
-        accdose[i] = prm->fm_dose * (i + 1);
+        const int st = prm->fm_int_start ? prm->fm_int_start[i] : i + prm->throw0;
+        const int sz = prm->fm_int_size ? prm->fm_int_size[i] : 1;
+        fmcent[i] = st + 0.5 * (sz - 1.0);
+        /* accumulated dose: FmDose (i+1) without integration (mSetup),
+         * the raw-frame dose times the raw frames up to and including this
+         * frame with it (mCalcIntFms, mThrowIntFrames) */
+        accdose[i] = prm->fm_int_size ? prm->fm_dose * (st + sz) : prm->fm_dose * (i + 1);
     }
     SumBank bank;
     bank.cx = g.cx;
@@ -1204,7 +1213,7 @@ int orc_align_movie(const float* frames, int W, int H, int N, const orc_params* 
             for (int i = 0; i < N; i++) { gx[i] = ov->xcf_shift[2 * i]; gy[i] = ov->xcf_shift[2 * i + 1]; }
             if (out->global_iters) out->global_iters[0] = 0;
         } else {
-            Groups grp0 = make_groups(prm->group[0], N, fmcent);
+            Groups grp0 = make_groups(prm->group[0], N, fmcent, prm->fm_int_size);
             IterResult r = iterative_align(xcf, grp0, fmcent, prm->bft[0], prm->phase_only != 0,
                                            prm->iter, tol, quirks);
             gx = r.sx;
@@ -1280,7 +1289,7 @@ int orc_align_movie(const float* frames, int W, int H, int N, const orc_params* 
     } else {
         /* Align/CMeasurePatches.cpp:22-83 with CExtractPatch
          * (Align/CExtractPatch.cpp:88-106) and an unnormalised patch R2C */
-        Groups grp1 = make_groups(prm->group[1], N, fmcent);
+        Groups grp1 = make_groups(prm->group[1], N, fmcent, prm->fm_int_size);
         for (int p = 0; p < np; p++) {
             const int sx0 = starts[2 * p], sy0 = starts[2 * p + 1];
             Stack pat;

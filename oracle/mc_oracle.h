@@ -34,6 +34,8 @@ typedef struct orc_params {
     int    split_sum;     /* -SplitSum           (0)             */
     int    throw0;        /* -Throw first (only shifts frame centres) */
     int    ref_quirks;    /* 1: reproduce reference quirks (MakeRelative k+2 etc.) */
+    const int* fm_int_start; /* -FmIntFile: raw start of each frame (NULL: i + throw0) */
+    const int* fm_int_size;  /* raw frames in each frame (NULL: 1) */
 } orc_params;
 
 /* Optional replacements for measured quantities.  Any NULL pointer means

@@ -29,6 +29,7 @@ class Params(C.Structure):
         ("group", C.c_int * 2), ("fm_ref", C.c_int), ("ft_bin", C.c_double),
         ("sum_range", C.c_double * 2), ("split_sum", C.c_int),
         ("throw0", C.c_int), ("ref_quirks", C.c_int),
+        ("fm_int_start", _ip), ("fm_int_size", _ip),
     ]
 
 
@@ -85,6 +86,10 @@ def default_params(**kw):
         if k in ("patch", "bft", "group", "sum_range"):
             arr = getattr(p, k)
             arr[0], arr[1] = v[0], v[1]
+        elif k in ("fm_int_start", "fm_int_size"):
+            a = np.ascontiguousarray(v, np.int32)
+            p.__dict__.setdefault("_keep", []).append(a)
+            setattr(p, k, a.ctypes.data_as(_ip))
         else:
             setattr(p, k, v)
     return p

@@ -133,3 +133,65 @@ def test_mrc_file_through_the_path(ctx, tmp_path):
     finally:
         ctx.set_refs(None, None)
         host.free()
+
+
+@pytest.mark.parametrize("mode", [0, 101])
+def test_integrated_frames_match_oracle(ctx, mode):
+    """-FmIntFile ingest: unsigned-byte sums of raw frames (wrapping at 256),
+    then dark and gain."""
+    rng = np.random.default_rng(40 + mode)
+    n, sy, sx = 11, 33, 70
+    raw, vals = _raw(rng, mode, n, sy, sx)
+    if mode == 0:
+        vals = rng.integers(100, 256, (n, sy, sx)).astype(np.uint8)  # sums wrap
+        raw = vals
+    start = np.array([0, 2, 5, 9], np.int32)
+    size = np.array([2, 3, 4, 2], np.int32)
+    gain = rng.uniform(0.5, 1.5, (sy, sx)).astype(np.float32)
+    dark = rng.uniform(0.0, 2.0, (sy, sx)).astype(np.float32)
+    ctx.set_refs(gain, dark)
+    try:
+        W, H = 64, 31
+        fr = ctx.empty((4, H, W), np.float32)
+        ctx.load_frames(raw, mode, sx, sy, n, fr, int_start=start, int_size=size)
+        summed = np.stack([(vals[s:s + k].astype(np.uint32).sum(0) & 0xFF).astype(np.uint8)
+                           for s, k in zip(start, size)])
+        assert np.array_equal(fr.numpy(), IO.apply_refs(summed, gain, dark, W, H))
+    finally:
+        ctx.set_refs(None, None)
+
+
+def test_integrated_alignment_matches_oracle(ctx):
+    """Frames integrated from raw frames (sizes 2 and 3): frame centres,
+    grouping by raw-frame count and the accumulated dose of the dose
+    weighting follow the table; GPU against the oracle with the same table."""
+    import oracle as O
+    W = H = 1024
+    n_raw = 30
+    t = L.fm_int_table(None, n_raw, 0.5)
+    r = MovieRecipe(W, H, n_raw, 400, seed=97, dose=1.0, local_max=2.0)
+    raw = np.minimum(r.movie(), 255).astype(np.uint8)
+    start = np.array(list(range(0, 12, 2)) + list(range(12, 30, 3)), np.int32)
+    size = np.array([2] * 6 + [3] * 6, np.int32)
+    N = len(size)
+    fr = ctx.empty((N, H, W), np.float32)
+    ctx.load_frames(raw, 0, W, H, n_raw, fr, int_start=start, int_size=size)
+    frames = fr.numpy()
+    kw = dict(patch=(3, 3), pix_size=1.0, kv=300, fm_dose=0.5)
+    pg = L.default_params(fm_int_start=start, fm_int_size=size, **kw)
+    po = O.default_params(fm_int_start=start, fm_int_size=size, **kw)
+    g = ctx.align_movie(fr, pg)
+    own = O.align_movie(frames, po, patch_starts=g["patch_starts"])
+    assert np.max(np.abs(g["global_shift"] - own["global_shift"])) <= 0.05
+    assert np.max(np.abs(g["patch_shift"] - own["patch_shift"])) <= 0.05
+    rep = O.align_movie(frames, po, replay_global=g["global_shift"], replay_patch=g["patch_shift"],
+                        replay_centers=g["patch_center"], replay_bad=g["patch_bad"])
+    for i in range(g["num_sums"]):
+        s, o = g["sums"][i].numpy(), rep["sums"][i]
+        assert np.linalg.norm(s - o) / np.linalg.norm(o) < 1e-4, i
+    # the dose weighting really used the table: the plain-dose result differs
+    plain = ctx.align_movie(fr, L.default_params(**kw), replay_global=g["global_shift"],
+                            replay_patch=g["patch_shift"], replay_centers=g["patch_center"],
+                            replay_bad=g["patch_bad"])
+    assert not np.array_equal(plain["sums"][1].numpy(), g["sums"][1].numpy())
+    assert t["size"].tolist() == [1] * n_raw
