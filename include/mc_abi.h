@@ -293,6 +293,27 @@ int mc_fm_int_table(const char* path, int n_raw, float fm_dose, int throw0, int 
                     int* integrated);
 int mc_mrc_read(const char* path, int first, int count, void* dst);
 
+/* ---- bad pixels (before the boundary) ----
+ * mc_detect_bad follows CDetectMain::DoIt (BadPixel/CDetectMain.cpp:54-245):
+ * on the sum of the N corrected frames, hot pixels above
+ * mean + max(threshold x sigma, 10) (GDetectHot.cu:25-55); defect patches
+ * where the local correlation with a defect_w x defect_h border template
+ * exceeds mean + threshold x sigma of the correlation map, marking the
+ * template's block (GLocalCC.cu, GDetectPatch.cu, CTemplate.cpp); then the
+ * "x y w h" rectangles of defect_file and the low bit of defect_map (host,
+ * W x H bytes), both optional.  bad (dev, W x H bytes) receives the map;
+ * n_bad and thresholds (hot, correlation) are optional host outputs.  The
+ * reference uses threshold 6 and a 6 x 6 template.  mc_correct_bad
+ * replaces every bad pixel of every frame by a good pixel of the same frame
+ * found by GCorrectBad's pseudo-random window search (GCorrectBad.cu:13-64,
+ * window max(win, 31)). */
+int mc_detect_bad(mc_ctx* ctx, const float* frames, int W, int H, int N, long long pitch,
+                  long long frame_stride, float threshold, int defect_w, int defect_h,
+                  const char* defect_file, const unsigned char* defect_map, unsigned char* bad,
+                  int* n_bad, float* thresholds);
+int mc_correct_bad(mc_ctx* ctx, float* frames, int W, int H, int N, long long pitch,
+                   long long frame_stride, const unsigned char* bad, int win);
+
 /* ---- output files after the alignment ----
  * mc_save_sums writes every output sum of `result` as a float MRC image
  * named like sGenOutputName (CSaveSerialCryoEM.cpp:13-32): out_mrc without

@@ -20,6 +20,7 @@
 #include "../../include/mc_abi.h"
 #include "mc_align.h"
 #include "mc_fft.h"
+#include "mc_badpix.h"
 #include "mc_ingest.h"
 #include "mc_output.h"
 #include "mc_prior.h"
@@ -2337,5 +2338,133 @@ int mc_fm_int_table(const char* path, int n_raw, float fm_dose, int throw0, int 
             if (acc_dose) acc_dose[i] = acc[i];
         }
     }
+    return MC_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Bad pixels (BadPixel/CDetectMain.cpp:54-245, BadPixel/CCorrectMain.cpp).
+namespace {
+
+// mean and E[x^2] of a W x H device image (FP64 partial sums, host total)
+int image_moments(mc_ctx* c, const float* img, long long n, double* mean, double* e2) {
+    MC_BUF(ps, double, "bp_ps", MC_BAD_BLOCKS);
+    MC_BUF(pq, double, "bp_pq", MC_BAD_BLOCKS);
+    MC_TRY(mc_launch_moments(img, n, ps, pq, c->stream));
+    std::vector<double> a(MC_BAD_BLOCKS), b(MC_BAD_BLOCKS);
+    MC_TRY(hipMemcpyAsync(a.data(), ps, sizeof(double) * MC_BAD_BLOCKS, hipMemcpyDeviceToHost, c->stream));
+    MC_TRY(hipMemcpyAsync(b.data(), pq, sizeof(double) * MC_BAD_BLOCKS, hipMemcpyDeviceToHost, c->stream));
+    MC_TRY(hipStreamSynchronize(c->stream));
+    double s1 = 0, s2 = 0;
+    for (int i = 0; i < MC_BAD_BLOCKS; i++) {
+        s1 += a[i];
+        s2 += b[i];
+    }
+    *mean = s1 / (double)n;
+    *e2 = s2 / (double)n;
+    return MC_OK;
+}
+
+}  // namespace
+
+int mc_detect_bad(mc_ctx* c, const float* frames, int W, int H, int N, long long pitch,
+                  long long frame_stride, float threshold, int defect_w, int defect_h,
+                  const char* defect_file, const unsigned char* defect_map, unsigned char* bad,
+                  int* n_bad, float* thresholds) {
+    if (!c) return MC_E_ARG;
+    if (!frames || !bad || W < 2 || H < 2 || N < 1 || pitch < W || frame_stride < pitch * H ||
+        defect_w < 1 || defect_h < 1 || defect_w * defect_h > 4096)
+        return fail(c, MC_E_ARG, "detect bad pixels: arguments");
+    MC_TRY(hipSetDevice(c->device));
+    MC_CHECK(wait_loads(c, nullptr, nullptr));
+    hipStream_t s = c->stream;
+    const long long n = (long long)W * H;
+    MC_BUF(sum, float, "bp_sum", (size_t)n);
+    MC_BUF(cc, float, "bp_cc", (size_t)n);
+    MC_BUF(tpl, float, "bp_tpl", (size_t)defect_w * defect_h);
+    // CSumFFTStack: the frame sum
+    MC_TRY(mc_launch_frame_sum(frames, W, H, N, pitch, frame_stride, sum, s));
+    // GDetectHot::DoIt (BadPixel/GDetectHot.cu:25-55), FP32 threshold
+    double m = 0, e2 = 0;
+    MC_CHECK(image_moments(c, sum, n, &m, &e2));
+    const float fm = (float)m, fe2 = (float)e2;
+    const float sd = sqrtf(std::max(fe2 - fm * fm, 0.0f));
+    float delta = threshold * sd;
+    if (delta < 10) delta = 10;
+    const float hot = fm + delta;
+    MC_TRY(mc_launch_hot(sum, n, hot, bad, s));
+    // CTemplate::Create (BadPixel/CTemplate.cpp:19-57): border -1, inside +1,
+    // normalised to zero mean and unit deviation
+    const int ts = defect_w * defect_h;
+    std::vector<float> t(ts);
+    for (int i = 0; i < ts; i++) {
+        const int x = i % defect_w, y = i / defect_w;
+        t[i] = (x == 0 || x == defect_w - 1 || y == 0 || y == defect_h - 1) ? -1.0f : 1.0f;
+    }
+    double dm = 0, ds = 0;
+    for (int i = 0; i < ts; i++) {
+        dm += t[i];
+        ds += t[i] * t[i];
+    }
+    const float tm = (float)(dm / ts);
+    ds = ds / ts - tm * tm;
+    float tsd = ds <= 0 ? 0.0f : (float)std::sqrt(ds);
+    tsd += (float)1e-30;
+    for (int i = 0; i < ts; i++) t[i] = (t[i] - tm) / tsd;
+    MC_TRY(hipMemcpyAsync(tpl, t.data(), sizeof(float) * ts, hipMemcpyHostToDevice, s));
+    // GLocalCC + GDetectPatch (BadPixel/GDetectPatch.cu:55-105)
+    MC_TRY(mc_launch_local_cc(sum, W, H, tpl, defect_w, defect_h, cc, s));
+    MC_CHECK(image_moments(c, cc, n, &m, &e2));
+    const float cm = (float)m, ce2 = (float)e2;
+    const float csd = sqrtf(std::max(0.0f, ce2 - cm * cm));
+    const float cthr = cm + threshold * csd;
+    MC_TRY(mc_launch_cc_mark(cc, W, H, defect_w, defect_h, cthr, bad, s));
+    // defect rectangles "x y w h" and an 8-bit defect map (CDetectMain.cpp:146-232)
+    std::vector<unsigned char> hm;
+    const bool extra = (defect_file && defect_file[0]) || defect_map;
+    if (extra || n_bad) {
+        hm.resize((size_t)n);
+        MC_TRY(hipMemcpyAsync(hm.data(), bad, (size_t)n, hipMemcpyDeviceToHost, s));
+        MC_TRY(hipStreamSynchronize(s));
+    }
+    if (defect_file && defect_file[0]) {
+        FILE* f = fopen(defect_file, "rt");
+        if (!f) return fail(c, MC_E_ARG, std::string("cannot read ") + defect_file);
+        char line[256];
+        int e[4];
+        while (fgets(line, sizeof(line), f)) {
+            if (sscanf(line, "%d  %d  %d  %d", e, e + 1, e + 2, e + 3) != 4) continue;
+            const int x0 = std::max(e[0], 0), y0 = std::max(e[1], 0);
+            const int x1 = std::min(e[0] + e[2], W), y1 = std::min(e[1] + e[3], H);
+            for (int y = y0; y < y1; y++)
+                for (int x = x0; x < x1; x++) hm[(size_t)y * W + x] = 1;
+        }
+        fclose(f);
+    }
+    if (defect_map)
+        for (long long i = 0; i < n; i++) hm[i] = (unsigned char)(hm[i] | (defect_map[i] & 1));
+    if (extra) MC_TRY(hipMemcpyAsync(bad, hm.data(), (size_t)n, hipMemcpyHostToDevice, s));
+    if (n_bad) {
+        long long cnt = 0;
+        for (long long i = 0; i < n; i++) cnt += hm[i];
+        *n_bad = (int)cnt;
+    }
+    if (thresholds) {
+        thresholds[0] = hot;
+        thresholds[1] = cthr;
+    }
+    MC_TRY(hipStreamSynchronize(s));
+    return MC_OK;
+}
+
+int mc_correct_bad(mc_ctx* c, float* frames, int W, int H, int N, long long pitch,
+                   long long frame_stride, const unsigned char* bad, int win) {
+    if (!c) return MC_E_ARG;
+    if (!frames || !bad || W < 2 || H < 2 || N < 1 || pitch < W || frame_stride < pitch * H)
+        return fail(c, MC_E_ARG, "correct bad pixels: arguments");
+    MC_TRY(hipSetDevice(c->device));
+    MC_CHECK(wait_loads(c, nullptr, nullptr));
+    // GCorrectBad::SetWinSize: the window is at least 31 pixels
+    MC_TRY(mc_launch_correct_bad(frames, W, H, N, pitch, frame_stride, bad, std::max(win, 31), c->stream));
+    MC_TRY(hipStreamSynchronize(c->stream));
     return MC_OK;
 }

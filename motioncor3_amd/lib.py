@@ -81,6 +81,7 @@ ABI_SYMBOLS = (
     "mc_host_alloc", "mc_host_free", "mc_set_refs", "mc_get_refs", "mc_load_frames", "mc_load_wait",
     "mc_mrc_info", "mc_mrc_read",
     "mc_save_sums", "mc_save_aln", "mc_load_aln", "mc_save_star", "mc_fm_int_table",
+    "mc_detect_bad", "mc_correct_bad",
 )
 
 
@@ -384,6 +385,32 @@ class Context:
         r.out_h, r.out_w = result["sums"][0].shape
         self._check(self.lib.mc_save_sums(self.ptr, C.byref(params), C.byref(r), out_mrc.encode()),
                     "mc_save_sums")
+
+    def detect_bad(self, frames, threshold=6.0, defect=(6, 6), defect_file=None, defect_map=None,
+                   width=None):
+        """mc_detect_bad on an [N, H, pitch] DevArray: (bad map DevArray [H, W]
+        uint8, number of bad pixels, (hot threshold, correlation threshold))."""
+        N, H, pitch = frames.shape
+        W = pitch if width is None else width
+        bad = self.empty((H, W), np.uint8)
+        nb = C.c_int()
+        th = (C.c_float * 2)()
+        dm = None if defect_map is None else np.ascontiguousarray(defect_map, np.uint8)
+        self._check(self.lib.mc_detect_bad(
+            self.ptr, frames.vp(), W, H, N, C.c_longlong(frames.stride(1)),
+            C.c_longlong(frames.stride(0)), C.c_float(threshold), defect[0], defect[1],
+            defect_file.encode() if defect_file else None,
+            None if dm is None else dm.ctypes.data_as(C.c_void_p), bad.vp(), C.byref(nb), th),
+            "mc_detect_bad")
+        return bad, nb.value, (th[0], th[1])
+
+    def correct_bad(self, frames, bad, win=7, width=None):
+        N, H, pitch = frames.shape
+        W = pitch if width is None else width
+        self._check(self.lib.mc_correct_bad(self.ptr, frames.vp(), W, H, N,
+                                            C.c_longlong(frames.stride(1)),
+                                            C.c_longlong(frames.stride(0)), bad.vp(), win),
+                    "mc_correct_bad")
 
     def autotune(self, W, H, N, patch=(0, 0), ft_bin=1.0):
         """mc_autotune: time the FFT tile variants of this geometry and keep
