@@ -813,12 +813,18 @@ __global__ __launch_bounds__(256) void k_patch_stats(double* s, const double* ce
 // fw = fma(w, ok, fw) and fsx = fma(w, ok*sx, fsx) reproduce the reference's
 // skip exactly; up to 16 kept patches the weights live in registers.
 #define MC_WARP_MAXP 64
+#ifndef MC_WARP_TX
 #define MC_WARP_TX 64
+#endif
+#ifndef MC_WARP_TY
 #define MC_WARP_TY 4
+#endif
 #define MC_WARP_NT (MC_WARP_TX * MC_WARP_TY)
 #define MC_WARP_FCH 16
 #define MC_WARP_KREG 16
+#ifndef MC_WARP_U
 #define MC_WARP_U 8
+#endif
 
 struct WarpArgs {
     const float* R;

@@ -171,7 +171,10 @@ static __device__ __forceinline__ void ct_stage_bigsym(cfloat* lds, int base, in
     constexpr int P = ct_radix(N, S), NS = ct_ns(N, S), M = N / P, HP = (P - 1) / 2;
     constexpr int KC = 4, NCH = (HP + KC - 1) / KC;
     constexpr int TWO = ct_twoff(N, S), ROOT = TWO + NS * (P - 1);
-    cfloat* uv = scr;            // [M][HP] u then [M][HP] v
+    // u and v pair sums stored [t][j], so that the lanes of one output chunk
+    // (consecutive j) read consecutive entries; roots then are one broadcast
+    // per t for all of them
+    cfloat* uv = scr;            // [HP][M] u then [HP][M] v
     cfloat* roots = scr + 2 * M * HP;
     for (int i = tid; i < P; i += TPT) roots[i] = tw[ROOT + i];
     for (int i = tid; i < M * HP; i += TPT) {
@@ -182,8 +185,8 @@ static __device__ __forceinline__ void ct_stage_bigsym(cfloat* lds, int base, in
             a = cmul(a, tw[TWO + k * (P - 1) + t - 1]);
             b = cmul(b, tw[TWO + k * (P - 1) + P - t - 1]);
         }
-        uv[j * HP + t - 1] = cadd(a, b);
-        uv[M * HP + j * HP + t - 1] = csub(a, b);
+        uv[(t - 1) * M + j] = cadd(a, b);
+        uv[M * HP + (t - 1) * M + j] = csub(a, b);
     }
     __syncthreads();
     constexpr int ITEMS = M * NCH, IT = (ITEMS + TPT - 1) / TPT;
@@ -192,7 +195,7 @@ static __device__ __forceinline__ void ct_stage_bigsym(cfloat* lds, int base, in
     for (int it = 0; it < IT; it++) {
         const int item = tid + it * TPT;
         if (item < ITEMS) {
-            const int j = item / NCH, ch = item % NCH;
+            const int ch = item / M, j = item % M;
             const cfloat x0 = lds[lp(base + j)];
             float ax[KC], ay[KC], bx[KC], by[KC];
             int idx[KC];
@@ -204,10 +207,10 @@ static __device__ __forceinline__ void ct_stage_bigsym(cfloat* lds, int base, in
                 idx[q] = 0;
             }
             cfloat s0 = x0;
-            const cfloat* u = uv + j * HP;
-            const cfloat* v = uv + M * HP + j * HP;
+            const cfloat* u = uv + j;
+            const cfloat* v = uv + M * HP + j;
             for (int t = 1; t <= HP; t++) {
-                const cfloat ut = u[t - 1], vt = v[t - 1];
+                const cfloat ut = u[(t - 1) * M], vt = v[(t - 1) * M];
                 if (ch == 0) s0 = cadd(s0, ut);
 #pragma unroll
                 for (int q = 0; q < KC; q++) {
@@ -236,7 +239,7 @@ static __device__ __forceinline__ void ct_stage_bigsym(cfloat* lds, int base, in
     for (int it = 0; it < IT; it++) {
         const int item = tid + it * TPT;
         if (item < ITEMS) {
-            const int j = item / NCH, ch = item % NCH;
+            const int ch = item / M, j = item % M;
             const int k = j % NS, qd = j / NS;
             const int d = base + qd * NS * P + k;
             if (ch == 0) lds[lp(d)] = y0[it];
