@@ -131,9 +131,13 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--spectra", default="f32", choices=["f32", "f16", "bf16"],
                     help="storage of the xcf and patch spectrum stacks (arithmetic stays FP32)")
-    ap.add_argument("--batch", type=int, default=1,
+    ap.add_argument("--batch", type=int, default=8,
                     help="movies per step: K > 1 aligns K resident movies in one mc_align_movies "
-                         "call (their alignment iterations share launches)")
+                         "call (each movie's forward transform overlaps the previous movie's "
+                         "alignment; --no-pipeline: their alignment iterations share launches)")
+    ap.add_argument("--no-pipeline", action="store_true",
+                    help="with --batch K > 1, batch the alignment iterations of the K movies "
+                         "instead of pipelining them")
     ap.add_argument("--no-h2d", action="store_true",
                     help="skip timing the host-to-device upload of one movie")
     ap.add_argument("--copy-calib", type=int, default=0, metavar="MIB",
@@ -163,7 +167,8 @@ def main():
     frames = ctx.empty((N, H, W), np.float32)
     ctx.synth_render(frames, recipe)
     kw = dict(patch=(5, 5), bft=(500, 100), pix_size=0.83, kv=300, fm_dose=1.0,
-              spectra_fmt={"f32": 0, "f16": 1, "bf16": 2}[args.spectra])
+              spectra_fmt={"f32": 0, "f16": 1, "bf16": 2}[args.spectra],
+              movie_pipeline=0 if args.no_pipeline else 1)
     prm = L.default_params(**kw)
     ns, ow, oh = ctx.output_geometry(prm, W, H, N)
     sums = [ctx.empty((oh, ow), np.float32) for _ in range(ns)]
@@ -339,6 +344,7 @@ def main():
             "iterations_global_patch": iters[-1],
             "spectra_storage": args.spectra,
             "movies_per_call": K,
+            "movie_pipeline": K > 1 and not args.no_pipeline,
             "spectra_storage_used": [["f32", "f16", "bf16"][v] for v in res["spectra_fmt_used"]],
             "learned_prior": args.prior > 0,
             "fft_tiles": tiles if tiles is not None else "defaults",

@@ -86,6 +86,11 @@ typedef struct mc_params {
                                 the accumulated dose FmDose x (start + size)
                                 follow DataUtil/CFmIntParam.cpp:138-287 and
                                 DataUtil/CFmGroupParam.cpp:26-97 */
+    int    movie_pipeline;   /* mc_align_movies only.  1 (default): movies of one
+                                size align one at a time while the next movie's
+                                frames are transformed on a second stream;
+                                0: they run their alignment iterations as one
+                                batch of jobs.  Same results either way. */
 } mc_params;
 
 /* N frames already gain/dark/defect corrected, resident in HBM
@@ -164,12 +169,16 @@ int mc_align_movie(mc_ctx* ctx, const mc_params* p, const mc_movie* movie,
                    const mc_override* ov, mc_result* res);
 
 /* ---- several movies: no reference counterpart (the reference aligns one
- * movie per CProcessThread pass, CProcessThread.cpp:178-182) ----
- * Movies of equal size and frame count run their alignment iterations as one
- * batch of jobs (the global alignment of each movie and every movie's patches
- * share the iteration launches); results are bit-identical to separate
- * mc_align_movie calls.  Movies of other sizes go in later batches, and a
- * batch holds the per-movie stacks of at most half the device memory.
+ * movie per CProcessThread pass, CProcessThread.cpp:82-107, while CMain
+ * loads the next one, CMain.cpp:193-210) ----
+ * Movies of equal size and frame count are grouped.  With movie_pipeline
+ * (default) a group's movies align one after another and each movie's
+ * forward transform runs on a second stream while the previous movie aligns
+ * (two stack sets).  Without it, a group runs its alignment iterations as one
+ * batch of jobs (the global alignment of each movie and every movie's
+ * patches share the iteration launches), holding the per-movie stacks of at
+ * most half the device memory per batch.  Either way results are
+ * bit-identical to separate mc_align_movie calls.
  * params[m] names movie m's logs; every other field must equal params[0]'s.
  * results[m] as for mc_align_movie.  No overrides. */
 int mc_align_movies(mc_ctx* ctx, const mc_params* params, const mc_movie* movies, int count,

@@ -381,8 +381,7 @@ __global__ __launch_bounds__(MC_NTHR) void k_rows_r2c_ct(RowR2CArgs a) {
         if (M % TPT == 0 || j < M) lds[lp(g * M + j)] = cmk(v[it].x, v[it].y);
     }
     __syncthreads();
-    ct_fft<M, TPT>(lds, g * M, tid, a.plan.tw,
-                   lds + mc_lds_padded(RR * (M + 1)) + g * ct_scratch(M));
+    ct_fft<M, TPT>(lds, g * M, tid, a.plan.tw);
     cfloat* tb = a.T + (long long)b * a.t_bstride + (long long)row0 * C;
     const long long kbs = (long long)a.H * C;
 #pragma unroll 4
@@ -455,8 +454,7 @@ __global__ __launch_bounds__(MC_NTHR) void k_rows_c2r_ct(RowC2RArgs a) {
         }
     }
     __syncthreads();
-    ct_fft<M, TPT>(lds, g * M, tid, a.plan.tw,
-                   lds + mc_lds_padded(RR * (M + 1)) + g * ct_scratch(M));
+    ct_fft<M, TPT>(lds, g * M, tid, a.plan.tw);
     if (g >= nrows) return;
     float* ob = a.out + (long long)b * a.out_bstride + (long long)(row0 + g) * a.pitch;
     const bool vec = ((a.pitch | a.out_bstride) & 1) == 0 && (((uintptr_t)a.out) & 7) == 0;
@@ -1057,6 +1055,24 @@ int build_plan1(FftPlan1* P, cfloat** dev_tw, int n) {
         }
         ns *= r;
     }
+    // chunk root table of the big-prime stage (ct_stage_bigsym), copied from
+    // that stage's roots so both forms use the same values
+    if (const int bp = ct_big_prime(n)) {
+        int root = -1;
+        for (int s = 0; s < P->nst; s++)
+            if (P->st[s].r == bp) root = P->st[s].root_off;
+        if (root < 0 || (int)h.size() != ct_twoff(n, ct_nst(n))) return -1;
+        if (h.size() & 1) h.push_back(cfloat{0.f, 0.f});
+        const int hp = (bp - 1) / 2, nch = (hp + MC_BIGSYM_KC - 1) / MC_BIGSYM_KC;
+        const std::vector<cfloat> roots(h.begin() + root, h.begin() + root + bp);
+        for (int ch = 0; ch < nch; ch++)
+            for (int t = 1; t <= hp; t++)
+                for (int q = 0; q < MC_BIGSYM_KC; q++) {
+                    const int k = ch * MC_BIGSYM_KC + q + 1;
+                    h.push_back(roots[k <= hp ? (t * k) % bp : 0]);
+                }
+        if ((int)h.size() != ct_rt_off(n) + ct_rt_elems(n)) return -1;
+    }
     if (h.empty()) h.push_back(cfloat{1.f, 0.f});
     if (hipMalloc(dev_tw, h.size() * sizeof(cfloat)) != hipSuccess) return -2;
     if (hipMemcpy(*dev_tw, h.data(), h.size() * sizeof(cfloat), hipMemcpyHostToDevice) != hipSuccess)
@@ -1109,8 +1125,12 @@ int mc_fft2d_create(Fft2D* p, int W, int H) {
     p->C = C;
     int R = 1;
     const int want = std::max((MC_TILE_RC + C - 1) / C, 2);
-    const int per_row = m + 1 + ct_scratch_host(m);
-    while (R * 2 <= want && (R * 2) * per_row <= budget) R *= 2;
+    const int per_row = m + 1;
+    // lengths with a prime factor above 31 (the 1441-point xcf rows): two
+    // rows per workgroup, measured faster than four on MI355X (more waves
+    // per CU for the direct big-prime stage)
+    const int rmax = ct_big_prime(m) ? 2 : want;
+    while (R * 2 <= rmax && (R * 2) * per_row <= budget) R *= 2;
     if (R * per_row > MC_LDS_ELEMS) return -1;
     p->R = R;
     if (build_plan1(&p->row, &p->row_tw, m) != 0) return -2;
@@ -1231,7 +1251,7 @@ hipError_t mc_rows_r2c_mod(const Fft2D& p, const float* in, long long pitch, lon
 #define X(MM, RRR, CCC)                                                             \
     if (p.m == MM && p.R == RRR && p.C == CCC) {                                    \
         const size_t ldsct =                                                        \
-            (size_t)(mc_lds_padded(RRR * (MM + 1)) + RRR * ct_scratch(MM)) * sizeof(cfloat); \
+            (size_t)mc_lds_padded(RRR * (MM + 1)) * sizeof(cfloat); \
         hipLaunchKernelGGL((k_rows_r2c_ct<MM, RRR, CCC>), grid, dim3(MC_NTHR), ldsct, s, a); \
         return hipGetLastError();                                                   \
     }
@@ -1273,7 +1293,7 @@ hipError_t mc_rows_c2r(const Fft2D& p, const cfloat* T, long long t_bstride, flo
 #define X(MM, RRR, CCC)                                                             \
     if (p.m == MM && p.R == RRR && p.C == CCC) {                                    \
         const size_t ldsct =                                                        \
-            (size_t)(mc_lds_padded(RRR * (MM + 1)) + RRR * ct_scratch(MM)) * sizeof(cfloat); \
+            (size_t)mc_lds_padded(RRR * (MM + 1)) * sizeof(cfloat); \
         hipLaunchKernelGGL((k_rows_c2r_ct<MM, RRR, CCC>), grid, dim3(MC_NTHR), ldsct, s, a); \
         return hipGetLastError();                                                   \
     }

@@ -144,49 +144,84 @@ static __device__ __forceinline__ void ct_stage_big(cfloat* lds, int base, int t
     __syncthreads();
 }
 
-// Largest prime radix above 31 in the factorisation of N (0 if none) and
-// the LDS scratch (complex elements) the symmetric big-prime stage needs per
-// transform: (P-1)/2 pair sums and differences per butterfly, plus P roots.
+// Largest prime radix above 31 in the factorisation of N (0 if none).
 __host__ __device__ constexpr int ct_big_prime(int n) {
     int b = 0;
     for (int s = 0; s < ct_nst(n); s++)
         if (ct_radix(n, s) > 31) b = ct_radix(n, s);
     return b;
 }
-__host__ __device__ constexpr int ct_scratch(int n) {
-    const int p = ct_big_prime(n);
-    return p ? (n / p) * (p - 1) + p : 0;
+
+// Output pairs per work item of the big-prime stage, and its chunk root
+// table: after the per-stage twiddles and roots of the whole transform (at
+// an even offset, so that rows of it are 16-byte aligned), for every output
+// chunk ch and t = 1..(P-1)/2 the roots e^{-2 pi i t k / P} of the chunk's
+// KC outputs k = ch*KC+1.. (root 0 where k > (P-1)/2).
+#define MC_BIGSYM_KC 4
+__host__ __device__ constexpr int ct_rt_off(int n) {
+    const int o = ct_twoff(n, ct_nst(n));
+    return o + (o & 1);
 }
-inline int ct_scratch_host(int n) { return ct_scratch(n); }
+__host__ __device__ constexpr int ct_rt_elems(int n) {
+    const int p = ct_big_prime(n), hp = (p - 1) / 2;
+    return p ? ((hp + MC_BIGSYM_KC - 1) / MC_BIGSYM_KC) * hp * MC_BIGSYM_KC : 0;
+}
 
 // Prime radix above 31, symmetric-pair form: phase 1 forms
-// u_t = x_t + x_{P-t}, v_t = x_t - x_{P-t} (twiddled) of every butterfly in
-// LDS scratch; phase 2 gives each work item KC output pairs
-// y_k = x0 + sum_t u_t cos(2 pi t k/P) - i sum_t v_t sin(2 pi t k/P) and
-// y_{P-k} (the conjugate-sign partner), with the roots read from LDS.
+// u_t = x_t + x_{P-t}, v_t = x_t - x_{P-t} (twiddled) of every butterfly and,
+// after all of the transform's values are in registers, stores them unpadded
+// in the transform's own LDS range ([t][j]: u at t*M + j, v after all u, then
+// x_0 of every butterfly: M (P-1) + M = N elements); phase 2 gives each work
+// item KC output pairs y_k = x0 + sum_t u_t cos(2 pi t k/P)
+// - i sum_t v_t sin(2 pi t k/P) and y_{P-k} (the conjugate-sign partner), with
+// the KC roots of each t read as two 16-byte rows of the chunk root table
+// (lanes of one chunk share them), so the loop carries no index arithmetic.
 template <int N, int S, int TPT>
 static __device__ __forceinline__ void ct_stage_bigsym(cfloat* lds, int base, int tid,
-                                                       const cfloat* __restrict__ tw,
-                                                       cfloat* scr) {
+                                                       const cfloat* __restrict__ tw) {
     constexpr int P = ct_radix(N, S), NS = ct_ns(N, S), M = N / P, HP = (P - 1) / 2;
-    constexpr int KC = 4, NCH = (HP + KC - 1) / KC;
-    constexpr int TWO = ct_twoff(N, S), ROOT = TWO + NS * (P - 1);
-    // u and v pair sums stored [t][j], so that the lanes of one output chunk
-    // (consecutive j) read consecutive entries; roots then are one broadcast
-    // per t for all of them
-    cfloat* uv = scr;            // [HP][M] u then [HP][M] v
-    cfloat* roots = scr + 2 * M * HP;
-    for (int i = tid; i < P; i += TPT) roots[i] = tw[ROOT + i];
-    for (int i = tid; i < M * HP; i += TPT) {
-        const int j = i / HP, t = i % HP + 1;
-        const int k = j % NS;
-        cfloat a = lds[lp(base + j + t * M)], b = lds[lp(base + j + (P - t) * M)];
-        if (NS > 1) {
-            a = cmul(a, tw[TWO + k * (P - 1) + t - 1]);
-            b = cmul(b, tw[TWO + k * (P - 1) + P - t - 1]);
+    constexpr int KC = MC_BIGSYM_KC, NCH = (HP + KC - 1) / KC;
+    constexpr int TWO = ct_twoff(N, S), RT = ct_rt_off(N);
+    constexpr int IT1 = (M * HP + TPT - 1) / TPT, IX0 = (M + TPT - 1) / TPT;
+    static_assert(KC == 4, "two float4 root rows per t");
+    static_assert(ct_big_prime(N) == P, "one prime radix above 31 per length");
+    // the transform occupies padded indices base..base+N-1, i.e. at least N
+    // consecutive elements from lp(base)
+    cfloat* row = lds + lp(base);
+    cfloat ua[IT1], va[IT1], x0a[IX0];
+#pragma unroll
+    for (int it = 0; it < IT1; it++) {
+        const int i = tid + it * TPT;
+        if ((M * HP) % TPT == 0 || i < M * HP) {
+            const int t = i / M + 1, j = i - (t - 1) * M;
+            const int k = j % NS;
+            cfloat a = lds[lp(base + j + t * M)], b = lds[lp(base + j + (P - t) * M)];
+            if (NS > 1) {
+                a = cmul(a, tw[TWO + k * (P - 1) + t - 1]);
+                b = cmul(b, tw[TWO + k * (P - 1) + P - t - 1]);
+            }
+            ua[it] = cadd(a, b);
+            va[it] = csub(a, b);
         }
-        uv[(t - 1) * M + j] = cadd(a, b);
-        uv[M * HP + (t - 1) * M + j] = csub(a, b);
+    }
+#pragma unroll
+    for (int it = 0; it < IX0; it++) {
+        const int j = tid + it * TPT;
+        if (M % TPT == 0 || j < M) x0a[it] = lds[lp(base + j)];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int it = 0; it < IT1; it++) {
+        const int i = tid + it * TPT;
+        if ((M * HP) % TPT == 0 || i < M * HP) {
+            row[i] = ua[it];
+            row[M * HP + i] = va[it];
+        }
+    }
+#pragma unroll
+    for (int it = 0; it < IX0; it++) {
+        const int j = tid + it * TPT;
+        if (M % TPT == 0 || j < M) row[2 * M * HP + j] = x0a[it];
     }
     __syncthreads();
     constexpr int ITEMS = M * NCH, IT = (ITEMS + TPT - 1) / TPT;
@@ -196,34 +231,32 @@ static __device__ __forceinline__ void ct_stage_bigsym(cfloat* lds, int base, in
         const int item = tid + it * TPT;
         if (item < ITEMS) {
             const int ch = item / M, j = item % M;
-            const cfloat x0 = lds[lp(base + j)];
+            const cfloat x0 = row[2 * M * HP + j];
             float ax[KC], ay[KC], bx[KC], by[KC];
-            int idx[KC];
 #pragma unroll
             for (int q = 0; q < KC; q++) {
                 ax[q] = x0.x;
                 ay[q] = x0.y;
                 bx[q] = by[q] = 0.f;
-                idx[q] = 0;
             }
-            cfloat s0 = x0;
-            const cfloat* u = uv + j;
-            const cfloat* v = uv + M * HP + j;
+            cfloat s0 = x0;  // y_0, kept by the items of chunk 0
+            const cfloat* u = row + j;
+            const cfloat* v = row + M * HP + j;
+            const float4* rt = reinterpret_cast<const float4*>(tw + RT + ch * HP * KC);
+#pragma unroll 13
             for (int t = 1; t <= HP; t++) {
                 const cfloat ut = u[(t - 1) * M], vt = v[(t - 1) * M];
-                if (ch == 0) s0 = cadd(s0, ut);
+                s0 = cadd(s0, ut);
+                // roots e^{-2 pi i t k / P} = (cos, -sin) of the chunk's KC outputs
+                const float4 r01 = rt[2 * (t - 1)], r23 = rt[2 * (t - 1) + 1];
+                const cfloat r[KC] = {cmk(r01.x, r01.y), cmk(r01.z, r01.w), cmk(r23.x, r23.y),
+                                      cmk(r23.z, r23.w)};
 #pragma unroll
                 for (int q = 0; q < KC; q++) {
-                    const int kk = ch * KC + q + 1;
-                    int id = idx[q] + (kk <= HP ? kk : 0);
-                    id = id >= P ? id - P : id;
-                    idx[q] = id;
-                    // roots hold e^{-2 pi i id / P} = (cos, -sin)
-                    const cfloat r = roots[id];
-                    ax[q] = fmaf(ut.x, r.x, ax[q]);
-                    ay[q] = fmaf(ut.y, r.x, ay[q]);
-                    bx[q] = fmaf(vt.x, -r.y, bx[q]);
-                    by[q] = fmaf(vt.y, -r.y, by[q]);
+                    ax[q] = fmaf(ut.x, r[q].x, ax[q]);
+                    ay[q] = fmaf(ut.y, r[q].x, ay[q]);
+                    bx[q] = fmaf(vt.x, -r[q].y, bx[q]);
+                    by[q] = fmaf(vt.y, -r[q].y, by[q]);
                 }
             }
             y0[it] = s0;
@@ -258,21 +291,17 @@ static __device__ __forceinline__ void ct_stage_bigsym(cfloat* lds, int base, in
 
 template <int N, int S, int TPT>
 struct CtFft {
-    static __device__ __forceinline__ void run(cfloat* lds, int base, int tid, const cfloat* tw,
-                                               cfloat* scr) {
+    static __device__ __forceinline__ void run(cfloat* lds, int base, int tid, const cfloat* tw) {
         if constexpr (S < ct_nst(N)) {
-            if constexpr (ct_radix(N, S) > 31) ct_stage_bigsym<N, S, TPT>(lds, base, tid, tw, scr);
+            if constexpr (ct_radix(N, S) > 31) ct_stage_bigsym<N, S, TPT>(lds, base, tid, tw);
             else ct_stage<N, S, TPT>(lds, base, tid, tw);
-            CtFft<N, S + 1, TPT>::run(lds, base, tid, tw, scr);
+            CtFft<N, S + 1, TPT>::run(lds, base, tid, tw);
         }
     }
 };
 
 // Forward FFT of the group's transform (all threads of the workgroup call).
-// scr: this group's LDS scratch of ct_scratch(N) elements (unused when N has
-// no prime factor above 31).
 template <int N, int TPT>
-static __device__ __forceinline__ void ct_fft(cfloat* lds, int base, int tid, const cfloat* tw,
-                                              cfloat* scr = nullptr) {
-    CtFft<N, 0, TPT>::run(lds, base, tid, tw, scr);
+static __device__ __forceinline__ void ct_fft(cfloat* lds, int base, int tid, const cfloat* tw) {
+    CtFft<N, 0, TPT>::run(lds, base, tid, tw);
 }

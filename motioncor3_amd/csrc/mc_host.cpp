@@ -81,6 +81,10 @@ struct mc_ctx {
     int ref_w = 0, ref_h = 0;
 };
 
+#ifndef MC_CHUNK_FRAMES
+#define MC_CHUNK_FRAMES 2
+#endif
+
 namespace {
 
 #define MC_TRY(x)                                                                  \
@@ -642,11 +646,11 @@ double kv_factor(int kv) {
 // ---------------------------------------------------------------------------
 // Make the context stream wait for the asynchronous loads (mc_load_frames)
 // whose output overlaps [lo, hi), or for all of them when lo is null.
-int wait_loads(mc_ctx* c, const char* lo, const char* hi) {
+int wait_loads(mc_ctx* c, const char* lo, const char* hi, hipStream_t st = nullptr) {
     std::vector<mc_ctx::PendingLoad> keep;
     for (auto& l : c->loads) {
         if (!lo || (l.lo < hi && lo < l.hi)) {
-            MC_TRY(hipStreamWaitEvent(c->stream, l.ev, 0));
+            MC_TRY(hipStreamWaitEvent(st ? st : c->stream, l.ev, 0));
             c->free_ev.push_back(l.ev);
         } else {
             keep.push_back(l);
@@ -664,6 +668,68 @@ int wait_loads(mc_ctx* c, const char* lo, const char* hi) {
     return MC_OK;
 }
 
+// Make `st` wait for the loads that write any of movie `mv`'s frames.
+int wait_movie_loads(mc_ctx* c, const mc_movie* mv, hipStream_t st) {
+    const char* lo = (const char*)mv->frames;
+    const size_t bytes = sizeof(float) * (size_t)(mv->frame_stride * (mv->N - 1) +
+                                                  mv->pitch * (mv->H - 1) + mv->W);
+    return wait_loads(c, lo, lo + bytes, st);
+}
+
+// ---------------------------------------------------------------------------
+// Align/CTransformStack.cpp + Align/CGenXcfStack.cpp: normalised R2C of every
+// frame of `count` movies and the Fourier crop to the xcf size (Sx == null:
+// none) in the same pass, CH frames per chunk.  Chunk k runs on lanes[k %
+// nl] with that lane's staging buffers; the caller orders the lanes.
+struct FwdLane {
+    hipStream_t s;
+    cfloat* T;   // CH * t_elems
+    float* dcf;  // CH * H, reduced stack formats only
+};
+int enqueue_forward(mc_ctx* c, Fft2D* pf, const mc_movie* const* mvs, int count, const Geo& g,
+                    int fx, cfloat* Sf_all, void* Sx_all, float* gx, float* ix, const FwdLane* lanes,
+                    int nl, int CH) {
+    const int W = mvs[0]->W, H = mvs[0]->H, N = mvs[0]->N;
+    const long long fs = (long long)g.cx * H, xfs = (long long)g.xcx * g.xh;
+    const float scale = (float)(1.0 / ((double)W * H));
+    int chunk = 0;
+    for (int m = 0; m < count; m++) {
+        const mc_movie* mv = mvs[m];
+        for (int f0 = 0; f0 < N; f0 += CH, chunk++) {
+            const int nb = std::min(CH, N - f0);
+            const long long fi = (long long)m * N + f0;
+            const FwdLane& L = lanes[chunk % nl];
+            {
+                KTimer kt(c, "fft_rows_r2c", L.s);
+                MC_TRY(mc_rows_r2c_mod(*pf, mv->frames + (long long)f0 * mv->frame_stride, mv->pitch,
+                                       mv->frame_stride, 0, nullptr, L.T, pf->t_elems, nb, scale, L.s,
+                                       L.dcf));
+            }
+            if (fx != MC_SPEC_F32) MC_TRY(mc_launch_spec_scale(L.dcf, H, nb, gx + fi, ix + fi, L.s));
+            {
+                KTimer kt(c, "fft_cols_fwd", L.s);
+                MC_TRY(mc_cols_fwd(*pf, L.T, pf->t_elems, Sf_all + fi * fs, fs,
+                                   Sx_all ? spec_at(Sx_all, fi * xfs, fx) : nullptr, g.xcx, g.xh, xfs,
+                                   nb, L.s, MC_SPEC_F32, nullptr, fx,
+                                   fx != MC_SPEC_F32 ? gx + fi : nullptr));
+            }
+        }
+    }
+    return MC_OK;
+}
+
+// Movie pipelining across align_batch calls of one geometry: the stacks of
+// the movie being aligned live in buffer slot `slot`; when `ready`, its
+// spectra were transformed by the previous call on the second stream and
+// `ev[slot]` marks their completion.  `next` (or null) is transformed into
+// the other slot on the second stream while this movie aligns.
+struct Pipe {
+    int slot = 0;
+    bool ready = false;
+    const mc_movie* next = nullptr;
+    hipEvent_t ev[2] = {nullptr, nullptr};
+};
+
 // ---------------------------------------------------------------------------
 // `count` movies of one geometry.  The alignment iterations of all movies
 // run as one batch of jobs (the global alignment of movie m is job m, patch
@@ -672,7 +738,7 @@ int wait_loads(mc_ctx* c, const char* lo, const char* hi) {
 // count == 1.  pp[m] supplies movie m's log name; everything else comes
 // from pp[0].
 int align_batch(mc_ctx* c, const mc_params* const* pp_, const mc_movie* const* mvs,
-                const mc_override* ov, mc_result* const* rs, int count) {
+                const mc_override* ov, mc_result* const* rs, int count, Pipe* pipe = nullptr) {
     using clk = std::chrono::steady_clock;
     const auto t0 = clk::now();
     if (count < 1 || (count > 1 && ov)) return fail(c, MC_E_ARG, "overrides need a single movie");
@@ -696,12 +762,10 @@ int align_batch(mc_ctx* c, const mc_params* const* pp_, const mc_movie* const* m
         return fail(c, MC_E_ARG, "patch replay needs local shifts and centres");
     const int quirks = p->ref_quirks;
     hipStream_t s = c->stream;
+    if (pipe && (count != 1 || ov)) return fail(c, MC_E_ARG, "pipelined movies go one at a time");
     // frames written by asynchronous loads: wait for exactly those loads
-    for (int m = 0; m < count; m++) {
-        const mc_movie* mv = mvs[m];
-        const char* lo = (const char*)mv->frames;
-        MC_CHECK(wait_loads(c, lo, lo + sizeof(float) * (size_t)(mv->frame_stride * (N - 1) + mv->pitch * (H - 1) + W)));
-    }
+    if (!(pipe && pipe->ready))
+        for (int m = 0; m < count; m++) MC_CHECK(wait_movie_loads(c, mvs[m], s));
 
     // DataUtil/CFmIntParam.cpp:80-123 and CBufferPool.cpp:185-202
     const bool dw = !(p->kv < 80 || p->kv > 300) && p->pix_size > 0 && p->fm_dose > 0;
@@ -741,24 +805,40 @@ int align_batch(mc_ctx* c, const mc_params* const* pp_, const mc_movie* const* m
         rs[m]->spectra_fmt_used[1] = fpch;
         rs[m]->prior_fallbacks = 0;
     }
-    const int CH = 2;  // frames per forward/inverse chunk (staging stays in the 256 MB MALL)
+    const int CH = MC_CHUNK_FRAMES;  // frames per forward/inverse chunk
     // movie m's full spectra at Sf_all + m * N * fs, its xcf stack at element
     // m * N * xfs of Sx_all, its patch stack at element m * P * N * pfs of Sp_all
-    MC_BUF(Sf_all, cfloat, "Sf", (size_t)count * N * fs);
+    // the stacks of a movie slot: full spectra, xcf stack, per-frame factors
+    struct Stacks {
+        cfloat* Sf = nullptr;
+        cfloat* Sx = nullptr;
+        float* gx = nullptr;
+        float* ix = nullptr;
+    };
+    auto stacks = [&](int slot, Stacks& o) -> bool {
+        const std::string k = slot ? "1" : "";
+        o.Sf = (cfloat*)buf(c, ("Sf" + k).c_str(), sizeof(cfloat) * (size_t)count * N * fs);
+        o.Sx = o.Sf;
+        if (o.Sf && !xcf_alias && !replay)
+            o.Sx = (cfloat*)buf(c, ("Sx" + k).c_str(), (fx == MC_SPEC_F32 ? 8 : 4) * (size_t)count * N * xfs);
+        if (o.Sx && fx != MC_SPEC_F32) {
+            o.gx = (float*)buf(c, ("gx" + k).c_str(), sizeof(float) * (size_t)count * N);
+            o.ix = (float*)buf(c, ("ix" + k).c_str(), sizeof(float) * (size_t)count * N);
+            return o.gx && o.ix;
+        }
+        return o.Sx != nullptr;
+    };
+    Stacks cur;
+    if (!stacks(pipe ? pipe->slot : 0, cur)) return fail(c, MC_E_NOMEM, "movie stacks");
+    cfloat* Sf_all = cur.Sf;
+    cfloat* Sx_all = cur.Sx;
+    float* gx = cur.gx;
+    float* ix = cur.ix;
     MC_BUF(T, cfloat, "T", (size_t)CH * pf->t_elems);
-    cfloat* Sx_all = Sf_all;
-    if (!xcf_alias && !replay) {
-        Sx_all = (cfloat*)buf(c, "Sx", (fx == MC_SPEC_F32 ? 8 : 4) * (size_t)count * N * xfs);
-        if (!Sx_all) return fail(c, MC_E_NOMEM, "xcf stack");
-    }
-    float* dcf = nullptr;   // |row DC| of the chunk's frames, and per-frame factors
-    float* gx = nullptr;
-    float* ix = nullptr;
+    float* dcf = nullptr;   // |row DC| of the chunk's frames
     if (fx != MC_SPEC_F32) {
         dcf = (float*)buf(c, "dcf", sizeof(float) * (size_t)CH * H);
-        gx = (float*)buf(c, "gx", sizeof(float) * (size_t)count * N);
-        ix = (float*)buf(c, "ix", sizeof(float) * (size_t)count * N);
-        if (!dcf || !gx || !ix) return fail(c, MC_E_NOMEM, "stack scale factors");
+        if (!dcf) return fail(c, MC_E_NOMEM, "stack scale factors");
     }
 
     // Frame-chunk loops alternate between the context stream and a second
@@ -781,37 +861,29 @@ int align_batch(mc_ctx* c, const mc_params* const* pp_, const mc_movie* const* m
         return e ? e : hipStreamWaitEvent(s, c->evj, 0);
     };
 
-    // Align/CTransformStack.cpp + Align/CGenXcfStack.cpp: normalised R2C of
-    // every frame and the Fourier crop to the xcf size in the same pass.
     const float scale = (float)(1.0 / ((double)W * H));
-    MC_TRY(fork());
-    int chunk = 0;
-    for (int m = 0; m < count; m++) {
-        const mc_movie* mv = mvs[m];
-        for (int f0 = 0; f0 < N; f0 += CH, chunk++) {
-            const int nb = std::min(CH, N - f0);
-            const long long fi = (long long)m * N + f0;
-            const bool odd = two && (chunk & 1);
-            hipStream_t sk = odd ? c->s2 : s;
-            cfloat* Tk = odd ? T2 : T;
-            float* dck = odd ? dcf2 : dcf;
-            {
-                KTimer kt(c, "fft_rows_r2c", sk);
-                MC_TRY(mc_rows_r2c_mod(*pf, mv->frames + (long long)f0 * mv->frame_stride, mv->pitch,
-                                       mv->frame_stride, 0, nullptr, Tk, pf->t_elems, nb, scale, sk,
-                                       dck));
-            }
-            if (fx != MC_SPEC_F32) MC_TRY(mc_launch_spec_scale(dck, H, nb, gx + fi, ix + fi, sk));
-            {
-                KTimer kt(c, "fft_cols_fwd", sk);
-                MC_TRY(mc_cols_fwd(*pf, Tk, pf->t_elems, Sf_all + fi * fs, fs,
-                                   (xcf_alias || replay) ? nullptr : spec_at(Sx_all, fi * xfs, fx),
-                                   g.xcx, g.xh, xfs, nb, sk, MC_SPEC_F32, nullptr, fx,
-                                   fx != MC_SPEC_F32 ? gx + fi : nullptr));
-            }
-        }
+    void* Sx_fwd = (xcf_alias || replay) ? nullptr : (void*)Sx_all;
+    if (pipe && pipe->ready) {
+        // transformed on the second stream while the previous movie aligned
+        MC_TRY(hipStreamWaitEvent(s, pipe->ev[pipe->slot], 0));
+    } else {
+        const FwdLane lanes[2] = {{s, T, dcf}, {c->s2, T2, dcf2}};
+        MC_TRY(fork());
+        MC_CHECK(enqueue_forward(c, pf, mvs, count, g, fx, Sf_all, Sx_fwd, gx, ix, lanes, two ? 2 : 1, CH));
+        MC_TRY(join());
     }
-    MC_TRY(join());
+    if (pipe && pipe->next) {
+        // the next movie's forward transform on the second stream, behind
+        // everything already queued there; the frame-chunk loops below
+        // queue their second-stream chunks after it
+        Stacks nx;
+        if (!stacks(pipe->slot ^ 1, nx)) return fail(c, MC_E_NOMEM, "next movie's stacks");
+        MC_CHECK(wait_movie_loads(c, pipe->next, c->s2));
+        const FwdLane lane{c->s2, T2, dcf2};
+        MC_CHECK(enqueue_forward(c, pf, &pipe->next, 1, g, fx, nx.Sf, Sx_fwd ? (void*)nx.Sx : nullptr,
+                                 nx.gx, nx.ix, &lane, 1, CH));
+        MC_TRY(hipEventRecord(pipe->ev[pipe->slot ^ 1], c->s2));
+    }
 
     struct MovieState {
         std::vector<double> gshift, ps, pc, gerr, xs;
@@ -1197,6 +1269,7 @@ void mc_default_params(mc_params* p) {
     p->chunk_streams = 2;
     p->prior = 0;
     p->prior_fallback = 0.2f;
+    p->movie_pipeline = 1;
 }
 
 int mc_ctx_create(int device, void* stream, mc_ctx** out) {
@@ -1339,7 +1412,8 @@ static bool same_alignment_flags(const mc_params& a, const mc_params& b) {
            a.spectra_fmt == b.spectra_fmt && a.launch_graphs == b.launch_graphs &&
            a.chunk_streams == b.chunk_streams && a.prior == b.prior &&
            a.fm_int_start == b.fm_int_start && a.fm_int_size == b.fm_int_size &&
-           a.prior_fallback == b.prior_fallback && a.verbose == b.verbose;
+           a.prior_fallback == b.prior_fallback && a.verbose == b.verbose &&
+           a.movie_pipeline == b.movie_pipeline;
 }
 
 int mc_align_movies(mc_ctx* c, const mc_params* params, const mc_movie* movies, int count,
@@ -1365,6 +1439,37 @@ int mc_align_movies(mc_ctx* c, const mc_params* params, const mc_movie* movies, 
         const mc_params& p = params[m0];
         const mc_movie& mv = movies[m0];
         if (mv.W < 8 || mv.H < 8 || mv.N < 2) return fail(c, MC_E_ARG, "invalid movie geometry");
+        if (p.movie_pipeline && grp.size() > 1) {
+            // one movie at a time; movie k+1's forward transform runs on the
+            // second stream while movie k aligns (stack slots alternate)
+            struct Events {
+                hipEvent_t e[2] = {nullptr, nullptr};
+                ~Events() {
+                    for (hipEvent_t x : e)
+                        if (x) (void)hipEventDestroy(x);
+                }
+            } evs;
+            for (hipEvent_t& e : evs.e)
+                if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess)
+                    return fail(c, MC_E_HIP, "hipEventCreate");
+            Pipe pipe;
+            pipe.ev[0] = evs.e[0];
+            pipe.ev[1] = evs.e[1];
+            for (size_t k = 0; k < grp.size(); k++) {
+                pipe.slot = (int)(k & 1);
+                pipe.ready = k > 0;
+                pipe.next = k + 1 < grp.size() ? &movies[grp[k + 1]] : nullptr;
+                const mc_params* pa = &params[grp[k]];
+                const mc_movie* ma = &movies[grp[k]];
+                mc_result* ra = &results[grp[k]];
+                const int rc = align_batch(c, &pa, &ma, nullptr, &ra, 1, &pipe);
+                if (rc != MC_OK) {
+                    (void)hipStreamSynchronize(c->s2);
+                    return rc;
+                }
+            }
+            continue;
+        }
         const bool patch = p.patch[0] > 1 && p.patch[1] > 1;
         const Geo g = make_geo(mv.W, mv.H, mv.N, p.patch[0], p.patch[1], p.ft_bin);
         const double per_movie = (double)mv.N * ((double)g.cx * mv.H * 8 + (double)g.xcx * g.xh * 8 +

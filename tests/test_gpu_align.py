@@ -168,19 +168,24 @@ def test_iteration_graphs_match_direct_launches(ctx):
         ctx.align_movie(mv, L.default_params(launch_graphs=0, **kw))
 
 
-def test_batched_movies_match_single_calls(ctx):
-    """mc_align_movies: two movies of one size share the alignment batches, a
-    third of another size goes in its own batch; every result is the one a
-    separate mc_align_movie call gives, bit for bit."""
-    kw = dict(patch=(3, 3), iter=10, pix_size=1.0, kv=300, fm_dose=1.0)
-    shapes = [(1024, 1024, 12, 11), (1152, 816, 6, 12), (1024, 1024, 12, 13)]
+@pytest.mark.parametrize("pipeline,patch,fmt", [(0, (3, 3), 0), (1, (3, 3), 0), (1, (0, 0), 0),
+                                                  (1, (3, 3), 1)])
+def test_batched_movies_match_single_calls(ctx, pipeline, patch, fmt):
+    """mc_align_movies: three movies of one size form a group (one batch of
+    alignment jobs, or a pipeline whose stack slots are reused by the third
+    movie), a fourth of another size goes in its own group; every result is
+    the one a separate mc_align_movie call gives, bit for bit."""
+    kw = dict(patch=patch, iter=10, pix_size=1.0, kv=300, fm_dose=1.0, spectra_fmt=fmt)
+    shapes = [(1024, 1024, 12, 11), (1152, 816, 6, 12), (1024, 1024, 12, 13), (1024, 1024, 12, 14)]
     movies = [ctx.upload(MovieRecipe(W, H, N, 300, seed=sd, dose=4.0, local_max=2.0).movie())
               for W, H, N, sd in shapes]
-    prm = L.default_params(**kw)
+    prm = L.default_params(movie_pipeline=pipeline, **kw)
     single = [ctx.align_movie(m, prm) for m in movies]
     batched = ctx.align_movies(movies, prm)
+    keys = ("global_shift",) + (("patch_shift", "patch_bad", "patch_center", "patch_starts")
+                                if patch[0] > 1 else ())
     for a, b in zip(batched, single):
-        for k in ("global_shift", "patch_shift", "patch_bad", "patch_center", "patch_starts"):
+        for k in keys:
             assert np.array_equal(a[k], b[k]), k
         assert a["global_iters"] == b["global_iters"]
         for x, y in zip(a["sums"], b["sums"]):

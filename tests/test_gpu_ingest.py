@@ -106,6 +106,38 @@ def test_async_loads_overlap_alignments(ctx):
             h.free()
 
 
+def test_async_loads_feed_the_movie_pipeline(ctx):
+    """Three movies loaded asynchronously, then aligned by one pipelined
+    mc_align_movies call: each movie's forward transform (on the second
+    stream) must wait for its own load, and every result must equal the
+    alignment of frames loaded synchronously."""
+    W = H = 1024
+    N = 8
+    pg = L.default_params(patch=(3, 3), pix_size=1.0, kv=300, fm_dose=1.0, movie_pipeline=1)
+    movies = [MovieRecipe(W, H, N, 400, seed=80 + k, dose=4.0, local_max=2.0).movie().astype(np.uint8)
+              for k in range(3)]
+    host = [ctx.host_array(m.shape, np.uint8) for m in movies]
+    for h, m in zip(host, movies):
+        h.a[...] = m
+    gain = np.random.default_rng(4).uniform(0.9, 1.1, (H, W)).astype(np.float32)
+    ctx.set_refs(gain, None)
+    try:
+        bufs = [ctx.empty((N, H, W), np.float32) for _ in range(3)]
+        ref = []
+        for m in range(3):
+            ctx.load_frames(host[m], 0, W, H, N, bufs[0])
+            ref.append(ctx.align_movie(bufs[0], pg))
+        for m in range(3):
+            ctx.load_frames(host[m], 0, W, H, N, bufs[m], async_=True)
+        got = ctx.align_movies(bufs, pg)
+        for a, b in zip(got, ref):
+            _same_alignment(a, b)
+    finally:
+        ctx.set_refs(None, None)
+        for h in host:
+            h.free()
+
+
 def test_mrc_file_through_the_path(ctx, tmp_path):
     """A 4-bit MRC stack read with mc_mrc_read into pinned memory, corrected
     and aligned, equals aligning the oracle-corrected frames."""
