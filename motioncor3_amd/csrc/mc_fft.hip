@@ -829,14 +829,26 @@ __global__ __launch_bounds__(NT) void k_rows_r2c_rp(RowR2CArgs a, const cfloat* 
     }
 }
 
+// C2R pass 0 straight from the staging layout: butterfly j of pass 0 reads
+// X[j + t M0] (t < R0) and needs the Hermitian partners X[M - j - t M0],
+// which are the inputs of butterfly M0 - j, so one work item takes the pair
+// (j, M0 - j) (j = 0 and j = M0/2 pair with themselves; j = 0 also reads the
+// Nyquist value X[M]) and forms the pre-processed inputs
+// Z[k] = conj(X[k] + conj X[M-k] - i e^{+2 pi i k/(2M)} (X[k] - conj X[M-k]))
+// (the half-length packing of the real inverse) in registers.
+__device__ __forceinline__ cfloat c2r_pre(cfloat p, cfloat q, cfloat w) {
+    const cfloat t = cmul(cconj(w), cmk(p.x - q.x, p.y + q.y));
+    return cconj(cmk(p.x + q.x - t.y, p.y - q.y + t.x));
+}
+
 template <int M, int RR, int C, int NT>
 __global__ __launch_bounds__(NT) void k_rows_c2r_rp(RowC2RArgs a, const cfloat* __restrict__ ptw) {
     constexpr int NP = rp_npass(M);
+    constexpr int R0 = rp_radix(M, 0), M0 = M / R0;
     constexpr int RL = rp_radix(M, NP - 1), ML = M / RL;
+    constexpr int NPAIR = M0 / 2 + 1;
+    constexpr int IT0 = (RR * NPAIR + NT - 1) / NT;
     constexpr int ITL = (RR * ML + NT - 1) / NT;
-    constexpr int NB = (M + 1 + C - 1) / C;
-    constexpr int TOT = NB * RR * C;
-    constexpr int HALF = M / 2 + 1;
     extern __shared__ cfloat lds[];
     const int nrb = (a.H + RR - 1) / RR;
     const int tt = xcd_tile(blockIdx.x, gridDim.x);
@@ -845,47 +857,58 @@ __global__ __launch_bounds__(NT) void k_rows_c2r_rp(RowC2RArgs a, const cfloat* 
     const int nrows = min(RR, a.H - row0);
     const cfloat* tb = a.T + (long long)b * a.t_bstride + (long long)row0 * C;
     const long long kbs = (long long)a.H * C;
-    for (int e0 = threadIdx.x; e0 < TOT; e0 += NT * MC_IO_UNROLL) {
-        cfloat v[MC_IO_UNROLL];
-        int dst[MC_IO_UNROLL];
 #pragma unroll
-        for (int u = 0; u < MC_IO_UNROLL; u++) {
-            const int e = e0 + u * NT;
-            dst[u] = -1;
-            if (e < TOT) {
-                const int c = e % C, r = (e / C) % RR, kb = e / (C * RR);
-                const int k = kb * C + c;
-                if (k <= M && r < nrows) {
-                    v[u] = tb[kb * kbs + r * C + c];
-                    dst[u] = (k < M) ? r * M + k : RR * M + r;
+    for (int it = 0; it < IT0; it++) {
+        const int i = threadIdx.x + it * NT;
+        if ((RR * NPAIR) % NT == 0 || i < RR * NPAIR) {
+            const int r = i / NPAIR, j = i - r * NPAIR;
+            if (r < nrows) {
+                const int j2 = j == 0 ? 0 : M0 - j;  // partner butterfly
+                const bool two = j2 != j;
+                const cfloat* xr = tb + r * C;
+                cfloat xa[R0], xb[R0];
+#pragma unroll
+                for (int t = 0; t < R0; t++) {
+                    const int k = j + t * M0;
+                    xa[t] = xr[(long long)(k / C) * kbs + k % C];
+                }
+                if (two) {
+#pragma unroll
+                    for (int t = 0; t < R0; t++) {
+                        const int k = j2 + t * M0;
+                        xb[t] = xr[(long long)(k / C) * kbs + k % C];
+                    }
+                }
+                cfloat v[R0];
+                if (j == 0) {
+                    const float xm = xr[(long long)(M / C) * kbs + M % C].x;
+                    v[0] = cconj(cmk(xa[0].x + xm, xa[0].x - xm));
+#pragma unroll
+                    for (int t = 1; t < R0; t++) v[t] = c2r_pre(xa[t], xa[R0 - t], a.post_tw[t * M0]);
+                } else {
+#pragma unroll
+                    for (int t = 0; t < R0; t++)
+                        v[t] = c2r_pre(xa[t], two ? xb[R0 - 1 - t] : xa[R0 - 1 - t],
+                                             a.post_tw[j + t * M0]);
+                }
+                RegDft<R0>::run(v);
+                cfloat* d = lds + r * M + j * R0;
+#pragma unroll
+                for (int t = 0; t < R0; t++) d[t] = v[t];
+                if (two) {
+#pragma unroll
+                    for (int t = 0; t < R0; t++)
+                        v[t] = c2r_pre(xb[t], xa[R0 - 1 - t], a.post_tw[j2 + t * M0]);
+                    RegDft<R0>::run(v);
+                    cfloat* d2 = lds + r * M + j2 * R0;
+#pragma unroll
+                    for (int t = 0; t < R0; t++) d2[t] = v[t];
                 }
             }
         }
-#pragma unroll
-        for (int u = 0; u < MC_IO_UNROLL; u++)
-            if (dst[u] >= 0) lds[dst[u]] = v[u];
     }
     __syncthreads();
-    for (int i = threadIdx.x; i < RR * HALF; i += NT) {
-        const int g = i / HALF, k = i - g * HALF;
-        const int row = g * M;
-        if (k == 0) {
-            const float x0 = lds[row].x, xm = lds[RR * M + g].x;
-            lds[row] = cconj(cmk(x0 + xm, x0 - xm));
-            continue;
-        }
-        const int k2 = M - k;
-        if (k2 < k) continue;
-        const cfloat xa = lds[row + k], xb = lds[row + k2];
-        const cfloat t1 = cmul(cconj(a.post_tw[k]), cmk(xa.x - xb.x, xa.y + xb.y));
-        lds[row + k] = cconj(cmk(xa.x + xb.x - t1.y, xa.y - xb.y + t1.x));
-        if (k2 != k) {
-            const cfloat t2 = cmul(cconj(a.post_tw[k2]), cmk(xb.x - xa.x, xb.y + xa.y));
-            lds[row + k2] = cconj(cmk(xb.x + xa.x - t2.y, xb.y - xa.y + t2.x));
-        }
-    }
-    __syncthreads();
-    RpRange<M, 0, NP - 1, RR, NT>::run(lds, ptw);
+    RpRange<M, 1, NP - 1, RR, NT>::run(lds, ptw);
     const bool vec = ((a.pitch | a.out_bstride) & 1) == 0 && (((uintptr_t)a.out) & 7) == 0;
 #pragma unroll
     for (int it = 0; it < ITL; it++) {
