@@ -302,6 +302,48 @@ int mc_fm_int_table(const char* path, int n_raw, float fm_dose, int throw0, int 
                     int* integrated);
 int mc_mrc_read(const char* path, int first, int count, void* dst);
 
+/* TIFF movies (TiffUtil/CLoadTiffHeader.cpp:18-147): size of the first
+ * directory, directory count, MRC mode of the samples (8 bits -> 0, 16 bits ->
+ * 1 signed or 6 unsigned, 32-bit float -> 2; -1 otherwise, e.g. EER), the
+ * compression code and the pixel size XResolution / 2.54e8 (0 if absent). */
+int mc_tiff_info(const char* path, int* sx, int* sy, int* n, int* mode, int* compression,
+                 float* pix_size);
+/* Frames first..first+count-1 decoded (uncompressed, LZW, Deflate, PackBits;
+ * strips or tiles; predictor 1 or 2) into dst in MRC-mode layout, rows stored
+ * bottom-up as TiffUtil/CLoadTiffImage.cpp:72-160 stores them.  The frames
+ * then go through mc_load_frames like an MRC stack. */
+int mc_tiff_read(const char* path, int first, int count, void* dst);
+
+/* EER files (EerUtil/CLoadEerHeader.cpp:22-47): camera size, fraction count,
+ * code width (compression 65001 -> 7, 65000 -> 8) and the total bytes of the
+ * fractions' raw strips. */
+int mc_eer_info(const char* path, int* cam_w, int* cam_h, int* n, int* bits, long long* bytes);
+/* The fractions' raw strips back to back (EerUtil/CLoadEerFrames.cpp:72-95):
+ * fraction k at data[offsets[k], offsets[k+1]); cap bytes of data, n + 1
+ * offsets. */
+int mc_eer_read(const char* path, unsigned char* data, long long cap, long long* offsets);
+
+/* An EER movie for mc_load_eer. */
+typedef struct mc_eer_stack {
+    const unsigned char* data;  /* compressed fractions (host; pinned from
+                                   mc_host_alloc makes an asynchronous load) */
+    const long long* offsets;   /* host, n + 1 entries, offsets[0] = 0 */
+    int n;                      /* EER fractions */
+    int cam_w, cam_h;           /* camera size */
+    int bits;                   /* 7 or 8 */
+    int sampling;               /* -EerSampling: 1, 2 (2x) or 3 (4x) */
+    int n_int;                  /* rendered frames (0: one per fraction) */
+    const int* int_start;       /* fraction start of each rendered frame */
+    const int* int_size;        /* fractions summed into each rendered frame */
+} mc_eer_stack;
+/* CRenderMrcStack (EerUtil/CRenderMrcStack.cpp:22-101) + CDecodeEerFrame
+ * (EerUtil/CDecodeEerFrame.cpp:46-215) on the GPU: every rendered frame is
+ * the unsigned-byte event count of its fractions at (cam x 1, 2 or 4) size,
+ * then the gain/dark correction and centre crop of mc_load_frames (references
+ * of the rendered size).  async as for mc_load_frames. */
+int mc_load_eer(mc_ctx* ctx, const mc_eer_stack* eer, float* frames, int W, int H, long long pitch,
+                long long frame_stride, int async);
+
 /* ---- bad pixels (before the boundary) ----
  * mc_detect_bad follows CDetectMain::DoIt (BadPixel/CDetectMain.cpp:54-245):
  * on the sum of the N corrected frames, hot pixels above

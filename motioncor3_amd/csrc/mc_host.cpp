@@ -24,6 +24,7 @@
 #include "mc_ingest.h"
 #include "mc_output.h"
 #include "mc_prior.h"
+#include "mc_tiff.h"
 
 // A learned shift prior on the context's device (mc_prior_create).
 constexpr float kPriorUnit = 2.0f;  // xcf px per unit of the prior's shift inputs and outputs
@@ -2134,7 +2135,9 @@ int mc_load_frames(mc_ctx* c, const mc_raw_stack* r, float* frames, int W, int H
         src = (const unsigned char*)r->data;
     } else {
         (void)hipGetLastError();  // a plain host pointer is not an error here
-        unsigned char* staging = (unsigned char*)buf(c, "raw_in", (size_t)bytes);
+        // asynchronous loads (in order on the ingest stream) and synchronous
+        // ones (context stream) stage through different buffers
+        unsigned char* staging = (unsigned char*)buf(c, async ? "raw_in_a" : "raw_in", (size_t)bytes);
         if (!staging) return fail(c, MC_E_NOMEM, "raw frame staging");
         MC_TRY(hipMemcpyAsync(staging, r->data, (size_t)bytes, hipMemcpyHostToDevice, s));
         src = staging;
@@ -2244,6 +2247,189 @@ int mc_mrc_read(const char* path, int first, int count, void* dst) {
             }
     }
     return rc;
+}
+
+int mc_tiff_info(const char* path, int* sx, int* sy, int* n, int* mode, int* compression,
+                 float* pix_size) {
+    if (!path) return MC_E_ARG;
+    TiffFile t;
+    if (!tiff_open(path, t)) return MC_E_ARG;
+    const TiffDir& d = t.dirs[0];
+    if (sx) *sx = d.w;
+    if (sy) *sy = d.h;
+    if (n) *n = (int)t.dirs.size();
+    if (mode) *mode = tiff_mode(d);
+    if (compression) *compression = d.comp;
+    // TiffUtil/CLoadTiffHeader.cpp:137-147
+    if (pix_size)
+        *pix_size = d.has_xres ? (float)(d.xres / 2.54e8) : (d.has_yres ? (float)(d.yres / 2.54e8) : 0.f);
+    return MC_OK;
+}
+
+int mc_tiff_read(const char* path, int first, int count, void* dst) {
+    if (!path || !dst || first < 0 || count < 1) return MC_E_ARG;
+    TiffFile t;
+    if (!tiff_open(path, t) || first + count > (int)t.dirs.size()) return MC_E_ARG;
+    const TiffDir& d0 = t.dirs[0];
+    const int mode = tiff_mode(d0);
+    if (mode < 0) return MC_E_ARG;
+    const size_t fb = (size_t)d0.w * d0.h * (d0.bits / 8);
+    for (int k = 0; k < count; k++) {
+        const TiffDir& d = t.dirs[first + k];
+        if (d.w != d0.w || d.h != d0.h || tiff_mode(d) != mode) return MC_E_ARG;
+        if (!tiff_decode(t, d, (unsigned char*)dst + fb * k)) return MC_E_ARG;
+    }
+    return MC_OK;
+}
+
+int mc_eer_info(const char* path, int* cam_w, int* cam_h, int* n, int* bits, long long* bytes) {
+    if (!path) return MC_E_ARG;
+    TiffFile t;
+    if (!tiff_open(path, t)) return MC_E_ARG;
+    const TiffDir& d = t.dirs[0];
+    // EerUtil/CLoadEerHeader.cpp:31-36,50-66
+    const int b = d.comp == 65000 ? 8 : (d.comp == 65001 ? 7 : -1);
+    if (b < 0 || d.w < 1 || d.h < 1) return MC_E_ARG;
+    uint64_t tot = 0;
+    for (const TiffDir& e : t.dirs) {
+        uint64_t k = 0;
+        if (!tiff_raw(t, e, nullptr, &k)) return MC_E_ARG;
+        tot += k;
+    }
+    if (cam_w) *cam_w = d.w;
+    if (cam_h) *cam_h = d.h;
+    if (n) *n = (int)t.dirs.size();
+    if (bits) *bits = b;
+    if (bytes) *bytes = (long long)tot;
+    return MC_OK;
+}
+
+int mc_eer_read(const char* path, unsigned char* data, long long cap, long long* offsets) {
+    if (!path || !data || !offsets || cap < 0) return MC_E_ARG;
+    TiffFile t;
+    if (!tiff_open(path, t)) return MC_E_ARG;
+    uint64_t tot = 0;
+    for (size_t k = 0; k < t.dirs.size(); k++) {
+        uint64_t nb = 0;
+        if (!tiff_raw(t, t.dirs[k], nullptr, &nb) || tot + nb > (uint64_t)cap) return MC_E_ARG;
+        offsets[k] = (long long)tot;
+        if (!tiff_raw(t, t.dirs[k], data + tot, &nb)) return MC_E_ARG;
+        tot += nb;
+    }
+    offsets[t.dirs.size()] = (long long)tot;
+    return MC_OK;
+}
+
+int mc_load_eer(mc_ctx* c, const mc_eer_stack* e, float* frames, int W, int H, long long pitch,
+                long long frame_stride, int async) {
+    if (!c || !e || !e->data || !e->offsets || !frames) return MC_E_ARG;
+    if (e->n < 1 || e->cam_w < 1 || e->cam_h < 1 || (e->bits != 7 && e->bits != 8) ||
+        e->sampling < 1 || e->sampling > 3 || e->offsets[0] != 0)
+        return fail(c, MC_E_ARG, "invalid EER stack");
+    for (int k = 0; k < e->n; k++)
+        if (e->offsets[k + 1] < e->offsets[k]) return fail(c, MC_E_ARG, "EER offsets must not decrease");
+    const int fact = e->sampling == 1 ? 1 : (e->sampling == 2 ? 2 : 4);
+    const long long fw = (long long)e->cam_w * fact, fh = (long long)e->cam_h * fact;
+    if (fw * fh >= (1LL << 31)) return fail(c, MC_E_ARG, "rendered EER frame too large");
+    if (W < 1 || H < 1 || W > fw || H > fh || pitch < W || frame_stride < pitch * H)
+        return fail(c, MC_E_ARG, "invalid frame geometry");
+    if ((c->gain || c->dark) && (c->ref_w != fw || c->ref_h != fh))
+        return fail(c, MC_E_ARG, "gain/dark reference size differs from the rendered EER size");
+    const int nout = e->n_int > 0 ? e->n_int : e->n;
+    std::vector<int> blob;  // offsets (two ints each), then (fraction, frame) jobs
+    blob.resize(2 * (size_t)(e->n + 1));
+    memcpy(blob.data(), e->offsets, sizeof(long long) * (size_t)(e->n + 1));
+    for (int o = 0; o < nout; o++) {
+        const int st = e->n_int > 0 ? (e->int_start ? e->int_start[o] : -1) : o;
+        const int sz = e->n_int > 0 ? (e->int_size ? e->int_size[o] : -1) : 1;
+        if (st < 0 || sz < 1 || st + sz > e->n)
+            return fail(c, MC_E_ARG, "EER integration table exceeds the fractions");
+        for (int k = 0; k < sz; k++) {
+            blob.push_back(st + k);
+            blob.push_back(o);
+        }
+    }
+    const int njob = (int)(blob.size() / 2) - (e->n + 1);
+    MC_TRY(hipSetDevice(c->device));
+    if (async && !c->s_in) MC_TRY(hipStreamCreate(&c->s_in));
+    hipStream_t s = async ? c->s_in : c->stream;
+    if (async) {
+        hipEvent_t ev = nullptr;
+        MC_TRY(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+        MC_TRY(hipEventRecord(ev, c->stream));
+        MC_TRY(hipStreamWaitEvent(c->s_in, ev, 0));
+        c->free_ev.push_back(ev);
+    }
+    const std::string sfx = async ? "_a" : "";
+    const size_t nbytes = (size_t)e->offsets[e->n];
+    unsigned char* din = (unsigned char*)buf(c, ("eer_in" + sfx).c_str(), nbytes + 4);
+    int* dtab = (int*)buf(c, ("eer_tab" + sfx).c_str(), sizeof(int) * blob.size());
+    unsigned int* cnt =
+        (unsigned int*)buf(c, ("eer_counts" + sfx).c_str(), sizeof(unsigned int) * (size_t)(nout * fw * fh));
+    if (!din || !dtab || !cnt) return fail(c, MC_E_NOMEM, "EER staging");
+    // the host tables stay alive until their copy has run
+    std::vector<std::pair<hipEvent_t, std::vector<int>>> keep;
+    for (auto& t : c->tab_keep) {
+        if (hipEventQuery(t.first) == hipSuccess) (void)hipEventDestroy(t.first);
+        else keep.push_back(std::move(t));
+    }
+    (void)hipGetLastError();
+    c->tab_keep.swap(keep);
+    hipEvent_t te = nullptr;
+    MC_TRY(hipEventCreateWithFlags(&te, hipEventDisableTiming));
+    c->tab_keep.emplace_back(te, std::move(blob));
+    const std::vector<int>& kt = c->tab_keep.back().second;
+    if (nbytes) MC_TRY(hipMemcpyAsync(din, e->data, nbytes, hipMemcpyHostToDevice, s));
+    MC_TRY(hipMemcpyAsync(dtab, kt.data(), sizeof(int) * kt.size(), hipMemcpyHostToDevice, s));
+    MC_TRY(hipEventRecord(te, s));
+    MC_TRY(mc_launch_zero_u32(cnt, nout * fw * fh, s));
+    EerArgs ea;
+    ea.data = din;
+    ea.off = reinterpret_cast<const long long*>(dtab);
+    ea.job = dtab + 2 * (e->n + 1);
+    ea.njob = njob;
+    ea.cam_w = e->cam_w;
+    ea.cam_h = e->cam_h;
+    ea.bits = e->bits;
+    ea.sampling = e->sampling;
+    ea.fw = (int)fw;
+    ea.fh = (int)fh;
+    ea.counts = cnt;
+    {
+        KTimer kt2(c, "eer_render", s);
+        MC_TRY(mc_launch_eer_render(ea, s));
+    }
+    IngestArgs a;
+    a.raw = reinterpret_cast<const unsigned char*>(cnt);
+    a.tab = nullptr;
+    a.n_out = nout;
+    a.frame_bytes = 4 * fw * fh;
+    a.row_bytes = 4 * fw;
+    a.sx = (int)fw;
+    a.sy = (int)fh;
+    a.n = nout;
+    a.x0 = (int)((fw - W) / 2);
+    a.y0 = (int)((fh - H) / 2);
+    a.W = W;
+    a.H = H;
+    a.gain = c->gain;
+    a.dark = c->dark;
+    a.out = frames;
+    a.pitch = pitch;
+    a.frame_stride = frame_stride;
+    {
+        KTimer kt2(c, "apply_refs", s);
+        MC_TRY(mc_launch_apply_refs(a, MC_MODE_EER_COUNTS, s));
+    }
+    if (async) {
+        hipEvent_t ev = nullptr;
+        MC_TRY(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+        MC_TRY(hipEventRecord(ev, c->s_in));
+        const char* lo = (const char*)frames;
+        c->loads.push_back(
+            {lo, lo + sizeof(float) * (size_t)(frame_stride * (nout - 1) + pitch * (H - 1) + W), ev});
+    }
+    return MC_OK;
 }
 
 // ---------------------------------------------------------------------------

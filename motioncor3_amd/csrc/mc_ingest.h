@@ -17,6 +17,24 @@ struct IngestArgs {
     long long pitch, frame_stride;
 };
 
+// Raw mode of rendered EER frames: 32-bit event counters, read modulo 256.
+#define MC_MODE_EER_COUNTS 1032
+
+struct EerArgs {
+    const unsigned char* data;  // compressed EER fractions
+    const long long* off;       // fraction k occupies data[off[k], off[k+1])
+    const int* job;             // (fraction, rendered frame) per thread
+    int njob;
+    int cam_w, cam_h;           // camera size
+    int bits;                   // 7 (compression 65001) or 8 (65000)
+    int sampling;               // 1 camera size, 2 and 3: 2x and 4x super-resolution
+    int fw, fh;                 // rendered frame size
+    unsigned int* counts;       // rendered frames, fw x fh counters each
+};
+
 // Bytes per raw row of an MRC mode (0, 1, 2, 5, 6, 101), or -1.
 long long mc_raw_row_bytes(int mode, int sx);
 hipError_t mc_launch_apply_refs(const IngestArgs& a, int mode, hipStream_t s);
+hipError_t mc_launch_eer_render(const EerArgs& a, hipStream_t s);
+// n 32-bit zeros by a kernel (no memset node if the caller captures a graph)
+hipError_t mc_launch_zero_u32(unsigned int* p, long long n, hipStream_t s);

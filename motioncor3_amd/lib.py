@@ -60,6 +60,13 @@ class McRawStack(C.Structure):
                 ("int_size", C.POINTER(C.c_int))]
 
 
+class McEerStack(C.Structure):
+    _fields_ = [("data", C.c_void_p), ("offsets", C.POINTER(C.c_longlong)), ("n", C.c_int),
+                ("cam_w", C.c_int), ("cam_h", C.c_int), ("bits", C.c_int), ("sampling", C.c_int),
+                ("n_int", C.c_int), ("int_start", C.POINTER(C.c_int)),
+                ("int_size", C.POINTER(C.c_int))]
+
+
 class McOverride(C.Structure):
     _fields_ = [
         ("replay_global", C.POINTER(C.c_float)), ("replay_patch", C.POINTER(C.c_float)),
@@ -80,7 +87,8 @@ ABI_SYMBOLS = (
     "mc_prior_forward", "mc_prior_loss_grad", "mc_prior_train", "mc_set_prior", "mc_gemm_bf16",
     "mc_autotune", "mc_fft_variant",
     "mc_host_alloc", "mc_host_free", "mc_set_refs", "mc_get_refs", "mc_load_frames", "mc_load_wait",
-    "mc_mrc_info", "mc_mrc_read",
+    "mc_mrc_info", "mc_mrc_read", "mc_tiff_info", "mc_tiff_read", "mc_eer_info", "mc_eer_read",
+    "mc_load_eer",
     "mc_save_sums", "mc_save_aln", "mc_load_aln", "mc_save_star", "mc_fm_int_table",
     "mc_detect_bad", "mc_correct_bad",
 )
@@ -373,6 +381,39 @@ class Context:
                     "mc_load_frames")
         if not async_:
             self.sync()
+
+    def load_eer(self, data, offsets, cam_w, cam_h, bits, sampling, frames, width=None,
+                 height=None, async_=False, int_start=None, int_size=None):
+        """mc_load_eer: `data` (numpy uint8 or HostArray) holds the compressed
+        EER fractions, `offsets` their n + 1 byte offsets; `frames` is an
+        [N, H, pitch] float32 DevArray receiving the rendered, corrected and
+        centre-cropped frames."""
+        N, H, pitch = frames.shape
+        W = pitch if width is None else width
+        Hh = H if height is None else height
+        if isinstance(data, HostArray):
+            ptr = data.vp()
+        else:
+            data = np.ascontiguousarray(data, np.uint8)
+            ptr = data.ctypes.data_as(C.c_void_p)
+        off = np.ascontiguousarray(offsets, np.int64)
+        es = McEerStack(ptr, off.ctypes.data_as(C.POINTER(C.c_longlong)), len(off) - 1, cam_w,
+                        cam_h, bits, sampling)
+        keep = [data, off]
+        if int_size is not None:
+            ist = np.ascontiguousarray(int_start, np.int32)
+            isz = np.ascontiguousarray(int_size, np.int32)
+            keep += [ist, isz]
+            es.n_int = len(isz)
+            es.int_start = ist.ctypes.data_as(C.POINTER(C.c_int))
+            es.int_size = isz.ctypes.data_as(C.POINTER(C.c_int))
+        self._check(self.lib.mc_load_eer(self.ptr, C.byref(es), frames.vp(), W, Hh,
+                                         C.c_longlong(frames.stride(1)),
+                                         C.c_longlong(frames.stride(0)), int(async_)),
+                    "mc_load_eer")
+        if not async_:
+            self.sync()
+        return keep
 
     def load_wait(self):
         self._check(self.lib.mc_load_wait(self.ptr), "mc_load_wait")
@@ -676,6 +717,58 @@ def mrc_read(path, first=0, count=None, out=None):
     if rc != MC_OK:
         raise RuntimeError("mc_mrc_read failed for %s" % path)
     return out
+
+
+def tiff_info(path):
+    """mc_tiff_info: dict(sx, sy, n, mode, compression, pix_size)."""
+    v = [C.c_int() for _ in range(5)]
+    px = C.c_float()
+    rc = load().mc_tiff_info(path.encode(), *[C.byref(x) for x in v], C.byref(px))
+    if rc != MC_OK:
+        raise RuntimeError("mc_tiff_info failed for %s" % path)
+    return dict(sx=v[0].value, sy=v[1].value, n=v[2].value, mode=v[3].value,
+                compression=v[4].value, pix_size=px.value)
+
+
+def tiff_read(path, first=0, count=None, out=None):
+    """mc_tiff_read: frames [first, first+count) decoded into `out` (numpy
+    array or HostArray; uint8 bytes when None), rows bottom-up."""
+    info = tiff_info(path)
+    if info["mode"] < 0:
+        raise ValueError("not a sample TIFF (mode %d)" % info["mode"])
+    count = info["n"] - first if count is None else count
+    nb = raw_frame_bytes(info["mode"], info["sx"], info["sy"]) * count
+    if out is None:
+        out = np.empty(nb, np.uint8)
+    ptr = out.vp() if isinstance(out, HostArray) else out.ctypes.data_as(C.c_void_p)
+    if out.nbytes < nb:
+        raise ValueError("output buffer too small")
+    rc = load().mc_tiff_read(path.encode(), first, count, ptr)
+    if rc != MC_OK:
+        raise RuntimeError("mc_tiff_read failed for %s" % path)
+    return out
+
+
+def eer_info(path):
+    """mc_eer_info: dict(cam_w, cam_h, n, bits, bytes)."""
+    v = [C.c_int() for _ in range(4)]
+    nb = C.c_longlong()
+    rc = load().mc_eer_info(path.encode(), *[C.byref(x) for x in v], C.byref(nb))
+    if rc != MC_OK:
+        raise RuntimeError("mc_eer_info failed for %s" % path)
+    return dict(cam_w=v[0].value, cam_h=v[1].value, n=v[2].value, bits=v[3].value, bytes=nb.value)
+
+
+def eer_read(path):
+    """mc_eer_read: (data uint8 array, offsets int64 array of n + 1)."""
+    info = eer_info(path)
+    data = np.empty(max(info["bytes"], 1), np.uint8)
+    off = np.empty(info["n"] + 1, np.int64)
+    rc = load().mc_eer_read(path.encode(), data.ctypes.data_as(C.POINTER(C.c_ubyte)),
+                            C.c_longlong(data.nbytes), off.ctypes.data_as(C.POINTER(C.c_longlong)))
+    if rc != MC_OK:
+        raise RuntimeError("mc_eer_read failed for %s" % path)
+    return data[:info["bytes"]], off
 
 
 def save_aln(path, params, sx, sy, n_stack, global_shift, patch_shift=None, patch_center=None):

@@ -227,3 +227,138 @@ def test_integrated_alignment_matches_oracle(ctx):
                             replay_bad=g["patch_bad"])
     assert not np.array_equal(plain["sums"][1].numpy(), g["sums"][1].numpy())
     assert t["size"].tolist() == [1] * n_raw
+
+
+# ---- EER ----
+
+def _eer_movie(rng, cam_w, cam_h, nfrac, bits, per_frac):
+    import eer_oracle as E
+    cam = cam_w * cam_h
+    streams = []
+    for _ in range(nfrac):
+        q = np.sort(rng.choice(cam, size=per_frac, replace=False))
+        if bits == 7:
+            sub = rng.integers(0, 16, per_frac)
+            streams.append(E.encode_7bit(list(zip(q.tolist(), sub.tolist())), cam))
+        else:
+            streams.append(E.encode_8bit(q.tolist(), cam))
+    off = np.cumsum([0] + [len(s) for s in streams]).astype(np.int64)
+    return np.concatenate(streams), off
+
+
+@pytest.mark.parametrize("bits,sampling", [(7, 1), (7, 2), (7, 3), (8, 1), (8, 2)])
+@pytest.mark.parametrize("integrate", [False, True])
+def test_eer_render_matches_oracle(ctx, bits, sampling, integrate):
+    """mc_load_eer renders EER fractions on the GPU (one thread per fraction
+    and rendered frame) exactly as the oracle restates CDecodeEerFrame and
+    CRenderMrcStack, then applies the gain and the centre crop; the 8-bit
+    super-resolution case follows the reference's byte-0 walk."""
+    import eer_oracle as E
+    rng = np.random.default_rng(10 * bits + sampling)
+    cam_w, cam_h = (4096, 6) if sampling > 1 else (96, 64)
+    nfrac = 12
+    data, off = _eer_movie(rng, cam_w, cam_h, nfrac, bits, 700)
+    f = E.factor(sampling)
+    fw, fh = cam_w * f, cam_h * f
+    ist, isz = ([0, 2, 5, 9], [2, 3, 4, 3]) if integrate else (None, None)
+    want = E.render(data, off, cam_w, cam_h, bits, sampling, ist, isz)
+    gain = rng.uniform(0.9, 1.1, (fh, fw)).astype(np.float32)
+    W, H = fw - 6, fh - 2
+    n_out = want.shape[0]
+    frames = ctx.empty((n_out, H, W), np.float32)
+    ctx.set_refs(gain, None)
+    try:
+        ctx.load_eer(data, off, cam_w, cam_h, bits, sampling, frames, int_start=ist, int_size=isz)
+    finally:
+        ctx.set_refs(None, None)
+    exp = IO.apply_refs(want, gain, None, W, H)
+    assert np.array_equal(frames.numpy(), exp)
+    assert want.sum() > 0
+
+
+def test_eer_counts_wrap_at_256(ctx):
+    """300 fractions integrated into one frame: a pixel hit by all of them
+    reads 300 mod 256, as the reference's unsigned-byte accumulation."""
+    import eer_oracle as E
+    cam_w, cam_h = 64, 16
+    streams = [E.encode_7bit([(7, 0), (9 + (k % 3), 0)], cam_w * cam_h) for k in range(300)]
+    off = np.cumsum([0] + [len(s) for s in streams]).astype(np.int64)
+    data = np.concatenate(streams)
+    frames = ctx.empty((1, cam_h, cam_w), np.float32)
+    ctx.load_eer(data, off, cam_w, cam_h, 7, 1, frames, int_start=[0], int_size=[300])
+    got = frames.numpy()[0]
+    assert got[0, 7] == 300 % 256
+    assert got[0, 9] == 100 and got[0, 10] == 100 and got[0, 11] == 100
+    assert np.array_equal(got, E.render(data, off, cam_w, cam_h, 7, 1, [0], [300])[0].astype(np.float32))
+
+
+def test_eer_file_through_the_path(ctx, tmp_path):
+    """An EER file (TIFF container, 7-bit codes) of a drifting synthetic
+    movie: read, rendered with a frame-integration table asynchronously, then
+    aligned; the alignment equals the one of the oracle-rendered frames."""
+    import eer_oracle as E
+    import tiffw
+    rng = np.random.default_rng(21)
+    W = H = 512
+    nfrac, per = 40, 16
+    truth = MovieRecipe(W, H, nfrac, 300, seed=77, dose=4.0, local_max=0.0).movie()
+    streams = []
+    for k in range(nfrac):
+        p = truth[k].ravel().astype(np.float64)
+        p /= p.sum()
+        q = np.unique(rng.choice(W * H, size=W * H // per, p=p))
+        streams.append(E.encode_7bit([(int(v), 0) for v in q], W * H))
+    path = str(tmp_path / "m.eer")
+    tiffw.write(path, [dict(w=W, h=H, bits=1, comp=65001, rps=H, blocks=[s.tobytes()])
+                       for s in streams])
+    info = L.eer_info(path)
+    assert (info["cam_w"], info["cam_h"], info["n"], info["bits"]) == (W, H, nfrac, 7)
+    data, off = L.eer_read(path)
+    ist, isz = list(range(0, nfrac, 4)), [4] * (nfrac // 4)
+    host = ctx.host_array((len(data),), np.uint8)
+    host.a[...] = data
+    try:
+        frames = ctx.empty((len(isz), H, W), np.float32)
+        ctx.load_eer(host, off, W, H, 7, 1, frames, async_=True, int_start=ist, int_size=isz)
+        prm = L.default_params(patch=(0, 0), pix_size=1.0, kv=300, fm_dose=1.0)
+        got = ctx.align_movie(frames, prm)
+        ref_frames = E.render(data, off, W, H, 7, 1, ist, isz).astype(np.float32)
+        ref = ctx.align_movie(ctx.upload(ref_frames), prm)
+        assert np.array_equal(got["global_shift"], ref["global_shift"])
+        for a, b in zip(got["sums"], ref["sums"]):
+            assert np.array_equal(a.numpy(), b.numpy())
+    finally:
+        host.free()
+
+
+def test_tiff_file_through_the_path(ctx, tmp_path):
+    """An 8-bit LZW TIFF movie (Pillow's libtiff writer) decoded by
+    mc_tiff_read into pinned memory, loaded with a gain reference and
+    aligned: the same result as the flipped frames loaded from memory."""
+    from PIL import Image
+    W = H = 512
+    N = 8
+    mv = MovieRecipe(W, H, N, 300, seed=31, dose=4.0, local_max=0.0).movie()
+    fr = np.clip(mv, 0, 255).astype(np.uint8)
+    path = str(tmp_path / "m.tif")
+    ims = [Image.fromarray(f) for f in fr]
+    ims[0].save(path, save_all=True, append_images=ims[1:], compression="tiff_lzw")
+    info = L.tiff_info(path)
+    assert (info["sx"], info["sy"], info["n"], info["mode"]) == (W, H, N, 0)
+    host = ctx.host_array((N, H, W), np.uint8)
+    gain = np.random.default_rng(2).uniform(0.95, 1.05, (H, W)).astype(np.float32)
+    ctx.set_refs(gain, None)
+    try:
+        L.tiff_read(path, out=host)
+        assert np.array_equal(host.a, fr[:, ::-1])
+        a = ctx.empty((N, H, W), np.float32)
+        b = ctx.empty((N, H, W), np.float32)
+        ctx.load_frames(host, 0, W, H, N, a)
+        ctx.load_frames(np.ascontiguousarray(fr[:, ::-1]), 0, W, H, N, b)
+        assert np.array_equal(a.numpy(), IO.apply_refs(fr[:, ::-1], gain, None, W, H))
+        prm = L.default_params(patch=(0, 0), pix_size=1.0, kv=300, fm_dose=1.0)
+        ra, rb = ctx.align_movie(a, prm), ctx.align_movie(b, prm)
+        assert np.array_equal(ra["global_shift"], rb["global_shift"])
+    finally:
+        ctx.set_refs(None, None)
+        host.free()
