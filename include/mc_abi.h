@@ -282,7 +282,11 @@ int mc_get_refs(mc_ctx* ctx, float* gain, float* dark, int* ref_w, int* ref_h);
  * on the context stream; async = 1 queues it on a separate ingest stream and
  * returns at once, so that the copy of the next movie overlaps the current
  * alignment.  An alignment whose frames an asynchronous load writes waits
- * for that load; mc_load_wait makes the context stream wait for all loads. */
+ * for that load; mc_load_wait makes the context stream wait for all loads.
+ * References of the stack frame size apply as they are; references f >= 2
+ * times smaller in both directions are enlarged by repeating every pixel
+ * f x f (CLoadRefs::AugmentRefs, CLoadRefs.cpp:231-266); other sizes are an
+ * error. */
 int mc_load_frames(mc_ctx* ctx, const mc_raw_stack* raw, float* frames, int W, int H,
                    long long pitch, long long frame_stride, int async);
 int mc_load_wait(mc_ctx* ctx);
@@ -340,7 +344,8 @@ typedef struct mc_eer_stack {
  * (EerUtil/CDecodeEerFrame.cpp:46-215) on the GPU: every rendered frame is
  * the unsigned-byte event count of its fractions at (cam x 1, 2 or 4) size,
  * then the gain/dark correction and centre crop of mc_load_frames (references
- * of the rendered size).  async as for mc_load_frames. */
+ * of the rendered size, or enlarged to it as in mc_load_frames: a camera-size
+ * gain for 2x or 4x rendering).  async as for mc_load_frames. */
 int mc_load_eer(mc_ctx* ctx, const mc_eer_stack* eer, float* frames, int W, int H, long long pitch,
                 long long frame_stride, int async);
 

@@ -80,6 +80,11 @@ struct mc_ctx {
     float* gain = nullptr;
     float* dark = nullptr;
     int ref_w = 0, ref_h = 0;
+    // references enlarged by an integer factor for larger frames
+    // (CLoadRefs::AugmentRefs), built on first use of that factor
+    float* gain_aug = nullptr;
+    float* dark_aug = nullptr;
+    int aug_fact = 0;
 };
 
 #ifndef MC_CHUNK_FRAMES
@@ -1330,8 +1335,8 @@ int mc_ctx_destroy(mc_ctx* c) {
     for (auto& l : c->loads) (void)hipEventDestroy(l.ev);
     for (hipEvent_t e : c->free_ev) (void)hipEventDestroy(e);
     for (auto& t : c->tab_keep) (void)hipEventDestroy(t.first);
-    if (c->gain) (void)hipFree(c->gain);
-    if (c->dark) (void)hipFree(c->dark);
+    for (float* p : {c->gain, c->dark, c->gain_aug, c->dark_aug})
+        if (p) (void)hipFree(p);
     if (c->own_stream) (void)hipStreamDestroy(c->stream);
     delete c;
     return MC_OK;
@@ -2022,12 +2027,13 @@ int mc_set_refs(mc_ctx* c, const float* gain, const float* dark, int rw, int rh,
     MC_TRY(hipSetDevice(c->device));
     if (c->s_in) MC_TRY(hipStreamSynchronize(c->s_in));
     MC_TRY(hipStreamSynchronize(c->stream));
-    for (float** p : {&c->gain, &c->dark})
+    for (float** p : {&c->gain, &c->dark, &c->gain_aug, &c->dark_aug})
         if (*p) {
             MC_TRY(hipFree(*p));
             *p = nullptr;
         }
     c->ref_w = c->ref_h = 0;
+    c->aug_fact = 0;
     if (!gain && !dark) return MC_OK;
     const size_t n = (size_t)rw * rh;
     std::vector<float> g, d;
@@ -2074,6 +2080,49 @@ int mc_get_refs(mc_ctx* c, float* gain, float* dark, int* rw, int* rh) {
     return MC_OK;
 }
 
+namespace {
+
+// The gain and dark references for sx x sy stack frames: the references
+// themselves at their own size; for frames an integer factor f >= 2 larger
+// in both directions, copies with every reference pixel repeated f x f
+// (CLoadRefs::AugmentRefs, CLoadRefs.cpp:231-266, MrcUtil/GAugmentRef.cu).
+int refs_for(mc_ctx* c, long long sx, long long sy, const float** g, const float** d) {
+    *g = c->gain;
+    *d = c->dark;
+    if ((!c->gain && !c->dark) || (sx == c->ref_w && sy == c->ref_h)) return MC_OK;
+    const long long f = sx / c->ref_w;
+    if (f < 2 || sx != f * c->ref_w || sy != f * c->ref_h)
+        return fail(c, MC_E_ARG, "gain/dark reference size is not the frame size or an integer "
+                                 "fraction of it (CLoadRefs::AugmentRefs)");
+    if (c->aug_fact != f) {
+        // earlier loads may still read the previous copies
+        if (c->s_in) MC_TRY(hipStreamSynchronize(c->s_in));
+        MC_TRY(hipStreamSynchronize(c->stream));
+        for (float** p : {&c->gain_aug, &c->dark_aug})
+            if (*p) {
+                MC_TRY(hipFree(*p));
+                *p = nullptr;
+            }
+        c->aug_fact = 0;
+        const size_t n = (size_t)(sx * sy);
+        if (c->gain) {
+            MC_TRY(hipMalloc(&c->gain_aug, sizeof(float) * n));
+            MC_TRY(mc_launch_augment_ref(c->gain, c->ref_w, c->ref_h, (int)f, c->gain_aug, c->stream));
+        }
+        if (c->dark) {
+            MC_TRY(hipMalloc(&c->dark_aug, sizeof(float) * n));
+            MC_TRY(mc_launch_augment_ref(c->dark, c->ref_w, c->ref_h, (int)f, c->dark_aug, c->stream));
+        }
+        MC_TRY(hipStreamSynchronize(c->stream));
+        c->aug_fact = (int)f;
+    }
+    *g = c->gain_aug;
+    *d = c->dark_aug;
+    return MC_OK;
+}
+
+}  // namespace
+
 int mc_load_frames(mc_ctx* c, const mc_raw_stack* r, float* frames, int W, int H, long long pitch,
                    long long frame_stride, int async) {
     if (!c || !r || !r->data || !frames) return MC_E_ARG;
@@ -2082,9 +2131,10 @@ int mc_load_frames(mc_ctx* c, const mc_raw_stack* r, float* frames, int W, int H
     if (r->sx < 1 || r->sy < 1 || r->n < 1 || W < 1 || H < 1 || W > r->sx || H > r->sy ||
         pitch < W || frame_stride < pitch * H)
         return fail(c, MC_E_ARG, "invalid raw stack or frame geometry");
-    if ((c->gain || c->dark) && (c->ref_w != r->sx || c->ref_h != r->sy))
-        return fail(c, MC_E_ARG, "gain/dark reference size differs from the stack frame size");
     MC_TRY(hipSetDevice(c->device));
+    const float* rg = nullptr;
+    const float* rdk = nullptr;
+    MC_CHECK(refs_for(c, r->sx, r->sy, &rg, &rdk));
     if (async && !c->s_in) MC_TRY(hipStreamCreate(&c->s_in));
     hipStream_t s = async ? c->s_in : c->stream;
     if (async) {
@@ -2155,8 +2205,8 @@ int mc_load_frames(mc_ctx* c, const mc_raw_stack* r, float* frames, int W, int H
     a.y0 = (r->sy - H) / 2;
     a.W = W;
     a.H = H;
-    a.gain = c->gain;
-    a.dark = c->dark;
+    a.gain = rg;
+    a.dark = rdk;
     a.out = frames;
     a.pitch = pitch;
     a.frame_stride = frame_stride;
@@ -2333,8 +2383,6 @@ int mc_load_eer(mc_ctx* c, const mc_eer_stack* e, float* frames, int W, int H, l
     if (fw * fh >= (1LL << 31)) return fail(c, MC_E_ARG, "rendered EER frame too large");
     if (W < 1 || H < 1 || W > fw || H > fh || pitch < W || frame_stride < pitch * H)
         return fail(c, MC_E_ARG, "invalid frame geometry");
-    if ((c->gain || c->dark) && (c->ref_w != fw || c->ref_h != fh))
-        return fail(c, MC_E_ARG, "gain/dark reference size differs from the rendered EER size");
     const int nout = e->n_int > 0 ? e->n_int : e->n;
     std::vector<int> blob;  // offsets (two ints each), then (fraction, frame) jobs
     blob.resize(2 * (size_t)(e->n + 1));
@@ -2351,6 +2399,9 @@ int mc_load_eer(mc_ctx* c, const mc_eer_stack* e, float* frames, int W, int H, l
     }
     const int njob = (int)(blob.size() / 2) - (e->n + 1);
     MC_TRY(hipSetDevice(c->device));
+    const float* rg = nullptr;
+    const float* rdk = nullptr;
+    MC_CHECK(refs_for(c, fw, fh, &rg, &rdk));
     if (async && !c->s_in) MC_TRY(hipStreamCreate(&c->s_in));
     hipStream_t s = async ? c->s_in : c->stream;
     if (async) {
@@ -2412,8 +2463,8 @@ int mc_load_eer(mc_ctx* c, const mc_eer_stack* e, float* frames, int W, int H, l
     a.y0 = (int)((fh - H) / 2);
     a.W = W;
     a.H = H;
-    a.gain = c->gain;
-    a.dark = c->dark;
+    a.gain = rg;
+    a.dark = rdk;
     a.out = frames;
     a.pitch = pitch;
     a.frame_stride = frame_stride;

@@ -36,5 +36,7 @@ struct EerArgs {
 long long mc_raw_row_bytes(int mode, int sx);
 hipError_t mc_launch_apply_refs(const IngestArgs& a, int mode, hipStream_t s);
 hipError_t mc_launch_eer_render(const EerArgs& a, hipStream_t s);
+// w x h reference -> (w f) x (h f), every pixel repeated f x f
+hipError_t mc_launch_augment_ref(const float* in, int w, int h, int f, float* out, hipStream_t s);
 // n 32-bit zeros by a kernel (no memset node if the caller captures a graph)
 hipError_t mc_launch_zero_u32(unsigned int* p, long long n, hipStream_t s);

@@ -170,6 +170,17 @@ __global__ __launch_bounds__(64) void k_eer_render(EerArgs a) {
     }
 }
 
+// GAugmentRef (MrcUtil/GAugmentRef.cu): out[y][x] = in[y / f][x / f].
+__global__ void k_augment_ref(const float* __restrict__ in, int w, int h, int f, float* __restrict__ out) {
+    const long long ow = (long long)w * f;
+    const long long n = ow * h * f;
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (long long)gridDim.x * blockDim.x) {
+        const long long y = i / ow, x = i - y * ow;
+        out[i] = in[(y / f) * w + x / f];
+    }
+}
+
 __global__ void k_fill_u32(unsigned int* p, long long n) {
     for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
          i += (long long)gridDim.x * blockDim.x)
@@ -177,6 +188,14 @@ __global__ void k_fill_u32(unsigned int* p, long long n) {
 }
 
 }  // namespace
+
+hipError_t mc_launch_augment_ref(const float* in, int w, int h, int f, float* out, hipStream_t s) {
+    if (w < 1 || h < 1 || f < 1) return hipErrorInvalidValue;
+    const long long n = (long long)w * h * f * f;
+    const long long blocks = std::min<long long>((n + 255) / 256, 16384);
+    hipLaunchKernelGGL(k_augment_ref, dim3((unsigned int)blocks), dim3(256), 0, s, in, w, h, f, out);
+    return hipGetLastError();
+}
 
 hipError_t mc_launch_zero_u32(unsigned int* p, long long n, hipStream_t s) {
     if (n < 1) return hipSuccess;

@@ -362,3 +362,43 @@ def test_tiff_file_through_the_path(ctx, tmp_path):
     finally:
         ctx.set_refs(None, None)
         host.free()
+
+
+@pytest.mark.parametrize("f", [2, 4])
+def test_references_enlarged_for_larger_frames(ctx, f):
+    """A gain and dark f times smaller than the frames are enlarged by
+    repeating each pixel f x f (CLoadRefs::AugmentRefs), for MRC frames and
+    for EER frames rendered at super-resolution from a camera-size gain."""
+    import eer_oracle as E
+    rng = np.random.default_rng(40 + f)
+    rh, rw = 12, 20
+    gain = rng.uniform(0.9, 1.1, (rh, rw)).astype(np.float32)
+    dark = rng.uniform(0.0, 2.0, (rh, rw)).astype(np.float32)
+    big_g = np.repeat(np.repeat(gain, f, 0), f, 1)
+    big_d = np.repeat(np.repeat(dark, f, 0), f, 1)
+    sy, sx, n = rh * f, rw * f, 3
+    raw = rng.integers(0, 255, (n, sy, sx)).astype(np.uint8)
+    ctx.set_refs(gain, dark)
+    try:
+        out = ctx.empty((n, sy - 2, sx - 4), np.float32)
+        ctx.load_frames(raw, 0, sx, sy, n, out)
+        assert np.array_equal(out.numpy(), IO.apply_refs(raw, big_g, big_d, sx - 4, sy - 2))
+        # a size that is not an integer multiple is refused
+        with pytest.raises(RuntimeError):
+            ctx.load_frames(raw[:, :, :sx - 1].copy(), 0, sx - 1, sy, n, ctx.empty((n, sy, sx - 1), np.float32))
+    finally:
+        ctx.set_refs(None, None)
+    # EER: camera 4096 x 3, gain of camera size, rendering at 2x / 4x
+    cam_w, cam_h = 4096, 3
+    sampling = 2 if f == 2 else 3
+    cg = rng.uniform(0.9, 1.1, (cam_h, cam_w)).astype(np.float32)
+    data, off = _eer_movie(rng, cam_w, cam_h, 3, 7, 500)
+    want = E.render(data, off, cam_w, cam_h, 7, sampling)
+    ctx.set_refs(cg, None)
+    try:
+        fr = ctx.empty((3, cam_h * f, cam_w * f), np.float32)
+        ctx.load_eer(data, off, cam_w, cam_h, 7, sampling, fr)
+        exp = IO.apply_refs(want, np.repeat(np.repeat(cg, f, 0), f, 1), None, cam_w * f, cam_h * f)
+        assert np.array_equal(fr.numpy(), exp)
+    finally:
+        ctx.set_refs(None, None)
