@@ -63,6 +63,8 @@ struct mc_ctx {
     // second stream for the frame-chunk loops, and its fork/join events
     hipStream_t s2 = nullptr;
     hipEvent_t evf = nullptr, evj = nullptr;
+    // end of a local-correction inverse queued early on the second stream
+    hipEvent_t evi = nullptr;
     // learned prior for guided patch alignment (not owned)
     mc_prior* prior = nullptr;
     // ingest: stream and completion event of asynchronous frame loads, and
@@ -724,6 +726,28 @@ int enqueue_forward(mc_ctx* c, Fft2D* pf, const mc_movie* const* mvs, int count,
     return MC_OK;
 }
 
+// Align/CFullAlign.cpp:160-163,96: global shifts in full-resolution pixels
+// (xcf bin multiply), truncated toward zero at 0.01 px.
+void global_shifts(const std::vector<double>& xs, const Geo& g, int N, double* out) {
+    for (int i = 0; i < N; i++) {
+        const float sx = (float)xs[2 * i] * g.xbin[0];
+        const float sy = (float)xs[2 * i + 1] * g.xbin[1];
+        out[2 * i] = 0.01 * (double)(long long)((double)sx * 100.0);
+        out[2 * i + 1] = 0.01 * (double)(long long)((double)sy * 100.0);
+    }
+}
+
+// Align/CPatchAlign.cpp:47-65: shifts relative to the reference frame
+// (-FmRef, 1-based; the middle frame when negative).
+void relative_to_ref(double* sh, int N, int fm_ref) {
+    const int ref = (fm_ref < 0) ? N / 2 : std::min(std::max(fm_ref - 1, 0), N - 1);
+    const double rx = sh[2 * ref], ry = sh[2 * ref + 1];
+    for (int i = 0; i < N; i++) {
+        sh[2 * i] -= rx;
+        sh[2 * i + 1] -= ry;
+    }
+}
+
 // Movie pipelining across align_batch calls of one geometry: the stacks of
 // the movie being aligned live in buffer slot `slot`; when `ready`, its
 // spectra were transformed by the previous call on the second stream and
@@ -907,6 +931,7 @@ int align_batch(mc_ctx* c, const mc_params* const* pp_, const mc_movie* const* m
     }
     const double tol = p->tol / std::max(g.xbin[0], g.xbin[1]);
     const auto ta = clk::now();
+    bool early_inverse = false;
 
     if (!replay) {
         Groups g0 = make_groups(p->group[0], N, p->throw0, p->fm_int_start, p->fm_int_size);
@@ -920,6 +945,42 @@ int align_batch(mc_ctx* c, const mc_params* const* pp_, const mc_movie* const* m
                               go.errors.begin() + (size_t)m * p->iter + ms[m].giters);
             ms[m].xs.assign(go.shifts.begin() + (size_t)m * N * 2,
                             go.shifts.begin() + (size_t)(m + 1) * N * 2);
+        }
+        // The local-correction inverse of a single movie needs only the
+        // global shifts (relative to the reference frame): queue it on the
+        // second stream now, so that its HBM-bound passes run beside the
+        // latency-bound patch alignment instead of after it.  The correction
+        // below then waits for it.  Same kernels and inputs, so the same
+        // results.
+        if (patch && count == 1 && two) {
+            std::vector<double> rel(2 * N);
+            global_shifts(ms[0].xs, g, N, rel.data());
+            relative_to_ref(rel.data(), N, p->fm_ref);
+            MC_BUF(gsh_e, double, "gsh", 2 * N);
+            MC_BUF(fA_e, cfloat, "fphA", (size_t)N * g.cx);
+            MC_BUF(fB_e, cfloat, "fphB", (size_t)N * H);
+            MC_BUF(Rr_e, float, "Rr", (size_t)N * W * H);
+            // upload and phasors on the idle context stream; a pageable copy
+            // on the busy second stream could hold the host until it drains
+            MC_TRY(hipMemcpyAsync(gsh_e, rel.data(), sizeof(double) * 2 * N, hipMemcpyHostToDevice, s));
+            MC_TRY(mc_launch_phasors(gsh_e, -1.0, 1, nullptr, N, W, g.cx, H, N, fA_e, fB_e, s));
+            MC_TRY(fork());
+            const long long rfs = (long long)W * H;
+            for (int f0 = 0; f0 < N; f0 += CH) {
+                const int nb = std::min(CH, N - f0);
+                PhasorTabs ph{fA_e + (long long)f0 * g.cx, fB_e + (long long)f0 * H};
+                {
+                    KTimer kt(c, "fft_cols_inv", c->s2);
+                    MC_TRY(mc_cols_inv(*pf, Sf_all + (long long)f0 * fs, fs, g.cx, H, &ph, T2,
+                                       pf->t_elems, nb, c->s2));
+                }
+                {
+                    KTimer kt(c, "fft_rows_c2r", c->s2);
+                    MC_TRY(mc_rows_c2r(*pf, T2, pf->t_elems, Rr_e + f0 * rfs, W, rfs, nb, 0, c->s2));
+                }
+            }
+            MC_TRY(hipEventRecord(c->evi, c->s2));
+            early_inverse = true;
         }
         if (patch) {
             const long long pfs = (long long)g.pcx * g.ph;
@@ -1077,16 +1138,7 @@ int align_batch(mc_ctx* c, const mc_params* const* pp_, const mc_movie* const* m
             MC_TRY(hipStreamSynchronize(s));
           }
         }
-        // Align/CFullAlign.cpp:160-163,96: xcf bin multiply, truncation to 0.01 px
-        for (int m = 0; m < count; m++) {
-            const std::vector<double>& xs = ms[m].xs;
-            for (int i = 0; i < N; i++) {
-                const float sx = (float)xs[2 * i] * g.xbin[0];
-                const float sy = (float)xs[2 * i + 1] * g.xbin[1];
-                ms[m].gshift[2 * i] = 0.01 * (double)(long long)((double)sx * 100.0);
-                ms[m].gshift[2 * i + 1] = 0.01 * (double)(long long)((double)sy * 100.0);
-            }
-        }
+        for (int m = 0; m < count; m++) global_shifts(ms[m].xs, g, N, ms[m].gshift.data());
     } else {
         MovieState& q0 = ms[0];
         for (int i = 0; i < 2 * N; i++) q0.gshift[i] = ov->replay_global[i];
@@ -1124,19 +1176,16 @@ int align_batch(mc_ctx* c, const mc_params* const* pp_, const mc_movie* const* m
             MC_TRY(hipStreamSynchronize(s));
         }
         if (patch) {
-            // Align/CPatchAlign.cpp:47-65: shifts relative to the reference frame
-            const int ref = (p->fm_ref < 0) ? N / 2 : std::min(std::max(p->fm_ref - 1, 0), N - 1);
-            const double rx = gshift[2 * ref], ry = gshift[2 * ref + 1];
-            for (int i = 0; i < N; i++) {
-                gshift[2 * i] -= rx;
-                gshift[2 * i + 1] -= ry;
-            }
-            MC_TRY(hipMemcpyAsync(gsh, gshift.data(), sizeof(double) * 2 * N, hipMemcpyHostToDevice, s));
-            MC_TRY(mc_launch_phasors(gsh, -1.0, 1, nullptr, N, W, g.cx, H, N, fA, fB, s));
+            relative_to_ref(gshift.data(), N, p->fm_ref);
             // Correct/GCorrectPatchShift.cu:214-285: C2R, warp, normalised R2C
             MC_BUF(Rr, float, "Rr", (size_t)N * W * H);
             MC_BUF(Rw, float, "Rw", (size_t)N * W * H);
             const long long rfs = (long long)W * H;
+            if (early_inverse) {
+                MC_TRY(hipStreamWaitEvent(s, c->evi, 0));
+            } else {
+            MC_TRY(hipMemcpyAsync(gsh, gshift.data(), sizeof(double) * 2 * N, hipMemcpyHostToDevice, s));
+            MC_TRY(mc_launch_phasors(gsh, -1.0, 1, nullptr, N, W, g.cx, H, N, fA, fB, s));
             MC_TRY(fork());
             for (int f0 = 0; f0 < N; f0 += CH) {
                 const int nb = std::min(CH, N - f0);
@@ -1154,6 +1203,7 @@ int align_batch(mc_ctx* c, const mc_params* const* pp_, const mc_movie* const* m
                 }
             }
             MC_TRY(join());
+            }
             std::vector<float> psf(ps.size()), pcf(pc.size());
             for (size_t i = 0; i < ps.size(); i++) psf[i] = (float)ps[i];
             for (size_t i = 0; i < pc.size(); i++) pcf[i] = (float)pc[i];
@@ -1296,7 +1346,8 @@ int mc_ctx_create(int device, void* stream, mc_ctx** out) {
     if (mc_fft_init_constants() != 0 || mc_align_init_constants() != 0 ||
         hipStreamCreate(&c->s2) != hipSuccess ||
         hipEventCreateWithFlags(&c->evf, hipEventDisableTiming) != hipSuccess ||
-        hipEventCreateWithFlags(&c->evj, hipEventDisableTiming) != hipSuccess) {
+        hipEventCreateWithFlags(&c->evj, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&c->evi, hipEventDisableTiming) != hipSuccess) {
         if (c->s2) (void)hipStreamDestroy(c->s2);
         if (c->own_stream) (void)hipStreamDestroy(c->stream);
         delete c;
@@ -1328,6 +1379,7 @@ int mc_ctx_destroy(mc_ctx* c) {
     }
     if (c->evf) (void)hipEventDestroy(c->evf);
     if (c->evj) (void)hipEventDestroy(c->evj);
+    if (c->evi) (void)hipEventDestroy(c->evi);
     if (c->s_in) {
         (void)hipStreamSynchronize(c->s_in);
         (void)hipStreamDestroy(c->s_in);
