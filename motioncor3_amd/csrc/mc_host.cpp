@@ -63,8 +63,11 @@ struct mc_ctx {
     // second stream for the frame-chunk loops, and its fork/join events
     hipStream_t s2 = nullptr;
     hipEvent_t evf = nullptr, evj = nullptr;
-    // end of a local-correction inverse queued early on the second stream
+    // end of a local-correction inverse queued early on the second stream,
+    // and the pinned host copy of its shifts
     hipEvent_t evi = nullptr;
+    double* pin_shifts = nullptr;
+    size_t pin_shifts_n = 0;
     // learned prior for guided patch alignment (not owned)
     mc_prior* prior = nullptr;
     // ingest: stream and completion event of asynchronous frame loads, and
@@ -947,12 +950,13 @@ int align_batch(mc_ctx* c, const mc_params* const* pp_, const mc_movie* const* m
                             go.shifts.begin() + (size_t)(m + 1) * N * 2);
         }
         // The local-correction inverse of a single movie needs only the
-        // global shifts (relative to the reference frame): queue it on the
-        // second stream now, so that its HBM-bound passes run beside the
-        // latency-bound patch alignment instead of after it.  The correction
-        // below then waits for it.  Same kernels and inputs, so the same
-        // results.
-        if (patch && count == 1 && two) {
+        // global shifts (relative to the reference frame): it is queued on
+        // the second stream once the patch preparation is queued (its 50
+        // launches then cost host time while the GPU is busy), so that its
+        // HBM-bound passes run beside the latency-bound patch alignment
+        // instead of after it.  The correction below waits for it.  Same
+        // kernels and inputs, so the same results.
+        auto early_inv = [&]() -> int {
             std::vector<double> rel(2 * N);
             global_shifts(ms[0].xs, g, N, rel.data());
             relative_to_ref(rel.data(), N, p->fm_ref);
@@ -960,11 +964,23 @@ int align_batch(mc_ctx* c, const mc_params* const* pp_, const mc_movie* const* m
             MC_BUF(fA_e, cfloat, "fphA", (size_t)N * g.cx);
             MC_BUF(fB_e, cfloat, "fphB", (size_t)N * H);
             MC_BUF(Rr_e, float, "Rr", (size_t)N * W * H);
-            // upload and phasors on the idle context stream; a pageable copy
-            // on the busy second stream could hold the host until it drains
-            MC_TRY(hipMemcpyAsync(gsh_e, rel.data(), sizeof(double) * 2 * N, hipMemcpyHostToDevice, s));
-            MC_TRY(mc_launch_phasors(gsh_e, -1.0, 1, nullptr, N, W, g.cx, H, N, fA_e, fB_e, s));
-            MC_TRY(fork());
+            // everything on the second stream, behind this movie's forward
+            // chunks (or its prefetch); the shifts go through pinned host
+            // memory, so the copy does not hold the host while the stream
+            // is busy; the previous copy out of it has run once its
+            // inverse's event has
+            MC_TRY(hipEventSynchronize(c->evi));
+            if (c->pin_shifts_n < (size_t)(2 * N)) {
+                if (c->pin_shifts) MC_TRY(hipHostFree(c->pin_shifts));
+                c->pin_shifts = nullptr;
+                c->pin_shifts_n = 0;
+                MC_TRY(hipHostMalloc((void**)&c->pin_shifts, sizeof(double) * 2 * N));
+                c->pin_shifts_n = (size_t)(2 * N);
+            }
+            std::copy(rel.begin(), rel.end(), c->pin_shifts);
+            MC_TRY(hipMemcpyAsync(gsh_e, c->pin_shifts, sizeof(double) * 2 * N, hipMemcpyHostToDevice,
+                                  c->s2));
+            MC_TRY(mc_launch_phasors(gsh_e, -1.0, 1, nullptr, N, W, g.cx, H, N, fA_e, fB_e, c->s2));
             const long long rfs = (long long)W * H;
             for (int f0 = 0; f0 < N; f0 += CH) {
                 const int nb = std::min(CH, N - f0);
@@ -981,7 +997,8 @@ int align_batch(mc_ctx* c, const mc_params* const* pp_, const mc_movie* const* m
             }
             MC_TRY(hipEventRecord(c->evi, c->s2));
             early_inverse = true;
-        }
+            return MC_OK;
+        };
         if (patch) {
             const long long pfs = (long long)g.pcx * g.ph;
             MC_BUF(Tp, cfloat, "Tp", (size_t)P * N * pp->t_elems);
@@ -1038,6 +1055,7 @@ int align_batch(mc_ctx* c, const mc_params* const* pp_, const mc_movie* const* m
             if (fpch != MC_SPEC_F32) MC_TRY(mc_launch_spec_scale(dcp, g.ph, P * N, gp + pm, ip + pm, s));
             MC_TRY(mc_cols_fwd(*pp, Tp, pp->t_elems, spec_at(Sp_all, pm * pfs, fpch), pfs, nullptr, 0,
                                0, 0, P * N, s, fpch, gp ? gp + pm : nullptr));
+            if (count == 1 && two) MC_CHECK(early_inv());
           }
             Groups g1 = make_groups(p->group[1], N, p->throw0, p->fm_int_start, p->fm_int_size);
             // Learned prior: features of every patch after its first iteration
@@ -1380,6 +1398,7 @@ int mc_ctx_destroy(mc_ctx* c) {
     if (c->evf) (void)hipEventDestroy(c->evf);
     if (c->evj) (void)hipEventDestroy(c->evj);
     if (c->evi) (void)hipEventDestroy(c->evi);
+    if (c->pin_shifts) (void)hipHostFree(c->pin_shifts);
     if (c->s_in) {
         (void)hipStreamSynchronize(c->s_in);
         (void)hipStreamDestroy(c->s_in);
