@@ -150,6 +150,10 @@ def main():
                     help="also time K movies delivered as 8-bit raw frames in pinned host memory "
                          "(gain-corrected on the GPU by mc_load_frames), each load overlapping "
                          "the previous movie's alignment")
+    ap.add_argument("--eer", type=int, default=0, metavar="K",
+                    help="also time rendering a synthetic EER movie of K fractions (4096^2 camera, "
+                         "about 100k electrons per fraction, 10 fractions per frame) with "
+                         "mc_load_eer from pinned host memory")
     ap.add_argument("--prior", type=int, default=0, metavar="K",
                     help="train the learned shift prior on K synthetic movies of the workload "
                          "(labels from the plain path), then time the guided path")
@@ -319,6 +323,39 @@ def main():
             hk.free()
         del bufs
 
+    eer = None
+    if args.eer > 0:
+        from motioncor3_amd.synth import eer_stream_7bit
+        cw = ch = 4096
+        nfr, per, rng = args.eer, 100000, np.random.default_rng(5)
+        streams, nev = [], 0
+        for k in range(nfr):
+            q = np.unique(rng.integers(0, cw * ch, per))
+            streams.append(eer_stream_7bit(q, rng.integers(0, 16, len(q)), cw * ch))
+            nev += len(q)
+        off = np.cumsum([0] + [len(x) for x in streams]).astype(np.int64)
+        host = ctx.host_array((int(off[-1]),), np.uint8)
+        host.a[...] = np.concatenate(streams)
+        del streams
+        nout = nfr // 10
+        ist, isz = [10 * k for k in range(nout)], [10] * nout
+        efr = ctx.empty((nout, ch, cw), np.float32)
+        ctx.load_eer(host, off, cw, ch, 7, 1, efr, int_start=ist, int_size=isz)  # warm-up
+        reps = 3
+        ctx.sync()
+        te = time.perf_counter()
+        for _ in range(reps):
+            ctx.load_eer(host, off, cw, ch, 7, 1, efr, int_start=ist, int_size=isz)
+        ctx.sync()
+        e_ms = (time.perf_counter() - te) * 1000.0 / reps
+        eer = {"fractions": nfr, "camera": [cw, ch], "electrons": nev, "code_bytes": int(off[-1]),
+               "rendered_frames": nout, "ms_per_movie": e_ms,
+               "electrons_per_s": nev / (e_ms * 1e-3),
+               "note": "7-bit codes in pinned host memory; the time includes their DMA, "
+                       "the rendering (one thread per fraction) and the byte-count frames"}
+        host.free()
+        del efr
+
     gs = res["global_shift"]
     d = recipe.displacement
     e = (gs - gs.mean(0)) + (d - d.mean(0))
@@ -374,6 +411,8 @@ def main():
     }
     if ingest is not None:
         out["ingest"] = ingest
+    if eer is not None:
+        out["eer"] = eer
     if prior_info is not None:
         prior_info["fallbacks_last_step"] = res["prior_fallbacks"]
         prior_info["patch_iters_mean_last_step"] = float(np.mean(res["patch_iters"]))

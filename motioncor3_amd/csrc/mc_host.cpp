@@ -2455,20 +2455,35 @@ int mc_load_eer(mc_ctx* c, const mc_eer_stack* e, float* frames, int W, int H, l
     if (W < 1 || H < 1 || W > fw || H > fh || pitch < W || frame_stride < pitch * H)
         return fail(c, MC_E_ARG, "invalid frame geometry");
     const int nout = e->n_int > 0 ? e->n_int : e->n;
-    std::vector<int> blob;  // offsets (two ints each), then (fraction, frame) jobs
-    blob.resize(2 * (size_t)(e->n + 1));
+    // offsets (two ints each), then (fraction, frame, first segment) per job,
+    // then the job of every segment
+    std::vector<int> blob(2 * (size_t)(e->n + 1));
     memcpy(blob.data(), e->offsets, sizeof(long long) * (size_t)(e->n + 1));
+    std::vector<int> jobs, segjob;
     for (int o = 0; o < nout; o++) {
         const int st = e->n_int > 0 ? (e->int_start ? e->int_start[o] : -1) : o;
         const int sz = e->n_int > 0 ? (e->int_size ? e->int_size[o] : -1) : 1;
         if (st < 0 || sz < 1 || st + sz > e->n)
             return fail(c, MC_E_ARG, "EER integration table exceeds the fractions");
         for (int k = 0; k < sz; k++) {
-            blob.push_back(st + k);
-            blob.push_back(o);
+            const long long nb = e->offsets[st + k + 1] - e->offsets[st + k];
+            const long long ns = std::max<long long>(
+                1, e->bits == 7 ? (8 * nb + MC_EER_SEG_BITS - 1) / MC_EER_SEG_BITS
+                                : (nb + MC_EER_SEG_BYTES8 - 1) / MC_EER_SEG_BYTES8);
+            if ((long long)segjob.size() + ns >= (1LL << 30))
+                return fail(c, MC_E_ARG, "EER movie too large");
+            const int jn = (int)(jobs.size() / 3);
+            jobs.push_back(st + k);
+            jobs.push_back(o);
+            jobs.push_back((int)segjob.size());
+            segjob.insert(segjob.end(), (size_t)ns, jn);
         }
     }
-    const int njob = (int)(blob.size() / 2) - (e->n + 1);
+    const int njob = (int)(jobs.size() / 3), nseg = (int)segjob.size();
+    const size_t jofs = blob.size();
+    blob.insert(blob.end(), jobs.begin(), jobs.end());
+    const size_t sofs = blob.size();
+    blob.insert(blob.end(), segjob.begin(), segjob.end());
     MC_TRY(hipSetDevice(c->device));
     const float* rg = nullptr;
     const float* rdk = nullptr;
@@ -2488,7 +2503,8 @@ int mc_load_eer(mc_ctx* c, const mc_eer_stack* e, float* frames, int W, int H, l
     int* dtab = (int*)buf(c, ("eer_tab" + sfx).c_str(), sizeof(int) * blob.size());
     unsigned int* cnt =
         (unsigned int*)buf(c, ("eer_counts" + sfx).c_str(), sizeof(unsigned int) * (size_t)(nout * fw * fh));
-    if (!din || !dtab || !cnt) return fail(c, MC_E_NOMEM, "EER staging");
+    int* dseg = (int*)buf(c, ("eer_seg" + sfx).c_str(), sizeof(int) * 2 * (size_t)nseg * (MC_EER_PHASES + 1));
+    if (!din || !dtab || !cnt || !dseg) return fail(c, MC_E_NOMEM, "EER staging");
     // the host tables stay alive until their copy has run
     std::vector<std::pair<hipEvent_t, std::vector<int>>> keep;
     for (auto& t : c->tab_keep) {
@@ -2508,8 +2524,12 @@ int mc_load_eer(mc_ctx* c, const mc_eer_stack* e, float* frames, int W, int H, l
     EerArgs ea;
     ea.data = din;
     ea.off = reinterpret_cast<const long long*>(dtab);
-    ea.job = dtab + 2 * (e->n + 1);
+    ea.job = dtab + jofs;
     ea.njob = njob;
+    ea.seg_job = dtab + sofs;
+    ea.nseg = nseg;
+    ea.tab = dseg;
+    ea.start = dseg + 2 * (size_t)nseg * MC_EER_PHASES;
     ea.cam_w = e->cam_w;
     ea.cam_h = e->cam_h;
     ea.bits = e->bits;

@@ -20,11 +20,21 @@ struct IngestArgs {
 // Raw mode of rendered EER frames: 32-bit event counters, read modulo 256.
 #define MC_MODE_EER_COUNTS 1032
 
+#define MC_EER_PHASES 11  // possible offsets (bits) of a segment's first 7-bit code
+// Segments of a fraction's codes: 2048 bits (7-bit codes) or 255 bytes
+// (8-bit codes, 85 code pairs); at least one per fraction.
+#define MC_EER_SEG_BITS 2048
+#define MC_EER_SEG_BYTES8 255
+
 struct EerArgs {
     const unsigned char* data;  // compressed EER fractions
     const long long* off;       // fraction k occupies data[off[k], off[k+1])
-    const int* job;             // (fraction, rendered frame) per thread
+    const int* job;             // (fraction, rendered frame, first segment) per job
     int njob;
+    const int* seg_job;         // job of every segment
+    int nseg;
+    int* tab;                   // per segment and start offset: (end offset, pixels advanced)
+    int* start;                 // per segment: (start offset, start pixel)
     int cam_w, cam_h;           // camera size
     int bits;                   // 7 (compression 65001) or 8 (65000)
     int sampling;               // 1 camera size, 2 and 3: 2x and 4x super-resolution

@@ -72,98 +72,210 @@ __global__ __launch_bounds__(256) void k_apply_refs(IngestArgs a) {
 }
 
 // ---------------------------------------------------------------------------
-// EER rendering (EerUtil/CDecodeEerFrame.cpp:46-215, CRenderMrcStack.cpp:36-101):
-// one thread per (EER fraction, rendered frame) job walks the fraction's
-// run-length codes in order and adds 1 to the rendered frame's counter of
-// every electron.  7-bit streams (compression 65001): a 7-bit zero run p,
-// then, unless p == 127, 4 sub-pixel bits; two codes are taken from one
-// 32-bit little-endian window.  8-bit streams (65000): three bytes hold two
-// 8-bit runs with their sub-pixel nibbles.  Sampling 1 renders at camera
-// size; 2 and 3 render 2x and 4x super-resolution from the sub-pixel bits
-// (s ^ 0x0A), with the reference's 4096-wide camera rows
+// EER rendering (EerUtil/CDecodeEerFrame.cpp:46-215, CRenderMrcStack.cpp:36-101).
+// A fraction is a chain of run codes: 7-bit streams (compression 65001) hold
+// a 7-bit zero run r and, unless r == 127, 4 sub-pixel bits (the reference
+// reads them two codes per 32-bit little-endian window, which parses the
+// same); 8-bit streams (65000) hold two 8-bit runs and their sub-pixel
+// nibbles in every three bytes.  Each electron adds 1 to its rendered
+// frame's counter at the pixel reached by the runs.  Sampling 1 renders at
+// camera size; 2 and 3 render 2x and 4x super-resolution from the sub-pixel
+// bits (s ^ 0x0A), with the reference's 4096-wide camera rows
 // (x = (pixel & 4095) << k, y = (pixel >> 12) << k).  The 8-bit
-// super-resolution walk reads its second code's high bits and sub-pixel
+// super-resolution walk reads its second run's high bits and sub-pixel
 // nibble from byte 0 of the fraction and takes the first code's sub-pixel
-// bits as byte1 & 0x0A, as the reference does.  Reads stop at the end of the
-// fraction and events outside the rendered frame are dropped (the reference
-// reads and writes past them).
-__device__ __forceinline__ unsigned int eer_byte(const unsigned char* p, long long k, long long nb) {
-    return k < nb ? p[k] : 0u;
+// bits as byte1 & 0x0A, as the reference does.
+//
+// The walk is split into segments of MC_EER_SEG_BITS bits (8-bit streams:
+// 255 bytes, 85 code pairs), decoded in three launches:
+//  1. k_eer_seg_table: for every segment and every possible offset of its
+//     first code (0..10 bits for 7-bit codes, none for 8-bit), the offset at
+//     which the walk leaves it and the pixels it advances;
+//  2. k_eer_scan: one thread per fraction chains the offsets and prefix-sums
+//     the advances, giving each segment its true start offset and pixel;
+//  3. k_eer_emit: every segment walks again from there and adds its
+//     electrons, stopping where the walk reaches the end of the camera.
+// Each thread copies its segment to its own slice of LDS first.  Reads stop
+// at the end of a fraction (bytes past it read as 0) and events outside the
+// rendered frame are dropped (the reference reads and writes past both,
+// which matters only for malformed streams).
+#define MC_EER_SLICE 67  // words per thread slice (odd: no LDS bank conflicts)
+
+struct EerSeg {
+    long long nb;       // fraction bytes
+    long long S;        // first logical bit of the segment
+    long long E;        // end bit (exclusive) of the segment's code starts
+    long long nbits;    // 8 * nb
+    int off0;           // bit of S within slice word 0
+    const unsigned int* sl;
+    unsigned int b0;    // byte 0 of the fraction (8-bit super-resolution)
+};
+
+// Segment `k` of fraction `fr`: its words to this thread's LDS slice.
+__device__ EerSeg eer_seg(const EerArgs& a, int fr, long long k, unsigned int* slice) {
+    EerSeg g;
+    const unsigned char* p = a.data + a.off[fr];
+    g.nb = a.off[fr + 1] - a.off[fr];
+    g.nbits = 8 * g.nb;
+    g.S = a.bits == 7 ? k * MC_EER_SEG_BITS : k * (8LL * MC_EER_SEG_BYTES8);
+    g.E = g.S + (a.bits == 7 ? MC_EER_SEG_BITS : 8LL * MC_EER_SEG_BYTES8);
+    const uintptr_t ad = (uintptr_t)p;
+    const int mis = (int)(ad & 3u);
+    const unsigned int* w = reinterpret_cast<const unsigned int*>(ad - mis);
+    const long long lim = g.nb + mis;  // physical end byte
+    const long long nw = (lim + 3) / 4;
+    const long long pb = g.S + 8LL * mis;  // physical bit of S
+    const long long w0 = pb >> 5;
+    g.off0 = (int)(pb & 31);
+#pragma unroll 8
+    for (int i = 0; i < MC_EER_SLICE; i++) {
+        const long long wi = w0 + i;
+        unsigned int v = 0u;
+        if (wi < nw) {
+            v = w[wi];
+            const long long b = lim - 4 * wi;
+            if (b < 4) v &= (1u << (8 * b)) - 1u;
+        }
+        slice[i] = v;
+    }
+    g.sl = slice;
+    g.b0 = g.nb > 0 ? p[0] : 0u;
+    return g;
 }
 
-__global__ __launch_bounds__(64) void k_eer_render(EerArgs a) {
-    const int i = blockIdx.x * 64 + threadIdx.x;
-    if (i >= a.njob) return;
-    const int fr = a.job[2 * i], o = a.job[2 * i + 1];
-    const unsigned char* p = a.data + a.off[fr];
-    const long long nb = a.off[fr + 1] - a.off[fr];
-    unsigned int* cnt = a.counts + (long long)o * a.fw * a.fh;
-    const unsigned int cam = (unsigned int)a.cam_w * (unsigned int)a.cam_h;
+// n (<= 24) bits at logical bit q (S <= q < E + 24)
+__device__ __forceinline__ unsigned int eer_bits(const EerSeg& g, long long q, int n) {
+    const int l = (int)(q - g.S) + g.off0;
+    const unsigned long long v =
+        ((unsigned long long)g.sl[(l >> 5) + 1] << 32 | g.sl[l >> 5]) >> (l & 31);
+    return (unsigned int)v & ((1u << n) - 1u);
+}
+
+__device__ __forceinline__ void eer_hit(const EerArgs& a, unsigned int* cnt, unsigned int num,
+                                        unsigned int sub) {
     const unsigned int fw = (unsigned int)a.fw, fh = (unsigned int)a.fh;
-    unsigned int and0 = 0, and1 = 0, sh0 = 0, sh1 = 0, sh2 = 0;
-    if (a.sampling == 2) { and0 = 2; and1 = 8; sh0 = 1; sh1 = 1; sh2 = 3; }
-    else if (a.sampling == 3) { and0 = 3; and1 = 12; sh0 = 2; sh1 = 0; sh2 = 2; }
-    const bool sr = a.sampling != 1;
-    auto hit = [&](unsigned int num, unsigned int sub) {
-        unsigned int x, y;
-        if (sr) {
-            x = ((num & 4095u) << sh0) | ((sub & and0) >> sh1);
-            y = ((num >> 12) << sh0) | ((sub & and1) >> sh2);
-        } else {
-            x = num % fw;
-            y = num / fw;
-        }
-        if (x < fw && y < fh) atomicAdd(cnt + (long long)y * fw + x, 1u);
-    };
-    unsigned int num = 0;
+    unsigned int x, y;
+    if (a.sampling == 1) {
+        x = num % fw;
+        y = num / fw;
+    } else if (a.sampling == 2) {
+        x = ((num & 4095u) << 1) | ((sub & 2u) >> 1);
+        y = ((num >> 12) << 1) | ((sub & 8u) >> 3);
+    } else {
+        x = ((num & 4095u) << 2) | (sub & 3u);
+        y = ((num >> 12) << 2) | ((sub & 12u) >> 2);
+    }
+    if (x < fw && y < fh) atomicAdd(cnt + (long long)y * fw + x, 1u);
+}
+
+__global__ __launch_bounds__(64) void k_eer_seg_table(EerArgs a) {
+    __shared__ unsigned int slices[64 * MC_EER_SLICE];
+    const int s = blockIdx.x * 64 + threadIdx.x;
+    if (s >= a.nseg) return;
+    const int j = a.seg_job[s];
+    const int fr = a.job[3 * j];
+    const EerSeg g = eer_seg(a, fr, s - a.job[3 * j + 2], slices + threadIdx.x * MC_EER_SLICE);
     if (a.bits == 7) {
-        unsigned long long bit = 0;
-        while (true) {
-            const long long b0 = (long long)(bit >> 3);
-            if (b0 >= nb) break;
-            const unsigned int chunk = (eer_byte(p, b0, nb) | (eer_byte(p, b0 + 1, nb) << 8) |
-                                        (eer_byte(p, b0 + 2, nb) << 16) | (eer_byte(p, b0 + 3, nb) << 24)) >>
-                                       (unsigned int)(bit & 7);
-            unsigned int r = chunk & 127u;
-            bit += 7;
+        const long long end = min(g.E, g.nbits);
+        for (int ph = 0; ph < MC_EER_PHASES; ph++) {
+            long long q = g.S + ph;
+            unsigned int dn = 0;
+            while (q < end) {
+                const unsigned int r = eer_bits(g, q, 7);
+                q += 7;
+                dn += r;
+                if (r != 127u) {
+                    q += 4;
+                    dn += 1;
+                }
+            }
+            a.tab[2 * ((long long)s * MC_EER_PHASES + ph)] = (int)(q - g.E);
+            a.tab[2 * ((long long)s * MC_EER_PHASES + ph) + 1] = (int)dn;
+        }
+    } else {
+        const long long kb = g.S >> 3, end = min(kb + MC_EER_SEG_BYTES8, g.nb);
+        const bool sr = a.sampling != 1;
+        unsigned int dn = 0;
+        for (long long pos = kb; pos < end; pos += 3) {
+            const unsigned int v = eer_bits(g, 8 * pos, 24);
+            const unsigned int c0 = v & 0xffu, c1 = (v >> 8) & 0xffu, c2 = (v >> 16) & 0xffu;
+            dn += c0 + (c0 < 255u ? 1u : 0u);
+            const unsigned int r2 = (sr ? ((c1 >> 4) | g.b0) : ((c1 >> 4) | (c2 << 4))) & 0xffu;
+            dn += r2 + (r2 < 255u ? 1u : 0u);
+        }
+        a.tab[2 * ((long long)s * MC_EER_PHASES)] = 0;
+        a.tab[2 * ((long long)s * MC_EER_PHASES) + 1] = (int)dn;
+    }
+}
+
+__global__ __launch_bounds__(64) void k_eer_scan(EerArgs a) {
+    const int j = blockIdx.x * 64 + threadIdx.x;
+    if (j >= a.njob) return;
+    const int s0 = a.job[3 * j + 2];
+    const int s1 = j + 1 < a.njob ? a.job[3 * (j + 1) + 2] : a.nseg;
+    const unsigned int cam = (unsigned int)a.cam_w * (unsigned int)a.cam_h;
+    unsigned int num = 0;
+    int ph = 0;
+    for (int s = s0; s < s1; s++) {
+        a.start[2 * s] = ph;
+        a.start[2 * s + 1] = (int)num;
+        if (num >= cam) continue;
+        const long long t = 2 * ((long long)s * MC_EER_PHASES + ph);
+        num += (unsigned int)a.tab[t + 1];
+        if (num > cam) num = cam;
+        ph = a.bits == 7 ? a.tab[t] : 0;
+        if (ph < 0 || ph >= MC_EER_PHASES) ph = 0;  // past the stream's end
+    }
+}
+
+__global__ __launch_bounds__(64) void k_eer_emit(EerArgs a) {
+    __shared__ unsigned int slices[64 * MC_EER_SLICE];
+    const int s = blockIdx.x * 64 + threadIdx.x;
+    if (s >= a.nseg) return;
+    const unsigned int cam = (unsigned int)a.cam_w * (unsigned int)a.cam_h;
+    unsigned int num = (unsigned int)a.start[2 * s + 1];
+    if (num >= cam) return;
+    const int j = a.seg_job[s];
+    const int fr = a.job[3 * j], o = a.job[3 * j + 1];
+    const EerSeg g = eer_seg(a, fr, s - a.job[3 * j + 2], slices + threadIdx.x * MC_EER_SLICE);
+    unsigned int* cnt = a.counts + (long long)o * a.fw * a.fh;
+    if (a.bits == 7) {
+        const long long end = min(g.E, g.nbits);
+        long long q = g.S + a.start[2 * s];
+        while (q < end) {
+            const unsigned int r = eer_bits(g, q, 7);
+            q += 7;
             num += r;
-            if (num >= cam) break;
+            if (num >= cam) return;
             if (r == 127u) continue;
-            bit += 4;
-            hit(num, ((chunk >> 7) & 15u) ^ 0x0Au);
-            num += 1;
-            r = (chunk >> 11) & 127u;
-            bit += 7;
-            num += r;
-            if (num >= cam) break;
-            if (r == 127u) continue;
-            bit += 4;
-            hit(num, ((chunk >> 18) & 15u) ^ 0x0Au);
+            eer_hit(a, cnt, num, eer_bits(g, q, 4) ^ 0x0Au);
+            q += 4;
             num += 1;
         }
     } else {
-        const unsigned int b0 = eer_byte(p, 0, nb);
-        for (long long pos = 0; pos < nb; pos += 3) {
-            const unsigned int c0 = eer_byte(p, pos, nb), c1 = eer_byte(p, pos + 1, nb);
-            const unsigned int r1 = c0;
-            num += r1;
-            if (num >= cam) break;
-            if (r1 < 255u) {
-                hit(num, c1 & 0x0Au);
+        const long long kb = g.S >> 3, end = min(kb + MC_EER_SEG_BYTES8, g.nb);
+        const bool sr = a.sampling != 1;
+        for (long long pos = kb; pos < end; pos += 3) {
+            const unsigned int v = eer_bits(g, 8 * pos, 24);
+            const unsigned int c0 = v & 0xffu, c1 = (v >> 8) & 0xffu, c2 = (v >> 16) & 0xffu;
+            num += c0;
+            if (num >= cam) return;
+            if (c0 < 255u) {
+                eer_hit(a, cnt, num, c1 & 0x0Au);
                 num += 1;
             }
             unsigned int r2, s2;
             if (sr) {
-                r2 = ((c1 >> 4) | b0) & 0xffu;
-                s2 = (b0 >> 4) ^ 0x0Au;
+                r2 = ((c1 >> 4) | g.b0) & 0xffu;
+                s2 = (g.b0 >> 4) ^ 0x0Au;
             } else {
-                r2 = ((c1 >> 4) | (eer_byte(p, pos + 2, nb) << 4)) & 0xffu;
+                r2 = ((c1 >> 4) | (c2 << 4)) & 0xffu;
                 s2 = 0;
             }
             num += r2;
-            if (num >= cam) break;
+            if (num >= cam) return;
             if (r2 < 255u) {
-                hit(num, s2);
+                eer_hit(a, cnt, num, s2);
                 num += 1;
             }
         }
@@ -205,10 +317,13 @@ hipError_t mc_launch_zero_u32(unsigned int* p, long long n, hipStream_t s) {
 }
 
 hipError_t mc_launch_eer_render(const EerArgs& a, hipStream_t s) {
-    if (a.njob < 1 || (a.bits != 7 && a.bits != 8) || a.sampling < 1 || a.sampling > 3 || a.fw < 1 ||
-        a.fh < 1)
+    if (a.njob < 1 || a.nseg < a.njob || (a.bits != 7 && a.bits != 8) || a.sampling < 1 ||
+        a.sampling > 3 || a.fw < 1 || a.fh < 1)
         return hipErrorInvalidValue;
-    hipLaunchKernelGGL(k_eer_render, dim3((a.njob + 63) / 64), dim3(64), 0, s, a);
+    const dim3 gs((a.nseg + 63) / 64), gj((a.njob + 63) / 64);
+    hipLaunchKernelGGL(k_eer_seg_table, gs, dim3(64), 0, s, a);
+    hipLaunchKernelGGL(k_eer_scan, gj, dim3(64), 0, s, a);
+    hipLaunchKernelGGL(k_eer_emit, gs, dim3(64), 0, s, a);
     return hipGetLastError();
 }
 

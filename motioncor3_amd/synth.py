@@ -112,3 +112,38 @@ class MovieRecipe:
             img = self.render(t)
             out[t] = rng.poisson(img).astype(np.float32) if noise else img.astype(np.float32)
         return out
+
+
+def eer_stream_7bit(positions, subpix, cam_pixels):
+    """A 7-bit EER code stream (compression 65001, as EerUtil/CDecodeEerFrame.cpp
+    reads it) of electrons at sorted, distinct pixel indices `positions` with
+    sub-pixel nibbles `subpix` (benchmark input).  Each electron is its zero
+    run (127 = run continues, no electron) then its nibble ^ 0x0A, least
+    significant bit first; a final run reaches the end of the camera."""
+    q = np.asarray(positions, np.int64)
+    sub = np.asarray(subpix, np.int64) & 15
+    prev = np.concatenate([[-1], q[:-1]]) if len(q) else np.zeros(0, np.int64)
+    runs = q - prev - 1
+    rest = int(cam_pixels - (q[-1] + 1 if len(q) else 0))
+    full = np.concatenate([runs // 127, [rest // 127]])
+    last = np.concatenate([runs % 127, [rest % 127]])
+    # fields per electron: full x (127, 7 bits), (run, 7 bits), (nibble, 4 bits);
+    # then the final run
+    nf = full + 2
+    nf[-1] = full[-1] + 1
+    vals = np.full(int(nf.sum()), 127, np.int64)
+    wid = np.full(len(vals), 7, np.int64)
+    end = np.cumsum(nf)
+    start = end - nf
+    lastpos = start + full
+    vals[lastpos] = last
+    nib = lastpos[:-1] + 1
+    vals[nib] = sub ^ 0x0A
+    wid[nib] = 4
+    bitpos = np.cumsum(wid) - wid
+    total = int(wid.sum())
+    bits = np.zeros(total + 40, np.uint8)
+    for j in range(7):
+        m = j < wid
+        bits[bitpos[m] + j] = (vals[m] >> j) & 1
+    return np.packbits(bits, bitorder="little")[:(total + 7) // 8 + 4]
