@@ -2370,7 +2370,7 @@ int mc_mrc_read(const char* path, int first, int count, void* dst) {
     return rc;
 }
 
-int mc_tiff_info(const char* path, int* sx, int* sy, int* n, int* mode, int* compression,
+static int mc_tiff_info_impl(const char* path, int* sx, int* sy, int* n, int* mode, int* compression,
                  float* pix_size) {
     if (!path) return MC_E_ARG;
     TiffFile t;
@@ -2387,7 +2387,17 @@ int mc_tiff_info(const char* path, int* sx, int* sy, int* n, int* mode, int* com
     return MC_OK;
 }
 
-int mc_tiff_read(const char* path, int first, int count, void* dst) {
+int mc_tiff_info(const char* path, int* sx, int* sy, int* n, int* mode, int* compression,
+                 float* pix_size) {
+    // malformed files end in an error code, never an exception through the C ABI
+    try {
+        return mc_tiff_info_impl(path, sx, sy, n, mode, compression, pix_size);
+    } catch (...) {
+        return MC_E_ARG;
+    }
+}
+
+static int mc_tiff_read_impl(const char* path, int first, int count, void* dst) {
     if (!path || !dst || first < 0 || count < 1) return MC_E_ARG;
     TiffFile t;
     if (!tiff_open(path, t) || first + count > (int)t.dirs.size()) return MC_E_ARG;
@@ -2403,7 +2413,16 @@ int mc_tiff_read(const char* path, int first, int count, void* dst) {
     return MC_OK;
 }
 
-int mc_eer_info(const char* path, int* cam_w, int* cam_h, int* n, int* bits, long long* bytes) {
+int mc_tiff_read(const char* path, int first, int count, void* dst) {
+    // malformed files end in an error code, never an exception through the C ABI
+    try {
+        return mc_tiff_read_impl(path, first, count, dst);
+    } catch (...) {
+        return MC_E_ARG;
+    }
+}
+
+static int mc_eer_info_impl(const char* path, int* cam_w, int* cam_h, int* n, int* bits, long long* bytes) {
     if (!path) return MC_E_ARG;
     TiffFile t;
     if (!tiff_open(path, t)) return MC_E_ARG;
@@ -2425,7 +2444,16 @@ int mc_eer_info(const char* path, int* cam_w, int* cam_h, int* n, int* bits, lon
     return MC_OK;
 }
 
-int mc_eer_read(const char* path, unsigned char* data, long long cap, long long* offsets) {
+int mc_eer_info(const char* path, int* cam_w, int* cam_h, int* n, int* bits, long long* bytes) {
+    // malformed files end in an error code, never an exception through the C ABI
+    try {
+        return mc_eer_info_impl(path, cam_w, cam_h, n, bits, bytes);
+    } catch (...) {
+        return MC_E_ARG;
+    }
+}
+
+static int mc_eer_read_impl(const char* path, unsigned char* data, long long cap, long long* offsets) {
     if (!path || !data || !offsets || cap < 0) return MC_E_ARG;
     TiffFile t;
     if (!tiff_open(path, t)) return MC_E_ARG;
@@ -2439,6 +2467,15 @@ int mc_eer_read(const char* path, unsigned char* data, long long cap, long long*
     }
     offsets[t.dirs.size()] = (long long)tot;
     return MC_OK;
+}
+
+int mc_eer_read(const char* path, unsigned char* data, long long cap, long long* offsets) {
+    // malformed files end in an error code, never an exception through the C ABI
+    try {
+        return mc_eer_read_impl(path, data, cap, offsets);
+    } catch (...) {
+        return MC_E_ARG;
+    }
 }
 
 int mc_load_eer(mc_ctx* c, const mc_eer_stack* e, float* frames, int W, int H, long long pitch,

@@ -254,6 +254,11 @@ bool tiff_open(const char* path, TiffFile& t) {
         t.err = std::string("cannot open ") + path;
         return false;
     }
+    if (fseeko(t.f, 0, SEEK_END) != 0) {
+        t.err = "cannot size file";
+        return false;
+    }
+    t.size = (uint64_t)ftello(t.f);
     unsigned char h[16];
     if (!rd(t.f, 0, h, 8)) {
         t.err = "short file";
@@ -318,7 +323,7 @@ bool tiff_open(const char* path, TiffFile& t) {
                 tag != 278 && tag != 282 && tag != 283 && tag != 317 && tag != 322 && tag != 323 &&
                 tag != 339)
                 continue;
-            if (!entry_values(t, type, count, val, inl, arr ? (1u << 26) : 16, v, &real)) {
+            if (!entry_values(t, type, count, val, inl, arr ? (1u << 24) : 16, v, &real)) {
                 t.err = "bad directory entry " + std::to_string(tag);
                 return false;
             }
@@ -340,6 +345,17 @@ bool tiff_open(const char* path, TiffFile& t) {
             }
         }
         if (d.rps <= 0) d.rps = d.h;
+        // every block inside the file; image and tile sizes bounded
+        for (size_t i = 0; i < d.off.size() && i < d.cnt.size(); i++)
+            if (d.off[i] > t.size || d.cnt[i] > t.size - d.off[i]) {
+                t.err = "strip or tile beyond the end of the file";
+                return false;
+            }
+        if (d.w < 0 || d.h < 0 || (uint64_t)d.w * (uint64_t)d.h > (1ull << 34) ||
+            (uint64_t)d.tw * (uint64_t)d.th > (1ull << 28) || d.bits < 0 || d.bits > 64) {
+            t.err = "image or tile size out of range";
+            return false;
+        }
         t.dirs.push_back(std::move(d));
         ifd = big ? u64(ents.data() + ne * es, t.be) : u32(ents.data() + ne * es, t.be);
     }
@@ -363,6 +379,10 @@ bool tiff_decode(TiffFile& t, const TiffDir& d, unsigned char* out) {
     const int mode = tiff_mode(d);
     if (mode < 0 || d.w < 1 || d.h < 1) return false;
     if (d.pred != 1 && d.pred != 2) return false;
+    const bool tiled = d.tw > 0 && d.th > 0;
+    const size_t nblk = tiled ? (size_t)((d.w + d.tw - 1) / d.tw) * ((d.h + d.th - 1) / d.th)
+                              : (size_t)((d.h + d.rps - 1) / d.rps);
+    if (d.off.size() < nblk || d.cnt.size() < nblk) return false;
     const int bs = d.bits / 8;
     const size_t row = (size_t)d.w * bs;
     std::vector<unsigned char> raw, blk;

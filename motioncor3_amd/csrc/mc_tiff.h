@@ -21,7 +21,8 @@ struct TiffDir {
 
 struct TiffFile {
     FILE* f = nullptr;
-    bool be = false;  // big-endian file
+    uint64_t size = 0;  // file bytes
+    bool be = false;    // big-endian file
     std::vector<TiffDir> dirs;
     std::string err;
     ~TiffFile() {

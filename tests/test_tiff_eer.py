@@ -88,6 +88,27 @@ def test_tiff_errors(tmp_path):
         L.tiff_info(str(bad))
     with pytest.raises(RuntimeError):
         L.tiff_info(str(tmp_path / "missing.tif"))
+    # a strip that runs past the end of the file, and a truncated file
+    good = tmp_path / "g.tif"
+    tiffw.write(str(good), [dict(w=8, h=4, bits=8, comp=1, rps=4, blocks=[bytes(32)])])
+    raw = bytearray(good.read_bytes())
+    short = tmp_path / "s.tif"
+    short.write_bytes(bytes(raw[:12]))
+    with pytest.raises(RuntimeError):
+        L.tiff_info(str(short))
+    # StripByteCounts (tag 279, LONG, count 1) patched to a huge value
+    import struct
+    ifd = struct.unpack_from("<I", raw, 4)[0]
+    n = struct.unpack_from("<H", raw, ifd)[0]
+    for e in range(n):
+        p = ifd + 2 + 12 * e
+        if struct.unpack_from("<H", raw, p)[0] == 279:
+            struct.pack_into("<I", raw, p + 8, 0x7fffffff)
+    huge = tmp_path / "h.tif"
+    huge.write_bytes(bytes(raw))
+    with pytest.raises(RuntimeError):
+        L.tiff_info(str(huge))
+    assert L.tiff_info(str(good))["n"] == 1
 
 
 # ---- EER ----
