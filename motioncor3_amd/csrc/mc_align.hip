@@ -379,174 +379,243 @@ __global__ void k_update(UpdArgs a) {
 // CSplineFit1D (Util/CSplineFit1D.cpp:31-115), CRemoveSpikes1D
 // (Util/CRemoveSpikes1D.cpp:31-148), CStackShift::Smooth
 // (Align/CStackShift.cpp:180-198) and the linear group->frame interpolation
-// (Align/CInterpolateShift.cpp:75-142).
-// The FP64 trajectory helpers below run on one thread per workgroup
-// (k_finalize, k_despike); their small fixed-size arrays are static LDS so
-// that indexing them does not spill to per-lane scratch memory.
-__device__ bool dev_lin_eqs(double* a, double* v, int n) {
-    __shared__ int ci[16], piv[16];
-    for (int i = 0; i < n; i++) piv[i] = 0;
+// (Align/CInterpolateShift.cpp:75-142).  One 64-lane wave works on one
+// trajectory (k_finalize, k_despike): every element of an array step is one
+// lane's, computed with the reference's operations for that element (a sum
+// over frames stays a sequential sum per element); the Gauss-Jordan pivot
+// search is a lane reduction that keeps the reference's first-maximum rule.
+#define MC_TL 64
+
+// Spline terms of CSplineFit1D for increasing knots: 1, x, x^2, then
+// (x - k_i)^2 while x >= k_i (the term loop stops at the first knot above
+// x); the curvature terms 0, 1, 2x, then 2 (x - k_i) only when x <= k_0
+// (the curvature loop stops at the first knot below x).
+__device__ __forceinline__ double spl_term(double x, int r, const double* knots) {
+    if (r == 0) return 1.0;
+    if (r == 1) return x;
+    if (r == 2) return x * x;
+    const double k = knots[r - 3];
+    if (x < k) return 0.0;
+    const double v = x - k;
+    return v * v;
+}
+__device__ __forceinline__ double spl_cur(double x, int r, const double* knots) {
+    if (r == 0) return 0.0;
+    if (r == 1) return 1.0;
+    if (r == 2) return 2.0 * x;
+    if (x - knots[0] > 0) return 0.0;
+    return 2.0 * (x - knots[r - 3]);
+}
+
+// Gauss-Jordan with full pivoting on the n x n system (n <= 9) in LDS;
+// false when singular.
+__device__ bool w_lin_eqs(double* a, double* v, int n) {
+    __shared__ int piv[16];
+    __shared__ int sel[2];
+    __shared__ double rv[MC_TL];
+    __shared__ int re[MC_TL];
+    const int l = threadIdx.x;
+    if (l < n) piv[l] = 0;
+    __syncthreads();
     for (int ip = 0; ip < n; ip++) {
-        int ix = 0, iy = 0;
-        bool found = false;
-        double fmax = 0.0;
-        for (int y = 0; y < n; y++) {
-            if (piv[y] == 1) continue;
-            for (int x = 0; x < n; x++) {
-                if (piv[x] == 1) continue;
-                const double c = fabs(a[y * n + x]);
-                if (c <= fmax) continue;
-                fmax = c;
-                ix = x;
-                iy = y;
-                found = true;
+        // first maximum of |a| over unpivoted rows and columns, row-major
+        double best = 0.0;
+        int be = -1;
+        for (int e = l; e < n * n; e += MC_TL) {
+            const int y = e / n, x = e - y * n;
+            if (piv[y] == 1 || piv[x] == 1) continue;
+            const double c = fabs(a[e]);
+            if (c > best) {
+                best = c;
+                be = e;
             }
         }
-        if (!found) return false;
-        piv[ix] += 1;
-        ci[ip] = ix;
+        rv[l] = best;
+        re[l] = be;
+        __syncthreads();
+        if (l == 0) {
+            double bv = 0.0;
+            int bi = -1;
+            for (int k = 0; k < MC_TL; k++) {
+                if (re[k] < 0) continue;
+                if (rv[k] > bv || (rv[k] == bv && re[k] < bi)) {
+                    bv = rv[k];
+                    bi = re[k];
+                }
+            }
+            sel[0] = bi < 0 ? -1 : bi % n;  // column
+            sel[1] = bi < 0 ? -1 : bi / n;  // row
+            if (bi >= 0) piv[bi % n] += 1;
+        }
+        __syncthreads();
+        const int ix = sel[0], iy = sel[1];
+        if (ix < 0) return false;
         if (a[iy * n + ix] == 0.0) return false;
         if (ix != iy) {
-            for (int x = 0; x < n; x++) {
-                const double t = a[ix * n + x];
-                a[ix * n + x] = a[iy * n + x];
-                a[iy * n + x] = t;
+            if (l < n) {
+                const double t = a[ix * n + l];
+                a[ix * n + l] = a[iy * n + l];
+                a[iy * n + l] = t;
             }
-            const double t = v[ix];
-            v[ix] = v[iy];
-            v[iy] = t;
+            if (l == 0) {
+                const double t = v[ix];
+                v[ix] = v[iy];
+                v[iy] = t;
+            }
+            __syncthreads();
         }
-        const int col = ci[ip];
+        const int col = ix;
         const double pinv = 1.0 / (a[col * n + col] + 1e-30);
-        for (int x = 0; x < n; x++) a[col * n + x] *= pinv;
-        v[col] *= pinv;
-        for (int y = 0; y < n; y++) {
-            if (y == col) continue;
-            const double dum = a[y * n + col];
-            for (int x = 0; x < n; x++) a[y * n + x] -= a[col * n + x] * dum;
-            v[y] -= v[col] * dum;
+        __syncthreads();
+        if (l < n) a[col * n + l] *= pinv;
+        if (l == 0) v[col] *= pinv;
+        __syncthreads();
+        // rows y != col: a[y][x] -= a[col][x] * a[y][col], v[y] -= v[col] * a[y][col]
+        double na[2] = {0.0, 0.0};
+        double nv = 0.0;
+#pragma unroll
+        for (int k = 0; k < 2; k++) {
+            const int e = l + k * MC_TL;
+            if (e < n * n) {
+                const int y = e / n, x = e - y * n;
+                if (y != col) na[k] = a[e] - a[col * n + x] * a[y * n + col];
+            }
         }
+        if (l < n && l != col) nv = v[l] - v[col] * a[l * n + col];
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < 2; k++) {
+            const int e = l + k * MC_TL;
+            if (e < n * n && e / n != col) a[e] = na[k];
+        }
+        if (l < n && l != col) v[l] = nv;
+        __syncthreads();
     }
     return true;
 }
 
-struct DevSpline {
-    int nk;
-    const double* knots;
-    __device__ void terms(double x, double* t) const {
-        t[0] = 1.0;
-        t[1] = x;
-        t[2] = x * x;
-        for (int i = 0; i < nk; i++) t[3 + i] = 0.0;
-        for (int i = 0; i < nk; i++) {
-            if (x < knots[i]) break;
-            const double v = x - knots[i];
-            t[3 + i] = v * v;
-        }
-    }
-    __device__ void cur(double x, double* c) const {
-        for (int i = 0; i < nk + 3; i++) c[i] = 0.0;
-        c[1] = 1.0;
-        c[2] = 2.0 * x;
-        for (int i = 0; i < nk; i++) {
-            const double v = x - knots[i];
-            if (v > 0) break;
-            c[3 + i] = 2.0 * v;
-        }
-    }
-    __device__ void fit(const double* x, const double* y, int n, double* out) const {
-        const int d = nk + 3;
-        __shared__ double m[81], cf[9], t[9], c[9];
-        for (int i = 0; i < d * d; i++) m[i] = 0.0;
-        for (int i = 0; i < d; i++) cf[i] = 0.0;
-        for (int i = 0; i < n; i++) {
-            terms(x[i], t);
-            cur(x[i], c);
-            for (int r = 0; r < d; r++) {
-                for (int q = 0; q < d; q++) m[r * d + q] += t[r] * t[q] + 1e-20 * c[r] * c[q];
-                cf[r] += t[r] * y[i];
+// CSplineFit1D::DoIt: least squares of y on the spline terms of x, then the
+// fit evaluated at every x into out (left untouched if the system is
+// singular).
+__device__ void w_fit(const double* knots, int nk, const double* x, const double* y, int n, double* out) {
+    __shared__ double m[81], cf[9];
+    const int d = nk + 3, l = threadIdx.x;
+    for (int e = l; e < d * d + d; e += MC_TL) {
+        double acc = 0.0;
+        if (e < d * d) {
+            const int r = e / d, q = e - r * d;
+            for (int i = 0; i < n; i++) {
+                const double xi = x[i];
+                acc += spl_term(xi, r, knots) * spl_term(xi, q, knots) +
+                       1e-20 * spl_cur(xi, r, knots) * spl_cur(xi, q, knots);
             }
-        }
-        if (!dev_lin_eqs(m, cf, d)) return;
-        for (int i = 0; i < n; i++) {
-            terms(x[i], t);
-            double f = 0.0;
-            for (int r = 0; r < d; r++) f += cf[r] * t[r];
-            out[i] = f;
+            m[e] = acc;
+        } else {
+            const int r = e - d * d;
+            for (int i = 0; i < n; i++) acc += spl_term(x[i], r, knots) * y[i];
+            cf[r] = acc;
         }
     }
-};
+    __syncthreads();
+    if (!w_lin_eqs(m, cf, d)) {
+        __syncthreads();
+        return;
+    }
+    for (int i = l; i < n; i += MC_TL) {
+        double f = 0.0;
+        for (int r = 0; r < d; r++) f += cf[r] * spl_term(x[i], r, knots);
+        out[i] = f;
+    }
+    __syncthreads();
+}
 
-__device__ void dev_remove_single(double* raw, double* buf, int n) {
-    for (int i = 0; i < n; i++) buf[i] = raw[i];
-    for (int i = 1; i < n - 1; i++) {
+__device__ void w_remove_single(double* raw, double* buf, int n) {
+    const int l = threadIdx.x;
+    for (int i = l; i < n; i += MC_TL) buf[i] = raw[i];
+    __syncthreads();
+    for (int i = l + 1; i < n - 1; i += MC_TL) {
         if (fabs(buf[i] - buf[i - 1]) < 5.0) continue;
         if (fabs(buf[i] - buf[i + 1]) < 5.0) continue;
         raw[i] = (buf[i - 1] + buf[i + 1]) / 2.0;
     }
-    const int e = n - 1;
-    if (fabs(buf[0] - buf[1]) > 10.0 && fabs(buf[0]) > fabs(buf[1])) raw[0] = 2.0 * buf[1] - buf[2];
-    if (fabs(buf[e] - buf[e - 1]) > 10.0 && fabs(buf[e]) > fabs(buf[e - 1]))
-        raw[e] = 2.0 * buf[e - 1] - buf[e - 2];
+    if (l == 0) {
+        const int e = n - 1;
+        if (fabs(buf[0] - buf[1]) > 10.0 && fabs(buf[0]) > fabs(buf[1])) raw[0] = 2.0 * buf[1] - buf[2];
+        if (fabs(buf[e] - buf[e - 1]) > 10.0 && fabs(buf[e]) > fabs(buf[e - 1]))
+            raw[e] = 2.0 * buf[e - 1] - buf[e - 2];
+    }
+    __syncthreads();
 }
 
-__device__ bool dev_find_bad(double* raw, const double* fit, int n, double tol) {
-    bool has = false;
-    for (int i = 0; i < n; i++) {
+__device__ bool w_find_bad(double* raw, const double* fit, int n, double tol) {
+    int has = 0;
+    for (int i = threadIdx.x; i < n; i += MC_TL) {
         if (fabs(fit[i] - raw[i]) > tol && fabs(raw[i]) > fabs(fit[i])) {
             raw[i] = fit[i];
-            has = true;
+            has = 1;
         }
     }
-    return has;
+    return __syncthreads_or(has) != 0;
 }
 
 // scratch: 5*n doubles
-__device__ void dev_remove_spikes(double* sx, double* sy, int n, int quirks, double* scratch) {
+__device__ void w_remove_spikes(double* sx, double* sy, int n, int quirks, double* scratch) {
     double* t = scratch;
     double* fx = scratch + n;
     double* fy = scratch + 2 * n;
     double* buf = scratch + 3 * n;
     double* buf2 = scratch + 4 * n;
+    const int l = threadIdx.x;
     for (int r = 0; r < 2; r++) {
-        dev_remove_single(sx, buf, n);
-        dev_remove_single(sy, buf, n);
+        w_remove_single(sx, buf, n);
+        w_remove_single(sy, buf, n);
     }
     const float ffact = 1.0f / n;
-    for (int i = 0; i < n; i++) t[i] = sqrt((double)(i * ffact));
+    for (int i = l; i < n; i += MC_TL) t[i] = sqrt((double)(i * ffact));
     int nk = (n - 30) / 15;
     if (nk < 0) nk = 0;
     else if (nk > 6) nk = 6;
     if (nk == 0 && n >= 30) nk = 1;
     __shared__ double knots[6];
-    for (int i = 0; i < nk; i++) knots[i] = t[(i + 1) * 15];
-    DevSpline sp;
-    sp.nk = nk;
-    sp.knots = knots;
-    for (int i = 0; i < n; i++) { fx[i] = sx[i]; fy[i] = sy[i]; }
-    sp.fit(t, sx, n, fx);
-    sp.fit(t, sy, n, fy);
-    for (int i = 0; i < n; i++) buf[i] = sx[i];
-    if (dev_find_bad(buf, fx, n, 3.0)) sp.fit(t, buf, n, fx);
-    for (int i = 0; i < n; i++) buf2[i] = sy[i];
-    if (dev_find_bad(buf2, fy, n, 3.0)) sp.fit(t, quirks ? sy : buf2, n, fy);
-    for (int i = 0; i < n; i++) {
+    __syncthreads();
+    if (l < nk) knots[l] = t[(l + 1) * 15];
+    for (int i = l; i < n; i += MC_TL) {
+        fx[i] = sx[i];
+        fy[i] = sy[i];
+    }
+    __syncthreads();
+    w_fit(knots, nk, t, sx, n, fx);
+    w_fit(knots, nk, t, sy, n, fy);
+    for (int i = l; i < n; i += MC_TL) buf[i] = sx[i];
+    __syncthreads();
+    if (w_find_bad(buf, fx, n, 3.0)) w_fit(knots, nk, t, buf, n, fx);
+    for (int i = l; i < n; i += MC_TL) buf2[i] = sy[i];
+    __syncthreads();
+    if (w_find_bad(buf2, fy, n, 3.0)) w_fit(knots, nk, t, quirks ? sy : buf2, n, fy);
+    for (int i = l; i < n; i += MC_TL) {
         double tol = 5.0;
         if (i > 20) tol = 4.0;
         if (i < 50) tol = 3.0;
         if (fabs(fx[i] - sx[i]) > tol) sx[i] = fx[i];
         if (fabs(fy[i] - sy[i]) > tol) sy[i] = fy[i];
     }
+    __syncthreads();
 }
 
-__device__ void dev_smooth(double* sx, double* sy, int n, double w, double* buf) {
+__device__ void w_smooth(double* sx, double* sy, int n, double w, double* buf) {
     if (w >= 1) return;
-    for (int i = 0; i < n; i++) { buf[i] = sx[i]; buf[n + i] = sy[i]; }
+    const int l = threadIdx.x;
+    for (int i = l; i < n; i += MC_TL) {
+        buf[i] = sx[i];
+        buf[n + i] = sy[i];
+    }
+    __syncthreads();
     const double w2 = 0.5 * (1.0 - w);
-    for (int i = 1; i < n - 1; i++) {
+    for (int i = l + 1; i < n - 1; i += MC_TL) {
         sx[i] = w * buf[i] + w2 * (buf[i - 1] + buf[i + 1]);
         sy[i] = w * buf[n + i] + w2 * (buf[n + i - 1] + buf[n + i + 1]);
     }
+    __syncthreads();
 }
 
 struct FinArgs {
@@ -563,51 +632,53 @@ struct FinArgs {
 
 // End of CIterativeAlign (Align/CIterativeAlign.cpp:63-85,146-149): linear
 // interpolation of the accumulated group shifts, RemoveSpikes, Smooth(0.5),
-// then Smooth(0.3 + 0.1 * binZ, capped at 0.9).  The chain is serial per
-// job; its work arrays live in LDS when they fit, so every access is an LDS
-// round trip rather than an HBM one.
-__global__ void k_finalize(FinArgs a) {
+// then Smooth(0.3 + 0.1 * binZ, capped at 0.9).  One wave per job; its work
+// arrays live in LDS when they fit.
+__global__ __launch_bounds__(MC_TL) void k_finalize(FinArgs a) {
     extern __shared__ double fsh[];
-    const int j = blockIdx.x;
-    if (threadIdx.x != 0) return;
+    const int j = blockIdx.x, l = threadIdx.x;
     const int N = a.N, G = a.G;
     double* sc = a.use_lds ? fsh : a.scratch + (long long)j * a.sstride;
     double* sx = sc;
     double* sy = sc + N;
     double* work = sc + 2 * N;
     const double* tot = a.total + (long long)j * G * 2;
-    if (N == G) {
-        for (int i = 0; i < N; i++) { sx[i] = tot[2 * i]; sy[i] = tot[2 * i + 1]; }
-    } else {
-        for (int i = 0; i < N; i++) {
-            const double fc = a.fcent[i];
-            int k;
-            if (fc <= a.gcent[0]) k = 1;
-            else {
-                k = G - 1;
-                for (int q = 1; q < G; q++)
-                    if (fc < a.gcent[q]) { k = q; break; }
-            }
-            const int f = k - 1;
-            const double dg = a.gcent[k] - a.gcent[f];
-            const double df = fc - a.gcent[f];
-            sx[i] = tot[2 * f] + (tot[2 * k] - tot[2 * f]) / dg * df;
-            sy[i] = tot[2 * f + 1] + (tot[2 * k + 1] - tot[2 * f + 1]) / dg * df;
+    for (int i = l; i < N; i += MC_TL) {
+        if (N == G) {
+            sx[i] = tot[2 * i];
+            sy[i] = tot[2 * i + 1];
+            continue;
         }
+        const double fc = a.fcent[i];
+        int k;
+        if (fc <= a.gcent[0]) k = 1;
+        else {
+            k = G - 1;
+            for (int q = 1; q < G; q++)
+                if (fc < a.gcent[q]) { k = q; break; }
+        }
+        const int f = k - 1;
+        const double dg = a.gcent[k] - a.gcent[f];
+        const double df = fc - a.gcent[f];
+        sx[i] = tot[2 * f] + (tot[2 * k] - tot[2 * f]) / dg * df;
+        sy[i] = tot[2 * f + 1] + (tot[2 * k + 1] - tot[2 * f + 1]) / dg * df;
     }
-    dev_remove_spikes(sx, sy, N, a.quirks, work);
-    dev_smooth(sx, sy, N, 0.5, work);
+    __syncthreads();
+    w_remove_spikes(sx, sy, N, a.quirks, work);
+    w_smooth(sx, sy, N, 0.5, work);
     double w = 0.3 + 0.1 * a.binz;
     if (w > 0.9) w = 0.9;
-    dev_smooth(sx, sy, N, w, work);
+    w_smooth(sx, sy, N, w, work);
     double* out = a.final_ + (long long)j * N * 2;
-    for (int i = 0; i < N; i++) { out[2 * i] = sx[i]; out[2 * i + 1] = sy[i]; }
+    for (int i = l; i < N; i += MC_TL) {
+        out[2 * i] = sx[i];
+        out[2 * i + 1] = sy[i];
+    }
 }
 
-__global__ void k_despike(double* traj, double* scratch, int n, int quirks) {
-    if (threadIdx.x != 0) return;
+__global__ __launch_bounds__(MC_TL) void k_despike(double* traj, double* scratch, int n, int quirks) {
     double* t = traj + (long long)blockIdx.x * 2 * n;
-    dev_remove_spikes(t, t + n, n, quirks, scratch + (long long)blockIdx.x * 5 * n);
+    w_remove_spikes(t, t + n, n, quirks, scratch + (long long)blockIdx.x * 5 * n);
 }
 
 // ---------------------------------------------------------------------------

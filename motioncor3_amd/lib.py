@@ -276,11 +276,14 @@ class Context:
         return out
 
     def despike(self, sx, sy, quirks=True):
-        t = np.ascontiguousarray(np.stack([sx, sy])[None], dtype=np.float64).copy()
+        """sx, sy: one trajectory [n] or several [count, n], despiked in one call."""
+        one = np.ndim(sx) == 1
+        t = np.ascontiguousarray(np.stack([np.atleast_2d(sx), np.atleast_2d(sy)], 1),
+                                 dtype=np.float64).copy()
         self._check(self.lib.mc_spline_despike_fp64(
-            self.ptr, t.ctypes.data_as(C.POINTER(C.c_double)), t.shape[-1], 1, int(quirks)),
-            "mc_spline_despike_fp64")
-        return t[0, 0], t[0, 1]
+            self.ptr, t.ctypes.data_as(C.POINTER(C.c_double)), t.shape[-1], t.shape[0],
+            int(quirks)), "mc_spline_despike_fp64")
+        return (t[0, 0], t[0, 1]) if one else (t[:, 0], t[:, 1])
 
     def patch_stats(self, shifts, centers, W, H, quirks=True):
         s = np.ascontiguousarray(shifts, dtype=np.float64).copy()

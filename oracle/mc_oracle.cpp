@@ -1042,8 +1042,8 @@ void orc_peak2d(const double* img, int nx, int ny, double* shift) { peak2d(img, 
 
 int orc_lin_eqs(double* coeff, double* vals, int dim) { return lin_eqs(coeff, vals, dim) ? 1 : 0; }
 
-void orc_remove_spikes(double* sx, double* sy, int n, int single) {
-    remove_spikes(sx, sy, n, single != 0, true);
+void orc_remove_spikes(double* sx, double* sy, int n, int single, int quirks) {
+    remove_spikes(sx, sy, n, single != 0, quirks != 0);
 }
 
 void orc_smooth(double* sx, double* sy, int n, double w) { smooth(sx, sy, n, w); }

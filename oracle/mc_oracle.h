@@ -88,7 +88,7 @@ void orc_correlate_window(const double* sum, const double* grp, int cmpx,
                           int seax, int seay, double* window);
 void orc_peak2d(const double* img, int nx, int ny, double* shift);
 int  orc_lin_eqs(double* coeff, double* vals, int dim);
-void orc_remove_spikes(double* sx, double* sy, int n, int single);
+void orc_remove_spikes(double* sx, double* sy, int n, int single, int quirks);
 void orc_smooth(double* sx, double* sy, int n, double weight);
 void orc_dose_weights(int cmpx, int cmpy, double pix, int kv,
                       const double* acc_dose, int nfrm, double* wsum);

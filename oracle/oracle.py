@@ -176,10 +176,10 @@ def ref_lin_eqs(a, v):
     return bool(ok), v
 
 
-def remove_spikes(sx, sy, single=False):
+def remove_spikes(sx, sy, single=False, quirks=True):
     x = np.ascontiguousarray(sx, dtype=np.float64).copy()
     y = np.ascontiguousarray(sy, dtype=np.float64).copy()
-    lib().orc_remove_spikes(_d(x), _d(y), len(x), int(single))
+    lib().orc_remove_spikes(_d(x), _d(y), len(x), int(single), int(quirks))
     return x, y
 
 

@@ -57,19 +57,38 @@ def test_peak_fit(ctx):
     assert np.allclose(out, ref, atol=1e-5)
 
 
+def _spiky(rng, n):
+    t = np.sqrt(np.arange(n) / n)
+    sx = 8 * t - 3 * t ** 2 + rng.normal(0, 0.3, n)
+    sy = -5 * t + rng.normal(0, 0.3, n)
+    sx[n // 3] += 12.0
+    sy[n // 2] -= 7.0
+    # a two-frame excursion survives the single-spike pass and is left to
+    # the spline refit
+    sy[n // 4:n // 4 + 2] += 4.5
+    return sx, sy
+
+
 @pytest.mark.parametrize("quirks", [True, False])
 def test_despike_fp64(ctx, quirks):
+    """One wave per trajectory: lengths below, at and above the 64-lane wave,
+    with 0 to 6 spline knots, against CRemoveSpikes1D restated in the oracle."""
     rng = np.random.default_rng(5)
-    for n in (20, 50, 73):
-        t = np.sqrt(np.arange(n) / n)
-        sx = 8 * t - 3 * t ** 2 + rng.normal(0, 0.3, n)
-        sy = -5 * t + rng.normal(0, 0.3, n)
-        sx[n // 3] += 12.0
-        sy[n // 2] -= 7.0
+    for n in (20, 50, 64, 73, 130, 211):
+        sx, sy = _spiky(rng, n)
         gx, gy = ctx.despike(sx, sy, quirks)
+        ox, oy = O.remove_spikes(sx, sy, quirks=quirks)
+        assert np.allclose(gx, ox, atol=1e-9) and np.allclose(gy, oy, atol=1e-9), n
+
+
+def test_despike_batch_matches_single(ctx):
+    rng = np.random.default_rng(11)
+    n, count = 97, 9
+    tr = [_spiky(rng, n) for _ in range(count)]
+    bx, by = ctx.despike(np.stack([a for a, _ in tr]), np.stack([b for _, b in tr]))
+    for k, (sx, sy) in enumerate(tr):
         ox, oy = O.remove_spikes(sx, sy)
-        if quirks:
-            assert np.allclose(gx, ox, atol=1e-9) and np.allclose(gy, oy, atol=1e-9)
+        assert np.allclose(bx[k], ox, atol=1e-9) and np.allclose(by[k], oy, atol=1e-9), k
 
 
 @pytest.mark.parametrize("quirks", [True, False])
