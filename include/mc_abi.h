@@ -91,6 +91,9 @@ typedef struct mc_params {
                                 frames are transformed on a second stream;
                                 0: they run their alignment iterations as one
                                 batch of jobs.  Same results either way. */
+    int    out_stack[2];     /* -OutStack (CInput.cpp:590-592): [0] 1 makes the
+                                aligned stack (mc_result.stack), [1] frames summed
+                                into each stack frame (z-bin, default 1) */
 } mc_params;
 
 /* N frames already gain/dark/defect corrected, resident in HBM
@@ -128,6 +131,11 @@ typedef struct mc_result {
     float* prior_targets;      /* host P x N x 2: converged frame shifts minus
                                   the first iteration's, in prior units */
     int    prior_fallbacks;    /* patches that fell back to the plain solve */
+    float* stack;              /* dev, out_stack[0] only: mc_stack_frames() frames of
+                                  out_w x out_h floats, each corrected frame before
+                                  dose weighting at the -FtBin size
+                                  (Correct/CCorrectFullShift.cpp:336-356), z-binned
+                                  as Align/CZbinStack.cpp:27-70 */
 } mc_result;
 
 /* Replacements of measured quantities.  replay_* follow -InAln
@@ -153,6 +161,9 @@ void mc_default_params(mc_params* p);
  * Align/CAlignBase.cpp:41-47). */
 int mc_output_geometry(const mc_params* p, int W, int H, int N, int* num_sums, int* out_w,
                        int* out_h);
+/* Frames of the aligned stack (0 without -OutStack): N, or the number of
+ * z-bin groups (Align/CZbinStack.cpp:36-52, DataUtil/CFmGroupParam.cpp:26-97). */
+int mc_stack_frames(const mc_params* p, int N, int* frames);
 /* xcf image size, patch image size and xcf bin (CBufferPool.cpp:220-262). */
 int mc_geometry(int W, int H, int px, int py, int* sizes4, float* xcf_bin2);
 
@@ -387,6 +398,12 @@ int mc_correct_bad(mc_ctx* ctx, float* frames, int W, int H, int N, long long pi
  * of CGenStarFile (CGenStarFile.cpp:39-120), global shifts relative to the
  * first frame. */
 int mc_save_sums(mc_ctx* ctx, const mc_params* p, const mc_result* result, const char* out_mrc);
+/* The aligned stack of `result` (`frames` frames) as out_mrc + "_Stk.mrc"
+ * (CSaveSerialCryoEM.cpp:146-206): nz = frames, 32 floats of extended header
+ * per frame holding the pixel size in float 11, header min / max = extremes
+ * over the frames, mean = mean of the frame means. */
+int mc_save_stack(mc_ctx* ctx, const mc_params* p, const mc_result* result, int frames,
+                  const char* out_mrc);
 int mc_save_aln(const char* path, const mc_params* p, int sx, int sy, int n_stack, int N,
                 const float* global_shift, const float* patch_shift, const float* patch_center);
 int mc_load_aln(const char* path, int* sx, int* sy, int* n, int* px, int* py, float* global_shift,

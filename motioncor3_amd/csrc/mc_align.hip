@@ -23,12 +23,12 @@
 // FP64 per transform.  skip != 0 applies the |s| < 0.1 px rule of
 // Util/GPhaseShift2D.cu:68-79 (unit factors).
 __global__ void k_phasors(const double* shifts, double mult, int skip, const unsigned char* done,
-                          int per_job, int W, int cx, int H, cfloat* A, cfloat* B) {
+                          int per_job, int W, int cx, int H, int hn, cfloat* A, cfloat* B) {
     const int b = blockIdx.y;
     if (done && done[b / per_job]) return;
     double sx = shifts[2 * b] * mult, sy = shifts[2 * b + 1] * mult;
     if (skip && sqrt(sx * sx + sy * sy) < 0.1) sx = sy = 0.0;
-    const double fsx = sx * (2.0 * MC_PI / W), fsy = sy * (2.0 * MC_PI / H);
+    const double fsx = sx * (2.0 * MC_PI / W), fsy = sy * (2.0 * MC_PI / hn);
     const int e = blockIdx.x * blockDim.x + threadIdx.x;
     if (e < cx) {
         double sn, cs;
@@ -1295,10 +1295,10 @@ int mc_align_init_constants() {
 
 hipError_t mc_launch_phasors(const double* shifts, double mult, int skip, const unsigned char* done,
                              int per_job, int W, int cx, int H, int batch, cfloat* A, cfloat* B,
-                             hipStream_t s) {
+                             hipStream_t s, int h_norm) {
     dim3 grid(mc_div_up(cx + H, 256), batch);
     hipLaunchKernelGGL(k_phasors, grid, dim3(256), 0, s, shifts, mult, skip, done,
-                       per_job > 0 ? per_job : 1, W, cx, H, A, B);
+                       per_job > 0 ? per_job : 1, W, cx, H, h_norm > 0 ? h_norm : H, A, B);
     return hipGetLastError();
 }
 

@@ -1270,6 +1270,48 @@ int align_batch(mc_ctx* c, const mc_params* const* pp_, const mc_movie* const* m
             MC_TRY(mc_cols_inv(*po, sums + (long long)i * fs, fs, g.cx, H, nullptr, To, po->t_elems, 1, s));
             MC_TRY(mc_rows_c2r(*po, To, po->t_elems, res->sums[i], g.ow, (long long)g.ow * g.oh, 1, 1, s));
         }
+        // -OutStack: Correct/CCorrectFullShift.cpp:336-356 (mCropFrame), every
+        // corrected frame before dose weighting cropped to the -FtBin size,
+        // C2R and unpadded, with no positivity clamp; then
+        // Align/CZbinStack.cpp:27-70 sums each z-bin group in frame order.
+        if (p->out_stack[0]) {
+            if (!res->stack) return fail(c, MC_E_ARG, "out_stack needs a result stack buffer");
+            const int zb = p->out_stack[1];
+            Groups zg;
+            if (zb > 1) zg = make_groups(zb, N, p->throw0, p->fm_int_start, p->fm_int_size);
+            const long long ofs = (long long)g.ow * g.oh;
+            const int SC = std::min(N, 8);
+            MC_BUF(Tst, cfloat, "To_stk", (size_t)SC * po->t_elems);
+            float* Rst = nullptr;
+            if (zb > 1) {
+                MC_BUF(r_, float, "stk_chunk", (size_t)SC * ofs);
+                Rst = r_;
+            }
+            // the global correction's phasors, at the output size when cropped
+            // (rows of the crop keep their source frequency)
+            const cfloat *oA = fA, *oB = fB;
+            if (!patch && (g.ow != W || g.oh != H)) {
+                MC_BUF(a_, cfloat, "stk_phA", (size_t)N * g.ocx);
+                MC_BUF(b_, cfloat, "stk_phB", (size_t)N * g.oh);
+                MC_TRY(mc_launch_phasors(gsh, -1.0, 1, nullptr, N, W, g.ocx, g.oh, N, a_, b_, s, H));
+                oA = a_;
+                oB = b_;
+            }
+            KTimer kt(c, "out_stack");
+            for (int f0 = 0; f0 < N; f0 += SC) {
+                const int nb = std::min(SC, N - f0);
+                PhasorTabs ph{oA + (long long)f0 * g.ocx, oB + (long long)f0 * g.oh};
+                MC_TRY(mc_cols_inv(*po, Sf + (long long)f0 * fs, fs, g.cx, H, patch ? nullptr : &ph, Tst,
+                                   po->t_elems, nb, s));
+                float* dst = zb > 1 ? Rst : res->stack + (long long)f0 * ofs;
+                MC_TRY(mc_rows_c2r(*po, Tst, po->t_elems, dst, g.ow, ofs, nb, 0, s));
+                for (int k = 0; zb > 1 && k < nb; k++) {
+                    const int f = f0 + k, gi = zg.fgroup[f];
+                    MC_TRY(mc_launch_stack_add(Rst + (long long)k * ofs, res->stack + (long long)gi * ofs, ofs,
+                                               f == zg.start[gi], s));
+                }
+            }
+        }
     }
     MC_TRY(hipStreamSynchronize(s));
     const auto tc = clk::now();
@@ -1344,6 +1386,7 @@ void mc_default_params(mc_params* p) {
     p->prior = 0;
     p->prior_fallback = 0.2f;
     p->movie_pipeline = 1;
+    p->out_stack[1] = 1;
 }
 
 int mc_ctx_create(int device, void* stream, mc_ctx** out) {
@@ -1470,6 +1513,14 @@ int mc_output_geometry(const mc_params* p, int W, int H, int N, int* ns, int* ow
     return MC_OK;
 }
 
+int mc_stack_frames(const mc_params* p, int N, int* frames) {
+    if (!p || !frames || N < 1) return MC_E_ARG;
+    if (!p->out_stack[0]) *frames = 0;
+    else if (p->out_stack[1] <= 1) *frames = N;
+    else *frames = make_groups(p->out_stack[1], N, p->throw0, p->fm_int_start, p->fm_int_size).G;
+    return MC_OK;
+}
+
 static int align_movie(mc_ctx* c, const mc_params* p, const mc_movie* mv, const mc_override* ov,
                        mc_result* res) {
     const mc_params* pa[1] = {p};
@@ -1490,7 +1541,8 @@ static bool same_alignment_flags(const mc_params& a, const mc_params& b) {
            a.chunk_streams == b.chunk_streams && a.prior == b.prior &&
            a.fm_int_start == b.fm_int_start && a.fm_int_size == b.fm_int_size &&
            a.prior_fallback == b.prior_fallback && a.verbose == b.verbose &&
-           a.movie_pipeline == b.movie_pipeline;
+           a.movie_pipeline == b.movie_pipeline && a.out_stack[0] == b.out_stack[0] &&
+           a.out_stack[1] == b.out_stack[1];
 }
 
 int mc_align_movies(mc_ctx* c, const mc_params* params, const mc_movie* movies, int count,
@@ -2657,6 +2709,61 @@ int mc_save_sums(mc_ctx* c, const mc_params* p, const mc_result* r, const char* 
             return fail(c, MC_E_ARG, "cannot write " + path);
     }
     return MC_OK;
+}
+
+int mc_save_stack(mc_ctx* c, const mc_params* p, const mc_result* r, int frames, const char* out_mrc) {
+    if (!c || !p || !r || !out_mrc || !out_mrc[0]) return c ? fail(c, MC_E_ARG, "save stack: arguments") : MC_E_ARG;
+    if (!r->stack || frames < 1 || r->out_w < 1 || r->out_h < 1)
+        return fail(c, MC_E_ARG, "save stack: no stack in the result");
+    const int w = r->out_w, h = r->out_h;
+    const long long npx = (long long)w * h;
+    MC_BUF(pmin, float, "st_min", MC_STATS_BLOCKS);
+    MC_BUF(pmax, float, "st_max", MC_STATS_BLOCKS);
+    MC_BUF(psum, double, "st_sum", MC_STATS_BLOCKS);
+    std::vector<float> hmin(MC_STATS_BLOCKS), hmax(MC_STATS_BLOCKS);
+    std::vector<double> hsum(MC_STATS_BLOCKS);
+    // CSaveSerialCryoEM::mSaveStack (:177-204): extremes over the frames and
+    // the mean of the frame means
+    float lo = 1e20f, hi = -1e20f;
+    double mean = 0.0;
+    for (int i = 0; i < frames; i++) {
+        MC_TRY(mc_launch_image_stats(r->stack + (long long)i * npx, w, h, w, pmin, pmax, psum, c->stream));
+        MC_TRY(hipMemcpyAsync(hmin.data(), pmin, sizeof(float) * MC_STATS_BLOCKS, hipMemcpyDeviceToHost, c->stream));
+        MC_TRY(hipMemcpyAsync(hmax.data(), pmax, sizeof(float) * MC_STATS_BLOCKS, hipMemcpyDeviceToHost, c->stream));
+        MC_TRY(hipMemcpyAsync(hsum.data(), psum, sizeof(double) * MC_STATS_BLOCKS, hipMemcpyDeviceToHost, c->stream));
+        MC_TRY(hipStreamSynchronize(c->stream));
+        double tot = 0.0;
+        for (int b = 0; b < MC_STATS_BLOCKS; b++) {
+            lo = std::min(lo, hmin[b]);
+            hi = std::max(hi, hmax[b]);
+            tot += hsum[b];
+        }
+        mean += (float)(tot / (double)npx);
+    }
+    mean /= frames;
+    // GetFinalPixelSize (CInput.cpp:709-715) with unit magnification
+    const float pix = p->pix_size * p->ft_bin;
+    unsigned char hdr[1024];
+    mc_mrc_header(hdr, w, h, frames, pix, lo, hi, (float)mean, 32);
+    std::vector<float> ext((size_t)32 * frames, 0.0f);
+    for (int i = 0; i < frames; i++) ext[(size_t)32 * i + 11] = pix;  // CSaveExtHeader::SetPixelSize
+    const std::string path = mc_output_name(out_mrc, "_Stk", ".mrc");
+    FILE* f = fopen(path.c_str(), "wb");
+    if (!f) return fail(c, MC_E_ARG, "cannot write " + path);
+    bool ok = fwrite(hdr, 1, 1024, f) == 1024;
+    ok = ok && fwrite(ext.data(), sizeof(float), ext.size(), f) == ext.size();
+    std::vector<float> img((size_t)npx);
+    for (int i = 0; ok && i < frames; i++) {
+        if (hipMemcpyAsync(img.data(), r->stack + (long long)i * npx, sizeof(float) * npx, hipMemcpyDeviceToHost,
+                           c->stream) != hipSuccess ||
+            hipStreamSynchronize(c->stream) != hipSuccess) {
+            fclose(f);
+            return fail(c, MC_E_HIP, "save stack: copy");
+        }
+        ok = fwrite(img.data(), sizeof(float), img.size(), f) == img.size();
+    }
+    ok = (fclose(f) == 0) && ok;
+    return ok ? MC_OK : fail(c, MC_E_ARG, "cannot write " + path);
 }
 
 int mc_save_aln(const char* path, const mc_params* p, int sx, int sy, int n_stack, int N,

@@ -49,7 +49,22 @@ __global__ __launch_bounds__(256) void k_image_stats(const float* __restrict__ i
     }
 }
 
+// One frame of the aligned stack into its z-bin: dst = src for the first
+// frame of a group, dst += src for the others (Align/CZbinStack.cpp:59-70).
+__global__ __launch_bounds__(256) void k_stack_add(const float* __restrict__ src, float* __restrict__ dst,
+                                                   long long n, int first) {
+    for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long long)gridDim.x * 256)
+        dst[i] = first ? src[i] : dst[i] + src[i];
+}
+
 }  // namespace
+
+hipError_t mc_launch_stack_add(const float* src, float* dst, long long n, int first, hipStream_t s) {
+    if (n < 1) return hipErrorInvalidValue;
+    const long long blocks = std::min<long long>((n + 255) / 256, 4096);
+    hipLaunchKernelGGL(k_stack_add, dim3((unsigned)blocks), dim3(256), 0, s, src, dst, n, first);
+    return hipGetLastError();
+}
 
 hipError_t mc_launch_image_stats(const float* img, int w, int h, long long pitch, float* pmin,
                                  float* pmax, double* psum, hipStream_t s) {
@@ -76,25 +91,27 @@ std::string mc_output_name(const char* out_mrc, const char* suffix, const char* 
 }
 
 // CSaveMainHeader defaults and SetImgSize / SetMinMaxMean
-// (LibSrc/Mrcfile/Header/CSaveMainHeader.cpp:39-81,183-222): one float
-// image, cell lengths = size x pixel size.
-int mc_write_mrc_float(const char* path, const float* img, int w, int h, float pix, float amin,
-                       float amax, float amean) {
-    unsigned char hdr[1024];
-    std::memset(hdr, 0, sizeof(hdr));
+// (LibSrc/Mrcfile/Header/CSaveMainHeader.cpp:39-81,183-222): nz float
+// images, cell lengths = size x pixel size; `nreal` floats of extended
+// header per image (CSaveMrc::SetExtHeader, LibSrc/Mrcfile/App/CSaveMrc.cpp:132-151).
+void mc_mrc_header(unsigned char* hdr, int w, int h, int nz, float pix, float amin, float amax,
+                   float amean, int nreal) {
+    std::memset(hdr, 0, 1024);
     auto put_i = [&](int off, int v) { std::memcpy(hdr + off, &v, 4); };
     auto put_f = [&](int off, float v) { std::memcpy(hdr + off, &v, 4); };
     auto put_s = [&](int off, short v) { std::memcpy(hdr + off, &v, 2); };
     put_i(0, w);
     put_i(4, h);
-    put_i(8, 1);
+    put_i(8, nz);
     put_i(12, 2);  // mode: float
     put_i(28, w);
     put_i(32, h);
-    put_i(36, 1);
+    put_i(36, nz);
     put_f(40, w * pix);
     put_f(44, h * pix);
-    put_f(48, 1 * pix);
+    put_f(48, nz * pix);
+    put_i(92, nreal * 4 * nz);  // nsymbt
+    put_s(130, (short)nreal);   // numfloats
     put_f(52, 90.f);
     put_f(56, 90.f);
     put_f(60, 90.f);
@@ -115,6 +132,12 @@ int mc_write_mrc_float(const char* path, const float* img, int w, int h, float p
     std::memcpy(hdr + 208, "MAP", 4);
     hdr[212] = 0x44;  // machine stamp: little-endian
     hdr[213] = 0x44;
+}
+
+int mc_write_mrc_float(const char* path, const float* img, int w, int h, float pix, float amin,
+                       float amax, float amean) {
+    unsigned char hdr[1024];
+    mc_mrc_header(hdr, w, h, 1, pix, amin, amax, amean, 0);
     FILE* f = fopen(path, "wb");
     if (!f) return -1;
     bool ok = fwrite(hdr, 1, 1024, f) == 1024;

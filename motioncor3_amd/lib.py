@@ -31,7 +31,7 @@ class McParams(C.Structure):
         ("spectra_fmt", C.c_int), ("launch_graphs", C.c_int), ("chunk_streams", C.c_int),
         ("prior", C.c_int), ("prior_fallback", C.c_float),
         ("fm_int_start", C.POINTER(C.c_int)), ("fm_int_size", C.POINTER(C.c_int)),
-        ("movie_pipeline", C.c_int),
+        ("movie_pipeline", C.c_int), ("out_stack", C.c_int * 2),
     ]
 
 
@@ -50,7 +50,7 @@ class McResult(C.Structure):
         ("patch_iters", C.POINTER(C.c_int)), ("seconds_align", C.c_double),
         ("seconds_total", C.c_double), ("spectra_fmt_used", C.c_int * 2),
         ("prior_features", C.POINTER(C.c_float)), ("prior_targets", C.POINTER(C.c_float)),
-        ("prior_fallbacks", C.c_int),
+        ("prior_fallbacks", C.c_int), ("stack", C.c_void_p),
     ]
 
 
@@ -78,7 +78,7 @@ class McOverride(C.Structure):
 # Every symbol include/mc_abi.h declares.
 ABI_SYMBOLS = (
     "mc_ctx_create", "mc_ctx_destroy", "mc_last_error", "mc_sync", "mc_default_params",
-    "mc_output_geometry", "mc_geometry", "mc_malloc", "mc_free", "mc_memcpy", "mc_align_movie",
+    "mc_output_geometry", "mc_stack_frames", "mc_save_stack", "mc_geometry", "mc_malloc", "mc_free", "mc_memcpy", "mc_align_movie",
     "mc_align_movies", "mc_rfft2", "mc_irfft2",
     "mc_phase_shift_accumulate", "mc_group_correlate_window", "mc_peak_fit",
     "mc_spline_despike_fp64", "mc_patch_stats_fp64", "mc_local_warp", "mc_dose_weight_sum",
@@ -114,7 +114,7 @@ def default_params(**kw):
     p = McParams()
     load().mc_default_params(C.byref(p))
     for k, v in kw.items():
-        if k in ("patch", "bft", "group", "sum_range"):
+        if k in ("patch", "bft", "group", "sum_range", "out_stack"):
             arr = getattr(p, k)
             arr[0], arr[1] = v[0], v[1]
         elif k in ("log_dir", "movie_name"):
@@ -431,6 +431,15 @@ class Context:
         self._check(self.lib.mc_save_sums(self.ptr, C.byref(params), C.byref(r), out_mrc.encode()),
                     "mc_save_sums")
 
+    def save_stack(self, params, result, out_mrc):
+        """mc_save_stack: the result's aligned stack as out_mrc + "_Stk.mrc"."""
+        r = McResult()
+        st = result["stack"]
+        r.stack = st.vp()
+        r.out_h, r.out_w = st.shape[1], st.shape[2]
+        self._check(self.lib.mc_save_stack(self.ptr, C.byref(params), C.byref(r), st.shape[0],
+                                           out_mrc.encode()), "mc_save_stack")
+
     def detect_bad(self, frames, threshold=6.0, defect=(6, 6), defect_file=None, defect_map=None,
                    width=None):
         """mc_detect_bad on an [N, H, pitch] DevArray: (bad map DevArray [H, W]
@@ -491,6 +500,12 @@ class Context:
                                                 C.byref(ow), C.byref(oh)), "mc_output_geometry")
         return ns.value, ow.value, oh.value
 
+    def stack_frames(self, params, N):
+        """mc_stack_frames: frames of the -OutStack aligned stack (0 without it)."""
+        n = C.c_int()
+        self._check(self.lib.mc_stack_frames(C.byref(params), N, C.byref(n)), "mc_stack_frames")
+        return n.value
+
     def _prepare(self, frames, params, sums, width, prior_data=False):
         N, H, pitch = frames.shape
         W = pitch if width is None else width
@@ -524,6 +539,10 @@ class Context:
                            prior_targets=np.zeros((P, N, 2), np.float32))
                 res.prior_features = _fptr(out["prior_features"])
                 res.prior_targets = _fptr(out["prior_targets"])
+        if params.out_stack[0]:
+            stack = self.empty((self.stack_frames(params, N), oh, ow), np.float32)
+            res.stack = stack.vp()
+            out["stack"] = stack
         out["sums"] = sums
         out["num_sums"] = ns
         return mv, res, out

@@ -990,6 +990,7 @@ void orc_default_params(orc_params* p) {
     p->sum_range[0] = 3.0;
     p->sum_range[1] = 25.0;
     p->ref_quirks = 1;
+    p->out_stack[1] = 1;
 }
 
 int orc_num_threads(void) {
@@ -1096,11 +1097,21 @@ struct SumBank {
     const double* accdose;
     std::vector<double> wsum;
     std::vector<cd> sums;
+    const Geo* geo = nullptr;
+    double* stack = nullptr;  // aligned stack, one output-size frame per frame
     void add(cd* f, int i) {
         const size_t fs = (size_t)cx * cy;
         int ns = 0;
         for (size_t q = 0; q < fs; q++) sums[q] += f[q];
         ns++;
+        if (stack) {
+            /* Correct/CCorrectFullShift.cpp:336-356 (mCropFrame): the
+             * corrected frame before dose weighting, cropped to the -FtBin
+             * size, C2R, unpadded; no positivity clamp */
+            std::vector<cd> ob((size_t)geo->ocx * geo->ocy, cd(0, 0));
+            fourier_resize(f, cx, cy, ob.data(), geo->ocx, geo->ocy, false);
+            irfft2(ob.data(), geo->ow, geo->oh, stack + (size_t)i * geo->ow * geo->oh);
+        }
         if (dw) {
             for (int y = 0; y < cy; y++)
                 for (int k = 0; k < cx; k++) {
@@ -1136,6 +1147,25 @@ struct SumBank {
         }
     }
 };
+
+/* Align/CZbinStack.cpp:27-70: the aligned stack summed over the frame
+ * groups of CFmGroupParam::Setup(zbin) in frame order, in place; returns the
+ * number of stack frames. */
+int zbin_stack(double* stack, int N, size_t npix, int zbin, const std::vector<double>& fmcent,
+               const int* fmsize) {
+    if (zbin <= 1) return N;
+    const Groups gr = make_groups(zbin, N, fmcent, fmsize);
+    for (int k = 0; k < gr.ng; k++) {
+        double* dst = stack + (size_t)k * npix;
+        const double* src = stack + (size_t)gr.start[k] * npix;
+        if (dst != src) std::copy(src, src + npix, dst);
+        for (int j = 1; j < gr.size[k]; j++) {
+            const double* fj = stack + (size_t)(gr.start[k] + j) * npix;
+            for (size_t q = 0; q < npix; q++) dst[q] += fj[q];
+        }
+    }
+    return gr.ng;
+}
 
 /* Align::CAlignMain::DoIt (Align/CAlignMain.cpp:39-75) for full-frame
  * (Align/CFullAlign.cpp:70-171) and patch (Align/CPatchAlign.cpp:25-92)
@@ -1177,6 +1207,14 @@ int orc_align_movie(const float* frames, int W, int H, int N, const orc_params* 
     bank.r0 = prm->sum_range[0];
     bank.r1 = prm->sum_range[1];
     bank.accdose = accdose.data();
+    bank.geo = &g;
+    bank.stack = prm->out_stack[0] ? out->stack : nullptr;
+    out->stack_frames = 0;
+    auto finish_stack = [&]() {
+        if (bank.stack)
+            out->stack_frames = zbin_stack(bank.stack, N, (size_t)g.ow * g.oh, prm->out_stack[1], fmcent,
+                                           prm->fm_int_size);
+    };
     out->out_w = g.ow;
     out->out_h = g.oh;
     out->num_sums = bank.nsum;
@@ -1267,6 +1305,7 @@ int orc_align_movie(const float* frames, int W, int H, int N, const orc_params* 
             bank.add(frm.frame(i), i);
         }
         bank.finish(g, out->sums);
+        finish_stack();
         return 0;
     }
 
@@ -1338,6 +1377,7 @@ int orc_align_movie(const float* frames, int W, int H, int N, const orc_params* 
         bank.add(f, i);
     }
     bank.finish(g, out->sums);
+    finish_stack();
     return 0;
 }
 

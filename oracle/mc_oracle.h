@@ -36,6 +36,7 @@ typedef struct orc_params {
     int    ref_quirks;    /* 1: reproduce reference quirks (MakeRelative k+2 etc.) */
     const int* fm_int_start; /* -FmIntFile: raw start of each frame (NULL: i + throw0) */
     const int* fm_int_size;  /* raw frames in each frame (NULL: 1) */
+    int    out_stack[2];  /* -OutStack: [0] 1 makes the aligned stack, [1] z-bin (1) */
 } orc_params;
 
 /* Optional replacements for measured quantities.  Any NULL pointer means
@@ -66,6 +67,8 @@ typedef struct orc_out {
     int*    patch_starts;    /* P x 2 xcf px */
     int*    patch_iters;     /* P */
     unsigned char* feature_map; /* 128 x 128, may be NULL */
+    double* stack;           /* caller: >= N * W * H doubles when out_stack[0]; [frame][y][x] */
+    int     stack_frames;    /* filled by the oracle */
 } orc_out;
 
 /* Geometry that CBufferPool derives from the stack size

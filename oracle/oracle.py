@@ -29,7 +29,7 @@ class Params(C.Structure):
         ("group", C.c_int * 2), ("fm_ref", C.c_int), ("ft_bin", C.c_double),
         ("sum_range", C.c_double * 2), ("split_sum", C.c_int),
         ("throw0", C.c_int), ("ref_quirks", C.c_int),
-        ("fm_int_start", _ip), ("fm_int_size", _ip),
+        ("fm_int_start", _ip), ("fm_int_size", _ip), ("out_stack", C.c_int * 2),
     ]
 
 
@@ -46,7 +46,7 @@ class Out(C.Structure):
         ("sums", _dp), ("global_shift", _dp), ("xcf_shift", _dp),
         ("global_iters", _ip), ("global_errors", _dp), ("patch_shift", _dp),
         ("patch_bad", _up), ("patch_center", _dp), ("patch_starts", _ip),
-        ("patch_iters", _ip), ("feature_map", _up),
+        ("patch_iters", _ip), ("feature_map", _up), ("stack", _dp), ("stack_frames", C.c_int),
     ]
 
 
@@ -83,7 +83,7 @@ def default_params(**kw):
     p = Params()
     lib().orc_default_params(C.byref(p))
     for k, v in kw.items():
-        if k in ("patch", "bft", "group", "sum_range"):
+        if k in ("patch", "bft", "group", "sum_range", "out_stack"):
             arr = getattr(p, k)
             arr[0], arr[1] = v[0], v[1]
         elif k in ("fm_int_start", "fm_int_size"):
@@ -286,12 +286,18 @@ def align_movie(frames, params=None, xcf_shift=None, patch_starts=None,
     o.patch_starts = res["patch_starts"].ctypes.data_as(_ip)
     o.patch_iters = res["patch_iters"].ctypes.data_as(_ip)
     o.feature_map = res["feature_map"].ctypes.data_as(_up)
+    if p.out_stack[0]:
+        res["stack"] = np.zeros((N, H, W))
+        o.stack = _d(res["stack"])
     rc = lib().orc_align_movie(frames.ctypes.data_as(_fp), W, H, N, C.byref(p),
                                C.byref(ov), C.byref(o))
     if rc != 0:
         raise RuntimeError("orc_align_movie failed: %d" % rc)
     ns, ow, oh = o.num_sums, o.out_w, o.out_h
     res["sums"] = res["sums"].reshape(-1)[: ns * ow * oh].reshape(ns, oh, ow)
+    if p.out_stack[0]:
+        nk = o.stack_frames
+        res["stack"] = res["stack"].reshape(-1)[: nk * ow * oh].reshape(nk, oh, ow)
     res["global_iters"] = int(res["global_iters"][0])
     res["global_errors"] = res["global_errors"][: res["global_iters"]]
     if npat == 0:

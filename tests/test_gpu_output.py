@@ -66,3 +66,64 @@ def test_aln_replay_matches_oracle(ctx, tmp_path):
     for i in range(g["num_sums"]):
         s, r = g["sums"][i].numpy(), o["sums"][i]
         assert np.linalg.norm(s - r) / np.linalg.norm(r) < 1e-4, i
+
+
+@pytest.mark.parametrize("patch,ft_bin,zbin", [(False, 1.0, 1), (False, 2.0, 3), (True, 1.0, 1),
+                                               (True, 2.0, 4)])
+def test_aligned_stack_matches_oracle(ctx, patch, ft_bin, zbin):
+    """-OutStack: every corrected frame (global phase shift, or local warp in
+    patch mode) before dose weighting at the -FtBin size, z-binned, against
+    the oracle replaying the same shifts."""
+    if patch:
+        W, H, N = 1024, 1024, 8
+        kw = dict(patch=(3, 3))
+    else:
+        W, H, N = 512, 384, 10
+        kw = {}
+    kw.update(pix_size=1.0, kv=300, fm_dose=1.0, ft_bin=ft_bin, out_stack=(1, zbin))
+    mvh = MovieRecipe(W, H, N, 300, seed=95, dose=4.0, local_max=2.0 if patch else 0.0).movie()
+    mv = ctx.upload(mvh)
+    pg = L.default_params(**kw)
+    res = ctx.align_movie(mv, pg)
+    rep = dict(replay_global=res["global_shift"])
+    if patch:
+        rep.update(replay_patch=res["patch_shift"], replay_centers=res["patch_center"])
+    g = ctx.align_movie(mv, pg, **rep)
+    o = O.align_movie(mvh, O.default_params(**kw), **rep)
+    st, want = g["stack"].numpy(), o["stack"]
+    assert st.shape == want.shape == (ctx.stack_frames(pg, N),) + g["sums"][0].shape
+    assert st.shape[0] == -(-N // zbin)
+    for k in range(st.shape[0]):
+        d = st[k] - want[k]
+        # the local warp's nearest-pixel gather (Correct/GCorrectPatchShift.cu:52-65)
+        # may pick the neighbouring pixel where an FP32 coordinate sits on a
+        # pixel edge; such flips stay below 1 in 1000 pixels
+        flip = np.abs(d) > 1e-3 * np.sqrt(np.mean(want[k] ** 2))
+        assert flip.mean() < (1e-3 if patch else 1e-9), (k, flip.mean())
+        assert np.linalg.norm(d[~flip]) / np.linalg.norm(want[k][~flip]) < 2e-5, k
+    # the stack is the unclamped, unweighted sum split by frame
+    s0 = g["sums"][0].numpy()
+    assert np.linalg.norm(np.maximum(st.astype(np.float64).sum(0), 0) - s0) / np.linalg.norm(s0) < 1e-5
+
+
+def test_save_stack_mrc(ctx, tmp_path):
+    W, H, N = 384, 256, 7
+    mv = ctx.upload(MovieRecipe(W, H, N, 100, seed=97, dose=4.0).movie())
+    pg = L.default_params(pix_size=0.8, ft_bin=2.0, out_stack=(1, 2))
+    res = ctx.align_movie(mv, pg)
+    st = res["stack"].numpy()
+    assert st.shape == (4, H // 2, W // 2)
+    ctx.save_stack(pg, res, str(tmp_path / "corr.mrc"))
+    path = str(tmp_path / "corr_Stk.mrc")
+    raw = open(path, "rb").read()
+    hi, hf, data = _read_mrc(path)
+    pix = np.float32(0.8) * np.float32(2.0)
+    assert tuple(hi[:4]) == (W // 2, H // 2, 4, 2) and tuple(hi[7:10]) == (W // 2, H // 2, 4)
+    assert np.allclose(hf[10:13], [W // 2 * pix, H // 2 * pix, 4 * pix])
+    assert hi[23] == 4 * 32 * 4 and np.frombuffer(raw[128:132], np.int16)[1] == 32
+    ext = np.frombuffer(raw[1024:1024 + hi[23]], np.float32).reshape(4, 32)
+    assert np.all(ext[:, 11] == pix) and np.count_nonzero(ext) == 4
+    assert np.array_equal(data, st)
+    assert hf[19] == st.min() and hf[20] == st.max()
+    means = [np.float32(f.astype(np.float64).mean()) for f in st]
+    assert abs(hf[21] - np.mean(np.array(means, np.float64))) <= 1e-6 * abs(np.mean(means))
