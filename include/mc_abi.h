@@ -94,6 +94,10 @@ typedef struct mc_params {
     int    out_stack[2];     /* -OutStack (CInput.cpp:590-592): [0] 1 makes the
                                 aligned stack (mc_result.stack), [1] frames summed
                                 into each stack frame (z-bin, default 1) */
+    int    align;            /* -Align (CInput.cpp:563-565): 1 (default) aligns;
+                                0 makes simple sums without alignment
+                                (Align/CSimpleSum.cpp:22-135): the sum and, with
+                                split_sum, the _ODD / _EVN sums, no dose weighting */
 } mc_params;
 
 /* N frames already gain/dark/defect corrected, resident in HBM

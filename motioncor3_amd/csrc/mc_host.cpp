@@ -770,6 +770,72 @@ struct Pipe {
 // batch of one, so results do not depend on the batching.  Overrides need
 // count == 1.  pp[m] supplies movie m's log name; everything else comes
 // from pp[0].
+// -Align 0 (Align/CSimpleSum.cpp:22-135, MrcUtil/CSumFFTStack.cpp:60-210):
+// no alignment.  The frames are summed as they are (with -SplitSum also the
+// even-indexed frames, and all minus those), and each sum, and with
+// -OutStack each frame, is cropped to the -FtBin size through a normalised
+// R2C and a C2R, with no clamp at zero (CSimpleSum::mCropFrame, mUnpad).
+int simple_sums(mc_ctx* c, const mc_params* p, const mc_movie* mv, mc_result* res) {
+    const int W = mv->W, H = mv->H, N = mv->N;
+    const Geo g = make_geo(W, H, N, 0, 0, p->ft_bin);
+    const int nsum = p->split_sum ? 3 : 1;
+    res->num_sums = nsum;
+    res->out_w = g.ow;
+    res->out_h = g.oh;
+    for (int i = 0; i < nsum; i++)
+        if (!res->sums[i]) return fail(c, MC_E_ARG, "missing output sum buffer");
+    if (p->out_stack[0] && !res->stack) return fail(c, MC_E_ARG, "out_stack needs a result stack buffer");
+    Fft2D* pf = plan(c, W, H);
+    Fft2D* po = (g.ow == W && g.oh == H) ? pf : plan(c, g.ow, g.oh);
+    if (!pf || !po) return fail(c, MC_E_ARG, "unsupported FFT size");
+    hipStream_t s = c->stream;
+    MC_CHECK(wait_movie_loads(c, mv, s));
+    const long long np = (long long)W * H, fs = (long long)g.cx * H, ofs = (long long)g.ow * g.oh;
+    const float scale = (float)(1.0 / ((double)W * H));
+    const int SC = std::min(N, 8);
+    MC_BUF(R, float, "ss_sums", (size_t)nsum * np);
+    MC_BUF(T, cfloat, "ss_T", (size_t)SC * pf->t_elems);
+    MC_BUF(S, cfloat, "ss_S", (size_t)SC * fs);
+    MC_BUF(To, cfloat, "ss_To", (size_t)SC * po->t_elems);
+    auto crop = [&](const float* in, long long pitch, long long bstride, int nb, float* out) -> int {
+        MC_TRY(mc_rows_r2c(*pf, in, pitch, bstride, nullptr, T, pf->t_elems, nb, scale, s));
+        MC_TRY(mc_cols_fwd(*pf, T, pf->t_elems, S, fs, nullptr, 0, 0, 0, nb, s));
+        MC_TRY(mc_cols_inv(*po, S, fs, g.cx, H, nullptr, To, po->t_elems, nb, s));
+        MC_TRY(mc_rows_c2r(*po, To, po->t_elems, out, g.ow, ofs, nb, 0, s));
+        return MC_OK;
+    };
+    {
+        KTimer kt(c, "simple_sums");
+        MC_TRY(mc_launch_simple_sums(mv->frames, W, H, N, mv->pitch, mv->frame_stride, p->split_sum, R, s));
+    }
+    for (int i = 0; i < nsum; i++) MC_CHECK(crop(R + (long long)i * np, W, np, 1, res->sums[i]));
+    if (p->out_stack[0]) {
+        // then Align/CZbinStack.cpp:27-70 as after an alignment
+        const int zb = p->out_stack[1];
+        Groups zg;
+        if (zb > 1) zg = make_groups(zb, N, p->throw0, p->fm_int_start, p->fm_int_size);
+        float* Rst = nullptr;
+        if (zb > 1) {
+            MC_BUF(r_, float, "stk_chunk", (size_t)SC * ofs);
+            Rst = r_;
+        }
+        for (int f0 = 0; f0 < N; f0 += SC) {
+            const int nb = std::min(SC, N - f0);
+            float* dst = zb > 1 ? Rst : res->stack + (long long)f0 * ofs;
+            MC_CHECK(crop(mv->frames + (long long)f0 * mv->frame_stride, mv->pitch, mv->frame_stride, nb, dst));
+            for (int k = 0; zb > 1 && k < nb; k++) {
+                const int f = f0 + k, gi = zg.fgroup[f];
+                MC_TRY(mc_launch_stack_add(Rst + (long long)k * ofs, res->stack + (long long)gi * ofs, ofs,
+                                           f == zg.start[gi], s));
+            }
+        }
+    }
+    MC_TRY(hipStreamSynchronize(s));
+    if (res->global_shift) std::fill(res->global_shift, res->global_shift + 2 * N, 0.0f);
+    if (res->global_iters) res->global_iters[0] = 0;
+    return MC_OK;
+}
+
 int align_batch(mc_ctx* c, const mc_params* const* pp_, const mc_movie* const* mvs,
                 const mc_override* ov, mc_result* const* rs, int count, Pipe* pipe = nullptr) {
     using clk = std::chrono::steady_clock;
@@ -782,6 +848,10 @@ int align_batch(mc_ctx* c, const mc_params* const* pp_, const mc_movie* const* m
         if (!mv->frames || mv->W != W || mv->H != H || mv->N != N || W < 8 || (W % 2) || H < 8 ||
             N < 2 || mv->pitch < W || mv->frame_stride < mv->pitch * H)
             return fail(c, MC_E_ARG, "invalid movie geometry");
+    }
+    if (p->align == 0) {
+        for (int m = 0; m < count; m++) MC_CHECK(simple_sums(c, pp_[m], mvs[m], rs[m]));
+        return MC_OK;
     }
     if (p->iter < 1) return fail(c, MC_E_ARG, "-Iter must be >= 1");
     const bool patch = p->patch[0] > 1 && p->patch[1] > 1;
@@ -1387,6 +1457,7 @@ void mc_default_params(mc_params* p) {
     p->prior_fallback = 0.2f;
     p->movie_pipeline = 1;
     p->out_stack[1] = 1;
+    p->align = 1;
 }
 
 int mc_ctx_create(int device, void* stream, mc_ctx** out) {
@@ -1507,7 +1578,10 @@ int mc_output_geometry(const mc_params* p, int W, int H, int N, int* ns, int* ow
     const bool dw = !(p->kv < 80 || p->kv > 300) && p->pix_size > 0 && p->fm_dose > 0;
     const bool dws = dw && !(p->sum_range[0] < 0) && !(p->sum_range[1] <= 0) &&
                      (p->sum_range[1] - p->sum_range[0] > 1.0f);
-    *ns = 1 + (p->split_sum ? 2 : 0) + (dw ? 1 + (dws ? 1 : 0) : 0);
+    // -Align 0 makes the sum and, with -SplitSum, the odd and even sums only
+    // (DataUtil/CAlnSums.cpp:23-28)
+    *ns = p->align == 0 ? 1 + (p->split_sum ? 2 : 0)
+                        : 1 + (p->split_sum ? 2 : 0) + (dw ? 1 + (dws ? 1 : 0) : 0);
     *ow = g.ow;
     *oh = g.oh;
     return MC_OK;
@@ -1542,7 +1616,7 @@ static bool same_alignment_flags(const mc_params& a, const mc_params& b) {
            a.fm_int_start == b.fm_int_start && a.fm_int_size == b.fm_int_size &&
            a.prior_fallback == b.prior_fallback && a.verbose == b.verbose &&
            a.movie_pipeline == b.movie_pipeline && a.out_stack[0] == b.out_stack[0] &&
-           a.out_stack[1] == b.out_stack[1];
+           a.out_stack[1] == b.out_stack[1] && a.align == b.align;
 }
 
 int mc_align_movies(mc_ctx* c, const mc_params* params, const mc_movie* movies, int count,
@@ -2674,8 +2748,8 @@ int mc_save_sums(mc_ctx* c, const mc_params* p, const mc_result* r, const char* 
     const bool dws = dw && !(p->sum_range[0] < 0) && !(p->sum_range[1] <= 0) &&
                      (p->sum_range[1] - p->sum_range[0] > 1.0f);
     std::vector<const char*> sfx = {""};
-    if (dw) sfx.push_back("_DW");
-    if (dws) sfx.push_back("_DWS");
+    if (dw && p->align != 0) sfx.push_back("_DW");
+    if (dws && p->align != 0) sfx.push_back("_DWS");
     if (p->split_sum) {
         sfx.push_back("_ODD");
         sfx.push_back("_EVN");

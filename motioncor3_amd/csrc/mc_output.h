@@ -13,6 +13,8 @@ int mc_write_mrc_float(const char* path, const float* img, int w, int h, float p
                        float amax, float amean);
 void mc_mrc_header(unsigned char* hdr, int w, int h, int nz, float pix, float amin, float amax,
                    float amean, int nreal);
+hipError_t mc_launch_simple_sums(const float* frames, int W, int H, int N, long long pitch, long long fstride,
+                                 int split, float* out, hipStream_t s);
 hipError_t mc_launch_stack_add(const float* src, float* dst, long long n, int first, hipStream_t s);
 int mc_write_aln(const char* path, int sx, int sy, int nstk, int px, int py, int throw0,
                  int throw1, int N, int P, const float* gs, const float* ps, const float* pc);

@@ -57,7 +57,40 @@ __global__ __launch_bounds__(256) void k_stack_add(const float* __restrict__ src
         dst[i] = first ? src[i] : dst[i] + src[i];
 }
 
+// -Align 0 sums (MrcUtil/CSumFFTStack.cpp:60-210): the frames added in frame
+// order; with split, also the even-indexed frames, and all minus those
+// (sDoSplitSum's GAddFrames(sum0, 1, sum1, -1)).
+__global__ __launch_bounds__(256) void k_simple_sums(const float* __restrict__ frames, int W, int H, int N,
+                                                     long long pitch, long long fstride, int split,
+                                                     float* __restrict__ out) {
+    const long long np = (long long)W * H;
+    for (long long q = (long long)blockIdx.x * 256 + threadIdx.x; q < np; q += (long long)gridDim.x * 256) {
+        const int y = (int)(q / W), x = (int)(q - (long long)y * W);
+        const float* f = frames + (long long)y * pitch + x;
+        float s0 = 0.f, s1 = 0.f;
+        for (int i = 0; i < N; i++) {
+            const float v = f[(long long)i * fstride];
+            s0 += v;
+            if (i % 2 == 0) s1 += v;
+        }
+        out[q] = s0;
+        if (split) {
+            out[np + q] = s1;
+            out[2 * np + q] = s0 - s1;
+        }
+    }
+}
+
 }  // namespace
+
+hipError_t mc_launch_simple_sums(const float* frames, int W, int H, int N, long long pitch, long long fstride,
+                                 int split, float* out, hipStream_t s) {
+    if (W < 1 || H < 1 || N < 1 || pitch < W) return hipErrorInvalidValue;
+    const long long blocks = std::min<long long>(((long long)W * H + 255) / 256, 8192);
+    hipLaunchKernelGGL(k_simple_sums, dim3((unsigned)blocks), dim3(256), 0, s, frames, W, H, N, pitch, fstride,
+                       split, out);
+    return hipGetLastError();
+}
 
 hipError_t mc_launch_stack_add(const float* src, float* dst, long long n, int first, hipStream_t s) {
     if (n < 1) return hipErrorInvalidValue;

@@ -31,7 +31,7 @@ class McParams(C.Structure):
         ("spectra_fmt", C.c_int), ("launch_graphs", C.c_int), ("chunk_streams", C.c_int),
         ("prior", C.c_int), ("prior_fallback", C.c_float),
         ("fm_int_start", C.POINTER(C.c_int)), ("fm_int_size", C.POINTER(C.c_int)),
-        ("movie_pipeline", C.c_int), ("out_stack", C.c_int * 2),
+        ("movie_pipeline", C.c_int), ("out_stack", C.c_int * 2), ("align", C.c_int),
     ]
 
 

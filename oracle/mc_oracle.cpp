@@ -991,6 +991,7 @@ void orc_default_params(orc_params* p) {
     p->sum_range[1] = 25.0;
     p->ref_quirks = 1;
     p->out_stack[1] = 1;
+    p->align = 1;
 }
 
 int orc_num_threads(void) {
@@ -1167,6 +1168,52 @@ int zbin_stack(double* stack, int N, size_t npix, int zbin, const std::vector<do
     return gr.ng;
 }
 
+/* -Align 0: Align/CSimpleSum.cpp:22-135 with MrcUtil/CSumFFTStack.cpp:60-210.
+ * The frames are summed as they are (all frames; with -SplitSum also the
+ * even-indexed frames, and all minus those), and every sum and every stack
+ * frame is cropped to the -FtBin size through a normalised R2C and a C2R,
+ * with no clamp at zero. */
+int simple_sum(const float* frames, int W, int H, int N, const orc_params* prm, orc_out* out,
+               const std::vector<double>& fmcent) {
+    const Geo g = make_geo(W, H, N, 0, 0, prm->ft_bin);
+    const bool split = prm->split_sum != 0;
+    const size_t np = (size_t)W * H, onp = (size_t)g.ow * g.oh;
+    std::vector<double> s((split ? 3 : 1) * np, 0.0);
+    for (int i = 0; i < N; i++)
+        for (size_t q = 0; q < np; q++) {
+            const double v = frames[(size_t)i * np + q];
+            s[q] += v;
+            if (split && i % 2 == 0) s[np + q] += v;
+        }
+    if (split)
+        for (size_t q = 0; q < np; q++) s[2 * np + q] = s[q] - s[np + q];
+    std::vector<cd> spec((size_t)g.cx * g.cy), ob((size_t)g.ocx * g.ocy);
+    const double sc = 1.0 / ((double)W * H);
+    auto crop = [&](const double* img, double* dst) {
+        rfft2(img, W, H, spec.data());
+        for (auto& v : spec) v *= sc;
+        std::fill(ob.begin(), ob.end(), cd(0, 0));
+        fourier_resize(spec.data(), g.cx, g.cy, ob.data(), g.ocx, g.ocy, false);
+        irfft2(ob.data(), g.ow, g.oh, dst);
+    };
+    out->out_w = g.ow;
+    out->out_h = g.oh;
+    out->num_sums = split ? 3 : 1;
+    for (int k = 0; k < out->num_sums; k++) crop(&s[k * np], out->sums + k * onp);
+    if (out->global_iters) out->global_iters[0] = 0;
+    if (out->global_shift) std::fill(out->global_shift, out->global_shift + 2 * N, 0.0);
+    out->stack_frames = 0;
+    if (prm->out_stack[0] && out->stack) {
+        std::vector<double> img(np);
+        for (int i = 0; i < N; i++) {
+            for (size_t q = 0; q < np; q++) img[q] = frames[(size_t)i * np + q];
+            crop(img.data(), out->stack + i * onp);
+        }
+        out->stack_frames = zbin_stack(out->stack, N, onp, prm->out_stack[1], fmcent, prm->fm_int_size);
+    }
+    return 0;
+}
+
 /* Align::CAlignMain::DoIt (Align/CAlignMain.cpp:39-75) for full-frame
  * (Align/CFullAlign.cpp:70-171) and patch (Align/CPatchAlign.cpp:25-92)
  * modes, with the sums of CCorrectFullShift / GCorrectPatchShift. */
@@ -1194,6 +1241,7 @@ int orc_align_movie(const float* frames, int W, int H, int N, const orc_params* 
          * frame with it (mCalcIntFms, mThrowIntFrames) */
         accdose[i] = prm->fm_int_size ? prm->fm_dose * (st + sz) : prm->fm_dose * (i + 1);
     }
+    if (prm->align == 0) return simple_sum(frames, W, H, N, prm, out, fmcent);
     SumBank bank;
     bank.cx = g.cx;
     bank.cy = g.cy;

@@ -37,6 +37,7 @@ typedef struct orc_params {
     const int* fm_int_start; /* -FmIntFile: raw start of each frame (NULL: i + throw0) */
     const int* fm_int_size;  /* raw frames in each frame (NULL: 1) */
     int    out_stack[2];  /* -OutStack: [0] 1 makes the aligned stack, [1] z-bin (1) */
+    int    align;         /* -Align (1); 0: simple sums without alignment */
 } orc_params;
 
 /* Optional replacements for measured quantities.  Any NULL pointer means

@@ -30,6 +30,7 @@ class Params(C.Structure):
         ("sum_range", C.c_double * 2), ("split_sum", C.c_int),
         ("throw0", C.c_int), ("ref_quirks", C.c_int),
         ("fm_int_start", _ip), ("fm_int_size", _ip), ("out_stack", C.c_int * 2),
+        ("align", C.c_int),
     ]
 
 

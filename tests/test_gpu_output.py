@@ -127,3 +127,32 @@ def test_save_stack_mrc(ctx, tmp_path):
     assert hf[19] == st.min() and hf[20] == st.max()
     means = [np.float32(f.astype(np.float64).mean()) for f in st]
     assert abs(hf[21] - np.mean(np.array(means, np.float64))) <= 1e-6 * abs(np.mean(means))
+
+
+@pytest.mark.parametrize("ft_bin,zbin", [(1.0, 1), (2.0, 2)])
+def test_simple_sums_match_oracle(ctx, tmp_path, ft_bin, zbin):
+    """-Align 0: sums without alignment (dose weighting does not apply), the
+    _ODD / _EVN split, and the stack, against the oracle."""
+    W, H, N = 512, 384, 7
+    mvh = MovieRecipe(W, H, N, 200, seed=99, dose=4.0).movie()
+    kw = dict(align=0, split_sum=1, ft_bin=ft_bin, out_stack=(1, zbin), pix_size=1.0, kv=300,
+              fm_dose=1.0)
+    pg = L.default_params(**kw)
+    assert ctx.output_geometry(pg, W, H, N)[0] == 3
+    g = ctx.align_movie(ctx.upload(mvh), pg)
+    o = O.align_movie(mvh, O.default_params(**kw))
+    assert g["num_sums"] == 3 and g["global_iters"] == 0
+    assert not g["global_shift"].any()
+    for i in range(3):
+        s, r = g["sums"][i].numpy(), o["sums"][i]
+        assert s.shape == r.shape
+        assert np.linalg.norm(s - r) / np.linalg.norm(r) < 1e-5, i
+    st, want = g["stack"].numpy(), o["stack"]
+    assert st.shape == want.shape and st.shape[0] == -(-N // zbin)
+    for k in range(st.shape[0]):
+        assert np.linalg.norm(st[k] - want[k]) / np.linalg.norm(want[k]) < 1e-5, k
+    ctx.save_sums(pg, g, str(tmp_path / "simple.mrc"))
+    for i, nm in enumerate(["simple", "simple_ODD", "simple_EVN"]):
+        _, _, data = _read_mrc(str(tmp_path / (nm + ".mrc")))
+        assert np.array_equal(data[0], g["sums"][i].numpy()), nm
+    assert not os.path.exists(str(tmp_path / "simple_DW.mrc"))
