@@ -98,6 +98,15 @@ typedef struct mc_params {
                                 0 makes simple sums without alignment
                                 (Align/CSimpleSum.cpp:22-135): the sum and, with
                                 split_sum, the _ODD / _EVN sums, no dose weighting */
+    int    in_fm_motion;     /* -InFmMotion (0): weight each corrected frame by a
+                                sinc along its motion to the next frame
+                                (MotionDecon/GMotionWeight.cu:14-110); in patch
+                                mode twice, as Correct/CGenRealStack.cpp:156-166
+                                and the sums (Correct/CCorrectFullShift.cpp:311-320)
+                                both apply it */
+    int    corr_interp;      /* -CorrInterp (0), patch mode: divide the globally
+                                corrected frames by sinc^(1/8)
+                                (Util/GCorrLinearInterp.cu:7-24) */
 } mc_params;
 
 /* N frames already gain/dark/defect corrected, resident in HBM

@@ -43,6 +43,64 @@ __global__ void k_phasors(const double* shifts, double mult, int skip, const uns
     }
 }
 
+// Per-frequency weights of a corrected frame spectrum [kx][y], in FP32 as the
+// reference computes them.  corr_interp: -CorrInterp (Util/GCorrLinearInterp.cu:7-24),
+// divide by sinc(pi r)^(1/8).  motion: -InFmMotion
+// (MotionDecon/GMotionWeight.cu:14-40,62-74,96-110), the frame's motion is
+// its shift to the next frame (the last frame's to the one before,
+// MotionDecon/CInFrameMotion.cpp:79-97) and the weight a sinc of the
+// distance to the line through the origin perpendicular to it.  DC is
+// untouched.
+__global__ __launch_bounds__(256) void k_frame_weights(cfloat* S, long long fs, int N, int cx, int H,
+                                                       const double* shifts, int motion, int corr_interp) {
+    const int kx = blockIdx.y, f = blockIdx.z;
+    const int y = blockIdx.x * blockDim.x + threadIdx.x;
+    if (y >= H || (kx == 0 && y == 0)) return;
+    cfloat* q = S + (long long)f * fs + (long long)kx * H + y;
+    cfloat v = *q;
+    if (corr_interp) {
+        float fx = kx * 0.5f / (cx - 1);
+        float fy = y / (float)H;
+        if (fy > 0.5f) fy -= 1.0f;
+        fx = sqrtf(fx * fx + fy * fy) * 3.141592654f;
+        fx = sinf(fx) / fx;
+        fx = powf(fx, 0.125f);
+        v.x /= fx;
+        v.y /= fx;
+    }
+    if (motion) {
+        int a = f, b = f + 1;
+        if (f == N - 1) {
+            a = f - 1;
+            b = f;
+        }
+        const float mx = (float)shifts[2 * b] - (float)shifts[2 * a];
+        const float my = (float)shifts[2 * b + 1] - (float)shifts[2 * a + 1];
+        const float ms = sqrtf(mx * mx + my * my);
+        if (!((double)ms < 0.1) && !(ms < 0.1f)) {
+            const float slope = -mx / (my + 1e-20f);
+            float dist;
+            if (fabsf(slope) > 1e5f) {
+                dist = (float)kx;
+            } else {
+                const int yy = (y > H / 2) ? y - H : y;
+                dist = fabsf(slope * kx - yy) / sqrtf(slope * slope + 1.0f);
+            }
+            if (!(dist < 0.1f)) {
+                dist = dist / (float)(2 * (cx - 1));
+                float w = 0.0f;
+                if (!(ms * dist > 1.0f)) {
+                    const float a_ = 3.141592654f * ms * dist;
+                    w = sinf(a_) / a_;
+                }
+                v.x *= w;
+                v.y *= w;
+            }
+        }
+    }
+    *q = v;
+}
+
 // CAlignedSum (Align/CAlignedSum.cpp:25-126): S = sum_i shift(F_i). Even and
 // odd frames accumulate separately and are added at the end, as the
 // reference's two streams do (:99-112).
@@ -1299,6 +1357,15 @@ hipError_t mc_launch_phasors(const double* shifts, double mult, int skip, const 
     dim3 grid(mc_div_up(cx + H, 256), batch);
     hipLaunchKernelGGL(k_phasors, grid, dim3(256), 0, s, shifts, mult, skip, done,
                        per_job > 0 ? per_job : 1, W, cx, H, h_norm > 0 ? h_norm : H, A, B);
+    return hipGetLastError();
+}
+
+hipError_t mc_launch_frame_weights(cfloat* S, long long fs, int N, int cx, int H, const double* shifts,
+                                   int motion, int corr_interp, hipStream_t s) {
+    if (N < 2 || cx < 2 || H < 1 || (motion && !shifts)) return hipErrorInvalidValue;
+    if (!motion && !corr_interp) return hipSuccess;
+    hipLaunchKernelGGL(k_frame_weights, dim3(mc_div_up(H, 256), cx, N), dim3(256), 0, s, S, fs, N, cx, H,
+                       shifts, motion, corr_interp);
     return hipGetLastError();
 }
 

@@ -1051,6 +1051,9 @@ int align_batch(mc_ctx* c, const mc_params* const* pp_, const mc_movie* const* m
             MC_TRY(hipMemcpyAsync(gsh_e, c->pin_shifts, sizeof(double) * 2 * N, hipMemcpyHostToDevice,
                                   c->s2));
             MC_TRY(mc_launch_phasors(gsh_e, -1.0, 1, nullptr, N, W, g.cx, H, N, fA_e, fB_e, c->s2));
+            // Correct/CGenRealStack.cpp:132-166: -CorrInterp, -InFmMotion
+            MC_TRY(mc_launch_frame_weights(Sf_all, fs, N, g.cx, H, gsh_e, p->in_fm_motion, p->corr_interp,
+                                           c->s2));
             const long long rfs = (long long)W * H;
             for (int f0 = 0; f0 < N; f0 += CH) {
                 const int nb = std::min(CH, N - f0);
@@ -1274,6 +1277,8 @@ int align_batch(mc_ctx* c, const mc_params* const* pp_, const mc_movie* const* m
             } else {
             MC_TRY(hipMemcpyAsync(gsh, gshift.data(), sizeof(double) * 2 * N, hipMemcpyHostToDevice, s));
             MC_TRY(mc_launch_phasors(gsh, -1.0, 1, nullptr, N, W, g.cx, H, N, fA, fB, s));
+            // Correct/CGenRealStack.cpp:132-166: -CorrInterp, -InFmMotion
+            MC_TRY(mc_launch_frame_weights(Sf, fs, N, g.cx, H, gsh, p->in_fm_motion, p->corr_interp, s));
             MC_TRY(fork());
             for (int f0 = 0; f0 < N; f0 += CH) {
                 const int nb = std::min(CH, N - f0);
@@ -1322,6 +1327,10 @@ int align_batch(mc_ctx* c, const mc_params* const* pp_, const mc_movie* const* m
                 }
             }
             MC_TRY(join());
+            // mGenSums' mMotionDecon (Correct/CCorrectFullShift.cpp:271-320) with
+            // the same full shifts (Correct/GCorrectPatchShift.cu:106): the
+            // motion weight a second time
+            MC_TRY(mc_launch_frame_weights(Sf, fs, N, g.cx, H, gsh, p->in_fm_motion, 0, s));
             KTimer kt(c, "dw_sums");
             MC_TRY(mc_launch_dw_sums(Sf, fs, N, g.cx, H, nullptr, nullptr, dw, p->pix_size, kvf,
                                      p->fm_dose, dws, p->sum_range[0], p->sum_range[1], p->split_sum,
@@ -1330,6 +1339,10 @@ int align_batch(mc_ctx* c, const mc_params* const* pp_, const mc_movie* const* m
             // Correct/CCorrectFullShift.cpp:229-309 with the absolute shifts
             MC_TRY(hipMemcpyAsync(gsh, gshift.data(), sizeof(double) * 2 * N, hipMemcpyHostToDevice, s));
             MC_TRY(mc_launch_phasors(gsh, -1.0, 1, nullptr, N, W, g.cx, H, N, fA, fB, s));
+            // mGenSums' mMotionDecon (Correct/CCorrectFullShift.cpp:271-320);
+            // the weight commutes with the phase shift, so it goes on the
+            // uncorrected spectra
+            MC_TRY(mc_launch_frame_weights(Sf, fs, N, g.cx, H, gsh, p->in_fm_motion, 0, s));
             KTimer kt(c, "dw_sums");
             MC_TRY(mc_launch_dw_sums(Sf, fs, N, g.cx, H, fA, fB, dw, p->pix_size, kvf, p->fm_dose, dws,
                                      p->sum_range[0], p->sum_range[1], p->split_sum, sums, fs, s, dcum));
@@ -1616,7 +1629,8 @@ static bool same_alignment_flags(const mc_params& a, const mc_params& b) {
            a.fm_int_start == b.fm_int_start && a.fm_int_size == b.fm_int_size &&
            a.prior_fallback == b.prior_fallback && a.verbose == b.verbose &&
            a.movie_pipeline == b.movie_pipeline && a.out_stack[0] == b.out_stack[0] &&
-           a.out_stack[1] == b.out_stack[1] && a.align == b.align;
+           a.out_stack[1] == b.out_stack[1] && a.align == b.align && a.in_fm_motion == b.in_fm_motion &&
+           a.corr_interp == b.corr_interp;
 }
 
 int mc_align_movies(mc_ctx* c, const mc_params* params, const mc_movie* movies, int count,

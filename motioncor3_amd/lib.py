@@ -32,6 +32,7 @@ class McParams(C.Structure):
         ("prior", C.c_int), ("prior_fallback", C.c_float),
         ("fm_int_start", C.POINTER(C.c_int)), ("fm_int_size", C.POINTER(C.c_int)),
         ("movie_pipeline", C.c_int), ("out_stack", C.c_int * 2), ("align", C.c_int),
+        ("in_fm_motion", C.c_int), ("corr_interp", C.c_int),
     ]
 
 

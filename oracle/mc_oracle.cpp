@@ -1089,6 +1089,68 @@ void orc_patch_centers(const unsigned char* feat, int px, int py, int xw, int xh
     patch_starts_from_features(feat, px, py, xw, xh, starts);
 }
 
+/* Per-frequency weights applied to a corrected frame spectrum.
+ * -CorrInterp (Util/GCorrLinearInterp.cu:7-24, patch mode only, in
+ * Correct/CGenRealStack.cpp:132-138): divide by sinc(pi r)^(1/8), r the
+ * frequency radius in cycles per pixel.  -InFmMotion
+ * (MotionDecon/GMotionWeight.cu:14-40,62-74,96-110): for the frame's motion
+ * m (CInFrameMotion::GetFullMotion, MotionDecon/CInFrameMotion.cpp:79-97),
+ * multiply by sinc(pi |m| d / (2 (cx - 1))) with d the distance of (kx, ky)
+ * from the line through the origin perpendicular to m (the reference's
+ * slope form, kx itself when the slope exceeds 1e5), 1 when d < 0.1 and 0
+ * once |m| d / (2 (cx - 1)) > 1; no weight when |m| < 0.1.  DC is untouched. */
+double frame_weight(int k, int y, int cx, int cy, double mx, double my, bool motion, bool corr) {
+    if (k == 0 && y == 0) return 1.0;
+    double w = 1.0;
+    if (corr) {
+        double fx = k * 0.5 / (cx - 1), fy = (double)y / cy;
+        if (fy > 0.5) fy -= 1.0;
+        const double r = std::sqrt(fx * fx + fy * fy) * M_PI;
+        w /= std::pow(std::sin(r) / r, 0.125);
+    }
+    const double ms = std::sqrt(mx * mx + my * my);
+    if (motion && !(ms < 0.1)) {
+        const double slope = -mx / (my + 1e-20);
+        double d;
+        if (std::fabs(slope) > 1e5) d = k;
+        else {
+            const int yy = (y > cy / 2) ? y - cy : y;
+            d = std::fabs(slope * k - yy) / std::sqrt(slope * slope + 1.0);
+        }
+        if (!(d < 0.1)) {
+            d /= 2.0 * (cx - 1);
+            if (ms * d > 1.0) w *= 0.0;
+            else w *= std::sin(M_PI * ms * d) / (M_PI * ms * d);
+        }
+    }
+    return w;
+}
+
+/* CInFrameMotion::GetFullMotion: the shift to the next frame, the last
+ * frame's to the one before (float differences of the stored shifts). */
+void frame_motion(const std::vector<double>& gx, const std::vector<double>& gy, int i, double* m) {
+    const int n = (int)gx.size();
+    int a = i, b = i + 1;
+    if (i == n - 1) {
+        a = i - 1;
+        b = i;
+    }
+    m[0] = (float)gx[b] - (float)gx[a];
+    m[1] = (float)gy[b] - (float)gy[a];
+}
+
+void weigh_frame(cd* f, int cx, int cy, const double* m, bool motion, bool corr) {
+    if (!motion && !corr) return;
+    for (int y = 0; y < cy; y++)
+        for (int k = 0; k < cx; k++) f[(size_t)y * cx + k] *= frame_weight(k, y, cx, cy, m[0], m[1], motion, corr);
+}
+
+void orc_frame_weight(int cx, int cy, double mx, double my, int motion, int corr, double* w) {
+    for (int y = 0; y < cy; y++)
+        for (int k = 0; k < cx; k++)
+            w[(size_t)y * cx + k] = frame_weight(k, y, cx, cy, mx, my, motion != 0, corr != 0);
+}
+
 /* Sum buffers of Correct/CCorrectFullShift.cpp:187-300 (mInit, mGenSums,
  * mDoseWeight, mSum) and the output step :146-185 (mUnpadSums). */
 struct SumBank {
@@ -1350,6 +1412,10 @@ int orc_align_movie(const float* frames, int W, int H, int N, const orc_params* 
             for (int i = 0; i < N; i++) { out->global_shift[2 * i] = gx[i]; out->global_shift[2 * i + 1] = gy[i]; }
         for (int i = 0; i < N; i++) {
             phase_shift_skip(frm.frame(i), g.cx, g.cy, -gx[i], -gy[i]);
+            /* mGenSums' mMotionDecon (Correct/CCorrectFullShift.cpp:271-320) */
+            double m[2];
+            frame_motion(gx, gy, i, m);
+            weigh_frame(frm.frame(i), g.cx, g.cy, m, prm->in_fm_motion != 0, false);
             bank.add(frm.frame(i), i);
         }
         bank.finish(g, out->sums);
@@ -1362,7 +1428,13 @@ int orc_align_movie(const float* frames, int W, int H, int N, const orc_params* 
     {
         const double rx = gx[ref_frame], ry = gy[ref_frame];
         for (int i = 0; i < N; i++) { gx[i] -= rx; gy[i] -= ry; }
-        for (int i = 0; i < N; i++) phase_shift_skip(frm.frame(i), g.cx, g.cy, -gx[i], -gy[i]);
+        for (int i = 0; i < N; i++) {
+            phase_shift_skip(frm.frame(i), g.cx, g.cy, -gx[i], -gy[i]);
+            /* CGenRealStack::mCorrectBilinear, mMotionDecon (:132-166) */
+            double m[2];
+            frame_motion(gx, gy, i, m);
+            weigh_frame(frm.frame(i), g.cx, g.cy, m, prm->in_fm_motion != 0, prm->corr_interp != 0);
+        }
     }
     if (out->global_shift)
         for (int i = 0; i < N; i++) { out->global_shift[2 * i] = gx[i]; out->global_shift[2 * i + 1] = gy[i]; }
@@ -1422,6 +1494,11 @@ int orc_align_movie(const float* frames, int W, int H, int N, const orc_params* 
                    wimg.data());
         rfft2(wimg.data(), W, H, f);
         for (size_t q = 0; q < (size_t)g.cx * g.cy; q++) f[q] *= sc;
+        /* mGenSums' mMotionDecon with the same full shifts
+         * (Correct/GCorrectPatchShift.cu:106): applied a second time */
+        double m[2];
+        frame_motion(gx, gy, i, m);
+        weigh_frame(f, g.cx, g.cy, m, prm->in_fm_motion != 0, false);
         bank.add(f, i);
     }
     bank.finish(g, out->sums);

@@ -38,6 +38,8 @@ typedef struct orc_params {
     const int* fm_int_size;  /* raw frames in each frame (NULL: 1) */
     int    out_stack[2];  /* -OutStack: [0] 1 makes the aligned stack, [1] z-bin (1) */
     int    align;         /* -Align (1); 0: simple sums without alignment */
+    int    in_fm_motion;  /* -InFmMotion (0) */
+    int    corr_interp;   /* -CorrInterp (0) */
 } orc_params;
 
 /* Optional replacements for measured quantities.  Any NULL pointer means
@@ -94,6 +96,10 @@ void orc_peak2d(const double* img, int nx, int ny, double* shift);
 int  orc_lin_eqs(double* coeff, double* vals, int dim);
 void orc_remove_spikes(double* sx, double* sy, int n, int single, int quirks);
 void orc_smooth(double* sx, double* sy, int n, double weight);
+/* Weight map [cy][cx] of a half spectrum: -CorrInterp's 1 / sinc^(1/8) when
+ * corr_interp, times -InFmMotion's directional sinc for motion (mx, my). */
+void orc_frame_weight(int cmpx, int cmpy, double mx, double my, int motion, int corr_interp,
+                      double* w);
 void orc_dose_weights(int cmpx, int cmpy, double pix, int kv,
                       const double* acc_dose, int nfrm, double* wsum);
 void orc_make_relative(double* shifts, unsigned char* flags_region, int nfrm,

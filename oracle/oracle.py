@@ -30,7 +30,7 @@ class Params(C.Structure):
         ("sum_range", C.c_double * 2), ("split_sum", C.c_int),
         ("throw0", C.c_int), ("ref_quirks", C.c_int),
         ("fm_int_start", _ip), ("fm_int_size", _ip), ("out_stack", C.c_int * 2),
-        ("align", C.c_int),
+        ("align", C.c_int), ("in_fm_motion", C.c_int), ("corr_interp", C.c_int),
     ]
 
 
@@ -182,6 +182,13 @@ def remove_spikes(sx, sy, single=False, quirks=True):
     y = np.ascontiguousarray(sy, dtype=np.float64).copy()
     lib().orc_remove_spikes(_d(x), _d(y), len(x), int(single), int(quirks))
     return x, y
+
+
+def frame_weight(cx, cy, mx, my, motion=True, corr_interp=False):
+    """Weight map [cy, cx] of -InFmMotion / -CorrInterp (see mc_oracle.cpp)."""
+    w = np.zeros((cy, cx))
+    lib().orc_frame_weight(cx, cy, C.c_double(mx), C.c_double(my), int(motion), int(corr_interp), _d(w))
+    return w
 
 
 def smooth(sx, sy, weight):

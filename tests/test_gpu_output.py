@@ -156,3 +156,45 @@ def test_simple_sums_match_oracle(ctx, tmp_path, ft_bin, zbin):
         _, _, data = _read_mrc(str(tmp_path / (nm + ".mrc")))
         assert np.array_equal(data[0], g["sums"][i].numpy()), nm
     assert not os.path.exists(str(tmp_path / "simple_DW.mrc"))
+
+
+@pytest.mark.parametrize("patch", [False, True])
+def test_in_frame_motion_and_interp_weights_match_oracle(ctx, patch):
+    """-InFmMotion (and -CorrInterp in patch mode): the corrected frames are
+    weighted before the sums and the stack, against the oracle replaying the
+    same shifts."""
+    if patch:
+        W, H, N = 1024, 1024, 8
+        kw = dict(patch=(3, 3), corr_interp=1)
+    else:
+        W, H, N = 512, 384, 10
+        kw = {}
+    kw.update(pix_size=1.0, kv=300, fm_dose=1.0, in_fm_motion=1, out_stack=(1, 1))
+    mvh = MovieRecipe(W, H, N, 300, seed=101, dose=4.0, step_sigma=1.5,
+                      local_max=2.0 if patch else 0.0).movie()
+    mv = ctx.upload(mvh)
+    pg = L.default_params(**kw)
+    res = ctx.align_movie(mv, pg)
+    assert (np.abs(np.diff(res["global_shift"], axis=0)).max(1) > 0.1).any()
+    rep = dict(replay_global=res["global_shift"])
+    if patch:
+        rep.update(replay_patch=res["patch_shift"], replay_centers=res["patch_center"],
+                   replay_bad=res["patch_bad"])
+    g = ctx.align_movie(mv, pg, **rep)
+    # the measured run weighs on the second stream beside the patch alignment;
+    # the replay of its shifts and flags weighs in order: same sums
+    for i in range(g["num_sums"]):
+        a, b = res["sums"][i].numpy(), g["sums"][i].numpy()
+        assert np.linalg.norm(a - b) / np.linalg.norm(b) < 1e-5, i
+    o = O.align_movie(mvh, O.default_params(**kw), **rep)
+    plain = O.align_movie(mvh, O.default_params(**dict(kw, in_fm_motion=0, corr_interp=0)), **rep)
+    for i in range(g["num_sums"]):
+        s, r = g["sums"][i].numpy(), o["sums"][i]
+        assert np.linalg.norm(s - r) / np.linalg.norm(r) < 1e-4, i
+        assert np.linalg.norm(plain["sums"][i] - r) / np.linalg.norm(r) > 1e-3, i
+    st, want = g["stack"].numpy(), o["stack"]
+    for k in range(N):
+        d = st[k] - want[k]
+        flip = np.abs(d) > 1e-3 * np.sqrt(np.mean(want[k] ** 2))
+        assert flip.mean() < (1e-3 if patch else 1e-9), (k, flip.mean())
+        assert np.linalg.norm(d[~flip]) / np.linalg.norm(want[k][~flip]) < 2e-5, k
