@@ -329,6 +329,11 @@ int mc_fm_int_table(const char* path, int n_raw, float fm_dose, int throw0, int 
                     int* n_out, int* start, int* size, float* acc_dose, float* raw_dose,
                     int* integrated);
 int mc_mrc_read(const char* path, int first, int count, void* dst);
+/* Movies in one MRC file (MrcUtil/CLoadStack.cpp:40-55,95-112): each has
+ * `frames` frames, movie s being frames [s * frames, (s + 1) * frames).
+ * frames = stack_z (-StackZ) when above 1, else the header's mz when above 1,
+ * else nz.  stacks = nz / stack_z with -StackZ, else 1. */
+int mc_mrc_stacks(const char* path, int stack_z, int* frames, int* stacks);
 
 /* TIFF movies (TiffUtil/CLoadTiffHeader.cpp:18-147): size of the first
  * directory, directory count, MRC mode of the samples (8 bits -> 0, 16 bits ->

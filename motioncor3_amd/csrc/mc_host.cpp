@@ -2447,7 +2447,7 @@ unsigned int bswap32(unsigned int v) {
     return (v >> 24) | ((v >> 8) & 0xff00u) | ((v << 8) & 0xff0000u) | (v << 24);
 }
 
-// Header words 1-4 (nx ny nz mode) and 24 (nsymbt); a file written with the
+// Header words 1-4 (nx ny nz mode), 24 (nsymbt) and 10 (mz); a file written with the
 // other byte order shows a negative or oversized value there
 // (LibSrc/Mrcfile/Header/CLoadMainHeader.cpp:27-35).
 int mrc_header(const char* path, int* w, bool* swapped) {
@@ -2457,12 +2457,12 @@ int mrc_header(const char* path, int* w, bool* swapped) {
     const size_t got = fread(hdr, 4, 256, f);
     fclose(f);
     if (got != 256) return MC_E_ARG;
-    int v[5] = {(int)hdr[0], (int)hdr[1], (int)hdr[2], (int)hdr[3], (int)hdr[23]};
+    int v[6] = {(int)hdr[0], (int)hdr[1], (int)hdr[2], (int)hdr[3], (int)hdr[23], (int)hdr[9]};
     const int lim = 32 * 1024;
     bool sw = v[3] < 0 || v[0] < 0 || v[1] < 0 || v[2] < 0 || v[0] > lim || v[1] > lim || v[2] > lim;
     if (sw)
-        for (int i = 0; i < 5; i++) v[i] = (int)bswap32((unsigned int)v[i]);
-    for (int i = 0; i < 5; i++) w[i] = v[i];
+        for (int i = 0; i < 6; i++) v[i] = (int)bswap32((unsigned int)v[i]);
+    for (int i = 0; i < 6; i++) w[i] = v[i];
     *swapped = sw;
     if (w[0] < 1 || w[1] < 1 || w[2] < 1 || mc_raw_row_bytes(w[3], w[0]) < 0 || w[4] < 0) return MC_E_ARG;
     return MC_OK;
@@ -2472,7 +2472,7 @@ int mrc_header(const char* path, int* w, bool* swapped) {
 
 int mc_mrc_info(const char* path, int* sx, int* sy, int* n, int* mode, long long* data_offset) {
     if (!path) return MC_E_ARG;
-    int w[5];
+    int w[6];
     bool sw;
     if (mrc_header(path, w, &sw) != MC_OK) return MC_E_ARG;
     if (sx) *sx = w[0];
@@ -2483,8 +2483,24 @@ int mc_mrc_info(const char* path, int* sx, int* sy, int* n, int* mode, long long
     return MC_OK;
 }
 
+int mc_mrc_stacks(const char* path, int stack_z, int* frames, int* stacks) {
+    if (!path || !frames || !stacks) return MC_E_ARG;
+    int w[6];
+    bool sw;
+    if (mrc_header(path, w, &sw) != MC_OK) return MC_E_ARG;
+    // MrcUtil/CLoadStack.cpp:49-53,100-107: -StackZ, else the header's mz
+    // when above 1, else every section; only -StackZ makes several movies
+    int z = w[2];
+    if (stack_z > 1) z = stack_z;
+    else if (w[5] > 1) z = w[5];
+    if (z > w[2]) return MC_E_ARG;
+    *frames = z;
+    *stacks = stack_z > 1 ? w[2] / stack_z : 1;
+    return MC_OK;
+}
+
 int mc_mrc_read(const char* path, int first, int count, void* dst) {
-    int w[5];
+    int w[6];
     bool sw;
     if (!path || !dst || mrc_header(path, w, &sw) != MC_OK) return MC_E_ARG;
     if (first < 0 || count < 1 || first + count > w[2]) return MC_E_ARG;

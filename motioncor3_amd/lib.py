@@ -88,7 +88,7 @@ ABI_SYMBOLS = (
     "mc_prior_forward", "mc_prior_loss_grad", "mc_prior_train", "mc_set_prior", "mc_gemm_bf16",
     "mc_autotune", "mc_fft_variant",
     "mc_host_alloc", "mc_host_free", "mc_set_refs", "mc_get_refs", "mc_load_frames", "mc_load_wait",
-    "mc_mrc_info", "mc_mrc_read", "mc_tiff_info", "mc_tiff_read", "mc_eer_info", "mc_eer_read",
+    "mc_mrc_info", "mc_mrc_read", "mc_mrc_stacks", "mc_tiff_info", "mc_tiff_read", "mc_eer_info", "mc_eer_read",
     "mc_load_eer",
     "mc_save_sums", "mc_save_aln", "mc_load_aln", "mc_save_star", "mc_fm_int_table",
     "mc_detect_bad", "mc_correct_bad",
@@ -722,6 +722,14 @@ def mrc_info(path):
         raise RuntimeError("mc_mrc_info failed for %s" % path)
     return dict(sx=v[0].value, sy=v[1].value, n=v[2].value, mode=v[3].value,
                 data_offset=off.value)
+
+
+def mrc_stacks(path, stack_z=0):
+    """mc_mrc_stacks: (frames per movie, movies) of an MRC file under -StackZ."""
+    fr, st = C.c_int(), C.c_int()
+    if load().mc_mrc_stacks(path.encode(), stack_z, C.byref(fr), C.byref(st)) != MC_OK:
+        raise RuntimeError("mc_mrc_stacks failed for %s" % path)
+    return fr.value, st.value
 
 
 def mrc_read(path, first=0, count=None, out=None):

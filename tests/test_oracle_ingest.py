@@ -94,3 +94,24 @@ def test_mrc_reader_round_trip(tmp_path, mode, ext, swap):
         assert np.array_equal(raw.view(dt).reshape(2, sy, sx), frames[1:3].astype(dt))
     with pytest.raises(RuntimeError):
         L.mrc_read(path, 2, 2)
+
+
+@pytest.mark.parametrize("swap", [False, True])
+def test_mrc_stacks(tmp_path, swap):
+    """-StackZ (MrcUtil/CLoadStack.cpp:40-55,95-112): the flag splits the
+    file into nz / StackZ movies; without it a header mz above 1 bounds the
+    one movie; otherwise the movie is every section."""
+    import struct
+    path = str(tmp_path / "s.mrc")
+    write_mrc(path, np.zeros((6, 4, 8)), 2, swap=swap)
+    assert L.mrc_stacks(path) == (6, 1)
+    assert L.mrc_stacks(path, 3) == (3, 2)
+    assert L.mrc_stacks(path, 4) == (4, 1)
+    with pytest.raises(RuntimeError):
+        L.mrc_stacks(path, 7)
+    raw = bytearray(open(path, "rb").read())
+    for mz, want in ((2, (2, 1)), (1, (6, 1))):
+        struct.pack_into(">i" if swap else "<i", raw, 36, mz)
+        open(path, "wb").write(bytes(raw))
+        assert L.mrc_stacks(path) == want, mz
+        assert L.mrc_stacks(path, 3) == (3, 2)
