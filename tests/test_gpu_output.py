@@ -1,6 +1,8 @@
-"""Output files on the GPU path: MRC sums with header statistics, and an
-.aln file written after an alignment and replayed (-InAln) against the
-oracle's replay of the same file."""
+"""Outputs on the GPU path against the oracle: MRC sums with header
+statistics; an .aln file written after an alignment and replayed (-InAln);
+the -OutStack aligned stack and its _Stk.mrc file; -Align 0 simple sums; the
+-InFmMotion / -CorrInterp frame weights."""
+import ctypes
 import os
 
 import numpy as np
@@ -198,3 +200,24 @@ def test_in_frame_motion_and_interp_weights_match_oracle(ctx, patch):
         flip = np.abs(d) > 1e-3 * np.sqrt(np.mean(want[k] ** 2))
         assert flip.mean() < (1e-3 if patch else 1e-9), (k, flip.mean())
         assert np.linalg.norm(d[~flip]) / np.linalg.norm(want[k][~flip]) < 2e-5, k
+
+
+def test_stack_arguments_are_checked(ctx, tmp_path):
+    """A result without a stack buffer under -OutStack, and saving a result
+    with no stack, fail with an error instead of writing anywhere."""
+    W, H, N = 256, 256, 4
+    mv = ctx.upload(MovieRecipe(W, H, N, 50, seed=103, dose=4.0).movie())
+    pg = L.default_params(out_stack=(1, 1))
+    for align in (1, 0):
+        pg.align = align
+        mvs, res, out = ctx._prepare(mv, pg, None, None)
+        res.stack = None
+        with pytest.raises(RuntimeError, match="stack"):
+            ctx._check(ctx.lib.mc_align_movie(ctx.ptr, ctypes.byref(pg), ctypes.byref(mvs), None,
+                                              ctypes.byref(res)), "mc_align_movie")
+    r = L.McResult()
+    r.out_w, r.out_h = W, H
+    with pytest.raises(RuntimeError, match="stack"):
+        ctx._check(ctx.lib.mc_save_stack(ctx.ptr, ctypes.byref(pg), ctypes.byref(r), 1,
+                                         str(tmp_path / "x.mrc").encode()), "mc_save_stack")
+    assert not os.path.exists(str(tmp_path / "x_Stk.mrc"))
