@@ -221,3 +221,17 @@ def test_stack_arguments_are_checked(ctx, tmp_path):
         ctx._check(ctx.lib.mc_save_stack(ctx.ptr, ctypes.byref(pg), ctypes.byref(r), 1,
                                          str(tmp_path / "x.mrc").encode()), "mc_save_stack")
     assert not os.path.exists(str(tmp_path / "x_Stk.mrc"))
+
+
+@pytest.mark.parametrize("pipeline", [1, 0])
+def test_stack_of_batched_movies_matches_single_calls(ctx, pipeline):
+    """mc_align_movies fills every movie's stack as separate calls do."""
+    W, H, N = 512, 384, 6
+    movies = [ctx.upload(MovieRecipe(W, H, N, 150, seed=105 + k, dose=4.0).movie()) for k in range(3)]
+    pg = L.default_params(out_stack=(1, 2), pix_size=1.0, kv=300, fm_dose=1.0,
+                          movie_pipeline=pipeline)
+    batch = ctx.align_movies(movies, pg)
+    for k, mv in enumerate(movies):
+        one = ctx.align_movie(mv, pg)
+        assert np.array_equal(batch[k]["stack"].numpy(), one["stack"].numpy()), k
+        assert np.array_equal(batch[k]["sums"][1].numpy(), one["sums"][1].numpy()), k
