@@ -1,5 +1,6 @@
-// mc_output.hip -- output files after the alignment: MRC sums with header
-// statistics, the .aln alignment file (save and load) and the Relion .star
+// mc_output.hip -- outputs after the alignment: MRC sums and the aligned
+// stack with header statistics, the z-binning of the stack, the -Align 0
+// frame sums, the .aln alignment file (save and load) and the Relion .star
 // file of the global shifts.
 #include <hip/hip_runtime.h>
 
