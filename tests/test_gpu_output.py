@@ -235,3 +235,42 @@ def test_stack_of_batched_movies_matches_single_calls(ctx, pipeline):
         one = ctx.align_movie(mv, pg)
         assert np.array_equal(batch[k]["stack"].numpy(), one["stack"].numpy()), k
         assert np.array_equal(batch[k]["sums"][1].numpy(), one["sums"][1].numpy()), k
+
+
+@pytest.mark.parametrize("patch", [False, True])
+def test_shift_logs(ctx, tmp_path, patch):
+    """-LogDir: the log files named as CAlignMain::mCreateLogFile does
+    (Align/CAlignMain.cpp:77-107) hold the result's shifts in the printf
+    formats of CFullAlign::LogShift (Align/CFullAlign.cpp:173-194) and
+    CPatchShifts::LogFullShifts / LogPatchShifts / LogFrameShifts
+    (Align/CPatchShifts.cpp:155-249)."""
+    W, H, N = (1024, 1024, 6) if patch else (512, 384, 6)
+    mv = ctx.upload(MovieRecipe(W, H, N, 200, seed=107, dose=4.0, local_max=2.0 if patch else 0.0).movie())
+    kw = dict(patch=(3, 3)) if patch else {}
+    pg = L.default_params(log_dir=str(tmp_path) + "/", movie_name="/data/Movie_007.TIFF", **kw)
+    r = ctx.align_movie(mv, pg)
+    stem = str(tmp_path / "Movie_007")
+    gs = r["global_shift"]
+    full = "".join("%4d  %8.2f  %8.2f\n" % (i + 1, gs[i, 0], gs[i, 1]) for i in range(N))
+    if not patch:
+        assert open(stem + "-Full.log").read() == "# full frame alignment\n" + full
+        assert not os.path.exists(stem + "-Patch-Full.log")
+        return
+    assert open(stem + "-Patch-Full.log").read() == (
+        "# Patch based alignment\n# Initial alignment based upon full frames\n\n" + full)
+    ps, pc, bad = r["patch_shift"], r["patch_center"], r["patch_bad"]
+    P = ps.shape[1]
+
+    def row(j, i):
+        return "%4d %8.2f %8.2f " % (j + 1, pc[i, 0], pc[i, 1]) + \
+            "%8.2f %8.2f %4d\n" % (ps[j, i, 0], ps[j, i, 1], bad[j, i])
+
+    head = "# Patch based alignment\n# Number of patches: %d\n" % P
+    size = "# Movie size: %d  %d  %d\n\n" % (W, H, N)
+    per_patch = "".join("#Patch %03d raw and fit shifts\n" % (i + 1) +
+                        "".join(row(j, i) for j in range(N)) + "\n" for i in range(P))
+    per_frame = "".join("".join(row(j, i) for i in range(P)) + "\n" for j in range(N))
+    assert open(stem + "-Patch-Patch.log").read() == (
+        head + "# Shifts are listed per patch.\n" + size + per_patch)
+    assert open(stem + "-Patch-Frame.log").read() == (
+        head + "# Shifts are listed per frame.\n" + size + per_frame)
