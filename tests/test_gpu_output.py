@@ -223,13 +223,18 @@ def test_stack_arguments_are_checked(ctx, tmp_path):
     assert not os.path.exists(str(tmp_path / "x_Stk.mrc"))
 
 
+@pytest.mark.parametrize("patch", [False, True])
 @pytest.mark.parametrize("pipeline", [1, 0])
-def test_stack_of_batched_movies_matches_single_calls(ctx, pipeline):
-    """mc_align_movies fills every movie's stack as separate calls do."""
-    W, H, N = 512, 384, 6
-    movies = [ctx.upload(MovieRecipe(W, H, N, 150, seed=105 + k, dose=4.0).movie()) for k in range(3)]
-    pg = L.default_params(out_stack=(1, 2), pix_size=1.0, kv=300, fm_dose=1.0,
-                          movie_pipeline=pipeline)
+def test_stack_of_batched_movies_matches_single_calls(ctx, pipeline, patch):
+    """mc_align_movies fills every movie's stack as separate calls do, with
+    the frame weights on (in patch mode the pipelined calls weigh on the
+    second stream beside the patch alignment)."""
+    W, H, N = (1024, 1024, 6) if patch else (512, 384, 6)
+    movies = [ctx.upload(MovieRecipe(W, H, N, 150, seed=105 + k, dose=4.0,
+                                     local_max=2.0 if patch else 0.0).movie()) for k in range(3)]
+    kw = dict(patch=(3, 3), corr_interp=1) if patch else {}
+    pg = L.default_params(out_stack=(1, 2), pix_size=1.0, kv=300, fm_dose=1.0, in_fm_motion=1,
+                          movie_pipeline=pipeline, **kw)
     batch = ctx.align_movies(movies, pg)
     for k, mv in enumerate(movies):
         one = ctx.align_movie(mv, pg)
