@@ -30,22 +30,24 @@ W, H, N = 5760, 4092, 50
 PEAK_HBM_GBS = 8000.0
 
 
-def algorithmic_bytes(kernel, g, nch):
+def algorithmic_bytes(kernel, g, launches):
     """Compulsory HBM bytes of one launch of `kernel` in this workload:
-    each stage reads its inputs once and writes its outputs once."""
+    each stage reads its inputs once and writes its outputs once; `launches`
+    is the kernel class's launch count per movie (the forward classes run in
+    the forward and local-correction passes, the inverse ones in the second
+    only)."""
     cx, xcx = W // 2 + 1, g["xcf"][0] // 2 + 1
     S_f = 8.0 * cx * H                         # full half spectrum
     R_f = 4.0 * W * H                          # full real frame
     S_x = 8.0 * xcx * g["xcf"][1]              # xcf half spectrum
-    launches = 2 * ((N + nch - 1) // nch)      # forward + local-correction passes
     if kernel == "fft_cols_fwd":               # T -> S (+ xcf crop in the first pass)
         return (N * 2 * S_f + N * S_x + N * 2 * S_f) / launches
     if kernel == "fft_rows_r2c":               # real -> T
         return (2 * N * (R_f + S_f)) / launches
     if kernel == "fft_cols_inv":               # S -> T
-        return N * 2 * S_f / (launches // 2)
+        return N * 2 * S_f / launches
     if kernel == "fft_rows_c2r":               # T -> real
-        return N * (S_f + R_f) / (launches // 2)
+        return N * (S_f + R_f) / launches
     if kernel == "local_warp":                 # all frames, read + write
         return N * 2 * R_f
     if kernel == "dw_sums":                    # N spectra in, 3 sums out
@@ -259,7 +261,8 @@ def main():
 
     ms = (t1 - t0) * 1000.0 / (args.steps * K)
     g = L.geometry(W, H, 5, 5)
-    per_launch = algorithmic_bytes(args.kernel, g, 2)
+    # frame chunks per pass from the launches counted in the timed calls
+    per_launch = algorithmic_bytes(args.kernel, g, max(1, round(klaunch / (args.steps * K))))
     avg_ms = kms / max(klaunch, 1)
     achieved = per_launch / (avg_ms * 1e-3) / 1e9 if klaunch else None
 

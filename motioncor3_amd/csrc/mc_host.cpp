@@ -93,7 +93,7 @@ struct mc_ctx {
 };
 
 #ifndef MC_CHUNK_FRAMES
-#define MC_CHUNK_FRAMES 2
+#define MC_CHUNK_FRAMES 4
 #endif
 
 namespace {
