@@ -1051,7 +1051,13 @@ int align_batch(mc_ctx* c, const mc_params* const* pp_, const mc_movie* const* m
             MC_TRY(hipMemcpyAsync(gsh_e, c->pin_shifts, sizeof(double) * 2 * N, hipMemcpyHostToDevice,
                                   c->s2));
             MC_TRY(mc_launch_phasors(gsh_e, -1.0, 1, nullptr, N, W, g.cx, H, N, fA_e, fB_e, c->s2));
-            // Correct/CGenRealStack.cpp:132-166: -CorrInterp, -InFmMotion
+            // Correct/CGenRealStack.cpp:132-166: -CorrInterp, -InFmMotion.
+            // The weights rewrite the frame spectra in place.  When the xcf
+            // stack is the frame stack itself (xcf bin 1), the patch
+            // preparation queued on the main stream still reads it
+            // unweighted (Align/CFullAlign.cpp:157-158), so the second
+            // stream waits for everything queued there so far.
+            if (p->in_fm_motion || p->corr_interp) MC_TRY(fork());
             MC_TRY(mc_launch_frame_weights(Sf_all, fs, N, g.cx, H, gsh_e, p->in_fm_motion, p->corr_interp,
                                            c->s2));
             const long long rfs = (long long)W * H;

@@ -344,7 +344,9 @@ bool tiff_open(const char* path, TiffFile& t) {
                 case 339: d.fmt = (int)v[0]; break;
             }
         }
-        if (d.rps <= 0) d.rps = d.h;
+        // RowsPerStrip defaults to 2^32 - 1 (one strip); a strip never
+        // holds more rows than the image has
+        if (d.rps <= 0 || d.rps > d.h) d.rps = d.h;
         // every block inside the file; image and tile sizes bounded
         for (size_t i = 0; i < d.off.size() && i < d.cnt.size(); i++)
             if (d.off[i] > t.size || d.cnt[i] > t.size - d.off[i]) {

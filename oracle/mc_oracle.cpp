@@ -728,27 +728,31 @@ void norm_by_var(std::vector<double>& img, int w, int h, int wx, int wy) {
         }
 }
 
-/* Align/CDetectFeatures.cpp:58-186 (statistics and thresholds). */
-void feature_map_from_sum(const cd* sumx, int xcx, int xcy, unsigned char* feat) {
-    const int B = 128, bcx = B / 2 + 1;
-    std::vector<cd> bin((size_t)bcx * B, cd(0, 0));
-    fourier_resize(sumx, xcx, xcy, bin.data(), bcx, B, false);
-    std::vector<double> img((size_t)B * B);
-    irfft2(bin.data(), B, B, img.data());
+/* Align/CDetectFeatures.cpp:58-100: the B x B real image made positive
+ * (+ 1 - min, Util/GPositivity2D AddVal) and divided by its local 5 x 5
+ * variance (Align/GNormByStd2D.cu). */
+void feature_prepare(std::vector<double>& img, int B) {
     double mn = 1e300;
     for (double v : img) mn = std::min(mn, v);
     for (double& v : img) v += 1.0 - mn;
     norm_by_var(img, B, B, 5, 5);
+}
+
+/* Align/CDetectFeatures.cpp:134-186: mean and sigma of the map, the same
+ * over the values inside mean +- 4 sigma, then a pixel is a feature when
+ * mu - 4 sigma <= v <= mu - 0.08 sigma of the clipped statistics. */
+void feature_threshold(const double* img, int n, unsigned char* feat, double* thr) {
     double mean = 0, m2 = 0;
-    for (double v : img) { mean += v; m2 += v * v; }
-    mean /= (B * B);
-    m2 /= (B * B);
+    for (int i = 0; i < n; i++) { mean += img[i]; m2 += img[i] * img[i]; }
+    mean /= n;
+    m2 /= n;
     double sd = m2 - mean * mean;
     sd = (sd < 0) ? 0.0 : std::sqrt(sd);
     double lo = mean - 4.0 * sd, hi = mean + 4.0 * sd;
     double dm = 0, ds = 0;
     int cnt = 0;
-    for (double v : img) {
+    for (int i = 0; i < n; i++) {
+        const double v = img[i];
         if (v < lo || v > hi) continue;
         dm += v;
         ds += v * v;
@@ -759,7 +763,20 @@ void feature_map_from_sum(const cd* sumx, int xcx, int xcy, unsigned char* feat)
     ds = (ds < 0) ? 0.0 : std::sqrt(ds);
     lo = dm - 4.0 * ds;
     hi = dm - 0.08 * ds;
-    for (int i = 0; i < B * B; i++) feat[i] = (img[i] < lo || img[i] > hi) ? 0 : 1;
+    if (thr) { thr[0] = lo; thr[1] = hi; }
+    for (int i = 0; i < n; i++) feat[i] = (img[i] < lo || img[i] > hi) ? 0 : 1;
+}
+
+/* Align/CDetectFeatures.cpp:58-186: the aligned xcf sum Fourier-cropped to
+ * 128 x 128, C2R, made positive, normalised, thresholded. */
+void feature_map_from_sum(const cd* sumx, int xcx, int xcy, unsigned char* feat) {
+    const int B = 128, bcx = B / 2 + 1;
+    std::vector<cd> bin((size_t)bcx * B, cd(0, 0));
+    fourier_resize(sumx, xcx, xcy, bin.data(), bcx, B, false);
+    std::vector<double> img((size_t)B * B);
+    irfft2(bin.data(), B, B, img.data());
+    feature_prepare(img, B);
+    feature_threshold(img.data(), B * B, feat, nullptr);
 }
 
 /* Align/CDetectFeatures.cpp:289-396 and Align/CPatchCenters.cpp:42-105. */
@@ -1087,6 +1104,33 @@ void orc_feature_map(const double* sum_cmp, int xw, int xh, unsigned char* feat)
 
 void orc_patch_centers(const unsigned char* feat, int px, int py, int xw, int xh, int* starts) {
     patch_starts_from_features(feat, px, py, xw, xh, starts);
+}
+
+void orc_norm_by_var(double* img, int w, int h) {
+    std::vector<double> v(img, img + (size_t)w * h);
+    norm_by_var(v, w, h, 5, 5);
+    std::copy(v.begin(), v.end(), img);
+}
+
+void orc_feature_threshold(const double* img, int n, unsigned char* feat, double* thr) {
+    feature_threshold(img, n, feat, thr);
+}
+
+void orc_interpolate(const double* gx, const double* gy, int binz, int n, const int* fmsize,
+                     int nearest, double* ox, double* oy, int* ngroups) {
+    std::vector<double> fmcent(n);
+    int st = 0;
+    for (int i = 0; i < n; i++) {
+        const int sz = fmsize ? fmsize[i] : 1;
+        fmcent[i] = st + 0.5 * (sz - 1.0);
+        st += sz;
+    }
+    const Groups g = make_groups(binz, n, fmcent, fmsize);
+    std::vector<double> vx(gx, gx + g.ng), vy(gy, gy + g.ng), rx, ry;
+    interpolate(vx, vy, g, fmcent, nearest != 0, rx, ry);
+    std::copy(rx.begin(), rx.end(), ox);
+    std::copy(ry.begin(), ry.end(), oy);
+    if (ngroups) *ngroups = g.ng;
 }
 
 /* Per-frequency weights applied to a corrected frame spectrum.

@@ -115,6 +115,16 @@ void orc_patch_centers(const unsigned char* features, int px, int py,
                        int xcfw, int xcfh, int* starts);
 
 /* Whole path: frames are N x H x W float32 counts. Returns 0 on success. */
+/* GNormByStd2D out of place (5 x 5 window, in-bounds neighbours only). */
+void orc_norm_by_var(double* img, int w, int h);
+/* CDetectFeatures thresholds on a normalised map of n values; thr gets
+ * {mu - 4 sigma, mu - 0.08 sigma} of the clipped statistics (may be NULL). */
+void orc_feature_threshold(const double* img, int n, unsigned char* feat, double* thr);
+/* CInterpolateShift: group values (binz-frame groups of n frames with raw
+ * sizes fmsize or NULL) to frames, nearest or linear with extrapolation. */
+void orc_interpolate(const double* gx, const double* gy, int binz, int n, const int* fmsize,
+                     int nearest, double* ox, double* oy, int* ngroups);
+
 int orc_align_movie(const float* frames, int W, int H, int N,
                     const orc_params* prm, const orc_override* ov,
                     orc_out* out);

@@ -252,6 +252,39 @@ def patch_starts(features, px, py, xw, xh):
     return st
 
 
+def norm_by_var(img):
+    """GNormByStd2D out of place: each pixel over the variance of its 5 x 5
+    in-bounds neighbourhood (Align/GNormByStd2D.cu:7-42)."""
+    a = np.ascontiguousarray(img, dtype=np.float64).copy()
+    lib().orc_norm_by_var(_d(a), a.shape[1], a.shape[0])
+    return a
+
+
+def feature_threshold(img):
+    """CDetectFeatures thresholds (Align/CDetectFeatures.cpp:134-159) on a
+    normalised map: (feature mask, (lo, hi))."""
+    a = np.ascontiguousarray(img, dtype=np.float64)
+    f = np.zeros(a.shape, np.uint8)
+    thr = np.zeros(2)
+    lib().orc_feature_threshold(_d(a), a.size, f.ctypes.data_as(_up), _d(thr))
+    return f, (thr[0], thr[1])
+
+
+def interpolate(gx, gy, binz, n, fmsize=None, nearest=False):
+    """CInterpolateShift (Align/CInterpolateShift.cpp:41-142): group values to
+    n frames grouped by binz (raw frame sizes fmsize)."""
+    gx = np.ascontiguousarray(gx, dtype=np.float64)
+    gy = np.ascontiguousarray(gy, dtype=np.float64)
+    ox, oy = np.zeros(n), np.zeros(n)
+    ng = C.c_int(0)
+    fs = None
+    if fmsize is not None:
+        fmsize = np.ascontiguousarray(fmsize, np.int32)
+        fs = fmsize.ctypes.data_as(_ip)
+    lib().orc_interpolate(_d(gx), _d(gy), binz, n, fs, int(nearest), _d(ox), _d(oy), C.byref(ng))
+    return ox, oy, ng.value
+
+
 def align_movie(frames, params=None, xcf_shift=None, patch_starts=None,
                 replay_global=None, replay_patch=None, replay_centers=None,
                 replay_bad=None):

@@ -183,3 +183,143 @@ def test_sums_are_linear_and_clamped():
     assert np.allclose(r2["sums"], 2 * r1["sums"], rtol=1e-9, atol=1e-9)
     # dose weighting leaves the DC (the frame mean) alone
     assert abs(r1["sums"][1].mean() - r1["sums"][0].mean()) < 1e-9 * abs(r1["sums"][0].mean())
+
+
+# ---- GWeightFrame (Correct/GWeightFrame.cu:19-30, 101-109) ----
+# Literal weights exp(-0.5 D / Dc(k)) with Dc = 0.24499 k^-1.6649 + 2.8141 at
+# 300 kV (the Grant & Grigorieff critical-exposure fit), k in 1/A.  The
+# oracle's normaliser with one frame is that frame's weight itself.
+@pytest.mark.parametrize("kx,y,pix,dose,want", [
+    (16, 0, 1.0, 10.0, 0.38773409603823816),    # k = 0.25
+    (32, 0, 1.0, 10.0, 0.24848023943515027),    # k = 0.5, x Nyquist: fx = kx / (2 (cx - 1))
+    (0, 32, 1.0, 10.0, 0.24848023943515027),    # y Nyquist: fy = 0.5 -> -0.5
+    (16, 40, 1.0, 10.0, 0.2624264636553408),    # fy = 40/64 - 1 = -0.375
+    (16, 24, 1.0, 37.5, 0.006626414602836957),  # |fy| = 0.375, larger dose
+    (16, 0, 0.83, 10.0, 0.33886409539204293),   # k = 0.25 / 0.83
+    (3, 59, 1.0, 37.5, 0.3106594568913417),     # fx = 3/64, fy = -5/64
+])
+def test_dose_weight_values(kx, y, pix, dose, want):
+    w = O.dose_weight_sum(33, 64, pix, 300, [dose])
+    assert w[y, kx] == pytest.approx(want, rel=1e-12)
+    assert w[0, 0] == 0.0   # DC is never weighted
+
+
+@pytest.mark.parametrize("kv,factor", [(300, 1.0), (400, 1.0), (250, 0.9), (200, 0.8),
+                                       (160, 0.625), (120, 0.45), (80, 0.45)])
+def test_dose_weight_kv_factor(kv, factor):
+    """Dc scales by the piecewise-linear kV factor (1 at >= 300 kV, 0.8 at
+    200, 0.45 at <= 120), so -ln w(kV) = -ln w(300) / factor everywhere."""
+    w300 = O.dose_weight_sum(33, 64, 1.0, 300, [12.0]).ravel()[1:]
+    wkv = O.dose_weight_sum(33, 64, 1.0, kv, [12.0]).ravel()[1:]
+    assert np.allclose(np.log(wkv) * factor, np.log(w300), rtol=1e-12, atol=0)
+
+
+def test_dose_weight_normaliser_over_frames():
+    """sqrt(sum_i w_i^2) over accumulated doses FmDose * (i + 1), against the
+    literal single-frame weights at k = 0.25."""
+    ws = O.dose_weight_sum(33, 64, 1.0, 300, [10.0, 37.5])
+    assert ws[0, 16] == pytest.approx(np.hypot(0.38773409603823816, 0.028641942865727247),
+                                      rel=1e-12)
+
+
+# ---- CInterpolateShift (Align/CInterpolateShift.cpp:41-142) ----
+def test_interpolate_linear_data_exact_with_uneven_groups():
+    """50 frames in groups of 4 form 12 x 4 + 1 x 2 groups (centres 1.5,
+    5.5, ..., 45.5 and 48.5).  Linear data at the group centres comes back
+    exactly at every frame centre, including the extrapolated frames 0, 1
+    (before the first centre) and 49 (after the last)."""
+    n = 50
+    cent = np.array([4 * g + 1.5 for g in range(12)] + [48.5])
+    ox, oy, ng = O.interpolate(0.3 * cent - 2.0, -1.7 * cent + 5.0, 4, n)
+    assert ng == 13
+    t = np.arange(n, dtype=float)
+    assert np.allclose(ox, 0.3 * t - 2.0, atol=1e-12)
+    assert np.allclose(oy, -1.7 * t + 5.0, atol=1e-12)
+
+
+def test_interpolate_piecewise_brackets_hand_values():
+    """Groups of 4 over 10 frames: centres 1.5, 5.5, 8.5 (the last group has
+    two frames).  With x = (0, 4, 1) the frames take the line through their
+    bracketing centres, the end lines extended outwards:
+    frame 0: 0 + 1.0 * (0 - 1.5) = -1.5;  frame 3: 1.5;  frame 5: 3.5;
+    frame 6: 4 - 1.0 * 0.5 = 3.5;  frame 8: 4 - 2.5 = 1.5;
+    frame 9: 4 - 3.5 = 0.5."""
+    ox, oy, ng = O.interpolate([0.0, 4.0, 1.0], [2.0, 2.0, 2.0], 4, 10)
+    assert ng == 3
+    want = [-1.5, -0.5, 0.5, 1.5, 2.5, 3.5, 3.5, 2.5, 1.5, 0.5]
+    assert np.allclose(ox, want, atol=1e-12)
+    assert np.allclose(oy, 2.0, atol=1e-12)
+
+
+def test_interpolate_nearest_and_one_to_one():
+    ox, oy, ng = O.interpolate([1.0, 2.0, 3.0], [7.0, 8.0, 9.0], 4, 10, nearest=True)
+    assert np.array_equal(ox, [1, 1, 1, 1, 2, 2, 2, 2, 3, 3])
+    assert np.array_equal(oy, [7, 7, 7, 7, 8, 8, 8, 8, 9, 9])
+    g = np.arange(6.0) ** 2
+    ox, oy, ng = O.interpolate(g, -g, 1, 6)
+    assert ng == 6 and np.array_equal(ox, g) and np.array_equal(oy, -g)
+
+
+def test_interpolate_raw_frame_sizes():
+    """Integrated frames of 1, 1, 2, 2, 1, 3 raw frames (centres 0, 1, 2.5,
+    4.5, 6, 8).  With -Group 2 a group holds frames until it has 2 x 1 raw
+    frames (DataUtil/CFmGroupParam.cpp:59-78): {0, 1}, {2}, {3}, {4, 5},
+    centres 0.5, 2.5, 4.5, 7.  x = centre (linear data) comes back as each
+    frame's centre."""
+    sizes = [1, 1, 2, 2, 1, 3]
+    gc = np.array([0.5, 2.5, 4.5, 7.0])
+    ox, oy, ng = O.interpolate(gc, 2 * gc, 2, 6, fmsize=sizes)
+    assert ng == 4
+    fc = np.array([0.0, 1.0, 2.5, 4.5, 6.0, 8.0])
+    assert np.allclose(ox, fc, atol=1e-12)
+    assert np.allclose(oy, 2 * fc, atol=1e-12)
+
+
+# ---- GNormByStd2D + CDetectFeatures thresholds ----
+def test_norm_by_var_brute_force():
+    """Each pixel over the population variance of its in-bounds 5 x 5
+    neighbourhood (fewer pixels at the borders)."""
+    rng = np.random.default_rng(4)
+    img = rng.uniform(1.0, 3.0, (9, 11))
+    got = O.norm_by_var(img)
+    for y in range(9):
+        for x in range(11):
+            nb = img[max(0, y - 2):y + 3, max(0, x - 2):x + 3]
+            assert got[y, x] == pytest.approx(img[y, x] / (nb.var() + 1e-20), rel=1e-12)
+
+
+def test_feature_thresholds_on_a_built_map():
+    """A 128 x 128 map: 16000 background values, 6 outliers far below and 6
+    far above (outside mu +- 4 sigma, dropped from the clipped statistics),
+    and probes placed at fractions of the clipped sigma around both
+    thresholds.  A feature is mu' - 4 sigma' <= v <= mu' - 0.08 sigma'; the
+    clipped mean and sigma are computed here by brute force.  A sign slip
+    in either threshold, or statistics taken over the outliers, moves a
+    probe across."""
+    rng = np.random.default_rng(12)
+    n = 128 * 128
+    base = rng.normal(10.0, 1.0, n - 12 - 8)
+    base = np.clip(base, 7.0, 13.0)
+    outl = np.r_[np.full(6, -200.0), np.full(6, 260.0)]
+    # the probes sit at known offsets from the clipped statistics, which do
+    # include them; solve for that fixed point by iteration
+    offs = np.array([-4.05, -3.95, -0.10, -0.06, 0.06, 0.10, -2.0, 1.0])
+    probes = 10.0 + offs
+    for _ in range(50):
+        allv = np.r_[base, probes]
+        mu, sd = allv.mean(), allv.std()
+        probes = mu + offs * sd
+    img = np.r_[base, outl, probes]
+    f, (lo, hi) = O.feature_threshold(img)
+    # brute force: first-pass statistics over everything, then the clipped set
+    m1, s1 = img.mean(), img.std()
+    keep = img[(img >= m1 - 4 * s1) & (img <= m1 + 4 * s1)]
+    assert keep.size == n - 12
+    mu, sd = keep.mean(), keep.std()
+    assert lo == pytest.approx(mu - 4 * sd, rel=1e-12)
+    assert hi == pytest.approx(mu - 0.08 * sd, rel=1e-12)
+    want = ((img >= mu - 4 * sd) & (img <= mu - 0.08 * sd)).astype(np.uint8)
+    assert np.array_equal(f, want)
+    pf = f[-8:]
+    assert list(pf) == [0, 1, 1, 0, 0, 0, 1, 0]   # -4.05 out, -3.95 in, -0.10 in, -0.06 out
+    assert f[n - 20:n - 8].sum() == 0             # outliers never features

@@ -219,3 +219,17 @@ def test_eer_container(tmp_path, bits):
     Image.fromarray(np.zeros((4, 4), np.uint8)).save(p2)
     with pytest.raises(RuntimeError):
         L.eer_info(p2)
+
+
+def test_rows_per_strip_default_is_one_strip(tmp_path):
+    """RowsPerStrip = 2^32 - 1 (the TIFF default: one strip for the whole
+    image) reads the image's own rows, not 2^30 rows of buffer."""
+    rng = np.random.default_rng(8)
+    h, w = 23, 41
+    fr = [rng.integers(0, 256, (h, w)).astype(np.uint8) for _ in range(2)]
+    path = str(tmp_path / "one.tif")
+    tiffw.write(path, [dict(w=w, h=h, bits=8, comp=1, blocks=[x.tobytes()], rps=0xFFFFFFFF)
+                       for x in fr])
+    got = L.tiff_read(path).view(np.uint8).reshape(2, h, w)
+    for k in range(2):
+        assert np.array_equal(got[k], np.flipud(fr[k])), k
