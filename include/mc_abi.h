@@ -253,6 +253,34 @@ int mc_local_warp(mc_ctx* ctx, const float* in, float* out, int W, int H, int N,
 int mc_dose_weight_sum(mc_ctx* ctx, int W, int H, int N, float pix, int kv, float fm_dose,
                        float* out_table);
 
+/* GFourierResize2D (Util/GFourierResize2D.cu:7-39, 110-125): `batch` half
+ * spectra [kx][ky] of in_w x in_h images cropped or padded to out_w x out_h,
+ * rows of negative frequencies kept at the end.  Output elements with no
+ * source are left as they are (zero the output first when padding);
+ * sum != 0 adds into the output (bSum). */
+int mc_fourier_crop(mc_ctx* ctx, const float* in_spec, int in_w, int in_h, float* out_spec,
+                    int out_w, int out_h, int batch, int sum);
+/* CDetectFeatures::DoIt + CPatchCenters::Calculate (Align/CDetectFeatures.cpp:
+ * 58-396, Align/CPatchCenters.cpp:42-105): from the aligned xcf sum (device
+ * half spectrum [kx][ky] of an xw x xh image) the 128 x 128 feature map
+ * (host, row-major, may be NULL) and the px x py patch starts in xcf pixels
+ * (host P x 2, patch-size rule of mc_geometry). */
+int mc_feature_centres(mc_ctx* ctx, const float* xcf_sum_spec, int xw, int xh, int px, int py,
+                       unsigned char* feature_map_host, int* starts_host);
+/* CExtractPatch + the patch R2C of CMeasurePatches (Align/CExtractPatch.cpp:
+ * 27-106, Align/CMeasurePatches.cpp:54-55): N real xcf frames (device, rows of
+ * xw floats, frames xw * xh apart) -> P * N unnormalised half spectra [kx][ky]
+ * of the pw x ph windows at starts_host (P x 2 xcf pixels); transform
+ * q * N + i is patch q of frame i. */
+int mc_extract_patches(mc_ctx* ctx, const float* xcf_frames, int xw, int xh, int N,
+                       const int* starts_host, int P, int pw, int ph, float* out_spec);
+/* CCorrectFullShift::mUnpadSums (Correct/CCorrectFullShift.cpp:146-185):
+ * `count` sum spectra [kx][ky] of W x H frames (device, W/2+1 by H each)
+ * Fourier-cropped to the -FtBin size (out_w x out_h of mc_output_geometry),
+ * C2R and clamped at zero into out (device, count images of out_w x out_h). */
+int mc_finalize_sums(mc_ctx* ctx, const float* sum_specs, int W, int H, int count, float ft_bin,
+                     float* out);
+
 /* ---- instrumentation (used by the benchmark) ---- */
 /* Record HIP events around every launch of the named kernel class
  * ("fft_cols_fwd", "fft_rows_r2c", ...); NULL disables. */

@@ -6,6 +6,7 @@
 // global alignment, or all patches at once) keeps its trajectory state in
 // device memory in FP64; one iteration is six launches over all jobs and
 // groups, and a converged job's work items exit on entry.
+#include <algorithm>
 #include <cmath>
 
 #include "mc_align.h"
@@ -41,6 +42,45 @@ __global__ void k_phasors(const double* shifts, double mult, int skip, const uns
         sincos(ys * fsy, &sn, &cs);
         B[(long long)b * H + y] = cmk((float)cs, (float)sn);
     }
+}
+
+// GFourierResize2D (Util/GFourierResize2D.cu:7-39) on [kx][y] half spectra:
+// output row yo reads input row yo (yo <= outy/2) or yo - outy + iny (the
+// negative frequencies), when that row exists in the input; columns beyond
+// the input's are not written, nor rows without a source.  Consecutive lanes
+// take consecutive rows of one column, so both sides are contiguous.
+__global__ __launch_bounds__(256) void k_fourier_resize(const cfloat* __restrict__ in, int inx, int iny,
+                                                        long long in_stride, cfloat* __restrict__ out,
+                                                        int outx, int outy, long long out_stride,
+                                                        int sum) {
+    const int yo = blockIdx.x * blockDim.x + threadIdx.x;
+    const int kx = blockIdx.y, b = blockIdx.z;
+    if (yo >= outy || kx >= inx) return;
+    int y = yo;
+    if (y > outy / 2) {
+        y -= outy;
+        if (y <= -iny / 2) return;
+        y += iny;
+    } else if (y > iny / 2) {
+        return;
+    }
+    const cfloat v = in[(long long)b * in_stride + (long long)kx * iny + y];
+    cfloat* q = out + (long long)b * out_stride + (long long)kx * outy + yo;
+    if (sum) {
+        const cfloat o = *q;
+        *q = cmk(o.x + v.x, o.y + v.y);
+    } else {
+        *q = v;
+    }
+}
+
+hipError_t mc_launch_fourier_resize(const cfloat* in, int inx, int iny, long long in_stride, cfloat* out,
+                                    int outx, int outy, long long out_stride, int batch, int sum,
+                                    hipStream_t s) {
+    dim3 grid((outy + 255) / 256, std::min(inx, outx), batch);
+    hipLaunchKernelGGL(k_fourier_resize, grid, dim3(256), 0, s, in, inx, iny, in_stride, out, outx, outy,
+                       out_stride, sum);
+    return hipGetLastError();
 }
 
 // Per-frequency weights of a corrected frame spectrum [kx][y], in FP32 as the

@@ -1867,6 +1867,84 @@ int mc_dose_weight_sum(mc_ctx* c, int W, int H, int N, float pix, int kv, float 
     return MC_OK;
 }
 
+int mc_fourier_crop(mc_ctx* c, const float* in_spec, int in_w, int in_h, float* out_spec, int out_w,
+                    int out_h, int batch, int sum) {
+    if (!c || !in_spec || !out_spec || batch < 1 || in_w < 2 || in_h < 1 || out_w < 2 || out_h < 1 ||
+        (in_w % 2) || (out_w % 2))
+        return MC_E_ARG;
+    const int inx = in_w / 2 + 1, outx = out_w / 2 + 1;
+    MC_TRY(mc_launch_fourier_resize((const cfloat*)in_spec, inx, in_h, (long long)inx * in_h,
+                                    (cfloat*)out_spec, outx, out_h, (long long)outx * out_h, batch,
+                                    sum != 0, c->stream));
+    return MC_OK;
+}
+
+int mc_feature_centres(mc_ctx* c, const float* xcf_sum_spec, int xw, int xh, int px, int py,
+                       unsigned char* feature_map_host, int* starts_host) {
+    if (!c || !xcf_sum_spec || !starts_host || xw < 128 || xh < 128 || (xw % 2) || px < 2 ||
+        py < 2)
+        return MC_E_ARG;
+    Fft2D* p128 = plan(c, 128, 128);
+    if (!p128) return fail(c, MC_E_ARG, "128x128 plan");
+    MC_BUF(T128, cfloat, "api_T128", (size_t)p128->t_elems);
+    MC_BUF(img, float, "api_img128", 128 * 128);
+    MC_BUF(feat, unsigned char, "api_feat", 128 * 128);
+    const int xcx = xw / 2 + 1;
+    // Align/CDetectFeatures.cpp:58-100: the crop to 128 x 128 and its C2R
+    MC_TRY(mc_cols_inv(*p128, (const cfloat*)xcf_sum_spec, (long long)xcx * xh, xcx, xh, nullptr, T128,
+                       p128->t_elems, 1, c->stream));
+    MC_TRY(mc_rows_c2r(*p128, T128, p128->t_elems, img, 128, 128 * 128, 1, 0, c->stream));
+    MC_TRY(mc_launch_feature_map(img, feat, c->stream));
+    std::vector<unsigned char> hf(128 * 128);
+    MC_TRY(hipMemcpyAsync(hf.data(), feat, hf.size(), hipMemcpyDeviceToHost, c->stream));
+    MC_TRY(hipStreamSynchronize(c->stream));
+    if (feature_map_host) std::copy(hf.begin(), hf.end(), feature_map_host);
+    patch_starts(hf.data(), px, py, xw, xh, starts_host);
+    return MC_OK;
+}
+
+int mc_extract_patches(mc_ctx* c, const float* xcf_frames, int xw, int xh, int N, const int* starts_host,
+                       int P, int pw, int ph, float* out_spec) {
+    if (!c || !xcf_frames || !starts_host || !out_spec || N < 1 || P < 1 || pw < 4 || ph < 4 ||
+        (pw % 2) || pw > xw || ph > xh)
+        return MC_E_ARG;
+    for (int q = 0; q < P; q++) {
+        const int x0 = starts_host[2 * q], y0 = starts_host[2 * q + 1];
+        if (x0 < 0 || y0 < 0 || x0 + pw > xw || y0 + ph > xh)
+            return fail(c, MC_E_ARG, "patch window outside the xcf frame");
+    }
+    Fft2D* pp = plan(c, pw, ph);
+    if (!pp) return fail(c, MC_E_ARG, "unsupported patch FFT size");
+    MC_BUF(Tp, cfloat, "api_Tp", (size_t)P * N * pp->t_elems);
+    MC_BUF(dst, int, "api_starts", 2 * P);
+    MC_TRY(hipMemcpyAsync(dst, starts_host, sizeof(int) * 2 * P, hipMemcpyHostToDevice, c->stream));
+    // Align/CExtractPatch.cpp:27-106 fused into the row pass (transform
+    // q * N + i reads frame i through window q), then the unnormalised R2C
+    // of Align/CMeasurePatches.cpp:54-55
+    MC_TRY(mc_rows_r2c_mod(*pp, xcf_frames, xw, (long long)xw * xh, N, dst, Tp, pp->t_elems, P * N, 1.0f,
+                           c->stream));
+    MC_TRY(mc_cols_fwd(*pp, Tp, pp->t_elems, (cfloat*)out_spec, pp->s_elems, nullptr, 0, 0, 0, P * N,
+                       c->stream));
+    return MC_OK;
+}
+
+int mc_finalize_sums(mc_ctx* c, const float* sum_specs, int W, int H, int count, float ft_bin, float* out) {
+    if (!c || !sum_specs || !out || count < 1 || W < 8 || H < 8 || (W % 2) || !(ft_bin >= 1.0f))
+        return MC_E_ARG;
+    const Geo g = make_geo(W, H, 1, 0, 0, ft_bin);
+    Fft2D* po = plan(c, g.ow, g.oh);
+    if (!po) return fail(c, MC_E_ARG, "unsupported output FFT size");
+    MC_BUF(To, cfloat, "api_To", (size_t)po->t_elems);
+    const long long fs = (long long)g.cx * H;
+    for (int i = 0; i < count; i++) {
+        MC_TRY(mc_cols_inv(*po, (const cfloat*)sum_specs + i * fs, fs, g.cx, H, nullptr, To, po->t_elems, 1,
+                           c->stream));
+        MC_TRY(mc_rows_c2r(*po, To, po->t_elems, out + (long long)i * g.ow * g.oh, g.ow,
+                           (long long)g.ow * g.oh, 1, 1, c->stream));
+    }
+    return MC_OK;
+}
+
 int mc_set_kernel_timing(mc_ctx* c, const char* kernel) {
     if (!c) return MC_E_ARG;
     c->timed = kernel ? kernel : "";

@@ -83,6 +83,7 @@ ABI_SYMBOLS = (
     "mc_align_movies", "mc_rfft2", "mc_irfft2",
     "mc_phase_shift_accumulate", "mc_group_correlate_window", "mc_peak_fit",
     "mc_spline_despike_fp64", "mc_patch_stats_fp64", "mc_local_warp", "mc_dose_weight_sum",
+    "mc_fourier_crop", "mc_feature_centres", "mc_extract_patches", "mc_finalize_sums",
     "mc_set_kernel_timing", "mc_get_kernel_timing", "mc_synth_render",
     "mc_prior_create", "mc_prior_destroy", "mc_prior_dims", "mc_prior_params",
     "mc_prior_forward", "mc_prior_loss_grad", "mc_prior_train", "mc_set_prior", "mc_gemm_bf16",
@@ -318,6 +319,52 @@ class Context:
         self._check(self.lib.mc_dose_weight_sum(self.ptr, W, H, N, C.c_float(pix), kv,
                                                 C.c_float(fm_dose), out.vp()),
                     "mc_dose_weight_sum")
+        self.sync()
+        return out
+
+    def fourier_crop(self, spec, in_w, out_w, out_h, out=None, sum=False):
+        """spec: complex64 [B, in_w/2+1, in_h] -> [B, out_w/2+1, out_h]
+        (GFourierResize2D); a new output starts at zero."""
+        B, _, in_h = spec.shape
+        if out is None:
+            out = self.upload(np.zeros((B, out_w // 2 + 1, out_h), np.complex64))
+        self._check(self.lib.mc_fourier_crop(self.ptr, spec.vp(), in_w, in_h, out.vp(), out_w, out_h,
+                                             B, int(sum)), "mc_fourier_crop")
+        self.sync()
+        return out
+
+    def feature_centres(self, xcf_sum_spec, xw, px, py):
+        """xcf_sum_spec: complex64 [xw/2+1, xh] -> (128 x 128 feature map,
+        patch starts [P, 2] in xcf pixels)."""
+        xh = xcf_sum_spec.shape[-1]
+        fm = np.zeros((128, 128), np.uint8)
+        st = np.zeros((px * py, 2), np.int32)
+        self._check(self.lib.mc_feature_centres(
+            self.ptr, xcf_sum_spec.vp(), xw, xh, px, py, fm.ctypes.data_as(C.POINTER(C.c_ubyte)),
+            st.ctypes.data_as(C.POINTER(C.c_int))), "mc_feature_centres")
+        return fm, st
+
+    def extract_patches(self, xcf_frames, starts, pw, ph):
+        """xcf_frames: float32 [N, xh, xw]; starts [P, 2] -> complex64
+        [P * N, pw/2+1, ph], transform q * N + i (unnormalised R2C)."""
+        N, xh, xw = xcf_frames.shape
+        st = np.ascontiguousarray(starts, np.int32)
+        P = st.shape[0]
+        out = self.empty((P * N, pw // 2 + 1, ph), np.complex64)
+        self._check(self.lib.mc_extract_patches(self.ptr, xcf_frames.vp(), xw, xh, N,
+                                                st.ctypes.data_as(C.POINTER(C.c_int)), P, pw, ph,
+                                                out.vp()), "mc_extract_patches")
+        self.sync()
+        return out
+
+    def finalize_sums(self, sum_specs, W, ft_bin=1.0):
+        """sum_specs: complex64 [K, W/2+1, H] -> float32 [K, oh, ow] (crop to
+        the -FtBin size, C2R, clamp at 0)."""
+        K, _, H = sum_specs.shape
+        _, ow, oh = self.output_geometry(default_params(ft_bin=ft_bin), W, H, 1)
+        out = self.empty((K, oh, ow), np.float32)
+        self._check(self.lib.mc_finalize_sums(self.ptr, sum_specs.vp(), W, H, K, C.c_float(ft_bin),
+                                              out.vp()), "mc_finalize_sums")
         self.sync()
         return out
 

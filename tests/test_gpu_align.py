@@ -63,13 +63,13 @@ def test_patch_alignment_matches_oracle(ctx):
     gres = ctx.align_movie(ctx.upload(mv), pg)
     own = O.align_movie(mv, po)
     # discrete decisions first: the feature-driven patch placement
-    same_starts = np.array_equal(own["patch_starts"], gres["patch_starts"])
-    ores = own if same_starts else O.align_movie(mv, po, patch_starts=gres["patch_starts"])
+    assert np.array_equal(own["patch_starts"], gres["patch_starts"])
+    ores = own
     assert np.max(np.abs(gres["global_shift"] - ores["global_shift"])) <= 0.05
     assert np.allclose(gres["patch_center"], ores["patch_center"], atol=1e-3)
     d = gres["patch_shift"] - ores["patch_shift"]
     assert np.max(np.abs(d)) <= 0.05
-    assert np.mean(gres["patch_bad"] != ores["patch_bad"]) <= 0.01
+    assert np.array_equal(gres["patch_bad"], ores["patch_bad"])
     orep = O.align_movie(mv, po, replay_global=gres["global_shift"],
                          replay_patch=gres["patch_shift"], replay_centers=gres["patch_center"],
                          replay_bad=gres["patch_bad"])
@@ -228,3 +228,22 @@ def test_super_resolution_fourier_binned(ctx):
         a = s_.numpy()
         assert np.isfinite(a).all() and a.min() >= 0
     assert np.isfinite(res["patch_shift"]).all()
+
+
+def test_frame_weights_do_not_race_the_patch_preparation(ctx):
+    """With an xcf bin of 1 the xcf stack is the frame stack.  -InFmMotion and
+    -CorrInterp weigh the frame spectra in place on the second stream, which
+    must not run before the patch preparation has read them unweighted:
+    two chunk streams give the bits of one."""
+    W, H, N = 1024, 1024, 10
+    r = MovieRecipe(W, H, N, 400, seed=31, dose=4.0, local_max=2.0)
+    mv = ctx.upload(r.movie())
+    kw = dict(patch=(3, 3), iter=10, pix_size=1.0, kv=300, fm_dose=1.0, in_fm_motion=1,
+              corr_interp=1)
+    one = ctx.align_movie(mv, L.default_params(chunk_streams=1, **kw))
+    for rep in range(2):
+        two = ctx.align_movie(mv, L.default_params(chunk_streams=2, **kw))
+        assert np.array_equal(one["patch_shift"], two["patch_shift"]), rep
+        assert np.array_equal(one["global_shift"], two["global_shift"]), rep
+        for a, b in zip(one["sums"], two["sums"]):
+            assert np.array_equal(a.numpy(), b.numpy()), rep
