@@ -24,6 +24,7 @@
 #include <cmath>
 #include <complex>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <mutex>
@@ -92,7 +93,7 @@ void fft_exec(const FftPlan& p, cd* x, cd* work) {
     cd* a = x;
     cd* b = work;
     int ns = 1;
-    std::vector<cd> v(256), y(256);
+    thread_local std::vector<cd> v(256), y(256);
     for (size_t s = 0; s < p.rad.size(); s++) {
         const int r = p.rad[s];
         const int m = n / r;
@@ -104,7 +105,12 @@ void fft_exec(const FftPlan& p, cd* x, cd* work) {
             for (int t = 0; t < r; t++) v[t] = a[j + t * m] * tw[(size_t)k * r + t];
             for (int q = 0; q < r; q++) {
                 cd acc = 0.0;
-                for (int t = 0; t < r; t++) acc += v[t] * ro[(t * q) % r];
+                int e = 0;   /* (t * q) mod r */
+                for (int t = 0; t < r; t++) {
+                    acc += v[t] * ro[e];
+                    e += q;
+                    if (e >= r) e -= r;
+                }
                 y[q] = acc;
             }
             const int idx = (j / ns) * ns * r + k;
@@ -119,7 +125,8 @@ void fft_exec(const FftPlan& p, cd* x, cd* work) {
 void fft1(cd* x, int n, int sign) {
     if (n <= 1) return;
     const FftPlan& p = get_plan(n, sign);
-    std::vector<cd> w(n);
+    thread_local std::vector<cd> w;
+    if ((int)w.size() < n) w.resize(n);
     fft_exec(p, x, w.data());
 }
 
@@ -136,12 +143,18 @@ void rfft2(const double* img, int W, int H, cd* cmp) {
             fft1(row.data(), W, -1);
             for (int k = 0; k < cx; k++) cmp[(size_t)y * cx + k] = row[k];
         }
-        std::vector<cd> col(H);
+        /* columns in blocks of CB adjacent kx, so the strided gathers read
+         * whole cache lines; each column's transform is unchanged */
+        const int CB = 8;
+        std::vector<cd> col((size_t)CB * H);
 #pragma omp for schedule(static)
-        for (int k = 0; k < cx; k++) {
-            for (int y = 0; y < H; y++) col[y] = cmp[(size_t)y * cx + k];
-            fft1(col.data(), H, -1);
-            for (int y = 0; y < H; y++) cmp[(size_t)y * cx + k] = col[y];
+        for (int k0 = 0; k0 < cx; k0 += CB) {
+            const int nb = std::min(CB, cx - k0);
+            for (int y = 0; y < H; y++)
+                for (int j = 0; j < nb; j++) col[(size_t)j * H + y] = cmp[(size_t)y * cx + k0 + j];
+            for (int j = 0; j < nb; j++) fft1(&col[(size_t)j * H], H, -1);
+            for (int y = 0; y < H; y++)
+                for (int j = 0; j < nb; j++) cmp[(size_t)y * cx + k0 + j] = col[(size_t)j * H + y];
         }
     }
 }
@@ -154,12 +167,16 @@ void irfft2(const cd* cmp_in, int W, int H, double* img) {
     std::vector<cd> cmp(cmp_in, cmp_in + (size_t)cx * H);
 #pragma omp parallel
     {
-        std::vector<cd> col(H);
+        const int CB = 8;
+        std::vector<cd> col((size_t)CB * H);
 #pragma omp for schedule(static)
-        for (int k = 0; k < cx; k++) {
-            for (int y = 0; y < H; y++) col[y] = cmp[(size_t)y * cx + k];
-            fft1(col.data(), H, +1);
-            for (int y = 0; y < H; y++) cmp[(size_t)y * cx + k] = col[y];
+        for (int k0 = 0; k0 < cx; k0 += CB) {
+            const int nb = std::min(CB, cx - k0);
+            for (int y = 0; y < H; y++)
+                for (int j = 0; j < nb; j++) col[(size_t)j * H + y] = cmp[(size_t)y * cx + k0 + j];
+            for (int j = 0; j < nb; j++) fft1(&col[(size_t)j * H], H, +1);
+            for (int y = 0; y < H; y++)
+                for (int j = 0; j < nb; j++) cmp[(size_t)y * cx + k0 + j] = col[(size_t)j * H + y];
         }
         std::vector<cd> row(W);
 #pragma omp for schedule(static)
@@ -187,6 +204,7 @@ void phase_shift(const cd* in, int cx, int cy, double sx, double sy, bool sum, c
     const int nx = (cx - 1) * 2;
     const double fsx = sx * (2.0 * kPi / nx);
     const double fsy = sy * (2.0 * kPi / cy);
+#pragma omp parallel for schedule(static)
     for (int y = 0; y < cy; y++) {
         const int ys = (y > cy / 2) ? y - cy : y;
         for (int k = 0; k < cx; k++) {
@@ -234,6 +252,7 @@ void correlate_window(const cd* sum, const cd* grp, int cx, int cy, double bft,
     const int W = (cx - 1) * 2, H = cy;
     std::vector<cd> X((size_t)cx * cy);
     const double filt0 = -0.25 * bft / ((double)(cx - 1) * cy);
+#pragma omp parallel for schedule(static)
     for (int y = 0; y < cy; y++) {
         const int ys = (y > cy / 2) ? y - cy : y;
         for (int k = 0; k < cx; k++) {
@@ -262,6 +281,7 @@ void correlate_window(const cd* sum, const cd* grp, int cx, int cy, double bft,
     }
     std::vector<cd> tw(W);
     for (int i = 0; i < W; i++) tw[i] = std::polar(1.0, 2.0 * kPi * i / W);
+#pragma omp parallel for schedule(static)
     for (int j = 0; j < seay; j++) {
         const cd* t = &T[(size_t)j * cx];
         for (int i = 0; i < seax; i++) {
@@ -953,20 +973,29 @@ void detect_bads(const double* s, const double* cen, int nf, int np, int W, int 
 
 /* Correct/GCorrectPatchShift.cu:17-66 (mGCorrect3D, upsample 1): weighted
  * patch shift with w = exp(-100 r), r <= 0.6, nearest gather with (int)
- * truncation and 811*|x| mod N reflection. */
+ * truncation and 811*|x| mod N reflection.  `wts`, when given, holds every
+ * pixel's np weights (0 where r > 0.6, see warp_weights): the same values the
+ * loop would compute, so the result is identical. */
 void local_warp(const double* img, int W, int H, const double* cen, const double* sh,
-                const unsigned char* bad, int np, double* out) {
+                const unsigned char* bad, int np, double* out, const double* wts = nullptr) {
 #pragma omp parallel for schedule(static)
     for (int y = 0; y < H; y++)
         for (int x = 0; x < W; x++) {
             double sx = 0, sy = 0, w = 0;
+            const double* wp = wts ? wts + ((size_t)y * W + x) * np : nullptr;
             for (int p = 0; p < np; p++) {
                 if (bad && bad[p]) continue;
-                double ax = (x - cen[2 * p]) / W;
-                double ay = (y - cen[2 * p + 1]) / H;
-                double r = std::sqrt(ax * ax + ay * ay);
-                if (r > 0.6) continue;
-                double e = std::exp(-100.0 * r);
+                double e;
+                if (wp) {
+                    e = wp[p];
+                    if (e == 0.0) continue;
+                } else {
+                    double ax = (x - cen[2 * p]) / W;
+                    double ay = (y - cen[2 * p + 1]) / H;
+                    double r = std::sqrt(ax * ax + ay * ay);
+                    if (r > 0.6) continue;
+                    e = std::exp(-100.0 * r);
+                }
                 w += e;
                 sx += sh[2 * p] * e;
                 sy += sh[2 * p + 1] * e;
@@ -984,6 +1013,22 @@ void local_warp(const double* img, int W, int H, const double* cen, const double
             }
             out[(size_t)y * W + x] = img[(size_t)ys * W + xs];
         }
+}
+
+/* The patch weights exp(-100 r) of every pixel (0 where r > 0.6): they do
+ * not depend on the frame, so a movie computes them once. */
+std::vector<double> warp_weights(int W, int H, const double* cen, int np) {
+    std::vector<double> wts((size_t)W * H * np);
+#pragma omp parallel for schedule(static)
+    for (int y = 0; y < H; y++)
+        for (int x = 0; x < W; x++)
+            for (int p = 0; p < np; p++) {
+                double ax = (x - cen[2 * p]) / W;
+                double ay = (y - cen[2 * p + 1]) / H;
+                double r = std::sqrt(ax * ax + ay * ay);
+                wts[((size_t)y * W + x) * np + p] = (r > 0.6) ? 0.0 : std::exp(-100.0 * r);
+            }
+    return wts;
 }
 
 }  // namespace
@@ -1069,6 +1114,7 @@ void orc_smooth(double* sx, double* sy, int n, double w) { smooth(sx, sy, n, w);
 
 void orc_dose_weights(int cx, int cy, double pix, int kv, const double* acc, int nf, double* wsum) {
     const double kvf = kv_factor(kv);
+#pragma omp parallel for schedule(static)
     for (int y = 0; y < cy; y++)
         for (int k = 0; k < cx; k++) {
             const size_t i = (size_t)y * cx + k;
@@ -1185,6 +1231,7 @@ void frame_motion(const std::vector<double>& gx, const std::vector<double>& gy, 
 
 void weigh_frame(cd* f, int cx, int cy, const double* m, bool motion, bool corr) {
     if (!motion && !corr) return;
+#pragma omp parallel for schedule(static)
     for (int y = 0; y < cy; y++)
         for (int k = 0; k < cx; k++) f[(size_t)y * cx + k] *= frame_weight(k, y, cx, cy, m[0], m[1], motion, corr);
 }
@@ -1206,9 +1253,12 @@ struct SumBank {
     std::vector<cd> sums;
     const Geo* geo = nullptr;
     double* stack = nullptr;  // aligned stack, one output-size frame per frame
+    /* every loop below is element-wise over the spectrum: the frames are
+     * still added in frame order, element by element */
     void add(cd* f, int i) {
         const size_t fs = (size_t)cx * cy;
         int ns = 0;
+#pragma omp parallel for schedule(static)
         for (size_t q = 0; q < fs; q++) sums[q] += f[q];
         ns++;
         if (stack) {
@@ -1220,6 +1270,7 @@ struct SumBank {
             irfft2(ob.data(), geo->ow, geo->oh, stack + (size_t)i * geo->ow * geo->oh);
         }
         if (dw) {
+#pragma omp parallel for schedule(static)
             for (int y = 0; y < cy; y++)
                 for (int k = 0; k < cx; k++) {
                     const size_t q = (size_t)y * cx + k;
@@ -1227,11 +1278,13 @@ struct SumBank {
                     f[q] *= dose_weight(k, y, cx, cy, pix, kvf, accdose[i]) / wsum[q];
                 }
             cd* s1 = &sums[(size_t)ns * fs];
+#pragma omp parallel for schedule(static)
             for (size_t q = 0; q < fs; q++) s1[q] += f[q];
             ns++;
             if (dws) {
                 if (!(accdose[i] < r0) && !(accdose[i] > r1)) {
                     cd* s2 = &sums[(size_t)ns * fs];
+#pragma omp parallel for schedule(static)
                     for (size_t q = 0; q < fs; q++) s2[q] += f[q];
                 }
                 ns++;
@@ -1239,6 +1292,7 @@ struct SumBank {
         }
         if (split) {
             cd* sp = &sums[(size_t)(ns + (i % 2)) * fs];
+#pragma omp parallel for schedule(static)
             for (size_t q = 0; q < fs; q++) sp[q] += f[q];
         }
     }
@@ -1320,12 +1374,26 @@ int simple_sum(const float* frames, int W, int H, int N, const orc_params* prm, 
     return 0;
 }
 
+/* Stage wall times on stderr when ORC_TIMING is set (where the oracle's
+ * time goes; no effect on results). */
+struct StageClock {
+    bool on = std::getenv("ORC_TIMING") != nullptr;
+    double t0 = omp_get_wtime();
+    void mark(const char* what) {
+        if (!on) return;
+        const double t = omp_get_wtime();
+        std::fprintf(stderr, "oracle %-28s %8.2f s\n", what, t - t0);
+        t0 = t;
+    }
+};
+
 /* Align::CAlignMain::DoIt (Align/CAlignMain.cpp:39-75) for full-frame
  * (Align/CFullAlign.cpp:70-171) and patch (Align/CPatchAlign.cpp:25-92)
  * modes, with the sums of CCorrectFullShift / GCorrectPatchShift. */
 int orc_align_movie(const float* frames, int W, int H, int N, const orc_params* prm,
                     const orc_override* ov, orc_out* out) {
     if (W % 2 || W < 8 || H < 8 || N < 2) return -1;
+    StageClock clk;
     orc_override none;
     std::memset(&none, 0, sizeof(none));
     if (!ov) ov = &none;
@@ -1383,9 +1451,11 @@ int orc_align_movie(const float* frames, int W, int H, int N, const orc_params* 
             for (size_t q = 0; q < (size_t)W * H; q++) img[q] = frames[(size_t)i * W * H + q];
             rfft2(img.data(), W, H, frm.frame(i));
             cd* f = frm.frame(i);
+#pragma omp parallel for schedule(static)
             for (size_t q = 0; q < (size_t)g.cx * g.cy; q++) f[q] *= sc;
         }
     }
+    clk.mark("forward R2C");
     const int ref_frame = (prm->fm_ref < 0) ? N / 2 : std::min(std::max(prm->fm_ref - 1, 0), N - 1);
     const double tol = prm->tol / std::max(g.xbin[0], g.xbin[1]);
 
@@ -1410,6 +1480,7 @@ int orc_align_movie(const float* frames, int W, int H, int N, const orc_params* 
                                            prm->iter, tol, quirks);
             gx = r.sx;
             gy = r.sy;
+            clk.mark("global alignment");
             if (out->global_iters) out->global_iters[0] = r.iters;
             if (out->global_errors)
                 for (int i = 0; i < r.iters; i++) out->global_errors[i] = r.errors[i];
@@ -1428,6 +1499,7 @@ int orc_align_movie(const float* frames, int W, int H, int N, const orc_params* 
             else patch_starts_from_features(feat.data(), prm->patch[0], prm->patch[1], g.xw, g.xh,
                                             starts.data());
             if (out->patch_starts) std::copy(starts.begin(), starts.end(), out->patch_starts);
+            clk.mark("features");
             xreal.assign((size_t)N * g.xw * g.xh, 0.0);
             for (int i = 0; i < N; i++) {
                 phase_shift_skip(xcf.frame(i), g.xcx, g.xcy, -gx[i], -gy[i]);
@@ -1520,6 +1592,7 @@ int orc_align_movie(const float* frames, int W, int H, int N, const orc_params* 
                 ps[((size_t)i * np + p) * 2 + 1] = r.sy[i] * (float)g.xbin[1];
             }
         }
+        clk.mark("patch alignment");
         xreal.clear();
         xreal.shrink_to_fit();
         make_relative(ps.data(), N, np, quirks);
@@ -1531,12 +1604,23 @@ int orc_align_movie(const float* frames, int W, int H, int N, const orc_params* 
 
     /* Correct/GCorrectPatchShift.cu:137-285: C2R, warp, normalised R2C, sums */
     std::vector<double> img((size_t)W * H), wimg((size_t)W * H);
+    /* frame-independent weights, kept when they take at most 8 GB */
+    std::vector<double> wts;
+    if ((double)W * H * np * 8.0 <= 8e9) wts = warp_weights(W, H, pc.data(), np);
+    double t_c2r = 0, t_warp = 0, t_r2c = 0;
     for (int i = 0; i < N; i++) {
         cd* f = frm.frame(i);
+        double ta = omp_get_wtime();
         irfft2(f, W, H, img.data());
+        double tb = omp_get_wtime();
         local_warp(img.data(), W, H, pc.data(), &ps[(size_t)i * np * 2], &bad[(size_t)i * np], np,
-                   wimg.data());
+                   wimg.data(), wts.empty() ? nullptr : wts.data());
+        double tc = omp_get_wtime();
         rfft2(wimg.data(), W, H, f);
+        t_c2r += tb - ta;
+        t_warp += tc - tb;
+        t_r2c += omp_get_wtime() - tc;
+#pragma omp parallel for schedule(static)
         for (size_t q = 0; q < (size_t)g.cx * g.cy; q++) f[q] *= sc;
         /* mGenSums' mMotionDecon with the same full shifts
          * (Correct/GCorrectPatchShift.cu:106): applied a second time */
@@ -1545,7 +1629,10 @@ int orc_align_movie(const float* frames, int W, int H, int N, const orc_params* 
         weigh_frame(f, g.cx, g.cy, m, prm->in_fm_motion != 0, false);
         bank.add(f, i);
     }
+    if (clk.on) std::fprintf(stderr, "oracle   C2R %.2f s, warp %.2f s, R2C %.2f s\n", t_c2r, t_warp, t_r2c);
+    clk.mark("local correction + sums");
     bank.finish(g, out->sums);
+    clk.mark("output");
     finish_stack();
     return 0;
 }
