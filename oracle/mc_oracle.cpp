@@ -799,19 +799,23 @@ void feature_map_from_sum(const cd* sumx, int xcx, int xcy, unsigned char* feat)
     feature_threshold(img.data(), B * B, feat, nullptr);
 }
 
-/* Align/CDetectFeatures.cpp:289-396 and Align/CPatchCenters.cpp:42-105. */
+/* Align/CDetectFeatures.cpp:272-396 and Align/CPatchCenters.cpp:42-105.
+ * These are discrete decisions (which feature is nearest, where a window
+ * starts), so they are evaluated in FP32 as the reference writes them: in
+ * FP64 a tie between two features at equal FP32 distance breaks the other
+ * way. */
 struct FeatureCenters {
     int B = 128, npx, npy;
-    double pat[2];
+    float pat[2];
     int sea[4];
     std::vector<unsigned char> feat, used;
-    std::vector<double> cent;   /* 3 per patch */
+    std::vector<float> cent;   /* 3 per patch */
     int check_range(int start, int size, const int* rng) const {
         if (start < rng[0]) return rng[0];
         if (start + size > rng[1]) return rng[1] - size;
         return start;
     }
-    void set_used(double cxv, double cyv) {
+    void set_used(float cxv, float cyv) {
         int wx = (int)(pat[0] * 0.8f), wy = (int)(pat[1] * 0.8f);
         int sx = (int)(cxv - wx / 2), sy = (int)(cyv - wy / 2);
         sx = check_range(sx, wx, &sea[0]);
@@ -819,26 +823,26 @@ struct FeatureCenters {
         for (int y = 0; y < wy; y++)
             for (int x = 0; x < wx; x++) used[(size_t)(sy + y) * B + sx + x] = 1;
     }
-    bool check_feature(double cxv, double cyv) const {
+    bool check_feature(float cxv, float cyv) const {
         int ix = (int)cxv, iy = (int)cyv;
         if (ix < sea[0] || ix > sea[1]) return false;
         if (iy < sea[2] || iy > sea[3]) return false;
         return feat[(size_t)iy * B + ix] != 0;
     }
-    void find_center(double* c) {
+    void find_center(float* c) {
         int wx = B / 3, wy = B / 3;
         int sx = (int)(c[0] - wx * 0.5f + 0.5f), sy = (int)(c[1] - wy * 0.5f + 0.5f);
         sx = check_range(sx, wx, &sea[0]);
         sy = check_range(sy, wy, &sea[2]);
-        double minr = 1e20;
-        double ac[2] = {-1.0, 0.0};
+        float minr = 1e20f;
+        float ac[2] = {-1.0f, 0.0f};
         for (int y = 0; y < wy; y++)
             for (int x = 0; x < wx; x++) {
                 size_t j = (size_t)(sy + y) * B + sx + x;
                 if (used[j] || !feat[j]) continue;
-                double fx = x + sx + 0.5, fy = y + sy + 0.5;
-                double dx = fx - c[0], dy = fy - c[1];
-                double r = std::sqrt(dx * dx + dy * dy);
+                float fx = x + sx + 0.5f, fy = y + sy + 0.5f;
+                float dx = fx - c[0], dy = fy - c[1];
+                float r = sqrtf(dx * dx + dy * dy);
                 if (r >= minr) continue;
                 minr = r;
                 ac[0] = fx;
@@ -860,15 +864,15 @@ struct FeatureCenters {
         if (sea[2] < 4) sea[2] = 4;
         sea[3] = B - sea[2];
         const int np = px * py;
-        cent.assign(3 * np, 0.0);
+        cent.assign(3 * np, 0.0f);
         for (int i = 0; i < np; i++) {
             int x = i % px, y = i / px;
-            double fx = (x + 0.5f) * pat[0], fy = (y + 0.5f) * pat[1];
+            float fx = (x + 0.5f) * pat[0], fy = (y + 0.5f) * pat[1];
             bool has = check_feature(fx, fy);
             cent[3 * i] = fx;
             cent[3 * i + 1] = fy;
-            if (has) { cent[3 * i + 2] = 1.0; set_used(fx, fy); }
-            else cent[3 * i + 2] = -1.0;
+            if (has) { cent[3 * i + 2] = 1.0f; set_used(fx, fy); }
+            else cent[3 * i + 2] = -1.0f;
         }
         for (int i = 0; i < np; i++) {
             if (cent[3 * i + 2] > 0) continue;
@@ -882,15 +886,18 @@ void patch_starts_from_features(const unsigned char* feat, int px, int py, int x
     FeatureCenters fc;
     fc.run(feat, px, py);
     const int psx = xw / px / 2 * 2, psy = xh / py / 2 * 2;
-    auto calc_start = [](double c, int ps, int xs) {
+    /* CPatchCenters::mCalcStart (Align/CPatchCenters.cpp:99-105) */
+    auto calc_start = [](float c, int ps, int xs) {
         int s = (int)(c - ps / 2);
         if (s < 0) s = 0;
         else if (s + ps > xs) s = xs - ps;
         return s;
     };
+    /* CDetectFeatures::GetCenter (Align/CDetectFeatures.cpp:188-195) */
+    const float binx = xw * 1.0f / 128, biny = xh * 1.0f / 128;
     for (int i = 0; i < px * py; i++) {
-        const double cxv = fc.cent[3 * i] * (xw * 1.0f / 128);
-        const double cyv = fc.cent[3 * i + 1] * (xh * 1.0f / 128);
+        const float cxv = fc.cent[3 * i] * binx;
+        const float cyv = fc.cent[3 * i + 1] * biny;
         starts[2 * i] = calc_start(cxv, psx, xw);
         starts[2 * i + 1] = calc_start(cyv, psy, xh);
     }

@@ -79,6 +79,23 @@ def test_patch_starts_single_feature_taken_by_first_patch():
     assert np.array_equal(st, want)
 
 
+def test_patch_starts_tie_goes_to_the_first_feature_in_scan_order():
+    """3 x 3 patches: patch 1's grid centre is x = 1.5 * (128 / 3) = 64.0 in
+    FP32 (64.0000019 in FP64), y = 21.33, and its cell holds no feature.
+    Two features in row 22 at x = 63 and x = 64 (centres 63.5 and 64.5) are
+    at the same FP32 distance from it: the strict < of mFindCenter keeps the
+    first one scanned, x = 63.5 (Align/CDetectFeatures.cpp:337-361, all
+    FP32).  An FP64 evaluation would take 64.5 (start 346)."""
+    px = py = 3
+    xw = xh = 1024
+    f = np.zeros((128, 128), np.uint8)
+    f[22, 63] = f[22, 64] = 1
+    st = O.patch_starts(f, px, py, xw, xh)
+    want = _grid_starts(px, py, xw, xh)
+    want[1] = (338, 10)     # (63.5 * 8 - 170, 22.5 * 8 - 170)
+    assert np.array_equal(st, want)
+
+
 # ---- CDetectFeatures feature map (Align/CDetectFeatures.cpp:58-186) ----
 def _xcf_sum(seed, W=512, H=384):
     rng = np.random.default_rng(seed)
