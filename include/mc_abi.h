@@ -248,6 +248,19 @@ int mc_patch_stats_fp64(mc_ctx* ctx, double* shifts_host, const double* centers_
 int mc_local_warp(mc_ctx* ctx, const float* in, float* out, int W, int H, int N, int P,
                   const float* centers_host, const float* shifts_host,
                   const unsigned char* bad_host);
+/* As mc_local_warp, with no device allocation: no counterpart in the
+ * reference.  For callers that cannot allocate device memory (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc).  The
+ * patch tables (centres, shifts, bad flags) go to a file-scope device array
+ * of 4 MiB per GPU (held until the process ends) instead of an allocation,
+ * so N * P * 9 bytes plus about 1 KiB must fit in it (e.g. 25 patches x
+ * 18000 frames); larger tables return MC_E_ARG.  Every call on one GPU
+ * shares that array and leaves its tables there: calls that overlap on one
+ * GPU (other streams or contexts) race.  Same kernel and results as
+ * mc_local_warp. */
+int mc_local_warp_noalloc(mc_ctx* ctx, const float* in, float* out, int W, int H, int N, int P,
+                          const float* centers_host, const float* shifts_host,
+                          const unsigned char* bad_host);
 /* GWeightFrame weight normaliser sqrt(sum_i w_i^2) (Correct/GWeightFrame.cu:
  * 19-30, 79-133) as a [kx][ky] table on the device. */
 int mc_dose_weight_sum(mc_ctx* ctx, int W, int H, int N, float pix, int kv, float fm_dose,

@@ -26,6 +26,7 @@ int mc_align_init_constants();
 hipError_t mc_launch_phasors(const double* shifts, double mult, int skip, const unsigned char* done,
                              int per_job, int W, int cx, int H, int batch, cfloat* A, cfloat* B,
                              hipStream_t s, int h_norm = 0);  // h_norm: y frequency scale (0: H)
+hipError_t mc_launch_fill_u8(unsigned char* p, long long n, unsigned char v, hipStream_t s);
 hipError_t mc_launch_fourier_resize(const cfloat* in, int inx, int iny, long long in_stride, cfloat* out,
                                     int outx, int outy, long long out_stride, int batch, int sum,
                                     hipStream_t s);

@@ -74,6 +74,19 @@ __global__ __launch_bounds__(256) void k_fourier_resize(const cfloat* __restrict
     }
 }
 
+// Byte fill of device memory by a kernel (graph-capturable, unlike a memset
+// node).
+__global__ __launch_bounds__(256) void k_fill_u8(unsigned char* p, long long n, unsigned char v) {
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+hipError_t mc_launch_fill_u8(unsigned char* p, long long n, unsigned char v, hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_fill_u8, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, p, n, v);
+    return hipGetLastError();
+}
+
 hipError_t mc_launch_fourier_resize(const cfloat* in, int inx, int iny, long long in_stride, cfloat* out,
                                     int outx, int outy, long long out_stride, int batch, int sum,
                                     hipStream_t s) {

@@ -14,6 +14,7 @@
 #include <cstdio>
 #include <cstring>
 #include <map>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -49,6 +50,8 @@ struct TimedEv {
 
 struct mc_ctx {
     int device = 0;
+    // this device's address of the file-scope `scratch` array
+    unsigned char* scratch = nullptr;
     hipStream_t stream = nullptr;
     bool own_stream = false;
     std::string err;
@@ -91,6 +94,32 @@ struct mc_ctx {
     float* dark_aug = nullptr;
     int aug_fact = 0;
 };
+
+// Device memory of the allocation-free entry point (mc_local_warp_noalloc):
+// its patch tables live in this array, one copy per GPU for the life of the
+// process, instead of in an allocation.  Host code never passes `scratch`
+// itself (on the host it names a stand-in, not device memory): each device's
+// address comes from hipGetSymbolAddress, taken once per device when a
+// context is created there.
+#define MC_SCRATCH_BYTES (4u << 20)
+#define MC_SCRATCH_DEVICES 64
+static __device__ __align__(256) unsigned char scratch[MC_SCRATCH_BYTES];
+static std::mutex g_scratch_mu;
+static unsigned char* g_scratch_ptr[MC_SCRATCH_DEVICES] = {};
+
+// With `device` current: its address of `scratch`, looked up on first use.
+static hipError_t scratch_for_device(int device, unsigned char** out) {
+    if (device < 0 || device >= MC_SCRATCH_DEVICES) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_ptr[device]) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return e;
+        g_scratch_ptr[device] = (unsigned char*)p;
+    }
+    *out = g_scratch_ptr[device];
+    return hipSuccess;
+}
 
 #ifndef MC_CHUNK_FRAMES
 #define MC_CHUNK_FRAMES 4
@@ -1485,6 +1514,10 @@ int mc_ctx_create(int device, void* stream, mc_ctx** out) {
     if (hipSetDevice(device) != hipSuccess) return MC_E_HIP;
     mc_ctx* c = new mc_ctx;
     c->device = device;
+    if (scratch_for_device(device, &c->scratch) != hipSuccess) {
+        delete c;
+        return MC_E_HIP;
+    }
     if (stream) {
         c->stream = (hipStream_t)stream;
     } else {
@@ -1856,6 +1889,41 @@ int mc_local_warp(mc_ctx* c, const float* in, float* out, int W, int H, int N, i
     MC_TRY(hipMemcpyAsync(ds, shifts, sizeof(float) * 2 * N * P, hipMemcpyHostToDevice, c->stream));
     if (bad) MC_TRY(hipMemcpyAsync(db, bad, (size_t)N * P, hipMemcpyHostToDevice, c->stream));
     else MC_TRY(hipMemsetAsync(db, 0, (size_t)N * P, c->stream));
+    MC_TRY(mc_launch_local_warp(in, out, W, (long long)W * H, W, H, N, P, dc, ds, db, c->stream));
+    return MC_OK;
+}
+
+int mc_local_warp_noalloc(mc_ctx* c, const float* in, float* out, int W, int H, int N, int P,
+                          const float* centers, const float* shifts, const unsigned char* bad) {
+    if (!c || !in || !out || !centers || !shifts || W < 2 || H < 2 || N < 1 || P < 1 || P > 64)
+        return MC_E_ARG;
+    {
+        const char* i0 = (const char*)in;
+        const char* o0 = (const char*)out;
+        const size_t bytes = sizeof(float) * (size_t)W * H * N;
+        if (i0 < o0 + bytes && o0 < i0 + bytes)
+            return fail(c, MC_E_ARG, "mc_local_warp_noalloc: in and out overlap");
+    }
+    // tables in the scratch array: centres, shifts, bad flags, each at a
+    // 256-byte boundary
+    auto up = [](size_t b) { return (b + 255) / 256 * 256; };
+    const size_t oc = 0, os = up(sizeof(float) * 2 * (size_t)P);
+    const size_t ob = os + up(sizeof(float) * 2 * (size_t)N * P);
+    const size_t need = ob + up((size_t)N * P);
+    if (need > MC_SCRATCH_BYTES)
+        return fail(c, MC_E_ARG, "mc_local_warp_noalloc: tables of N x P = " + std::to_string(N) + " x " +
+                                     std::to_string(P) + " need " + std::to_string(need) +
+                                     " bytes of scratch, more than its " +
+                                     std::to_string(MC_SCRATCH_BYTES));
+    if (!c->scratch) return fail(c, MC_E_STATE, "mc_local_warp_noalloc: no scratch address");
+    float* dc = (float*)(c->scratch + oc);
+    float* ds = (float*)(c->scratch + os);
+    unsigned char* db = c->scratch + ob;
+    MC_TRY(hipMemcpyAsync(dc, centers, sizeof(float) * 2 * P, hipMemcpyHostToDevice, c->stream));
+    MC_TRY(hipMemcpyAsync(ds, shifts, sizeof(float) * 2 * (size_t)N * P, hipMemcpyHostToDevice, c->stream));
+    // the previous call's flags are still there: write or clear all of them
+    if (bad) MC_TRY(hipMemcpyAsync(db, bad, (size_t)N * P, hipMemcpyHostToDevice, c->stream));
+    else MC_TRY(mc_launch_fill_u8(db, (long long)N * P, 0, c->stream));
     MC_TRY(mc_launch_local_warp(in, out, W, (long long)W * H, W, H, N, P, dc, ds, db, c->stream));
     return MC_OK;
 }

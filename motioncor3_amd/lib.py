@@ -17,6 +17,7 @@ LIB_PATH = os.path.join(_HERE, "libmc3.so")
 _LIB = None
 
 MC_OK = 0
+MC_E_ARG = -1
 
 
 class McParams(C.Structure):
@@ -84,6 +85,7 @@ ABI_SYMBOLS = (
     "mc_phase_shift_accumulate", "mc_group_correlate_window", "mc_peak_fit",
     "mc_spline_despike_fp64", "mc_patch_stats_fp64", "mc_local_warp", "mc_dose_weight_sum",
     "mc_fourier_crop", "mc_feature_centres", "mc_extract_patches", "mc_finalize_sums",
+    "mc_local_warp_noalloc",
     "mc_set_kernel_timing", "mc_get_kernel_timing", "mc_synth_render",
     "mc_prior_create", "mc_prior_destroy", "mc_prior_dims", "mc_prior_params",
     "mc_prior_forward", "mc_prior_loss_grad", "mc_prior_train", "mc_set_prior", "mc_gemm_bf16",
@@ -227,11 +229,15 @@ class Context:
         arr = np.asarray(arr)
         return DevArray(self, arr.shape, dtype or arr.dtype).upload(arr)
 
-    def copy(self, dst, src):
-        """Device-to-device copy of src into dst (same byte size)."""
-        assert dst.nbytes == src.nbytes
+    def copy(self, dst, src, nbytes=None):
+        """Device-to-device copy of src into dst (same byte size), or of the
+        first nbytes of src."""
+        if nbytes is None:
+            assert dst.nbytes == src.nbytes
+            nbytes = src.nbytes
+        assert nbytes <= min(dst.nbytes, src.nbytes)
         self._check(self.lib.mc_memcpy(self.ptr, C.c_void_p(dst.ptr), C.c_void_p(src.ptr),
-                                       C.c_ulonglong(src.nbytes), 2), "mc_memcpy")
+                                       C.c_ulonglong(nbytes), 2), "mc_memcpy")
 
     # ---- stage entry points ----
     def rfft2(self, frames, scale=1.0):

@@ -85,6 +85,23 @@ class MovieRecipe:
         t = frame / max(self.N - 1, 1)
         return self._local_raw(np.asarray(x) / self.W - 0.5, np.asarray(y) / self.H - 0.5, t)
 
+    def mean_local(self, frame, grid=64):
+        """Frame mean of the local displacement (over a grid x grid lattice of
+        pixel centres): the part of the local field a whole-frame alignment
+        sees as drift."""
+        if self.local_coef is None:
+            return 0.0, 0.0
+        xs = (np.arange(grid) + 0.5) * self.W / grid
+        ys = (np.arange(grid) + 0.5) * self.H / grid
+        X, Y = np.meshgrid(xs, ys)
+        dx, dy = self.local_field(X, Y, frame)
+        return float(dx.mean()), float(dy.mean())
+
+    def global_truth(self):
+        """N x 2 content displacement of each frame as a whole-frame alignment
+        sees it: the global drift plus the frame mean of the local field."""
+        return self.displacement + np.array([self.mean_local(t) for t in range(self.N)])
+
     def render(self, frame):
         """Noiseless frame (H x W float64), periodic wrap."""
         W, H = self.W, self.H
