@@ -108,7 +108,7 @@ def cpu_baseline(ctx, frames, gres, params_kw, frames_needed):
     from motioncor3_amd import lib as L
 
     po = O.default_params(**{k: v for k, v in params_kw.items()
-                             if k not in ("spectra_fmt", "movie_pipeline")})
+                             if k not in ("spectra_fmt", "frame_fmt", "movie_pipeline")})
     if frames_needed >= N:
         host = frames.numpy()
         g = gres
@@ -159,6 +159,9 @@ def main():
                          "fewer scale its time to a movie)")
     ap.add_argument("--spectra", default="f32", choices=["f32", "f16", "bf16"],
                     help="storage of the xcf and patch spectrum stacks (arithmetic stays FP32)")
+    ap.add_argument("--frames", default="f32", choices=["f32", "f16", "bf16"],
+                    help="storage of the full-frame spectra (forward transforms, local-correction "
+                         "output, what the sums read); arithmetic stays FP32")
     ap.add_argument("--batch", type=int, default=8,
                     help="movies per step: K > 1 aligns K resident movies in one mc_align_movies "
                          "call (each movie's forward transform overlaps the previous movie's "
@@ -200,6 +203,7 @@ def main():
     ctx.synth_render(frames, recipe)
     kw = dict(patch=(5, 5), bft=(500, 100), pix_size=0.83, kv=300, fm_dose=1.0,
               spectra_fmt={"f32": 0, "f16": 1, "bf16": 2}[args.spectra],
+              frame_fmt={"f32": 0, "f16": 1, "bf16": 2}[args.frames],
               movie_pipeline=0 if args.no_pipeline else 1)
     prm = L.default_params(**kw)
     ns, ow, oh = ctx.output_geometry(prm, W, H, N)
@@ -440,6 +444,7 @@ def main():
             "movies_per_call": K,
             "movie_pipeline": K > 1 and not args.no_pipeline,
             "spectra_storage_used": [["f32", "f16", "bf16"][v] for v in res["spectra_fmt_used"]],
+            "frame_storage_used": ["f32", "f16", "bf16"][res["frame_fmt_used"]],
             "learned_prior": args.prior > 0,
             "fft_tiles": tiles if tiles is not None else "defaults",
         },

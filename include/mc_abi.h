@@ -107,6 +107,17 @@ typedef struct mc_params {
     int    corr_interp;      /* -CorrInterp (0), patch mode: divide the globally
                                 corrected frames by sinc^(1/8)
                                 (Util/GCorrLinearInterp.cu:7-24) */
+    int    frame_fmt;        /* storage of the full-frame spectra (the frm
+                                stack: forward transforms, local-correction
+                                output, the spectra the sums read): 0 FP32
+                                (default), 1 FP16, 2 BF16, each frame scaled
+                                like the reduced stacks; arithmetic stays FP32.
+                                No reference counterpart.  Kept at FP32 when the
+                                frame stack is the xcf stack (xcf bin 1) or
+                                -InFmMotion / -CorrInterp weigh it in place;
+                                mc_result.frame_fmt_used reports it.  With a
+                                reduced format and spectra_fmt reduced too, the
+                                xcf stack takes the frame format. */
 } mc_params;
 
 /* N frames already gain/dark/defect corrected, resident in HBM
@@ -149,6 +160,7 @@ typedef struct mc_result {
                                   dose weighting at the -FtBin size
                                   (Correct/CCorrectFullShift.cpp:336-356), z-binned
                                   as Align/CZbinStack.cpp:27-70 */
+    int    frame_fmt_used;     /* storage actually used for the full-frame spectra */
 } mc_result;
 
 /* Replacements of measured quantities.  replay_* follow -InAln

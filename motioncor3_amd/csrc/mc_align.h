@@ -59,7 +59,8 @@ hipError_t mc_launch_local_warp(const float* R, float* Rp, long long pitch, long
 hipError_t mc_launch_dw_sums(const cfloat* F, long long fstride, int N, int cx, int H,
                              const cfloat* A, const cfloat* B, int dw, double pix, double kvf,
                              double dose, int dws, double r0, double r1, int split, cfloat* sums,
-                             long long sstride, hipStream_t s, const int* cum = nullptr);
+                             long long sstride, hipStream_t s, const int* cum = nullptr,
+                             int f_fmt = MC_SPEC_F32, const float* f_inv = nullptr);
 hipError_t mc_launch_dose_wsum(int cx, int H, int N, double pix, double kvf, double dose,
                                float* out, hipStream_t s);
 hipError_t mc_launch_despike(double* traj, double* scratch, int n, int count, int quirks,

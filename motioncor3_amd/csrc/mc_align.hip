@@ -1228,7 +1228,8 @@ static size_t warp_lds_bytes(int P) {
 // (DataUtil/CFmIntParam.cpp:128-135) are generated as powers of
 // q = exp(-0.5 dose / Dc) in FP64.
 struct DwArgs {
-    const cfloat* F;
+    const cfloat* F;      // frame spectra, stored in the kernel's format FF
+    const float* finv;    // per-frame load factors of a reduced format
     long long fstride;
     int N, cx, H;
     const cfloat* A;
@@ -1253,6 +1254,7 @@ __device__ __forceinline__ double crit_q(int kx, int y, int cx, int H, double pi
     return exp(-0.5 * dose / crit);
 }
 
+template <int FF>
 __global__ __launch_bounds__(256) void k_dw_sums(DwArgs a) {
     const int kx = blockIdx.y;
     const int y = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1276,7 +1278,7 @@ __global__ __launch_bounds__(256) void k_dw_sums(DwArgs a) {
     double w = 1.0;
     int prev = 0;
     for (int i = 0; i < a.N; i++) {
-        cfloat v = a.F[(long long)i * a.fstride + q];
+        cfloat v = Spec<FF>::ld(a.F, (long long)i * a.fstride + q, FF == MC_SPEC_F32 ? 1.f : a.finv[i]);
         if (a.A) v = cmul(v, cmul(a.A[(long long)i * a.cx + kx], a.B[(long long)i * a.H + y]));
         s0 = cadd(s0, v);
         if (a.dw) {
@@ -1593,9 +1595,11 @@ hipError_t mc_launch_local_warp(const float* R, float* Rp, long long pitch, long
 hipError_t mc_launch_dw_sums(const cfloat* F, long long fstride, int N, int cx, int H,
                              const cfloat* A, const cfloat* B, int dw, double pix, double kvf,
                              double dose, int dws, double r0, double r1, int split, cfloat* sums,
-                             long long sstride, hipStream_t s, const int* cum) {
+                             long long sstride, hipStream_t s, const int* cum, int f_fmt,
+                             const float* f_inv) {
     DwArgs a;
     a.F = F;
+    a.finv = f_inv;
     a.fstride = fstride;
     a.N = N;
     a.cx = cx;
@@ -1614,7 +1618,12 @@ hipError_t mc_launch_dw_sums(const cfloat* F, long long fstride, int N, int cx, 
     a.sstride = sstride;
     a.cum = cum;
     dim3 grid(mc_div_up(H, 256), cx);
-    hipLaunchKernelGGL(k_dw_sums, grid, dim3(256), 0, s, a);
+    switch (f_fmt) {
+        case MC_SPEC_F32: hipLaunchKernelGGL(k_dw_sums<MC_SPEC_F32>, grid, dim3(256), 0, s, a); break;
+        case MC_SPEC_F16: hipLaunchKernelGGL(k_dw_sums<MC_SPEC_F16>, grid, dim3(256), 0, s, a); break;
+        case MC_SPEC_BF16: hipLaunchKernelGGL(k_dw_sums<MC_SPEC_BF16>, grid, dim3(256), 0, s, a); break;
+        default: return hipErrorInvalidValue;
+    }
     return hipGetLastError();
 }
 

@@ -963,7 +963,9 @@ static hipError_t launch_cols_fwd_rp(const ColFwdArgs& a, const cfloat* ptw, int
         case 1: hipLaunchKernelGGL((k_cols_fwd_rp<H, CW, NT, 0, 1>), grid, blk, lds, s, a, ptw); break;
         case 2: hipLaunchKernelGGL((k_cols_fwd_rp<H, CW, NT, 0, 2>), grid, blk, lds, s, a, ptw); break;
         case 3: hipLaunchKernelGGL((k_cols_fwd_rp<H, CW, NT, 1, 0>), grid, blk, lds, s, a, ptw); break;
+        case 4: hipLaunchKernelGGL((k_cols_fwd_rp<H, CW, NT, 1, 1>), grid, blk, lds, s, a, ptw); break;
         case 6: hipLaunchKernelGGL((k_cols_fwd_rp<H, CW, NT, 2, 0>), grid, blk, lds, s, a, ptw); break;
+        case 8: hipLaunchKernelGGL((k_cols_fwd_rp<H, CW, NT, 2, 2>), grid, blk, lds, s, a, ptw); break;
         default: return hipErrorInvalidValue;
     }
     return hipGetLastError();

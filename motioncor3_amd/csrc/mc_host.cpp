@@ -728,7 +728,7 @@ struct FwdLane {
 };
 int enqueue_forward(mc_ctx* c, Fft2D* pf, const mc_movie* const* mvs, int count, const Geo& g,
                     int fx, cfloat* Sf_all, void* Sx_all, float* gx, float* ix, const FwdLane* lanes,
-                    int nl, int CH) {
+                    int nl, int CH, int ff = MC_SPEC_F32, float* gf = nullptr, float* iff = nullptr) {
     const int W = mvs[0]->W, H = mvs[0]->H, N = mvs[0]->N;
     const long long fs = (long long)g.cx * H, xfs = (long long)g.xcx * g.xh;
     const float scale = (float)(1.0 / ((double)W * H));
@@ -746,11 +746,12 @@ int enqueue_forward(mc_ctx* c, Fft2D* pf, const mc_movie* const* mvs, int count,
                                        L.dcf));
             }
             if (fx != MC_SPEC_F32) MC_TRY(mc_launch_spec_scale(L.dcf, H, nb, gx + fi, ix + fi, L.s));
+            if (ff != MC_SPEC_F32) MC_TRY(mc_launch_spec_scale(L.dcf, H, nb, gf + fi, iff + fi, L.s));
             {
                 KTimer kt(c, "fft_cols_fwd", L.s);
-                MC_TRY(mc_cols_fwd(*pf, L.T, pf->t_elems, Sf_all + fi * fs, fs,
+                MC_TRY(mc_cols_fwd(*pf, L.T, pf->t_elems, spec_at(Sf_all, fi * fs, ff), fs,
                                    Sx_all ? spec_at(Sx_all, fi * xfs, fx) : nullptr, g.xcx, g.xh, xfs,
-                                   nb, L.s, MC_SPEC_F32, nullptr, fx,
+                                   nb, L.s, ff, ff != MC_SPEC_F32 ? gf + fi : nullptr, fx,
                                    fx != MC_SPEC_F32 ? gx + fi : nullptr));
             }
         }
@@ -932,9 +933,22 @@ int align_batch(mc_ctx* c, const mc_params* const* pp_, const mc_movie* const* m
     if (!patch || replay || !mc_fft_rp_rows(*pp) || !mc_fft_rp_cols(*pp) ||
         !mc_correlate_has_rp(g.ph))
         fpch = MC_SPEC_F32;
+    // Full-frame spectra in a reduced format: every stage that reads or
+    // writes them must run the register-pass kernels; frame weights
+    // (in place, FP32) and a frame stack that is also the xcf stack keep FP32.
+    if (p->frame_fmt < MC_SPEC_F32 || p->frame_fmt > MC_SPEC_BF16)
+        return fail(c, MC_E_ARG, "frame_fmt must be 0 (FP32), 1 (FP16) or 2 (BF16)");
+    int ff = p->frame_fmt;
+    if (xcf_alias || !mc_fft_rp_rows(*pf) || !mc_fft_rp_cols(*pf) || p->in_fm_motion || p->corr_interp ||
+        (po != pf && !mc_fft_rp_cols(*po)))
+        ff = MC_SPEC_F32;
+    // the forward column pass stores both the frames and the xcf crop; mixed
+    // reduced formats are not compiled
+    if (ff != MC_SPEC_F32 && fx != MC_SPEC_F32 && ff != fx) fx = ff;
     for (int m = 0; m < count; m++) {
         rs[m]->spectra_fmt_used[0] = fx;
         rs[m]->spectra_fmt_used[1] = fpch;
+        rs[m]->frame_fmt_used = ff;
         rs[m]->prior_fallbacks = 0;
     }
     const int CH = MC_CHUNK_FRAMES;  // frames per forward/inverse chunk
@@ -942,14 +956,21 @@ int align_batch(mc_ctx* c, const mc_params* const* pp_, const mc_movie* const* m
     // m * N * xfs of Sx_all, its patch stack at element m * P * N * pfs of Sp_all
     // the stacks of a movie slot: full spectra, xcf stack, per-frame factors
     struct Stacks {
-        cfloat* Sf = nullptr;
+        cfloat* Sf = nullptr;   // stored in format ff
         cfloat* Sx = nullptr;
         float* gx = nullptr;
         float* ix = nullptr;
+        float* gf = nullptr;    // per-frame store / load factors of a reduced ff
+        float* iff = nullptr;
     };
     auto stacks = [&](int slot, Stacks& o) -> bool {
         const std::string k = slot ? "1" : "";
-        o.Sf = (cfloat*)buf(c, ("Sf" + k).c_str(), sizeof(cfloat) * (size_t)count * N * fs);
+        o.Sf = (cfloat*)buf(c, ("Sf" + k).c_str(), (ff == MC_SPEC_F32 ? 8 : 4) * (size_t)count * N * fs);
+        if (o.Sf && ff != MC_SPEC_F32) {
+            o.gf = (float*)buf(c, ("gf" + k).c_str(), sizeof(float) * (size_t)count * N);
+            o.iff = (float*)buf(c, ("iff" + k).c_str(), sizeof(float) * (size_t)count * N);
+            if (!o.gf || !o.iff) return false;
+        }
         o.Sx = o.Sf;
         if (o.Sf && !xcf_alias && !replay)
             o.Sx = (cfloat*)buf(c, ("Sx" + k).c_str(), (fx == MC_SPEC_F32 ? 8 : 4) * (size_t)count * N * xfs);
@@ -966,9 +987,11 @@ int align_batch(mc_ctx* c, const mc_params* const* pp_, const mc_movie* const* m
     cfloat* Sx_all = cur.Sx;
     float* gx = cur.gx;
     float* ix = cur.ix;
+    float* gf = cur.gf;
+    float* iff = cur.iff;
     MC_BUF(T, cfloat, "T", (size_t)CH * pf->t_elems);
     float* dcf = nullptr;   // |row DC| of the chunk's frames
-    if (fx != MC_SPEC_F32) {
+    if (fx != MC_SPEC_F32 || ff != MC_SPEC_F32) {
         dcf = (float*)buf(c, "dcf", sizeof(float) * (size_t)CH * H);
         if (!dcf) return fail(c, MC_E_NOMEM, "stack scale factors");
     }
@@ -980,7 +1003,7 @@ int align_batch(mc_ctx* c, const mc_params* const* pp_, const mc_movie* const* m
     const bool two = p->chunk_streams >= 2;
     MC_BUF(T2, cfloat, "T2", (size_t)CH * pf->t_elems);
     float* dcf2 = nullptr;
-    if (fx != MC_SPEC_F32) {
+    if (fx != MC_SPEC_F32 || ff != MC_SPEC_F32) {
         dcf2 = (float*)buf(c, "dcf2", sizeof(float) * (size_t)CH * H);
         if (!dcf2) return fail(c, MC_E_NOMEM, "stack scale factors");
     }
@@ -1001,7 +1024,8 @@ int align_batch(mc_ctx* c, const mc_params* const* pp_, const mc_movie* const* m
     } else {
         const FwdLane lanes[2] = {{s, T, dcf}, {c->s2, T2, dcf2}};
         MC_TRY(fork());
-        MC_CHECK(enqueue_forward(c, pf, mvs, count, g, fx, Sf_all, Sx_fwd, gx, ix, lanes, two ? 2 : 1, CH));
+        MC_CHECK(enqueue_forward(c, pf, mvs, count, g, fx, Sf_all, Sx_fwd, gx, ix, lanes, two ? 2 : 1, CH,
+                                 ff, gf, iff));
         MC_TRY(join());
     }
     if (pipe && pipe->next) {
@@ -1013,7 +1037,7 @@ int align_batch(mc_ctx* c, const mc_params* const* pp_, const mc_movie* const* m
         MC_CHECK(wait_movie_loads(c, pipe->next, c->s2));
         const FwdLane lane{c->s2, T2, dcf2};
         MC_CHECK(enqueue_forward(c, pf, &pipe->next, 1, g, fx, nx.Sf, Sx_fwd ? (void*)nx.Sx : nullptr,
-                                 nx.gx, nx.ix, &lane, 1, CH));
+                                 nx.gx, nx.ix, &lane, 1, CH, ff, nx.gf, nx.iff));
         MC_TRY(hipEventRecord(pipe->ev[pipe->slot ^ 1], c->s2));
     }
 
@@ -1095,8 +1119,8 @@ int align_batch(mc_ctx* c, const mc_params* const* pp_, const mc_movie* const* m
                 PhasorTabs ph{fA_e + (long long)f0 * g.cx, fB_e + (long long)f0 * H};
                 {
                     KTimer kt(c, "fft_cols_inv", c->s2);
-                    MC_TRY(mc_cols_inv(*pf, Sf_all + (long long)f0 * fs, fs, g.cx, H, &ph, T2,
-                                       pf->t_elems, nb, c->s2));
+                    MC_TRY(mc_cols_inv(*pf, spec_at(Sf_all, (long long)f0 * fs, ff), fs, g.cx, H, &ph, T2,
+                                       pf->t_elems, nb, c->s2, ff, iff ? iff + f0 : nullptr));
                 }
                 {
                     KTimer kt(c, "fft_rows_c2r", c->s2);
@@ -1284,7 +1308,9 @@ int align_batch(mc_ctx* c, const mc_params* const* pp_, const mc_movie* const* m
         const std::vector<double>& ps = ms[m].ps;
         const std::vector<double>& pc = ms[m].pc;
         const std::vector<unsigned char>& bad = ms[m].bad;
-        cfloat* Sf = Sf_all + (long long)m * N * fs;
+        cfloat* Sf = spec_at(Sf_all, (long long)m * N * fs, ff);
+        float* gfm = gf ? gf + (long long)m * N : nullptr;
+        float* ifm = iff ? iff + (long long)m * N : nullptr;
         MC_BUF(sums, cfloat, "sums", (size_t)nsum * fs);
         MC_BUF(gsh, double, "gsh", 2 * N);
         MC_BUF(fA, cfloat, "fphA", (size_t)N * g.cx);
@@ -1323,7 +1349,8 @@ int align_batch(mc_ctx* c, const mc_params* const* pp_, const mc_movie* const* m
                 PhasorTabs ph{fA + (long long)f0 * g.cx, fB + (long long)f0 * H};
                 {
                     KTimer kt(c, "fft_cols_inv", sk);
-                    MC_TRY(mc_cols_inv(*pf, Sf + (long long)f0 * fs, fs, g.cx, H, &ph, Tk, pf->t_elems, nb, sk));
+                    MC_TRY(mc_cols_inv(*pf, spec_at(Sf, (long long)f0 * fs, ff), fs, g.cx, H, &ph, Tk, pf->t_elems,
+                                       nb, sk, ff, ifm ? ifm + f0 : nullptr));
                 }
                 {
                     KTimer kt(c, "fft_rows_c2r", sk);
@@ -1351,14 +1378,20 @@ int align_batch(mc_ctx* c, const mc_params* const* pp_, const mc_movie* const* m
                 const bool odd = two && ((f0 / CH) & 1);
                 hipStream_t sk = odd ? c->s2 : s;
                 cfloat* Tk = odd ? T2 : T;
+                float* dck = odd ? dcf2 : dcf;
                 {
                     KTimer kt(c, "fft_rows_r2c", sk);
-                    MC_TRY(mc_rows_r2c(*pf, Rw + f0 * rfs, W, rfs, nullptr, Tk, pf->t_elems, nb, scale, sk));
+                    MC_TRY(mc_rows_r2c_mod(*pf, Rw + f0 * rfs, W, rfs, 0, nullptr, Tk, pf->t_elems, nb, scale, sk,
+                                           ff != MC_SPEC_F32 ? dck : nullptr));
                 }
+                // a reduced format: new factors from the warped frames' row
+                // DCs (the inverse above has read this chunk's old ones)
+                if (ff != MC_SPEC_F32)
+                    MC_TRY(mc_launch_spec_scale(dck, H, nb, gfm + f0, ifm + f0, sk));
                 {
                     KTimer kt(c, "fft_cols_fwd", sk);
-                    MC_TRY(mc_cols_fwd(*pf, Tk, pf->t_elems, Sf + (long long)f0 * fs, fs, nullptr, 0, 0, 0,
-                                       nb, sk));
+                    MC_TRY(mc_cols_fwd(*pf, Tk, pf->t_elems, spec_at(Sf, (long long)f0 * fs, ff), fs, nullptr, 0,
+                                       0, 0, nb, sk, ff, gfm ? gfm + f0 : nullptr));
                 }
             }
             MC_TRY(join());
@@ -1369,7 +1402,7 @@ int align_batch(mc_ctx* c, const mc_params* const* pp_, const mc_movie* const* m
             KTimer kt(c, "dw_sums");
             MC_TRY(mc_launch_dw_sums(Sf, fs, N, g.cx, H, nullptr, nullptr, dw, p->pix_size, kvf,
                                      p->fm_dose, dws, p->sum_range[0], p->sum_range[1], p->split_sum,
-                                     sums, fs, s, dcum));
+                                     sums, fs, s, dcum, ff, ifm));
         } else {
             // Correct/CCorrectFullShift.cpp:229-309 with the absolute shifts
             MC_TRY(hipMemcpyAsync(gsh, gshift.data(), sizeof(double) * 2 * N, hipMemcpyHostToDevice, s));
@@ -1380,7 +1413,8 @@ int align_batch(mc_ctx* c, const mc_params* const* pp_, const mc_movie* const* m
             MC_TRY(mc_launch_frame_weights(Sf, fs, N, g.cx, H, gsh, p->in_fm_motion, 0, s));
             KTimer kt(c, "dw_sums");
             MC_TRY(mc_launch_dw_sums(Sf, fs, N, g.cx, H, fA, fB, dw, p->pix_size, kvf, p->fm_dose, dws,
-                                     p->sum_range[0], p->sum_range[1], p->split_sum, sums, fs, s, dcum));
+                                     p->sum_range[0], p->sum_range[1], p->split_sum, sums, fs, s, dcum,
+                                     ff, ifm));
         }
         // Correct/CCorrectFullShift.cpp:146-185: crop to -FtBin size, C2R, >= 0
         MC_BUF(To, cfloat, "To", (size_t)po->t_elems);
@@ -1419,8 +1453,8 @@ int align_batch(mc_ctx* c, const mc_params* const* pp_, const mc_movie* const* m
             for (int f0 = 0; f0 < N; f0 += SC) {
                 const int nb = std::min(SC, N - f0);
                 PhasorTabs ph{oA + (long long)f0 * g.ocx, oB + (long long)f0 * g.oh};
-                MC_TRY(mc_cols_inv(*po, Sf + (long long)f0 * fs, fs, g.cx, H, patch ? nullptr : &ph, Tst,
-                                   po->t_elems, nb, s));
+                MC_TRY(mc_cols_inv(*po, spec_at(Sf, (long long)f0 * fs, ff), fs, g.cx, H, patch ? nullptr : &ph,
+                                   Tst, po->t_elems, nb, s, ff, ifm ? ifm + f0 : nullptr));
                 float* dst = zb > 1 ? Rst : res->stack + (long long)f0 * ofs;
                 MC_TRY(mc_rows_c2r(*po, Tst, po->t_elems, dst, g.ow, ofs, nb, 0, s));
                 for (int k = 0; zb > 1 && k < nb; k++) {
@@ -1504,6 +1538,7 @@ void mc_default_params(mc_params* p) {
     p->prior = 0;
     p->prior_fallback = 0.2f;
     p->movie_pipeline = 1;
+    p->frame_fmt = 0;
     p->out_stack[1] = 1;
     p->align = 1;
 }
@@ -1669,7 +1704,7 @@ static bool same_alignment_flags(const mc_params& a, const mc_params& b) {
            a.prior_fallback == b.prior_fallback && a.verbose == b.verbose &&
            a.movie_pipeline == b.movie_pipeline && a.out_stack[0] == b.out_stack[0] &&
            a.out_stack[1] == b.out_stack[1] && a.align == b.align && a.in_fm_motion == b.in_fm_motion &&
-           a.corr_interp == b.corr_interp;
+           a.corr_interp == b.corr_interp && a.frame_fmt == b.frame_fmt;
 }
 
 int mc_align_movies(mc_ctx* c, const mc_params* params, const mc_movie* movies, int count,

@@ -33,7 +33,7 @@ class McParams(C.Structure):
         ("prior", C.c_int), ("prior_fallback", C.c_float),
         ("fm_int_start", C.POINTER(C.c_int)), ("fm_int_size", C.POINTER(C.c_int)),
         ("movie_pipeline", C.c_int), ("out_stack", C.c_int * 2), ("align", C.c_int),
-        ("in_fm_motion", C.c_int), ("corr_interp", C.c_int),
+        ("in_fm_motion", C.c_int), ("corr_interp", C.c_int), ("frame_fmt", C.c_int),
     ]
 
 
@@ -52,7 +52,7 @@ class McResult(C.Structure):
         ("patch_iters", C.POINTER(C.c_int)), ("seconds_align", C.c_double),
         ("seconds_total", C.c_double), ("spectra_fmt_used", C.c_int * 2),
         ("prior_features", C.POINTER(C.c_float)), ("prior_targets", C.POINTER(C.c_float)),
-        ("prior_fallbacks", C.c_int), ("stack", C.c_void_p),
+        ("prior_fallbacks", C.c_int), ("stack", C.c_void_p), ("frame_fmt_used", C.c_int),
     ]
 
 
@@ -609,6 +609,7 @@ class Context:
         out["seconds_total"] = res.seconds_total
         out["spectra_fmt_used"] = (res.spectra_fmt_used[0], res.spectra_fmt_used[1])
         out["prior_fallbacks"] = res.prior_fallbacks
+        out["frame_fmt_used"] = res.frame_fmt_used
         return out
 
     def align_movie(self, frames, params, replay_global=None, replay_patch=None,
