@@ -103,6 +103,9 @@ def cpu_baseline(ctx, frames, gres, params_kw, frames_needed):
     `frames_needed`): its wall time is the CPU baseline, and the GPU's shifts
     for that movie are compared with its shifts."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    # the oracle prints each stage's wall time on stderr as it finishes (a
+    # long run shows its progress)
+    os.environ.setdefault("ORC_TIMING", "1")
     import oracle as O
 
     from motioncor3_amd import lib as L

@@ -2,10 +2,38 @@
 #pragma once
 #include "mc_fft.h"
 
+// Geometry of one job of an iteration batch (device table, one entry per
+// job).  The jobs of a batch share the column length H and the search
+// window; they may differ in frame count, group count and row length, so
+// movies of different sizes align in the same launches.  Frame-indexed
+// arrays of the batch (running and final shifts, A and B phasor rows) are
+// concatenated over jobs from frame `fo`, group-indexed ones (accumulated
+// and per-iteration group shifts, windows) from group `go`; the group table
+// of the job's movie (start, size, centre) starts at `gt`, its frame table
+// (group of each frame, frame centre) at `ft`.
+struct JobGeo {
+    const cfloat* F;       // frame 0 of the job's spectra (stored format)
+    const float* finv;     // the job's per-frame load factors (reduced formats)
+    const cfloat* twW;     // e^{2 pi i k / W} for the job's row length W
+    long long ao;          // first element of the job's A rows (N x cx)
+    long long so;          // first element of the job's sum S (cx x H)
+    long long t1o;         // first element of the job's correlation rows (G x seay x cx)
+    long long sco;         // first element of the job's finalize scratch
+    double tol;            // convergence tolerance (xcf px)
+    int N, G, cx, W;
+    int fo, go, gt, ft;
+    int tile0;             // first tile of the job in the register-pass correlation
+    int binz;
+};
+
 // Device-resident state of a batch of CIterativeAlign jobs (one job for the
-// global alignment, one per patch).  Doubles, [job][...].
+// global alignment, one per patch).  Doubles, [job][...] (see JobGeo).
 struct IterState {
     int J = 0, N = 0, G = 0, max_iter = 0;
+    const JobGeo* jobs = nullptr;  // [J] device; null: J jobs of one geometry (N, G)
+    const int* fjob = nullptr;     // [frames of the batch] job of each frame
+    int NT = 0, cx_max = 0, G_max = 0, N_max = 0, W_max = 0;
+    int tiles = 0;                 // register-pass correlation tiles of all jobs
     double* tmp = nullptr;     // [J][N][2] running frame shifts
     double* total = nullptr;   // [J][G][2] accumulated group shifts
     double* gshift = nullptr;  // [J][G][2] this iteration's group shifts
@@ -32,22 +60,28 @@ hipError_t mc_launch_fourier_resize(const cfloat* in, int inx, int iny, long lon
                                     hipStream_t s);
 hipError_t mc_launch_frame_weights(cfloat* S, long long fs, int N, int cx, int H, const double* shifts,
                                    int motion, int corr_interp, hipStream_t s);
+// Running-shift phase factors of an iteration batch with a job table.
+hipError_t mc_launch_phasors_jobs(const IterState& st, const double* shifts, int H, cfloat* A, cfloat* B,
+                                  hipStream_t s);
 hipError_t mc_launch_aligned_sum(const cfloat* F, const cfloat* A, const cfloat* B,
                                  const unsigned char* done, int J, int N, int cx, int H,
                                  cfloat* S, hipStream_t s, int f_fmt = MC_SPEC_F32,
-                                 const float* f_inv = nullptr);
+                                 const float* f_inv = nullptr, const JobGeo* jobs = nullptr);
 // Whether the correlation of column length H runs the register-pass kernel
-// (the one that reads reduced spectrum formats).
+// (the one that reads reduced spectrum formats), and its columns per
+// workgroup (the tile width of JobGeo::tile0; 0 without one).
 bool mc_correlate_has_rp(int H);
+int mc_correlate_rp_cols(int H);
 hipError_t mc_launch_correlate(const FftPlan1& colplan, const cfloat* ptw, int Cc, const cfloat* F,
                                const cfloat* S, const cfloat* A, const cfloat* B,
                                const IterState& st, int cx, int H, float bft, int phase_only,
                                int seay, cfloat* T1, hipStream_t s, int f_fmt = MC_SPEC_F32,
                                const float* f_inv = nullptr);
 hipError_t mc_launch_row_dft(const cfloat* T1, const cfloat* twW, const unsigned char* done, int J,
-                             int G, int cx, int W, int seax, int seay, float* win, hipStream_t s);
+                             int G, int cx, int W, int seax, int seay, float* win, hipStream_t s,
+                             const JobGeo* jobs = nullptr);
 hipError_t mc_launch_peaks(const float* win, const unsigned char* done, int J, int G, int nx,
-                           int ny, double* gshift, hipStream_t s);
+                           int ny, double* gshift, hipStream_t s, const JobGeo* jobs = nullptr);
 hipError_t mc_launch_update(const IterState& st, int it, double tol, hipStream_t s);
 hipError_t mc_launch_finalize(const IterState& st, int binz, int quirks, hipStream_t s);
 hipError_t mc_launch_feature_map(const float* img128, unsigned char* feat, hipStream_t s);

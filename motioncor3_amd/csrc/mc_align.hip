@@ -44,6 +44,33 @@ __global__ void k_phasors(const double* shifts, double mult, int skip, const uns
     }
 }
 
+// The same phase factors for the running shifts of an iteration batch whose
+// jobs differ in size: frame b of the batch belongs to job fjob[b] (its row
+// length and A rows from the job table), factors e^{-i(...)} (mult -1), no
+// skip rule.  Grid x covers the widest job's cx + H.
+__global__ void k_phasors_jobs(const double* shifts, const unsigned char* done, const JobGeo* jobs,
+                               const int* fjob, int H, cfloat* A, cfloat* B) {
+    const int b = blockIdx.y;
+    const int j = fjob[b];
+    if (done && done[j]) return;
+    const JobGeo& g = jobs[j];
+    const int cx = g.cx;
+    const double sx = -shifts[2 * b], sy = -shifts[2 * b + 1];
+    const double fsx = sx * (2.0 * MC_PI / g.W), fsy = sy * (2.0 * MC_PI / H);
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e < cx) {
+        double sn, cs;
+        sincos(e * fsx, &sn, &cs);
+        A[g.ao + (long long)(b - g.fo) * cx + e] = cmk((float)cs, (float)sn);
+    } else if (e < cx + H) {
+        const int y = e - cx;
+        const int ys = (y > H / 2) ? y - H : y;
+        double sn, cs;
+        sincos(ys * fsy, &sn, &cs);
+        B[(long long)b * H + y] = cmk((float)cs, (float)sn);
+    }
+}
+
 // GFourierResize2D (Util/GFourierResize2D.cu:7-39) on [kx][y] half spectra:
 // output row yo reads input row yo (yo <= outy/2) or yo - outy + iny (the
 // negative frequencies), when that row exists in the input; columns beyond
@@ -157,26 +184,43 @@ __global__ __launch_bounds__(256) void k_frame_weights(cfloat* S, long long fs, 
 // CAlignedSum (Align/CAlignedSum.cpp:25-126): S = sum_i shift(F_i). Even and
 // odd frames accumulate separately and are added at the end, as the
 // reference's two streams do (:99-112).
+// With a job table the jobs may differ in frame count and row length
+// (grid y covers the widest); without one they are J jobs of N frames.
 template <int FF>
 __global__ void k_aligned_sum(const cfloat* __restrict__ F, const float* __restrict__ finv,
                               const cfloat* __restrict__ A, const cfloat* __restrict__ B,
-                              const unsigned char* done, int N, int cx, int H, cfloat* S) {
+                              const unsigned char* done, int N, int cx, int H, cfloat* S,
+                              const JobGeo* __restrict__ jobs) {
     const int j = blockIdx.z;
     if (done && done[j]) return;
+    long long ao = (long long)j * N * cx, bo = (long long)j * N * H, so = (long long)j * cx * H;
+    if (jobs) {
+        const JobGeo& g = jobs[j];
+        F = g.F;
+        finv = g.finv;
+        N = g.N;
+        cx = g.cx;
+        ao = g.ao;
+        bo = (long long)g.fo * H;
+        so = g.so;
+    } else {
+        F = reinterpret_cast<const cfloat*>(reinterpret_cast<const char*>(F) +
+                                            (long long)j * N * cx * H * Spec<FF>::bytes);
+        if (finv) finv += (long long)j * N;
+    }
     const long long fs = (long long)cx * H;
     const int kx = blockIdx.y;
     const int y = blockIdx.x * blockDim.x + threadIdx.x;
-    if (y >= H) return;
+    if (y >= H || kx >= cx) return;
     const long long q = (long long)kx * H + y;
     cfloat a0 = cmk(0.f, 0.f), a1 = cmk(0.f, 0.f);
     for (int i = 0; i < N; i++) {
-        const long long b = (long long)j * N + i;
-        const cfloat f = Spec<FF>::ld(F, b * fs + q, FF == MC_SPEC_F32 ? 1.f : finv[b]);
-        const cfloat v = cmul(f, cmul(A[b * cx + kx], B[b * H + y]));
+        const cfloat f = Spec<FF>::ld(F, i * fs + q, FF == MC_SPEC_F32 ? 1.f : finv[i]);
+        const cfloat v = cmul(f, cmul(A[ao + (long long)i * cx + kx], B[bo + (long long)i * H + y]));
         if (i & 1) a1 = cadd(a1, v);
         else a0 = cadd(a0, v);
     }
-    S[(long long)j * fs + q] = cadd(a0, a1);
+    S[so + q] = cadd(a0, a1);
 }
 
 // GCorrelateSum2D for one column block of one group of one job
@@ -200,46 +244,102 @@ struct CorrArgs {
     int phase_only, seay;
     cfloat* T1;
     const float* finv;   // load factors of a reduced F format (per transform)
+    const JobGeo* jobs;  // job table (jobs of different sizes), or null
+    int J;
+    float nbq;           // -0.25 B: a job's filter constant is nbq / ((cx - 1) H)
 };
 
+// One job's view of the correlation arguments: its frames, phasor rows,
+// sum, output rows and group table.
+struct CorrJob {
+    const cfloat* F;
+    const float* finv;
+    const cfloat* A;
+    const cfloat* B;
+    const cfloat* S;
+    cfloat* T1;
+    const int* gstart;
+    const int* gsize;
+    int N, G, cx;
+    float filt0;
+};
+
+__device__ __forceinline__ CorrJob corr_job(const CorrArgs& a, int j, int H, int fbytes) {
+    CorrJob r;
+    if (a.jobs) {
+        const JobGeo& g = a.jobs[j];
+        r.F = g.F;
+        r.finv = g.finv;
+        r.A = a.A + g.ao;
+        r.B = a.B + (long long)g.fo * H;
+        r.S = a.S + g.so;
+        r.T1 = a.T1 + g.t1o;
+        r.gstart = a.gstart + g.gt;
+        r.gsize = a.gsize + g.gt;
+        r.N = g.N;
+        r.G = g.G;
+        r.cx = g.cx;
+        r.filt0 = a.nbq / (float)((g.cx - 1) * H);
+    } else {
+        const long long fs = (long long)a.cx * H;
+        r.F = reinterpret_cast<const cfloat*>(reinterpret_cast<const char*>(a.F) +
+                                              (long long)j * a.N * fs * fbytes);
+        r.finv = a.finv ? a.finv + (long long)j * a.N : nullptr;
+        r.A = a.A + (long long)j * a.N * a.cx;
+        r.B = a.B + (long long)j * a.N * H;
+        r.S = a.S + (long long)j * fs;
+        r.T1 = a.T1 + (long long)j * a.G * a.seay * a.cx;
+        r.gstart = a.gstart;
+        r.gsize = a.gsize;
+        r.N = a.N;
+        r.G = a.G;
+        r.cx = a.cx;
+        r.filt0 = a.filt0;
+    }
+    return r;
+}
+
+// Grid: (column blocks of the widest job, groups of the job with most, jobs).
 __global__ __launch_bounds__(MC_NTHR) void k_correlate(CorrArgs a) {
     extern __shared__ cfloat lds[];
     const int j = blockIdx.z, g = blockIdx.y;
     if (a.done && a.done[j]) return;
+    const int H = a.H;
+    const CorrJob r = corr_job(a, j, H, 8);
     const int c0 = blockIdx.x * a.Cc;
-    const int nc = min(a.Cc, a.cx - c0);
-    const int gs = a.gstart[g], gn = a.gsize[g];
-    const long long fs = (long long)a.cx * a.H;
-    const cfloat* S = a.S + (long long)j * fs;
-    const float rH = 1.0f / a.H;
-    for (int e = threadIdx.x; e < nc * a.H; e += MC_NTHR) {
-        const int c = fdiv(e, a.H, rH);
-        const int y = e - c * a.H;
+    if (g >= r.G || c0 >= r.cx) return;
+    const int nc = min(a.Cc, r.cx - c0);
+    const int gs = r.gstart[g], gn = r.gsize[g];
+    const long long fs = (long long)r.cx * H;
+    const float rH = 1.0f / H;
+    for (int e = threadIdx.x; e < nc * H; e += MC_NTHR) {
+        const int c = fdiv(e, H, rH);
+        const int y = e - c * H;
         const int kx = c0 + c;
-        const long long q = (long long)kx * a.H + y;
+        const long long q = (long long)kx * H + y;
         cfloat G = cmk(0.f, 0.f);
         for (int t = 0; t < gn; t++) {
-            const long long b = (long long)j * a.N + gs + t;
-            G = cadd(G, cmul(a.F[b * fs + q], cmul(a.A[b * a.cx + kx], a.B[b * a.H + y])));
+            const long long b = gs + t;
+            G = cadd(G, cmul(r.F[b * fs + q], cmul(r.A[b * r.cx + kx], r.B[b * H + y])));
         }
-        const cfloat sd = csub(S[q], G);
+        const cfloat sd = csub(r.S[q], G);
         cfloat X = cmk(G.x * sd.x + G.y * sd.y, G.x * sd.y - G.y * sd.x);
-        const int ys = (y > a.H / 2) ? y - a.H : y;
-        float f = expf(a.filt0 * (float)(kx * kx + ys * ys));
+        const int ys = (y > H / 2) ? y - H : y;
+        float f = expf(r.filt0 * (float)(kx * kx + ys * ys));
         if (a.phase_only) f = f / (sqrtf(X.x * X.x + X.y * X.y) + 1e-20f);
         X = (kx == 0 && y == 0) ? cmk(0.f, 0.f) : cscale(X, f);
         lds[lp(e)] = cconj(X);
     }
     __syncthreads();
     fft_lds_fwd(lds, nc, a.plan);
-    cfloat* out = a.T1 + ((long long)j * a.G + g) * a.seay * a.cx;
+    cfloat* out = r.T1 + (long long)g * a.seay * r.cx;
     const float rnc = 1.0f / nc;
     for (int e = threadIdx.x; e < a.seay * nc; e += MC_NTHR) {
         const int jy = fdiv(e, nc, rnc);
         const int c = e - jy * nc;
         int lag = jy - a.seay / 2;
-        if (lag < 0) lag += a.H;
-        out[(long long)jy * a.cx + c0 + c] = cconj(lds[lp(c * a.H + lag)]);
+        if (lag < 0) lag += H;
+        out[(long long)jy * r.cx + c0 + c] = cconj(lds[lp(c * H + lag)]);
     }
 }
 
@@ -253,13 +353,35 @@ __global__ __launch_bounds__(NT) void k_correlate_rp(CorrArgs a, const cfloat* _
     constexpr int RL = rp_radix(H, NP - 1), ML = H / RL;
     constexpr int IT0 = (C * M0 + NT - 1) / NT, ITL = (C * ML + NT - 1) / NT;
     extern __shared__ cfloat lds[];
-    const int j = blockIdx.z, g = blockIdx.y;
+    // one-dimensional grid over (job, column block, group), group fastest,
+    // dealt to the XCDs in contiguous ranges: the workgroups of one column
+    // block run on one XCD one after another, so its columns of the sum S
+    // (and the phasor tables) come from that XCD's L2 after the first group
+    const int tile = xcd_tile(blockIdx.x, gridDim.x);
+    int j, r0;
+    if (a.jobs) {
+        // the job whose tile range holds this tile (tile0 ascending)
+        int lo = 0, hi = a.J - 1;
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (a.jobs[mid].tile0 <= tile) lo = mid;
+            else hi = mid - 1;
+        }
+        j = lo;
+        r0 = tile - a.jobs[j].tile0;
+    } else {
+        const int per_job = ((a.cx + C - 1) / C) * a.G;
+        j = tile / per_job;
+        r0 = tile - j * per_job;
+    }
     if (a.done && a.done[j]) return;
-    const int c0 = blockIdx.x * C;
-    const int nc = min(C, a.cx - c0);
-    const int gs = a.gstart[g], gn = a.gsize[g];
-    const long long fs = (long long)a.cx * H;
-    const cfloat* S = a.S + (long long)j * fs;
+    const CorrJob r = corr_job(a, j, H, Spec<FF>::bytes);
+    const int cb = r0 / r.G;
+    const int g = r0 - cb * r.G;
+    const int c0 = cb * C;
+    const int nc = min(C, r.cx - c0);
+    const int gs = r.gstart[g], gn = r.gsize[g];
+    const long long fs = (long long)r.cx * H;
 #pragma unroll
     for (int it = 0; it < IT0; it++) {
         const int i = threadIdx.x + it * NT;
@@ -271,17 +393,17 @@ __global__ __launch_bounds__(NT) void k_correlate_rp(CorrArgs a, const cfloat* _
             for (int t = 0; t < R0; t++) v[t] = cmk(0.f, 0.f);
             if (c < nc) {
                 for (int f = 0; f < gn; f++) {
-                    const long long b = (long long)j * a.N + gs + f;
+                    const long long b = gs + f;
                     const long long fo = b * fs + (long long)kx * H + jj;
-                    const float fi = FF == MC_SPEC_F32 ? 1.f : a.finv[b];
-                    const cfloat* Bb = a.B + b * H + jj;
-                    const cfloat pa = a.A[b * a.cx + kx];
+                    const float fi = FF == MC_SPEC_F32 ? 1.f : r.finv[b];
+                    const cfloat* Bb = r.B + b * H + jj;
+                    const cfloat pa = r.A[b * r.cx + kx];
 #pragma unroll
                     for (int t = 0; t < R0; t++)
-                        v[t] = cadd(v[t], cmul(Spec<FF>::ld(a.F, fo + t * M0, fi),
+                        v[t] = cadd(v[t], cmul(Spec<FF>::ld(r.F, fo + t * M0, fi),
                                                cmul(pa, Bb[t * M0])));
                 }
-                const cfloat* Sc = S + (long long)kx * H + jj;
+                const cfloat* Sc = r.S + (long long)kx * H + jj;
 #pragma unroll
                 for (int t = 0; t < R0; t++) {
                     const int y = jj + t * M0;
@@ -289,7 +411,7 @@ __global__ __launch_bounds__(NT) void k_correlate_rp(CorrArgs a, const cfloat* _
                     const cfloat sd = csub(Sc[t * M0], G);
                     cfloat X = cmk(G.x * sd.x + G.y * sd.y, G.x * sd.y - G.y * sd.x);
                     const int ys = (y > H / 2) ? y - H : y;
-                    float fl = expf(a.filt0 * (float)(kx * kx + ys * ys));
+                    float fl = expf(r.filt0 * (float)(kx * kx + ys * ys));
                     if (a.phase_only) fl = fl / (sqrtf(X.x * X.x + X.y * X.y) + 1e-20f);
                     X = (kx == 0 && y == 0) ? cmk(0.f, 0.f) : cscale(X, fl);
                     v[t] = cconj(X);
@@ -303,7 +425,7 @@ __global__ __launch_bounds__(NT) void k_correlate_rp(CorrArgs a, const cfloat* _
     }
     __syncthreads();
     RpMid<H, 1, C, NT>::run(lds, ptw);
-    cfloat* out = a.T1 + ((long long)j * a.G + g) * a.seay * a.cx;
+    cfloat* out = r.T1 + (long long)g * a.seay * r.cx;
     const int half = a.seay / 2;
 #pragma unroll
     for (int it = 0; it < ITL; it++) {
@@ -328,7 +450,7 @@ __global__ __launch_bounds__(NT) void k_correlate_rp(CorrArgs a, const cfloat* _
                 for (int t = 0; t < RL; t++) {
                     const int y = k + t * ML;
                     const int jy = (y < H / 2 ? y : y - H) + half;
-                    if (jy >= 0 && jy < a.seay) out[(long long)jy * a.cx + c0 + c] = cconj(v[t]);
+                    if (jy >= 0 && jy < a.seay) out[(long long)jy * r.cx + c0 + c] = cconj(v[t]);
                 }
             }
         }
@@ -341,16 +463,29 @@ __global__ __launch_bounds__(NT) void k_correlate_rp(CorrArgs a, const cfloat* _
 
 // Second half of the windowed C2R: for one lag row, the Hermitian sum over
 // kx at sea_x lags, c(l) = Re T0 + (-1)^l Re T_M + 2 sum_{0<k<M} Re(T_k e^{2 pi i k l/W}).
+// With a job table, grid y covers the job with most groups and each job
+// reads its own row length and twiddles (LDS sized for the widest).
 __global__ __launch_bounds__(256) void k_row_dft(const cfloat* T1, const cfloat* twW,
                                                  const unsigned char* done, int G, int cx, int W,
-                                                 int seax, int seay, float* win) {
+                                                 int seax, int seay, float* win,
+                                                 const JobGeo* __restrict__ jobs) {
     extern __shared__ cfloat lds[];
     const int jy = blockIdx.x, g = blockIdx.y, j = blockIdx.z;
     if (done && done[j]) return;
+    long long t1o = ((long long)j * G + g) * seay * cx, wo = (long long)j * G + g;
+    if (jobs) {
+        const JobGeo& jg = jobs[j];
+        if (g >= jg.G) return;
+        cx = jg.cx;
+        W = jg.W;
+        twW = jg.twW;
+        t1o = jg.t1o + (long long)g * seay * cx;
+        wo = jg.go + g;
+    }
     cfloat* row = lds;
     cfloat* tw = lds + cx;
     float* red = reinterpret_cast<float*>(tw + W);
-    const cfloat* src = T1 + (((long long)j * G + g) * seay + jy) * cx;
+    const cfloat* src = T1 + t1o + (long long)jy * cx;
     for (int e = threadIdx.x; e < cx; e += blockDim.x) row[e] = src[e];
     for (int e = threadIdx.x; e < W; e += blockDim.x) tw[e] = twW[e];
     __syncthreads();
@@ -379,7 +514,7 @@ __global__ __launch_bounds__(256) void k_row_dft(const cfloat* T1, const cfloat*
     if (threadIdx.x < 64 && jx < seax) {
         const float v = red[threadIdx.x] + red[threadIdx.x + 64] + red[threadIdx.x + 128] +
                         red[threadIdx.x + 192];
-        win[(((long long)j * G + g) * seay + jy) * seax + jx] = v;
+        win[(wo * seay + jy) * seax + jx] = v;
     }
 }
 
@@ -436,10 +571,16 @@ __device__ void peak_fit(const float* img, int nx, int ny, double* out) {
 }
 
 __global__ __launch_bounds__(256) void k_peaks(const float* win, const unsigned char* done, int G,
-                                               int nx, int ny, double* gshift) {
+                                               int nx, int ny, double* gshift,
+                                               const JobGeo* __restrict__ jobs) {
     const int g = blockIdx.x, j = blockIdx.y;
     if (done && done[j]) return;
-    peak_fit(win + ((long long)j * G + g) * nx * ny, nx, ny, gshift + ((long long)j * G + g) * 2);
+    long long o = (long long)j * G + g;
+    if (jobs) {
+        if (g >= jobs[j].G) return;
+        o = jobs[j].go + g;
+    }
+    peak_fit(win + o * nx * ny, nx, ny, gshift + o * 2);
 }
 
 __global__ __launch_bounds__(256) void k_peaks_f(const float* win, int nx, int ny, double* out) {
@@ -462,27 +603,41 @@ struct UpdArgs {
     const int* fgroup;
     int N, G, max_iter, it;
     double tol;
+    const JobGeo* jobs;  // per-job frame and group counts, offsets, tolerance (or null)
 };
 
 __global__ void k_update(UpdArgs a) {
     const int j = blockIdx.x;
     if (a.done[j] || threadIdx.x != 0) return;
-    const double* gs = a.gshift + (long long)j * a.G * 2;
+    int N = a.N, G = a.G;
+    long long fo = (long long)j * N, go = (long long)j * G;
+    const int* fgroup = a.fgroup;
+    double tol = a.tol;
+    if (a.jobs) {
+        const JobGeo& jg = a.jobs[j];
+        N = jg.N;
+        G = jg.G;
+        fo = jg.fo;
+        go = jg.go;
+        fgroup += jg.ft;
+        tol = jg.tol;
+    }
+    const double* gs = a.gshift + go * 2;
     double err = 0.0;
-    for (int g = 0; g < a.G; g++) {
+    for (int g = 0; g < G; g++) {
         const double e = sqrt(gs[2 * g] * gs[2 * g] + gs[2 * g + 1] * gs[2 * g + 1]);
         if (err < e) err = e;
-        a.total[((long long)j * a.G + g) * 2] += gs[2 * g];
-        a.total[((long long)j * a.G + g) * 2 + 1] += gs[2 * g + 1];
+        a.total[(go + g) * 2] += gs[2 * g];
+        a.total[(go + g) * 2 + 1] += gs[2 * g + 1];
     }
-    for (int i = 0; i < a.N; i++) {
-        const int g = a.fgroup[i];
-        a.tmp[((long long)j * a.N + i) * 2] += gs[2 * g];
-        a.tmp[((long long)j * a.N + i) * 2 + 1] += gs[2 * g + 1];
+    for (int i = 0; i < N; i++) {
+        const int g = fgroup[i];
+        a.tmp[(fo + i) * 2] += gs[2 * g];
+        a.tmp[(fo + i) * 2 + 1] += gs[2 * g + 1];
     }
     a.errors[(long long)j * a.max_iter + a.it] = err;
     a.iters[j] = a.it + 1;
-    if (err < a.tol && a.it > 0) a.done[j] = 1;
+    if (err < tol && a.it > 0) a.done[j] = 1;
 }
 
 // ---------------------------------------------------------------------------
@@ -739,6 +894,7 @@ struct FinArgs {
     int N, G, binz, quirks;
     long long sstride;
     int use_lds;   // work in LDS (sstride doubles) instead of the global scratch
+    const JobGeo* jobs;  // per-job counts, offsets and group bin (or null)
 };
 
 // End of CIterativeAlign (Align/CIterativeAlign.cpp:63-85,146-149): linear
@@ -748,39 +904,53 @@ struct FinArgs {
 __global__ __launch_bounds__(MC_TL) void k_finalize(FinArgs a) {
     extern __shared__ double fsh[];
     const int j = blockIdx.x, l = threadIdx.x;
-    const int N = a.N, G = a.G;
-    double* sc = a.use_lds ? fsh : a.scratch + (long long)j * a.sstride;
+    int N = a.N, G = a.G, binz = a.binz;
+    long long fo = (long long)j * N, go = (long long)j * G, sco = (long long)j * a.sstride;
+    const double* gcent = a.gcent;
+    const double* fcent = a.fcent;
+    if (a.jobs) {
+        const JobGeo& jg = a.jobs[j];
+        N = jg.N;
+        G = jg.G;
+        binz = jg.binz;
+        fo = jg.fo;
+        go = jg.go;
+        sco = jg.sco;
+        gcent += jg.gt;
+        fcent += jg.ft;
+    }
+    double* sc = a.use_lds ? fsh : a.scratch + sco;
     double* sx = sc;
     double* sy = sc + N;
     double* work = sc + 2 * N;
-    const double* tot = a.total + (long long)j * G * 2;
+    const double* tot = a.total + go * 2;
     for (int i = l; i < N; i += MC_TL) {
         if (N == G) {
             sx[i] = tot[2 * i];
             sy[i] = tot[2 * i + 1];
             continue;
         }
-        const double fc = a.fcent[i];
+        const double fc = fcent[i];
         int k;
-        if (fc <= a.gcent[0]) k = 1;
+        if (fc <= gcent[0]) k = 1;
         else {
             k = G - 1;
             for (int q = 1; q < G; q++)
-                if (fc < a.gcent[q]) { k = q; break; }
+                if (fc < gcent[q]) { k = q; break; }
         }
         const int f = k - 1;
-        const double dg = a.gcent[k] - a.gcent[f];
-        const double df = fc - a.gcent[f];
+        const double dg = gcent[k] - gcent[f];
+        const double df = fc - gcent[f];
         sx[i] = tot[2 * f] + (tot[2 * k] - tot[2 * f]) / dg * df;
         sy[i] = tot[2 * f + 1] + (tot[2 * k + 1] - tot[2 * f + 1]) / dg * df;
     }
     __syncthreads();
     w_remove_spikes(sx, sy, N, a.quirks, work);
     w_smooth(sx, sy, N, 0.5, work);
-    double w = 0.3 + 0.1 * a.binz;
+    double w = 0.3 + 0.1 * binz;
     if (w > 0.9) w = 0.9;
     w_smooth(sx, sy, N, w, work);
-    double* out = a.final_ + (long long)j * N * 2;
+    double* out = a.final_ + fo * 2;
     for (int i = l; i < N; i += MC_TL) {
         out[2 * i] = sx[i];
         out[2 * i + 1] = sy[i];
@@ -1415,6 +1585,14 @@ hipError_t mc_launch_phasors(const double* shifts, double mult, int skip, const 
     return hipGetLastError();
 }
 
+hipError_t mc_launch_phasors_jobs(const IterState& st, const double* shifts, int H, cfloat* A, cfloat* B,
+                                  hipStream_t s) {
+    if (!st.jobs || !st.fjob || st.NT < 1) return hipErrorInvalidValue;
+    dim3 grid(mc_div_up(st.cx_max + H, 256), st.NT);
+    hipLaunchKernelGGL(k_phasors_jobs, grid, dim3(256), 0, s, shifts, st.done, st.jobs, st.fjob, H, A, B);
+    return hipGetLastError();
+}
+
 hipError_t mc_launch_frame_weights(cfloat* S, long long fs, int N, int cx, int H, const double* shifts,
                                    int motion, int corr_interp, hipStream_t s) {
     if (N < 2 || cx < 2 || H < 1 || (motion && !shifts)) return hipErrorInvalidValue;
@@ -1426,20 +1604,22 @@ hipError_t mc_launch_frame_weights(cfloat* S, long long fs, int N, int cx, int H
 
 hipError_t mc_launch_aligned_sum(const cfloat* F, const cfloat* A, const cfloat* B,
                                  const unsigned char* done, int J, int N, int cx, int H,
-                                 cfloat* S, hipStream_t s, int f_fmt, const float* f_inv) {
+                                 cfloat* S, hipStream_t s, int f_fmt, const float* f_inv,
+                                 const JobGeo* jobs) {
+    // with a job table, cx is the widest job's
     dim3 grid(mc_div_up(H, 256), cx, J);
     switch (f_fmt) {
         case MC_SPEC_F32:
             hipLaunchKernelGGL(k_aligned_sum<MC_SPEC_F32>, grid, dim3(256), 0, s, F, f_inv, A, B,
-                               done, N, cx, H, S);
+                               done, N, cx, H, S, jobs);
             break;
         case MC_SPEC_F16:
             hipLaunchKernelGGL(k_aligned_sum<MC_SPEC_F16>, grid, dim3(256), 0, s, F, f_inv, A, B,
-                               done, N, cx, H, S);
+                               done, N, cx, H, S, jobs);
             break;
         case MC_SPEC_BF16:
             hipLaunchKernelGGL(k_aligned_sum<MC_SPEC_BF16>, grid, dim3(256), 0, s, F, f_inv, A, B,
-                               done, N, cx, H, S);
+                               done, N, cx, H, S, jobs);
             break;
         default: return hipErrorInvalidValue;
     }
@@ -1470,9 +1650,16 @@ hipError_t mc_launch_correlate(const FftPlan1& colplan, const cfloat* ptw, int C
     a.seay = seay;
     a.T1 = T1;
     a.finv = f_inv;
+    a.jobs = st.jobs;
+    a.J = st.J;
+    a.nbq = -0.25f * bft;
+    // a job table brings its tile count (st.tiles, for mc_correlate_rp_cols(H))
+    // and its widest job (st.cx_max, st.G_max)
+    if (st.jobs && (st.cx_max < 1 || st.G_max < 1)) return hipErrorInvalidValue;
 #define X(HH, CC, NT)                                                                     \
     if (H == HH && ptw && seay <= HH) {                                                  \
-        const dim3 grid(mc_div_up(cx, CC), st.G, st.J);                                  \
+        if (st.jobs && st.tiles < 1) return hipErrorInvalidValue;                        \
+        const dim3 grid(st.jobs ? st.tiles : mc_div_up(cx, CC) * st.G * st.J);           \
         const size_t lds = (size_t)CC * HH * sizeof(cfloat);                             \
         if (f_fmt == MC_SPEC_F32)                                                        \
             hipLaunchKernelGGL((k_correlate_rp<HH, CC, NT, 0>), grid, dim3(NT), lds, s, a, ptw); \
@@ -1490,7 +1677,7 @@ hipError_t mc_launch_correlate(const FftPlan1& colplan, const cfloat* ptw, int C
     while (Cc > 1 && (long long)Cc * H > MC_LDS_ELEMS) Cc /= 2;
     if ((long long)Cc * H > MC_LDS_ELEMS) return hipErrorInvalidValue;
     a.Cc = Cc;
-    dim3 grid(mc_div_up(cx, Cc), st.G, st.J);
+    dim3 grid(mc_div_up(st.jobs ? st.cx_max : cx, Cc), st.jobs ? st.G_max : st.G, st.J);
     const size_t lds = (size_t)mc_lds_padded(Cc * H) * sizeof(cfloat);
     hipLaunchKernelGGL(k_correlate, grid, dim3(MC_NTHR), lds, s, a);
     return hipGetLastError();
@@ -1504,17 +1691,28 @@ bool mc_correlate_has_rp(int H) {
     return false;
 }
 
+int mc_correlate_rp_cols(int H) {
+#define X(HH, CC, NT) \
+    if (H == HH) return CC;
+    MC_RP_CORR_SPECS(X)
+#undef X
+    return 0;
+}
+
 hipError_t mc_launch_row_dft(const cfloat* T1, const cfloat* twW, const unsigned char* done, int J,
-                             int G, int cx, int W, int seax, int seay, float* win, hipStream_t s) {
+                             int G, int cx, int W, int seax, int seay, float* win, hipStream_t s,
+                             const JobGeo* jobs) {
+    // with a job table: G, cx and W are the largest over the jobs
     dim3 grid(seay, G, J);
     const size_t lds = (size_t)(cx + W) * sizeof(cfloat) + 256 * sizeof(float);
-    hipLaunchKernelGGL(k_row_dft, grid, dim3(256), lds, s, T1, twW, done, G, cx, W, seax, seay, win);
+    hipLaunchKernelGGL(k_row_dft, grid, dim3(256), lds, s, T1, twW, done, G, cx, W, seax, seay, win,
+                       jobs);
     return hipGetLastError();
 }
 
 hipError_t mc_launch_peaks(const float* win, const unsigned char* done, int J, int G, int nx,
-                           int ny, double* gshift, hipStream_t s) {
-    hipLaunchKernelGGL(k_peaks, dim3(G, J), dim3(256), 0, s, win, done, G, nx, ny, gshift);
+                           int ny, double* gshift, hipStream_t s, const JobGeo* jobs) {
+    hipLaunchKernelGGL(k_peaks, dim3(G, J), dim3(256), 0, s, win, done, G, nx, ny, gshift, jobs);
     return hipGetLastError();
 }
 
@@ -1538,6 +1736,7 @@ hipError_t mc_launch_update(const IterState& st, int it, double tol, hipStream_t
     a.max_iter = st.max_iter;
     a.it = it;
     a.tol = tol;
+    a.jobs = st.jobs;
     hipLaunchKernelGGL(k_update, dim3(st.J), dim3(64), 0, s, a);
     return hipGetLastError();
 }
@@ -1554,7 +1753,9 @@ hipError_t mc_launch_finalize(const IterState& st, int binz, int quirks, hipStre
     a.G = st.G;
     a.binz = binz;
     a.quirks = quirks;
-    a.sstride = 16LL * st.N + 128;
+    a.jobs = st.jobs;
+    // with a job table, sized for the job with most frames
+    a.sstride = 16LL * (st.jobs ? st.N_max : st.N) + 128;
     const size_t bytes = (size_t)a.sstride * sizeof(double);
     a.use_lds = bytes <= 65536;
     hipLaunchKernelGGL(k_finalize, dim3(st.J), dim3(64), a.use_lds ? bytes : 0, s, a);

@@ -11,11 +11,13 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <condition_variable>
 #include <cstdio>
 #include <cstring>
 #include <map>
 #include <mutex>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/mc_abi.h"
@@ -147,8 +149,13 @@ int fail(mc_ctx* c, int code, const std::string& msg) {
     return code;
 }
 
+// Buffer names of the calling host thread carry this prefix: the threads
+// of one mc_align_movies call that align movies of different sizes side by
+// side (IterMeet) keep their stacks apart.  Empty on every other thread.
+thread_local std::string tl_buf_prefix;
+
 void* buf(mc_ctx* c, const char* name, size_t bytes) {
-    DevBuf& b = c->bufs[name];
+    DevBuf& b = tl_buf_prefix.empty() ? c->bufs[name] : c->bufs[tl_buf_prefix + name];
     if (b.bytes >= bytes && b.p) return b.p;
     if (b.p) (void)hipFree(b.p);
     b.p = nullptr;
@@ -420,57 +427,150 @@ static cfloat* spec_at(void* base, long long elems, int fmt) {
     return (cfloat*)((char*)base + elems * (fmt == MC_SPEC_F32 ? 8 : 4));
 }
 
-int run_iterative(mc_ctx* c, const char* tag, const cfloat* F, int J, int N, int W, int H,
-                  Fft2D* pl, const Groups& grp, float bft, int phase_only, int max_iter,
-                  double tol, int quirks, IterOut* out, int f_fmt = MC_SPEC_F32,
-                  const float* f_inv = nullptr, bool graphs = false, PriorRun* pr = nullptr) {
-    const int cx = W / 2 + 1, G = grp.G;
-    // linear group->frame interpolation needs two groups
-    // (Align/CInterpolateShift.cpp:129-142 reads group 1)
-    if (G < 2 && G != N) return fail(c, MC_E_ARG, "-Group leaves fewer than two frame groups");
-    const int seax = std::min(64, W), seay = std::min(64, H);
+// One class of jobs of an iteration batch: J jobs of one movie geometry
+// (N frames of row length W), their spectra at F [job][frame] in the
+// batch's stored format, the movie's frame groups and its tolerance.
+struct IterClass {
+    const cfloat* F = nullptr;
+    const float* f_inv = nullptr;  // [job][frame] load factors of a reduced format, or null
+    int J = 0, N = 0, W = 0;
+    const Groups* grp = nullptr;
+    double tol = 0.0;               // xcf px (-Tol over the movie's xcf bin)
+};
+
+// CIterativeAlign for all jobs of all classes at once: every iteration stage
+// is one launch over every job (the classes share the column length H; each
+// job reads its own geometry from the job table).  A job's arithmetic does
+// not depend on the other jobs, so results equal separate runs.  Outputs are
+// concatenated over the classes in order.
+int run_iterative_classes(mc_ctx* c, const char* tag, const IterClass* cls, int ncls, int H, Fft2D* pl,
+                          float bft, int phase_only, int max_iter, int quirks, IterOut* out,
+                          int f_fmt = MC_SPEC_F32, bool graphs = false, PriorRun* pr = nullptr) {
+    if (ncls < 1) return fail(c, MC_E_ARG, "no iteration jobs");
+    if (pr && ncls != 1) return fail(c, MC_E_ARG, "the learned prior needs movies of one geometry");
+    const int seay = std::min(64, H);
+    const int seax = std::min(64, cls[0].W);
+    const int C = mc_correlate_rp_cols(H);
+    const size_t fb = f_fmt == MC_SPEC_F32 ? 8 : 4;
+    std::vector<JobGeo> jg;
+    std::vector<int> fjob, gstart, gsize, fgroup;
+    std::vector<double> gcent, fcent;
+    long long ao = 0, so = 0, t1o = 0, sco = 0;
+    int fo = 0, go = 0, tiles = 0;
+    int cx_max = 0, G_max = 0, N_max = 0, W_max = 0;
+    std::string sig;
+    for (int k = 0; k < ncls; k++) {
+        const IterClass& q = cls[k];
+        const Groups& grp = *q.grp;
+        const int cx = q.W / 2 + 1, G = grp.G, N = q.N;
+        // linear group->frame interpolation needs two groups
+        // (Align/CInterpolateShift.cpp:129-142 reads group 1)
+        if (G < 2 && G != N) return fail(c, MC_E_ARG, "-Group leaves fewer than two frame groups");
+        if (std::min(64, q.W) != seax) return fail(c, MC_E_ARG, "batched jobs differ in search window");
+        cfloat* twW = row_twiddles(c, q.W);
+        if (!twW) return fail(c, MC_E_NOMEM, "twiddle table");
+        const int gt = (int)gstart.size(), ft = (int)fgroup.size();
+        gstart.insert(gstart.end(), grp.start.begin(), grp.start.end());
+        gsize.insert(gsize.end(), grp.size.begin(), grp.size.end());
+        gcent.insert(gcent.end(), grp.gcent.begin(), grp.gcent.end());
+        fgroup.insert(fgroup.end(), grp.fgroup.begin(), grp.fgroup.end());
+        fcent.insert(fcent.end(), grp.fcent.begin(), grp.fcent.end());
+        for (int j = 0; j < q.J; j++) {
+            JobGeo e;
+            e.F = (const cfloat*)((const char*)q.F + (size_t)j * N * cx * H * fb);
+            e.finv = q.f_inv ? q.f_inv + (size_t)j * N : nullptr;
+            e.twW = twW;
+            e.ao = ao;
+            e.so = so;
+            e.t1o = t1o;
+            e.sco = sco;
+            e.tol = q.tol;
+            e.N = N;
+            e.G = G;
+            e.cx = cx;
+            e.W = q.W;
+            e.fo = fo;
+            e.go = go;
+            e.gt = gt;
+            e.ft = ft;
+            e.tile0 = tiles;
+            e.binz = grp.binz;
+            jg.push_back(e);
+            for (int i = 0; i < N; i++) fjob.push_back((int)jg.size() - 1);
+            ao += (long long)N * cx;
+            so += (long long)cx * H;
+            t1o += (long long)G * seay * cx;
+            sco += 16LL * N + 128;
+            fo += N;
+            go += G;
+            if (C > 0) tiles += ((cx + C - 1) / C) * G;
+        }
+        cx_max = std::max(cx_max, cx);
+        G_max = std::max(G_max, G);
+        N_max = std::max(N_max, N);
+        W_max = std::max(W_max, q.W);
+        char b[160];
+        snprintf(b, sizeof(b), "%d:%d:%d:%d:%.17g:%p:%p;", q.J, N, q.W, G, q.tol, (const void*)q.F,
+                 (const void*)q.f_inv);
+        sig += b;
+    }
+    const int J = (int)jg.size(), NT = fo, GT = go;
     std::string t(tag);
     auto nm = [&](const char* s) { return (t + s); };
     IterState st;
     st.J = J;
-    st.N = N;
-    st.G = G;
+    st.N = cls[0].N;
+    st.G = cls[0].grp->G;
     st.max_iter = max_iter;
-    const size_t sstride = 16 * (size_t)N + 128;
-    st.tmp = (double*)buf(c, nm("tmp").c_str(), sizeof(double) * J * N * 2);
-    st.total = (double*)buf(c, nm("total").c_str(), sizeof(double) * J * G * 2);
-    st.gshift = (double*)buf(c, nm("gshift").c_str(), sizeof(double) * J * G * 2);
-    st.final_ = (double*)buf(c, nm("final").c_str(), sizeof(double) * J * N * 2);
+    st.NT = NT;
+    st.cx_max = cx_max;
+    st.G_max = G_max;
+    st.N_max = N_max;
+    st.W_max = W_max;
+    st.tiles = tiles;
+    st.tmp = (double*)buf(c, nm("tmp").c_str(), sizeof(double) * NT * 2);
+    st.total = (double*)buf(c, nm("total").c_str(), sizeof(double) * GT * 2);
+    st.gshift = (double*)buf(c, nm("gshift").c_str(), sizeof(double) * GT * 2);
+    st.final_ = (double*)buf(c, nm("final").c_str(), sizeof(double) * NT * 2);
     st.errors = (double*)buf(c, nm("errors").c_str(), sizeof(double) * J * max_iter);
-    st.scratch = (double*)buf(c, nm("scratch").c_str(), sizeof(double) * J * sstride);
+    st.scratch = (double*)buf(c, nm("scratch").c_str(), sizeof(double) * sco);
     st.iters = (int*)buf(c, nm("iters").c_str(), sizeof(int) * J);
     st.done = (unsigned char*)buf(c, nm("done").c_str(), J);
-    st.gstart = (int*)buf(c, nm("gstart").c_str(), sizeof(int) * G);
-    st.gsize = (int*)buf(c, nm("gsize").c_str(), sizeof(int) * G);
-    st.fgroup = (int*)buf(c, nm("fgroup").c_str(), sizeof(int) * N);
-    st.gcent = (double*)buf(c, nm("gcent").c_str(), sizeof(double) * G);
-    st.fcent = (double*)buf(c, nm("fcent").c_str(), sizeof(double) * N);
+    st.gstart = (int*)buf(c, nm("gstart").c_str(), sizeof(int) * gstart.size());
+    st.gsize = (int*)buf(c, nm("gsize").c_str(), sizeof(int) * gsize.size());
+    st.fgroup = (int*)buf(c, nm("fgroup").c_str(), sizeof(int) * fgroup.size());
+    st.gcent = (double*)buf(c, nm("gcent").c_str(), sizeof(double) * gcent.size());
+    st.fcent = (double*)buf(c, nm("fcent").c_str(), sizeof(double) * fcent.size());
+    JobGeo* djobs = (JobGeo*)buf(c, nm("jobs").c_str(), sizeof(JobGeo) * J);
+    int* dfjob = (int*)buf(c, nm("fjob").c_str(), sizeof(int) * NT);
     if (!st.tmp || !st.total || !st.gshift || !st.final_ || !st.errors || !st.scratch ||
-        !st.iters || !st.done || !st.gstart || !st.gsize || !st.fgroup || !st.gcent || !st.fcent)
+        !st.iters || !st.done || !st.gstart || !st.gsize || !st.fgroup || !st.gcent || !st.fcent ||
+        !djobs || !dfjob)
         return fail(c, MC_E_NOMEM, "iteration state allocation failed");
-    MC_BUF(A, cfloat, nm("phA").c_str(), (size_t)J * N * cx);
-    MC_BUF(B, cfloat, nm("phB").c_str(), (size_t)J * N * H);
-    MC_BUF(S, cfloat, nm("sum").c_str(), (size_t)J * cx * H);
-    MC_BUF(T1, cfloat, nm("t1").c_str(), (size_t)J * G * seay * cx);
-    MC_BUF(win, float, nm("win").c_str(), (size_t)J * G * seay * seax);
-    cfloat* twW = row_twiddles(c, W);
-    if (!twW) return fail(c, MC_E_NOMEM, "twiddle table");
+    st.jobs = djobs;
+    st.fjob = dfjob;
+    MC_BUF(A, cfloat, nm("phA").c_str(), (size_t)ao);
+    MC_BUF(B, cfloat, nm("phB").c_str(), (size_t)NT * H);
+    MC_BUF(S, cfloat, nm("sum").c_str(), (size_t)so);
+    MC_BUF(T1, cfloat, nm("t1").c_str(), (size_t)t1o);
+    MC_BUF(win, float, nm("win").c_str(), (size_t)GT * seay * seax);
     hipStream_t s = c->stream;
-    MC_TRY(hipMemsetAsync(st.tmp, 0, sizeof(double) * J * N * 2, s));
-    MC_TRY(hipMemsetAsync(st.total, 0, sizeof(double) * J * G * 2, s));
+    MC_TRY(hipMemsetAsync(st.tmp, 0, sizeof(double) * NT * 2, s));
+    MC_TRY(hipMemsetAsync(st.total, 0, sizeof(double) * GT * 2, s));
     MC_TRY(hipMemsetAsync(st.errors, 0, sizeof(double) * J * max_iter, s));
     MC_TRY(hipMemsetAsync(st.iters, 0, sizeof(int) * J, s));
     MC_TRY(hipMemsetAsync(st.done, 0, J, s));
-    MC_TRY(hipMemcpyAsync(st.gstart, grp.start.data(), sizeof(int) * G, hipMemcpyHostToDevice, s));
-    MC_TRY(hipMemcpyAsync(st.gsize, grp.size.data(), sizeof(int) * G, hipMemcpyHostToDevice, s));
-    MC_TRY(hipMemcpyAsync(st.fgroup, grp.fgroup.data(), sizeof(int) * N, hipMemcpyHostToDevice, s));
-    MC_TRY(hipMemcpyAsync(st.gcent, grp.gcent.data(), sizeof(double) * G, hipMemcpyHostToDevice, s));
-    MC_TRY(hipMemcpyAsync(st.fcent, grp.fcent.data(), sizeof(double) * N, hipMemcpyHostToDevice, s));
+    MC_TRY(hipMemcpyAsync(st.gstart, gstart.data(), sizeof(int) * gstart.size(), hipMemcpyHostToDevice, s));
+    MC_TRY(hipMemcpyAsync(st.gsize, gsize.data(), sizeof(int) * gsize.size(), hipMemcpyHostToDevice, s));
+    MC_TRY(hipMemcpyAsync(st.fgroup, fgroup.data(), sizeof(int) * fgroup.size(), hipMemcpyHostToDevice, s));
+    MC_TRY(hipMemcpyAsync(st.gcent, gcent.data(), sizeof(double) * gcent.size(), hipMemcpyHostToDevice, s));
+    MC_TRY(hipMemcpyAsync(st.fcent, fcent.data(), sizeof(double) * fcent.size(), hipMemcpyHostToDevice, s));
+    MC_TRY(hipMemcpyAsync(djobs, jg.data(), sizeof(JobGeo) * J, hipMemcpyHostToDevice, s));
+    MC_TRY(hipMemcpyAsync(dfjob, fjob.data(), sizeof(int) * NT, hipMemcpyHostToDevice, s));
+    // the host tables go out of scope with this call; the copies above read
+    // pageable memory, so they are done when hipMemcpyAsync returns
+    const int N = st.N, G = st.G;  // the prior hooks (one class)
+    const double tol0 = cls[0].tol;
     bool feat = pr && pr->X;  // write the prior features after iteration 0
     // One iteration (Align/CIterativeAlign.cpp:117-140): B-factor -2 per
     // iteration with a floor of 20.
@@ -487,24 +587,28 @@ int run_iterative(mc_ctx* c, const char* tag, const cfloat* F, int J, int N, int
         hipError_t e;
         {
             KTimer kt(c, timed ? "phasors" : "");
-            if ((e = mc_launch_phasors(st.tmp, -1.0, 0, st.done, N, W, cx, H, J * N, A, B, s))) return e;
+            if ((e = mc_launch_phasors_jobs(st, st.tmp, H, A, B, s))) return e;
         }
         {
             KTimer kt(c, timed ? "aligned_sum" : "");
-            if ((e = mc_launch_aligned_sum(F, A, B, st.done, J, N, cx, H, S, s, f_fmt, f_inv))) return e;
+            if ((e = mc_launch_aligned_sum(nullptr, A, B, st.done, J, 0, cx_max, H, S, s, f_fmt, nullptr,
+                                           st.jobs)))
+                return e;
         }
         {
             KTimer kt(c, timed ? "correlate" : "");
-            if ((e = mc_launch_correlate(pl->col, pl->col_ptw, pl->C, F, S, A, B, st, cx, H, b,
-                                         phase_only, seay, T1, s, f_fmt, f_inv)))
+            if ((e = mc_launch_correlate(pl->col, pl->col_ptw, pl->C, nullptr, S, A, B, st, cx_max, H, b,
+                                         phase_only, seay, T1, s, f_fmt, nullptr)))
                 return e;
         }
         {
             KTimer kt(c, timed ? "row_dft" : "");
-            if ((e = mc_launch_row_dft(T1, twW, st.done, J, G, cx, W, seax, seay, win, s))) return e;
+            if ((e = mc_launch_row_dft(T1, nullptr, st.done, J, G_max, cx_max, W_max, seax, seay, win, s,
+                                       st.jobs)))
+                return e;
         }
-        if ((e = mc_launch_peaks(win, st.done, J, G, seax, seay, st.gshift, s))) return e;
-        if ((e = mc_launch_update(st, it, tol, s))) return e;
+        if ((e = mc_launch_peaks(win, st.done, J, G_max, seax, seay, st.gshift, s, st.jobs))) return e;
+        if ((e = mc_launch_update(st, it, tol0, s))) return e;
         if (it == 0 && feat)
             return prior_features(pr->side, pr->ns, st.tmp, pr->unit, win, J, N, G, st.fgroup, seax,
                                   seay, pr->X, pr->ns + 66 * N, s);
@@ -538,19 +642,19 @@ int run_iterative(mc_ctx* c, const char* tag, const cfloat* F, int J, int N, int
     for (int it0 = 0; it0 < max_iter && !skip_loop; it0 += 2) {
         const int it1 = std::min(it0 + 2, max_iter);
         if (use_graph) {
-            char key[1024];
-            snprintf(key, sizeof(key),
-                     "%s|%d|%d|%d %d %d %d %d %d|%.9g %.17g %d %d %d %d|%p %p %p %p %p %p %p %p %p %p "
-                     "%p %p %p %p %p %p %p %p %p %p %p %p %d %p|%d %p %p %d",
-                     tag, it0, it1, J, N, W, H, G, max_iter, (double)bft, tol, phase_only, seax,
-                     seay, pl->C, (const void*)F, (void*)A, (void*)B, (void*)S, (void*)T1,
-                     (void*)win, (void*)twW, (void*)st.tmp, (void*)st.total, (void*)st.gshift,
-                     (void*)st.final_, (void*)st.errors, (void*)st.iters, (void*)st.done,
-                     (void*)st.gstart, (void*)st.gsize, (void*)st.fgroup, (void*)st.gcent,
-                     (void*)st.fcent, (const void*)pl->col_ptw, (const void*)pl->col.tw,
-                     (const void*)f_inv, f_fmt, (const void*)pl, feat ? 1 : 0,
-                     feat ? (const void*)pr->side : nullptr, feat ? (void*)pr->X : nullptr,
-                     feat ? pr->ns : 0);
+            char key0[1024];
+            snprintf(key0, sizeof(key0),
+                     "%s|%d|%d|%d %d %d %d %d %d %d|%.9g %d %d %d %d|%p %p %p %p %p %p %p %p %p %p "
+                     "%p %p %p %p %p %p %p %p %p %p %p %d %p|%d %p %p %d|",
+                     tag, it0, it1, J, NT, H, max_iter, cx_max, G_max, tiles, (double)bft, phase_only,
+                     seax, seay, pl->C, (void*)A, (void*)B, (void*)S, (void*)T1, (void*)win,
+                     (void*)st.tmp, (void*)st.total, (void*)st.gshift, (void*)st.final_,
+                     (void*)st.errors, (void*)st.iters, (void*)st.done, (void*)st.gstart,
+                     (void*)st.gsize, (void*)st.fgroup, (void*)st.gcent, (void*)st.fcent,
+                     (void*)djobs, (void*)dfjob, (const void*)pl->col_ptw, (const void*)pl->col.tw,
+                     f_fmt, (const void*)pl, feat ? 1 : 0, feat ? (const void*)pr->side : nullptr,
+                     feat ? (void*)pr->X : nullptr, feat ? pr->ns : 0);
+            const std::string key = std::string(key0) + sig;
             auto itg = c->graphs.find(key);
             hipGraphExec_t ex = nullptr;
             if (itg == c->graphs.end()) {
@@ -587,11 +691,11 @@ int run_iterative(mc_ctx* c, const char* tag, const cfloat* F, int J, int N, int
             if (all) break;
         }
     }
-    MC_TRY(mc_launch_finalize(st, grp.binz, quirks, s));
-    out->shifts.resize((size_t)J * N * 2);
+    MC_TRY(mc_launch_finalize(st, cls[0].grp->binz, quirks, s));
+    out->shifts.resize((size_t)NT * 2);
     out->errors.resize((size_t)J * max_iter);
     out->iters.resize(J);
-    MC_TRY(hipMemcpyAsync(out->shifts.data(), st.final_, sizeof(double) * J * N * 2,
+    MC_TRY(hipMemcpyAsync(out->shifts.data(), st.final_, sizeof(double) * NT * 2,
                           hipMemcpyDeviceToHost, s));
     MC_TRY(hipMemcpyAsync(out->errors.data(), st.errors, sizeof(double) * J * max_iter,
                           hipMemcpyDeviceToHost, s));
@@ -606,6 +710,134 @@ int run_iterative(mc_ctx* c, const char* tag, const cfloat* F, int J, int N, int
     }
     MC_TRY(hipStreamSynchronize(s));
     return MC_OK;
+}
+
+// J jobs of one geometry.
+int run_iterative(mc_ctx* c, const char* tag, const cfloat* F, int J, int N, int W, int H,
+                  Fft2D* pl, const Groups& grp, float bft, int phase_only, int max_iter,
+                  double tol, int quirks, IterOut* out, int f_fmt = MC_SPEC_F32,
+                  const float* f_inv = nullptr, bool graphs = false, PriorRun* pr = nullptr) {
+    IterClass k;
+    k.F = F;
+    k.f_inv = f_inv;
+    k.J = J;
+    k.N = N;
+    k.W = W;
+    k.grp = &grp;
+    k.tol = tol;
+    return run_iterative_classes(c, tag, &k, 1, H, pl, bft, phase_only, max_iter, quirks, out, f_fmt,
+                                 graphs, pr);
+}
+
+// ---------------------------------------------------------------------------
+// Alignment iterations shared by movies of different sizes.  mc_align_movies
+// runs one align_batch per movie geometry, each on its own host thread; the
+// threads take turns (a thread runs only while it holds `mu`), so they never
+// touch the context at the same time.  At each iteration stage a thread
+// leaves its request and waits; when every live thread has asked, the last
+// one runs all requests as one batch per column length (one launch per
+// iteration stage over the jobs of every movie) and wakes the others.
+struct IterMeet {
+    struct Req {
+        std::string tag;
+        IterClass cls;
+        int H = 0;
+        Fft2D* pl = nullptr;
+        float bft = 0.f;
+        int phase_only = 0, max_iter = 0, quirks = 0, f_fmt = 0;
+        bool graphs = false;
+        IterOut* out = nullptr;
+        int rc = MC_OK;
+        std::string err;
+        bool done = false;
+    };
+    std::mutex mu;
+    std::condition_variable cv;
+    int live = 0;
+    std::vector<Req*> pending;
+
+    // With mu held: every pending request, in batches of requests that can
+    // share launches, then wake the waiting threads.
+    void run_pending(mc_ctx* c) {
+        std::vector<Req*> todo;
+        todo.swap(pending);
+        std::vector<char> used(todo.size(), 0);
+        for (size_t a = 0; a < todo.size(); a++) {
+            if (used[a]) continue;
+            std::vector<Req*> part;
+            for (size_t b = a; b < todo.size(); b++) {
+                const Req& x = *todo[a];
+                const Req& y = *todo[b];
+                if (!used[b] && x.tag == y.tag && x.H == y.H && x.bft == y.bft &&
+                    x.phase_only == y.phase_only && x.max_iter == y.max_iter && x.quirks == y.quirks &&
+                    x.f_fmt == y.f_fmt && x.graphs == y.graphs && std::min(64, x.cls.W) == std::min(64, y.cls.W)) {
+                    used[b] = 1;
+                    part.push_back(todo[b]);
+                }
+            }
+            std::vector<IterClass> cls;
+            for (Req* r : part) cls.push_back(r->cls);
+            const Req& r0 = *part[0];
+            IterOut all;
+            const int rc = run_iterative_classes(c, r0.tag.c_str(), cls.data(), (int)cls.size(), r0.H, r0.pl,
+                                                 r0.bft, r0.phase_only, r0.max_iter, r0.quirks, &all,
+                                                 r0.f_fmt, r0.graphs);
+            size_t fo = 0, jo = 0;
+            for (Req* r : part) {
+                r->rc = rc;
+                r->err = rc == MC_OK ? std::string() : c->err;
+                if (rc == MC_OK) {
+                    const size_t nf = (size_t)r->cls.J * r->cls.N, nj = (size_t)r->cls.J;
+                    r->out->shifts.assign(all.shifts.begin() + 2 * fo, all.shifts.begin() + 2 * (fo + nf));
+                    r->out->errors.assign(all.errors.begin() + jo * r0.max_iter,
+                                          all.errors.begin() + (jo + nj) * r0.max_iter);
+                    r->out->iters.assign(all.iters.begin() + jo, all.iters.begin() + jo + nj);
+                    fo += nf;
+                    jo += nj;
+                }
+                r->done = true;
+            }
+        }
+        cv.notify_all();
+    }
+};
+
+// The meeting of the calling thread and its lock (null: not in a meeting).
+thread_local IterMeet* tl_meet = nullptr;
+thread_local std::unique_lock<std::mutex>* tl_lock = nullptr;
+
+// run_iterative for one class of jobs, shared with the other threads of a
+// meeting when the calling thread is in one.
+int run_iter_meet(mc_ctx* c, const char* tag, const cfloat* F, int J, int N, int W, int H, Fft2D* pl,
+                  const Groups& grp, float bft, int phase_only, int max_iter, double tol, int quirks,
+                  IterOut* out, int f_fmt, const float* f_inv, bool graphs) {
+    if (!tl_meet)
+        return run_iterative(c, tag, F, J, N, W, H, pl, grp, bft, phase_only, max_iter, tol, quirks, out,
+                             f_fmt, f_inv, graphs);
+    IterMeet& m = *tl_meet;
+    IterMeet::Req r;
+    r.tag = tag;
+    r.cls.F = F;
+    r.cls.f_inv = f_inv;
+    r.cls.J = J;
+    r.cls.N = N;
+    r.cls.W = W;
+    r.cls.grp = &grp;
+    r.cls.tol = tol;
+    r.H = H;
+    r.pl = pl;
+    r.bft = bft;
+    r.phase_only = phase_only;
+    r.max_iter = max_iter;
+    r.quirks = quirks;
+    r.f_fmt = f_fmt;
+    r.graphs = graphs;
+    r.out = out;
+    m.pending.push_back(&r);
+    if ((int)m.pending.size() == m.live) m.run_pending(c);
+    else m.cv.wait(*tl_lock, [&] { return r.done; });
+    if (r.rc != MC_OK) c->err = r.err;
+    return r.rc;
 }
 
 // ---------------------------------------------------------------------------
@@ -1062,7 +1294,7 @@ int align_batch(mc_ctx* c, const mc_params* const* pp_, const mc_movie* const* m
     if (!replay) {
         Groups g0 = make_groups(p->group[0], N, p->throw0, p->fm_int_start, p->fm_int_size);
         IterOut go;
-        MC_CHECK(run_iterative(c, "g_", Sx_all, count, N, g.xw, g.xh, px, g0, p->bft[0],
+        MC_CHECK(run_iter_meet(c, "g_", Sx_all, count, N, g.xw, g.xh, px, g0, p->bft[0],
                                p->phase_only, p->iter, tol, quirks, &go, fx, ix,
                                p->launch_graphs != 0));
         for (int m = 0; m < count; m++) {
@@ -1229,9 +1461,14 @@ int align_batch(mc_ctx* c, const mc_params* const* pp_, const mc_movie* const* m
                 prr.unit = kPriorUnit;
             }
             IterOut po_;
-            MC_CHECK(run_iterative(c, "p_", Sp_all, J, N, g.pw, g.ph, pp, g1, p->bft[1],
-                                   p->phase_only, p->iter, tol, quirks, &po_, fpch, ip,
-                                   p->launch_graphs != 0, (guided || want_feat) ? &prr : nullptr));
+            if (guided || want_feat)
+                MC_CHECK(run_iterative(c, "p_", Sp_all, J, N, g.pw, g.ph, pp, g1, p->bft[1],
+                                       p->phase_only, p->iter, tol, quirks, &po_, fpch, ip,
+                                       p->launch_graphs != 0, &prr));
+            else
+                MC_CHECK(run_iter_meet(c, "p_", Sp_all, J, N, g.pw, g.ph, pp, g1, p->bft[1],
+                                       p->phase_only, p->iter, tol, quirks, &po_, fpch, ip,
+                                       p->launch_graphs != 0));
             std::vector<float> hx;
             if (want_feat) {
                 hx.resize((size_t)J * D);
@@ -1707,6 +1944,66 @@ static bool same_alignment_flags(const mc_params& a, const mc_params& b) {
            a.corr_interp == b.corr_interp && a.frame_fmt == b.frame_fmt;
 }
 
+// The movies `grp` (one geometry) of an mc_align_movies call: pipelined one
+// at a time, or in batches of at most what `budget` bytes of device memory
+// hold.
+static int align_group(mc_ctx* c, const mc_params* params, const mc_movie* movies, mc_result* results,
+                       const std::vector<int>& grp, double budget) {
+    const mc_params& p = params[grp[0]];
+    const mc_movie& mv = movies[grp[0]];
+    if (mv.W < 8 || mv.H < 8 || mv.N < 2) return fail(c, MC_E_ARG, "invalid movie geometry");
+    if (p.movie_pipeline && grp.size() > 1) {
+        // one movie at a time; movie k+1's forward transform runs on the
+        // second stream while movie k aligns (stack slots alternate)
+        struct Events {
+            hipEvent_t e[2] = {nullptr, nullptr};
+            ~Events() {
+                for (hipEvent_t x : e)
+                    if (x) (void)hipEventDestroy(x);
+            }
+        } evs;
+        for (hipEvent_t& e : evs.e)
+            if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess)
+                return fail(c, MC_E_HIP, "hipEventCreate");
+        Pipe pipe;
+        pipe.ev[0] = evs.e[0];
+        pipe.ev[1] = evs.e[1];
+        for (size_t k = 0; k < grp.size(); k++) {
+            pipe.slot = (int)(k & 1);
+            pipe.ready = k > 0;
+            pipe.next = k + 1 < grp.size() ? &movies[grp[k + 1]] : nullptr;
+            const mc_params* pa = &params[grp[k]];
+            const mc_movie* ma = &movies[grp[k]];
+            mc_result* ra = &results[grp[k]];
+            const int rc = align_batch(c, &pa, &ma, nullptr, &ra, 1, &pipe);
+            if (rc != MC_OK) {
+                (void)hipStreamSynchronize(c->s2);
+                return rc;
+            }
+        }
+        return MC_OK;
+    }
+    const bool patch = p.patch[0] > 1 && p.patch[1] > 1;
+    const Geo g = make_geo(mv.W, mv.H, mv.N, p.patch[0], p.patch[1], p.ft_bin);
+    const double per_movie = (double)mv.N * ((double)g.cx * mv.H * 8 + (double)g.xcx * g.xh * 8 +
+                                             (patch ? (double)p.patch[0] * p.patch[1] * g.pcx * g.ph * 8
+                                                    : 0.0));
+    const int cap = std::max(1, (int)std::min<double>(grp.size(), budget / per_movie));
+    for (size_t b0 = 0; b0 < grp.size(); b0 += cap) {
+        const int nb = (int)std::min<size_t>(cap, grp.size() - b0);
+        std::vector<const mc_params*> pa(nb);
+        std::vector<const mc_movie*> ma(nb);
+        std::vector<mc_result*> ra(nb);
+        for (int k = 0; k < nb; k++) {
+            pa[k] = &params[grp[b0 + k]];
+            ma[k] = &movies[grp[b0 + k]];
+            ra[k] = &results[grp[b0 + k]];
+        }
+        MC_CHECK(align_batch(c, pa.data(), ma.data(), nullptr, ra.data(), nb));
+    }
+    return MC_OK;
+}
+
 int mc_align_movies(mc_ctx* c, const mc_params* params, const mc_movie* movies, int count,
                     mc_result* results) {
     if (!c || !params || !movies || !results || count < 1) return MC_E_ARG;
@@ -1716,10 +2013,11 @@ int mc_align_movies(mc_ctx* c, const mc_params* params, const mc_movie* movies, 
             return fail(c, MC_E_ARG, "mc_align_movies: movies differ in alignment flags");
     size_t fr = 0, tot = 0;
     if (hipMemGetInfo(&fr, &tot) != hipSuccess) return fail(c, MC_E_HIP, "hipMemGetInfo");
+    // movies of equal size, groups in order of their first movie
+    std::vector<std::vector<int>> groups;
     std::vector<char> taken(count, 0);
     for (int m0 = 0; m0 < count; m0++) {
         if (taken[m0]) continue;
-        // movies of movie m0's size, in order
         std::vector<int> grp;
         for (int m = m0; m < count; m++)
             if (!taken[m] && movies[m].W == movies[m0].W && movies[m].H == movies[m0].H &&
@@ -1727,60 +2025,48 @@ int mc_align_movies(mc_ctx* c, const mc_params* params, const mc_movie* movies, 
                 grp.push_back(m);
                 taken[m] = 1;
             }
-        const mc_params& p = params[m0];
-        const mc_movie& mv = movies[m0];
-        if (mv.W < 8 || mv.H < 8 || mv.N < 2) return fail(c, MC_E_ARG, "invalid movie geometry");
-        if (p.movie_pipeline && grp.size() > 1) {
-            // one movie at a time; movie k+1's forward transform runs on the
-            // second stream while movie k aligns (stack slots alternate)
-            struct Events {
-                hipEvent_t e[2] = {nullptr, nullptr};
-                ~Events() {
-                    for (hipEvent_t x : e)
-                        if (x) (void)hipEventDestroy(x);
-                }
-            } evs;
-            for (hipEvent_t& e : evs.e)
-                if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess)
-                    return fail(c, MC_E_HIP, "hipEventCreate");
-            Pipe pipe;
-            pipe.ev[0] = evs.e[0];
-            pipe.ev[1] = evs.e[1];
-            for (size_t k = 0; k < grp.size(); k++) {
-                pipe.slot = (int)(k & 1);
-                pipe.ready = k > 0;
-                pipe.next = k + 1 < grp.size() ? &movies[grp[k + 1]] : nullptr;
-                const mc_params* pa = &params[grp[k]];
-                const mc_movie* ma = &movies[grp[k]];
-                mc_result* ra = &results[grp[k]];
-                const int rc = align_batch(c, &pa, &ma, nullptr, &ra, 1, &pipe);
-                if (rc != MC_OK) {
-                    (void)hipStreamSynchronize(c->s2);
-                    return rc;
-                }
-            }
-            continue;
-        }
-        const bool patch = p.patch[0] > 1 && p.patch[1] > 1;
-        const Geo g = make_geo(mv.W, mv.H, mv.N, p.patch[0], p.patch[1], p.ft_bin);
-        const double per_movie = (double)mv.N * ((double)g.cx * mv.H * 8 + (double)g.xcx * g.xh * 8 +
-                                                 (patch ? (double)p.patch[0] * p.patch[1] *
-                                                              g.pcx * g.ph * 8
-                                                        : 0.0));
-        const int cap = std::max(1, (int)std::min<double>(grp.size(), 0.5 * (double)tot / per_movie));
-        for (size_t b0 = 0; b0 < grp.size(); b0 += cap) {
-            const int nb = (int)std::min<size_t>(cap, grp.size() - b0);
-            std::vector<const mc_params*> pa(nb);
-            std::vector<const mc_movie*> ma(nb);
-            std::vector<mc_result*> ra(nb);
-            for (int k = 0; k < nb; k++) {
-                pa[k] = &params[grp[b0 + k]];
-                ma[k] = &movies[grp[b0 + k]];
-                ra[k] = &results[grp[b0 + k]];
-            }
-            MC_CHECK(align_batch(c, pa.data(), ma.data(), nullptr, ra.data(), nb));
-        }
+        groups.push_back(grp);
     }
+    const mc_params& p0 = params[0];
+    if (groups.size() == 1 || p0.movie_pipeline || p0.prior || p0.align == 0) {
+        for (const auto& grp : groups) MC_CHECK(align_group(c, params, movies, results, grp, 0.5 * (double)tot));
+        return MC_OK;
+    }
+    // Movies of different sizes with batched iterations: one host thread per
+    // size, taking turns; their alignment iterations meet (IterMeet), so each
+    // iteration stage is one launch over the jobs of all sizes that share a
+    // column length.  The device memory budget is split between the sizes.
+    IterMeet meet;
+    meet.live = (int)groups.size();
+    const double share = 0.5 * (double)tot / (double)groups.size();
+    std::vector<int> rcs(groups.size(), MC_OK);
+    std::vector<std::string> errs(groups.size());
+    std::vector<std::thread> th;
+    for (size_t k = 0; k < groups.size(); k++) {
+        th.emplace_back([&, k]() {
+            std::unique_lock<std::mutex> lk(meet.mu);
+            if (hipSetDevice(c->device) != hipSuccess) {
+                rcs[k] = fail(c, MC_E_HIP, "hipSetDevice");
+            } else {
+                tl_buf_prefix = k ? "m" + std::to_string(k) + ":" : std::string();
+                tl_meet = &meet;
+                tl_lock = &lk;
+                rcs[k] = align_group(c, params, movies, results, groups[k], share);
+                tl_meet = nullptr;
+                tl_lock = nullptr;
+                tl_buf_prefix.clear();
+            }
+            if (rcs[k] != MC_OK) errs[k] = c->err;
+            meet.live--;
+            if (!meet.pending.empty() && (int)meet.pending.size() == meet.live) meet.run_pending(c);
+        });
+    }
+    for (auto& t : th) t.join();
+    for (size_t k = 0; k < groups.size(); k++)
+        if (rcs[k] != MC_OK) {
+            c->err = errs[k];
+            return rcs[k];
+        }
     return MC_OK;
 }
 

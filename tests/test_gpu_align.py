@@ -192,6 +192,42 @@ def test_batched_movies_match_single_calls(ctx, pipeline, patch, fmt):
             assert np.array_equal(x.numpy(), y.numpy())
 
 
+@pytest.mark.parametrize("case", ["1024", "production"])
+def test_uneven_movies_share_iteration_launches(ctx, case):
+    """mc_align_movies without the pipeline, on movies of different sizes
+    whose alignment stacks share the column length: the global iterations of
+    all movies run as one batch of jobs, and so do their patch iterations,
+    each job with its own frame count, group count and row length (1024 x
+    1024 x 12 with 1536 x 1024 x 8: 1024-point xcf and 340-point patch
+    columns; 5760 x 4092 x 6 with 4096 x 4096 x 8: 2048-point xcf and
+    408-point patch columns, the register-pass correlation).  Every result
+    equals a separate mc_align_movie call's, bit for bit."""
+    if case == "1024":
+        kw = dict(patch=(3, 3), iter=10, pix_size=1.0, kv=300, fm_dose=1.0)
+        shapes = [(1024, 1024, 12, 21), (1536, 1024, 8, 22), (1024, 1024, 12, 23)]
+        movies = [ctx.upload(MovieRecipe(W, H, N, 300, seed=sd, dose=4.0, local_max=2.0).movie())
+                  for W, H, N, sd in shapes]
+    else:
+        kw = dict(patch=(5, 5), bft=(500, 100), pix_size=0.83, kv=300, fm_dose=1.0)
+        shapes = [(5760, 4092, 6, 31), (4096, 4096, 8, 32)]
+        movies = []
+        for W, H, N, sd in shapes:
+            fr = ctx.empty((N, H, W), np.float32)
+            ctx.synth_render(fr, MovieRecipe(W, H, N, 20000, seed=sd, dose=1.0, local_max=3.0))
+            movies.append(fr)
+    prm = L.default_params(movie_pipeline=0, **kw)
+    print("uneven batch", case, shapes, flush=True)
+    single = [ctx.align_movie(m, prm) for m in movies]
+    batched = ctx.align_movies(movies, prm)
+    keys = ("global_shift", "patch_shift", "patch_bad", "patch_center", "patch_starts", "patch_iters")
+    for a, b in zip(batched, single):
+        for k in keys:
+            assert np.array_equal(a[k], b[k]), k
+        assert a["global_iters"] == b["global_iters"]
+        for x, y in zip(a["sums"], b["sums"]):
+            assert np.array_equal(x.numpy(), y.numpy())
+
+
 def test_chunk_streams_match_single_stream(ctx):
     """Frame chunks on two streams give the same bits as on one stream."""
     W, H, N = 5760, 4092, 6
