@@ -161,6 +161,17 @@ typedef struct mc_result {
                                   (Correct/CCorrectFullShift.cpp:336-356), z-binned
                                   as Align/CZbinStack.cpp:27-70 */
     int    frame_fmt_used;     /* storage actually used for the full-frame spectra */
+    double seconds_stage[5];   /* GPU time of each stage on the context stream,
+                                  between events recorded at its ends: [0] forward
+                                  transform, [1] global alignment, [2] patch alignment
+                                  ("Patch alignment time", Align/CPatchAlign.cpp:35-39),
+                                  [3] correction of the frames ("Global shifts are
+                                  corrected" + "Correction of local motion",
+                                  Align/CPatchAlign.cpp:59-63,
+                                  Correct/GCorrectPatchShift.cu:133-134), [4] sums
+                                  and output.  A stage that did not run reads 0; in a
+                                  multi-movie call every movie gets the call's
+                                  figures. */
 } mc_result;
 
 /* Replacements of measured quantities.  replay_* follow -InAln

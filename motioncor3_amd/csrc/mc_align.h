@@ -89,7 +89,7 @@ hipError_t mc_launch_patch_stats(double* shifts, const double* centers, int N, i
                                  int quirks, unsigned char* bad, int* ref, hipStream_t s);
 hipError_t mc_launch_local_warp(const float* R, float* Rp, long long pitch, long long fstride,
                                 int W, int H, int N, int P, const float* cen, const float* sh,
-                                const unsigned char* bad, hipStream_t s);
+                                const unsigned char* bad, unsigned long long bad_any, hipStream_t s);
 hipError_t mc_launch_dw_sums(const cfloat* F, long long fstride, int N, int cx, int H,
                              const cfloat* A, const cfloat* B, int dw, double pix, double kvf,
                              double dose, int dws, double r0, double r1, int split, cfloat* sums,

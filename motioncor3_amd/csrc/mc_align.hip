@@ -418,9 +418,9 @@ __global__ __launch_bounds__(NT) void k_correlate_rp(CorrArgs a, const cfloat* _
                 }
             }
             RegDft<R0>::run(v);
-            cfloat* d = lds + c * H + jj * R0;
+            cfloat* d = lds + c * rp_ld(H);
 #pragma unroll
-            for (int t = 0; t < R0; t++) d[t] = v[t];
+            for (int t = 0; t < R0; t++) d[rp_ix<H>(jj * R0 + t)] = v[t];
         }
     }
     __syncthreads();
@@ -441,9 +441,9 @@ __global__ __launch_bounds__(NT) void k_correlate_rp(CorrArgs a, const cfloat* _
             }
             if (c < nc && need) {
                 cfloat v[RL];
-                const cfloat* sl = lds + c * H + k;
+                const cfloat* sl = lds + c * rp_ld(H);
 #pragma unroll
-                for (int t = 0; t < RL; t++) v[t] = sl[t * ML];
+                for (int t = 0; t < RL; t++) v[t] = sl[rp_ix<H>(k + t * ML)];
                 rp_twiddle<H, NP - 1>(v, k, ptw);
                 RegDft<RL>::run(v);
 #pragma unroll
@@ -1157,10 +1157,13 @@ __global__ __launch_bounds__(256) void k_patch_stats(double* s, const double* ce
 // sums w, w*sx, w*sy in patch order in FP32, divides, and gathers the pixel at
 // (int)(x - sx), (int)(y - sy) with 811*|.| mod N reflection.  Here the
 // weights of a 64x4-pixel tile are formed once for all frames.  A patch whose
-// weight is below 1e-14 of the pixel's largest weight (r > r_min + 0.3224)
-// for every pixel of the tile is left out: adding it cannot change any of the
-// FP32 sums (it is far below half an ulp), so the kept patches, taken in the
-// reference's order, give the same sums.  The frame table holds
+// weight is below 1e-14 of the largest weight among a frame's good patches
+// (r > r_min + 0.3224) for every pixel of the tile is left out: adding it
+// cannot change any of the FP32 sums (it is far below half an ulp), so the
+// kept patches, taken in the reference's order, give the same sums.  r_min
+// is taken over the patches that are good in every frame (bad_any clear), an
+// upper bound on every frame's nearest good patch; with none, every patch
+// within r <= 0.6 is kept.  The frame table holds
 // {ok*sx, ok*sy, ok} per kept patch (ok = 0 for a bad patch), so
 // fw = fma(w, ok, fw) and fsx = fma(w, ok*sx, fsx) reproduce the reference's
 // skip exactly; up to 16 kept patches the weights live in registers.
@@ -1186,6 +1189,7 @@ struct WarpArgs {
     const float* cen;
     const float* sh;
     const unsigned char* bad;
+    unsigned long long bad_any;  // bit p: patch p is bad in some frame
 };
 
 // Source offset (within a frame) of output pixel (x, y) for the weighted
@@ -1336,6 +1340,7 @@ __global__ __launch_bounds__(MC_WARP_NT) void k_local_warp(WarpArgs a) {
     const float rw = 1.0f / (float)a.W, rh = 1.0f / (float)a.H;
     float r2min = 1e30f;
     for (int p = 0; p < a.P && live; p++) {
+        if ((a.bad_any >> p) & 1ull) continue;
         const float ax = ((float)x - a.cen[2 * p]) * rw;
         const float ay = ((float)y - a.cen[2 * p + 1]) * rh;
         r2min = fminf(r2min, ax * ax + ay * ay);
@@ -1660,7 +1665,7 @@ hipError_t mc_launch_correlate(const FftPlan1& colplan, const cfloat* ptw, int C
     if (H == HH && ptw && seay <= HH) {                                                  \
         if (st.jobs && st.tiles < 1) return hipErrorInvalidValue;                        \
         const dim3 grid(st.jobs ? st.tiles : mc_div_up(cx, CC) * st.G * st.J);           \
-        const size_t lds = (size_t)CC * HH * sizeof(cfloat);                             \
+        const size_t lds = (size_t)CC * rp_ld(HH) * sizeof(cfloat);                      \
         if (f_fmt == MC_SPEC_F32)                                                        \
             hipLaunchKernelGGL((k_correlate_rp<HH, CC, NT, 0>), grid, dim3(NT), lds, s, a, ptw); \
         else if (f_fmt == MC_SPEC_F16)                                                   \
@@ -1784,10 +1789,10 @@ hipError_t mc_launch_patch_stats(double* shifts, const double* centers, int N, i
 
 hipError_t mc_launch_local_warp(const float* R, float* Rp, long long pitch, long long fstride,
                                 int W, int H, int N, int P, const float* cen, const float* sh,
-                                const unsigned char* bad, hipStream_t s) {
+                                const unsigned char* bad, unsigned long long bad_any, hipStream_t s) {
     if (P < 1 || P > MC_WARP_MAXP) return hipErrorInvalidValue;
     if (pitch * H >= (1LL << 31) || pitch < W) return hipErrorInvalidValue;
-    WarpArgs a{R, Rp, pitch, fstride, W, H, N, P, cen, sh, bad};
+    WarpArgs a{R, Rp, pitch, fstride, W, H, N, P, cen, sh, bad, bad_any};
     dim3 grid(mc_div_up(W, MC_WARP_TX), mc_div_up(H, MC_WARP_TY));
     hipLaunchKernelGGL(k_local_warp, grid, dim3(MC_WARP_NT), warp_lds_bytes(P), s, a);
     return hipGetLastError();

@@ -628,9 +628,9 @@ __global__ __launch_bounds__(NT) void k_cols_fwd_rp(ColFwdArgs a, const cfloat* 
 #pragma unroll
             for (int t = 0; t < R0; t++) v[t] = ok ? src[(j + t * M0) * CT + c] : cmk(0.f, 0.f);
             RegDft<R0>::run(v);
-            cfloat* d = lds + c * H + j * R0;
+            cfloat* d = lds + c * rp_ld(H);
 #pragma unroll
-            for (int t = 0; t < R0; t++) d[t] = v[t];
+            for (int t = 0; t < R0; t++) d[rp_ix<H>(j * R0 + t)] = v[t];
         }
     }
     __syncthreads();
@@ -643,9 +643,9 @@ __global__ __launch_bounds__(NT) void k_cols_fwd_rp(ColFwdArgs a, const cfloat* 
             const int c = i / ML, k = i - c * ML;
             if (c < nc) {
                 cfloat v[RL];
-                const cfloat* sl = lds + c * H + k;
+                const cfloat* sl = lds + c * rp_ld(H);
 #pragma unroll
-                for (int t = 0; t < RL; t++) v[t] = sl[t * ML];
+                for (int t = 0; t < RL; t++) v[t] = sl[rp_ix<H>(k + t * ML)];
                 rp_twiddle<H, NP - 1>(v, k, ptw);
                 RegDft<RL>::run(v);
                 const int kx = c0 + c;
@@ -721,9 +721,9 @@ __global__ __launch_bounds__(NT) void k_cols_inv_rp(ColInvArgs a, const cfloat* 
                 for (int t = 0; t < R0; t++) v[t] = cmk(0.f, 0.f);
             }
             RegDft<R0>::run(v);
-            cfloat* d = lds + c * H + j * R0;
+            cfloat* d = lds + c * rp_ld(H);
 #pragma unroll
-            for (int t = 0; t < R0; t++) d[t] = v[t];
+            for (int t = 0; t < R0; t++) d[rp_ix<H>(j * R0 + t)] = v[t];
         }
     }
     __syncthreads();
@@ -737,9 +737,9 @@ __global__ __launch_bounds__(NT) void k_cols_inv_rp(ColInvArgs a, const cfloat* 
             const int c = i % C, k = i / C;
             if (c < nc) {
                 cfloat v[RL];
-                const cfloat* sl = lds + c * H + k;
+                const cfloat* sl = lds + c * rp_ld(H);
 #pragma unroll
-                for (int t = 0; t < RL; t++) v[t] = sl[t * ML];
+                for (int t = 0; t < RL; t++) v[t] = sl[rp_ix<H>(k + t * ML)];
                 rp_twiddle<H, NP - 1>(v, k, ptw);
                 RegDft<RL>::run(v);
 #pragma unroll
@@ -804,9 +804,9 @@ __global__ __launch_bounds__(NT) void k_rows_r2c_rp(RowR2CArgs a, const cfloat* 
                 for (int t = 0; t < R0; t++) v[t] = cmk(0.f, 0.f);
             }
             RegDft<R0>::run(v);
-            cfloat* d = lds + r * M + j * R0;
+            cfloat* d = lds + r * rp_ld(M);
 #pragma unroll
-            for (int t = 0; t < R0; t++) d[t] = v[t];
+            for (int t = 0; t < R0; t++) d[rp_ix<M>(j * R0 + t)] = v[t];
         }
     }
     __syncthreads();
@@ -818,8 +818,8 @@ __global__ __launch_bounds__(NT) void k_rows_r2c_rp(RowR2CArgs a, const cfloat* 
         const int c = e % C, r = (e / C) % RR, kb = e / (C * RR);
         const int k = kb * C + c;
         if (k > M || r >= nrows) continue;
-        const cfloat zk = lds[r * M + (k == M ? 0 : k)];
-        const cfloat zm = lds[r * M + (k == 0 ? 0 : M - k)];
+        const cfloat zk = lds[r * rp_ld(M) + rp_ix<M>(k == M ? 0 : k)];
+        const cfloat zm = lds[r * rp_ld(M) + rp_ix<M>(k == 0 ? 0 : M - k)];
         const cfloat E = cmk(0.5f * (zk.x + zm.x), 0.5f * (zk.y - zm.y));
         const cfloat D = cmk(zk.x - zm.x, zk.y + zm.y);
         const cfloat O = cmk(0.5f * D.y, -0.5f * D.x);
@@ -892,17 +892,16 @@ __global__ __launch_bounds__(NT) void k_rows_c2r_rp(RowC2RArgs a, const cfloat* 
                                              a.post_tw[j + t * M0]);
                 }
                 RegDft<R0>::run(v);
-                cfloat* d = lds + r * M + j * R0;
+                cfloat* d = lds + r * rp_ld(M);
 #pragma unroll
-                for (int t = 0; t < R0; t++) d[t] = v[t];
+                for (int t = 0; t < R0; t++) d[rp_ix<M>(j * R0 + t)] = v[t];
                 if (two) {
 #pragma unroll
                     for (int t = 0; t < R0; t++)
                         v[t] = c2r_pre(xb[t], xa[R0 - 1 - t], a.post_tw[j2 + t * M0]);
                     RegDft<R0>::run(v);
-                    cfloat* d2 = lds + r * M + j2 * R0;
 #pragma unroll
-                    for (int t = 0; t < R0; t++) d2[t] = v[t];
+                    for (int t = 0; t < R0; t++) d[rp_ix<M>(j2 * R0 + t)] = v[t];
                 }
             }
         }
@@ -917,9 +916,9 @@ __global__ __launch_bounds__(NT) void k_rows_c2r_rp(RowC2RArgs a, const cfloat* 
             const int r = i / ML, k = i - r * ML;
             if (r < nrows) {
                 cfloat v[RL];
-                const cfloat* sl = lds + r * M + k;
+                const cfloat* sl = lds + r * rp_ld(M);
 #pragma unroll
-                for (int t = 0; t < RL; t++) v[t] = sl[t * ML];
+                for (int t = 0; t < RL; t++) v[t] = sl[rp_ix<M>(k + t * ML)];
                 rp_twiddle<M, NP - 1>(v, k, ptw);
                 RegDft<RL>::run(v);
                 float* ob = a.out + (long long)b * a.out_bstride + (long long)(row0 + r) * a.pitch;
@@ -956,7 +955,7 @@ template <int H, int CW, int NT>
 static hipError_t launch_cols_fwd_rp(const ColFwdArgs& a, const cfloat* ptw, int batch, int s_fmt,
                                      int x_fmt, hipStream_t s) {
     const dim3 grid(mc_div_up(a.ncols, CW) * batch), blk(NT);
-    const size_t lds = (size_t)CW * H * sizeof(cfloat);
+    const size_t lds = (size_t)CW * rp_ld(H) * sizeof(cfloat);
     const int key = s_fmt * 3 + x_fmt;
     switch (key) {
         case 0: hipLaunchKernelGGL((k_cols_fwd_rp<H, CW, NT, 0, 0>), grid, blk, lds, s, a, ptw); break;
@@ -974,7 +973,7 @@ template <int H, int CW, int NT>
 static hipError_t launch_cols_inv_rp(const ColInvArgs& a, const cfloat* ptw, int batch, int s_fmt,
                                      hipStream_t s) {
     const dim3 grid(mc_div_up(a.ncols, CW) * batch), blk(NT);
-    const size_t lds = (size_t)CW * H * sizeof(cfloat);
+    const size_t lds = (size_t)CW * rp_ld(H) * sizeof(cfloat);
     switch (s_fmt) {
         case 0: hipLaunchKernelGGL((k_cols_inv_rp<H, CW, NT, 0>), grid, blk, lds, s, a, ptw); break;
         case 1: hipLaunchKernelGGL((k_cols_inv_rp<H, CW, NT, 1>), grid, blk, lds, s, a, ptw); break;
@@ -1267,7 +1266,7 @@ hipError_t mc_rows_r2c_mod(const Fft2D& p, const float* in, long long pitch, lon
     if (p.m == MM && p.C == CCC && p.row_ptw && v_++ == p.row_var) {                       \
         hipLaunchKernelGGL((k_rows_r2c_rp<MM, RRR, CCC, NT>),                              \
                            dim3(mc_div_up(p.H, RRR) * batch), dim3(NT),                    \
-                           (size_t)RRR * MM * sizeof(cfloat), s, a, (const cfloat*)p.row_ptw); \
+                           (size_t)RRR * rp_ld(MM) * sizeof(cfloat), s, a, (const cfloat*)p.row_ptw); \
         return hipGetLastError();                                                          \
     }
     MC_RP_ROW_SPECS(X)
@@ -1309,7 +1308,7 @@ hipError_t mc_rows_c2r(const Fft2D& p, const cfloat* T, long long t_bstride, flo
     if (p.m == MM && p.C == CCC && p.row_ptw && v_++ == p.row_var) {                       \
         hipLaunchKernelGGL((k_rows_c2r_rp<MM, RRR, CCC, NT>),                              \
                            dim3(mc_div_up(p.H, RRR) * batch), dim3(NT),                    \
-                           (size_t)RRR * (MM + 1) * sizeof(cfloat), s, a,                  \
+                           (size_t)RRR * (rp_ld(MM) + 1) * sizeof(cfloat), s, a,           \
                            (const cfloat*)p.row_ptw);                                      \
         return hipGetLastError();                                                          \
     }

@@ -163,6 +163,24 @@ __host__ __device__ constexpr int rp_twoff(int n, int p) {
 }
 __host__ __device__ constexpr int rp_twcount(int n) { return rp_twoff(n, rp_npass(n)); }
 
+// LDS layout of one transform.  Pass 0 stores an item's R_0 outputs at a
+// stride of R_0 complex values across lanes; for an even R_0 (the power-of-
+// two lengths) that maps a wave's lanes onto one or two LDS banks, and the
+// middle passes' stores at NS = 16 collide in groups of 16 lanes.  Those
+// lengths get one pad element per 16 (index i -> i + i/16), which makes both
+// strides odd in 8-byte units.  Lengths with an odd R_0 keep the unpadded
+// layout, so their kernels are unchanged.
+__host__ __device__ constexpr bool rp_pad(int n) {
+    return rp_has(n) && rp_radix(n, 0) % 2 == 0 && n % 16 == 0;
+}
+// LDS elements of one transform (the column / row stride in LDS)
+__host__ __device__ constexpr int rp_ld(int n) { return rp_pad(n) ? n + n / 16 : n; }
+template <int N>
+static __device__ __forceinline__ int rp_ix(int i) {
+    if constexpr (rp_pad(N)) return i + (i >> 4);
+    else return i;
+}
+
 // Multiply input t (1 <= t < R) of an item with twiddle index k.
 template <int N, int P>
 static __device__ __forceinline__ void rp_twiddle(cfloat* v, int k, const cfloat* __restrict__ tw) {
@@ -173,36 +191,38 @@ static __device__ __forceinline__ void rp_twiddle(cfloat* v, int k, const cfloat
     }
 }
 
-// A middle pass over the C columns held in LDS at col0 + c*N, NT threads;
+// A middle pass over the C columns held in LDS at c * rp_ld(N), NT threads;
 // item i -> column i / M, butterfly i % M.  Ends with a barrier.
 template <int N, int P, int C, int NT>
 static __device__ __forceinline__ void rp_pass_lds(cfloat* lds, const cfloat* __restrict__ tw) {
     constexpr int R = rp_radix(N, P), NS = rp_ns(N, P), M = N / R;
-    constexpr int IT = (C * M + NT - 1) / NT;
+    constexpr int IT = (C * M + NT - 1) / NT, LD = rp_ld(N);
     cfloat v[IT][R];
-    int dst[IT];
+    int dcol[IT], dst[IT];
 #pragma unroll
     for (int it = 0; it < IT; it++) {
         const int i = threadIdx.x + it * NT;
         dst[it] = -1;
+        dcol[it] = 0;
         if ((C * M) % NT == 0 || i < C * M) {
             const int c = i / M, j = i - c * M;
             const int k = j % NS, q = j / NS;
-            const cfloat* src = lds + c * N + j;
+            const cfloat* src = lds + c * LD;
 #pragma unroll
-            for (int t = 0; t < R; t++) v[it][t] = src[t * M];
+            for (int t = 0; t < R; t++) v[it][t] = src[rp_ix<N>(j + t * M)];
             rp_twiddle<N, P>(v[it], k, tw);
             RegDft<R>::run(v[it]);
-            dst[it] = c * N + q * NS * R + k;
+            dcol[it] = c * LD;
+            dst[it] = q * NS * R + k;
         }
     }
     __syncthreads();
 #pragma unroll
     for (int it = 0; it < IT; it++) {
         if (dst[it] >= 0) {
-            cfloat* d = lds + dst[it];
+            cfloat* d = lds + dcol[it];
 #pragma unroll
-            for (int t = 0; t < R; t++) d[t * NS] = v[it][t];
+            for (int t = 0; t < R; t++) d[rp_ix<N>(dst[it] + t * NS)] = v[it][t];
         }
     }
     __syncthreads();

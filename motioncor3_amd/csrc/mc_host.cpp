@@ -7,6 +7,7 @@
 // per-frame and per-patch loop of the reference is a batched launch here and
 // all intermediate data stays in HBM.  The host only sees the shift tables.
 #include <hip/hip_runtime.h>
+#include <rocprofiler-sdk-roctx/roctx.h>
 
 #include <algorithm>
 #include <chrono>
@@ -75,6 +76,8 @@ struct mc_ctx {
     size_t pin_shifts_n = 0;
     // learned prior for guided patch alignment (not owned)
     mc_prior* prior = nullptr;
+    // timing events of the stage clock (mc_result.seconds_stage), reused
+    std::vector<hipEvent_t> stage_ev;
     // ingest: stream and completion event of asynchronous frame loads, and
     // the gain / dark references (device, ref_w x ref_h)
     hipStream_t s_in = nullptr;
@@ -227,6 +230,71 @@ struct KTimer {
         if (on) (void)hipEventRecord(c->evs[idx].b, st);
     }
 };
+
+// Stage clock of one call: events recorded on the context stream at the
+// ends of each stage interval; after the call's final synchronisation the
+// intervals are summed per stage (mc_result.seconds_stage).  The events are
+// the context's, created on first use.
+struct StageClock {
+    mc_ctx* c;
+    hipStream_t s;
+    size_t used = 0;
+    struct Span {
+        int stage;
+        size_t a, b;
+    };
+    std::vector<Span> spans;
+    size_t open_at = 0;
+    int open_stage = -1;
+    StageClock(mc_ctx* ctx, hipStream_t st) : c(ctx), s(st) {}
+    hipError_t mark(size_t* idx) {
+        if (used >= c->stage_ev.size()) {
+            hipEvent_t e;
+            const hipError_t r = hipEventCreate(&e);
+            if (r != hipSuccess) return r;
+            c->stage_ev.push_back(e);
+        }
+        *idx = used++;
+        return hipEventRecord(c->stage_ev[*idx], s);
+    }
+    // close the open interval (if any) and open one for `stage` (-1: none)
+    hipError_t begin(int stage) {
+        size_t i = 0;
+        const hipError_t r = mark(&i);
+        if (r != hipSuccess) return r;
+        if (open_stage >= 0) spans.push_back({open_stage, open_at, i});
+        open_stage = stage;
+        open_at = i;
+        return hipSuccess;
+    }
+    // after the stream has drained: seconds per stage
+    void read(double* out5) const {
+        for (int k = 0; k < 5; k++) out5[k] = 0.0;
+        for (const Span& sp : spans) {
+            float ms = 0.f;
+            if (hipEventElapsedTime(&ms, c->stage_ev[sp.a], c->stage_ev[sp.b]) == hipSuccess)
+                out5[sp.stage] += ms * 1e-3;
+        }
+    }
+};
+
+// roctx range for the profiler's timeline (the reference's NVTX ranges,
+// e.g. Align/CAlignMain.cpp:41,48, Align/CPatchAlign.cpp:49,69)
+struct Range {
+    explicit Range(const char* name) { roctxRangePushA(name); }
+    ~Range() { roctxRangePop(); }
+    Range(const Range&) = delete;
+    Range& operator=(const Range&) = delete;
+};
+
+// Patches flagged bad in any frame of an N x P host table (bit p), for the
+// local warp's patch selection.
+unsigned long long bad_any_of(const unsigned char* bad, int N, int P) {
+    unsigned long long m = 0;
+    for (long long e = 0; bad && e < (long long)N * P; e++)
+        if (bad[e]) m |= 1ull << (e % P);
+    return m;
+}
 
 // ---------------------------------------------------------------------------
 // geometry: CBufferPool.cpp:185-262, Align/CAlignBase.cpp:41-47
@@ -1111,6 +1179,7 @@ int align_batch(mc_ctx* c, const mc_params* const* pp_, const mc_movie* const* m
             N < 2 || mv->pitch < W || mv->frame_stride < mv->pitch * H)
             return fail(c, MC_E_ARG, "invalid movie geometry");
     }
+    for (int m = 0; m < count; m++) std::fill(rs[m]->seconds_stage, rs[m]->seconds_stage + 5, 0.0);
     if (p->align == 0) {
         for (int m = 0; m < count; m++) MC_CHECK(simple_sums(c, pp_[m], mvs[m], rs[m]));
         return MC_OK;
@@ -1250,6 +1319,9 @@ int align_batch(mc_ctx* c, const mc_params* const* pp_, const mc_movie* const* m
 
     const float scale = (float)(1.0 / ((double)W * H));
     void* Sx_fwd = (xcf_alias || replay) ? nullptr : (void*)Sx_all;
+    Range r_call("mc::align_batch");
+    StageClock sc(c, s);
+    MC_TRY(sc.begin(0));
     if (pipe && pipe->ready) {
         // transformed on the second stream while the previous movie aligned
         MC_TRY(hipStreamWaitEvent(s, pipe->ev[pipe->slot], 0));
@@ -1294,6 +1366,8 @@ int align_batch(mc_ctx* c, const mc_params* const* pp_, const mc_movie* const* m
     if (!replay) {
         Groups g0 = make_groups(p->group[0], N, p->throw0, p->fm_int_start, p->fm_int_size);
         IterOut go;
+        MC_TRY(sc.begin(1));
+        Range r_g("mc::global_align");
         MC_CHECK(run_iter_meet(c, "g_", Sx_all, count, N, g.xw, g.xh, px, g0, p->bft[0],
                                p->phase_only, p->iter, tol, quirks, &go, fx, ix,
                                p->launch_graphs != 0));
@@ -1364,6 +1438,8 @@ int align_batch(mc_ctx* c, const mc_params* const* pp_, const mc_movie* const* m
             return MC_OK;
         };
         if (patch) {
+            MC_TRY(sc.begin(2));
+            Range r_p("mc::patch_align");
             const long long pfs = (long long)g.pcx * g.ph;
             MC_BUF(Tp, cfloat, "Tp", (size_t)P * N * pp->t_elems);
             cfloat* Sp_all = (cfloat*)buf(c, "Sp", (fpch == MC_SPEC_F32 ? 8 : 4) * (size_t)count * P * N * pfs);
@@ -1540,6 +1616,8 @@ int align_batch(mc_ctx* c, const mc_params* const* pp_, const mc_movie* const* m
 
     // ---- correction and sums, one movie at a time ----
     for (int m = 0; m < count; m++) {
+        MC_TRY(sc.begin(3));
+        Range r_c("mc::correct");
         mc_result* res = rs[m];
         std::vector<double>& gshift = ms[m].gshift;
         const std::vector<double>& ps = ms[m].ps;
@@ -1607,7 +1685,8 @@ int align_batch(mc_ctx* c, const mc_params* const* pp_, const mc_movie* const* m
             MC_TRY(hipMemcpyAsync(dbad2, bad.data(), bad.size(), hipMemcpyHostToDevice, s));
             {
                 KTimer kt(c, "local_warp");
-                MC_TRY(mc_launch_local_warp(Rr, Rw, W, rfs, W, H, N, P, dpcf, dpsf, dbad2, s));
+                MC_TRY(mc_launch_local_warp(Rr, Rw, W, rfs, W, H, N, P, dpcf, dpsf, dbad2,
+                                            bad_any_of(bad.data(), N, P), s));
             }
             MC_TRY(fork());
             for (int f0 = 0; f0 < N; f0 += CH) {
@@ -1635,6 +1714,7 @@ int align_batch(mc_ctx* c, const mc_params* const* pp_, const mc_movie* const* m
             // mGenSums' mMotionDecon (Correct/CCorrectFullShift.cpp:271-320) with
             // the same full shifts (Correct/GCorrectPatchShift.cu:106): the
             // motion weight a second time
+            MC_TRY(sc.begin(4));
             MC_TRY(mc_launch_frame_weights(Sf, fs, N, g.cx, H, gsh, p->in_fm_motion, 0, s));
             KTimer kt(c, "dw_sums");
             MC_TRY(mc_launch_dw_sums(Sf, fs, N, g.cx, H, nullptr, nullptr, dw, p->pix_size, kvf,
@@ -1644,6 +1724,7 @@ int align_batch(mc_ctx* c, const mc_params* const* pp_, const mc_movie* const* m
             // Correct/CCorrectFullShift.cpp:229-309 with the absolute shifts
             MC_TRY(hipMemcpyAsync(gsh, gshift.data(), sizeof(double) * 2 * N, hipMemcpyHostToDevice, s));
             MC_TRY(mc_launch_phasors(gsh, -1.0, 1, nullptr, N, W, g.cx, H, N, fA, fB, s));
+            MC_TRY(sc.begin(4));
             // mGenSums' mMotionDecon (Correct/CCorrectFullShift.cpp:271-320);
             // the weight commutes with the phase shift, so it goes on the
             // uncorrected spectra
@@ -1702,8 +1783,11 @@ int align_batch(mc_ctx* c, const mc_params* const* pp_, const mc_movie* const* m
             }
         }
     }
+    MC_TRY(sc.begin(-1));
     MC_TRY(hipStreamSynchronize(s));
     const auto tc = clk::now();
+    double stage_s[5];
+    sc.read(stage_s);
 
     for (int m = 0; m < count; m++) {
         mc_result* res = rs[m];
@@ -1747,6 +1831,14 @@ int align_batch(mc_ctx* c, const mc_params* const* pp_, const mc_movie* const* m
         }
         res->seconds_align = std::chrono::duration<double>(tb - ta).count();
         res->seconds_total = std::chrono::duration<double>(tc - t0).count();
+        std::copy(stage_s, stage_s + 5, res->seconds_stage);
+        if (p->verbose && m == 0) {
+            // the reference's stage lines (Align/CPatchAlign.cpp:38,61,
+            // Correct/GCorrectPatchShift.cu:134), GPU time of the call
+            if (patch) printf("Patch alignment time: %.2f(sec)\n\n", stage_s[2]);
+            printf("%s: %f sec\n\n", patch ? "Correction of local motion" : "Global shifts are corrected",
+                   stage_s[3]);
+        }
     }
     return MC_OK;
 }
@@ -1829,6 +1921,7 @@ int mc_ctx_destroy(mc_ctx* c) {
         (void)hipEventDestroy(e.b);
     }
     for (auto& kv : c->graphs) (void)hipGraphExecDestroy(kv.second);
+    for (hipEvent_t e : c->stage_ev) (void)hipEventDestroy(e);
     if (c->s2) {
         (void)hipStreamSynchronize(c->s2);
         (void)hipStreamDestroy(c->s2);
@@ -2210,7 +2303,8 @@ int mc_local_warp(mc_ctx* c, const float* in, float* out, int W, int H, int N, i
     MC_TRY(hipMemcpyAsync(ds, shifts, sizeof(float) * 2 * N * P, hipMemcpyHostToDevice, c->stream));
     if (bad) MC_TRY(hipMemcpyAsync(db, bad, (size_t)N * P, hipMemcpyHostToDevice, c->stream));
     else MC_TRY(hipMemsetAsync(db, 0, (size_t)N * P, c->stream));
-    MC_TRY(mc_launch_local_warp(in, out, W, (long long)W * H, W, H, N, P, dc, ds, db, c->stream));
+    MC_TRY(mc_launch_local_warp(in, out, W, (long long)W * H, W, H, N, P, dc, ds, db, bad_any_of(bad, N, P),
+                                c->stream));
     return MC_OK;
 }
 
@@ -2245,7 +2339,8 @@ int mc_local_warp_noalloc(mc_ctx* c, const float* in, float* out, int W, int H, 
     // the previous call's flags are still there: write or clear all of them
     if (bad) MC_TRY(hipMemcpyAsync(db, bad, (size_t)N * P, hipMemcpyHostToDevice, c->stream));
     else MC_TRY(mc_launch_fill_u8(db, (long long)N * P, 0, c->stream));
-    MC_TRY(mc_launch_local_warp(in, out, W, (long long)W * H, W, H, N, P, dc, ds, db, c->stream));
+    MC_TRY(mc_launch_local_warp(in, out, W, (long long)W * H, W, H, N, P, dc, ds, db, bad_any_of(bad, N, P),
+                                c->stream));
     return MC_OK;
 }
 

@@ -18,6 +18,8 @@ _LIB = None
 
 MC_OK = 0
 MC_E_ARG = -1
+# names of mc_result.seconds_stage[0..4]
+STAGES = ("forward", "global_align", "patch_align", "correct", "sums")
 
 
 class McParams(C.Structure):
@@ -53,6 +55,7 @@ class McResult(C.Structure):
         ("seconds_total", C.c_double), ("spectra_fmt_used", C.c_int * 2),
         ("prior_features", C.POINTER(C.c_float)), ("prior_targets", C.POINTER(C.c_float)),
         ("prior_fallbacks", C.c_int), ("stack", C.c_void_p), ("frame_fmt_used", C.c_int),
+        ("seconds_stage", C.c_double * 5),
     ]
 
 
@@ -610,6 +613,7 @@ class Context:
         out["spectra_fmt_used"] = (res.spectra_fmt_used[0], res.spectra_fmt_used[1])
         out["prior_fallbacks"] = res.prior_fallbacks
         out["frame_fmt_used"] = res.frame_fmt_used
+        out["seconds_stage"] = dict(zip(STAGES, res.seconds_stage))
         return out
 
     def align_movie(self, frames, params, replay_global=None, replay_patch=None,

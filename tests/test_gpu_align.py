@@ -75,6 +75,12 @@ def test_patch_alignment_matches_oracle(ctx):
                          replay_bad=gres["patch_bad"])
     for i in range(gres["num_sums"]):
         assert _rel(gres["sums"][i].numpy(), orep["sums"][i]) < 1e-4, i
+    # stage clock: every stage of a patch alignment ran on the GPU, and the
+    # stages are intervals of the call
+    st = gres["seconds_stage"]
+    assert set(st) == set(L.STAGES)
+    assert all(st[k] > 0 for k in ("global_align", "patch_align", "correct", "sums")), st
+    assert sum(st.values()) <= gres["seconds_total"] * 1.05 + 1e-3, (st, gres["seconds_total"])
 
 
 def test_production_geometry_ground_truth(ctx):
