@@ -126,12 +126,25 @@ def cpu_baseline(ctx, frames, gres, params_kw, frames_needed):
     t0 = time.time()
     ores = O.align_movie(host, po)
     dt = time.time() - t0
-    del host
     dg = g["global_shift"] - ores["global_shift"]
     rms_g = float(np.sqrt(np.mean(np.sum(dg ** 2, axis=1))))
     same = bool(np.array_equal(g["patch_starts"], ores["patch_starts"]))
     dp = g["patch_shift"] - ores["patch_shift"]
     rms_p = float(np.sqrt(np.mean(np.sum(dp ** 2, axis=2))))
+    # each output sum against the oracle's own (its own shifts: a shift
+    # difference moves the skip rule and the warp's pixel truncation too)
+    gsums = [g["sums"][i].numpy() for i in range(len(ores["sums"]))]
+    sum_l2 = [float(np.linalg.norm(gsums[i] - ores["sums"][i]) / np.linalg.norm(ores["sums"][i]))
+              for i in range(len(ores["sums"]))]
+    # and against the oracle's correction with the GPU's shifts (the -InAln
+    # replay): the sums' own arithmetic, without the discrete effects
+    t1 = time.time()
+    orep = O.align_movie(host, po, replay_global=g["global_shift"], replay_patch=g["patch_shift"],
+                         replay_centers=g["patch_center"], replay_bad=g["patch_bad"])
+    dt_rep = time.time() - t1
+    sum_l2_rep = [float(np.linalg.norm(gsums[i] - orep["sums"][i]) / np.linalg.norm(orep["sums"][i]))
+                  for i in range(len(orep["sums"]))]
+    del orep, host
     threads = O.lib().orc_num_threads()
     hi = host_info()
     movie_s = dt * N / frames_needed if frames_needed < N else dt
@@ -145,6 +158,8 @@ def cpu_baseline(ctx, frames, gres, params_kw, frames_needed):
                patch_max_vs_fp64_px=float(np.max(np.abs(dp))),
                fp64_frames=min(frames_needed, N), same_patch_starts=same,
                same_bad_patches=bool(np.array_equal(g["patch_bad"], ores["patch_bad"])),
+               sum_rel_l2_vs_fp64=sum_l2, sum_rel_l2_vs_fp64_replay=sum_l2_rep,
+               fp64_replay_s=dt_rep,
                fp64_global_iters=int(ores["global_iters"]),
                fp64_patch_iters_max=int(np.max(ores["patch_iters"])))
     return cb, acc
@@ -424,6 +439,17 @@ def main():
     d0 = recipe.displacement
     e0 = (gs - gs.mean(0)) + (d0 - d0.mean(0))
     truth_rms_drift_only = float(np.sqrt(np.mean(np.sum(e0 ** 2, axis=1))))
+    # patch shifts against the local field at each patch centre, less the
+    # frame mean the global alignment already took out (reported shift =
+    # -residual displacement + a constant per patch)
+    pc, ps = res0["patch_center"], res0["patch_shift"]
+    res_x = np.stack([recipe.local_field(pc[:, 0], pc[:, 1], t)[0] - recipe.mean_local(t)[0]
+                      for t in range(N)])
+    res_y = np.stack([recipe.local_field(pc[:, 0], pc[:, 1], t)[1] - recipe.mean_local(t)[1]
+                      for t in range(N)])
+    ep = ps + np.stack([res_x, res_y], axis=2)
+    ep = ep - ep.mean(0, keepdims=True)
+    patch_truth_rms = float(np.sqrt(np.mean(np.sum(ep ** 2, axis=2))))
 
     out = {
         "metric": METRIC,
@@ -473,6 +499,7 @@ def main():
             "path_frac": pb / (ms * 1e-3) / 1e9 / PEAK_HBM_GBS,
         },
         "accuracy": {"global_shift_rms_vs_truth_px": truth_rms,
+                     "patch_shift_rms_vs_truth_px": patch_truth_rms,
                      "global_shift_rms_vs_drift_only_px": truth_rms_drift_only,
                      "truth": "global drift plus the frame mean of the local field"},
     }

@@ -418,7 +418,7 @@ __global__ __launch_bounds__(NT) void k_correlate_rp(CorrArgs a, const cfloat* _
                 }
             }
             RegDft<R0>::run(v);
-            cfloat* d = lds + c * rp_ld(H);
+            cfloat* d = lds + c * rp_cs(H, C);
 #pragma unroll
             for (int t = 0; t < R0; t++) d[rp_ix<H>(jj * R0 + t)] = v[t];
         }
@@ -441,7 +441,7 @@ __global__ __launch_bounds__(NT) void k_correlate_rp(CorrArgs a, const cfloat* _
             }
             if (c < nc && need) {
                 cfloat v[RL];
-                const cfloat* sl = lds + c * rp_ld(H);
+                const cfloat* sl = lds + c * rp_cs(H, C);
 #pragma unroll
                 for (int t = 0; t < RL; t++) v[t] = sl[rp_ix<H>(k + t * ML)];
                 rp_twiddle<H, NP - 1>(v, k, ptw);
@@ -1214,9 +1214,11 @@ __device__ __forceinline__ int warp_src(int W, int H, int pitch, int x, int y, f
     return ys * pitch + xs;
 }
 
-// Fill the table of frames f0..f0+nf-1 for the K (>= nk) kept patches.
+// Fill the table of frames f0..f0+nf-1 for the K (>= nk) kept patches;
+// fbad[fi] (cleared by the caller) becomes 1 when a kept patch is bad in
+// frame f0 + fi.
 __device__ __forceinline__ void warp_table(const WarpArgs& a, const int* klist, int nk, int K,
-                                           int f0, int nf, float4* tab) {
+                                           int f0, int nf, float4* tab, int* fbad) {
     for (int e = threadIdx.x; e < nf * K; e += MC_WARP_NT) {
         const int fi = e / K, q = e - fi * K;
         float4 t = make_float4(0.f, 0.f, 0.f, 0.f);
@@ -1227,15 +1229,19 @@ __device__ __forceinline__ void warp_table(const WarpArgs& a, const int* klist, 
                 t.x = a.sh[2 * o];
                 t.y = a.sh[2 * o + 1];
                 t.z = 1.f;
+            } else {
+                fbad[fi] = 1;
             }
         }
         tab[e] = t;
     }
 }
 
+// A frame without bad kept patches has fw = sum_q w_q, the same FMA chain
+// (fma(w, 1, fw) is w + fw) for every such frame: it is formed once.
 template <int K>
 __device__ __forceinline__ void warp_tile_reg(const WarpArgs& a, const int* klist, int nk,
-                                              float4* tab, int x, int y, bool live) {
+                                              float4* tab, int* fbad, int x, int y, bool live) {
     float w[K];
 #pragma unroll
     for (int q = 0; q < K; q++) {
@@ -1248,10 +1254,15 @@ __device__ __forceinline__ void warp_tile_reg(const WarpArgs& a, const int* klis
             w[q] = (r > 0.6f) ? 0.f : expf(-100.0f * r);
         }
     }
+    float fw_all = 0.f;
+#pragma unroll
+    for (int q = 0; q < K; q++) fw_all = fmaf(w[q], q < nk ? 1.f : 0.f, fw_all);
     for (int f0 = 0; f0 < a.N; f0 += MC_WARP_FCH) {
         const int nf = min(MC_WARP_FCH, a.N - f0);
         __syncthreads();
-        warp_table(a, klist, nk, K, f0, nf, tab);
+        if (threadIdx.x < MC_WARP_FCH) fbad[threadIdx.x] = 0;
+        __syncthreads();
+        warp_table(a, klist, nk, K, f0, nf, tab, fbad);
         __syncthreads();
         if (!live) continue;
         // gathers of MC_WARP_U frames are issued before their stores so that
@@ -1263,12 +1274,17 @@ __device__ __forceinline__ void warp_tile_reg(const WarpArgs& a, const int* klis
             int so[MC_WARP_U];
 #pragma unroll
             for (int u = 0; u < MC_WARP_U; u++) {
-                const float4* t = tab + min(fb + u, nf - 1) * K;
-                float fw = 0.f, fsx = 0.f, fsy = 0.f;
+                const int fi = min(fb + u, nf - 1);
+                const float4* t = tab + fi * K;
+                float fw = fw_all, fsx = 0.f, fsy = 0.f;
+                if (fbad[fi]) {
+                    fw = 0.f;
+#pragma unroll
+                    for (int q = 0; q < K; q++) fw = fmaf(w[q], t[q].z, fw);
+                }
 #pragma unroll
                 for (int q = 0; q < K; q++) {
                     const float4 v = t[q];
-                    fw = fmaf(w[q], v.z, fw);
                     fsx = fmaf(v.x, w[q], fsx);
                     fsy = fmaf(v.y, w[q], fsy);
                 }
@@ -1287,7 +1303,7 @@ __device__ __forceinline__ void warp_tile_reg(const WarpArgs& a, const int* klis
 
 // More than MC_WARP_KREG kept patches: weights in LDS, run-time loop.
 __device__ __forceinline__ void warp_tile_lds(const WarpArgs& a, const int* klist, int nk,
-                                           float4* tab, float* wk, int x, int y, bool live) {
+                                           float4* tab, int* fbad, float* wk, int x, int y, bool live) {
     for (int q = 0; q < nk; q++) {
         float w = 0.f;
         if (live) {
@@ -1302,7 +1318,7 @@ __device__ __forceinline__ void warp_tile_lds(const WarpArgs& a, const int* klis
     for (int f0 = 0; f0 < a.N; f0 += MC_WARP_FCH) {
         const int nf = min(MC_WARP_FCH, a.N - f0);
         __syncthreads();
-        warp_table(a, klist, nk, nk, f0, nf, tab);
+        warp_table(a, klist, nk, nk, f0, nf, tab, fbad);
         __syncthreads();
         if (!live) continue;
         for (int fi = 0; fi < nf; fi++) {
@@ -1329,6 +1345,7 @@ __global__ __launch_bounds__(MC_WARP_NT) void k_local_warp(WarpArgs a) {
     __shared__ unsigned int kmask[2];
     __shared__ int klist[MC_WARP_MAXP];
     __shared__ int nk_s;
+    __shared__ int fbad[MC_WARP_FCH];
     const int x = blockIdx.x * MC_WARP_TX + (threadIdx.x % MC_WARP_TX);
     const int y = blockIdx.y * MC_WARP_TY + (threadIdx.x / MC_WARP_TX);
     const bool live = x < a.W && y < a.H;
@@ -1371,17 +1388,17 @@ __global__ __launch_bounds__(MC_WARP_NT) void k_local_warp(WarpArgs a) {
     const int nk = nk_s;
     switch (nk) {
 #define MC_WK(K) \
-    case K: warp_tile_reg<K>(a, klist, nk, tab, x, y, live); return;
+    case K: warp_tile_reg<K>(a, klist, nk, tab, fbad, x, y, live); return;
         MC_WK(1) MC_WK(2) MC_WK(3) MC_WK(4) MC_WK(5) MC_WK(6) MC_WK(7) MC_WK(8)
         MC_WK(9) MC_WK(10) MC_WK(11) MC_WK(12) MC_WK(13) MC_WK(14) MC_WK(15) MC_WK(16)
 #undef MC_WK
         default: break;
     }
     if (nk == 0) {
-        warp_tile_reg<1>(a, klist, 0, tab, x, y, live);
+        warp_tile_reg<1>(a, klist, 0, tab, fbad, x, y, live);
         return;
     }
-        warp_tile_lds(a, klist, nk, tab, reinterpret_cast<float*>(tab + MC_WARP_FCH * a.P), x, y,
+        warp_tile_lds(a, klist, nk, tab, fbad, reinterpret_cast<float*>(tab + MC_WARP_FCH * a.P), x, y,
                       live);
 }
 
@@ -1665,7 +1682,7 @@ hipError_t mc_launch_correlate(const FftPlan1& colplan, const cfloat* ptw, int C
     if (H == HH && ptw && seay <= HH) {                                                  \
         if (st.jobs && st.tiles < 1) return hipErrorInvalidValue;                        \
         const dim3 grid(st.jobs ? st.tiles : mc_div_up(cx, CC) * st.G * st.J);           \
-        const size_t lds = (size_t)CC * rp_ld(HH) * sizeof(cfloat);                      \
+        const size_t lds = (size_t)CC * rp_cs(HH, CC) * sizeof(cfloat);                  \
         if (f_fmt == MC_SPEC_F32)                                                        \
             hipLaunchKernelGGL((k_correlate_rp<HH, CC, NT, 0>), grid, dim3(NT), lds, s, a, ptw); \
         else if (f_fmt == MC_SPEC_F16)                                                   \

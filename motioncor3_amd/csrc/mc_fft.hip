@@ -628,7 +628,7 @@ __global__ __launch_bounds__(NT) void k_cols_fwd_rp(ColFwdArgs a, const cfloat* 
 #pragma unroll
             for (int t = 0; t < R0; t++) v[t] = ok ? src[(j + t * M0) * CT + c] : cmk(0.f, 0.f);
             RegDft<R0>::run(v);
-            cfloat* d = lds + c * rp_ld(H);
+            cfloat* d = lds + c * rp_cs(H, C);
 #pragma unroll
             for (int t = 0; t < R0; t++) d[rp_ix<H>(j * R0 + t)] = v[t];
         }
@@ -643,7 +643,7 @@ __global__ __launch_bounds__(NT) void k_cols_fwd_rp(ColFwdArgs a, const cfloat* 
             const int c = i / ML, k = i - c * ML;
             if (c < nc) {
                 cfloat v[RL];
-                const cfloat* sl = lds + c * rp_ld(H);
+                const cfloat* sl = lds + c * rp_cs(H, C);
 #pragma unroll
                 for (int t = 0; t < RL; t++) v[t] = sl[rp_ix<H>(k + t * ML)];
                 rp_twiddle<H, NP - 1>(v, k, ptw);
@@ -721,7 +721,7 @@ __global__ __launch_bounds__(NT) void k_cols_inv_rp(ColInvArgs a, const cfloat* 
                 for (int t = 0; t < R0; t++) v[t] = cmk(0.f, 0.f);
             }
             RegDft<R0>::run(v);
-            cfloat* d = lds + c * rp_ld(H);
+            cfloat* d = lds + c * rp_cs(H, C);
 #pragma unroll
             for (int t = 0; t < R0; t++) d[rp_ix<H>(j * R0 + t)] = v[t];
         }
@@ -737,7 +737,7 @@ __global__ __launch_bounds__(NT) void k_cols_inv_rp(ColInvArgs a, const cfloat* 
             const int c = i % C, k = i / C;
             if (c < nc) {
                 cfloat v[RL];
-                const cfloat* sl = lds + c * rp_ld(H);
+                const cfloat* sl = lds + c * rp_cs(H, C);
 #pragma unroll
                 for (int t = 0; t < RL; t++) v[t] = sl[rp_ix<H>(k + t * ML)];
                 rp_twiddle<H, NP - 1>(v, k, ptw);
@@ -804,7 +804,7 @@ __global__ __launch_bounds__(NT) void k_rows_r2c_rp(RowR2CArgs a, const cfloat* 
                 for (int t = 0; t < R0; t++) v[t] = cmk(0.f, 0.f);
             }
             RegDft<R0>::run(v);
-            cfloat* d = lds + r * rp_ld(M);
+            cfloat* d = lds + r * rp_cs(M, RR);
 #pragma unroll
             for (int t = 0; t < R0; t++) d[rp_ix<M>(j * R0 + t)] = v[t];
         }
@@ -818,8 +818,8 @@ __global__ __launch_bounds__(NT) void k_rows_r2c_rp(RowR2CArgs a, const cfloat* 
         const int c = e % C, r = (e / C) % RR, kb = e / (C * RR);
         const int k = kb * C + c;
         if (k > M || r >= nrows) continue;
-        const cfloat zk = lds[r * rp_ld(M) + rp_ix<M>(k == M ? 0 : k)];
-        const cfloat zm = lds[r * rp_ld(M) + rp_ix<M>(k == 0 ? 0 : M - k)];
+        const cfloat zk = lds[r * rp_cs(M, RR) + rp_ix<M>(k == M ? 0 : k)];
+        const cfloat zm = lds[r * rp_cs(M, RR) + rp_ix<M>(k == 0 ? 0 : M - k)];
         const cfloat E = cmk(0.5f * (zk.x + zm.x), 0.5f * (zk.y - zm.y));
         const cfloat D = cmk(zk.x - zm.x, zk.y + zm.y);
         const cfloat O = cmk(0.5f * D.y, -0.5f * D.x);
@@ -892,7 +892,7 @@ __global__ __launch_bounds__(NT) void k_rows_c2r_rp(RowC2RArgs a, const cfloat* 
                                              a.post_tw[j + t * M0]);
                 }
                 RegDft<R0>::run(v);
-                cfloat* d = lds + r * rp_ld(M);
+                cfloat* d = lds + r * rp_cs(M, RR);
 #pragma unroll
                 for (int t = 0; t < R0; t++) d[rp_ix<M>(j * R0 + t)] = v[t];
                 if (two) {
@@ -916,7 +916,7 @@ __global__ __launch_bounds__(NT) void k_rows_c2r_rp(RowC2RArgs a, const cfloat* 
             const int r = i / ML, k = i - r * ML;
             if (r < nrows) {
                 cfloat v[RL];
-                const cfloat* sl = lds + r * rp_ld(M);
+                const cfloat* sl = lds + r * rp_cs(M, RR);
 #pragma unroll
                 for (int t = 0; t < RL; t++) v[t] = sl[rp_ix<M>(k + t * ML)];
                 rp_twiddle<M, NP - 1>(v, k, ptw);
@@ -955,7 +955,7 @@ template <int H, int CW, int NT>
 static hipError_t launch_cols_fwd_rp(const ColFwdArgs& a, const cfloat* ptw, int batch, int s_fmt,
                                      int x_fmt, hipStream_t s) {
     const dim3 grid(mc_div_up(a.ncols, CW) * batch), blk(NT);
-    const size_t lds = (size_t)CW * rp_ld(H) * sizeof(cfloat);
+    const size_t lds = (size_t)CW * rp_cs(H, CW) * sizeof(cfloat);
     const int key = s_fmt * 3 + x_fmt;
     switch (key) {
         case 0: hipLaunchKernelGGL((k_cols_fwd_rp<H, CW, NT, 0, 0>), grid, blk, lds, s, a, ptw); break;
@@ -973,7 +973,7 @@ template <int H, int CW, int NT>
 static hipError_t launch_cols_inv_rp(const ColInvArgs& a, const cfloat* ptw, int batch, int s_fmt,
                                      hipStream_t s) {
     const dim3 grid(mc_div_up(a.ncols, CW) * batch), blk(NT);
-    const size_t lds = (size_t)CW * rp_ld(H) * sizeof(cfloat);
+    const size_t lds = (size_t)CW * rp_cs(H, CW) * sizeof(cfloat);
     switch (s_fmt) {
         case 0: hipLaunchKernelGGL((k_cols_inv_rp<H, CW, NT, 0>), grid, blk, lds, s, a, ptw); break;
         case 1: hipLaunchKernelGGL((k_cols_inv_rp<H, CW, NT, 1>), grid, blk, lds, s, a, ptw); break;
@@ -1096,6 +1096,32 @@ int build_plan1(FftPlan1* P, cfloat** dev_tw, int n) {
                     h.push_back(roots[k <= hp ? (t * k) % bp : 0]);
                 }
         if ((int)h.size() != ct_rt_off(n) + ct_rt_elems(n)) return -1;
+        // Rader table of the same stage (ct_stage_rader): Bh = DFT_L(b) / L
+        // with b_n = e^{-2 pi i g^-n / P}, then e^{-2 pi i n2 k1 / L}
+        if (ct_rd_elems(n)) {
+            const int L = bp - 1, L1 = ct_rd_l1(bp), L2 = ct_rd_l2(bp);
+            const int gi = ct_powmod(ct_proot(bp), bp - 2, bp);
+            std::vector<double> br(L), bi(L);
+            for (int k = 0, e = 1; k < L; k++, e = (int)((long long)e * gi % bp)) {
+                br[k] = std::cos(-2.0 * M_PI * e / bp);
+                bi[k] = std::sin(-2.0 * M_PI * e / bp);
+            }
+            for (int k = 0; k < L; k++) {
+                double sr = 0.0, si = 0.0;
+                for (int q = 0; q < L; q++) {
+                    const double a = -2.0 * M_PI * (double)((long long)q * k % L) / L;
+                    sr += br[q] * std::cos(a) - bi[q] * std::sin(a);
+                    si += br[q] * std::sin(a) + bi[q] * std::cos(a);
+                }
+                h.push_back(cfloat{(float)(sr / L), (float)(si / L)});
+            }
+            for (int n2 = 0; n2 < L2; n2++)
+                for (int k1 = 0; k1 < L1; k1++) {
+                    const double a = -2.0 * M_PI * (double)(n2 * k1) / L;
+                    h.push_back(cfloat{(float)std::cos(a), (float)std::sin(a)});
+                }
+            if ((int)h.size() != ct_rd_off(n) + ct_rd_elems(n)) return -1;
+        }
     }
     if (h.empty()) h.push_back(cfloat{1.f, 0.f});
     if (hipMalloc(dev_tw, h.size() * sizeof(cfloat)) != hipSuccess) return -2;
@@ -1266,7 +1292,7 @@ hipError_t mc_rows_r2c_mod(const Fft2D& p, const float* in, long long pitch, lon
     if (p.m == MM && p.C == CCC && p.row_ptw && v_++ == p.row_var) {                       \
         hipLaunchKernelGGL((k_rows_r2c_rp<MM, RRR, CCC, NT>),                              \
                            dim3(mc_div_up(p.H, RRR) * batch), dim3(NT),                    \
-                           (size_t)RRR * rp_ld(MM) * sizeof(cfloat), s, a, (const cfloat*)p.row_ptw); \
+                           (size_t)RRR * rp_cs(MM, RRR) * sizeof(cfloat), s, a, (const cfloat*)p.row_ptw); \
         return hipGetLastError();                                                          \
     }
     MC_RP_ROW_SPECS(X)
@@ -1308,7 +1334,7 @@ hipError_t mc_rows_c2r(const Fft2D& p, const cfloat* T, long long t_bstride, flo
     if (p.m == MM && p.C == CCC && p.row_ptw && v_++ == p.row_var) {                       \
         hipLaunchKernelGGL((k_rows_c2r_rp<MM, RRR, CCC, NT>),                              \
                            dim3(mc_div_up(p.H, RRR) * batch), dim3(NT),                    \
-                           (size_t)RRR * (rp_ld(MM) + 1) * sizeof(cfloat), s, a,           \
+                           (size_t)RRR * (rp_cs(MM, RRR) + 1) * sizeof(cfloat), s, a,      \
                            (const cfloat*)p.row_ptw);                                      \
         return hipGetLastError();                                                          \
     }

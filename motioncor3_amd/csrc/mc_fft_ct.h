@@ -17,6 +17,113 @@ __host__ __device__ constexpr int ct_spf(int n) {
         if (n % p == 0) return p;
     return n;
 }
+// ---- compile-time trigonometry (Taylor series in double, |x| <= pi) ----
+__host__ __device__ constexpr double rp_pi() { return 3.14159265358979323846; }
+__host__ __device__ constexpr double rp_sin_red(double x) {
+    double term = x, sum = x;
+    for (int n = 1; n < 30; n++) {
+        term *= -x * x / ((2.0 * n) * (2.0 * n + 1.0));
+        sum += term;
+    }
+    return sum;
+}
+__host__ __device__ constexpr double rp_cos_red(double x) {
+    double term = 1.0, sum = 1.0;
+    for (int n = 1; n < 30; n++) {
+        term *= -x * x / ((2.0 * n - 1.0) * (2.0 * n));
+        sum += term;
+    }
+    return sum;
+}
+// cos and sin of 2 pi a / b for integers, argument reduced to [-pi, pi]
+__host__ __device__ constexpr double rp_cos2pi(int a, int b) {
+    int r = a % b;
+    if (r < 0) r += b;
+    if (2 * r > b) r -= b;
+    return rp_cos_red(2.0 * rp_pi() * r / b);
+}
+__host__ __device__ constexpr double rp_sin2pi(int a, int b) {
+    int r = a % b;
+    if (r < 0) r += b;
+    if (2 * r > b) r -= b;
+    return rp_sin_red(2.0 * rp_pi() * r / b);
+}
+
+// ---- composite register DFT: v[k] = sum_t v[t] e^{-2 pi i t k / R} ----
+__host__ __device__ constexpr int rp_first(int r) {
+    // leading factor of a composite register DFT: 4 or 2 for even sizes
+    // (small butterflies), else the smallest odd prime
+    return r % 4 == 0 && r != 4 && r != 8 ? 4 : (r % 2 == 0 && r != 2 && r != 4 && r != 8 ? 2 : ct_spf(r));
+}
+__host__ __device__ constexpr bool rp_leaf(int r) {
+    return r == 1 || r == 2 || r == 4 || r == 8 || (ct_spf(r) == r && r <= 31);
+}
+
+template <int R, bool LEAF = rp_leaf(R)>
+struct RegDft;
+
+template <int R>
+struct RegDft<R, true> {
+    static __device__ __forceinline__ void run(cfloat* v) {
+        if constexpr (R > 1) bfly<R>(v);
+    }
+};
+
+// cos/sin(2 pi i / R) for a composite register DFT, built at compile time.
+struct RpTab {
+    float c[64];
+    float s[64];
+};
+template <int R>
+__host__ __device__ constexpr RpTab rp_tab() {
+    RpTab t{};
+    for (int i = 0; i < R && i < 64; i++) {
+        t.c[i] = (float)rp_cos2pi(i, R);
+        t.s[i] = (float)rp_sin2pi(i, R);
+    }
+    return t;
+}
+
+// R = A * B: A-point DFTs over n1 (stride B), twiddle W_R^{n2 k1},
+// B-point DFTs over n2; output k1 + A k2.
+template <int R>
+struct RegDft<R, false> {
+    static_assert(R <= 64, "composite register DFTs up to 64 points");
+    static __device__ __forceinline__ void run(cfloat* v) {
+        constexpr int A = rp_first(R), B = R / A;
+        constexpr RpTab TW = rp_tab<R>();
+        cfloat w[R];
+#pragma unroll
+        for (int n2 = 0; n2 < B; n2++) {
+            cfloat s[A];
+#pragma unroll
+            for (int n1 = 0; n1 < A; n1++) s[n1] = v[n2 + B * n1];
+            RegDft<A>::run(s);
+#pragma unroll
+            for (int k1 = 0; k1 < A; k1++) {
+                if (n2 == 0 || k1 == 0) {
+                    w[n2 * A + k1] = s[k1];
+                } else {
+                    const float c = TW.c[(n2 * k1) % R];
+                    const float sn = TW.s[(n2 * k1) % R];
+                    // multiply by e^{-i theta} = (c, -sn)
+                    w[n2 * A + k1] = cmk(s[k1].x * c + s[k1].y * sn, s[k1].y * c - s[k1].x * sn);
+                }
+            }
+        }
+#pragma unroll
+        for (int k1 = 0; k1 < A; k1++) {
+            cfloat s[B];
+#pragma unroll
+            for (int n2 = 0; n2 < B; n2++) s[n2] = w[n2 * A + k1];
+            RegDft<B>::run(s);
+#pragma unroll
+            for (int k2 = 0; k2 < B; k2++) v[k1 + A * k2] = s[k2];
+        }
+    }
+};
+
+
 __host__ __device__ constexpr int ct_next(int n) {
     return n % 8 == 0 ? 8 : (n % 4 == 0 ? 4 : (n % 2 == 0 ? 2 : ct_spf(n)));
 }
@@ -289,11 +396,163 @@ static __device__ __forceinline__ void ct_stage_bigsym(cfloat* lds, int base, in
     __syncthreads();
 }
 
+// ---- Rader stage for the prime radix above 31 ----
+// y_m = sum_t x_t w^{tm} (w = e^{-2 pi i/P}) over the P inputs of a
+// butterfly.  With a primitive root g, t = g^q and m = g^-p (q, p < L =
+// P - 1) turn the non-zero part into a cyclic convolution:
+// y_{g^-p} = x_0 + c_p, c = a (*) b, a_q = x_{g^q}, b_n = w^{g^-n}, and
+// y_0 = x_0 + sum_q a_q.  The convolution is done by length-L DFTs,
+// L = L1 * L2 (Cooley-Tukey, L2 the largest prime factor): forward
+// A[k1 + L1 k2] = DFT_L2 over n2 of (w_L^{n2 k1} DFT_L1 over n1 of
+// a[L2 n1 + n2]), product with Bh = DFT_L(b) / L, then the inverse steps in
+// reverse order, so c comes out in natural order.  Work per butterfly is
+// O(L (L1 + L2)) instead of the direct form's O(P^2).
+__host__ __device__ constexpr int ct_powmod(int g, int e, int p) {
+    long long r = 1, b = g % p;
+    while (e > 0) {
+        if (e & 1) r = (r * b) % p;
+        b = (b * b) % p;
+        e >>= 1;
+    }
+    return (int)r;
+}
+__host__ __device__ constexpr int ct_proot(int p) {
+    for (int g = 2; g < p; g++) {
+        bool ok = true;
+        for (int d = 2; d <= p - 1 && ok; d++)
+            if ((p - 1) % d == 0 && ct_spf(d) == d && ct_powmod(g, (p - 1) / d, p) == 1) ok = false;
+        if (ok) return g;
+    }
+    return 0;
+}
+__host__ __device__ constexpr int ct_lpf(int n) {
+    int l = 1;
+    for (int f = 2; f <= n; f++)
+        while (n % f == 0) {
+            l = f;
+            n /= f;
+        }
+    return l;
+}
+// L2 and L1 of the convolution length of prime P (0 when the stage has no
+// Rader form: L2 a prime of at most 31 and L1 at most 32 are compiled)
+__host__ __device__ constexpr int ct_rd_l2(int p) {
+    return (ct_lpf(p - 1) <= 31 && (p - 1) / ct_lpf(p - 1) <= 32) ? ct_lpf(p - 1) : 0;
+}
+__host__ __device__ constexpr int ct_rd_l1(int p) { return ct_rd_l2(p) ? (p - 1) / ct_rd_l2(p) : 0; }
+// Rader table after the chunk root table: Bh[k] (k < L), then w_L^{n2 k1}
+// at n2 * L1 + k1
+__host__ __device__ constexpr int ct_rd_off(int n) { return ct_rt_off(n) + ct_rt_elems(n); }
+__host__ __device__ constexpr int ct_rd_elems(int n) {
+    return ct_big_prime(n) && ct_rd_l2(ct_big_prime(n)) ? 2 * (ct_big_prime(n) - 1) : 0;
+}
+// whether the big-prime stage of N runs in the Rader form with TPT threads
+// per transform (one item per thread in each of its three steps)
+__host__ __device__ constexpr bool ct_rader(int n, int tpt) {
+    const int p = ct_big_prime(n);
+    return p && ct_rd_l2(p) && (n / p) * ct_rd_l2(p) <= tpt && (n / p) * ct_rd_l1(p) <= tpt;
+}
+
+template <int N, int S, int TPT>
+static __device__ __forceinline__ void ct_stage_rader(cfloat* lds, int base, int tid,
+                                                      const cfloat* __restrict__ tw) {
+    constexpr int P = ct_radix(N, S), NS = ct_ns(N, S), M = N / P, L = P - 1;
+    constexpr int L2 = ct_rd_l2(P), L1 = ct_rd_l1(P);
+    constexpr int G = ct_proot(P), GI = ct_powmod(G, P - 2, P);
+    constexpr int GS = ct_powmod(G, L2, P), GIS = ct_powmod(GI, L2, P);
+    constexpr int TWO = ct_twoff(N, S), RD = ct_rd_off(N);
+    static_assert(ct_rader(N, TPT), "Rader form not available for this length");
+    static_assert(M * L + M == N, "workspace and x0 stash fill the transform's range");
+    const cfloat* Bh = tw + RD;
+    const cfloat* wl = tw + RD + L;
+    // step 1: item (j, n2) gathers a[L2 n1 + n2] = x_{g^(L2 n1 + n2)},
+    // transforms over n1 and twiddles by w_L^{n2 k1}
+    const bool i1 = tid < M * L2;
+    const int j1 = tid / L2, n21 = tid - (tid / L2) * L2;
+    cfloat v1[L1];
+    cfloat x0 = cmk(0.f, 0.f);
+    if (i1) {
+        const int k = j1 % NS;
+        int t = 1;
+        for (int e = 0; e < n21; e++) t = (t * G) % P;
+#pragma unroll
+        for (int n1 = 0; n1 < L1; n1++) {
+            cfloat x = lds[lp(base + j1 + t * M)];
+            if (NS > 1) x = cmul(x, tw[TWO + k * (P - 1) + t - 1]);
+            v1[n1] = x;
+            t = (t * GS) % P;
+        }
+        if (n21 == 0) x0 = lds[lp(base + j1)];
+        RegDft<L1>::run(v1);
+#pragma unroll
+        for (int k1 = 1; k1 < L1; k1++) v1[k1] = cmul(v1[k1], wl[n21 * L1 + k1]);
+    }
+    __syncthreads();
+    if (i1) {
+        // workspace W[j][k1][n2] at base + j L + k1 L2 + n2
+#pragma unroll
+        for (int k1 = 0; k1 < L1; k1++) lds[lp(base + j1 * L + k1 * L2 + n21)] = v1[k1];
+        if (n21 == 0) lds[lp(base + M * L + j1)] = x0;
+    }
+    __syncthreads();
+    // step 2: item (j, k1) transforms over n2, multiplies by Bh, transforms
+    // back over k2 and applies the conjugate twiddle; (j, 0) also forms y_0
+    const bool i2 = tid < M * L1;
+    const int j2 = tid / L1, k12 = tid - (tid / L1) * L1;
+    cfloat y0 = cmk(0.f, 0.f);
+    if (i2) {
+        cfloat v2[L2];
+        const int o = base + j2 * L + k12 * L2;
+#pragma unroll
+        for (int n2 = 0; n2 < L2; n2++) v2[n2] = lds[lp(o + n2)];
+        RegDft<L2>::run(v2);
+        if (k12 == 0) y0 = cadd(lds[lp(base + M * L + j2)], v2[0]);
+#pragma unroll
+        for (int k2 = 0; k2 < L2; k2++) v2[k2] = cconj(cmul(v2[k2], Bh[k12 + L1 * k2]));
+        RegDft<L2>::run(v2);
+#pragma unroll
+        for (int n2 = 0; n2 < L2; n2++) {
+            cfloat z = cconj(v2[n2]);
+            if (n2 > 0 && k12 > 0) z = cmul(z, cconj(wl[n2 * L1 + k12]));
+            lds[lp(o + n2)] = z;
+        }
+    }
+    __syncthreads();
+    // step 3: item (j, n2) transforms back over k1: c[L2 n1 + n2], which is
+    // output g^-(L2 n1 + n2), plus x_0
+    cfloat v3[L1];
+    cfloat x03 = cmk(0.f, 0.f);
+    if (i1) {
+#pragma unroll
+        for (int k1 = 0; k1 < L1; k1++) v3[k1] = cconj(lds[lp(base + j1 * L + k1 * L2 + n21)]);
+        x03 = lds[lp(base + M * L + j1)];
+        RegDft<L1>::run(v3);
+    }
+    __syncthreads();
+    if (i1) {
+        const int q = j1 / NS, k = j1 - q * NS;
+        const int d = base + q * NS * P + k;
+        int m = 1;
+        for (int e = 0; e < n21; e++) m = (m * GI) % P;
+#pragma unroll
+        for (int n1 = 0; n1 < L1; n1++) {
+            lds[lp(d + m * NS)] = cadd(x03, cconj(v3[n1]));
+            m = (m * GIS) % P;
+        }
+    }
+    if (i2 && k12 == 0) {
+        const int q = j2 / NS, k = j2 - q * NS;
+        lds[lp(base + q * NS * P + k)] = y0;
+    }
+    __syncthreads();
+}
+
 template <int N, int S, int TPT>
 struct CtFft {
     static __device__ __forceinline__ void run(cfloat* lds, int base, int tid, const cfloat* tw) {
         if constexpr (S < ct_nst(N)) {
-            if constexpr (ct_radix(N, S) > 31) ct_stage_bigsym<N, S, TPT>(lds, base, tid, tw);
+            if constexpr (ct_radix(N, S) > 31 && ct_rader(N, TPT)) ct_stage_rader<N, S, TPT>(lds, base, tid, tw);
+            else if constexpr (ct_radix(N, S) > 31) ct_stage_bigsym<N, S, TPT>(lds, base, tid, tw);
             else ct_stage<N, S, TPT>(lds, base, tid, tw);
             CtFft<N, S + 1, TPT>::run(lds, base, tid, tw);
         }

@@ -20,112 +20,6 @@
 #include <vector>
 #include "mc_fft_ct.h"
 
-// ---- compile-time trigonometry (Taylor series in double, |x| <= pi) ----
-__host__ __device__ constexpr double rp_pi() { return 3.14159265358979323846; }
-__host__ __device__ constexpr double rp_sin_red(double x) {
-    double term = x, sum = x;
-    for (int n = 1; n < 30; n++) {
-        term *= -x * x / ((2.0 * n) * (2.0 * n + 1.0));
-        sum += term;
-    }
-    return sum;
-}
-__host__ __device__ constexpr double rp_cos_red(double x) {
-    double term = 1.0, sum = 1.0;
-    for (int n = 1; n < 30; n++) {
-        term *= -x * x / ((2.0 * n - 1.0) * (2.0 * n));
-        sum += term;
-    }
-    return sum;
-}
-// cos and sin of 2 pi a / b for integers, argument reduced to [-pi, pi]
-__host__ __device__ constexpr double rp_cos2pi(int a, int b) {
-    int r = a % b;
-    if (r < 0) r += b;
-    if (2 * r > b) r -= b;
-    return rp_cos_red(2.0 * rp_pi() * r / b);
-}
-__host__ __device__ constexpr double rp_sin2pi(int a, int b) {
-    int r = a % b;
-    if (r < 0) r += b;
-    if (2 * r > b) r -= b;
-    return rp_sin_red(2.0 * rp_pi() * r / b);
-}
-
-// ---- composite register DFT: v[k] = sum_t v[t] e^{-2 pi i t k / R} ----
-__host__ __device__ constexpr int rp_first(int r) {
-    // leading factor of a composite register DFT: 4 or 2 for even sizes
-    // (small butterflies), else the smallest odd prime
-    return r % 4 == 0 && r != 4 && r != 8 ? 4 : (r % 2 == 0 && r != 2 && r != 4 && r != 8 ? 2 : ct_spf(r));
-}
-__host__ __device__ constexpr bool rp_leaf(int r) {
-    return r == 1 || r == 2 || r == 4 || r == 8 || (ct_spf(r) == r && r <= 31);
-}
-
-template <int R, bool LEAF = rp_leaf(R)>
-struct RegDft;
-
-template <int R>
-struct RegDft<R, true> {
-    static __device__ __forceinline__ void run(cfloat* v) {
-        if constexpr (R > 1) bfly<R>(v);
-    }
-};
-
-// cos/sin(2 pi i / R) for a composite register DFT, built at compile time.
-struct RpTab {
-    float c[64];
-    float s[64];
-};
-template <int R>
-__host__ __device__ constexpr RpTab rp_tab() {
-    RpTab t{};
-    for (int i = 0; i < R && i < 64; i++) {
-        t.c[i] = (float)rp_cos2pi(i, R);
-        t.s[i] = (float)rp_sin2pi(i, R);
-    }
-    return t;
-}
-
-// R = A * B: A-point DFTs over n1 (stride B), twiddle W_R^{n2 k1},
-// B-point DFTs over n2; output k1 + A k2.
-template <int R>
-struct RegDft<R, false> {
-    static_assert(R <= 64, "composite register DFTs up to 64 points");
-    static __device__ __forceinline__ void run(cfloat* v) {
-        constexpr int A = rp_first(R), B = R / A;
-        constexpr RpTab TW = rp_tab<R>();
-        cfloat w[R];
-#pragma unroll
-        for (int n2 = 0; n2 < B; n2++) {
-            cfloat s[A];
-#pragma unroll
-            for (int n1 = 0; n1 < A; n1++) s[n1] = v[n2 + B * n1];
-            RegDft<A>::run(s);
-#pragma unroll
-            for (int k1 = 0; k1 < A; k1++) {
-                if (n2 == 0 || k1 == 0) {
-                    w[n2 * A + k1] = s[k1];
-                } else {
-                    const float c = TW.c[(n2 * k1) % R];
-                    const float sn = TW.s[(n2 * k1) % R];
-                    // multiply by e^{-i theta} = (c, -sn)
-                    w[n2 * A + k1] = cmk(s[k1].x * c + s[k1].y * sn, s[k1].y * c - s[k1].x * sn);
-                }
-            }
-        }
-#pragma unroll
-        for (int k1 = 0; k1 < A; k1++) {
-            cfloat s[B];
-#pragma unroll
-            for (int n2 = 0; n2 < B; n2++) s[n2] = w[n2 * A + k1];
-            RegDft<B>::run(s);
-#pragma unroll
-            for (int k2 = 0; k2 < B; k2++) v[k1 + A * k2] = s[k2];
-        }
-    }
-};
-
 // ---- pass schedules ----
 // Radix of pass p for the lengths with a register-pass schedule (0 = none).
 __host__ __device__ constexpr int rp_radix(int n, int p) {
@@ -175,6 +69,10 @@ __host__ __device__ constexpr bool rp_pad(int n) {
 }
 // LDS elements of one transform (the column / row stride in LDS)
 __host__ __device__ constexpr int rp_ld(int n) { return rp_pad(n) ? n + n / 16 : n; }
+// LDS stride between the C transforms of a workgroup: odd in 8-byte units
+// when C > 1, so that lanes that step over transforms (the column-fastest
+// item order of the column passes) fall on different banks.
+__host__ __device__ constexpr int rp_cs(int n, int c) { return c > 1 ? (rp_ld(n) | 1) : rp_ld(n); }
 template <int N>
 static __device__ __forceinline__ int rp_ix(int i) {
     if constexpr (rp_pad(N)) return i + (i >> 4);
@@ -191,12 +89,12 @@ static __device__ __forceinline__ void rp_twiddle(cfloat* v, int k, const cfloat
     }
 }
 
-// A middle pass over the C columns held in LDS at c * rp_ld(N), NT threads;
+// A middle pass over the C columns held in LDS at c * rp_cs(N, C), NT threads;
 // item i -> column i / M, butterfly i % M.  Ends with a barrier.
 template <int N, int P, int C, int NT>
 static __device__ __forceinline__ void rp_pass_lds(cfloat* lds, const cfloat* __restrict__ tw) {
     constexpr int R = rp_radix(N, P), NS = rp_ns(N, P), M = N / R;
-    constexpr int IT = (C * M + NT - 1) / NT, LD = rp_ld(N);
+    constexpr int IT = (C * M + NT - 1) / NT, LD = rp_cs(N, C);
     cfloat v[IT][R];
     int dcol[IT], dst[IT];
 #pragma unroll
