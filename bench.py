@@ -203,6 +203,11 @@ def main():
                     help="also time rendering a synthetic EER movie of K fractions (4096^2 camera, "
                          "about 100k electrons per fraction, 10 fractions per frame) with "
                          "mc_load_eer from pinned host memory")
+    ap.add_argument("--mixed", type=int, default=0, metavar="K",
+                    help="also time K movies of uneven size in one mc_align_movies call "
+                         "(alternately 5760x4092x50 and 4096x4096x40, same flags): with their "
+                         "alignment iterations batched over one job table per stage, and "
+                         "pipelined size by size")
     ap.add_argument("--prior", type=int, default=0, metavar="K",
                     help="train the learned shift prior on K synthetic movies of the workload "
                          "(labels from the plain path), then time the guided path")
@@ -396,6 +401,32 @@ def main():
             hk.free()
         del bufs
 
+    mixed = None
+    if args.mixed > 0:
+        # movies of two sizes; the 4096x4096 ones get 40 frames
+        shapes = [(W, H, N) if k % 2 == 0 else (4096, 4096, 40) for k in range(args.mixed)]
+        mv = []
+        for k, (w, h, n) in enumerate(shapes):
+            fk = ctx.empty((n, h, w), np.float32)
+            ctx.synth_render(fk, MovieRecipe(w, h, n, 20000, seed=20250301 + k, dose=1.0, local_max=3.0))
+            mv.append(fk)
+        mixed = {"movies": [list(sh) for sh in shapes]}
+        for name, pipe in (("batched", 0), ("pipelined", 1)):
+            pm = L.default_params(**dict(kw, movie_pipeline=pipe))
+            ctx.align_movies(mv, pm)  # warm-up (plans, buffers, graphs)
+            ctx.sync()
+            tm = time.perf_counter()
+            reps = 3
+            for _ in range(reps):
+                ctx.align_movies(mv, pm)
+            ctx.sync()
+            dt = (time.perf_counter() - tm) / reps
+            mixed[name] = {"movies_per_s": len(mv) / dt, "ms_per_call": dt * 1000.0}
+        mixed["note"] = ("batched: each alignment iteration stage is one launch over the jobs of "
+                         "both sizes (one host thread per size); pipelined: the sizes one after "
+                         "another, each movie's forward transform beside the previous alignment")
+        del mv
+
     eer = None
     if args.eer > 0:
         from motioncor3_amd.synth import eer_stream_7bit
@@ -498,6 +529,7 @@ def main():
             "path_achieved": pb / (ms * 1e-3) / 1e9,
             "path_frac": pb / (ms * 1e-3) / 1e9 / PEAK_HBM_GBS,
         },
+        "stages_ms": {k: v * 1000.0 for k, v in res["seconds_stage"].items()},
         "accuracy": {"global_shift_rms_vs_truth_px": truth_rms,
                      "patch_shift_rms_vs_truth_px": patch_truth_rms,
                      "global_shift_rms_vs_drift_only_px": truth_rms_drift_only,
@@ -507,6 +539,8 @@ def main():
         out["ingest"] = ingest
     if eer is not None:
         out["eer"] = eer
+    if mixed is not None:
+        out["mixed"] = mixed
     if prior_info is not None:
         prior_info["fallbacks_last_step"] = res["prior_fallbacks"]
         prior_info["patch_iters_mean_last_step"] = float(np.mean(res["patch_iters"]))

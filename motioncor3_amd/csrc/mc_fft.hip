@@ -581,7 +581,7 @@ __global__ __launch_bounds__(NT) void k_cols_inv_ct(ColInvArgs a) {
 // patches 576x408; the 128^2 feature map.
 #ifndef MC_ROW_SPECS
 #define MC_ROW_SPECS(X) \
-    X(2880, 2, MC_TILE_CT) X(2048, 2, MC_TILE_CT) X(512, 2, 8) X(1441, 2, MC_TILE_CT) X(1024, 2, 8) X(288, 2, 16) X(64, 2, 16)
+    X(2880, 2, MC_TILE_CT) X(2048, 2, MC_TILE_CT) X(512, 2, 8) X(1441, 4, MC_TILE_CT) X(1024, 2, 8) X(288, 2, 16) X(64, 2, 16)
 #endif
 #ifndef MC_COL_SPECS
 #define MC_COL_SPECS(X) \
@@ -946,7 +946,7 @@ __global__ __launch_bounds__(NT) void k_rows_c2r_rp(RowC2RArgs a, const cfloat* 
 // entry of a length is its default, later ones are autotuner variants.
 #define MC_RP_ROW_SPECS(X) X(2880, 2, MC_TILE_CT, 512) X(2048, 2, MC_TILE_CT, 512) X(288, 16, 16, 512) \
     X(5760, 1, MC_TILE_CT, 512) X(2880, 1, MC_TILE_CT, 256) X(2880, 2, MC_TILE_CT, 384) \
-    X(2880, 1, MC_TILE_CT, 192) X(288, 8, 16, 256)
+    X(2880, 1, MC_TILE_CT, 192) X(288, 8, 16, 256) X(2880, 4, MC_TILE_CT, 1024) X(2880, 4, MC_TILE_CT, 768)
 #endif
 
 
@@ -1176,10 +1176,11 @@ int mc_fft2d_create(Fft2D* p, int W, int H) {
     int R = 1;
     const int want = std::max((MC_TILE_RC + C - 1) / C, 2);
     const int per_row = m + 1;
-    // lengths with a prime factor above 31 (the 1441-point xcf rows): two
-    // rows per workgroup, measured faster than four on MI355X (more waves
-    // per CU for the direct big-prime stage)
-    const int rmax = ct_big_prime(m) ? 2 : want;
+    // a prime factor above 31 without a Rader form: two rows per workgroup
+    // (more waves per CU for the direct big-prime stage); with the Rader
+    // form (the 1441-point xcf rows) four, so that a row pass reads 128
+    // contiguous bytes of each staging chunk
+    const int rmax = (ct_big_prime(m) && !ct_rader(m)) ? 2 : want;
     while (R * 2 <= rmax && (R * 2) * per_row <= budget) R *= 2;
     if (R * per_row > MC_LDS_ELEMS) return -1;
     p->R = R;

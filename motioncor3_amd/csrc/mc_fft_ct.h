@@ -446,11 +446,10 @@ __host__ __device__ constexpr int ct_rd_off(int n) { return ct_rt_off(n) + ct_rt
 __host__ __device__ constexpr int ct_rd_elems(int n) {
     return ct_big_prime(n) && ct_rd_l2(ct_big_prime(n)) ? 2 * (ct_big_prime(n) - 1) : 0;
 }
-// whether the big-prime stage of N runs in the Rader form with TPT threads
-// per transform (one item per thread in each of its three steps)
-__host__ __device__ constexpr bool ct_rader(int n, int tpt) {
+// whether the big-prime stage of N runs in the Rader form
+__host__ __device__ constexpr bool ct_rader(int n) {
     const int p = ct_big_prime(n);
-    return p && ct_rd_l2(p) && (n / p) * ct_rd_l2(p) <= tpt && (n / p) * ct_rd_l1(p) <= tpt;
+    return p && ct_rd_l2(p) && (n / p) * (p - 1) + n / p == n;
 }
 
 template <int N, int S, int TPT>
@@ -461,88 +460,120 @@ static __device__ __forceinline__ void ct_stage_rader(cfloat* lds, int base, int
     constexpr int G = ct_proot(P), GI = ct_powmod(G, P - 2, P);
     constexpr int GS = ct_powmod(G, L2, P), GIS = ct_powmod(GI, L2, P);
     constexpr int TWO = ct_twoff(N, S), RD = ct_rd_off(N);
-    static_assert(ct_rader(N, TPT), "Rader form not available for this length");
-    static_assert(M * L + M == N, "workspace and x0 stash fill the transform's range");
+    constexpr int I1 = M * L2, I2 = M * L1;  // items of steps 1 and 3, and of step 2
+    constexpr int IT1 = (I1 + TPT - 1) / TPT, IT2 = (I2 + TPT - 1) / TPT;
+    static_assert(ct_rader(N), "Rader form not available for this length");
     const cfloat* Bh = tw + RD;
     const cfloat* wl = tw + RD + L;
     // step 1: item (j, n2) gathers a[L2 n1 + n2] = x_{g^(L2 n1 + n2)},
-    // transforms over n1 and twiddles by w_L^{n2 k1}
-    const bool i1 = tid < M * L2;
-    const int j1 = tid / L2, n21 = tid - (tid / L2) * L2;
-    cfloat v1[L1];
-    cfloat x0 = cmk(0.f, 0.f);
-    if (i1) {
-        const int k = j1 % NS;
-        int t = 1;
-        for (int e = 0; e < n21; e++) t = (t * G) % P;
+    // transforms over n1 and twiddles by w_L^{n2 k1}; the workspace
+    // W[j][k1][n2] (base + j L + k1 L2 + n2) and the x_0 stash (base + M L + j)
+    // overwrite the stage's input only after everything is read
+    cfloat v1[IT1][L1];
+    cfloat x0[IT1];
 #pragma unroll
-        for (int n1 = 0; n1 < L1; n1++) {
-            cfloat x = lds[lp(base + j1 + t * M)];
-            if (NS > 1) x = cmul(x, tw[TWO + k * (P - 1) + t - 1]);
-            v1[n1] = x;
-            t = (t * GS) % P;
+    for (int it = 0; it < IT1; it++) {
+        const int i = tid + it * TPT;
+        x0[it] = cmk(0.f, 0.f);
+        if (I1 % TPT == 0 || i < I1) {
+            const int j = i / L2, n2 = i - j * L2, k = j % NS;
+            int t = 1;
+            for (int e = 0; e < n2; e++) t = (t * G) % P;
+#pragma unroll
+            for (int n1 = 0; n1 < L1; n1++) {
+                cfloat x = lds[lp(base + j + t * M)];
+                if (NS > 1) x = cmul(x, tw[TWO + k * (P - 1) + t - 1]);
+                v1[it][n1] = x;
+                t = (t * GS) % P;
+            }
+            if (n2 == 0) x0[it] = lds[lp(base + j)];
+            RegDft<L1>::run(v1[it]);
+#pragma unroll
+            for (int k1 = 1; k1 < L1; k1++) v1[it][k1] = cmul(v1[it][k1], wl[n2 * L1 + k1]);
         }
-        if (n21 == 0) x0 = lds[lp(base + j1)];
-        RegDft<L1>::run(v1);
-#pragma unroll
-        for (int k1 = 1; k1 < L1; k1++) v1[k1] = cmul(v1[k1], wl[n21 * L1 + k1]);
     }
     __syncthreads();
-    if (i1) {
-        // workspace W[j][k1][n2] at base + j L + k1 L2 + n2
 #pragma unroll
-        for (int k1 = 0; k1 < L1; k1++) lds[lp(base + j1 * L + k1 * L2 + n21)] = v1[k1];
-        if (n21 == 0) lds[lp(base + M * L + j1)] = x0;
+    for (int it = 0; it < IT1; it++) {
+        const int i = tid + it * TPT;
+        if (I1 % TPT == 0 || i < I1) {
+            const int j = i / L2, n2 = i - j * L2;
+#pragma unroll
+            for (int k1 = 0; k1 < L1; k1++) lds[lp(base + j * L + k1 * L2 + n2)] = v1[it][k1];
+            if (n2 == 0) lds[lp(base + M * L + j)] = x0[it];
+        }
     }
     __syncthreads();
     // step 2: item (j, k1) transforms over n2, multiplies by Bh, transforms
-    // back over k2 and applies the conjugate twiddle; (j, 0) also forms y_0
-    const bool i2 = tid < M * L1;
-    const int j2 = tid / L1, k12 = tid - (tid / L1) * L1;
-    cfloat y0 = cmk(0.f, 0.f);
-    if (i2) {
-        cfloat v2[L2];
-        const int o = base + j2 * L + k12 * L2;
+    // back over k2 and applies the conjugate twiddle in place; (j, 0) also
+    // forms y_0 = x_0 + sum_q a_q, written after step 3
+    cfloat y0[IT2];
 #pragma unroll
-        for (int n2 = 0; n2 < L2; n2++) v2[n2] = lds[lp(o + n2)];
-        RegDft<L2>::run(v2);
-        if (k12 == 0) y0 = cadd(lds[lp(base + M * L + j2)], v2[0]);
+    for (int it = 0; it < IT2; it++) {
+        const int i = tid + it * TPT;
+        y0[it] = cmk(0.f, 0.f);
+        if (I2 % TPT == 0 || i < I2) {
+            const int j = i / L1, k1 = i - j * L1;
+            cfloat v2[L2];
+            const int o = base + j * L + k1 * L2;
 #pragma unroll
-        for (int k2 = 0; k2 < L2; k2++) v2[k2] = cconj(cmul(v2[k2], Bh[k12 + L1 * k2]));
-        RegDft<L2>::run(v2);
+            for (int n2 = 0; n2 < L2; n2++) v2[n2] = lds[lp(o + n2)];
+            RegDft<L2>::run(v2);
+            if (k1 == 0) y0[it] = cadd(lds[lp(base + M * L + j)], v2[0]);
 #pragma unroll
-        for (int n2 = 0; n2 < L2; n2++) {
-            cfloat z = cconj(v2[n2]);
-            if (n2 > 0 && k12 > 0) z = cmul(z, cconj(wl[n2 * L1 + k12]));
-            lds[lp(o + n2)] = z;
+            for (int k2 = 0; k2 < L2; k2++) v2[k2] = cconj(cmul(v2[k2], Bh[k1 + L1 * k2]));
+            RegDft<L2>::run(v2);
+#pragma unroll
+            for (int n2 = 0; n2 < L2; n2++) {
+                cfloat z = cconj(v2[n2]);
+                if (n2 > 0 && k1 > 0) z = cmul(z, cconj(wl[n2 * L1 + k1]));
+                lds[lp(o + n2)] = z;
+            }
         }
     }
     __syncthreads();
     // step 3: item (j, n2) transforms back over k1: c[L2 n1 + n2], which is
     // output g^-(L2 n1 + n2), plus x_0
-    cfloat v3[L1];
-    cfloat x03 = cmk(0.f, 0.f);
-    if (i1) {
+    cfloat v3[IT1][L1];
 #pragma unroll
-        for (int k1 = 0; k1 < L1; k1++) v3[k1] = cconj(lds[lp(base + j1 * L + k1 * L2 + n21)]);
-        x03 = lds[lp(base + M * L + j1)];
-        RegDft<L1>::run(v3);
-    }
-    __syncthreads();
-    if (i1) {
-        const int q = j1 / NS, k = j1 - q * NS;
-        const int d = base + q * NS * P + k;
-        int m = 1;
-        for (int e = 0; e < n21; e++) m = (m * GI) % P;
+    for (int it = 0; it < IT1; it++) {
+        const int i = tid + it * TPT;
+        x0[it] = cmk(0.f, 0.f);
+        if (I1 % TPT == 0 || i < I1) {
+            const int j = i / L2, n2 = i - j * L2;
 #pragma unroll
-        for (int n1 = 0; n1 < L1; n1++) {
-            lds[lp(d + m * NS)] = cadd(x03, cconj(v3[n1]));
-            m = (m * GIS) % P;
+            for (int k1 = 0; k1 < L1; k1++) v3[it][k1] = cconj(lds[lp(base + j * L + k1 * L2 + n2)]);
+            x0[it] = lds[lp(base + M * L + j)];
+            RegDft<L1>::run(v3[it]);
         }
     }
-    if (i2 && k12 == 0) {
-        const int q = j2 / NS, k = j2 - q * NS;
-        lds[lp(base + q * NS * P + k)] = y0;
+    __syncthreads();
+#pragma unroll
+    for (int it = 0; it < IT1; it++) {
+        const int i = tid + it * TPT;
+        if (I1 % TPT == 0 || i < I1) {
+            const int j = i / L2, n2 = i - j * L2;
+            const int q = j / NS, k = j - q * NS;
+            const int d = base + q * NS * P + k;
+            int m = 1;
+            for (int e = 0; e < n2; e++) m = (m * GI) % P;
+#pragma unroll
+            for (int n1 = 0; n1 < L1; n1++) {
+                lds[lp(d + m * NS)] = cadd(x0[it], cconj(v3[it][n1]));
+                m = (m * GIS) % P;
+            }
+        }
+    }
+#pragma unroll
+    for (int it = 0; it < IT2; it++) {
+        const int i = tid + it * TPT;
+        if (I2 % TPT == 0 || i < I2) {
+            const int j = i / L1, k1 = i - j * L1;
+            if (k1 == 0) {
+                const int q = j / NS, k = j - q * NS;
+                lds[lp(base + q * NS * P + k)] = y0[it];
+            }
+        }
     }
     __syncthreads();
 }
@@ -551,7 +582,7 @@ template <int N, int S, int TPT>
 struct CtFft {
     static __device__ __forceinline__ void run(cfloat* lds, int base, int tid, const cfloat* tw) {
         if constexpr (S < ct_nst(N)) {
-            if constexpr (ct_radix(N, S) > 31 && ct_rader(N, TPT)) ct_stage_rader<N, S, TPT>(lds, base, tid, tw);
+            if constexpr (ct_radix(N, S) > 31 && ct_rader(N)) ct_stage_rader<N, S, TPT>(lds, base, tid, tw);
             else if constexpr (ct_radix(N, S) > 31) ct_stage_bigsym<N, S, TPT>(lds, base, tid, tw);
             else ct_stage<N, S, TPT>(lds, base, tid, tw);
             CtFft<N, S + 1, TPT>::run(lds, base, tid, tw);
