@@ -192,6 +192,9 @@ def main():
     ap.add_argument("--copy-calib", type=int, default=0, metavar="MIB",
                     help="one plain device-to-device copy of MIB MiB before the steps, as the "
                          "known-size reference for hardware byte counters")
+    ap.add_argument("--no-radix", action="store_true",
+                    help="let the autotuner time tile variants of the default radix orders only "
+                         "(by default it also times the other compiled radix orders)")
     ap.add_argument("--no-autotune", action="store_true",
                     help="keep the default FFT tiles instead of timing the compiled variants "
                          "(mc_autotune) before the steps")
@@ -233,7 +236,7 @@ def main():
     sums = [ctx.empty((oh, ow), np.float32) for _ in range(ns)]
     tiles = None
     if not args.no_autotune:
-        rep = ctx.autotune(W, H, N, patch=(5, 5))
+        rep = ctx.autotune(W, H, N, patch=(5, 5), radix=not args.no_radix)
         tiles = [ln.strip() for ln in rep.splitlines() if "->" in ln]
     if args.copy_calib > 0:
         nb = args.copy_calib << 20
@@ -264,6 +267,9 @@ def main():
             X.append(rr["prior_features"])
             T.append(rr["prior_targets"].reshape(rr["prior_targets"].shape[0], -1))
             pit.append(rr["patch_iters"])
+            if (k + 1) % 250 == 0:
+                print("bench: prior training data %d/%d movies" % (k + 1, args.prior),
+                      file=sys.stderr, flush=True)
         del fx
         X, T = np.concatenate(X), np.concatenate(T)
         t_d = time.perf_counter()

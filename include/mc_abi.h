@@ -497,14 +497,21 @@ int mc_save_star(const char* path, const mc_params* p, int sx, int sy, int n_sta
  * kernels, Util/CCufft2D.cpp:23-53) ----
  * The register-pass FFT kernels have several compiled tile variants per
  * length (rows or columns per workgroup, threads), all bit-identical in
- * results.  mc_autotune times them on the batch shapes an alignment of a
- * W x H x N movie with px x py patches uses and keeps the fastest in the
- * context's plans; report (may be NULL) receives the timings as text.
- * mc_fft_variant queries (set < 0) or forces (set >= 0) the variant of the
- * W x H plan's rows (cols = 0) or columns (cols = 1). */
+ * results, and for some lengths a second radix order (the same transform up
+ * to FP32 rounding).  mc_autotune times the tile variants of each length's
+ * default radix order on the batch shapes an alignment of a W x H x N movie
+ * with px x py patches uses and keeps the fastest in the context's plans;
+ * mc_autotune_schedules with radix = 1 also times the other radix orders.
+ * report (may be NULL) receives the timings as text.  mc_fft_variant queries
+ * (set < 0) or forces (set >= 0) the variant of the W x H plan's rows
+ * (cols = 0) or columns (cols = 1); mc_fft_variant_name gives variant v's
+ * tile and radix order as text. */
 int mc_autotune(mc_ctx* ctx, int W, int H, int N, int px, int py, float ft_bin, char* report,
                 int report_len);
+int mc_autotune_schedules(mc_ctx* ctx, int W, int H, int N, int px, int py, float ft_bin, int radix,
+                          char* report, int report_len);
 int mc_fft_variant(mc_ctx* ctx, int W, int H, int cols, int set, int* count, int* current);
+int mc_fft_variant_name(mc_ctx* ctx, int W, int H, int cols, int v, char* name, int len);
 
 /* ---- learned shift prior (no reference counterpart) ----
  * An MLP in -> H -> H -> H -> out (ReLU) over rows of

@@ -347,8 +347,12 @@ __global__ __launch_bounds__(MC_NTHR) void k_correlate(CorrArgs a) {
 // spectrum of C columns is formed straight into the registers of pass 0, and
 // the last pass only evaluates the butterflies that own one of the seay
 // kept lags (y in [0, seay/2) and [H - seay/2, H)).
-template <int H, int C, int NT, int FF>
-__global__ __launch_bounds__(NT) void k_correlate_rp(CorrArgs a, const cfloat* __restrict__ ptw) {
+// WPE: waves per SIMD the tile is compiled for (a register budget of 512 /
+// WPE VGPRs): the 408-point tile is built for five, so that three of its
+// 52 KB workgroups share a CU (18 waves instead of 12), at the cost of a
+// few register spills.
+template <int H, int C, int NT, int FF, int WPE>
+__global__ __launch_bounds__(NT, WPE) void k_correlate_rp(CorrArgs a, const cfloat* __restrict__ ptw) {
     constexpr int NP = rp_npass(H), R0 = rp_radix(H, 0), M0 = H / R0;
     constexpr int RL = rp_radix(H, NP - 1), ML = H / RL;
     constexpr int IT0 = (C * M0 + NT - 1) / NT, ITL = (C * ML + NT - 1) / NT;
@@ -458,7 +462,8 @@ __global__ __launch_bounds__(NT) void k_correlate_rp(CorrArgs a, const cfloat* _
 }
 
 #ifndef MC_RP_CORR_SPECS
-#define MC_RP_CORR_SPECS(X) X(2048, 4, 512) X(408, 16, 384)
+// (column length, columns per workgroup, threads, waves per SIMD)
+#define MC_RP_CORR_SPECS(X) X(2048, 4, 512, 1) X(408, 16, 384, 5)
 #endif
 
 // Second half of the windowed C2R: for one lag row, the Hermitian sum over
@@ -1581,12 +1586,12 @@ int mc_align_init_constants() {
     const int bytes = (mc_lds_padded(MC_LDS_ELEMS) + 64) * (int)sizeof(cfloat);
     attr_ok &= hipSuccess == hipFuncSetAttribute((const void*)k_correlate, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
     attr_ok &= hipSuccess == hipFuncSetAttribute((const void*)k_finalize, hipFuncAttributeMaxDynamicSharedMemorySize, 65536);
-#define X(HH, CC, NT)                                                                     \
-    attr_ok &= hipSuccess == hipFuncSetAttribute((const void*)k_correlate_rp<HH, CC, NT, 0>,                      \
+#define X(HH, CC, NT, WPE)                                                                \
+    attr_ok &= hipSuccess == hipFuncSetAttribute((const void*)k_correlate_rp<HH, CC, NT, 0, WPE>,                 \
                         hipFuncAttributeMaxDynamicSharedMemorySize, bytes);              \
-    attr_ok &= hipSuccess == hipFuncSetAttribute((const void*)k_correlate_rp<HH, CC, NT, 1>,                      \
+    attr_ok &= hipSuccess == hipFuncSetAttribute((const void*)k_correlate_rp<HH, CC, NT, 1, WPE>,                 \
                         hipFuncAttributeMaxDynamicSharedMemorySize, bytes);              \
-    attr_ok &= hipSuccess == hipFuncSetAttribute((const void*)k_correlate_rp<HH, CC, NT, 2>,                      \
+    attr_ok &= hipSuccess == hipFuncSetAttribute((const void*)k_correlate_rp<HH, CC, NT, 2, WPE>,                 \
                         hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
     MC_RP_CORR_SPECS(X)
 #undef X
@@ -1678,17 +1683,17 @@ hipError_t mc_launch_correlate(const FftPlan1& colplan, const cfloat* ptw, int C
     // a job table brings its tile count (st.tiles, for mc_correlate_rp_cols(H))
     // and its widest job (st.cx_max, st.G_max)
     if (st.jobs && (st.cx_max < 1 || st.G_max < 1)) return hipErrorInvalidValue;
-#define X(HH, CC, NT)                                                                     \
+#define X(HH, CC, NT, WPE)                                                                \
     if (H == HH && ptw && seay <= HH) {                                                  \
         if (st.jobs && st.tiles < 1) return hipErrorInvalidValue;                        \
         const dim3 grid(st.jobs ? st.tiles : mc_div_up(cx, CC) * st.G * st.J);           \
         const size_t lds = (size_t)CC * rp_cs(HH, CC) * sizeof(cfloat);                  \
         if (f_fmt == MC_SPEC_F32)                                                        \
-            hipLaunchKernelGGL((k_correlate_rp<HH, CC, NT, 0>), grid, dim3(NT), lds, s, a, ptw); \
+            hipLaunchKernelGGL((k_correlate_rp<HH, CC, NT, 0, WPE>), grid, dim3(NT), lds, s, a, ptw); \
         else if (f_fmt == MC_SPEC_F16)                                                   \
-            hipLaunchKernelGGL((k_correlate_rp<HH, CC, NT, 1>), grid, dim3(NT), lds, s, a, ptw); \
+            hipLaunchKernelGGL((k_correlate_rp<HH, CC, NT, 1, WPE>), grid, dim3(NT), lds, s, a, ptw); \
         else if (f_fmt == MC_SPEC_BF16)                                                  \
-            hipLaunchKernelGGL((k_correlate_rp<HH, CC, NT, 2>), grid, dim3(NT), lds, s, a, ptw); \
+            hipLaunchKernelGGL((k_correlate_rp<HH, CC, NT, 2, WPE>), grid, dim3(NT), lds, s, a, ptw); \
         else                                                                             \
             return hipErrorInvalidValue;                                                 \
         return hipGetLastError();                                                        \
@@ -1706,7 +1711,7 @@ hipError_t mc_launch_correlate(const FftPlan1& colplan, const cfloat* ptw, int C
 }
 
 bool mc_correlate_has_rp(int H) {
-#define X(HH, CC, NT) \
+#define X(HH, CC, NT, WPE) \
     if (H == HH) return true;
     MC_RP_CORR_SPECS(X)
 #undef X
@@ -1714,7 +1719,7 @@ bool mc_correlate_has_rp(int H) {
 }
 
 int mc_correlate_rp_cols(int H) {
-#define X(HH, CC, NT) \
+#define X(HH, CC, NT, WPE) \
     if (H == HH) return CC;
     MC_RP_CORR_SPECS(X)
 #undef X

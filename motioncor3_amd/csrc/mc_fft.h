@@ -39,6 +39,8 @@ struct Fft2D {
     cfloat* col_tw = nullptr;
     cfloat* col_ptw = nullptr;  // register-pass twiddles of the column length (or null)
     cfloat* row_ptw = nullptr;  // register-pass twiddles of the row length m (or null)
+    cfloat* col_ptw1 = nullptr; // the same for the alternative radix order (schedule 1)
+    cfloat* row_ptw1 = nullptr;
     long long t_elems = 0;     // T elements per transform
     long long s_elems = 0;     // S elements per transform (cx * H)
     // tile variant of the register-pass row / column kernels (index among the
@@ -99,6 +101,7 @@ hipError_t mc_launch_spec_scale(const float* dc, int rows, int batch, float* g, 
 bool mc_fft_rp_rows(const Fft2D& p);
 bool mc_fft_rp_cols(const Fft2D& p);
 // Compiled tile variants of the plan's register-pass row (cols = 0) or
-// column (cols = 1) kernels, with a printable name each (0 when the plan has
-// no register-pass kernels on that side).
-int mc_fft_variants(const Fft2D& p, int cols, std::vector<std::string>* names);
+// column (cols = 1) kernels, with a printable name and the radix schedule
+// of each (0 when the plan has no register-pass kernels on that side).
+int mc_fft_variants(const Fft2D& p, int cols, std::vector<std::string>* names,
+                    std::vector<int>* scheds = nullptr);

@@ -602,12 +602,12 @@ __device__ __forceinline__ int crop_row(int y, int xcy, int H) {
     return (y > xcy / 2 + d) ? y - d : -1;
 }
 
-template <int H, int CW, int NT, int FS, int FX>
+template <int H, int CW, int NT, int FS, int FX, int SC>
 __global__ __launch_bounds__(NT) void k_cols_fwd_rp(ColFwdArgs a, const cfloat* __restrict__ ptw) {
     constexpr int C = CW;
-    constexpr int NP = rp_npass(H);
-    constexpr int R0 = rp_radix(H, 0), M0 = H / R0;
-    constexpr int RL = rp_radix(H, NP - 1), ML = H / RL;
+    constexpr int NP = rp_npass(H, SC);
+    constexpr int R0 = rp_radix(H, 0, SC), M0 = H / R0;
+    constexpr int RL = rp_radix(H, NP - 1, SC), ML = H / RL;
     constexpr int IT0 = (C * M0 + NT - 1) / NT, ITL = (C * ML + NT - 1) / NT;
     extern __shared__ cfloat lds[];
     const int CT = a.C;  // chunk width of the staging layout (a multiple of CW)
@@ -628,13 +628,13 @@ __global__ __launch_bounds__(NT) void k_cols_fwd_rp(ColFwdArgs a, const cfloat* 
 #pragma unroll
             for (int t = 0; t < R0; t++) v[t] = ok ? src[(j + t * M0) * CT + c] : cmk(0.f, 0.f);
             RegDft<R0>::run(v);
-            cfloat* d = lds + c * rp_cs(H, C);
+            cfloat* d = lds + c * rp_cs(H, C, SC);
 #pragma unroll
-            for (int t = 0; t < R0; t++) d[rp_ix<H>(j * R0 + t)] = v[t];
+            for (int t = 0; t < R0; t++) d[rp_ix<H, SC>(j * R0 + t)] = v[t];
         }
     }
     __syncthreads();
-    RpMid<H, 1, C, NT>::run(lds, ptw);
+    RpMid<H, 1, C, NT, SC>::run(lds, ptw);
     // last pass: LDS -> registers -> S (column-major) and the crop X
 #pragma unroll
     for (int it = 0; it < ITL; it++) {
@@ -643,10 +643,10 @@ __global__ __launch_bounds__(NT) void k_cols_fwd_rp(ColFwdArgs a, const cfloat* 
             const int c = i / ML, k = i - c * ML;
             if (c < nc) {
                 cfloat v[RL];
-                const cfloat* sl = lds + c * rp_cs(H, C);
+                const cfloat* sl = lds + c * rp_cs(H, C, SC);
 #pragma unroll
-                for (int t = 0; t < RL; t++) v[t] = sl[rp_ix<H>(k + t * ML)];
-                rp_twiddle<H, NP - 1>(v, k, ptw);
+                for (int t = 0; t < RL; t++) v[t] = sl[rp_ix<H, SC>(k + t * ML)];
+                rp_twiddle<H, NP - 1, SC>(v, k, ptw);
                 RegDft<RL>::run(v);
                 const int kx = c0 + c;
                 if (a.S) {
@@ -669,12 +669,12 @@ __global__ __launch_bounds__(NT) void k_cols_fwd_rp(ColFwdArgs a, const cfloat* 
     }
 }
 
-template <int H, int CW, int NT, int FS>
+template <int H, int CW, int NT, int FS, int SC>
 __global__ __launch_bounds__(NT) void k_cols_inv_rp(ColInvArgs a, const cfloat* __restrict__ ptw) {
     constexpr int C = CW;
-    constexpr int NP = rp_npass(H);
-    constexpr int R0 = rp_radix(H, 0), M0 = H / R0;
-    constexpr int RL = rp_radix(H, NP - 1), ML = H / RL;
+    constexpr int NP = rp_npass(H, SC);
+    constexpr int R0 = rp_radix(H, 0, SC), M0 = H / R0;
+    constexpr int RL = rp_radix(H, NP - 1, SC), ML = H / RL;
     constexpr int IT0 = (C * M0 + NT - 1) / NT, ITL = (C * ML + NT - 1) / NT;
     extern __shared__ cfloat lds[];
     const int CT = a.C;  // chunk width of the staging layout (a multiple of CW)
@@ -721,13 +721,13 @@ __global__ __launch_bounds__(NT) void k_cols_inv_rp(ColInvArgs a, const cfloat* 
                 for (int t = 0; t < R0; t++) v[t] = cmk(0.f, 0.f);
             }
             RegDft<R0>::run(v);
-            cfloat* d = lds + c * rp_cs(H, C);
+            cfloat* d = lds + c * rp_cs(H, C, SC);
 #pragma unroll
-            for (int t = 0; t < R0; t++) d[rp_ix<H>(j * R0 + t)] = v[t];
+            for (int t = 0; t < R0; t++) d[rp_ix<H, SC>(j * R0 + t)] = v[t];
         }
     }
     __syncthreads();
-    RpMid<H, 1, C, NT>::run(lds, ptw);
+    RpMid<H, 1, C, NT, SC>::run(lds, ptw);
     // last pass: LDS -> registers -> staging [y][c] (items: column fastest)
     cfloat* dst = a.T + (long long)b * a.t_bstride + (long long)kb * H * CT + coff;
 #pragma unroll
@@ -737,10 +737,10 @@ __global__ __launch_bounds__(NT) void k_cols_inv_rp(ColInvArgs a, const cfloat* 
             const int c = i % C, k = i / C;
             if (c < nc) {
                 cfloat v[RL];
-                const cfloat* sl = lds + c * rp_cs(H, C);
+                const cfloat* sl = lds + c * rp_cs(H, C, SC);
 #pragma unroll
-                for (int t = 0; t < RL; t++) v[t] = sl[rp_ix<H>(k + t * ML)];
-                rp_twiddle<H, NP - 1>(v, k, ptw);
+                for (int t = 0; t < RL; t++) v[t] = sl[rp_ix<H, SC>(k + t * ML)];
+                rp_twiddle<H, NP - 1, SC>(v, k, ptw);
                 RegDft<RL>::run(v);
 #pragma unroll
                 for (int t = 0; t < RL; t++) dst[(k + t * ML) * CT + c] = cconj(v[t]);
@@ -750,11 +750,13 @@ __global__ __launch_bounds__(NT) void k_cols_inv_rp(ColInvArgs a, const cfloat* 
 }
 
 #ifndef MC_RP_COL_SPECS
-// (height, columns per workgroup, threads).  The first entry of a height is
-// its default tile; later entries are variants that mc_autotune may choose.
-#define MC_RP_COL_SPECS(X) X(4092, 2, 512) X(4096, 2, 256) X(2048, 4, 512) X(408, 16, 384) X(8184, 1, 264) \
-    X(4092, 2, 264) X(4092, 2, 384) X(4092, 2, 448) X(4092, 1, 192) X(4092, 1, 256) X(4092, 4, 528) \
-    X(4092, 4, 1024) X(4096, 2, 384) X(4096, 2, 512) X(4096, 4, 512) X(2048, 2, 256) X(408, 8, 384)
+// (height, columns per workgroup, threads, radix schedule).  The first entry
+// of a height is its default tile; later entries are variants that
+// mc_autotune may choose (schedule 1: another radix order, rp_radix).
+#define MC_RP_COL_SPECS(X) X(4092, 2, 512, 0) X(4096, 2, 256, 0) X(2048, 4, 512, 0) X(408, 16, 384, 0) \
+    X(8184, 1, 264, 0) X(4092, 2, 264, 0) X(4092, 2, 384, 0) X(4092, 2, 448, 0) X(4092, 1, 192, 0) \
+    X(4092, 1, 256, 0) X(4092, 4, 528, 0) X(4092, 4, 1024, 0) X(4096, 2, 384, 0) X(4096, 2, 512, 0) \
+    X(4096, 4, 512, 0) X(2048, 2, 256, 0) X(408, 8, 384, 0) X(4092, 2, 512, 1) X(408, 16, 384, 1)
 #endif
 
 // ---------------------------------------------------------------------------
@@ -763,9 +765,9 @@ __global__ __launch_bounds__(NT) void k_cols_inv_rp(ColInvArgs a, const cfloat* 
 // (pass 0) -> LDS passes -> post-processing -> staging T.  C2R: staging T ->
 // LDS -> pre-processing -> LDS passes -> last pass -> real rows.
 
-template <int M, int RR, int C, int NT>
+template <int M, int RR, int C, int NT, int SC>
 __global__ __launch_bounds__(NT) void k_rows_r2c_rp(RowR2CArgs a, const cfloat* __restrict__ ptw) {
-    constexpr int NP = rp_npass(M), R0 = rp_radix(M, 0), M0 = M / R0;
+    constexpr int NP = rp_npass(M, SC), R0 = rp_radix(M, 0, SC), M0 = M / R0;
     constexpr int IT0 = (RR * M0 + NT - 1) / NT;
     constexpr int NB = (M + 1 + C - 1) / C;
     constexpr int TOT = NB * RR * C;
@@ -804,13 +806,13 @@ __global__ __launch_bounds__(NT) void k_rows_r2c_rp(RowR2CArgs a, const cfloat* 
                 for (int t = 0; t < R0; t++) v[t] = cmk(0.f, 0.f);
             }
             RegDft<R0>::run(v);
-            cfloat* d = lds + r * rp_cs(M, RR);
+            cfloat* d = lds + r * rp_cs(M, RR, SC);
 #pragma unroll
-            for (int t = 0; t < R0; t++) d[rp_ix<M>(j * R0 + t)] = v[t];
+            for (int t = 0; t < R0; t++) d[rp_ix<M, SC>(j * R0 + t)] = v[t];
         }
     }
     __syncthreads();
-    RpRange<M, 1, NP, RR, NT>::run(lds, ptw);
+    RpRange<M, 1, NP, RR, NT, SC>::run(lds, ptw);
     cfloat* tb = a.T + (long long)b * a.t_bstride + (long long)row0 * C;
     const long long kbs = (long long)a.H * C;
 #pragma unroll 4
@@ -818,8 +820,8 @@ __global__ __launch_bounds__(NT) void k_rows_r2c_rp(RowR2CArgs a, const cfloat* 
         const int c = e % C, r = (e / C) % RR, kb = e / (C * RR);
         const int k = kb * C + c;
         if (k > M || r >= nrows) continue;
-        const cfloat zk = lds[r * rp_cs(M, RR) + rp_ix<M>(k == M ? 0 : k)];
-        const cfloat zm = lds[r * rp_cs(M, RR) + rp_ix<M>(k == 0 ? 0 : M - k)];
+        const cfloat zk = lds[r * rp_cs(M, RR, SC) + rp_ix<M, SC>(k == M ? 0 : k)];
+        const cfloat zm = lds[r * rp_cs(M, RR, SC) + rp_ix<M, SC>(k == 0 ? 0 : M - k)];
         const cfloat E = cmk(0.5f * (zk.x + zm.x), 0.5f * (zk.y - zm.y));
         const cfloat D = cmk(zk.x - zm.x, zk.y + zm.y);
         const cfloat O = cmk(0.5f * D.y, -0.5f * D.x);
@@ -841,11 +843,11 @@ __device__ __forceinline__ cfloat c2r_pre(cfloat p, cfloat q, cfloat w) {
     return cconj(cmk(p.x + q.x - t.y, p.y - q.y + t.x));
 }
 
-template <int M, int RR, int C, int NT>
+template <int M, int RR, int C, int NT, int SC>
 __global__ __launch_bounds__(NT) void k_rows_c2r_rp(RowC2RArgs a, const cfloat* __restrict__ ptw) {
-    constexpr int NP = rp_npass(M);
-    constexpr int R0 = rp_radix(M, 0), M0 = M / R0;
-    constexpr int RL = rp_radix(M, NP - 1), ML = M / RL;
+    constexpr int NP = rp_npass(M, SC);
+    constexpr int R0 = rp_radix(M, 0, SC), M0 = M / R0;
+    constexpr int RL = rp_radix(M, NP - 1, SC), ML = M / RL;
     constexpr int NPAIR = M0 / 2 + 1;
     constexpr int IT0 = (RR * NPAIR + NT - 1) / NT;
     constexpr int ITL = (RR * ML + NT - 1) / NT;
@@ -892,22 +894,22 @@ __global__ __launch_bounds__(NT) void k_rows_c2r_rp(RowC2RArgs a, const cfloat* 
                                              a.post_tw[j + t * M0]);
                 }
                 RegDft<R0>::run(v);
-                cfloat* d = lds + r * rp_cs(M, RR);
+                cfloat* d = lds + r * rp_cs(M, RR, SC);
 #pragma unroll
-                for (int t = 0; t < R0; t++) d[rp_ix<M>(j * R0 + t)] = v[t];
+                for (int t = 0; t < R0; t++) d[rp_ix<M, SC>(j * R0 + t)] = v[t];
                 if (two) {
 #pragma unroll
                     for (int t = 0; t < R0; t++)
                         v[t] = c2r_pre(xb[t], xa[R0 - 1 - t], a.post_tw[j2 + t * M0]);
                     RegDft<R0>::run(v);
 #pragma unroll
-                    for (int t = 0; t < R0; t++) d[rp_ix<M>(j2 * R0 + t)] = v[t];
+                    for (int t = 0; t < R0; t++) d[rp_ix<M, SC>(j2 * R0 + t)] = v[t];
                 }
             }
         }
     }
     __syncthreads();
-    RpRange<M, 1, NP - 1, RR, NT>::run(lds, ptw);
+    RpRange<M, 1, NP - 1, RR, NT, SC>::run(lds, ptw);
     const bool vec = ((a.pitch | a.out_bstride) & 1) == 0 && (((uintptr_t)a.out) & 7) == 0;
 #pragma unroll
     for (int it = 0; it < ITL; it++) {
@@ -916,10 +918,10 @@ __global__ __launch_bounds__(NT) void k_rows_c2r_rp(RowC2RArgs a, const cfloat* 
             const int r = i / ML, k = i - r * ML;
             if (r < nrows) {
                 cfloat v[RL];
-                const cfloat* sl = lds + r * rp_cs(M, RR);
+                const cfloat* sl = lds + r * rp_cs(M, RR, SC);
 #pragma unroll
-                for (int t = 0; t < RL; t++) v[t] = sl[rp_ix<M>(k + t * ML)];
-                rp_twiddle<M, NP - 1>(v, k, ptw);
+                for (int t = 0; t < RL; t++) v[t] = sl[rp_ix<M, SC>(k + t * ML)];
+                rp_twiddle<M, NP - 1, SC>(v, k, ptw);
                 RegDft<RL>::run(v);
                 float* ob = a.out + (long long)b * a.out_bstride + (long long)(row0 + r) * a.pitch;
 #pragma unroll
@@ -942,42 +944,51 @@ __global__ __launch_bounds__(NT) void k_rows_c2r_rp(RowC2RArgs a, const cfloat* 
 }
 
 #ifndef MC_RP_ROW_SPECS
-// (row length, rows per workgroup, staging chunk width, threads); the first
-// entry of a length is its default, later ones are autotuner variants.
-#define MC_RP_ROW_SPECS(X) X(2880, 2, MC_TILE_CT, 512) X(2048, 2, MC_TILE_CT, 512) X(288, 16, 16, 512) \
-    X(5760, 1, MC_TILE_CT, 512) X(2880, 1, MC_TILE_CT, 256) X(2880, 2, MC_TILE_CT, 384) \
-    X(2880, 1, MC_TILE_CT, 192) X(288, 8, 16, 256) X(2880, 4, MC_TILE_CT, 1024) X(2880, 4, MC_TILE_CT, 768)
+// (row length, rows per workgroup, staging chunk width, threads, radix
+// schedule); the first entry of a length is its default, later ones are
+// autotuner variants.
+#define MC_RP_ROW_SPECS(X) X(2880, 2, MC_TILE_CT, 512, 0) X(2048, 2, MC_TILE_CT, 512, 0) \
+    X(288, 16, 16, 512, 0) X(5760, 1, MC_TILE_CT, 512, 0) X(2880, 1, MC_TILE_CT, 256, 0) \
+    X(2880, 2, MC_TILE_CT, 384, 0) X(2880, 1, MC_TILE_CT, 192, 0) X(288, 8, 16, 256, 0) \
+    X(2880, 4, MC_TILE_CT, 1024, 0) X(2880, 4, MC_TILE_CT, 768, 0) X(2880, 2, MC_TILE_CT, 512, 1) \
+    X(288, 16, 16, 512, 1)
 #endif
+
+// Twiddle table of a plan's register-pass rows (cols = 0) or columns for
+// radix schedule sc (null when the plan has none).
+static const cfloat* rp_tw(const Fft2D& p, int cols, int sc) {
+    return cols ? (sc ? p.col_ptw1 : p.col_ptw) : (sc ? p.row_ptw1 : p.row_ptw);
+}
 
 
 // Launch a register-pass column kernel for the given storage formats.
-template <int H, int CW, int NT>
+template <int H, int CW, int NT, int SC>
 static hipError_t launch_cols_fwd_rp(const ColFwdArgs& a, const cfloat* ptw, int batch, int s_fmt,
                                      int x_fmt, hipStream_t s) {
     const dim3 grid(mc_div_up(a.ncols, CW) * batch), blk(NT);
-    const size_t lds = (size_t)CW * rp_cs(H, CW) * sizeof(cfloat);
+    const size_t lds = (size_t)CW * rp_cs(H, CW, SC) * sizeof(cfloat);
     const int key = s_fmt * 3 + x_fmt;
     switch (key) {
-        case 0: hipLaunchKernelGGL((k_cols_fwd_rp<H, CW, NT, 0, 0>), grid, blk, lds, s, a, ptw); break;
-        case 1: hipLaunchKernelGGL((k_cols_fwd_rp<H, CW, NT, 0, 1>), grid, blk, lds, s, a, ptw); break;
-        case 2: hipLaunchKernelGGL((k_cols_fwd_rp<H, CW, NT, 0, 2>), grid, blk, lds, s, a, ptw); break;
-        case 3: hipLaunchKernelGGL((k_cols_fwd_rp<H, CW, NT, 1, 0>), grid, blk, lds, s, a, ptw); break;
-        case 4: hipLaunchKernelGGL((k_cols_fwd_rp<H, CW, NT, 1, 1>), grid, blk, lds, s, a, ptw); break;
-        case 6: hipLaunchKernelGGL((k_cols_fwd_rp<H, CW, NT, 2, 0>), grid, blk, lds, s, a, ptw); break;
-        case 8: hipLaunchKernelGGL((k_cols_fwd_rp<H, CW, NT, 2, 2>), grid, blk, lds, s, a, ptw); break;
+        case 0: hipLaunchKernelGGL((k_cols_fwd_rp<H, CW, NT, 0, 0, SC>), grid, blk, lds, s, a, ptw); break;
+        case 1: hipLaunchKernelGGL((k_cols_fwd_rp<H, CW, NT, 0, 1, SC>), grid, blk, lds, s, a, ptw); break;
+        case 2: hipLaunchKernelGGL((k_cols_fwd_rp<H, CW, NT, 0, 2, SC>), grid, blk, lds, s, a, ptw); break;
+        case 3: hipLaunchKernelGGL((k_cols_fwd_rp<H, CW, NT, 1, 0, SC>), grid, blk, lds, s, a, ptw); break;
+        case 4: hipLaunchKernelGGL((k_cols_fwd_rp<H, CW, NT, 1, 1, SC>), grid, blk, lds, s, a, ptw); break;
+        case 6: hipLaunchKernelGGL((k_cols_fwd_rp<H, CW, NT, 2, 0, SC>), grid, blk, lds, s, a, ptw); break;
+        case 8: hipLaunchKernelGGL((k_cols_fwd_rp<H, CW, NT, 2, 2, SC>), grid, blk, lds, s, a, ptw); break;
         default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
 }
-template <int H, int CW, int NT>
+template <int H, int CW, int NT, int SC>
 static hipError_t launch_cols_inv_rp(const ColInvArgs& a, const cfloat* ptw, int batch, int s_fmt,
                                      hipStream_t s) {
     const dim3 grid(mc_div_up(a.ncols, CW) * batch), blk(NT);
-    const size_t lds = (size_t)CW * rp_cs(H, CW) * sizeof(cfloat);
+    const size_t lds = (size_t)CW * rp_cs(H, CW, SC) * sizeof(cfloat);
     switch (s_fmt) {
-        case 0: hipLaunchKernelGGL((k_cols_inv_rp<H, CW, NT, 0>), grid, blk, lds, s, a, ptw); break;
-        case 1: hipLaunchKernelGGL((k_cols_inv_rp<H, CW, NT, 1>), grid, blk, lds, s, a, ptw); break;
-        case 2: hipLaunchKernelGGL((k_cols_inv_rp<H, CW, NT, 2>), grid, blk, lds, s, a, ptw); break;
+        case 0: hipLaunchKernelGGL((k_cols_inv_rp<H, CW, NT, 0, SC>), grid, blk, lds, s, a, ptw); break;
+        case 1: hipLaunchKernelGGL((k_cols_inv_rp<H, CW, NT, 1, SC>), grid, blk, lds, s, a, ptw); break;
+        case 2: hipLaunchKernelGGL((k_cols_inv_rp<H, CW, NT, 2, SC>), grid, blk, lds, s, a, ptw); break;
         default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
@@ -1022,14 +1033,16 @@ static void set_ct_attrs() {
 #define X(H, C, NT) set_lds_attr(k_cols_fwd_ct<H, C, NT>); set_lds_attr(k_cols_inv_ct<H, C, NT>);
     MC_COL_SPECS(X)
 #undef X
-#define X(H, C, NT)                                                                        \
-    set_lds_attr(k_cols_fwd_rp<H, C, NT, 0, 0>); set_lds_attr(k_cols_fwd_rp<H, C, NT, 0, 1>); \
-    set_lds_attr(k_cols_fwd_rp<H, C, NT, 0, 2>); set_lds_attr(k_cols_fwd_rp<H, C, NT, 1, 0>); \
-    set_lds_attr(k_cols_fwd_rp<H, C, NT, 2, 0>); set_lds_attr(k_cols_inv_rp<H, C, NT, 0>);    \
-    set_lds_attr(k_cols_inv_rp<H, C, NT, 1>); set_lds_attr(k_cols_inv_rp<H, C, NT, 2>);
+#define X(H, C, NT, SC)                                                                            \
+    set_lds_attr(k_cols_fwd_rp<H, C, NT, 0, 0, SC>); set_lds_attr(k_cols_fwd_rp<H, C, NT, 0, 1, SC>); \
+    set_lds_attr(k_cols_fwd_rp<H, C, NT, 0, 2, SC>); set_lds_attr(k_cols_fwd_rp<H, C, NT, 1, 0, SC>); \
+    set_lds_attr(k_cols_fwd_rp<H, C, NT, 2, 0, SC>); set_lds_attr(k_cols_fwd_rp<H, C, NT, 1, 1, SC>); \
+    set_lds_attr(k_cols_fwd_rp<H, C, NT, 2, 2, SC>); set_lds_attr(k_cols_inv_rp<H, C, NT, 0, SC>);    \
+    set_lds_attr(k_cols_inv_rp<H, C, NT, 1, SC>); set_lds_attr(k_cols_inv_rp<H, C, NT, 2, SC>);
     MC_RP_COL_SPECS(X)
 #undef X
-#define X(M, RR, C, NT) set_lds_attr(k_rows_r2c_rp<M, RR, C, NT>); set_lds_attr(k_rows_c2r_rp<M, RR, C, NT>);
+#define X(M, RR, C, NT, SC) \
+    set_lds_attr(k_rows_r2c_rp<M, RR, C, NT, SC>); set_lds_attr(k_rows_c2r_rp<M, RR, C, NT, SC>);
     MC_RP_ROW_SPECS(X)
 #undef X
 }
@@ -1196,15 +1209,16 @@ int mc_fft2d_create(Fft2D* p, int W, int H) {
     if (hipMemcpy(p->post_tw, pt.data(), pt.size() * sizeof(cfloat), hipMemcpyHostToDevice) !=
         hipSuccess)
         return -2;
-    for (int dim = 0; dim < 2; dim++) {
-        const std::vector<cfloat> tw = rp_twiddle_table(dim ? H : m);
-        if (tw.empty()) continue;
-        cfloat** dstp = dim ? &p->col_ptw : &p->row_ptw;
-        if (hipMalloc(dstp, tw.size() * sizeof(cfloat)) != hipSuccess) return -2;
-        if (hipMemcpy(*dstp, tw.data(), tw.size() * sizeof(cfloat), hipMemcpyHostToDevice) !=
-            hipSuccess)
-            return -2;
-    }
+    for (int dim = 0; dim < 2; dim++)
+        for (int sc = 0; sc < 2; sc++) {
+            const std::vector<cfloat> tw = rp_twiddle_table(dim ? H : m, sc);
+            if (tw.empty()) continue;
+            cfloat** dstp = dim ? (sc ? &p->col_ptw1 : &p->col_ptw) : (sc ? &p->row_ptw1 : &p->row_ptw);
+            if (hipMalloc(dstp, tw.size() * sizeof(cfloat)) != hipSuccess) return -2;
+            if (hipMemcpy(*dstp, tw.data(), tw.size() * sizeof(cfloat), hipMemcpyHostToDevice) !=
+                hipSuccess)
+                return -2;
+        }
     const int nb = (m + 1 + C - 1) / C;
     p->t_elems = (long long)nb * H * C;
     p->s_elems = (long long)(m + 1) * H;
@@ -1215,13 +1229,13 @@ void mc_fft2d_destroy(Fft2D* p) {
     if (p->row_tw) (void)hipFree(p->row_tw);
     if (p->col_tw) (void)hipFree(p->col_tw);
     if (p->post_tw) (void)hipFree(p->post_tw);
-    if (p->col_ptw) (void)hipFree(p->col_ptw);
-    if (p->row_ptw) (void)hipFree(p->row_ptw);
-    p->row_tw = p->col_tw = p->post_tw = p->col_ptw = p->row_ptw = nullptr;
+    for (cfloat* t : {p->col_ptw, p->row_ptw, p->col_ptw1, p->row_ptw1})
+        if (t) (void)hipFree(t);
+    p->row_tw = p->col_tw = p->post_tw = p->col_ptw = p->row_ptw = p->col_ptw1 = p->row_ptw1 = nullptr;
 }
 
 bool mc_fft_rp_rows(const Fft2D& p) {
-#define X(MM, RRR, CCC, NT) \
+#define X(MM, RRR, CCC, NT, SC) \
     if (p.m == MM && p.C == CCC && p.row_ptw) return true;
     MC_RP_ROW_SPECS(X)
 #undef X
@@ -1229,30 +1243,41 @@ bool mc_fft_rp_rows(const Fft2D& p) {
 }
 
 bool mc_fft_rp_cols(const Fft2D& p) {
-#define X(HH, CC, NT) \
+#define X(HH, CC, NT, SC) \
     if (p.H == HH && p.C % CC == 0 && p.col_ptw) return true;
     MC_RP_COL_SPECS(X)
 #undef X
     return false;
 }
 
-int mc_fft_variants(const Fft2D& p, int cols, std::vector<std::string>* names) {
+// "a.b.c" radix order of schedule sc of length n
+static std::string rp_order(int n, int sc) {
+    std::string o;
+    for (int p = 0; p < rp_npass(n, sc); p++) o += (p ? "." : "") + std::to_string(rp_radix(n, p, sc));
+    return o;
+}
+
+int mc_fft_variants(const Fft2D& p, int cols, std::vector<std::string>* names, std::vector<int>* scheds) {
     int n = 0;
-    char nm[96];
+    char nm[128];
     if (cols) {
-#define X(HH, CC, NT)                                                                          \
-    if (p.H == HH && p.C % CC == 0 && p.col_ptw) {                                             \
-        snprintf(nm, sizeof(nm), "cols<%d> %d col/wg %d thr", HH, CC, NT);                    \
+#define X(HH, CC, NT, SC)                                                                      \
+    if (p.H == HH && p.C % CC == 0 && rp_tw(p, 1, SC)) {                                       \
+        snprintf(nm, sizeof(nm), "cols<%d> %d col/wg %d thr radix %s", HH, CC, NT,            \
+                 rp_order(HH, SC).c_str());                                                    \
         if (names) names->push_back(nm);                                                       \
+        if (scheds) scheds->push_back(SC);                                                     \
         n++;                                                                                   \
     }
         MC_RP_COL_SPECS(X)
 #undef X
     } else {
-#define X(MM, RRR, CCC, NT)                                                                    \
-    if (p.m == MM && p.C == CCC && p.row_ptw) {                                                \
-        snprintf(nm, sizeof(nm), "rows<%d> %d row/wg %d thr", MM, RRR, NT);                   \
+#define X(MM, RRR, CCC, NT, SC)                                                                \
+    if (p.m == MM && p.C == CCC && rp_tw(p, 0, SC)) {                                          \
+        snprintf(nm, sizeof(nm), "rows<%d> %d row/wg %d thr radix %s", MM, RRR, NT,           \
+                 rp_order(MM, SC).c_str());                                                    \
         if (names) names->push_back(nm);                                                       \
+        if (scheds) scheds->push_back(SC);                                                     \
         n++;                                                                                   \
     }
         MC_RP_ROW_SPECS(X)
@@ -1289,11 +1314,11 @@ hipError_t mc_rows_r2c_mod(const Fft2D& p, const float* in, long long pitch, lon
     dim3 grid(mc_div_up(p.H, p.R) * batch);
     const size_t lds = (size_t)mc_lds_padded(p.R * p.m) * sizeof(cfloat);
     int v_ = 0;
-#define X(MM, RRR, CCC, NT)                                                                 \
-    if (p.m == MM && p.C == CCC && p.row_ptw && v_++ == p.row_var) {                       \
-        hipLaunchKernelGGL((k_rows_r2c_rp<MM, RRR, CCC, NT>),                              \
+#define X(MM, RRR, CCC, NT, SC)                                                             \
+    if (p.m == MM && p.C == CCC && rp_tw(p, 0, SC) && v_++ == p.row_var) {                 \
+        hipLaunchKernelGGL((k_rows_r2c_rp<MM, RRR, CCC, NT, SC>),                          \
                            dim3(mc_div_up(p.H, RRR) * batch), dim3(NT),                    \
-                           (size_t)RRR * rp_cs(MM, RRR) * sizeof(cfloat), s, a, (const cfloat*)p.row_ptw); \
+                           (size_t)RRR * rp_cs(MM, RRR, SC) * sizeof(cfloat), s, a, rp_tw(p, 0, SC)); \
         return hipGetLastError();                                                          \
     }
     MC_RP_ROW_SPECS(X)
@@ -1331,12 +1356,12 @@ hipError_t mc_rows_c2r(const Fft2D& p, const cfloat* T, long long t_bstride, flo
     dim3 grid(mc_div_up(p.H, p.R) * batch);
     const size_t lds = (size_t)mc_lds_padded(p.R * (p.m + 1)) * sizeof(cfloat);
     int v_ = 0;
-#define X(MM, RRR, CCC, NT)                                                                 \
-    if (p.m == MM && p.C == CCC && p.row_ptw && v_++ == p.row_var) {                       \
-        hipLaunchKernelGGL((k_rows_c2r_rp<MM, RRR, CCC, NT>),                              \
+#define X(MM, RRR, CCC, NT, SC)                                                             \
+    if (p.m == MM && p.C == CCC && rp_tw(p, 0, SC) && v_++ == p.row_var) {                 \
+        hipLaunchKernelGGL((k_rows_c2r_rp<MM, RRR, CCC, NT, SC>),                          \
                            dim3(mc_div_up(p.H, RRR) * batch), dim3(NT),                    \
-                           (size_t)RRR * (rp_cs(MM, RRR) + 1) * sizeof(cfloat), s, a,      \
-                           (const cfloat*)p.row_ptw);                                      \
+                           (size_t)RRR * (rp_cs(MM, RRR, SC) + 1) * sizeof(cfloat), s, a,  \
+                           rp_tw(p, 0, SC));                                               \
         return hipGetLastError();                                                          \
     }
     MC_RP_ROW_SPECS(X)
@@ -1376,9 +1401,9 @@ hipError_t mc_cols_fwd(const Fft2D& p, const cfloat* T, long long t_bstride, cfl
     dim3 grid(mc_div_up(p.cx, p.C) * batch);
     const size_t lds = (size_t)mc_lds_padded(p.C * p.H) * sizeof(cfloat);
     int v_ = 0;
-#define X(HH, CC, NT)                                                              \
-    if (p.H == HH && p.C % CC == 0 && p.col_ptw && v_++ == p.col_var)              \
-        return launch_cols_fwd_rp<HH, CC, NT>(a, p.col_ptw, batch, s_fmt, x_fmt, s);
+#define X(HH, CC, NT, SC)                                                          \
+    if (p.H == HH && p.C % CC == 0 && rp_tw(p, 1, SC) && v_++ == p.col_var)        \
+        return launch_cols_fwd_rp<HH, CC, NT, SC>(a, rp_tw(p, 1, SC), batch, s_fmt, x_fmt, s);
     MC_RP_COL_SPECS(X)
 #undef X
     if (s_fmt != MC_SPEC_F32 || x_fmt != MC_SPEC_F32) return hipErrorInvalidValue;
@@ -1414,9 +1439,9 @@ hipError_t mc_cols_inv(const Fft2D& p, const cfloat* S, long long s_bstride, int
     dim3 grid(mc_div_up(p.cx, p.C) * batch);
     const size_t lds = (size_t)mc_lds_padded(p.C * p.H) * sizeof(cfloat);
     int v_ = 0;
-#define X(HH, CC, NT)                                                              \
-    if (p.H == HH && p.C % CC == 0 && p.col_ptw && v_++ == p.col_var)              \
-        return launch_cols_inv_rp<HH, CC, NT>(a, p.col_ptw, batch, s_fmt, s);
+#define X(HH, CC, NT, SC)                                                          \
+    if (p.H == HH && p.C % CC == 0 && rp_tw(p, 1, SC) && v_++ == p.col_var)        \
+        return launch_cols_inv_rp<HH, CC, NT, SC>(a, rp_tw(p, 1, SC), batch, s_fmt, s);
     MC_RP_COL_SPECS(X)
 #undef X
     if (s_fmt != MC_SPEC_F32) return hipErrorInvalidValue;

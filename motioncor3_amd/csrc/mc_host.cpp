@@ -2633,7 +2633,9 @@ int time_median(mc_ctx* c, F fn, int reps, double* ms) {
 }
 
 // Tune one plan's rows and columns for `batch` transforms.
-int tune_plan(mc_ctx* c, Fft2D* p, int batch, std::string* rep) {
+// Time the plan's row and column variants on `batch` transforms and keep
+// the fastest; variants of another radix schedule only when `radix`.
+int tune_plan(mc_ctx* c, Fft2D* p, int batch, std::string* rep, bool radix) {
     const long long te = p->t_elems, se = p->s_elems;
     float* real = nullptr;
     cfloat* T = nullptr;
@@ -2654,11 +2656,13 @@ int tune_plan(mc_ctx* c, Fft2D* p, int batch, std::string* rep) {
     char line[256];
     for (int cols = 0; cols < 2 && rc == MC_OK; cols++) {
         std::vector<std::string> names;
-        const int nv = mc_fft_variants(*p, cols, &names);
+        std::vector<int> scheds;
+        const int nv = mc_fft_variants(*p, cols, &names, &scheds);
         if (nv < 2) continue;
         int best = 0;
         double best_ms = 1e30;
         for (int v = 0; v < nv && rc == MC_OK; v++) {
+            if (scheds[v] != 0 && !radix) continue;
             (cols ? p->col_var : p->row_var) = v;
             double ms = 0;
             if (cols)
@@ -2694,8 +2698,8 @@ int tune_plan(mc_ctx* c, Fft2D* p, int batch, std::string* rep) {
 
 }  // namespace
 
-int mc_autotune(mc_ctx* c, int W, int H, int N, int px, int py, float ft_bin, char* report,
-                int report_len) {
+int mc_autotune_schedules(mc_ctx* c, int W, int H, int N, int px, int py, float ft_bin, int radix,
+                          char* report, int report_len) {
     if (!c || W < 8 || H < 8 || (W % 2) || N < 1) return c ? fail(c, MC_E_ARG, "autotune: bad geometry") : MC_E_ARG;
     MC_TRY(hipSetDevice(c->device));
     const bool patch = px > 1 && py > 1;
@@ -2713,7 +2717,7 @@ int mc_autotune(mc_ctx* c, int W, int H, int N, int px, int py, float ft_bin, ch
         if (!p) return fail(c, MC_E_ARG, "autotune: unsupported FFT size");
         if (std::find(done.begin(), done.end(), p) != done.end()) continue;
         done.push_back(p);
-        MC_CHECK(tune_plan(c, p, it.batch, &rep));
+        MC_CHECK(tune_plan(c, p, it.batch, &rep, radix != 0));
     }
     // captured iteration graphs bake in launch shapes: drop them
     for (auto& kv : c->graphs) (void)hipGraphExecDestroy(kv.second);
@@ -2726,6 +2730,22 @@ int mc_autotune(mc_ctx* c, int W, int H, int N, int px, int py, float ft_bin, ch
     return MC_OK;
 }
 
+int mc_autotune(mc_ctx* c, int W, int H, int N, int px, int py, float ft_bin, char* report,
+                int report_len) {
+    return mc_autotune_schedules(c, W, H, N, px, py, ft_bin, 0, report, report_len);
+}
+int mc_fft_variant_name(mc_ctx* c, int W, int H, int cols, int v, char* name, int len) {
+    if (!c || !name || len < 1) return MC_E_ARG;
+    Fft2D* p = plan(c, W, H);
+    if (!p) return fail(c, MC_E_ARG, "unsupported FFT size");
+    std::vector<std::string> names;
+    const int n = mc_fft_variants(*p, cols ? 1 : 0, &names);
+    if (v < 0 || v >= n) return fail(c, MC_E_ARG, "no such FFT tile variant");
+    const size_t k = std::min((size_t)len - 1, names[v].size());
+    std::memcpy(name, names[v].data(), k);
+    name[k] = 0;
+    return MC_OK;
+}
 int mc_fft_variant(mc_ctx* c, int W, int H, int cols, int set, int* count, int* current) {
     if (!c) return MC_E_ARG;
     Fft2D* p = plan(c, W, H);

@@ -11,7 +11,7 @@
 // iterations continue from the corrected guess.
 //
 // Weights are FP32 masters with BF16 copies; every layer product runs on
-// MFMA (v_mfma_f32_32x32x8 bf16, FP32 accumulation).  Training (MSE loss,
+// MFMA (v_mfma_f32_32x32x16_bf16, FP32 accumulation).  Training (MSE loss,
 // Adam) uses the kernels below, so the network is trained on the GPU by the
 // same forward pass that the alignment uses.
 #include <hip/hip_runtime.h>
@@ -24,7 +24,7 @@
 
 namespace {
 
-typedef short bf16x4 __attribute__((ext_vector_type(4)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x16 __attribute__((ext_vector_type(16)));
 
 __device__ __forceinline__ unsigned short to_bf16(float f) { return (unsigned short)mc_bf16_bits(f); }
@@ -33,8 +33,8 @@ __device__ __forceinline__ unsigned short to_bf16(float f) { return (unsigned sh
 // B(k,j) = B[k*sbk + j*sbj], both BF16; C FP32 row-major with ldc.  One wave
 // per 32x32 tile of C; operands outside [M) x [K) x [N) read as zero and C
 // is written only inside [M) x [N).
-// Operand layout of v_mfma_f32_32x32x8 bf16 (lane l, 4 values each):
-//   A: row l % 32, k = 4 (l / 32) + 0..3;  B: column l % 32, same k.
+// Operand layout of gfx950's v_mfma_f32_32x32x16_bf16 (lane l, 8 values
+// each): A: row l % 32, k = 8 (l / 32) + 0..7;  B: column l % 32, same k.
 //   D: 16 values per lane, value e: row 8 (e / 4) + 4 (l / 32) + e % 4,
 //      column l % 32.
 __global__ __launch_bounds__(64) void k_gemm_bf16(const unsigned short* __restrict__ A, long long sai,
@@ -43,20 +43,22 @@ __global__ __launch_bounds__(64) void k_gemm_bf16(const unsigned short* __restri
                                                   int ldc, int M, int N, int K) {
     const int l = threadIdx.x;
     const int i0 = blockIdx.y * 32, j0 = blockIdx.x * 32;
-    const int ra = i0 + (l & 31), cb = j0 + (l & 31), kq = 4 * (l >> 5);
+    const int ra = i0 + (l & 31), cb = j0 + (l & 31), kq = 8 * (l >> 5);
     f32x16 acc;
 #pragma unroll
     for (int e = 0; e < 16; e++) acc[e] = 0.f;
     const bool rok = ra < M, cok = cb < N;
-    for (int k0 = 0; k0 < K; k0 += 8) {
-        bf16x4 a, b;
+    for (int k0 = 0; k0 < K; k0 += 16) {
+        bf16x8 a, b;
 #pragma unroll
-        for (int t = 0; t < 4; t++) {
+        for (int t = 0; t < 8; t++) {
             const int k = k0 + kq + t;
-            a[t] = (rok && k < K) ? (short)A[ra * sai + k * sak] : (short)0;
-            b[t] = (cok && k < K) ? (short)B[k * sbk + cb * sbj] : (short)0;
+            const unsigned short ua = (rok && k < K) ? A[ra * sai + k * sak] : (unsigned short)0;
+            const unsigned short ub = (cok && k < K) ? B[k * sbk + cb * sbj] : (unsigned short)0;
+            a[t] = __builtin_bit_cast(__bf16, ua);
+            b[t] = __builtin_bit_cast(__bf16, ub);
         }
-        acc = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a, b, acc, 0, 0, 0);
+        acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, acc, 0, 0, 0);
     }
 #pragma unroll
     for (int e = 0; e < 16; e++) {

@@ -92,7 +92,7 @@ ABI_SYMBOLS = (
     "mc_set_kernel_timing", "mc_get_kernel_timing", "mc_synth_render",
     "mc_prior_create", "mc_prior_destroy", "mc_prior_dims", "mc_prior_params",
     "mc_prior_forward", "mc_prior_loss_grad", "mc_prior_train", "mc_set_prior", "mc_gemm_bf16",
-    "mc_autotune", "mc_fft_variant",
+    "mc_autotune", "mc_autotune_schedules", "mc_fft_variant", "mc_fft_variant_name",
     "mc_host_alloc", "mc_host_free", "mc_set_refs", "mc_get_refs", "mc_load_frames", "mc_load_wait",
     "mc_mrc_info", "mc_mrc_read", "mc_mrc_stacks", "mc_tiff_info", "mc_tiff_read", "mc_eer_info", "mc_eer_read",
     "mc_load_eer",
@@ -523,12 +523,21 @@ class Context:
                                             C.c_longlong(frames.stride(0)), bad.vp(), win),
                     "mc_correct_bad")
 
-    def autotune(self, W, H, N, patch=(0, 0), ft_bin=1.0):
-        """mc_autotune: time the FFT tile variants of this geometry and keep
-        the fastest.  Returns the report text."""
+    def autotune(self, W, H, N, patch=(0, 0), ft_bin=1.0, radix=False):
+        """mc_autotune (radix: mc_autotune_schedules with other radix orders
+        too): time the FFT variants of this geometry and keep the fastest.
+        Returns the report text."""
         buf = C.create_string_buffer(16384)
-        self._check(self.lib.mc_autotune(self.ptr, W, H, N, patch[0], patch[1], C.c_float(ft_bin),
-                                         buf, len(buf)), "mc_autotune")
+        self._check(self.lib.mc_autotune_schedules(self.ptr, W, H, N, patch[0], patch[1],
+                                                   C.c_float(ft_bin), int(bool(radix)), buf,
+                                                   len(buf)), "mc_autotune_schedules")
+        return buf.value.decode()
+
+    def fft_variant_name(self, W, H, cols, v):
+        """Tile and radix order of variant v of the W x H plan's rows or columns."""
+        buf = C.create_string_buffer(256)
+        self._check(self.lib.mc_fft_variant_name(self.ptr, W, H, int(cols), v, buf, len(buf)),
+                    "mc_fft_variant_name")
         return buf.value.decode()
 
     def fft_variant(self, W, H, cols, set=-1):
