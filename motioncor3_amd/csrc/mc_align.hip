@@ -348,9 +348,10 @@ __global__ __launch_bounds__(MC_NTHR) void k_correlate(CorrArgs a) {
 // the last pass only evaluates the butterflies that own one of the seay
 // kept lags (y in [0, seay/2) and [H - seay/2, H)).
 // WPE: waves per SIMD the tile is compiled for (a register budget of 512 /
-// WPE VGPRs): the 408-point tile is built for five, so that three of its
-// 52 KB workgroups share a CU (18 waves instead of 12), at the cost of a
-// few register spills.
+// WPE VGPRs).  Both tiles are built for five, so that three of their 52 KB
+// workgroups share a CU (18 waves): the 408-point tile at the cost of a few
+// register spills, the 2048-point tile with three columns per workgroup
+// (80 VGPRs).
 template <int H, int C, int NT, int FF, int WPE>
 __global__ __launch_bounds__(NT, WPE) void k_correlate_rp(CorrArgs a, const cfloat* __restrict__ ptw) {
     constexpr int NP = rp_npass(H), R0 = rp_radix(H, 0), M0 = H / R0;
@@ -463,7 +464,7 @@ __global__ __launch_bounds__(NT, WPE) void k_correlate_rp(CorrArgs a, const cflo
 
 #ifndef MC_RP_CORR_SPECS
 // (column length, columns per workgroup, threads, waves per SIMD)
-#define MC_RP_CORR_SPECS(X) X(2048, 4, 512, 1) X(408, 16, 384, 5)
+#define MC_RP_CORR_SPECS(X) X(2048, 3, 384, 5) X(408, 16, 384, 5)
 #endif
 
 // Second half of the windowed C2R: for one lag row, the Hermitian sum over

@@ -46,6 +46,8 @@ struct Fft2D {
     // tile variant of the register-pass row / column kernels (index among the
     // compiled variants for this length; 0 = the default, mc_autotune picks)
     int row_var = 0, col_var = 0;
+    // the same for the inverse passes (C2R rows, inverse columns)
+    int row_var_inv = 0, col_var_inv = 0;
 };
 
 // Per-transform phase factors e^{i(kx*ax + ky*ay)} applied on load (skip

@@ -765,8 +765,13 @@ __global__ __launch_bounds__(NT) void k_cols_inv_rp(ColInvArgs a, const cfloat* 
 // (pass 0) -> LDS passes -> post-processing -> staging T.  C2R: staging T ->
 // LDS -> pre-processing -> LDS passes -> last pass -> real rows.
 
+// The 288-point patch rows (16 rows, 37 KB of LDS per workgroup) are
+// compiled for more waves per SIMD than their register use alone allows.
+#ifndef MC_R2C288_WPE
+#define MC_R2C288_WPE 8
+#endif
 template <int M, int RR, int C, int NT, int SC>
-__global__ __launch_bounds__(NT) void k_rows_r2c_rp(RowR2CArgs a, const cfloat* __restrict__ ptw) {
+__global__ __launch_bounds__(NT, (M == 288 ? MC_R2C288_WPE : 1)) void k_rows_r2c_rp(RowR2CArgs a, const cfloat* __restrict__ ptw) {
     constexpr int NP = rp_npass(M, SC), R0 = rp_radix(M, 0, SC), M0 = M / R0;
     constexpr int IT0 = (RR * M0 + NT - 1) / NT;
     constexpr int NB = (M + 1 + C - 1) / C;
@@ -1357,7 +1362,7 @@ hipError_t mc_rows_c2r(const Fft2D& p, const cfloat* T, long long t_bstride, flo
     const size_t lds = (size_t)mc_lds_padded(p.R * (p.m + 1)) * sizeof(cfloat);
     int v_ = 0;
 #define X(MM, RRR, CCC, NT, SC)                                                             \
-    if (p.m == MM && p.C == CCC && rp_tw(p, 0, SC) && v_++ == p.row_var) {                 \
+    if (p.m == MM && p.C == CCC && rp_tw(p, 0, SC) && v_++ == p.row_var_inv) {                 \
         hipLaunchKernelGGL((k_rows_c2r_rp<MM, RRR, CCC, NT, SC>),                          \
                            dim3(mc_div_up(p.H, RRR) * batch), dim3(NT),                    \
                            (size_t)RRR * (rp_cs(MM, RRR, SC) + 1) * sizeof(cfloat), s, a,  \
@@ -1440,7 +1445,7 @@ hipError_t mc_cols_inv(const Fft2D& p, const cfloat* S, long long s_bstride, int
     const size_t lds = (size_t)mc_lds_padded(p.C * p.H) * sizeof(cfloat);
     int v_ = 0;
 #define X(HH, CC, NT, SC)                                                          \
-    if (p.H == HH && p.C % CC == 0 && rp_tw(p, 1, SC) && v_++ == p.col_var)        \
+    if (p.H == HH && p.C % CC == 0 && rp_tw(p, 1, SC) && v_++ == p.col_var_inv)        \
         return launch_cols_inv_rp<HH, CC, NT, SC>(a, rp_tw(p, 1, SC), batch, s_fmt, s);
     MC_RP_COL_SPECS(X)
 #undef X

@@ -2659,35 +2659,36 @@ int tune_plan(mc_ctx* c, Fft2D* p, int batch, std::string* rep, bool radix) {
         std::vector<int> scheds;
         const int nv = mc_fft_variants(*p, cols, &names, &scheds);
         if (nv < 2) continue;
-        int best = 0;
-        double best_ms = 1e30;
-        for (int v = 0; v < nv && rc == MC_OK; v++) {
-            if (scheds[v] != 0 && !radix) continue;
-            (cols ? p->col_var : p->row_var) = v;
-            double ms = 0;
-            if (cols)
+        // forward (R2C rows / forward columns) and inverse passes separately
+        for (int inv = 0; inv < 2 && rc == MC_OK; inv++) {
+            int& var = cols ? (inv ? p->col_var_inv : p->col_var) : (inv ? p->row_var_inv : p->row_var);
+            int best = 0;
+            double best_ms = 1e30;
+            for (int v = 0; v < nv && rc == MC_OK; v++) {
+                if (scheds[v] != 0 && !radix) continue;
+                var = v;
+                double ms = 0;
                 rc = time_median(c, [&]() -> hipError_t {
-                    hipError_t e = mc_cols_fwd(*p, T, te, S, se, nullptr, 0, 0, 0, batch, s);
-                    return e ? e : mc_cols_inv(*p, S, se, p->cx, p->H, nullptr, T, te, batch, s);
+                    if (cols)
+                        return inv ? mc_cols_inv(*p, S, se, p->cx, p->H, nullptr, T, te, batch, s)
+                                   : mc_cols_fwd(*p, T, te, S, se, nullptr, 0, 0, 0, batch, s);
+                    return inv ? mc_rows_c2r(*p, T, te, real, p->W, (long long)p->W * p->H, batch, 0, s)
+                               : mc_rows_r2c(*p, real, p->W, (long long)p->W * p->H, nullptr, T, te, batch,
+                                             1.0f, s);
                 }, 5, &ms);
-            else
-                rc = time_median(c, [&]() -> hipError_t {
-                    hipError_t e = mc_rows_r2c(*p, real, p->W, (long long)p->W * p->H, nullptr, T, te,
-                                               batch, 1.0f, s);
-                    return e ? e : mc_rows_c2r(*p, T, te, real, p->W, (long long)p->W * p->H, batch, 0, s);
-                }, 5, &ms);
-            snprintf(line, sizeof(line), "%dx%d x%d %s: %.4f ms (forward + inverse)\n", p->W, p->H,
-                     batch, names[v].c_str(), ms);
-            *rep += line;
-            if (ms < best_ms) {
-                best_ms = ms;
-                best = v;
+                snprintf(line, sizeof(line), "%dx%d x%d %s: %.4f ms (%s)\n", p->W, p->H, batch,
+                         names[v].c_str(), ms, inv ? "inverse" : "forward");
+                *rep += line;
+                if (ms < best_ms) {
+                    best_ms = ms;
+                    best = v;
+                }
             }
+            var = best;
+            snprintf(line, sizeof(line), "%dx%d %s%s -> %s\n", p->W, p->H, cols ? "columns" : "rows",
+                     inv ? " (inverse)" : "", names[best].c_str());
+            *rep += line;
         }
-        (cols ? p->col_var : p->row_var) = best;
-        snprintf(line, sizeof(line), "%dx%d %s -> %s\n", p->W, p->H, cols ? "columns" : "rows",
-                 names[best].c_str());
-        *rep += line;
     }
     (void)hipStreamSynchronize(s);
     (void)hipFree(real);
@@ -2755,6 +2756,7 @@ int mc_fft_variant(mc_ctx* c, int W, int H, int cols, int set, int* count, int* 
         if (set >= std::max(n, 1)) return fail(c, MC_E_ARG, "no such FFT tile variant");
         MC_TRY(hipStreamSynchronize(c->stream));
         (cols ? p->col_var : p->row_var) = set;
+        (cols ? p->col_var_inv : p->row_var_inv) = set;
     }
     if (count) *count = n;
     if (current) *current = cols ? p->col_var : p->row_var;
