@@ -52,7 +52,7 @@ def test_clamped_c2r(ctx):
 
 
 def _radix(name):
-    return name.split("radix ")[1]
+    return name.split("radix ")[1].split()[0]
 
 
 @pytest.mark.parametrize("W,H", [(5760, 4092), (2882, 2048), (576, 408), (8192, 4096)])
