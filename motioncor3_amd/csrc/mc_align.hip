@@ -1181,7 +1181,9 @@ __global__ __launch_bounds__(256) void k_patch_stats(double* s, const double* ce
 #define MC_WARP_TY 4
 #endif
 #define MC_WARP_NT (MC_WARP_TX * MC_WARP_TY)
+#ifndef MC_WARP_FCH
 #define MC_WARP_FCH 16
+#endif
 #define MC_WARP_KREG 16
 #ifndef MC_WARP_U
 #define MC_WARP_U 8
@@ -1272,12 +1274,13 @@ __device__ __forceinline__ void warp_tile_reg(const WarpArgs& a, const int* klis
         __syncthreads();
         if (!live) continue;
         // gathers of MC_WARP_U frames are issued before their stores so that
-        // each wave keeps several loads in flight; offsets within a frame
+        // each wave keeps several loads in flight, and the next group's
+        // source offsets are formed while they are; offsets within a frame
         // are 32-bit (the host checks pitch * H < 2^31)
         const int pitch = (int)a.pitch;
         const int oo = y * pitch + x;
-        for (int fb = 0; fb < nf; fb += MC_WARP_U) {
-            int so[MC_WARP_U];
+        int so[MC_WARP_U];
+        auto offsets = [&](int fb) {
 #pragma unroll
             for (int u = 0; u < MC_WARP_U; u++) {
                 const int fi = min(fb + u, nf - 1);
@@ -1296,13 +1299,38 @@ __device__ __forceinline__ void warp_tile_reg(const WarpArgs& a, const int* klis
                 }
                 so[u] = warp_src(a.W, a.H, pitch, x, y, fw, fsx, fsy);
             }
+        };
+        // Whole groups load and store without per-frame conditions: a
+        // conditional store sits in a block of its own, and the compiler
+        // then waits for every outstanding memory operation (the previous
+        // stores included) before it.  The last nf % MC_WARP_U frames go one
+        // at a time.
+        const int nfull = nf - nf % MC_WARP_U;
+        if (nfull) offsets(0);
+        for (int fb = 0; fb < nfull; fb += MC_WARP_U) {
             float v[MC_WARP_U];
 #pragma unroll
-            for (int u = 0; u < MC_WARP_U; u++)
-                if (fb + u < nf) v[u] = (a.R + (long long)(f0 + fb + u) * a.fstride)[so[u]];
+            for (int u = 0; u < MC_WARP_U; u++) v[u] = (a.R + (long long)(f0 + fb + u) * a.fstride)[so[u]];
+            if (fb + MC_WARP_U < nfull) offsets(fb + MC_WARP_U);
 #pragma unroll
-            for (int u = 0; u < MC_WARP_U; u++)
-                if (fb + u < nf) (a.Rp + (long long)(f0 + fb + u) * a.fstride)[oo] = v[u];
+            for (int u = 0; u < MC_WARP_U; u++) (a.Rp + (long long)(f0 + fb + u) * a.fstride)[oo] = v[u];
+        }
+        for (int fi = nfull; fi < nf; fi++) {
+            const float4* t = tab + fi * K;
+            float fw = fw_all, fsx = 0.f, fsy = 0.f;
+            if (fbad[fi]) {
+                fw = 0.f;
+#pragma unroll
+                for (int q = 0; q < K; q++) fw = fmaf(w[q], t[q].z, fw);
+            }
+#pragma unroll
+            for (int q = 0; q < K; q++) {
+                const float4 v = t[q];
+                fsx = fmaf(v.x, w[q], fsx);
+                fsy = fmaf(v.y, w[q], fsy);
+            }
+            const long long fo = (long long)(f0 + fi) * a.fstride;
+            a.Rp[fo + oo] = a.R[fo + warp_src(a.W, a.H, pitch, x, y, fw, fsx, fsy)];
         }
     }
 }
