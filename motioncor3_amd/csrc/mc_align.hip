@@ -348,11 +348,14 @@ __global__ __launch_bounds__(MC_NTHR) void k_correlate(CorrArgs a) {
 // the last pass only evaluates the butterflies that own one of the seay
 // kept lags (y in [0, seay/2) and [H - seay/2, H)).
 // WPE: waves per SIMD the tile is compiled for (a register budget of 512 /
-// WPE VGPRs).  Both tiles are built for five, so that three of their 52 KB
-// workgroups share a CU (18 waves): the 408-point tile at the cost of a few
-// register spills, the 2048-point tile with three columns per workgroup
-// (80 VGPRs).
-template <int H, int C, int NT, int FF, int WPE>
+// WPE VGPRs).  The 2048-point tile is built for five, so that three of its
+// 49 KB workgroups share a CU (18 waves), with three columns per workgroup
+// (80 VGPRs); batching its loads (BL) spills at five waves.  The 408-point
+// tile is built for four (128 VGPRs) with both BL bits: each frame's
+// group-sum loads and then the sum's loads go out as one batch each (at five
+// waves its registers spill, and every spilled value costs a wait for all
+// outstanding memory operations).
+template <int H, int C, int NT, int FF, int WPE, int BL>
 __global__ __launch_bounds__(NT, WPE) void k_correlate_rp(CorrArgs a, const cfloat* __restrict__ ptw) {
     constexpr int NP = rp_npass(H), R0 = rp_radix(H, 0), M0 = H / R0;
     constexpr int RL = rp_radix(H, NP - 1), ML = H / RL;
@@ -403,23 +406,60 @@ __global__ __launch_bounds__(NT, WPE) void k_correlate_rp(CorrArgs a, const cflo
                     const float fi = FF == MC_SPEC_F32 ? 1.f : r.finv[b];
                     const cfloat* Bb = r.B + b * H + jj;
                     const cfloat pa = r.A[b * r.cx + kx];
+                    if constexpr (BL & 1) {
+                        // all loads of the frame first, then the products
+                        cfloat fv[R0], bv[R0];
 #pragma unroll
-                    for (int t = 0; t < R0; t++)
-                        v[t] = cadd(v[t], cmul(Spec<FF>::ld(r.F, fo + t * M0, fi),
-                                               cmul(pa, Bb[t * M0])));
+                        for (int t = 0; t < R0; t++) {
+                            fv[t] = Spec<FF>::ld(r.F, fo + t * M0, fi);
+                            bv[t] = Bb[t * M0];
+                        }
+#pragma unroll
+                        for (int t = 0; t < R0; t++) v[t] = cadd(v[t], cmul(fv[t], cmul(pa, bv[t])));
+                    } else {
+#pragma unroll
+                        for (int t = 0; t < R0; t++)
+                            v[t] = cadd(v[t], cmul(Spec<FF>::ld(r.F, fo + t * M0, fi), cmul(pa, Bb[t * M0])));
+                    }
                 }
+                // BL bit 1: the sum's values first (one batch of loads), then
+                // the filter, with the phase-only branch outside the loop;
+                // otherwise each value is loaded where the filter needs it
+                // (fewer registers)
                 const cfloat* Sc = r.S + (long long)kx * H + jj;
+                if constexpr (BL & 2) {
+                    cfloat sv[R0];
 #pragma unroll
-                for (int t = 0; t < R0; t++) {
-                    const int y = jj + t * M0;
-                    const cfloat G = v[t];
-                    const cfloat sd = csub(Sc[t * M0], G);
-                    cfloat X = cmk(G.x * sd.x + G.y * sd.y, G.x * sd.y - G.y * sd.x);
-                    const int ys = (y > H / 2) ? y - H : y;
-                    float fl = expf(r.filt0 * (float)(kx * kx + ys * ys));
-                    if (a.phase_only) fl = fl / (sqrtf(X.x * X.x + X.y * X.y) + 1e-20f);
-                    X = (kx == 0 && y == 0) ? cmk(0.f, 0.f) : cscale(X, fl);
-                    v[t] = cconj(X);
+                    for (int t = 0; t < R0; t++) sv[t] = Sc[t * M0];
+                    auto filter = [&](auto phase_only) {
+#pragma unroll
+                        for (int t = 0; t < R0; t++) {
+                            const int y = jj + t * M0;
+                            const cfloat G = v[t];
+                            const cfloat sd = csub(sv[t], G);
+                            cfloat X = cmk(G.x * sd.x + G.y * sd.y, G.x * sd.y - G.y * sd.x);
+                            const int ys = (y > H / 2) ? y - H : y;
+                            float fl = expf(r.filt0 * (float)(kx * kx + ys * ys));
+                            if (decltype(phase_only)::value) fl = fl / (sqrtf(X.x * X.x + X.y * X.y) + 1e-20f);
+                            X = (kx == 0 && y == 0) ? cmk(0.f, 0.f) : cscale(X, fl);
+                            v[t] = cconj(X);
+                        }
+                    };
+                    if (a.phase_only) filter(std::true_type{});
+                    else filter(std::false_type{});
+                } else {
+#pragma unroll
+                    for (int t = 0; t < R0; t++) {
+                        const int y = jj + t * M0;
+                        const cfloat G = v[t];
+                        const cfloat sd = csub(Sc[t * M0], G);
+                        cfloat X = cmk(G.x * sd.x + G.y * sd.y, G.x * sd.y - G.y * sd.x);
+                        const int ys = (y > H / 2) ? y - H : y;
+                        float fl = expf(r.filt0 * (float)(kx * kx + ys * ys));
+                        if (a.phase_only) fl = fl / (sqrtf(X.x * X.x + X.y * X.y) + 1e-20f);
+                        X = (kx == 0 && y == 0) ? cmk(0.f, 0.f) : cscale(X, fl);
+                        v[t] = cconj(X);
+                    }
                 }
             }
             RegDft<R0>::run(v);
@@ -463,8 +503,9 @@ __global__ __launch_bounds__(NT, WPE) void k_correlate_rp(CorrArgs a, const cflo
 }
 
 #ifndef MC_RP_CORR_SPECS
-// (column length, columns per workgroup, threads, waves per SIMD)
-#define MC_RP_CORR_SPECS(X) X(2048, 3, 384, 5) X(408, 16, 384, 5)
+// (column length, columns per workgroup, threads, waves per SIMD, batched
+// loads: bit 0 the group sums', bit 1 the sum's)
+#define MC_RP_CORR_SPECS(X) X(2048, 3, 384, 5, 0) X(408, 16, 384, 4, 3)
 #endif
 
 // Second half of the windowed C2R: for one lag row, the Hermitian sum over
@@ -1615,12 +1656,12 @@ int mc_align_init_constants() {
     const int bytes = (mc_lds_padded(MC_LDS_ELEMS) + 64) * (int)sizeof(cfloat);
     attr_ok &= hipSuccess == hipFuncSetAttribute((const void*)k_correlate, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
     attr_ok &= hipSuccess == hipFuncSetAttribute((const void*)k_finalize, hipFuncAttributeMaxDynamicSharedMemorySize, 65536);
-#define X(HH, CC, NT, WPE)                                                                \
-    attr_ok &= hipSuccess == hipFuncSetAttribute((const void*)k_correlate_rp<HH, CC, NT, 0, WPE>,                 \
+#define X(HH, CC, NT, WPE, BL)                                                                \
+    attr_ok &= hipSuccess == hipFuncSetAttribute((const void*)k_correlate_rp<HH, CC, NT, 0, WPE, BL>,                 \
                         hipFuncAttributeMaxDynamicSharedMemorySize, bytes);              \
-    attr_ok &= hipSuccess == hipFuncSetAttribute((const void*)k_correlate_rp<HH, CC, NT, 1, WPE>,                 \
+    attr_ok &= hipSuccess == hipFuncSetAttribute((const void*)k_correlate_rp<HH, CC, NT, 1, WPE, BL>,                 \
                         hipFuncAttributeMaxDynamicSharedMemorySize, bytes);              \
-    attr_ok &= hipSuccess == hipFuncSetAttribute((const void*)k_correlate_rp<HH, CC, NT, 2, WPE>,                 \
+    attr_ok &= hipSuccess == hipFuncSetAttribute((const void*)k_correlate_rp<HH, CC, NT, 2, WPE, BL>,                 \
                         hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
     MC_RP_CORR_SPECS(X)
 #undef X
@@ -1712,17 +1753,17 @@ hipError_t mc_launch_correlate(const FftPlan1& colplan, const cfloat* ptw, int C
     // a job table brings its tile count (st.tiles, for mc_correlate_rp_cols(H))
     // and its widest job (st.cx_max, st.G_max)
     if (st.jobs && (st.cx_max < 1 || st.G_max < 1)) return hipErrorInvalidValue;
-#define X(HH, CC, NT, WPE)                                                                \
+#define X(HH, CC, NT, WPE, BL)                                                                \
     if (H == HH && ptw && seay <= HH) {                                                  \
         if (st.jobs && st.tiles < 1) return hipErrorInvalidValue;                        \
         const dim3 grid(st.jobs ? st.tiles : mc_div_up(cx, CC) * st.G * st.J);           \
         const size_t lds = (size_t)CC * rp_cs(HH, CC) * sizeof(cfloat);                  \
         if (f_fmt == MC_SPEC_F32)                                                        \
-            hipLaunchKernelGGL((k_correlate_rp<HH, CC, NT, 0, WPE>), grid, dim3(NT), lds, s, a, ptw); \
+            hipLaunchKernelGGL((k_correlate_rp<HH, CC, NT, 0, WPE, BL>), grid, dim3(NT), lds, s, a, ptw); \
         else if (f_fmt == MC_SPEC_F16)                                                   \
-            hipLaunchKernelGGL((k_correlate_rp<HH, CC, NT, 1, WPE>), grid, dim3(NT), lds, s, a, ptw); \
+            hipLaunchKernelGGL((k_correlate_rp<HH, CC, NT, 1, WPE, BL>), grid, dim3(NT), lds, s, a, ptw); \
         else if (f_fmt == MC_SPEC_BF16)                                                  \
-            hipLaunchKernelGGL((k_correlate_rp<HH, CC, NT, 2, WPE>), grid, dim3(NT), lds, s, a, ptw); \
+            hipLaunchKernelGGL((k_correlate_rp<HH, CC, NT, 2, WPE, BL>), grid, dim3(NT), lds, s, a, ptw); \
         else                                                                             \
             return hipErrorInvalidValue;                                                 \
         return hipGetLastError();                                                        \
@@ -1740,7 +1781,7 @@ hipError_t mc_launch_correlate(const FftPlan1& colplan, const cfloat* ptw, int C
 }
 
 bool mc_correlate_has_rp(int H) {
-#define X(HH, CC, NT, WPE) \
+#define X(HH, CC, NT, WPE, BL) \
     if (H == HH) return true;
     MC_RP_CORR_SPECS(X)
 #undef X
@@ -1748,7 +1789,7 @@ bool mc_correlate_has_rp(int H) {
 }
 
 int mc_correlate_rp_cols(int H) {
-#define X(HH, CC, NT, WPE) \
+#define X(HH, CC, NT, WPE, BL) \
     if (H == HH) return CC;
     MC_RP_CORR_SPECS(X)
 #undef X

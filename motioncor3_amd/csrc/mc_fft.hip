@@ -635,7 +635,11 @@ __global__ __launch_bounds__(NT) void k_cols_fwd_rp(ColFwdArgs a, const cfloat* 
     }
     __syncthreads();
     RpMid<H, 1, C, NT, SC>::run(lds, ptw);
-    // last pass: LDS -> registers -> S (column-major) and the crop X
+    // last pass: LDS -> registers -> S (column-major) and the crop X; the
+    // store factors of a reduced format are loaded before any store (a load
+    // behind a store could only be waited for together with it)
+    const float gs = FS == MC_SPEC_F32 ? 1.f : a.s_g[b];
+    const float gx = FX == MC_SPEC_F32 ? 1.f : a.x_g[b];
 #pragma unroll
     for (int it = 0; it < ITL; it++) {
         const int i = threadIdx.x + it * NT;
@@ -650,18 +654,16 @@ __global__ __launch_bounds__(NT) void k_cols_fwd_rp(ColFwdArgs a, const cfloat* 
                 RegDft<RL>::run(v);
                 const int kx = c0 + c;
                 if (a.S) {
-                    const float g = FS == MC_SPEC_F32 ? 1.f : a.s_g[b];
                     const long long d = (long long)b * a.s_bstride + (long long)kx * H + k;
 #pragma unroll
-                    for (int t = 0; t < RL; t++) Spec<FS>::st(a.S, d + t * ML, v[t], g);
+                    for (int t = 0; t < RL; t++) Spec<FS>::st(a.S, d + t * ML, v[t], gs);
                 }
                 if (a.X && kx < a.xcx) {
-                    const float g = FX == MC_SPEC_F32 ? 1.f : a.x_g[b];
                     const long long xd = (long long)b * a.x_bstride + (long long)kx * a.xcy;
 #pragma unroll
                     for (int t = 0; t < RL; t++) {
                         const int yo = crop_row(k + t * ML, a.xcy, H);
-                        if (yo >= 0) Spec<FX>::st(a.X, xd + yo, v[t], g);
+                        if (yo >= 0) Spec<FX>::st(a.X, xd + yo, v[t], gx);
                     }
                 }
             }
@@ -766,9 +768,10 @@ __global__ __launch_bounds__(NT) void k_cols_inv_rp(ColInvArgs a, const cfloat* 
 // LDS -> pre-processing -> LDS passes -> last pass -> real rows.
 
 // The 288-point patch rows (16 rows, 37 KB of LDS per workgroup) are
-// compiled for more waves per SIMD than their register use alone allows.
+// compiled for six waves per SIMD (80 VGPRs, a few spilled): 1.07 ms per
+// launch at eight waves (44 VGPRs spilled), 0.80 at seven, 0.70 at six.
 #ifndef MC_R2C288_WPE
-#define MC_R2C288_WPE 8
+#define MC_R2C288_WPE 6
 #endif
 template <int M, int RR, int C, int NT, int SC>
 __global__ __launch_bounds__(NT, (M == 288 ? MC_R2C288_WPE : 1)) void k_rows_r2c_rp(RowR2CArgs a, const cfloat* __restrict__ ptw) {
@@ -820,17 +823,31 @@ __global__ __launch_bounds__(NT, (M == 288 ? MC_R2C288_WPE : 1)) void k_rows_r2c
     RpRange<M, 1, NP, RR, NT, SC>::run(lds, ptw);
     cfloat* tb = a.T + (long long)b * a.t_bstride + (long long)row0 * C;
     const long long kbs = (long long)a.H * C;
-#pragma unroll 4
-    for (int e = threadIdx.x; e < TOT; e += NT) {
+    // All post-twiddles of the thread's items are loaded before its first
+    // store: a load issued after a (conditional) store can only be waited
+    // for together with it (one memory counter), which would hold every
+    // store to a full round trip.
+    constexpr int ITP = (TOT + NT - 1) / NT;
+    cfloat twa[ITP];
+#pragma unroll
+    for (int u = 0; u < ITP; u++) {
+        const int e = threadIdx.x + u * NT;
+        twa[u] = a.post_tw[min((e / (C * RR)) * C + e % C, M)];
+    }
+#pragma unroll
+    for (int u = 0; u < ITP; u++) {
+        const int e = threadIdx.x + u * NT;
+        if (TOT % NT != 0 && e >= TOT) break;
         const int c = e % C, r = (e / C) % RR, kb = e / (C * RR);
         const int k = kb * C + c;
+        const cfloat tw = twa[u];
         if (k > M || r >= nrows) continue;
         const cfloat zk = lds[r * rp_cs(M, RR, SC) + rp_ix<M, SC>(k == M ? 0 : k)];
         const cfloat zm = lds[r * rp_cs(M, RR, SC) + rp_ix<M, SC>(k == 0 ? 0 : M - k)];
         const cfloat E = cmk(0.5f * (zk.x + zm.x), 0.5f * (zk.y - zm.y));
         const cfloat D = cmk(zk.x - zm.x, zk.y + zm.y);
         const cfloat O = cmk(0.5f * D.y, -0.5f * D.x);
-        const cfloat X = cadd(E, cmul(a.post_tw[k], O));
+        const cfloat X = cadd(E, cmul(tw, O));
         tb[kb * kbs + r * C + c] = cscale(X, a.scale);
         if (a.dc && k == 0) a.dc[(long long)b * a.H + row0 + r] = fabsf(X.x * a.scale);
     }
