@@ -479,12 +479,23 @@ static __device__ __forceinline__ void ct_stage_rader(cfloat* lds, int base, int
             const int j = i / L2, n2 = i - j * L2, k = j % NS;
             int t = 1;
             for (int e = 0; e < n2; e++) t = (t * G) % P;
+            // the gather indices first, then all twiddle loads as one batch
+            // (an index chain between them would hold each load to its own
+            // round trip)
+            int ti[L1];
 #pragma unroll
             for (int n1 = 0; n1 < L1; n1++) {
-                cfloat x = lds[lp(base + j + t * M)];
-                if (NS > 1) x = cmul(x, tw[TWO + k * (P - 1) + t - 1]);
-                v1[it][n1] = x;
+                ti[n1] = t;
                 t = (t * GS) % P;
+            }
+            if (NS > 1) {
+#pragma unroll
+                for (int n1 = 0; n1 < L1; n1++) v1[it][n1] = tw[TWO + k * (P - 1) + ti[n1] - 1];
+            }
+#pragma unroll
+            for (int n1 = 0; n1 < L1; n1++) {
+                const cfloat x = lds[lp(base + j + ti[n1] * M)];
+                v1[it][n1] = NS > 1 ? cmul(x, v1[it][n1]) : x;
             }
             if (n2 == 0) x0[it] = lds[lp(base + j)];
             RegDft<L1>::run(v1[it]);
@@ -514,19 +525,27 @@ static __device__ __forceinline__ void ct_stage_rader(cfloat* lds, int base, int
         y0[it] = cmk(0.f, 0.f);
         if (I2 % TPT == 0 || i < I2) {
             const int j = i / L1, k1 = i - j * L1;
-            cfloat v2[L2];
+            cfloat v2[L2], bh[L2];
             const int o = base + j * L + k1 * L2;
+            // the item's table values as batches of loads ahead of their
+            // uses (a load placed at its use, behind a condition, waits out
+            // a round trip of its own)
+#pragma unroll
+            for (int k2 = 0; k2 < L2; k2++) bh[k2] = Bh[k1 + L1 * k2];
 #pragma unroll
             for (int n2 = 0; n2 < L2; n2++) v2[n2] = lds[lp(o + n2)];
             RegDft<L2>::run(v2);
             if (k1 == 0) y0[it] = cadd(lds[lp(base + M * L + j)], v2[0]);
 #pragma unroll
-            for (int k2 = 0; k2 < L2; k2++) v2[k2] = cconj(cmul(v2[k2], Bh[k1 + L1 * k2]));
+            for (int k2 = 0; k2 < L2; k2++) v2[k2] = cconj(cmul(v2[k2], bh[k2]));
+            cfloat wv[L2];
+#pragma unroll
+            for (int n2 = 1; n2 < L2; n2++) wv[n2] = wl[n2 * L1 + k1];
             RegDft<L2>::run(v2);
 #pragma unroll
             for (int n2 = 0; n2 < L2; n2++) {
                 cfloat z = cconj(v2[n2]);
-                if (n2 > 0 && k1 > 0) z = cmul(z, cconj(wl[n2 * L1 + k1]));
+                if (n2 > 0 && k1 > 0) z = cmul(z, cconj(wv[n2]));
                 lds[lp(o + n2)] = z;
             }
         }
