@@ -214,6 +214,9 @@ __global__ void k_aligned_sum(const cfloat* __restrict__ F, const float* __restr
     if (y >= H || kx >= cx) return;
     const long long q = (long long)kx * H + y;
     cfloat a0 = cmk(0.f, 0.f), a1 = cmk(0.f, 0.f);
+    // unrolled so that several frames' loads are in flight at once (the
+    // sums still run in frame order)
+#pragma unroll 4
     for (int i = 0; i < N; i++) {
         const cfloat f = Spec<FF>::ld(F, i * fs + q, FF == MC_SPEC_F32 ? 1.f : finv[i]);
         const cfloat v = cmul(f, cmul(A[ao + (long long)i * cx + kx], B[bo + (long long)i * H + y]));
