@@ -713,9 +713,10 @@ int run_iterative_classes(mc_ctx* c, const char* tag, const IterClass* cls, int 
         feat = false;  // the restarted jobs keep the features of the guided pass
     }
     // Blocks of two iterations; with `graphs` each block is captured once
-    // into a HIP graph (no memset nodes) and replayed.  Kernel timing needs
-    // the direct launches.
-    const bool use_graph = graphs && c->timed.empty();
+    // into a HIP graph (no memset nodes) and replayed.  Timing one of the
+    // block's kernels needs the direct launches.
+    const bool use_graph = graphs && c->timed != "phasors" && c->timed != "aligned_sum" &&
+                           c->timed != "correlate" && c->timed != "row_dft";
     for (int it0 = 0; it0 < max_iter && !skip_loop; it0 += 2) {
         const int it1 = std::min(it0 + 2, max_iter);
         if (use_graph) {
