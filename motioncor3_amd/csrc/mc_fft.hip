@@ -7,6 +7,7 @@
 // between the passes is column-blocked so both passes move contiguous runs.
 #include <cmath>
 #include <cstdio>
+#include <type_traits>
 #include <vector>
 
 #include "mc_fft.h"
@@ -853,26 +854,35 @@ __global__ __launch_bounds__(NT, (M == 288 ? MC_R2C288_WPE : 1)) void k_rows_r2c
     }
 }
 
-// C2R pass 0 straight from the staging layout: butterfly j of pass 0 reads
-// X[j + t M0] (t < R0) and needs the Hermitian partners X[M - j - t M0],
-// which are the inputs of butterfly M0 - j, so one work item takes the pair
-// (j, M0 - j) (j = 0 and j = M0/2 pair with themselves; j = 0 also reads the
-// Nyquist value X[M]) and forms the pre-processed inputs
+// The half-length packing of the real inverse: from X[k] and its Hermitian
+// partner X[M-k] of the complex row (length M + 1 in staging),
 // Z[k] = conj(X[k] + conj X[M-k] - i e^{+2 pi i k/(2M)} (X[k] - conj X[M-k]))
-// (the half-length packing of the real inverse) in registers.
+// (w = e^{-2 pi i k/(2M)}, the post-twiddle table entry).
 __device__ __forceinline__ cfloat c2r_pre(cfloat p, cfloat q, cfloat w) {
     const cfloat t = cmul(cconj(w), cmk(p.x - q.x, p.y + q.y));
     return cconj(cmk(p.x + q.x - t.y, p.y - q.y + t.x));
 }
 
+// C2R rows: staging T -> pre-processing (one work item per Hermitian pair
+// (k, M-k), k <= M/2; Z[0] also takes the Nyquist value X[M]) -> LDS in
+// natural order -> passes 0 .. NP-2 in LDS -> last pass -> real rows.  The
+// items run in the staging layout's order (C inputs of one row, the next
+// row's, then the next C), so the RR rows' C consecutive inputs are read by
+// consecutive lanes as one contiguous run.  Each item holds two inputs and
+// one twiddle: the kernel needs no more registers than the forward rows.
+// Z[M-k] = c2r_pre(X[M-k], X[k], e^{-2 pi i (M-k)/(2M)}), and that twiddle is
+// -conj(w_k); the float table holds that symmetry exactly for 0 < k < M/2
+// (only k = 0, M/2 and M break it, and they are never paired), so one load
+// serves both.
 template <int M, int RR, int C, int NT, int SC>
 __global__ __launch_bounds__(NT) void k_rows_c2r_rp(RowC2RArgs a, const cfloat* __restrict__ ptw) {
     constexpr int NP = rp_npass(M, SC);
-    constexpr int R0 = rp_radix(M, 0, SC), M0 = M / R0;
     constexpr int RL = rp_radix(M, NP - 1, SC), ML = M / RL;
-    constexpr int NPAIR = M0 / 2 + 1;
-    constexpr int IT0 = (RR * NPAIR + NT - 1) / NT;
+    constexpr int NK = M / 2 + 1;                       // pre-processing items per row
+    constexpr int NIP = ((NK + C - 1) / C) * C * RR;    // in staging order, padded
+    constexpr int ITP = (NIP + NT - 1) / NT;
     constexpr int ITL = (RR * ML + NT - 1) / NT;
+    constexpr int LD = rp_cs(M, RR, SC);
     extern __shared__ cfloat lds[];
     const int nrb = (a.H + RR - 1) / RR;
     const int tt = xcd_tile(blockIdx.x, gridDim.x);
@@ -882,87 +892,63 @@ __global__ __launch_bounds__(NT) void k_rows_c2r_rp(RowC2RArgs a, const cfloat* 
     const cfloat* tb = a.T + (long long)b * a.t_bstride + (long long)row0 * C;
     const long long kbs = (long long)a.H * C;
 #pragma unroll
-    for (int it = 0; it < IT0; it++) {
-        const int i = threadIdx.x + it * NT;
-        if ((RR * NPAIR) % NT == 0 || i < RR * NPAIR) {
-            const int r = i / NPAIR, j = i - r * NPAIR;
-            if (r < nrows) {
-                const int j2 = j == 0 ? 0 : M0 - j;  // partner butterfly
-                const bool two = j2 != j;
-                const cfloat* xr = tb + r * C;
-                cfloat xa[R0], xb[R0];
-#pragma unroll
-                for (int t = 0; t < R0; t++) {
-                    const int k = j + t * M0;
-                    xa[t] = xr[(long long)(k / C) * kbs + k % C];
-                }
-                if (two) {
-#pragma unroll
-                    for (int t = 0; t < R0; t++) {
-                        const int k = j2 + t * M0;
-                        xb[t] = xr[(long long)(k / C) * kbs + k % C];
-                    }
-                }
-                cfloat v[R0];
-                if (j == 0) {
-                    const float xm = xr[(long long)(M / C) * kbs + M % C].x;
-                    v[0] = cconj(cmk(xa[0].x + xm, xa[0].x - xm));
-#pragma unroll
-                    for (int t = 1; t < R0; t++) v[t] = c2r_pre(xa[t], xa[R0 - t], a.post_tw[t * M0]);
-                } else {
-#pragma unroll
-                    for (int t = 0; t < R0; t++)
-                        v[t] = c2r_pre(xa[t], two ? xb[R0 - 1 - t] : xa[R0 - 1 - t],
-                                             a.post_tw[j + t * M0]);
-                }
-                RegDft<R0>::run(v);
-                cfloat* d = lds + r * rp_cs(M, RR, SC);
-#pragma unroll
-                for (int t = 0; t < R0; t++) d[rp_ix<M, SC>(j * R0 + t)] = v[t];
-                if (two) {
-#pragma unroll
-                    for (int t = 0; t < R0; t++)
-                        v[t] = c2r_pre(xb[t], xa[R0 - 1 - t], a.post_tw[j2 + t * M0]);
-                    RegDft<R0>::run(v);
-#pragma unroll
-                    for (int t = 0; t < R0; t++) d[rp_ix<M, SC>(j2 * R0 + t)] = v[t];
-                }
+    for (int it = 0; it < ITP; it++) {
+        const int e = threadIdx.x + it * NT;
+        const int c = e % C, r = (e / C) % RR, k = (e / (C * RR)) * C + c;
+        if ((NIP % NT == 0 || e < NIP) && k < NK && r < nrows) {
+            const cfloat* xr = tb + r * C;
+            const int m = k == 0 ? M : M - k;
+            const cfloat p = xr[(long long)(k / C) * kbs + k % C];
+            const cfloat q = xr[(long long)(m / C) * kbs + m % C];
+            cfloat* d = lds + r * LD;
+            if (k == 0) {
+                d[rp_ix<M, SC>(0)] = cconj(cmk(p.x + q.x, p.x - q.x));
+            } else {
+                const cfloat w = a.post_tw[k];
+                d[rp_ix<M, SC>(k)] = c2r_pre(p, q, w);
+                if (m != k) d[rp_ix<M, SC>(m)] = c2r_pre(q, p, cmk(-w.x, w.y));
             }
         }
     }
     __syncthreads();
-    RpRange<M, 1, NP - 1, RR, NT, SC>::run(lds, ptw);
-    const bool vec = ((a.pitch | a.out_bstride) & 1) == 0 && (((uintptr_t)a.out) & 7) == 0;
+    RpRange<M, 0, NP - 1, RR, NT, SC>::run(lds, ptw);
+    // Last pass, in one straight-line copy per store form: a run-time choice
+    // between paired and single stores (and the clamp as a branch) inside
+    // the unrolled store loop keeps about 40 more registers live.
+    const bool clamp = a.clamp != 0;
+    auto last = [&](auto vec) {
 #pragma unroll
-    for (int it = 0; it < ITL; it++) {
-        const int i = threadIdx.x + it * NT;
-        if ((RR * ML) % NT == 0 || i < RR * ML) {
-            const int r = i / ML, k = i - r * ML;
-            if (r < nrows) {
-                cfloat v[RL];
-                const cfloat* sl = lds + r * rp_cs(M, RR, SC);
+        for (int it = 0; it < ITL; it++) {
+            const int i = threadIdx.x + it * NT;
+            if ((RR * ML) % NT == 0 || i < RR * ML) {
+                const int r = i / ML, k = i - r * ML;
+                if (r < nrows) {
+                    cfloat v[RL];
+                    const cfloat* sl = lds + r * rp_cs(M, RR, SC);
 #pragma unroll
-                for (int t = 0; t < RL; t++) v[t] = sl[rp_ix<M, SC>(k + t * ML)];
-                rp_twiddle<M, NP - 1, SC>(v, k, ptw);
-                RegDft<RL>::run(v);
-                float* ob = a.out + (long long)b * a.out_bstride + (long long)(row0 + r) * a.pitch;
+                    for (int t = 0; t < RL; t++) v[t] = sl[rp_ix<M, SC>(k + t * ML)];
+                    rp_twiddle<M, NP - 1, SC>(v, k, ptw);
+                    RegDft<RL>::run(v);
+                    float* ob = a.out + (long long)b * a.out_bstride + (long long)(row0 + r) * a.pitch;
 #pragma unroll
-                for (int t = 0; t < RL; t++) {
-                    const int y = k + t * ML;
-                    float v0 = v[t].x, v1 = -v[t].y;
-                    if (a.clamp) {
-                        v0 = fmaxf(v0, 0.f);
-                        v1 = fmaxf(v1, 0.f);
-                    }
-                    if (vec) *reinterpret_cast<float2*>(ob + 2 * y) = make_float2(v0, v1);
-                    else {
-                        ob[2 * y] = v0;
-                        ob[2 * y + 1] = v1;
+                    for (int t = 0; t < RL; t++) {
+                        const int y = k + t * ML;
+                        float v0 = v[t].x, v1 = -v[t].y;
+                        v0 = clamp ? fmaxf(v0, 0.f) : v0;
+                        v1 = clamp ? fmaxf(v1, 0.f) : v1;
+                        if constexpr (decltype(vec)::value) {
+                            *reinterpret_cast<float2*>(ob + 2 * y) = make_float2(v0, v1);
+                        } else {
+                            ob[2 * y] = v0;
+                            ob[2 * y + 1] = v1;
+                        }
                     }
                 }
             }
         }
-    }
+    };
+    if (((a.pitch | a.out_bstride) & 1) == 0 && (((uintptr_t)a.out) & 7) == 0) last(std::true_type{});
+    else last(std::false_type{});
 }
 
 #ifndef MC_RP_ROW_SPECS
