@@ -458,23 +458,28 @@ __global__ __launch_bounds__(MC_NTHR) void k_rows_c2r_ct(RowC2RArgs a) {
     ct_fft<M, TPT>(lds, g * M, tid, a.plan.tw);
     if (g >= nrows) return;
     float* ob = a.out + (long long)b * a.out_bstride + (long long)(row0 + g) * a.pitch;
-    const bool vec = ((a.pitch | a.out_bstride) & 1) == 0 && (((uintptr_t)a.out) & 7) == 0;
+    // one straight-line copy per store form, the clamp as a select (as in
+    // k_rows_c2r_rp)
+    const bool clamp = a.clamp != 0;
+    auto store = [&](auto vec) {
 #pragma unroll
-    for (int it = 0; it < SIT; it++) {
-        const int j = tid + it * TPT;
-        if (M % TPT != 0 && j >= M) continue;
-        const cfloat z = lds[lp(row + j)];
-        float v0 = z.x, v1 = -z.y;
-        if (a.clamp) {
-            v0 = fmaxf(v0, 0.f);
-            v1 = fmaxf(v1, 0.f);
+        for (int it = 0; it < SIT; it++) {
+            const int j = tid + it * TPT;
+            if (M % TPT != 0 && j >= M) continue;
+            const cfloat z = lds[lp(row + j)];
+            float v0 = z.x, v1 = -z.y;
+            v0 = clamp ? fmaxf(v0, 0.f) : v0;
+            v1 = clamp ? fmaxf(v1, 0.f) : v1;
+            if constexpr (decltype(vec)::value) {
+                *reinterpret_cast<float2*>(ob + 2 * j) = make_float2(v0, v1);
+            } else {
+                ob[2 * j] = v0;
+                ob[2 * j + 1] = v1;
+            }
         }
-        if (vec) *reinterpret_cast<float2*>(ob + 2 * j) = make_float2(v0, v1);
-        else {
-            ob[2 * j] = v0;
-            ob[2 * j + 1] = v1;
-        }
-    }
+    };
+    if (((a.pitch | a.out_bstride) & 1) == 0 && (((uintptr_t)a.out) & 7) == 0) store(std::true_type{});
+    else store(std::false_type{});
 }
 
 template <int H, int C, int NT>
