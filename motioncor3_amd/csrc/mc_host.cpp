@@ -15,6 +15,7 @@
 #include <condition_variable>
 #include <cstdio>
 #include <cstring>
+#include <functional>
 #include <map>
 #include <mutex>
 #include <string>
@@ -75,6 +76,11 @@ struct mc_ctx {
     hipEvent_t evi = nullptr, evw = nullptr;
     double* pin_shifts = nullptr;
     size_t pin_shifts_n = 0;
+    // pinned staging of an iteration state's host tables (one upload per
+    // alignment stage) and the event of its last upload
+    unsigned char* pin_tab = nullptr;
+    size_t pin_tab_n = 0;
+    hipEvent_t evt = nullptr;
     // learned prior for guided patch alignment (not owned)
     mc_prior* prior = nullptr;
     // timing events of the stage clock (mc_result.seconds_stage), reused
@@ -239,6 +245,11 @@ struct KTimer {
         if (on) (void)hipEventRecord(c->evs[idx].b, st);
     }
 };
+
+// Host work the caller wants done once the first iterations of the next
+// iterative alignment are queued (and the GPU is busy with them), instead
+// of before: it then does not hold back the alignment's first launch.
+thread_local std::function<int()>* tl_after_launch = nullptr;
 
 // Stage clock of one call: events recorded on the context stream at the
 // ends of each stage interval; after the call's final synchronisation the
@@ -605,25 +616,47 @@ int run_iterative_classes(mc_ctx* c, const char* tag, const IterClass* cls, int 
     st.N_max = N_max;
     st.W_max = W_max;
     st.tiles = tiles;
-    st.tmp = (double*)buf(c, nm("tmp").c_str(), sizeof(double) * NT * 2);
-    st.total = (double*)buf(c, nm("total").c_str(), sizeof(double) * GT * 2);
-    st.gshift = (double*)buf(c, nm("gshift").c_str(), sizeof(double) * GT * 2);
-    st.final_ = (double*)buf(c, nm("final").c_str(), sizeof(double) * NT * 2);
-    st.errors = (double*)buf(c, nm("errors").c_str(), sizeof(double) * J * max_iter);
-    st.scratch = (double*)buf(c, nm("scratch").c_str(), sizeof(double) * sco);
-    st.iters = (int*)buf(c, nm("iters").c_str(), sizeof(int) * J);
-    st.done = (unsigned char*)buf(c, nm("done").c_str(), J);
-    st.gstart = (int*)buf(c, nm("gstart").c_str(), sizeof(int) * gstart.size());
-    st.gsize = (int*)buf(c, nm("gsize").c_str(), sizeof(int) * gsize.size());
-    st.fgroup = (int*)buf(c, nm("fgroup").c_str(), sizeof(int) * fgroup.size());
-    st.gcent = (double*)buf(c, nm("gcent").c_str(), sizeof(double) * gcent.size());
-    st.fcent = (double*)buf(c, nm("fcent").c_str(), sizeof(double) * fcent.size());
-    JobGeo* djobs = (JobGeo*)buf(c, nm("jobs").c_str(), sizeof(JobGeo) * J);
-    int* dfjob = (int*)buf(c, nm("fjob").c_str(), sizeof(int) * NT);
-    if (!st.tmp || !st.total || !st.gshift || !st.final_ || !st.errors || !st.scratch ||
-        !st.iters || !st.done || !st.gstart || !st.gsize || !st.fgroup || !st.gcent || !st.fcent ||
-        !djobs || !dfjob)
-        return fail(c, MC_E_NOMEM, "iteration state allocation failed");
+    // The iteration state is one device block: the part that starts at zero
+    // (tmp, total, errors, iters, done), the rest of the state, then the
+    // host tables, each region 256-byte aligned.  The tables go up in one
+    // copy from pinned memory and the zeroed part is cleared by one fill
+    // kernel: five small memsets and seven pageable copies each held the
+    // stream (and the host) for about 60 us.
+    size_t off = 0;
+    auto take = [&](size_t bytes) {
+        const size_t o = off;
+        off += (bytes + 255) & ~(size_t)255;
+        return o;
+    };
+    const size_t o_tmp = take(sizeof(double) * NT * 2), o_total = take(sizeof(double) * GT * 2),
+                 o_errors = take(sizeof(double) * J * max_iter), o_iters = take(sizeof(int) * J),
+                 o_done = take(J);
+    const size_t clear_end = off;
+    const size_t o_gshift = take(sizeof(double) * GT * 2), o_final = take(sizeof(double) * NT * 2),
+                 o_scratch = take(sizeof(double) * sco);
+    const size_t tab0 = off;
+    const size_t o_gstart = take(sizeof(int) * gstart.size()), o_gsize = take(sizeof(int) * gsize.size()),
+                 o_fgroup = take(sizeof(int) * fgroup.size()), o_gcent = take(sizeof(double) * gcent.size()),
+                 o_fcent = take(sizeof(double) * fcent.size()), o_jobs = take(sizeof(JobGeo) * J),
+                 o_fjob = take(sizeof(int) * NT);
+    const size_t tab_bytes = off - tab0;
+    unsigned char* blob = (unsigned char*)buf(c, nm("state").c_str(), off);
+    if (!blob) return fail(c, MC_E_NOMEM, "iteration state allocation failed");
+    st.tmp = (double*)(blob + o_tmp);
+    st.total = (double*)(blob + o_total);
+    st.errors = (double*)(blob + o_errors);
+    st.iters = (int*)(blob + o_iters);
+    st.done = blob + o_done;
+    st.gshift = (double*)(blob + o_gshift);
+    st.final_ = (double*)(blob + o_final);
+    st.scratch = (double*)(blob + o_scratch);
+    st.gstart = (int*)(blob + o_gstart);
+    st.gsize = (int*)(blob + o_gsize);
+    st.fgroup = (int*)(blob + o_fgroup);
+    st.gcent = (double*)(blob + o_gcent);
+    st.fcent = (double*)(blob + o_fcent);
+    JobGeo* djobs = (JobGeo*)(blob + o_jobs);
+    int* dfjob = (int*)(blob + o_fjob);
     st.jobs = djobs;
     st.fjob = dfjob;
     MC_BUF(A, cfloat, nm("phA").c_str(), (size_t)ao);
@@ -632,20 +665,28 @@ int run_iterative_classes(mc_ctx* c, const char* tag, const IterClass* cls, int 
     MC_BUF(T1, cfloat, nm("t1").c_str(), (size_t)t1o);
     MC_BUF(win, float, nm("win").c_str(), (size_t)GT * seay * seax);
     hipStream_t s = c->stream;
-    MC_TRY(hipMemsetAsync(st.tmp, 0, sizeof(double) * NT * 2, s));
-    MC_TRY(hipMemsetAsync(st.total, 0, sizeof(double) * GT * 2, s));
-    MC_TRY(hipMemsetAsync(st.errors, 0, sizeof(double) * J * max_iter, s));
-    MC_TRY(hipMemsetAsync(st.iters, 0, sizeof(int) * J, s));
-    MC_TRY(hipMemsetAsync(st.done, 0, J, s));
-    MC_TRY(hipMemcpyAsync(st.gstart, gstart.data(), sizeof(int) * gstart.size(), hipMemcpyHostToDevice, s));
-    MC_TRY(hipMemcpyAsync(st.gsize, gsize.data(), sizeof(int) * gsize.size(), hipMemcpyHostToDevice, s));
-    MC_TRY(hipMemcpyAsync(st.fgroup, fgroup.data(), sizeof(int) * fgroup.size(), hipMemcpyHostToDevice, s));
-    MC_TRY(hipMemcpyAsync(st.gcent, gcent.data(), sizeof(double) * gcent.size(), hipMemcpyHostToDevice, s));
-    MC_TRY(hipMemcpyAsync(st.fcent, fcent.data(), sizeof(double) * fcent.size(), hipMemcpyHostToDevice, s));
-    MC_TRY(hipMemcpyAsync(djobs, jg.data(), sizeof(JobGeo) * J, hipMemcpyHostToDevice, s));
-    MC_TRY(hipMemcpyAsync(dfjob, fjob.data(), sizeof(int) * NT, hipMemcpyHostToDevice, s));
-    // the host tables go out of scope with this call; the copies above read
-    // pageable memory, so they are done when hipMemcpyAsync returns
+    // the staging is rewritten once its previous upload has run
+    MC_TRY(hipEventSynchronize(c->evt));
+    if (c->pin_tab_n < tab_bytes) {
+        if (c->pin_tab) MC_TRY(hipHostFree(c->pin_tab));
+        c->pin_tab = nullptr;
+        c->pin_tab_n = 0;
+        MC_TRY(hipHostMalloc((void**)&c->pin_tab, tab_bytes));
+        c->pin_tab_n = tab_bytes;
+    }
+    auto stage = [&](size_t o, const void* src, size_t n) {
+        if (n) std::memcpy(c->pin_tab + (o - tab0), src, n);
+    };
+    stage(o_gstart, gstart.data(), sizeof(int) * gstart.size());
+    stage(o_gsize, gsize.data(), sizeof(int) * gsize.size());
+    stage(o_fgroup, fgroup.data(), sizeof(int) * fgroup.size());
+    stage(o_gcent, gcent.data(), sizeof(double) * gcent.size());
+    stage(o_fcent, fcent.data(), sizeof(double) * fcent.size());
+    stage(o_jobs, jg.data(), sizeof(JobGeo) * J);
+    stage(o_fjob, fjob.data(), sizeof(int) * NT);
+    MC_TRY(hipMemcpyAsync(blob + tab0, c->pin_tab, tab_bytes, hipMemcpyHostToDevice, s));
+    MC_TRY(hipEventRecord(c->evt, s));
+    MC_TRY(mc_launch_fill_u8(blob, (long long)clear_end, 0, s));
     const int N = st.N, G = st.G;  // the prior hooks (one class)
     const double tol0 = cls[0].tol;
     bool feat = pr && pr->X;  // write the prior features after iteration 0
@@ -691,6 +732,11 @@ int run_iterative_classes(mc_ctx* c, const char* tag, const IterClass* cls, int 
                                   seay, pr->X, pr->ns + 66 * N, s);
         return hipSuccess;
     };
+    auto after_launch = [&]() -> int {
+        std::function<int()>* h = tl_after_launch;
+        tl_after_launch = nullptr;
+        return (h && *h) ? (*h)() : MC_OK;
+    };
     // Guided start: iterations 0 and 1 with the prior's correction between
     // them, then the residual gate; the block loop below then runs only for
     // the jobs that fell back (all others are done).
@@ -707,6 +753,7 @@ int run_iterative_classes(mc_ctx* c, const char* tag, const IterClass* cls, int 
         MC_TRY(enqueue(1, true));
         MC_TRY(prior_gate(st.errors, max_iter, st.done, st.iters, st.tmp, st.total, J, N, G,
                           pr->fallback, pr->fell, s));
+        MC_CHECK(after_launch());
         MC_TRY(hipMemcpyAsync(hdone.data(), st.done, J, hipMemcpyDeviceToHost, s));
         MC_TRY(hipStreamSynchronize(s));
         skip_loop = std::all_of(hdone.begin(), hdone.end(), [](unsigned char d) { return d != 0; });
@@ -757,6 +804,7 @@ int run_iterative_classes(mc_ctx* c, const char* tag, const IterClass* cls, int 
         } else {
             for (int it = it0; it < it1; it++) MC_TRY(enqueue(it, true));
         }
+        MC_CHECK(after_launch());
         // A converged job skips every kernel of the later iterations, so once
         // all jobs are done the remaining launches would change nothing: look
         // at the flags after every block of two iterations (2 is the
@@ -1342,18 +1390,29 @@ int align_batch(mc_ctx* c, const mc_params* const* pp_, const mc_movie* const* m
                                  ff, gf, iff));
         MC_TRY(join());
     }
+    // The next movie's forward transform on the second stream, behind
+    // everything already queued there; the frame-chunk loops below queue
+    // their second-stream chunks after it.  Its launches are queued once the
+    // global alignment's first iterations are (queue_next through
+    // tl_after_launch, or right after that alignment): launching them first
+    // kept the main stream idle for about a millisecond per movie.
+    Stacks nx;
+    bool next_queued = true;
     if (pipe && pipe->next) {
-        // the next movie's forward transform on the second stream, behind
-        // everything already queued there; the frame-chunk loops below
-        // queue their second-stream chunks after it
-        Stacks nx;
         if (!stacks(pipe->slot ^ 1, nx)) return fail(c, MC_E_NOMEM, "next movie's stacks");
         MC_CHECK(wait_movie_loads(c, pipe->next, c->s2));
+        next_queued = false;
+    }
+    std::function<int()> queue_next = [&]() -> int {
+        if (next_queued) return MC_OK;
+        next_queued = true;
         const FwdLane lane{c->s2, T2, dcf2};
         MC_CHECK(enqueue_forward(c, pf, &pipe->next, 1, g, fx, nx.Sf, Sx_fwd ? (void*)nx.Sx : nullptr,
                                  nx.gx, nx.ix, &lane, 1, CH, ff, nx.gf, nx.iff));
         MC_TRY(hipEventRecord(pipe->ev[pipe->slot ^ 1], c->s2));
-    }
+        return MC_OK;
+    };
+    if (replay) MC_CHECK(queue_next());
 
     struct MovieState {
         std::vector<double> gshift, ps, pc, gerr, xs;
@@ -1379,9 +1438,13 @@ int align_batch(mc_ctx* c, const mc_params* const* pp_, const mc_movie* const* m
         IterOut go;
         MC_TRY(sc.begin(1));
         Range r_g("mc::global_align");
-        MC_CHECK(run_iter_meet(c, "g_", Sx_all, count, N, g.xw, g.xh, px, g0, p->bft[0],
-                               p->phase_only, p->iter, tol, quirks, &go, fx, ix,
-                               p->launch_graphs != 0));
+        tl_after_launch = &queue_next;
+        const int rc_g = run_iter_meet(c, "g_", Sx_all, count, N, g.xw, g.xh, px, g0, p->bft[0],
+                                       p->phase_only, p->iter, tol, quirks, &go, fx, ix,
+                                       p->launch_graphs != 0);
+        tl_after_launch = nullptr;
+        MC_CHECK(rc_g);
+        MC_CHECK(queue_next());  // (no-op once the hook has run)
         for (int m = 0; m < count; m++) {
             ms[m].giters = go.iters[m];
             ms[m].gerr.assign(go.errors.begin() + (size_t)m * p->iter,
@@ -1927,7 +1990,8 @@ int mc_ctx_create(int device, void* stream, mc_ctx** out) {
         hipEventCreateWithFlags(&c->evf, hipEventDisableTiming) != hipSuccess ||
         hipEventCreateWithFlags(&c->evj, hipEventDisableTiming) != hipSuccess ||
         hipEventCreateWithFlags(&c->evi, hipEventDisableTiming) != hipSuccess ||
-        hipEventCreateWithFlags(&c->evw, hipEventDisableTiming) != hipSuccess) {
+        hipEventCreateWithFlags(&c->evw, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&c->evt, hipEventDisableTiming) != hipSuccess) {
         if (c->s2) (void)hipStreamDestroy(c->s2);
         if (c->own_stream) (void)hipStreamDestroy(c->stream);
         delete c;
@@ -1962,7 +2026,9 @@ int mc_ctx_destroy(mc_ctx* c) {
     if (c->evj) (void)hipEventDestroy(c->evj);
     if (c->evi) (void)hipEventDestroy(c->evi);
     if (c->evw) (void)hipEventDestroy(c->evw);
+    if (c->evt) (void)hipEventDestroy(c->evt);
     if (c->pin_shifts) (void)hipHostFree(c->pin_shifts);
+    if (c->pin_tab) (void)hipHostFree(c->pin_tab);
     if (c->s_in) {
         (void)hipStreamSynchronize(c->s_in);
         (void)hipStreamDestroy(c->s_in);
