@@ -1350,14 +1350,29 @@ __device__ __forceinline__ void warp_tile_reg(const WarpArgs& a, const int* klis
         // stores included) before it.  The last nf % MC_WARP_U frames go one
         // at a time.
         const int nfull = nf - nf % MC_WARP_U;
-        if (nfull) offsets(0);
-        for (int fb = 0; fb < nfull; fb += MC_WARP_U) {
+        // Software-pipelined: group g+1's gathers go out before group g's
+        // stores.  Loads and stores share one memory counter, counted in
+        // issue order, so a wave then waits only for the loads it is about
+        // to store, while the previous group's stores stay in flight.
+        if (nfull) {
+            offsets(0);
             float v[MC_WARP_U];
 #pragma unroll
-            for (int u = 0; u < MC_WARP_U; u++) v[u] = (a.R + (long long)(f0 + fb + u) * a.fstride)[so[u]];
-            if (fb + MC_WARP_U < nfull) offsets(fb + MC_WARP_U);
+            for (int u = 0; u < MC_WARP_U; u++) v[u] = (a.R + (long long)(f0 + u) * a.fstride)[so[u]];
+            for (int fb = 0; fb + MC_WARP_U < nfull; fb += MC_WARP_U) {
+                offsets(fb + MC_WARP_U);
+                float vn[MC_WARP_U];
 #pragma unroll
-            for (int u = 0; u < MC_WARP_U; u++) (a.Rp + (long long)(f0 + fb + u) * a.fstride)[oo] = v[u];
+                for (int u = 0; u < MC_WARP_U; u++)
+                    vn[u] = (a.R + (long long)(f0 + fb + MC_WARP_U + u) * a.fstride)[so[u]];
+#pragma unroll
+                for (int u = 0; u < MC_WARP_U; u++) (a.Rp + (long long)(f0 + fb + u) * a.fstride)[oo] = v[u];
+#pragma unroll
+                for (int u = 0; u < MC_WARP_U; u++) v[u] = vn[u];
+            }
+#pragma unroll
+            for (int u = 0; u < MC_WARP_U; u++)
+                (a.Rp + (long long)(f0 + nfull - MC_WARP_U + u) * a.fstride)[oo] = v[u];
         }
         for (int fi = nfull; fi < nf; fi++) {
             const float4* t = tab + fi * K;
