@@ -51,6 +51,32 @@ def test_clamped_c2r(ctx):
     assert np.allclose(out, np.maximum(x, 0), atol=1e-5)
 
 
+@pytest.mark.parametrize("W,H", [(5760, 4092), (576, 408), (2882, 2048)])
+@pytest.mark.parametrize("clamp", [0, 1])
+def test_c2r_odd_pitch_and_unaligned_output(ctx, W, H, clamp):
+    """The C2R row passes store float pairs when the output allows it and
+    single floats otherwise (odd pitch, 4-byte-aligned pointer).  Both forms
+    must give the same bits and touch nothing outside the rows."""
+    import ctypes as C
+    print("c2r unaligned", W, H, clamp, flush=True)
+    B = 2 if W * H < 5_000_000 else 1
+    rng = np.random.default_rng(W + 3 * H + clamp)
+    x = rng.standard_normal((B, H, W)).astype(np.float32)
+    spec = ctx.rfft2(ctx.upload(x), scale=1.0 / (W * H))
+    ref = ctx.irfft2(spec, W, clamp=bool(clamp)).numpy()
+    P = W + 1
+    n = B * H * P
+    buf = ctx.upload(np.full(n + 2, np.nan, np.float32))
+    ctx._check(ctx.lib.mc_irfft2(ctx.ptr, spec.vp(), W, H, B, buf.vp(1), C.c_longlong(P),
+                                 C.c_longlong(H * P), clamp), "mc_irfft2")
+    ctx.sync()
+    got = buf.numpy()
+    assert np.isnan(got[0]) and np.isnan(got[n + 1])
+    rows = got[1:n + 1].reshape(B, H, P)
+    assert np.all(np.isnan(rows[:, :, W]))
+    assert np.array_equal(rows[:, :, :W], ref)
+
+
 def _radix(name):
     return name.split("radix ")[1].split()[0]
 
