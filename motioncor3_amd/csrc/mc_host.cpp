@@ -135,14 +135,17 @@ static hipError_t scratch_for_device(int device, unsigned char** out) {
 
 // Chunks of the early local-correction inverse that the second stream takes
 // (it also runs the next movie's forward transform); the correction stream
-// transforms the rest once its patch alignment is done.  At 4 frames per
-// chunk and 50 frames: 7 of 13 measured 37.3-37.5 movies/s against 36.0-37.0
-// with all 13 on the second stream (9: 37.0, 5: 37.1-37.2).
+// transforms the rest once its patch alignment is done.  Frames per chunk of
+// the full-frame FFT loops: larger launches leave less of each pass to its
+// tail.  At 50 frames, alternating on one box: 10 frames per chunk with 3
+// early chunks 38.72-38.75 movies/s, 8 with 4 (or 3) 38.1-38.6, 12 with 2
+// 38.4-38.5, 16 with 2 38.4, against 37.3-37.8 for 4 with 7 (which had
+// measured 37.3-37.5 against 36.0-37.0 with all 13 on the second stream).
 #ifndef MC_EARLY_INV_CHUNKS
-#define MC_EARLY_INV_CHUNKS 7
+#define MC_EARLY_INV_CHUNKS 3
 #endif
 #ifndef MC_CHUNK_FRAMES
-#define MC_CHUNK_FRAMES 4
+#define MC_CHUNK_FRAMES 10
 #endif
 
 namespace {
