@@ -237,8 +237,9 @@ def test_uneven_movies_share_iteration_launches(ctx, case):
 @pytest.mark.parametrize("W,H,N", [(5760, 4092, 6), (2048, 2048, 36)])
 def test_chunk_streams_match_single_stream(ctx, W, H, N):
     """Frame chunks on two streams give the same bits as on one stream.  With
-    36 frames (9 chunks of 4) the early local-correction inverse leaves its
-    last chunks to the correction stream, which is checked here too."""
+    36 frames (chunks of 10, 10, 10 and 6) the early local-correction inverse
+    takes the first three and leaves the last to the correction stream,
+    which is checked here too."""
     r = MovieRecipe(W, H, N, 20000, seed=9, dose=1.0, local_max=3.0)
     frames = ctx.empty((N, H, W), np.float32)
     ctx.synth_render(frames, r)
