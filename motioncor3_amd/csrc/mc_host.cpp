@@ -2807,10 +2807,10 @@ int mc_autotune_schedules(mc_ctx* c, int W, int H, int N, int px, int py, float 
     MC_TRY(hipSetDevice(c->device));
     const bool patch = px > 1 && py > 1;
     const Geo g = make_geo(W, H, N, px, py, ft_bin > 0 ? ft_bin : 1.0f);
-    // the batches the alignment launches: 2-frame chunks of full frames and
-    // output sums, N xcf frames, P * N patch frames
+    // the batches the alignment launches: frame chunks of full frames (and
+    // output sums), N xcf frames, P * N patch frames
     struct Item { int w, h, batch; };
-    std::vector<Item> items = {{W, H, 2}, {g.xw, g.xh, N}};
+    std::vector<Item> items = {{W, H, std::min(N, MC_CHUNK_FRAMES)}, {g.xw, g.xh, N}};
     if (patch) items.push_back({g.pw, g.ph, px * py * N});
     if (g.ow != W || g.oh != H) items.push_back({g.ow, g.oh, 2});
     std::string rep;
